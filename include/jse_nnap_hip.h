@@ -1,0 +1,120 @@
+/*
+ * libjsenn_hip: MI355X (gfx950) implementation of jse's NNAP evaluation path.
+ *
+ * C ABI: plain pointers and sizes, opaque handles, int status (0 = ok) and a
+ * thread-local message from jse_nnap_last_error().  Every reference location
+ * below is a path relative to the root of the jse repository.
+ *
+ * Status codes: 0 ok, 1 invalid argument/model, 2 unsupported model shape,
+ * 3 HIP runtime error, 4 neighbour capacity, 5 atom type out of range,
+ * 6 device memory budget exhausted.  (The reference turns a non-zero native
+ * return into IllegalStateException("Exit code: n"), NNAP.java:756,769,787.)
+ */
+#ifndef JSE_NNAP_HIP_H
+#define JSE_NNAP_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct jse_nnap_model_s *jse_nnap_model;
+typedef struct jse_nnap_system_s *jse_nnap_system;
+
+/* precision of a system (the reference's CUDA path is FP32 only, NNAP.java:145-147) */
+#define JSE_NNAP_FP32 0      /* descriptor, MLP and backward in FP32; FP64 wrap/displacement/sums */
+#define JSE_NNAP_BF16_MLP 1  /* as FP32, but the MLP on BF16 MFMA with FP32 accumulation */
+
+/* what jse_nnap_system_compute evaluates */
+#define JSE_NNAP_ENERGY 1
+#define JSE_NNAP_FORCE 2
+#define JSE_NNAP_VIRIAL 4
+
+const char *jse_nnap_last_error(void);
+
+/* ---- model ---------------------------------------------------------------
+ * Replaces `new NNAP(String path)` / `new NNAP(Map)` (NNAP.java:402-414,
+ * model parsing :129-307).  `json_or_path` is a path to a .json model file or
+ * the JSON text itself (a text starting with '{').  YAML is converted to JSON
+ * by the caller. */
+int jse_nnap_model_load(const char *json_or_path, jse_nnap_model *out);
+
+/* Host-only check of a model file: parses it as jse_nnap_model_load does and
+ * reports ntypes and, per species (arrays of >= ntypes entries, may be NULL),
+ * the radial channel count P and the descriptor length S.  No device use. */
+int jse_nnap_model_parse(const char *json_or_path, int *ntypes, int *size_np, int *fp_size);
+
+/* Replaces the packed parameter blocks the reference hands to its native
+ * code (NNAP.java:226-307): per species t (0-based here):
+ *   fp_param[t]   = W[ntypes][size_np[t]][nmax[t]+1]      (WTypeBasis.java:202-304)
+ *   nn_param[t]   = [hidden W | output W | hidden b | output b] (FeedForward.java:310-316)
+ *   norm_param[t] = [muE+refE, sigmaE, mu[S], sigma[S]], S = size_np*(lmax+1)
+ *   hidden[t*8 + k] = width of hidden layer k (nlayers[t] <= 8)
+ * rcut[t] is the basis hyper-parameter (MergeableBasis.java:24-30). */
+int jse_nnap_model_create(int ntypes, const int *nmax, const int *lmax, const int *size_np, const double *rcut,
+                          const int *nlayers, const int *hidden, const double *const *fp_param,
+                          const double *const *nn_param, const double *const *norm_param, jse_nnap_model *out);
+/* ntypes, max rcut (NNAP.rcutMax, NNAP.java:485) and the descriptor length of species t (1-based) */
+int jse_nnap_model_info(jse_nnap_model m, int *ntypes, double *rcut_max);
+int jse_nnap_model_fp_size(jse_nnap_model m, int ctype, int *size);
+void jse_nnap_model_destroy(jse_nnap_model m);
+
+/* ---- system: one periodic configuration, device resident ------------------
+ * Replaces the AtomData -> AtomicParameterCalculator step
+ * (IPairPotential.calEnergyForceVirials(IAtomData, ...), IPairPotential.java:1137-1141;
+ * AtomicParameterCalculator.java:130-157): natoms, 1-based model types (after
+ * the symbol type map, IHasSymbol.java:51-99) and an orthogonal box
+ * [0,Lx) x [0,Ly) x [0,Lz).  Types are fixed for the life of the system. */
+int jse_nnap_system_create(jse_nnap_model m, int natoms, const int *types, const double *box, int precision,
+                           jse_nnap_system *out);
+void jse_nnap_system_destroy(jse_nnap_system s);
+int jse_nnap_system_set_box(jse_nnap_system s, const double *box);
+/* positions as an [natoms][3] FP64 row-major host array (the reference's
+ * getMatRow layout), copied to the device; wrapping happens on the device */
+int jse_nnap_system_set_positions(jse_nnap_system s, const double *xyz);
+/* the same from a device pointer (no host round trip) */
+int jse_nnap_system_set_positions_device(jse_nnap_system s, const double *d_xyz);
+
+/* Replaces NNAP.calEnergy / NNAP.calEnergyForceVirial driven by
+ * IPairPotential.calEnergyForceVirials (NNAP.java:531-594,
+ * IPairPotential.java:811-1022): builds the cell list, evaluates every atom
+ * and accumulates energies, forces and virials.  Asynchronous on the
+ * system's stream; results are read with jse_nnap_system_get_results. */
+int jse_nnap_system_compute(jse_nnap_system s, int what);
+
+/* Copies results to host arrays; any pointer may be NULL.
+ *   energy: total E;  energies[natoms];  forces[natoms][3];
+ *   virial[9] totals and virials[natoms][9] per atom on the neighbour side,
+ *   component order xx yy zz xy xz yz yx zx zy (IPairPotential.java:972-1002).
+ * Stress follows as -virial/volume (IPotential.java:197-311). */
+int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energies, double *forces, double *virial,
+                                double *virials);
+
+/* Per-atom descriptors fp[natoms][Smax] (Smax = max descriptor length over
+ * species) of the last neighbour list: jse_nnap_calFp for every atom
+ * (nnap_interface.cpp:11-24).  Runs the neighbour build itself. */
+int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax);
+
+/* The neighbour list of the last compute: count[natoms] and, for rows of
+ * `cap` slots, key = j*32 + shift_code (shift_code = (sx+1)+3(sy+1)+9(sz+1))
+ * sorted ascending and the FP32 displacement d[natoms][cap][3].  Returns
+ * status 4 if a row does not fit in `cap` (count[] is still written). */
+int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *key, float *d);
+
+/* Timing of `iters` compute() calls after `warmup` untimed ones, with HIP
+ * events on the system's stream.  ms_step = wall time per call; phase_ms[k]
+ * = average device time of phase k per call, k = 0 neighbour build, 1
+ * reverse index, 2 descriptor forward, 3 MLP, 4 descriptor backward,
+ * 5 forces/virials/totals (nphase <= 6). */
+int jse_nnap_system_benchmark(jse_nnap_system s, int what, int warmup, int iters, double *ms_step, double *phase_ms,
+                              int nphase);
+
+/* Neighbour-list statistics of the last compute: current row capacity, the
+ * largest row and the number of pairs without a reverse partner. */
+int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long long *pairs, long long *unmatched);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* JSE_NNAP_HIP_H */

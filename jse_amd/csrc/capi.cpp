@@ -1,0 +1,653 @@
+// C-ABI of libjsenn_hip: model and system handles, device memory, the
+// evaluation pipeline and its timing.  See include/jse_nnap_hip.h.
+#include "../../include/jse_nnap_hip.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <fstream>
+#include <memory>
+#include <numeric>
+#include <sstream>
+
+#include "common.h"
+#include "kernels.h"
+
+using namespace jnn;
+
+namespace {
+thread_local std::string g_last_error;
+
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host memory exhausted";
+        return ERR_MEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return ERR_ARG;
+    }
+}
+
+template <class T>
+T *dalloc(size_t n) {
+    if (n == 0) n = 1;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw Error(ERR_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return (T *)p;
+}
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+template <class T>
+T *dupload(const std::vector<T> &v) {
+    T *p = dalloc<T>(v.size());
+    if (!v.empty()) JNN_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+
+uint16_t to_bf16(float f) {  // round to nearest even
+    uint32_t u;
+    std::memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (uint16_t)(u >> 16);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+struct SpeciesDev {
+    float *W = nullptr, *mu = nullptr, *sig = nullptr;
+    float *w_fwd = nullptr, *w_bwd = nullptr, *bias = nullptr;
+    uint16_t *wb_fwd = nullptr, *wb_bwd = nullptr;
+};
+}  // namespace
+
+struct jse_nnap_model_s {
+    ModelHost host;
+    std::vector<SpeciesDev> dev;
+    float *zeros = nullptr, *ones = nullptr;  // identity normalisation for raw descriptors
+    int smax = 0, pmmax = 0;
+
+    void upload() {
+        int nt = host.ntypes();
+        dev.resize(nt);
+        for (int t = 0; t < nt; ++t) {
+            const SpeciesHost &sp = host.species[t];
+            SpeciesDev &d = dev[t];
+            std::vector<float> W(sp.W.begin(), sp.W.end());
+            d.W = dupload(W);
+            const int S = sp.S();
+            std::vector<float> mu(sp.norm.begin() + 2, sp.norm.begin() + 2 + S);
+            std::vector<float> sg(sp.norm.begin() + 2 + S, sp.norm.end());
+            d.mu = dupload(mu);
+            d.sig = dupload(sg);
+            // forward layout: per layer [in][out], then output W; backward layout: per layer [out][in]
+            std::vector<float> wf, wb, bias;
+            size_t off = 0;
+            int in = S;
+            for (int h : sp.hidden) {
+                for (int i = 0; i < in; ++i)
+                    for (int o = 0; o < h; ++o) wf.push_back((float)sp.nn[off + (size_t)o * in + i]);
+                for (size_t q = 0; q < (size_t)in * h; ++q) wb.push_back((float)sp.nn[off + q]);
+                off += (size_t)in * h;
+                in = h;
+            }
+            for (int i = 0; i < in; ++i) wf.push_back((float)sp.nn[off + i]);
+            off += in;
+            for (size_t q = off; q < sp.nn.size(); ++q) bias.push_back((float)sp.nn[q]);
+            d.w_fwd = dupload(wf);
+            d.w_bwd = dupload(wb);
+            d.bias = dupload(bias);
+            std::vector<uint16_t> bf(wf.size()), bb(wb.size());
+            for (size_t q = 0; q < wf.size(); ++q) bf[q] = to_bf16(wf[q]);
+            for (size_t q = 0; q < wb.size(); ++q) bb[q] = to_bf16(wb[q]);
+            d.wb_fwd = dupload(bf);
+            d.wb_bwd = dupload(bb);
+            smax = std::max(smax, S);
+            pmmax = std::max(pmmax, sp.P * sp.M());
+        }
+        std::vector<float> z((size_t)smax, 0.f), o((size_t)smax, 1.f);
+        zeros = dupload(z);
+        ones = dupload(o);
+    }
+    ~jse_nnap_model_s() {
+        for (auto &d : dev) {
+            dfree(d.W); dfree(d.mu); dfree(d.sig); dfree(d.w_fwd); dfree(d.w_bwd); dfree(d.bias);
+            dfree(d.wb_fwd); dfree(d.wb_bwd);
+        }
+        dfree(zeros);
+        dfree(ones);
+    }
+};
+
+struct jse_nnap_system_s {
+    jse_nnap_model model = nullptr;
+    int n = 0;
+    int precision = JSE_NNAP_FP32;
+    double box[3] = {0, 0, 0};
+    std::vector<int> types;
+    std::vector<int> seg;  // species segment offsets into rows
+    CellGrid grid{};
+    hipStream_t stream = nullptr;
+    bool have_pos = false, have_nl = false;
+    int last_max_count = 0;
+
+    double *d_xyz = nullptr, *d_xw = nullptr, *d_sxw = nullptr;
+    int *d_cell = nullptr, *d_rank = nullptr, *d_sidx = nullptr, *d_types = nullptr, *d_rows = nullptr;
+    int *d_cell_cnt = nullptr, *d_cell_start = nullptr;
+    int ncell_alloc = 0;
+    int *d_overflow = nullptr;
+    NeighborList nl{};
+    float *d_x = nullptr, *d_A = nullptr, *d_G = nullptr;
+    double *d_E = nullptr, *d_F = nullptr, *d_V = nullptr, *d_partial = nullptr, *d_tot = nullptr;
+    static constexpr int kPartials = 256;
+    hipEvent_t ev[8] = {};
+
+    ~jse_nnap_system_s() {
+        if (stream) (void)hipStreamSynchronize(stream);
+        free_nl();
+        dfree(d_xyz); dfree(d_xw); dfree(d_sxw); dfree(d_cell); dfree(d_rank); dfree(d_sidx); dfree(d_types);
+        dfree(d_rows); dfree(d_cell_cnt); dfree(d_cell_start); dfree(d_overflow); dfree(d_x); dfree(d_A); dfree(d_G);
+        dfree(d_E); dfree(d_F); dfree(d_V); dfree(d_partial); dfree(d_tot);
+        for (auto &e : ev)
+            if (e) (void)hipEventDestroy(e);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+    void free_nl() {
+        dfree(nl.cnt); dfree(nl.key); dfree(nl.dx); dfree(nl.dy); dfree(nl.dz); dfree(nl.rev);
+        dfree(nl.gx); dfree(nl.gy); dfree(nl.gz);
+    }
+    void alloc_nl(int cap) {
+        free_nl();
+        nl.cap = cap;
+        const size_t m = (size_t)n * cap;
+        nl.cnt = dalloc<int>(n);
+        nl.key = dalloc<int>(m);
+        nl.dx = dalloc<float>(m); nl.dy = dalloc<float>(m); nl.dz = dalloc<float>(m);
+        nl.rev = dalloc<int>(m);
+        nl.gx = dalloc<float>(m); nl.gy = dalloc<float>(m); nl.gz = dalloc<float>(m);
+    }
+
+    void set_box(const double *b) {
+        for (int d = 0; d < 3; ++d) {
+            JNN_CHECK(std::isfinite(b[d]) && b[d] > 0, ERR_ARG, "box lengths must be positive and finite");
+            box[d] = b[d];
+        }
+        // NeighborListGetter.getProperLinkedCell for an orthogonal box (NeighborListGetter.java:952-1001)
+        const double rmax = model->host.rcut_max();
+        const double minbox = std::min(box[0], std::min(box[1], box[2]));
+        JNN_CHECK(rmax <= minbox, ERR_UNSUPPORTED,
+                  "rcut larger than the smallest box length (the reference's expanded cell list) is not supported");
+        const int div = (int)std::floor(minbox / rmax);
+        const double cell_len = minbox / (double)div;
+        for (int d = 0; d < 3; ++d) {
+            int sz = (int)std::floor(box[d] / cell_len);
+            grid.size[d] = std::max(sz, div);
+            grid.cb[d] = box[d] / grid.size[d];
+            grid.box[d] = box[d];
+        }
+        grid.ncell = grid.size[0] * grid.size[1] * grid.size[2];
+        grid.rc2 = rmax * rmax;
+        if (grid.ncell + 1 > ncell_alloc) {
+            dfree(d_cell_cnt);
+            dfree(d_cell_start);
+            ncell_alloc = grid.ncell + 1;
+            d_cell_cnt = dalloc<int>(ncell_alloc);
+            d_cell_start = dalloc<int>(ncell_alloc + 1);
+        }
+        have_nl = false;
+    }
+
+    void init(jse_nnap_model m, int natoms, const int *ty, const double *b, int prec) {
+        model = m;
+        n = natoms;
+        precision = prec;
+        JNN_CHECK(natoms >= 0 && natoms < (1 << 26), ERR_ARG, "natoms out of range [0, 2^26)");
+        JNN_CHECK(prec == JSE_NNAP_FP32 || prec == JSE_NNAP_BF16_MLP, ERR_ARG, "unknown precision");
+        const int nt = m->host.ntypes();
+        types.assign(ty, ty + natoms);
+        for (int i = 0; i < natoms; ++i)
+            JNN_CHECK(types[i] >= 1 && types[i] <= nt, ERR_TYPE,
+                      "Exist type (" + std::to_string(types[i]) + ") greater than the input ntypes (" +
+                          std::to_string(nt) + ")");
+        // species-grouped row order (stable), as the reference's LAMMPS path regroups by type
+        std::vector<int> rows(natoms);
+        std::iota(rows.begin(), rows.end(), 0);
+        std::stable_sort(rows.begin(), rows.end(), [&](int a, int c) { return types[a] < types[c]; });
+        seg.assign(nt + 1, 0);
+        for (int i = 0; i < natoms; ++i) seg[types[i]]++;
+        for (int t = 1; t <= nt; ++t) seg[t] += seg[t - 1];
+        for (int t = 0; t < nt; ++t) {
+            const SpeciesHost &sp = m->host.species[t];
+            if (seg[t + 1] > seg[t]) {
+                // reject shapes without compiled kernels at creation time
+                JNN_CHECK(sp.lmax <= 12, ERR_UNSUPPORTED, "lmax > 12");
+            }
+        }
+        JNN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        for (auto &e : ev) JNN_HIP(hipEventCreate(&e));
+        d_types = dupload(types);
+        d_rows = dupload(rows);
+        d_xyz = dalloc<double>((size_t)3 * n);
+        d_xw = dalloc<double>((size_t)3 * n);
+        d_sxw = dalloc<double>((size_t)3 * n);
+        d_cell = dalloc<int>(n);
+        d_rank = dalloc<int>(n);
+        d_sidx = dalloc<int>(n);
+        d_overflow = dalloc<int>(1);
+        d_x = dalloc<float>((size_t)n * m->smax);
+        d_G = dalloc<float>((size_t)n * m->smax);
+        d_A = dalloc<float>((size_t)n * m->pmmax);
+        d_E = dalloc<double>(n);
+        d_F = dalloc<double>((size_t)3 * n);
+        d_V = dalloc<double>((size_t)9 * n);
+        d_partial = dalloc<double>((size_t)kPartials * 9);
+        d_tot = dalloc<double>(10);
+        set_box(b);
+        // initial row capacity from the number density, rounded up to whole wavefronts
+        const double vol = box[0] * box[1] * box[2];
+        const double rmax = m->host.rcut_max();
+        double est = (n > 0 ? n / vol : 0.0) * 4.18879 * rmax * rmax * rmax;
+        int cap = (int)std::ceil((est * 1.5 + 16) / 64.0) * 64;
+        cap = std::max(64, std::min(cap, 512));
+        alloc_nl(cap);
+    }
+
+    void build_nl(bool timed) {
+        JNN_CHECK(have_pos, ERR_ARG, "positions not set");
+        for (int attempt = 0; attempt < 4; ++attempt) {
+            if (timed) JNN_HIP(hipEventRecord(ev[0], stream));
+            JNN_HIP(hipMemsetAsync(d_cell_cnt, 0, sizeof(int) * grid.ncell, stream));
+            JNN_HIP(hipMemsetAsync(d_overflow, 0, sizeof(int), stream));
+            launch_wrap_bin(stream, n, d_xyz, grid, d_xw, d_cell, d_cell_cnt, d_rank);
+            launch_cell_scan(stream, grid.ncell, d_cell_cnt, d_cell_start);
+            launch_cell_scatter(stream, n, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw);
+            launch_nl_build(stream, n, grid, d_xw, d_cell, d_cell_start, d_sidx, d_sxw, nl, d_overflow);
+            int ovf = 0;
+            JNN_HIP(hipMemcpyAsync(&ovf, d_overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
+            JNN_HIP(hipStreamSynchronize(stream));
+            if (ovf == 0) {
+                if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
+                launch_nl_reverse(stream, n, nl);
+                if (timed) JNN_HIP(hipEventRecord(ev[2], stream));
+                have_nl = true;
+                return;
+            }
+            JNN_CHECK(ovf <= 512, ERR_CAPACITY,
+                      "an atom has " + std::to_string(ovf) + " neighbours within rcut; at most 512 are supported");
+            alloc_nl(((ovf + 16 + 63) / 64) * 64);
+        }
+        throw Error(ERR_CAPACITY, "neighbour list capacity retries exhausted");
+    }
+
+    SegmentArgs seg_args(int t, bool raw) const {
+        const SpeciesHost &sp = model->host.species[t];
+        const SpeciesDev &dv = model->dev[t];
+        SegmentArgs a{};
+        a.nrows = seg[t + 1] - seg[t];
+        a.rows_atom = d_rows + seg[t];
+        a.types = d_types;
+        a.rc = (float)sp.rcut;
+        a.W = dv.W;
+        a.mu = raw ? model->zeros : dv.mu;
+        a.sig = raw ? model->ones : dv.sig;
+        a.x = d_x + (size_t)seg[t] * model->smax;
+        a.A = d_A + (size_t)seg[t] * model->pmmax;
+        a.G = d_G + (size_t)seg[t] * model->smax;
+        return a;
+    }
+
+    void desc_fwd(bool raw) {
+        for (int t = 0; t < model->host.ntypes(); ++t) {
+            const SpeciesHost &sp = model->host.species[t];
+            SegmentArgs a = seg_args(t, raw);
+            if (!launch_desc_fwd(stream, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
+                throw Error(ERR_UNSUPPORTED, "no descriptor kernel compiled for lmax=" + std::to_string(sp.lmax) +
+                                                 " nmax=" + std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P));
+        }
+    }
+
+    void compute(int what, bool timed) {
+        build_nl(timed);
+        const bool grad = (what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL)) != 0;
+        desc_fwd(false);
+        if (timed) JNN_HIP(hipEventRecord(ev[3], stream));
+        for (int t = 0; t < model->host.ntypes(); ++t) {
+            const SpeciesHost &sp = model->host.species[t];
+            const SpeciesDev &dv = model->dev[t];
+            SegmentArgs sa = seg_args(t, false);
+            MlpArgs a{};
+            a.nrows = sa.nrows;
+            a.S = sp.S();
+            a.L1 = sp.lmax + 1;
+            a.nlayers = (int)sp.hidden.size();
+            for (int l = 0; l < a.nlayers; ++l) a.hidden[l] = sp.hidden[l];
+            a.rows_atom = sa.rows_atom;
+            a.x = sa.x;
+            a.w_in_out = dv.w_fwd;
+            a.w_out_in = dv.w_bwd;
+            a.bias = dv.bias;
+            a.sig = dv.sig;
+            a.sig_e = (float)sp.norm[1];
+            a.mu_e = (float)sp.norm[0];
+            a.E = d_E;
+            a.G = sa.G;
+            a.want_grad = grad;
+            if (precision == JSE_NNAP_BF16_MLP) launch_mlp_bf16(stream, a, dv.wb_fwd, dv.wb_bwd);
+            else launch_mlp_fp32(stream, a);
+        }
+        if (timed) JNN_HIP(hipEventRecord(ev[4], stream));
+        if (grad) {
+            for (int t = 0; t < model->host.ntypes(); ++t) {
+                const SpeciesHost &sp = model->host.species[t];
+                SegmentArgs a = seg_args(t, false);
+                if (!launch_desc_bwd(stream, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
+                    throw Error(ERR_UNSUPPORTED, "no descriptor backward kernel compiled for this shape");
+            }
+        }
+        if (timed) JNN_HIP(hipEventRecord(ev[5], stream));
+        if (grad) launch_gather(stream, n, d_xw, box, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0);
+        launch_reduce(stream, n, 1, 1, d_E, d_partial, kPartials, d_tot);
+        if (what & JSE_NNAP_VIRIAL) launch_reduce(stream, n, 9, 9, d_V, d_partial, kPartials, d_tot + 1);
+        if (timed) JNN_HIP(hipEventRecord(ev[6], stream));
+        last_what = what;
+    }
+    int last_what = 0;
+};
+
+extern "C" {
+
+const char *jse_nnap_last_error(void) { return g_last_error.c_str(); }
+
+int jse_nnap_model_load(const char *json_or_path, jse_nnap_model *out) {
+    return guard([&] {
+        JNN_CHECK(json_or_path && out, ERR_ARG, "null argument");
+        std::string text(json_or_path);
+        size_t k = text.find_first_not_of(" \t\r\n");
+        if (k == std::string::npos || text[k] != '{') {
+            std::ifstream f(text);
+            JNN_CHECK(f.good(), ERR_ARG, "cannot open model file: " + text);
+            std::stringstream ss;
+            ss << f.rdbuf();
+            text = ss.str();
+        }
+        auto m = std::make_unique<jse_nnap_model_s>();
+        m->host = parse_model_json(text);
+        m->upload();
+        *out = m.release();
+    });
+}
+
+int jse_nnap_model_parse(const char *json_or_path, int *ntypes, int *size_np, int *fp_size) {
+    return guard([&] {
+        JNN_CHECK(json_or_path && ntypes, ERR_ARG, "null argument");
+        std::string text(json_or_path);
+        size_t k = text.find_first_not_of(" \t\r\n");
+        if (k == std::string::npos || text[k] != '{') {
+            std::ifstream f(text);
+            JNN_CHECK(f.good(), ERR_ARG, "cannot open model file: " + text);
+            std::stringstream ss;
+            ss << f.rdbuf();
+            text = ss.str();
+        }
+        ModelHost h = parse_model_json(text);
+        *ntypes = h.ntypes();
+        for (int t = 0; t < h.ntypes(); ++t) {
+            if (size_np) size_np[t] = h.species[t].P;
+            if (fp_size) fp_size[t] = h.species[t].S();
+        }
+    });
+}
+
+int jse_nnap_model_create(int ntypes, const int *nmax, const int *lmax, const int *size_np, const double *rcut,
+                          const int *nlayers, const int *hidden, const double *const *fp_param,
+                          const double *const *nn_param, const double *const *norm_param, jse_nnap_model *out) {
+    return guard([&] {
+        JNN_CHECK(out && ntypes >= 1 && nmax && lmax && size_np && rcut && nlayers && hidden && fp_param && nn_param &&
+                      norm_param,
+                  ERR_ARG, "null argument");
+        auto m = std::make_unique<jse_nnap_model_s>();
+        for (int t = 0; t < ntypes; ++t) {
+            SpeciesHost sp;
+            sp.symbol = std::to_string(t + 1);
+            sp.nmax = nmax[t];
+            sp.lmax = lmax[t];
+            sp.P = size_np[t];
+            sp.rcut = rcut[t];
+            JNN_CHECK(sp.lmax >= 0 && sp.lmax <= 12 && sp.nmax >= 0 && sp.nmax < 32 && sp.P >= 1 && sp.P <= 128 &&
+                          sp.rcut > 0 && nlayers[t] >= 1 && nlayers[t] <= kMaxLayers,
+                      ERR_ARG, "species shape out of range");
+            for (int l = 0; l < nlayers[t]; ++l) sp.hidden.push_back(hidden[t * kMaxLayers + l]);
+            const size_t nw = (size_t)ntypes * sp.P * sp.K();
+            sp.W.assign(fp_param[t], fp_param[t] + nw);
+            size_t nn = 0;
+            int c = sp.S();
+            for (int h : sp.hidden) { nn += (size_t)c * h + h; c = h; }
+            nn += (size_t)c + 1;
+            sp.nn.assign(nn_param[t], nn_param[t] + nn);
+            sp.norm.assign(norm_param[t], norm_param[t] + 2 + 2 * sp.S());
+            m->host.species.push_back(std::move(sp));
+        }
+        finalize_species(m->host);
+        m->upload();
+        *out = m.release();
+    });
+}
+
+int jse_nnap_model_info(jse_nnap_model m, int *ntypes, double *rcut_max) {
+    return guard([&] {
+        JNN_CHECK(m, ERR_ARG, "null model");
+        if (ntypes) *ntypes = m->host.ntypes();
+        if (rcut_max) *rcut_max = m->host.rcut_max();
+    });
+}
+int jse_nnap_model_fp_size(jse_nnap_model m, int ctype, int *size) {
+    return guard([&] {
+        JNN_CHECK(m && size, ERR_ARG, "null argument");
+        JNN_CHECK(ctype >= 1 && ctype <= m->host.ntypes(), ERR_TYPE, "type out of range");
+        *size = m->host.species[ctype - 1].S();
+    });
+}
+void jse_nnap_model_destroy(jse_nnap_model m) { delete m; }
+
+int jse_nnap_system_create(jse_nnap_model m, int natoms, const int *types, const double *box, int precision,
+                           jse_nnap_system *out) {
+    return guard([&] {
+        JNN_CHECK(m && out && box && (types || natoms == 0), ERR_ARG, "null argument");
+        auto s = std::make_unique<jse_nnap_system_s>();
+        s->init(m, natoms, types, box, precision);
+        *out = s.release();
+    });
+}
+void jse_nnap_system_destroy(jse_nnap_system s) { delete s; }
+
+int jse_nnap_system_set_box(jse_nnap_system s, const double *box) {
+    return guard([&] {
+        JNN_CHECK(s && box, ERR_ARG, "null argument");
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        s->set_box(box);
+    });
+}
+
+int jse_nnap_system_set_positions(jse_nnap_system s, const double *xyz) {
+    return guard([&] {
+        JNN_CHECK(s && (xyz || s->n == 0), ERR_ARG, "null argument");
+        for (size_t q = 0; q < (size_t)3 * s->n; ++q)
+            JNN_CHECK(std::isfinite(xyz[q]), ERR_ARG, "non-finite position at atom " + std::to_string(q / 3));
+        if (s->n) JNN_HIP(hipMemcpyAsync(s->d_xyz, xyz, sizeof(double) * 3 * s->n, hipMemcpyHostToDevice, s->stream));
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        s->have_pos = true;
+        s->have_nl = false;
+    });
+}
+int jse_nnap_system_set_positions_device(jse_nnap_system s, const double *d_xyz) {
+    return guard([&] {
+        JNN_CHECK(s && (d_xyz || s->n == 0), ERR_ARG, "null argument");
+        if (s->n)
+            JNN_HIP(hipMemcpyAsync(s->d_xyz, d_xyz, sizeof(double) * 3 * s->n, hipMemcpyDeviceToDevice, s->stream));
+        s->have_pos = true;
+        s->have_nl = false;
+    });
+}
+
+int jse_nnap_system_compute(jse_nnap_system s, int what) {
+    return guard([&] {
+        JNN_CHECK(s, ERR_ARG, "null system");
+        JNN_CHECK((what & ~7) == 0, ERR_ARG, "unknown compute flags");
+        s->compute(what | JSE_NNAP_ENERGY, false);
+    });
+}
+
+int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energies, double *forces, double *virial,
+                                double *virials) {
+    return guard([&] {
+        JNN_CHECK(s, ERR_ARG, "null system");
+        JNN_CHECK(s->last_what != 0, ERR_ARG, "nothing computed yet");
+        const bool grad = (s->last_what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL)) != 0;
+        const bool vir = (s->last_what & JSE_NNAP_VIRIAL) != 0;
+        JNN_CHECK(!(forces && !grad), ERR_ARG, "forces were not computed");
+        JNN_CHECK(!((virial || virials) && !vir), ERR_ARG, "virials were not computed");
+        double tot[10] = {0};
+        JNN_HIP(hipMemcpyAsync(tot, s->d_tot, sizeof(double) * (vir ? 10 : 1), hipMemcpyDeviceToHost, s->stream));
+        if (energies && s->n)
+            JNN_HIP(hipMemcpyAsync(energies, s->d_E, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
+        if (forces && s->n)
+            JNN_HIP(hipMemcpyAsync(forces, s->d_F, sizeof(double) * 3 * s->n, hipMemcpyDeviceToHost, s->stream));
+        if (virials && s->n)
+            JNN_HIP(hipMemcpyAsync(virials, s->d_V, sizeof(double) * 9 * s->n, hipMemcpyDeviceToHost, s->stream));
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        if (energy) *energy = tot[0];
+        if (virial)
+            for (int c = 0; c < 9; ++c) virial[c] = tot[1 + c];
+    });
+}
+
+int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
+    return guard([&] {
+        JNN_CHECK(s && (fp || s->n == 0), ERR_ARG, "null argument");
+        JNN_CHECK(smax >= s->model->smax, ERR_ARG, "smax smaller than the largest descriptor");
+        s->build_nl(false);
+        s->desc_fwd(true);
+        std::vector<float> x((size_t)s->n * s->model->smax);
+        std::vector<int> rows(s->n);
+        if (s->n) {
+            JNN_HIP(hipMemcpyAsync(x.data(), s->d_x, sizeof(float) * x.size(), hipMemcpyDeviceToHost, s->stream));
+            JNN_HIP(hipMemcpyAsync(rows.data(), s->d_rows, sizeof(int) * s->n, hipMemcpyDeviceToHost, s->stream));
+        }
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        for (int t = 0; t < s->model->host.ntypes(); ++t) {
+            const int S = s->model->host.species[t].S();
+            for (int r = s->seg[t]; r < s->seg[t + 1]; ++r) {
+                double *o = fp + (size_t)rows[r] * smax;
+                for (int e = 0; e < smax; ++e) o[e] = e < S ? (double)x[(size_t)r * s->model->smax + e] : 0.0;
+            }
+        }
+    });
+}
+
+int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *key, float *d) {
+    int rc = guard([&] {
+        JNN_CHECK(s && count, ERR_ARG, "null argument");
+        if (!s->have_nl) s->build_nl(false);
+        const int n = s->n, c = s->nl.cap;
+        std::vector<int> cnt(n), k((size_t)n * c);
+        std::vector<float> dx((size_t)n * c), dy((size_t)n * c), dz((size_t)n * c);
+        if (n) {
+            JNN_HIP(hipMemcpyAsync(cnt.data(), s->nl.cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));
+            JNN_HIP(hipMemcpyAsync(k.data(), s->nl.key, sizeof(int) * k.size(), hipMemcpyDeviceToHost, s->stream));
+            JNN_HIP(hipMemcpyAsync(dx.data(), s->nl.dx, sizeof(float) * k.size(), hipMemcpyDeviceToHost, s->stream));
+            JNN_HIP(hipMemcpyAsync(dy.data(), s->nl.dy, sizeof(float) * k.size(), hipMemcpyDeviceToHost, s->stream));
+            JNN_HIP(hipMemcpyAsync(dz.data(), s->nl.dz, sizeof(float) * k.size(), hipMemcpyDeviceToHost, s->stream));
+        }
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        bool over = false;
+        for (int i = 0; i < n; ++i) {
+            count[i] = cnt[i];
+            const int m = std::min(std::min(cnt[i], c), cap);
+            if (cnt[i] > cap) over = true;
+            for (int q = 0; q < m; ++q) {
+                const size_t src = (size_t)i * c + q, dst = (size_t)i * cap + q;
+                if (key) key[dst] = k[src];
+                if (d) {
+                    d[3 * dst] = dx[src];
+                    d[3 * dst + 1] = dy[src];
+                    d[3 * dst + 2] = dz[src];
+                }
+            }
+        }
+        if (over) throw Error(ERR_CAPACITY, "a neighbour row is longer than cap");
+    });
+    return rc;
+}
+
+int jse_nnap_system_benchmark(jse_nnap_system s, int what, int warmup, int iters, double *ms_step, double *phase_ms,
+                              int nphase) {
+    return guard([&] {
+        JNN_CHECK(s && iters > 0 && warmup >= 0, ERR_ARG, "bad benchmark arguments");
+        for (int w = 0; w < warmup; ++w) s->compute(what | JSE_NNAP_ENERGY, false);
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        double acc[6] = {0};
+        hipEvent_t t0, t1;
+        JNN_HIP(hipEventCreate(&t0));
+        JNN_HIP(hipEventCreate(&t1));
+        JNN_HIP(hipEventRecord(t0, s->stream));
+        for (int it = 0; it < iters; ++it) {
+            s->compute(what | JSE_NNAP_ENERGY, true);
+            JNN_HIP(hipEventSynchronize(s->ev[6]));
+            float ms;
+            for (int k = 0; k < 6; ++k) {
+                JNN_HIP(hipEventElapsedTime(&ms, s->ev[k], s->ev[k + 1]));
+                acc[k] += ms;
+            }
+        }
+        JNN_HIP(hipEventRecord(t1, s->stream));
+        JNN_HIP(hipEventSynchronize(t1));
+        float tot;
+        JNN_HIP(hipEventElapsedTime(&tot, t0, t1));
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        if (ms_step) *ms_step = tot / iters;
+        for (int k = 0; k < nphase && k < 6; ++k) phase_ms[k] = acc[k] / iters;
+    });
+}
+
+int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long long *pairs, long long *unmatched) {
+    return guard([&] {
+        JNN_CHECK(s, ERR_ARG, "null system");
+        if (!s->have_nl) s->build_nl(false);
+        const int n = s->n, c = s->nl.cap;
+        std::vector<int> cnt(n), rev((size_t)n * c);
+        if (n) {
+            JNN_HIP(hipMemcpyAsync(cnt.data(), s->nl.cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));
+            JNN_HIP(hipMemcpyAsync(rev.data(), s->nl.rev, sizeof(int) * rev.size(), hipMemcpyDeviceToHost, s->stream));
+        }
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        long long p = 0, u = 0;
+        int mx = 0;
+        for (int i = 0; i < n; ++i) {
+            mx = std::max(mx, cnt[i]);
+            const int m = std::min(cnt[i], c);
+            p += m;
+            for (int q = 0; q < m; ++q)
+                if (rev[(size_t)i * c + q] < 0) ++u;
+        }
+        if (cap) *cap = c;
+        if (max_count) *max_count = mx;
+        if (pairs) *pairs = p;
+        if (unmatched) *unmatched = u;
+    });
+}
+
+}  // extern "C"

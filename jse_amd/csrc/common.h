@@ -1,0 +1,75 @@
+// Shared host-side definitions of libjsenn_hip.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace jnn {
+
+// Error raised inside the library; the C-ABI turns it into a status code and
+// a message readable with jse_nnap_last_error().
+struct Error : std::runtime_error {
+    int code;
+    Error(int c, const std::string &m) : std::runtime_error(m), code(c) {}
+};
+
+enum Status : int {
+    OK = 0,
+    ERR_ARG = 1,        // invalid argument / model
+    ERR_UNSUPPORTED = 2,// model shape without a compiled kernel
+    ERR_HIP = 3,        // HIP runtime error
+    ERR_CAPACITY = 4,   // neighbour capacity exceeded (internal retry failed)
+    ERR_TYPE = 5,       // atom type outside [1, ntypes]
+    ERR_MEMORY = 6,     // device memory budget exhausted
+};
+
+#define JNN_HIP(call)                                                                          \
+    do {                                                                                       \
+        hipError_t e_ = (call);                                                                \
+        if (e_ != hipSuccess)                                                                  \
+            throw ::jnn::Error(::jnn::ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e_)); \
+    } while (0)
+
+#define JNN_CHECK(cond, code, msg)                  \
+    do {                                            \
+        if (!(cond)) throw ::jnn::Error((code), (msg)); \
+    } while (0)
+
+// radial mixing specialisations of the kernels
+enum MixKind : int { MIX_NONE = 0, MIX_DEFAULT = 1, MIX_DENSE = 2 };
+
+constexpr int kMaxLayers = 8;
+
+// One species of a model, FP64 blocks in the reference's packing plus derived shapes.
+struct SpeciesHost {
+    std::string symbol;
+    int nmax = 5, lmax = 6, P = 6, wtype = 0, typed_wtype = 0;
+    double rcut = 6.0;
+    std::vector<int> hidden;          // hidden layer widths
+    std::vector<double> W;            // [ntypes][P][K]
+    std::vector<double> nn;           // [hidden W | out W | hidden b | out b]
+    std::vector<double> norm;         // [muE+refE, sigmaE, mu[S], sigma[S]]
+    int K() const { return nmax + 1; }
+    int M() const { return (lmax + 1) * (lmax + 1); }
+    int S() const { return P * (lmax + 1); }
+    int mix = MIX_DENSE;              // kernel specialisation for W
+};
+
+struct ModelHost {
+    std::vector<SpeciesHost> species;
+    int ntypes() const { return (int)species.size(); }
+    double rcut_max() const {
+        double r = 0;
+        for (auto &s : species) r = s.rcut > r ? s.rcut : r;
+        return r;
+    }
+};
+
+// model_host.cpp
+ModelHost parse_model_json(const std::string &text);
+void finalize_species(ModelHost &m);   // derive mix kind, validate sizes
+
+}  // namespace jnn

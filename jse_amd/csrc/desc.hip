@@ -1,0 +1,353 @@
+// Spherical-Chebyshev descriptor, forward and backward, FP32, one wavefront
+// per atom.
+//
+// Forward (reference: basis_SphericalChebyshev.hpp:264-351, the L2 power
+// spectrum of basis_SphericalUtil.hpp:378-394):
+//   per neighbour j (r < rc):  c_j[p] = fc(r) * sum_n W[type_j][p][n] T_n(1-2r/rc),
+//                              Y_j[lm] real orthonormal harmonics
+//   A[p][lm] = sum_j c_j[p] Y_j[lm]          (per-atom rank-n contraction)
+//   fp[p*(lmax+1)+l] = 4pi/(2l+1) sum_m A[p][l^2+l+m]^2
+//   x = (fp - mu) / sigma                    (nnap_main.hpp:274-279)
+// Lanes first compute one neighbour each (c_j, Y_j -> LDS); then the P x M
+// output is cut into TP x TM register tiles, NT tiles per neighbour group and
+// G = 64/NT groups that take every G-th neighbour, and the G partial tiles
+// are summed in a fixed order through LDS.
+//
+// Backward (reference: basis_SphericalChebyshev.hpp:353-465): with
+// Gc[p][l] = 2*4pi/(2l+1) * dE/dfp[p][l] (from the MLP pass) the gradient of
+// E with respect to A is Gc[p][l(m)] * A[p][m].  One lane per neighbour:
+//   gY[m] = sum_p Gc[p][l(m)] A[p][m] c[p],  gc[p] = sum_l Gc[p][l] sum_{m in l} A[p][m] Y[m]
+// with A and Gc read at wave-uniform addresses; then the chain rule through
+// fc, Rn, W and (r, theta, phi) gives g_j = -dE_i/d(r_j - r_i).  The pole
+// convention of the reference (dxy == 0 -> cos phi = 1, sin phi = 0, the phi
+// term dropped; basis_SphericalUtil.hpp:165-181) is kept.
+#include "common.h"
+#include "device_math.h"
+#include "kernels.h"
+
+namespace jnn {
+
+template <int LMAX, int K, int P, int MIX>
+struct DescShape {
+    static constexpr int L1 = LMAX + 1;
+    static constexpr int M = L1 * L1;
+    static constexpr int S = P * L1;
+    static constexpr int TP = P >= 6 ? 6 : P;
+    static constexpr int NPB = (P + TP - 1) / TP;
+    static constexpr int TM = 7;
+    static constexpr int NMB = (M + TM - 1) / TM;
+    static constexpr int NT = NPB * NMB;
+    static constexpr int G = NT >= 64 ? 1 : 64 / NT;
+    static_assert(NT <= 64, "descriptor tile layout needs NT <= 64");
+    static constexpr int RED = G * NPB * TP * NMB * TM;  // reduction buffer floats
+    static constexpr int YBUF = (64 * M > RED) ? 64 * M : RED;
+};
+
+// c[p] = fc * sum_n W[t][p][n] Rn[n] for the three mixing specialisations
+template <int K, int P, int MIX>
+__device__ __forceinline__ void mix_rnp(const float *rn, int tj, const float *__restrict__ W, float *rnp) {
+    if (MIX == MIX_NONE) {
+#pragma unroll
+        for (int p = 0; p < P; ++p) rnp[p] = rn[p];
+    } else if (MIX == MIX_DEFAULT) {
+        const float wt = (tj & 1) ? (float)tj : -(float)tj;
+#pragma unroll
+        for (int q = 0; q < K; ++q) {
+            rnp[q] = rn[q];
+            rnp[K + q] = wt * rn[q];
+        }
+    } else {
+        const float *w = W + (size_t)(tj - 1) * P * K;
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            float s = 0.f;
+#pragma unroll
+            for (int q = 0; q < K; ++q) s += w[p * K + q] * rn[q];
+            rnp[p] = s;
+        }
+    }
+}
+// gRn[n] = sum_p W[t][p][n] gRnp[p]
+template <int K, int P, int MIX>
+__device__ __forceinline__ void mix_rnp_back(const float *grnp, int tj, const float *__restrict__ W, float *grn) {
+    if (MIX == MIX_NONE) {
+#pragma unroll
+        for (int q = 0; q < K; ++q) grn[q] = grnp[q];
+    } else if (MIX == MIX_DEFAULT) {
+        const float wt = (tj & 1) ? (float)tj : -(float)tj;
+#pragma unroll
+        for (int q = 0; q < K; ++q) grn[q] = grnp[q] + wt * grnp[K + q];
+    } else {
+        const float *w = W + (size_t)(tj - 1) * P * K;
+#pragma unroll
+        for (int q = 0; q < K; ++q) grn[q] = 0.f;
+#pragma unroll
+        for (int p = 0; p < P; ++p)
+#pragma unroll
+            for (int q = 0; q < K; ++q) grn[q] += w[p * K + q] * grnp[p];
+    }
+}
+
+struct PairGeom {
+    float r, ct, st, cp, sp, dxy;
+    bool pole;
+};
+__device__ __forceinline__ PairGeom pair_geom(float dx, float dy, float dz) {
+    PairGeom g;
+    g.r = sqrtf(dx * dx + dy * dy + dz * dz);
+    g.dxy = hypotf(dx, dy);
+    g.ct = dz / g.r;
+    g.st = g.dxy / g.r;
+    g.pole = !(g.dxy >= 8.f * 1.17549435e-38f);
+    g.cp = g.pole ? 1.f : dx / g.dxy;
+    g.sp = g.pole ? 0.f : dy / g.dxy;
+    return g;
+}
+
+template <int LMAX, int K, int P, int MIX>
+__global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl) {
+    using SH = DescShape<LMAX, K, P, MIX>;
+    constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
+    constexpr int TP = SH::TP, TM = SH::TM, NMB = SH::NMB, NT = SH::NT, G = SH::G;
+    __shared__ float sY[SH::YBUF];
+    __shared__ float sC[64 * P];
+    __shared__ float sA[P * M];
+    const int lane = threadIdx.x;
+    const int grp = lane / NT, tile = lane % NT;
+    const int pb = tile / NMB, mb = tile % NMB;
+    const int p0 = pb * TP, m0 = mb * TM;
+    const bool tile_on = grp < G;
+    const float rc = a.rc, inv_rc = 1.f / a.rc;
+
+    for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
+        const int i = a.rows_atom[row];
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        float acc[TP][TM];
+#pragma unroll
+        for (int u = 0; u < TP; ++u)
+#pragma unroll
+            for (int v = 0; v < TM; ++v) acc[u][v] = 0.f;
+
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int j = c0 + lane;
+            float y[M];
+            float c[P];
+#pragma unroll
+            for (int m = 0; m < M; ++m) y[m] = 0.f;
+#pragma unroll
+            for (int p = 0; p < P; ++p) c[p] = 0.f;
+            if (j < n) {
+                const float dx = nl.dx[base + j], dy = nl.dy[base + j], dz = nl.dz[base + j];
+                const int tj = a.types[nl.key[base + j] >> 5];
+                const PairGeom g = pair_geom(dx, dy, dz);
+                if (g.r < rc) {
+                    real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
+                    float rn[K];
+                    cheb_rn<K>(1.f - 2.f * g.r * inv_rc, rn);
+                    const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+                    const float t2 = t * t;
+                    const float fc = t2 * t2;
+                    float rnp[P];
+                    mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
+#pragma unroll
+                    for (int p = 0; p < P; ++p) c[p] = fc * rnp[p];
+                }
+            }
+#pragma unroll
+            for (int m = 0; m < M; ++m) sY[lane * M + m] = y[m];
+#pragma unroll
+            for (int p = 0; p < P; ++p) sC[lane * P + p] = c[p];
+            __syncthreads();
+            const int nc = min(64, n - c0);
+            if (tile_on) {
+                for (int jj = grp; jj < nc; jj += G) {
+                    float cv[TP], yv[TM];
+#pragma unroll
+                    for (int u = 0; u < TP; ++u) cv[u] = (p0 + u < P) ? sC[jj * P + p0 + u] : 0.f;
+#pragma unroll
+                    for (int v = 0; v < TM; ++v) yv[v] = (m0 + v < M) ? sY[jj * M + m0 + v] : 0.f;
+#pragma unroll
+                    for (int u = 0; u < TP; ++u)
+#pragma unroll
+                        for (int v = 0; v < TM; ++v) acc[u][v] += cv[u] * yv[v];
+                }
+            }
+            __syncthreads();
+        }
+        // fixed-order sum of the G partial tiles
+        if (tile_on) {
+#pragma unroll
+            for (int u = 0; u < TP; ++u)
+#pragma unroll
+                for (int v = 0; v < TM; ++v) {
+                    const int p = p0 + u, m = m0 + v;
+                    if (p < P && m < M) sY[(grp * P + p) * M + m] = acc[u][v];
+                }
+        }
+        __syncthreads();
+        for (int e = lane; e < P * M; e += 64) {
+            float s = 0.f;
+#pragma unroll
+            for (int q = 0; q < G; ++q) s += sY[q * P * M + e];
+            sA[e] = s;
+            a.A[(size_t)row * (P * M) + e] = s;
+        }
+        __syncthreads();
+        for (int e = lane; e < S; e += 64) {
+            const int p = e / L1, l = e % L1;
+            float s = 0.f;
+            for (int m = l * l; m < (l + 1) * (l + 1); ++m) {
+                const float q = sA[p * M + m];
+                s += q * q;
+            }
+            const float fp = (kPi4 / (float)(2 * l + 1)) * s;
+            a.x[(size_t)row * S + e] = (fp - a.mu[e]) / a.sig[e];
+        }
+        __syncthreads();
+    }
+}
+
+template <int LMAX, int K, int P, int MIX>
+__global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl) {
+    using SH = DescShape<LMAX, K, P, MIX>;
+    constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
+    __shared__ float sGA[P * M];  // dE/dA of the current atom
+    const int lane = threadIdx.x;
+    const float rc = a.rc, inv_rc = 1.f / a.rc;
+    for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
+        const int i = a.rows_atom[row];
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        const float *__restrict__ Arow = a.A + (size_t)row * (P * M);
+        const float *__restrict__ Grow = a.G + (size_t)row * S;
+        for (int e = lane; e < P * M; e += 64) {
+            const int p = e / M, m = e % M;
+            int l = 0;
+            while ((l + 1) * (l + 1) <= m) ++l;
+            sGA[e] = Grow[p * L1 + l] * Arow[e];
+        }
+        __syncthreads();
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int j = c0 + lane;
+            if (j >= n) continue;
+            const float dx = nl.dx[base + j], dy = nl.dy[base + j], dz = nl.dz[base + j];
+            const int tj = a.types[nl.key[base + j] >> 5];
+            const PairGeom g = pair_geom(dx, dy, dz);
+            float ox = 0.f, oy = 0.f, oz = 0.f;
+            if (g.r < rc) {
+                float y[M];
+                real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
+                const float xr = 1.f - 2.f * g.r * inv_rc;
+                float rn[K];
+                cheb_rn<K>(xr, rn);
+                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+                const float t2 = t * t;
+                const float fc = t2 * t2;
+                float rnp[P];
+                mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
+                // gY[m] = sum_p gA[p][m] c[p],  gc[p] = sum_m gA[p][m] Y[m]
+                float gy[M];
+#pragma unroll
+                for (int m = 0; m < M; ++m) gy[m] = 0.f;
+                float gfc = 0.f;
+                float grnp[P];
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    const float cp_ = fc * rnp[p];
+                    float s = 0.f;
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        const float ga = sGA[p * M + m];
+                        gy[m] += ga * cp_;
+                        s += ga * y[m];
+                    }
+                    gfc += s * rnp[p];
+                    grnp[p] = s * fc;
+                    // keep the LDS reads of one p-row from being hoisted across rows (register pressure)
+                    __builtin_amdgcn_sched_barrier(0);
+                }
+                float grn[K];
+                mix_rnp_back<K, P, MIX>(grnp, tj, a.W, grn);
+                float drn[K];
+                cheb_drn<K>(xr, inv_rc, drn);
+                float dEdr = 0.f;
+#pragma unroll
+                for (int q = 0; q < K; ++q) dEdr += grn[q] * drn[q];
+                const float dfc = -8.f * g.r * inv_rc * inv_rc * t2 * t;
+                dEdr += gfc * dfc;
+                const float dEdth = dtheta_contract<LMAX>(g.ct, g.st, g.cp, g.sp, gy);
+                float dEdph = 0.f;
+#pragma unroll
+                for (int l = 1; l < L1; ++l)
+#pragma unroll
+                    for (int m = 1; m <= l; ++m) {
+                        const int idx = l * l + l;
+                        // dY_{l,m}/dphi = -m Y_{l,-m},  dY_{l,-m}/dphi = m Y_{l,m}
+                        dEdph += (float)m * (gy[idx - m] * y[idx + m] - gy[idx + m] * y[idx - m]);
+                    }
+                const float ir = 1.f / g.r;
+                // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi))
+                ox = -(dEdr * dx * ir + dEdth * g.ct * g.cp * ir);
+                oy = -(dEdr * dy * ir + dEdth * g.ct * g.sp * ir);
+                oz = -(dEdr * dz * ir - dEdth * g.st * ir);
+                if (!g.pole) {
+                    const float idxy = 1.f / g.dxy;
+                    ox -= dEdph * (-g.sp * idxy);
+                    oy -= dEdph * (g.cp * idxy);
+                }
+            }
+            nl.gx[base + j] = ox;
+            nl.gy[base + j] = oy;
+            nl.gz[base + j] = oz;
+        }
+        __syncthreads();
+    }
+}
+
+// ---- dispatch table over the compiled shapes ----
+template <int LMAX, int K, int P, int MIX>
+static void run_fwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
+    const int grid = a.nrows < 262144 ? a.nrows : 262144;
+    hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, a, nl);
+}
+template <int LMAX, int K, int P, int MIX>
+static void run_bwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
+    const int grid = a.nrows < 262144 ? a.nrows : 262144;
+    hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, a, nl);
+}
+
+#define JNN_SHAPES(X)              \
+    X(6, 6, 6, MIX_NONE)           \
+    X(6, 6, 12, MIX_DEFAULT)       \
+    X(6, 6, 12, MIX_DENSE)         \
+    X(6, 9, 9, MIX_NONE)           \
+    X(6, 9, 18, MIX_DEFAULT)       \
+    X(6, 9, 18, MIX_DENSE)         \
+    X(4, 6, 6, MIX_NONE)           \
+    X(4, 6, 12, MIX_DEFAULT)
+
+bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl) {
+    if (a.nrows <= 0) return true;
+#define X(L_, K_, P_, M_)                                              \
+    if (lmax == L_ && K == K_ && P == P_ && mix == M_) {              \
+        run_fwd<L_, K_, P_, M_>(s, a, nl);                            \
+        return true;                                                   \
+    }
+    JNN_SHAPES(X)
+#undef X
+    return false;
+}
+bool launch_desc_bwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl) {
+    if (a.nrows <= 0) return true;
+#define X(L_, K_, P_, M_)                                              \
+    if (lmax == L_ && K == K_ && P == P_ && mix == M_) {              \
+        run_bwd<L_, K_, P_, M_>(s, a, nl);                            \
+        return true;                                                   \
+    }
+    JNN_SHAPES(X)
+#undef X
+    return false;
+}
+
+}  // namespace jnn

@@ -1,0 +1,80 @@
+// Host-callable launchers of the HIP kernels in libjsenn_hip.  Every launcher
+// enqueues on the given stream and never synchronises.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace jnn {
+
+// Periodic cell grid of an orthogonal box (reference: NeighborListGetter.java:952-1001).
+struct CellGrid {
+    int size[3];
+    int ncell;
+    double cb[3];   // cell edge per axis = L_d / size_d
+    double box[3];
+    double rc2;     // rmax*rmax
+};
+
+// Neighbour-list storage of one system (device pointers), rows of `cap` slots per atom,
+// each row sorted by key = j*32 + shift_code.
+struct NeighborList {
+    int cap = 0;
+    int *cnt = nullptr;     // [N] true neighbour count (may exceed cap -> overflow)
+    int *key = nullptr;     // [N*cap]
+    float *dx = nullptr, *dy = nullptr, *dz = nullptr;  // [N*cap] FP32 of the FP64 displacement
+    int *rev = nullptr;     // [N*cap] slot of the reverse pair in j's row, -1 if absent
+    float *gx = nullptr, *gy = nullptr, *gz = nullptr;  // [N*cap] g_j = -dE_i/d(r_j - r_i)
+};
+
+// ---- neighbour build ----
+void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const CellGrid &g, double *xw, int *cell,
+                     int *cell_cnt, int *rank);
+void launch_cell_scan(hipStream_t s, int ncell, const int *cell_cnt, int *cell_start);
+void launch_cell_scatter(hipStream_t s, int n, const int *cell, const int *rank, const int *cell_start,
+                         const double *xw, int *sidx, double *sxw);
+void launch_nl_build(hipStream_t s, int n, const CellGrid &g, const double *xw, const int *cell, const int *cell_start,
+                     const int *sidx, const double *sxw, NeighborList nl, int *overflow);
+void launch_nl_reverse(hipStream_t s, int n, NeighborList nl);
+
+// ---- descriptor / MLP / backward for one species segment ----
+struct SegmentArgs {
+    int nrows;              // atoms of this species
+    const int *rows_atom;   // atom index of each row (species-sorted order)
+    const int *types;       // [N] 1-based types
+    float rc;
+    const float *W;         // [ntypes][P][K] FP32 mixing (used by the dense kernel)
+    const float *mu, *sig;  // normalisation [S]
+    float *x;               // [nrows][S] normalised descriptors (forward out, MLP in)
+    float *A;               // [nrows][P*M]
+    float *G;               // [nrows][S]  2*(4pi/(2l+1)) * dE/dfp
+};
+// returns false when no kernel is compiled for this (lmax, K, P, mix)
+bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);
+bool launch_desc_bwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);
+
+struct MlpArgs {
+    int nrows, S, L1, nlayers;
+    int hidden[8];
+    const int *rows_atom;
+    const float *x;          // [nrows][S]
+    const float *w_in_out;   // per layer [in][out] (forward layout) then output W
+    const float *w_out_in;   // per layer [out][in] (backward layout)
+    const float *bias;       // hidden biases, then output bias
+    const float *sig;        // [S]
+    float sig_e, mu_e;
+    double *E;               // [N] per-atom energy (by atom index)
+    float *G;                // [nrows][S]
+    int want_grad;
+};
+void launch_mlp_fp32(hipStream_t s, const MlpArgs &a);
+void launch_mlp_bf16(hipStream_t s, const MlpArgs &a, const uint16_t *w_bf16_fwd, const uint16_t *w_bf16_bwd);
+
+// ---- forces / virials / totals ----
+void launch_gather(hipStream_t s, int n, const double *xw, const double *box, const NeighborList &nl, double *F,
+                   double *V, int want_virial);
+// deterministic FP64 sums: out[c] = sum_i in[i*stride + c] for c < ncomp
+void launch_reduce(hipStream_t s, int n, int stride, int ncomp, const double *in, double *partial, int npartial,
+                   double *out);
+
+}  // namespace jnn

@@ -1,0 +1,360 @@
+// Model-file loading for libjsenn_hip: a small JSON reader and the
+// construction of the packed per-species blocks, following the reference's
+// NNAP constructor (src/nnap/java/jsex/nnap/NNAP.java:129-307), the basis
+// loader (basis/SphericalChebyshev.java:60-103, basis/WTypeBasis.java:202-354)
+// and FeedForward.load (nn/FeedForward.java:131-221).
+#include <cmath>
+#include <cstdlib>
+#include <map>
+#include <memory>
+
+#include "common.h"
+
+namespace jnn {
+namespace {
+
+struct JVal {
+    enum Kind { NUL, BOOL, NUM, STR, ARR, OBJ } kind = NUL;
+    bool b = false;
+    double num = 0;
+    std::string str;
+    std::vector<JVal> arr;
+    std::vector<std::pair<std::string, JVal>> obj;
+    const JVal *get(const char *k) const {
+        if (kind != OBJ) return nullptr;
+        for (auto &kv : obj)
+            if (kv.first == k) return kv.second.kind == NUL ? nullptr : &kv.second;
+        return nullptr;
+    }
+};
+
+struct JParser {
+    const std::string &s;
+    size_t p = 0;
+    explicit JParser(const std::string &t) : s(t) {}
+    [[noreturn]] void fail(const char *what) {
+        throw Error(ERR_ARG, std::string("model JSON: ") + what + " at offset " + std::to_string(p));
+    }
+    void ws() {
+        while (p < s.size() && (s[p] == ' ' || s[p] == '\n' || s[p] == '\r' || s[p] == '\t')) ++p;
+    }
+    JVal value() {
+        ws();
+        if (p >= s.size()) fail("unexpected end");
+        char c = s[p];
+        JVal v;
+        if (c == '{') {
+            v.kind = JVal::OBJ;
+            ++p;
+            ws();
+            if (p < s.size() && s[p] == '}') { ++p; return v; }
+            for (;;) {
+                ws();
+                if (p >= s.size() || s[p] != '"') fail("expected key");
+                std::string k = str();
+                ws();
+                if (p >= s.size() || s[p] != ':') fail("expected ':'");
+                ++p;
+                v.obj.emplace_back(k, value());
+                ws();
+                if (p < s.size() && s[p] == ',') { ++p; continue; }
+                if (p < s.size() && s[p] == '}') { ++p; break; }
+                fail("expected ',' or '}'");
+            }
+        } else if (c == '[') {
+            v.kind = JVal::ARR;
+            ++p;
+            ws();
+            if (p < s.size() && s[p] == ']') { ++p; return v; }
+            for (;;) {
+                v.arr.push_back(value());
+                ws();
+                if (p < s.size() && s[p] == ',') { ++p; continue; }
+                if (p < s.size() && s[p] == ']') { ++p; break; }
+                fail("expected ',' or ']'");
+            }
+        } else if (c == '"') {
+            v.kind = JVal::STR;
+            v.str = str();
+        } else if (s.compare(p, 4, "true") == 0) {
+            v.kind = JVal::BOOL; v.b = true; p += 4;
+        } else if (s.compare(p, 5, "false") == 0) {
+            v.kind = JVal::BOOL; v.b = false; p += 5;
+        } else if (s.compare(p, 4, "null") == 0) {
+            p += 4;
+        } else if (s.compare(p, 3, "NaN") == 0) {
+            v.kind = JVal::NUM; v.num = NAN; p += 3;
+        } else {
+            const char *b = s.c_str() + p;
+            char *e = nullptr;
+            v.num = std::strtod(b, &e);
+            if (e == b) fail("bad value");
+            v.kind = JVal::NUM;
+            p += (size_t)(e - b);
+        }
+        return v;
+    }
+    std::string str() {
+        ++p;  // opening quote
+        std::string out;
+        while (p < s.size() && s[p] != '"') {
+            if (s[p] == '\\' && p + 1 < s.size()) {
+                ++p;
+                char e = s[p];
+                if (e == 'n') out += '\n';
+                else if (e == 't') out += '\t';
+                else if (e == 'u') { out += '?'; p += 4; }
+                else out += e;
+                ++p;
+            } else {
+                out += s[p++];
+            }
+        }
+        if (p >= s.size()) fail("unterminated string");
+        ++p;
+        return out;
+    }
+};
+
+double num(const JVal *v, double dflt) {
+    if (!v) return dflt;
+    if (v->kind == JVal::NUM) return v->num;
+    if (v->kind == JVal::BOOL) return v->b ? 1.0 : 0.0;
+    throw Error(ERR_ARG, "model JSON: expected a number");
+}
+std::vector<double> vec(const JVal *v) {
+    std::vector<double> out;
+    if (!v) return out;
+    if (v->kind != JVal::ARR) throw Error(ERR_ARG, "model JSON: expected an array");
+    out.reserve(v->arr.size());
+    for (auto &e : v->arr) out.push_back(num(&e, 0));
+    return out;
+}
+const JVal *first(const JVal &m, std::initializer_list<const char *> ks) {
+    for (auto k : ks)
+        if (auto v = m.get(k)) return v;
+    return nullptr;
+}
+
+constexpr int WT_DEFAULT = 0, WT_NONE = -1, WT_FULL = 2, WT_EXFULL = 3, WT_FUSE = 4, WT_RFUSE = 5, WT_EXFUSE = 6;
+
+int size_n(int wtype, int ntypes, int nmax, int fuse) {  // WTypeBasis.java:331-354
+    switch (wtype) {
+        case WT_EXFULL: return ntypes > 1 ? (ntypes + 1) * (nmax + 1) : nmax + 1;
+        case WT_FULL: case WT_RFUSE: return ntypes * (nmax + 1);
+        case WT_NONE: return nmax + 1;
+        case WT_DEFAULT: return ntypes > 1 ? 2 * nmax + 2 : nmax + 1;
+        case WT_FUSE: return fuse * (nmax + 1);
+        case WT_EXFUSE: return (fuse + 1) * (nmax + 1);
+    }
+    throw Error(ERR_ARG, "Input wtype MUST be in {-1, 0, 2, 3, 4, 5, 6}, input: " + std::to_string(wtype));
+}
+
+}  // namespace
+
+ModelHost parse_model_json(const std::string &text) {
+    JParser jp(text);
+    JVal root = jp.value();
+    if (root.kind != JVal::OBJ) throw Error(ERR_ARG, "model JSON: top level must be an object");
+    if (auto v = root.get("version"))
+        if ((int)num(v, 0) > 6) throw Error(ERR_ARG, "Unsupported version: " + std::to_string((int)num(v, 0)));
+    const JVal *models = root.get("models");
+    if (!models || models->kind != JVal::ARR) throw Error(ERR_ARG, "No models in ModelInfo");
+    const int ntypes = (int)models->arr.size();
+    JNN_CHECK(ntypes >= 1, ERR_ARG, "No models in ModelInfo");
+    double sig_e = 1.0, mu_e = 0.0;
+    bool have_s = false, have_m = false;
+    for (auto &m : models->arr) {  // NNAP.java:272-278
+        if (!have_s && m.get("norm_sigma_eng")) { sig_e = num(m.get("norm_sigma_eng"), 1.0); have_s = true; }
+        if (!have_m && m.get("norm_mu_eng")) { mu_e = num(m.get("norm_mu_eng"), 0.0); have_m = true; }
+    }
+    ModelHost out;
+    for (auto &m : models->arr) {
+        SpeciesHost sp;
+        const JVal *sym = m.get("symbol");
+        JNN_CHECK(sym && sym->kind == JVal::STR, ERR_ARG, "No symbol in model");
+        sp.symbol = sym->str;
+        JNN_CHECK(!m.get("torch"), ERR_ARG, "torch model is invalid now.");
+        JVal dflt_basis;
+        dflt_basis.kind = JVal::OBJ;
+        const JVal *b = m.get("basis");
+        if (!b) b = &dflt_basis;
+        if (auto t = b->get("type")) {
+            JNN_CHECK(t->kind == JVal::STR && (t->str == "spherical_chebyshev" || t->str == "sph_cheby"), ERR_UNSUPPORTED,
+                      "Unsupported basis type: " + (t->kind == JVal::STR ? t->str : std::string("?")));
+        }
+        sp.nmax = (int)num(b->get("nmax"), 5);
+        sp.lmax = (int)num(b->get("lmax"), 6);
+        JNN_CHECK(sp.lmax >= 0 && sp.lmax <= 12, ERR_ARG, "Input lmax MUST be in [0, 12], input: " + std::to_string(sp.lmax));
+        JNN_CHECK(sp.nmax >= 0 && sp.nmax < 32, ERR_ARG, "Input nmax out of range");
+        JNN_CHECK((int)num(b->get("l3max"), 0) == 0 && (int)num(b->get("l4max"), 0) == 0, ERR_UNSUPPORTED,
+                  "l3max/l4max > 0 are not supported by this build");
+        sp.rcut = num(b->get("rcut"), 6.0);
+        JNN_CHECK(sp.rcut > 0, ERR_ARG, "rcut must be positive");
+        int wtype = WT_DEFAULT;
+        if (auto w = b->get("wtype")) {
+            if (w->kind == JVal::NUM) wtype = (int)w->num;
+            else {
+                static const std::map<std::string, int> names = {{"default", WT_DEFAULT}, {"none", WT_NONE},
+                                                                 {"full", WT_FULL},       {"exfull", WT_EXFULL},
+                                                                 {"fuse", WT_FUSE},       {"rfuse", WT_RFUSE},
+                                                                 {"exfuse", WT_EXFUSE}};
+                auto it = names.find(w->str);
+                JNN_CHECK(it != names.end(), ERR_ARG, "Input wtype MUST be in {-1, 0, 2, 3, 4, 5, 6}, input: " + w->str);
+                wtype = it->second;
+            }
+        }
+        sp.wtype = wtype;
+        std::vector<double> fuse_w = vec(b->get("fuse_weight"));
+        std::vector<double> rfuse_w = vec(b->get("rfuse_weight"));
+        double rfuse_scale = num(b->get("rfuse_scale"), 1.0);
+        bool ws = num(first(*b, {"weight_standardization", "ws"}), 0.0) != 0.0;
+        if ((wtype == WT_FUSE || wtype == WT_EXFUSE) && fuse_w.empty())
+            throw Error(ERR_ARG, "Input fuse_weight MUST NOT be null when wtype=='fuse' or 'exfuse'");
+        if (wtype == WT_RFUSE && rfuse_w.empty())
+            throw Error(ERR_ARG, "Input rfuse_weight MUST NOT be null when wtype=='rfuse'");
+        int typed = wtype;
+        if (ntypes == 1 && (wtype == WT_EXFULL || wtype == WT_FULL || wtype == WT_NONE || wtype == WT_DEFAULT))
+            typed = WT_NONE;
+        sp.typed_wtype = typed;
+        const int fuse = (wtype == WT_FUSE || wtype == WT_EXFUSE) ? (int)fuse_w.size() / ntypes : 0;
+        const int sn = size_n(wtype, ntypes, sp.nmax, fuse);
+        const int K = sp.K();
+        sp.P = rfuse_w.empty() ? sn : (int)rfuse_w.size() / sn;
+        JNN_CHECK(sp.P >= 1 && sp.P <= 128, ERR_ARG, "basis size_np out of range");
+        sp.W.assign((size_t)ntypes * sp.P * K, 0.0);
+        auto Wat = [&](int t, int p, int n) -> double & { return sp.W[((size_t)(t - 1) * sp.P + p) * K + n]; };
+        if (!rfuse_w.empty()) {
+            JNN_CHECK(rfuse_w.size() == (size_t)sp.P * sn, ERR_ARG, "Size of rfuse weight mismatch");
+            std::vector<double> rw = rfuse_w;
+            if (ws) {
+                for (int p = 0; p < sp.P; ++p) {
+                    double norm = 1e-8;
+                    for (int t = 1; t <= ntypes; ++t)
+                        for (int n = 0; n < K; ++n) norm += std::fabs(rfuse_w[((size_t)(t - 1) * sp.P + p) * K + n]);
+                    norm /= sn;
+                    for (int t = 1; t <= ntypes; ++t)
+                        for (int n = 0; n < K; ++n) rw[((size_t)(t - 1) * sp.P + p) * K + n] /= norm;
+                }
+            }
+            for (size_t q = 0; q < rw.size(); ++q) sp.W[q] = rfuse_scale * rw[q];
+        } else {
+            for (int t = 1; t <= ntypes; ++t)
+                for (int n = 0; n < K; ++n) {
+                    switch (typed) {
+                        case WT_NONE: Wat(t, n, n) = 1.0; break;
+                        case WT_DEFAULT: {
+                            double wt = (t & 1) ? (double)t : -(double)t;
+                            Wat(t, n, n) = 1.0;
+                            Wat(t, K + n, n) = wt;
+                            break;
+                        }
+                        case WT_FULL: Wat(t, (t - 1) * K + n, n) = 1.0; break;
+                        case WT_EXFULL: Wat(t, n, n) = 1.0; Wat(t, t * K + n, n) = 1.0; break;
+                        case WT_FUSE:
+                            for (int k = 0; k < fuse; ++k) Wat(t, k * K + n, n) = fuse_w[(size_t)(t - 1) * fuse + k];
+                            break;
+                        case WT_EXFUSE:
+                            Wat(t, n, n) = 1.0;
+                            for (int k = 0; k < fuse; ++k) Wat(t, (k + 1) * K + n, n) = fuse_w[(size_t)(t - 1) * fuse + k];
+                            break;
+                        default: throw Error(ERR_ARG, "unsupported wtype");
+                    }
+                }
+        }
+        const int S = sp.S();
+        JVal dflt_nn;
+        dflt_nn.kind = JVal::OBJ;
+        const JVal *nn = m.get("nn");
+        if (!nn) nn = &dflt_nn;
+        if (auto t = nn->get("type"))
+            JNN_CHECK(t->kind == JVal::STR && t->str == "feed_forward", ERR_UNSUPPORTED, "Unsupported nn type");
+        if (auto d = nn->get("input_dim"))
+            JNN_CHECK((int)num(d, 0) == S, ERR_ARG,
+                      "Input dimension mismatch: " + std::to_string((int)num(d, 0)) + " vs " + std::to_string(S));
+        if (auto h = first(*nn, {"hidden_dims", "nnarch"})) {
+            for (double v : vec(h)) sp.hidden.push_back((int)v);
+        } else {
+            sp.hidden = {32, 32};
+        }
+        JNN_CHECK(!sp.hidden.empty(), ERR_ARG, "At least one hidden layer is required");
+        JNN_CHECK((int)sp.hidden.size() <= kMaxLayers, ERR_UNSUPPORTED, "too many hidden layers");
+        std::vector<double> hw, hb;
+        const JVal *hws = nn->get("hidden_weights");
+        const JVal *hbs = nn->get("hidden_biases");
+        if (hws) JNN_CHECK(hws->kind == JVal::ARR && hws->arr.size() == sp.hidden.size(), ERR_ARG, "The number of hidden weights mismatch");
+        if (hbs) JNN_CHECK(hbs->kind == JVal::ARR && hbs->arr.size() == sp.hidden.size(), ERR_ARG, "The number of hidden biases mismatch");
+        int cols = S;
+        for (size_t li = 0; li < sp.hidden.size(); ++li) {
+            int h = sp.hidden[li];
+            if (hws) {
+                const JVal &rows = hws->arr[li];
+                JNN_CHECK(rows.kind == JVal::ARR && (int)rows.arr.size() == h, ERR_ARG,
+                          "nrows of hidden weight '" + std::to_string(li) + "' mismatch");
+                for (auto &r : rows.arr) {
+                    std::vector<double> rr = vec(&r);
+                    JNN_CHECK((int)rr.size() == cols, ERR_ARG, "ncols of hidden weight '" + std::to_string(li) + "' mismatch");
+                    hw.insert(hw.end(), rr.begin(), rr.end());
+                }
+            } else {
+                hw.insert(hw.end(), (size_t)h * cols, 0.0);
+            }
+            if (hbs) {
+                std::vector<double> bb = vec(&hbs->arr[li]);
+                JNN_CHECK((int)bb.size() == h, ERR_ARG, "Size of hidden bias '" + std::to_string(li) + "' mismatch");
+                hb.insert(hb.end(), bb.begin(), bb.end());
+            } else {
+                hb.insert(hb.end(), (size_t)h, 0.0);
+            }
+            cols = h;
+        }
+        std::vector<double> ow = vec(nn->get("output_weight"));
+        if (ow.empty()) ow.assign((size_t)cols, 0.0);
+        JNN_CHECK((int)ow.size() == cols, ERR_ARG, "Size of output weight mismatch");
+        double ob = num(nn->get("output_bias"), 0.0);
+        sp.nn = hw;
+        sp.nn.insert(sp.nn.end(), ow.begin(), ow.end());
+        sp.nn.insert(sp.nn.end(), hb.begin(), hb.end());
+        sp.nn.push_back(ob);
+        double ref = num(m.get("ref_eng"), 0.0);
+        std::vector<double> mu = vec(m.get("norm_mu"));
+        std::vector<double> sg = vec(first(m, {"norm_sigma", "norm_vec"}));
+        if (mu.empty()) mu.assign((size_t)S, 0.0);
+        if (sg.empty()) sg.assign((size_t)S, 1.0);
+        JNN_CHECK((int)mu.size() == S && (int)sg.size() == S, ERR_ARG, "norm_mu/norm_sigma size mismatch");
+        sp.norm = {mu_e + ref, sig_e};
+        sp.norm.insert(sp.norm.end(), mu.begin(), mu.end());
+        sp.norm.insert(sp.norm.end(), sg.begin(), sg.end());
+        out.species.push_back(std::move(sp));
+    }
+    finalize_species(out);
+    return out;
+}
+
+void finalize_species(ModelHost &m) {
+    const int nt = m.ntypes();
+    for (auto &sp : m.species) {
+        const int K = sp.K(), P = sp.P;
+        JNN_CHECK((int)sp.W.size() == nt * P * K, ERR_ARG, "fp_param size mismatch");
+        size_t need = 0;
+        int cols = sp.S();
+        for (int h : sp.hidden) { need += (size_t)h * cols + h; cols = h; }
+        need += (size_t)cols + 1;
+        JNN_CHECK(sp.nn.size() == need, ERR_ARG, "nn_param size mismatch");
+        JNN_CHECK((int)sp.norm.size() == 2 + 2 * sp.S(), ERR_ARG, "norm_param size mismatch");
+        // recognise the structured mixings so the kernels can skip the dense product
+        bool none = (P == K), def = (P == 2 * K);
+        for (int t = 1; t <= nt; ++t)
+            for (int p = 0; p < P; ++p)
+                for (int n = 0; n < K; ++n) {
+                    double w = sp.W[((size_t)(t - 1) * P + p) * K + n];
+                    if (none && w != (p == n ? 1.0 : 0.0)) none = false;
+                    double wt = (t & 1) ? (double)t : -(double)t;
+                    double want = (p == n) ? 1.0 : (p == K + n ? wt : 0.0);
+                    if (def && w != want) def = false;
+                }
+        sp.mix = none ? MIX_NONE : (def ? MIX_DEFAULT : MIX_DENSE);
+    }
+}
+
+}  // namespace jnn

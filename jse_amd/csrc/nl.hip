@@ -1,0 +1,252 @@
+// Periodic neighbour-list build on the GPU.
+//
+// Semantics follow the reference's cell list (paths relative to the reference
+// repository root): positions wrapped as IBox.wrapPBC (src/main/java/jse/atom/
+// IBox.java:145-151); cell grid of NeighborListGetter.getProperLinkedCell
+// (NeighborListGetter.java:952-1001); a pair (j, image) is kept when the FP64
+// displacement dx = (x_j + shift) - x_i satisfies dx*dx+dy*dy+dz*dz < rc*rc,
+// evaluated left to right without fused multiply-add
+// (NeighborListGetter.java:1041-1048); self is excluded only at zero shift
+// (:129, :153-171).  Each atom's row is sorted by key = j*32 + shift_code
+// (shift_code = (sx+1) + 3(sy+1) + 9(sz+1)), so the set can be compared
+// bit-exactly with the reference's (which visits cells in a fixed order).
+//
+// Layout: positions wrapped into SoA-free AoS FP64 [N][3]; atoms counting-
+// sorted by cell so a cell's members are contiguous; one wavefront per atom
+// scans the 27 neighbouring cells as one flattened candidate range, compacts
+// hits with a wave ballot into LDS, sorts them with a bitonic network and
+// writes one row.
+#include "kernels.h"
+
+namespace jnn {
+
+constexpr int kNlBlock = 64;
+constexpr int kNlLds = 512;  // candidate hits staged per atom (rows longer than this overflow)
+
+__global__ void k_wrap_bin(int n, const double *__restrict__ xyz_in, CellGrid g, double *__restrict__ xw,
+                           int *__restrict__ cell, int *__restrict__ cell_cnt, int *__restrict__ rank) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int c[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        double s = xyz_in[3 * i + d] / g.box[d];
+        if (s < 0.0 || s >= 1.0) s -= floor(s);
+        const double x = s * g.box[d];
+        xw[3 * i + d] = x;
+        int ci = (int)floor(x / g.cb[d]);
+        ci = ci < 0 ? 0 : (ci >= g.size[d] ? g.size[d] - 1 : ci);
+        c[d] = ci;
+    }
+    const int cc = c[0] + g.size[0] * (c[1] + g.size[1] * c[2]);
+    cell[i] = cc;
+    rank[i] = atomicAdd(&cell_cnt[cc], 1);
+}
+
+// single-block exclusive scan of the cell counts
+__global__ __launch_bounds__(1024) void k_cell_scan(int ncell, const int *__restrict__ cnt, int *__restrict__ start) {
+    __shared__ int part[1024];
+    const int t = threadIdx.x;
+    const int per = (ncell + 1023) / 1024;
+    const int b = t * per, e = min(b + per, ncell);
+    int s = 0;
+    for (int c = b; c < e; ++c) s += cnt[c];
+    part[t] = s;
+    __syncthreads();
+    for (int off = 1; off < 1024; off <<= 1) {
+        int v = (t >= off) ? part[t - off] : 0;
+        __syncthreads();
+        part[t] += v;
+        __syncthreads();
+    }
+    int run = part[t] - s;
+    for (int c = b; c < e; ++c) {
+        start[c] = run;
+        run += cnt[c];
+    }
+    if (t == 1023) start[ncell] = part[1023];
+}
+
+__global__ void k_cell_scatter(int n, const int *__restrict__ cell, const int *__restrict__ rank,
+                               const int *__restrict__ start, const double *__restrict__ xw, int *__restrict__ sidx,
+                               double *__restrict__ sxw) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const int slot = start[cell[i]] + rank[i];
+    sidx[slot] = i;
+    sxw[3 * slot] = xw[3 * i];
+    sxw[3 * slot + 1] = xw[3 * i + 1];
+    sxw[3 * slot + 2] = xw[3 * i + 2];
+}
+
+__device__ __forceinline__ double dist2_no_fma(double dx, double dy, double dz) {
+#pragma clang fp contract(off)
+    const double a = dx * dx;
+    const double b = dy * dy;
+    const double c = dz * dz;
+    return (a + b) + c;
+}
+
+__global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, CellGrid g, const double *__restrict__ xw,
+                                                       const int *__restrict__ cell, const int *__restrict__ cstart,
+                                                       const int *__restrict__ sidx, const double *__restrict__ sxw,
+                                                       NeighborList nl, int *__restrict__ overflow) {
+    __shared__ int s_start[27], s_cnt[27], s_code[27], s_pref[28];
+    __shared__ int s_key[kNlLds];
+    __shared__ float s_d[3][kNlLds];
+    const int lane = threadIdx.x;
+    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+        const double xi = xw[3 * i], yi = xw[3 * i + 1], zi = xw[3 * i + 2];
+        const int c = cell[i];
+        const int cx = c % g.size[0], cy = (c / g.size[0]) % g.size[1], cz = c / (g.size[0] * g.size[1]);
+        if (lane < 27) {
+            const int w = lane;
+            int ix = cx + (w % 3) - 1, iy = cy + (w / 3) % 3 - 1, iz = cz + w / 9 - 1;
+            int sx = 0, sy = 0, sz = 0;
+            if (ix >= g.size[0]) { ix -= g.size[0]; sx = 1; } else if (ix < 0) { ix += g.size[0]; sx = -1; }
+            if (iy >= g.size[1]) { iy -= g.size[1]; sy = 1; } else if (iy < 0) { iy += g.size[1]; sy = -1; }
+            if (iz >= g.size[2]) { iz -= g.size[2]; sz = 1; } else if (iz < 0) { iz += g.size[2]; sz = -1; }
+            const int cc = ix + g.size[0] * (iy + g.size[1] * iz);
+            s_start[w] = cstart[cc];
+            s_cnt[w] = cstart[cc + 1] - cstart[cc];
+            s_code[w] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+        }
+        __syncthreads();
+        if (lane == 0) {
+            int run = 0;
+            for (int w = 0; w < 27; ++w) {
+                s_pref[w] = run;
+                run += s_cnt[w];
+            }
+            s_pref[27] = run;
+        }
+        __syncthreads();
+        const int total = s_pref[27];
+        int hits = 0;
+        for (int q0 = 0; q0 < total; q0 += kNlBlock) {
+            const int q = q0 + lane;
+            bool hit = false;
+            int key = 0;
+            float fdx = 0.f, fdy = 0.f, fdz = 0.f;
+            if (q < total) {
+                int w = 0;
+                while (s_pref[w + 1] <= q) ++w;
+                const int slot = s_start[w] + (q - s_pref[w]);
+                const int j = sidx[slot];
+                const int code = s_code[w];
+                const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
+                double xj = sxw[3 * slot], yj = sxw[3 * slot + 1], zj = sxw[3 * slot + 2];
+                if (sx) xj = xj + (sx > 0 ? g.box[0] : -g.box[0]);
+                if (sy) yj = yj + (sy > 0 ? g.box[1] : -g.box[1]);
+                if (sz) zj = zj + (sz > 0 ? g.box[2] : -g.box[2]);
+                const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+                const double d2 = dist2_no_fma(dx, dy, dz);
+                hit = (d2 < g.rc2) && !(j == i && code == 13);
+                key = j * 32 + code;
+                fdx = (float)dx;
+                fdy = (float)dy;
+                fdz = (float)dz;
+            }
+            const unsigned long long mask = __ballot(hit);
+            const unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+            const int pos = hits + __popcll(below);
+            if (hit && pos < kNlLds) {
+                s_key[pos] = key;
+                s_d[0][pos] = fdx;
+                s_d[1][pos] = fdy;
+                s_d[2][pos] = fdz;
+            }
+            hits += __popcll(mask);
+        }
+        const int nst = hits < kNlLds ? hits : kNlLds;
+        int p2 = 1;
+        while (p2 < nst) p2 <<= 1;
+        for (int t = nst + lane; t < p2; t += kNlBlock) s_key[t] = 0x7fffffff;
+        __syncthreads();
+        // bitonic sort of (key, d) by key
+        for (int k = 2; k <= p2; k <<= 1) {
+            for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                for (int t = lane; t < p2; t += kNlBlock) {
+                    const int u = t ^ jj;
+                    if (u > t) {
+                        const bool up = (t & k) == 0;
+                        const int ka = s_key[t], kb = s_key[u];
+                        if ((ka > kb) == up) {
+                            s_key[t] = kb;
+                            s_key[u] = ka;
+#pragma unroll
+                            for (int d = 0; d < 3; ++d) {
+                                const float tmp = s_d[d][t];
+                                s_d[d][t] = s_d[d][u];
+                                s_d[d][u] = tmp;
+                            }
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        }
+        const int nw = nst < nl.cap ? nst : nl.cap;
+        const size_t row = (size_t)i * nl.cap;
+        for (int t = lane; t < nw; t += kNlBlock) {
+            nl.key[row + t] = s_key[t];
+            nl.dx[row + t] = s_d[0][t];
+            nl.dy[row + t] = s_d[1][t];
+            nl.dz[row + t] = s_d[2][t];
+        }
+        if (lane == 0) {
+            nl.cnt[i] = hits;
+            if (hits > nl.cap) atomicMax(overflow, hits);
+        }
+        __syncthreads();
+    }
+}
+
+// rev[i][k] = slot of (i, -shift) in j's row (binary search on the sorted keys)
+__global__ void k_nl_reverse(int n, NeighborList nl) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = (int)(t / nl.cap), k = (int)(t % nl.cap);
+    if (i >= n) return;
+    const int ci = min(nl.cnt[i], nl.cap);
+    if (k >= ci) return;
+    const int key = nl.key[(size_t)i * nl.cap + k];
+    const int j = key >> 5, code = key & 31;
+    const int want = i * 32 + (26 - code);
+    const int cj = min(nl.cnt[j], nl.cap);
+    const int *rj = nl.key + (size_t)j * nl.cap;
+    int lo = 0, hi = cj - 1, found = -1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = rj[mid];
+        if (v == want) { found = mid; break; }
+        if (v < want) lo = mid + 1; else hi = mid - 1;
+    }
+    nl.rev[(size_t)i * nl.cap + k] = found;
+}
+
+void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const CellGrid &g, double *xw, int *cell,
+                     int *cell_cnt, int *rank) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_wrap_bin, dim3((n + 255) / 256), dim3(256), 0, s, n, xyz_in, g, xw, cell, cell_cnt, rank);
+}
+void launch_cell_scan(hipStream_t s, int ncell, const int *cell_cnt, int *cell_start) {
+    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s, ncell, cell_cnt, cell_start);
+}
+void launch_cell_scatter(hipStream_t s, int n, const int *cell, const int *rank, const int *cell_start,
+                         const double *xw, int *sidx, double *sxw) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, cell, rank, cell_start, xw, sidx, sxw);
+}
+void launch_nl_build(hipStream_t s, int n, const CellGrid &g, const double *xw, const int *cell, const int *cell_start,
+                     const int *sidx, const double *sxw, NeighborList nl, int *overflow) {
+    if (n <= 0) return;
+    const int grid = n < 65536 * 4 ? n : 65536 * 4;
+    hipLaunchKernelGGL(k_nl_build, dim3(grid), dim3(kNlBlock), 0, s, n, g, xw, cell, cell_start, sidx, sxw, nl, overflow);
+}
+void launch_nl_reverse(hipStream_t s, int n, NeighborList nl) {
+    if (n <= 0) return;
+    const long long tot = (long long)n * nl.cap;
+    hipLaunchKernelGGL(k_nl_reverse, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nl);
+}
+
+}  // namespace jnn

@@ -1,0 +1,334 @@
+"""NNAP model files: load the reference's JSON/YAML format into the packed
+per-species parameter blocks, and write seeded random-init models.
+
+The packing follows the reference exactly (paths relative to the reference
+repository root):
+
+* ``fp_param[t]``   internal radial mixing ``W[ntypes][P][K]`` built from the
+  basis ``wtype`` as in ``src/nnap/java/jsex/nnap/basis/WTypeBasis.java:202-304``
+  (sizes ``:331-354``; a one-species model collapses full/exfull/none/default
+  to ``none``, ``:89-101``).
+* ``nn_param[t]``   ``[hidden W (per layer, [out][in]) | output W | hidden b |
+  output b]`` (``nn/FeedForward.java:131-221,310-316``).
+* ``norm_param[t]`` ``[mu_E + ref_E, sigma_E, mu[S], sigma[S]]`` with
+  ``norm_mu_eng``/``norm_sigma_eng`` taken from the first species that has
+  them (``NNAP.java:272-306``).
+
+Only ``spherical_chebyshev`` bases with ``l3max = l4max = 0`` are accepted,
+and every error is raised as ``ValueError`` with the reference's wording
+where it has one (the reference raises ``IllegalArgumentException``).
+"""
+from __future__ import annotations
+
+import json
+import math
+from dataclasses import dataclass, field
+from typing import Any, Dict, List, Optional, Sequence
+
+import numpy as np
+
+VERSION = 6  # NNAP.java:77
+
+WTYPES = {"default": 0, "none": -1, "full": 2, "exfull": 3, "fuse": 4, "rfuse": 5, "exfuse": 6}
+WTYPE_DEFAULT, WTYPE_NONE, WTYPE_FULL, WTYPE_EXFULL, WTYPE_FUSE, WTYPE_RFUSE, WTYPE_EXFUSE = 0, -1, 2, 3, 4, 5, 6
+
+DEFAULT_NMAX, DEFAULT_LMAX, DEFAULT_RCUT = 5, 6, 6.0  # SphericalChebyshev.java:18-22
+DEFAULT_HIDDEN = (32, 32)  # FeedForward.java:28
+
+
+@dataclass
+class Species:
+    symbol: str
+    nmax: int
+    lmax: int
+    rcut: float
+    wtype: int          # wtype as written in the file
+    typed_wtype: int    # after the one-species collapse
+    P: int
+    W: np.ndarray       # [ntypes][P][K]
+    hidden: List[int]
+    nn_param: np.ndarray
+    norm_param: np.ndarray
+
+    @property
+    def K(self) -> int:
+        return self.nmax + 1
+
+    @property
+    def M(self) -> int:
+        return (self.lmax + 1) ** 2
+
+    @property
+    def S(self) -> int:
+        return self.P * (self.lmax + 1)
+
+
+@dataclass
+class Model:
+    species: List[Species]
+    units: Optional[str] = None
+    source: Dict[str, Any] = field(default_factory=dict)
+
+    @property
+    def ntypes(self) -> int:
+        return len(self.species)
+
+    @property
+    def symbols(self) -> List[str]:
+        return [s.symbol for s in self.species]
+
+    @property
+    def rcut_max(self) -> float:
+        return max(s.rcut for s in self.species)
+
+
+def _get(m: dict, *keys, default=None):
+    for k in keys:
+        if k in m and m[k] is not None:
+            return m[k]
+    return default
+
+
+def _size_n(wtype: int, ntypes: int, nmax: int, fuse_size: int) -> int:
+    """WTypeBasis.getSizeN_ (WTypeBasis.java:331-354)."""
+    if wtype == WTYPE_EXFULL:
+        return (ntypes + 1) * (nmax + 1) if ntypes > 1 else nmax + 1
+    if wtype in (WTYPE_FULL, WTYPE_RFUSE):
+        return ntypes * (nmax + 1)
+    if wtype == WTYPE_NONE:
+        return nmax + 1
+    if wtype == WTYPE_DEFAULT:
+        return 2 * nmax + 2 if ntypes > 1 else nmax + 1
+    if wtype == WTYPE_FUSE:
+        return fuse_size * (nmax + 1)
+    if wtype == WTYPE_EXFUSE:
+        return (fuse_size + 1) * (nmax + 1)
+    raise ValueError(f"Input wtype MUST be in {{-1, 0, 2, 3, 4, 5, 6}}, input: {wtype}")
+
+
+def _mixing(wtype: int, ntypes: int, nmax: int, fuse_w, rfuse_w, rfuse_scale: float, ws: bool):
+    """Internal W[ntypes][P][K] of WTypeBasis.updateParameters_ (WTypeBasis.java:202-304)."""
+    K = nmax + 1
+    typed = wtype
+    if ntypes == 1 and wtype in (WTYPE_EXFULL, WTYPE_FULL, WTYPE_NONE, WTYPE_DEFAULT):
+        typed = WTYPE_NONE
+    fuse_size = 0 if wtype not in (WTYPE_FUSE, WTYPE_EXFUSE) else len(fuse_w) // ntypes
+    size_n = _size_n(wtype, ntypes, nmax, fuse_size)
+    if rfuse_w is not None:
+        P = len(rfuse_w) // size_n
+    else:
+        P = size_n
+    W = np.zeros((ntypes, P, K))
+    if rfuse_w is not None:
+        rw = np.asarray(rfuse_w, dtype=float)
+        # the rfuse weight is the internal weight itself (fillD of mRFuseWeight)
+        rw = rw.reshape(ntypes, P, K)
+        if ws:
+            out = np.empty_like(rw)
+            for p in range(P):
+                norm = 1e-8 + np.abs(rw[:, p, :]).sum()   # WS_EPS
+                norm /= size_n
+                out[:, p, :] = rw[:, p, :] / norm
+            rw = out
+        W[:] = rfuse_scale * rw
+        return typed, P, W
+    for t in range(1, ntypes + 1):
+        for n in range(K):
+            if typed == WTYPE_NONE:
+                W[t - 1, n, n] = 1.0
+            elif typed == WTYPE_DEFAULT:
+                wt = float(t) if (t & 1) == 1 else float(-t)
+                W[t - 1, n, n] = 1.0
+                W[t - 1, K + n, n] = wt
+            elif typed == WTYPE_FULL:
+                W[t - 1, (t - 1) * K + n, n] = 1.0
+            elif typed == WTYPE_EXFULL:
+                W[t - 1, n, n] = 1.0
+                W[t - 1, t * K + n, n] = 1.0
+            elif typed == WTYPE_FUSE:
+                for k in range(fuse_size):
+                    W[t - 1, k * K + n, n] = fuse_w[(t - 1) * fuse_size + k]
+            elif typed == WTYPE_EXFUSE:
+                W[t - 1, n, n] = 1.0
+                for k in range(fuse_size):
+                    W[t - 1, (k + 1) * K + n, n] = fuse_w[(t - 1) * fuse_size + k]
+            else:
+                raise ValueError(f"Unsupported wtype {typed}")
+    return typed, P, W
+
+
+def load_model(src) -> Model:
+    """Parse a model (path to .json/.yaml/.yml, JSON text, or a dict) as
+    ``new NNAP(path)`` does (NNAP.java:129-307, 402-414)."""
+    if isinstance(src, dict):
+        info = src
+    elif isinstance(src, str) and src.lstrip().startswith("{"):
+        info = json.loads(src)
+    else:
+        path = str(src)
+        with open(path, "r") as f:
+            if path.endswith((".yaml", ".yml")):
+                import yaml
+                info = yaml.safe_load(f)
+            else:
+                info = json.load(f)
+    ver = info.get("version")
+    if ver is not None and int(ver) > VERSION:
+        raise ValueError(f"Unsupported version: {int(ver)}")
+    models = info.get("models")
+    if models is None:
+        raise ValueError("No models in ModelInfo")
+    ntypes = len(models)
+    norm_sigma_eng = None
+    norm_mu_eng = None
+    for m in models:
+        if norm_sigma_eng is None:
+            norm_sigma_eng = m.get("norm_sigma_eng")
+        if norm_mu_eng is None:
+            norm_mu_eng = m.get("norm_mu_eng")
+    sig_e = 1.0 if norm_sigma_eng is None else float(norm_sigma_eng)
+    mu_e = 0.0 if norm_mu_eng is None else float(norm_mu_eng)
+    out: List[Species] = []
+    for m in models:
+        sym = m.get("symbol")
+        if sym is None:
+            raise ValueError("No symbol in model")
+        if m.get("torch") is not None:
+            raise ValueError("torch model is invalid now.")
+        b = m.get("basis") or {"type": "spherical_chebyshev"}
+        btype = str(b.get("type", "spherical_chebyshev"))
+        if btype not in ("spherical_chebyshev", "sph_cheby"):
+            raise ValueError(f"Unsupported basis type: {btype}")
+        nmax = int(_get(b, "nmax", default=DEFAULT_NMAX))
+        lmax = int(_get(b, "lmax", default=DEFAULT_LMAX))
+        l3 = int(_get(b, "l3max", default=0))
+        l4 = int(_get(b, "l4max", default=0))
+        if lmax < 0 or lmax > 12:
+            raise ValueError(f"Input lmax MUST be in [0, 12], input: {lmax}")
+        if l3 != 0 or l4 != 0:
+            raise ValueError("l3max/l4max > 0 are not supported by this build")
+        rcut = float(_get(b, "rcut", default=DEFAULT_RCUT))
+        wt = _get(b, "wtype", default="default")
+        wtype = int(wt) if isinstance(wt, (int, float)) else WTYPES.get(str(wt))
+        if wtype is None:
+            raise ValueError(f"Input wtype MUST be in {{-1, 0, 2, 3, 4, 5, 6}}, input: {wt}")
+        fuse_w = b.get("fuse_weight")
+        rfuse_w = b.get("rfuse_weight")
+        rfuse_scale = float(b.get("rfuse_scale", 1.0) or 1.0)
+        ws = bool(_get(b, "weight_standardization", "ws", default=False))
+        if wtype in (WTYPE_FUSE, WTYPE_EXFUSE) and fuse_w is None:
+            raise ValueError("Input fuse_weight MUST NOT be null when wtype=='fuse' or 'exfuse'")
+        if wtype == WTYPE_RFUSE and rfuse_w is None:
+            raise ValueError("Input rfuse_weight MUST NOT be null when wtype=='rfuse'")
+        typed, P, W = _mixing(wtype, ntypes, nmax, fuse_w, rfuse_w, rfuse_scale, ws)
+        S = P * (lmax + 1)
+        nn = m.get("nn") or {}
+        if str(nn.get("type", "feed_forward")) != "feed_forward":
+            raise ValueError(f"Unsupported nn type: {nn.get('type')}")
+        in_dim = nn.get("input_dim")
+        if in_dim is not None and int(in_dim) != S:
+            raise ValueError(f"Input dimension mismatch: {int(in_dim)} vs {S}")
+        hidden = [int(h) for h in (_get(nn, "hidden_dims", "nnarch", default=list(DEFAULT_HIDDEN)))]
+        if len(hidden) == 0:
+            raise ValueError("At least one hidden layer is required")
+        hw_list, hb_list = [], []
+        hws = nn.get("hidden_weights")
+        hbs = nn.get("hidden_biases")
+        cols = S
+        for li, h in enumerate(hidden):
+            if hws is not None:
+                w = np.asarray(hws[li], dtype=float)
+                if w.shape != (h, cols):
+                    raise ValueError(f"shape of hidden weight '{li}' mismatch: {w.shape} vs {(h, cols)}")
+            else:
+                w = np.zeros((h, cols))
+            hw_list.append(w.reshape(-1))
+            if hbs is not None:
+                bb = np.asarray(hbs[li], dtype=float)
+                if bb.shape != (h,):
+                    raise ValueError(f"Size of hidden bias '{li}' mismatch: {bb.shape[0]} vs {h}")
+            else:
+                bb = np.zeros(h)
+            hb_list.append(bb)
+            cols = h
+        ow = np.asarray(nn.get("output_weight", np.zeros(hidden[-1])), dtype=float)
+        if ow.shape != (hidden[-1],):
+            raise ValueError(f"Size of output weight mismatch: {ow.shape[0]} vs {hidden[-1]}")
+        ob = float(nn.get("output_bias", 0.0) or 0.0)
+        nn_param = np.concatenate(hw_list + [ow] + hb_list + [np.array([ob])])
+        ref_eng = float(m.get("ref_eng", 0.0) or 0.0)
+        nsig = _get(m, "norm_sigma", "norm_vec")
+        nmu = m.get("norm_mu")
+        mu = np.zeros(S) if nmu is None else np.asarray(nmu, dtype=float)
+        sg = np.ones(S) if nsig is None else np.asarray(nsig, dtype=float)
+        if mu.shape != (S,) or sg.shape != (S,):
+            raise ValueError("norm_mu/norm_sigma size mismatch")
+        norm = np.concatenate([[mu_e + ref_eng, sig_e], mu, sg])
+        out.append(Species(str(sym), nmax, lmax, rcut, wtype, typed, P, W, hidden, nn_param, norm))
+    return Model(out, info.get("units"), info)
+
+
+def random_model(symbols: Sequence[str], nmax: int, lmax: int, rcut: float, hidden: Sequence[int],
+                 seed: int, wtype: str = "default") -> dict:
+    """Seeded random-init model in the reference file format, drawn from the
+    distributions of FeedForward.initParameters (FeedForward.java:105-128):
+    hidden W ~ U(+-sqrt(6/fan_in)), biases ~ U(+-sqrt(1/fan_in)),
+    output W ~ U(+-sqrt(3/fan_in)).  Normalisation is identity until
+    ``set_normalization`` is called."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ntypes = len(symbols)
+    K = nmax + 1
+    P = _size_n(WTYPES[wtype], ntypes, nmax, 0)
+    if ntypes == 1 and WTYPES[wtype] in (WTYPE_DEFAULT, WTYPE_FULL, WTYPE_EXFULL):
+        P = _size_n(WTYPE_NONE, 1, nmax, 0)
+    S = P * (lmax + 1)
+    models = []
+    for sym in symbols:
+        cols = S
+        hws, hbs = [], []
+        for h in hidden:
+            bw = math.sqrt(6.0 / cols)
+            hws.append(rng.uniform(-bw, bw, size=(h, cols)).tolist())
+            cols = h
+        cols = S
+        for h in hidden:
+            bb = math.sqrt(1.0 / cols)
+            hbs.append(rng.uniform(-bb, bb, size=h).tolist())
+            cols = h
+        bo = math.sqrt(3.0 / cols)
+        ow = rng.uniform(-bo, bo, size=cols).tolist()
+        ob = float(rng.uniform(-math.sqrt(1.0 / cols), math.sqrt(1.0 / cols)))
+        models.append({
+            "symbol": sym,
+            "basis": {"type": "spherical_chebyshev", "nmax": nmax, "lmax": lmax, "l3max": 0, "l4max": 0,
+                      "rcut": rcut, "wtype": wtype},
+            "nn": {"type": "feed_forward", "input_dim": S, "hidden_dims": list(hidden),
+                   "hidden_weights": hws, "hidden_biases": hbs, "output_weight": ow, "output_bias": ob},
+            "ref_eng": 0.0,
+            "norm_mu": [0.0] * S,
+            "norm_sigma": [1.0] * S,
+        })
+    models[0]["norm_mu_eng"] = 0.0
+    models[0]["norm_sigma_eng"] = 1.0
+    return {"version": VERSION, "units": "metal", "models": models}
+
+
+def set_normalization(model: dict, fp_by_species: Sequence[Optional[np.ndarray]], digits: int = 6) -> dict:
+    """Fix norm_mu / norm_sigma per species from descriptor samples (rows =
+    atoms).  Values are rounded to ``digits`` significant figures so that a
+    model regenerated on another host is identical."""
+    def rnd(v):
+        return float(f"{v:.{digits}g}")
+    for m, fp in zip(model["models"], fp_by_species):
+        if fp is None or len(fp) == 0:
+            continue
+        mu = fp.mean(axis=0)
+        sg = fp.std(axis=0)
+        sg = np.where(sg > 1e-12 * np.maximum(1.0, np.abs(mu)), sg, 1.0)
+        m["norm_mu"] = [rnd(v) for v in mu]
+        m["norm_sigma"] = [rnd(v) for v in sg]
+    return model
+
+
+def model_json(model: dict) -> str:
+    return json.dumps(model)

@@ -1,0 +1,335 @@
+"""Host-side mirror of jse's NNAP potential over libjsenn_hip.
+
+``NNAP`` keeps the names, argument meaning and error behaviour of the
+reference's Java API (paths relative to the jse repository root):
+
+* ``NNAP(model_path_or_dict)``                 NNAP.java:402-414
+* ``calEnergy / calEnergies / calForces``      IPotential.java:145-185
+* ``calStress / calStresses``                  IPotential.java:197-311 (stress = -virial / volume)
+* ``calEnergyForceVirials``                    IPotential.java:536-562 -> IPairPotential.java:811-1022
+* ``symbols / ntypes / rcutMax / close``       NNAP.java:470-490
+
+An ``AtomData`` carries an orthogonal box, Cartesian FP64 positions and
+1-based types (optionally with species symbols; types are then mapped onto
+the model's species by symbol, as IHasSymbol.typeMap does, IHasSymbol.java:
+51-99).  Every evaluation runs the whole path on the GPU: wrap, cell list,
+descriptor, MLP, backward, FP64 force/virial accumulation.  Native errors are
+raised as ``NNAPError`` carrying the library's status code (the reference
+raises IllegalStateException("Exit code: n"), NNAP.java:756-787).
+
+The native library is required: importing works without it, but any
+evaluation raises ``RuntimeError`` when ``libjsenn_hip.so`` cannot be loaded.
+"""
+from __future__ import annotations
+
+import ctypes
+import json
+import os
+from dataclasses import dataclass
+from typing import Dict, List, Optional, Sequence, Union
+
+import numpy as np
+
+from . import model as _model
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libjsenn_hip.so")
+
+FP32 = 0
+BF16_MLP = 1
+ENERGY, FORCE, VIRIAL = 1, 2, 4
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+_c_flt_p = ctypes.POINTER(ctypes.c_float)
+_c_ll_p = ctypes.POINTER(ctypes.c_longlong)
+
+EXPORTS = [
+    "jse_nnap_last_error", "jse_nnap_model_load", "jse_nnap_model_parse", "jse_nnap_model_create", "jse_nnap_model_info",
+    "jse_nnap_model_fp_size", "jse_nnap_model_destroy", "jse_nnap_system_create", "jse_nnap_system_destroy",
+    "jse_nnap_system_set_box", "jse_nnap_system_set_positions", "jse_nnap_system_set_positions_device",
+    "jse_nnap_system_compute", "jse_nnap_system_get_results", "jse_nnap_system_compute_fp",
+    "jse_nnap_system_get_neighbors", "jse_nnap_system_benchmark", "jse_nnap_system_nl_stats",
+]
+
+
+class NNAPError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"Exit code: {code}: {msg}")
+        self.code = code
+
+
+_LIB = None
+
+
+def lib():
+    """Load libjsenn_hip.so (raises if it is missing: there is no fallback)."""
+    global _LIB
+    if _LIB is not None:
+        return _LIB
+    if not os.path.exists(LIB_PATH):
+        raise RuntimeError(f"{LIB_PATH} is not built; run `python -m jse_amd.build`")
+    L = ctypes.CDLL(LIB_PATH)
+    vp = ctypes.c_void_p
+    L.jse_nnap_last_error.restype = ctypes.c_char_p
+    L.jse_nnap_model_load.argtypes = [ctypes.c_char_p, ctypes.POINTER(vp)]
+    L.jse_nnap_model_parse.argtypes = [ctypes.c_char_p, _c_int_p, _c_int_p, _c_int_p]
+    L.jse_nnap_model_create.argtypes = [ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p, _c_int_p, _c_int_p,
+                                        ctypes.POINTER(_c_dbl_p), ctypes.POINTER(_c_dbl_p), ctypes.POINTER(_c_dbl_p),
+                                        ctypes.POINTER(vp)]
+    L.jse_nnap_model_info.argtypes = [vp, _c_int_p, _c_dbl_p]
+    L.jse_nnap_model_fp_size.argtypes = [vp, ctypes.c_int, _c_int_p]
+    L.jse_nnap_model_destroy.argtypes = [vp]
+    L.jse_nnap_model_destroy.restype = None
+    L.jse_nnap_system_create.argtypes = [vp, ctypes.c_int, _c_int_p, _c_dbl_p, ctypes.c_int, ctypes.POINTER(vp)]
+    L.jse_nnap_system_destroy.argtypes = [vp]
+    L.jse_nnap_system_destroy.restype = None
+    L.jse_nnap_system_set_box.argtypes = [vp, _c_dbl_p]
+    L.jse_nnap_system_set_positions.argtypes = [vp, _c_dbl_p]
+    L.jse_nnap_system_set_positions_device.argtypes = [vp, vp]
+    L.jse_nnap_system_compute.argtypes = [vp, ctypes.c_int]
+    L.jse_nnap_system_get_results.argtypes = [vp, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
+    L.jse_nnap_system_compute_fp.argtypes = [vp, _c_dbl_p, ctypes.c_int]
+    L.jse_nnap_system_get_neighbors.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_flt_p]
+    L.jse_nnap_system_benchmark.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_dbl_p,
+                                            ctypes.c_int]
+    L.jse_nnap_system_nl_stats.argtypes = [vp, _c_int_p, _c_int_p, _c_ll_p, _c_ll_p]
+    _LIB = L
+    return L
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise NNAPError(rc, lib().jse_nnap_last_error().decode(errors="replace"))
+
+
+def _dp(a):
+    return a.ctypes.data_as(_c_dbl_p) if a is not None else None
+
+
+def _ip(a):
+    return a.ctypes.data_as(_c_int_p) if a is not None else None
+
+
+@dataclass
+class AtomData:
+    """Orthogonal periodic box [0,L) per axis, FP64 positions [N][3] and 1-based types."""
+    box: np.ndarray
+    xyz: np.ndarray
+    types: np.ndarray
+    symbols: Optional[Sequence[str]] = None   # symbol of type t at symbols[t-1]
+
+    @property
+    def natoms(self) -> int:
+        return int(np.asarray(self.xyz).shape[0])
+
+    def volume(self) -> float:
+        b = np.asarray(self.box, dtype=float)
+        return float(b[0] * b[1] * b[2])
+
+
+class System:
+    """A device-resident configuration bound to one model: positions,
+    neighbour list and outputs stay on the GPU between evaluations."""
+
+    def __init__(self, nnap: "NNAP", natoms: int, types, box, precision: int = FP32):
+        self.nnap = nnap
+        self.natoms = int(natoms)
+        self.types = np.ascontiguousarray(types, dtype=np.int32)
+        if self.types.shape != (self.natoms,):
+            raise ValueError("types must have one entry per atom")
+        self.box = np.ascontiguousarray(box, dtype=np.float64)
+        h = ctypes.c_void_p()
+        _check(lib().jse_nnap_system_create(nnap._h, self.natoms, _ip(self.types), _dp(self.box), int(precision),
+                                            ctypes.byref(h)))
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().jse_nnap_system_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_box(self, box):
+        self.box = np.ascontiguousarray(box, dtype=np.float64)
+        _check(lib().jse_nnap_system_set_box(self._h, _dp(self.box)))
+
+    def set_positions(self, xyz):
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        if xyz.shape != (self.natoms, 3):
+            raise ValueError("positions must be [natoms][3]")
+        _check(lib().jse_nnap_system_set_positions(self._h, _dp(xyz)))
+
+    def compute(self, what: int = ENERGY | FORCE | VIRIAL):
+        _check(lib().jse_nnap_system_compute(self._h, int(what)))
+
+    def results(self, energies=True, forces=True, virials=True) -> Dict[str, np.ndarray]:
+        n = self.natoms
+        e = np.zeros(1)
+        E = np.zeros(n) if energies else None
+        F = np.zeros((n, 3)) if forces else None
+        v = np.zeros(9) if virials else None
+        V = np.zeros((n, 9)) if virials else None
+        _check(lib().jse_nnap_system_get_results(self._h, _dp(e), _dp(E), _dp(F), _dp(v), _dp(V)))
+        out = {"energy": float(e[0])}
+        if energies:
+            out["energies"] = E
+        if forces:
+            out["forces"] = F
+        if virials:
+            out["virial"] = v
+            out["virials"] = V
+        return out
+
+    def descriptors(self) -> np.ndarray:
+        smax = max(s.S for s in self.nnap.model.species)
+        fp = np.zeros((self.natoms, smax))
+        _check(lib().jse_nnap_system_compute_fp(self._h, _dp(fp), smax))
+        return fp
+
+    def neighbors(self, cap: Optional[int] = None):
+        n = self.natoms
+        c = ctypes.c_int()
+        mx = ctypes.c_int()
+        pairs = ctypes.c_longlong()
+        un = ctypes.c_longlong()
+        _check(lib().jse_nnap_system_nl_stats(self._h, ctypes.byref(c), ctypes.byref(mx), ctypes.byref(pairs),
+                                              ctypes.byref(un)))
+        cap = max(1, mx.value) if cap is None else cap
+        cnt = np.zeros(n, dtype=np.int32)
+        key = np.zeros((n, cap), dtype=np.int32)
+        d = np.zeros((n, cap, 3), dtype=np.float32)
+        _check(lib().jse_nnap_system_get_neighbors(self._h, cap, _ip(cnt), _ip(key),
+                                                   d.ctypes.data_as(_c_flt_p)))
+        return cnt, key, d
+
+    def nl_stats(self):
+        c, mx, pairs, un = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong()
+        _check(lib().jse_nnap_system_nl_stats(self._h, ctypes.byref(c), ctypes.byref(mx), ctypes.byref(pairs),
+                                              ctypes.byref(un)))
+        return {"cap": c.value, "max_count": mx.value, "pairs": pairs.value, "unmatched": un.value}
+
+    def benchmark(self, what: int = ENERGY | FORCE | VIRIAL, warmup: int = 3, iters: int = 10):
+        ms = ctypes.c_double()
+        ph = np.zeros(6)
+        _check(lib().jse_nnap_system_benchmark(self._h, int(what), int(warmup), int(iters), ctypes.byref(ms),
+                                               _dp(ph), 6))
+        return ms.value, ph
+
+
+class NNAP:
+    """jse's NNAP potential on one MI355X.  ``nthreads`` is accepted for API
+    parity with the reference and ignored (the work runs on the GPU)."""
+
+    def __init__(self, model_src: Union[str, dict], nthreads: int = 1, precision: int = FP32):
+        if isinstance(model_src, str) and model_src.endswith((".yaml", ".yml")):
+            import yaml
+            with open(model_src) as f:
+                model_src = yaml.safe_load(f)
+        self.model = _model.load_model(model_src)
+        self.precision = int(precision)
+        text = json.dumps(model_src) if isinstance(model_src, dict) else model_src
+        if not isinstance(text, str):
+            raise TypeError("model must be a path, JSON text or a dict")
+        h = ctypes.c_void_p()
+        _check(lib().jse_nnap_model_load(text.encode(), ctypes.byref(h)))
+        self._h = h
+        self._dead = False
+        self._systems: Dict[tuple, System] = {}
+
+    # --- reference accessors ---
+    def symbols(self) -> List[str]:
+        return self.model.symbols
+
+    def ntypes(self) -> int:
+        return self.model.ntypes
+
+    def rcutMax(self) -> float:
+        return self.model.rcut_max
+
+    def nthreads(self) -> int:
+        return 1
+
+    def isClosed(self) -> bool:
+        return self._dead
+
+    def close(self):
+        if self._dead:
+            return
+        for s in self._systems.values():
+            s.close()
+        self._systems.clear()
+        lib().jse_nnap_model_destroy(self._h)
+        self._dead = True
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    # --- evaluation ---
+    def _types_of(self, atoms: AtomData) -> np.ndarray:
+        t = np.asarray(atoms.types, dtype=np.int32)
+        if atoms.symbols is not None:
+            syms = self.model.symbols
+            mp = np.zeros(len(atoms.symbols) + 1, dtype=np.int32)
+            for k, s in enumerate(atoms.symbols):
+                if s not in syms:
+                    raise ValueError(f"Symbol {s} not in the model symbols {syms}")
+                mp[k + 1] = syms.index(s) + 1
+            t = mp[t]
+        if t.size and (t.min() < 1 or t.max() > self.ntypes()):
+            raise ValueError(f"Exist type ({int(t.max())}) greater than the input ntypes ({self.ntypes()})")
+        return t
+
+    def system(self, atoms: AtomData) -> System:
+        if self._dead:
+            raise RuntimeError("This NNAP is dead")
+        t = self._types_of(atoms)
+        key = (atoms.natoms, t.tobytes())
+        s = self._systems.get(key)
+        if s is None:
+            s = System(self, atoms.natoms, t, atoms.box, self.precision)
+            self._systems = {key: s}   # keep the last configuration resident
+        s.set_box(atoms.box)
+        s.set_positions(atoms.xyz)
+        return s
+
+    def calEnergyForceVirials(self, atoms: AtomData, energies=True, forces=True, virials=True):
+        what = ENERGY | (FORCE if forces else 0) | (VIRIAL if virials else 0)
+        s = self.system(atoms)
+        s.compute(what)
+        return s.results(energies=energies, forces=forces or virials, virials=virials)
+
+    def calEnergy(self, atoms: AtomData) -> float:
+        s = self.system(atoms)
+        s.compute(ENERGY)
+        return s.results(energies=False, forces=False, virials=False)["energy"]
+
+    def calEnergies(self, atoms: AtomData) -> np.ndarray:
+        s = self.system(atoms)
+        s.compute(ENERGY)
+        return s.results(energies=True, forces=False, virials=False)["energies"]
+
+    def calForces(self, atoms: AtomData) -> np.ndarray:
+        return self.calEnergyForceVirials(atoms, energies=False, forces=True, virials=False)["forces"]
+
+    def calStresses(self, atoms: AtomData) -> np.ndarray:
+        """per-atom stress columns xx yy zz xy xz yz yx zx zy = -virial (IPotential.java:197-253)"""
+        return -self.calEnergyForceVirials(atoms, energies=False, forces=False, virials=True)["virials"]
+
+    def calStress(self, atoms: AtomData) -> List[float]:
+        """[xx, yy, zz, xy, xz, yz] = -virial / volume (IPotential.java:264-311)"""
+        v = self.calEnergyForceVirials(atoms, energies=False, forces=False, virials=True)["virial"]
+        vol = atoms.volume()
+        return [-v[0] / vol, -v[1] / vol, -v[2] / vol, -v[3] / vol, -v[4] / vol, -v[5] / vol]
+
+    def calFp(self, atoms: AtomData) -> np.ndarray:
+        """descriptors of every atom (jse_nnap_calFp, nnap_interface.cpp:11-24)"""
+        return self.system(atoms).descriptors()

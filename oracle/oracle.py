@@ -1,0 +1,199 @@
+"""ctypes front end of the FP64 oracle (nnap_oracle.c).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, by the smoke check and by the
+CPU-baseline leg of bench.py, never by the product package.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+MAX_LAYERS = 8
+
+_c_int_p = ctypes.POINTER(ctypes.c_int)
+_c_dbl_p = ctypes.POINTER(ctypes.c_double)
+
+
+class OrcModel(ctypes.Structure):
+    _fields_ = [
+        ("ntypes", ctypes.c_int),
+        ("nmax", _c_int_p),
+        ("lmax", _c_int_p),
+        ("size_np", _c_int_p),
+        ("rcut", _c_dbl_p),
+        ("nlayers", _c_int_p),
+        ("hidden", _c_int_p),
+        ("fp_param", ctypes.POINTER(_c_dbl_p)),
+        ("nn_param", ctypes.POINTER(_c_dbl_p)),
+        ("norm_param", ctypes.POINTER(_c_dbl_p)),
+    ]
+
+
+def build():
+    subprocess.check_call(["make", "-s", "-C", HERE])
+
+
+_LIBS = {}
+
+
+def lib(fast: bool = False):
+    name = "liborc_fast.so" if fast else "liborc.so"
+    if name not in _LIBS:
+        path = os.path.join(HERE, name)
+        if not os.path.exists(path):
+            raise RuntimeError(f"{path} not built (make -C oracle)")
+        L = ctypes.CDLL(path)
+        L.orc_neighbors_celllist.restype = ctypes.c_int
+        L.orc_neighbors_bruteforce.restype = ctypes.c_int
+        L.orc_system_efv.restype = ctypes.c_int
+        _LIBS[name] = L
+    return _LIBS[name]
+
+
+def _ip(a):
+    return a.ctypes.data_as(_c_int_p)
+
+
+def _dp(a):
+    return a.ctypes.data_as(_c_dbl_p)
+
+
+class Oracle:
+    """Holds a model (jse_amd.model.Model) in the oracle's C layout."""
+
+    def __init__(self, mdl, fast: bool = False):
+        self.mdl = mdl
+        self.L = lib(fast)
+        nt = mdl.ntypes
+        sp = mdl.species
+        self._nmax = np.array([s.nmax for s in sp], dtype=np.int32)
+        self._lmax = np.array([s.lmax for s in sp], dtype=np.int32)
+        self._P = np.array([s.P for s in sp], dtype=np.int32)
+        self._rc = np.array([s.rcut for s in sp], dtype=np.float64)
+        self._nl = np.array([len(s.hidden) for s in sp], dtype=np.int32)
+        hid = np.zeros((nt, MAX_LAYERS), dtype=np.int32)
+        for t, s in enumerate(sp):
+            hid[t, :len(s.hidden)] = s.hidden
+        self._hid = hid
+        self._W = [np.ascontiguousarray(s.W.reshape(-1), dtype=np.float64) for s in sp]
+        self._nn = [np.ascontiguousarray(s.nn_param, dtype=np.float64) for s in sp]
+        self._norm = [np.ascontiguousarray(s.norm_param, dtype=np.float64) for s in sp]
+        self._Wp = (_c_dbl_p * nt)(*[_dp(a) for a in self._W])
+        self._nnp = (_c_dbl_p * nt)(*[_dp(a) for a in self._nn])
+        self._normp = (_c_dbl_p * nt)(*[_dp(a) for a in self._norm])
+        self.c = OrcModel(nt, _ip(self._nmax), _ip(self._lmax), _ip(self._P), _dp(self._rc), _ip(self._nl),
+                          _ip(self._hid), self._Wp, self._nnp, self._normp)
+
+    # --- per-atom entry points (jse_nnap_calFp / calEnergy / calEnergyForce) ---
+    def cal_fp(self, ctype, d, tj):
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+        n = len(d)
+        dx, dy, dz = (np.ascontiguousarray(d[:, k]) for k in range(3))
+        tj = np.ascontiguousarray(tj, dtype=np.int32)
+        S = self.mdl.species[ctype - 1].S
+        fp = np.zeros(S)
+        rc = self.L.orc_cal_fp(ctypes.byref(self.c), ctype, _dp(dx), _dp(dy), _dp(dz), _ip(tj), n, _dp(fp))
+        if rc:
+            raise ValueError(f"oracle cal_fp code {rc}")
+        return fp
+
+    def cal_energy_force(self, ctype, d, tj):
+        d = np.ascontiguousarray(d, dtype=np.float64).reshape(-1, 3)
+        n = len(d)
+        dx, dy, dz = (np.ascontiguousarray(d[:, k]) for k in range(3))
+        tj = np.ascontiguousarray(tj, dtype=np.int32)
+        e = np.zeros(1)
+        g = np.zeros((3, max(n, 1)))
+        rc = self.L.orc_cal_energy_force(ctypes.byref(self.c), ctype, _dp(dx), _dp(dy), _dp(dz), _ip(tj), n,
+                                         _dp(e), _dp(g[0]), _dp(g[1]), _dp(g[2]))
+        if rc:
+            raise ValueError(f"oracle cal_energy_force code {rc}")
+        return float(e[0]), g[:, :n].T.copy()
+
+    def mlp(self, ctype, fp):
+        fp = np.ascontiguousarray(fp, dtype=np.float64)
+        e = np.zeros(1)
+        g = np.zeros_like(fp)
+        rc = self.L.orc_mlp(ctypes.byref(self.c), ctype, _dp(fp), _dp(e), _dp(g))
+        if rc:
+            raise ValueError(f"oracle mlp code {rc}")
+        return float(e[0]), g
+
+    # --- whole system ---
+    def system_efv(self, st, cap: int = 256):
+        n = st.natoms
+        xyz = np.ascontiguousarray(st.xyz, dtype=np.float64)
+        ty = np.ascontiguousarray(st.types, dtype=np.int32)
+        box = np.ascontiguousarray(st.box, dtype=np.float64)
+        E = np.zeros(n)
+        F = np.zeros((n, 3))
+        V = np.zeros((n, 9))
+        cnt = np.zeros(n, dtype=np.int32)
+        while True:
+            rc = self.L.orc_system_efv(ctypes.byref(self.c), n, _dp(xyz), _ip(ty), _dp(box), cap, _dp(E), _dp(F),
+                                       _dp(V), _ip(cnt))
+            if rc == 2:
+                cap *= 2
+                continue
+            if rc:
+                raise ValueError(f"oracle system_efv code {rc}")
+            return {"energies": E, "forces": F, "virials": V, "energy": E.sum(), "virial": V.sum(0),
+                    "nei_count": cnt}
+
+
+def wrap(xyz, box):
+    xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+    out = np.zeros_like(xyz)
+    lib().orc_wrap(len(xyz), _dp(xyz), _dp(np.ascontiguousarray(box, dtype=np.float64)), _dp(out))
+    return out
+
+
+def neighbors(xyz_wrapped, box, rmax, mode="celllist", cap=256):
+    """Returns (count, key, d) with key = j*32 + shift_code; celllist keeps the
+    reference's visiting order, bruteforce is sorted by key."""
+    x = np.ascontiguousarray(xyz_wrapped, dtype=np.float64)
+    n = len(x)
+    box = np.ascontiguousarray(box, dtype=np.float64)
+    fn = lib().orc_neighbors_celllist if mode == "celllist" else lib().orc_neighbors_bruteforce
+    while True:
+        cnt = np.zeros(n, dtype=np.int32)
+        key = np.zeros((n, cap), dtype=np.int32)
+        d = np.zeros((n, cap, 3))
+        rc = fn(n, _dp(x), _dp(box), ctypes.c_double(rmax), cap, _ip(cnt), _ip(key), _dp(d))
+        if rc:
+            raise ValueError(f"oracle neighbours code {rc}")
+        if n and cnt.max() > cap:
+            cap = int(cnt.max()) + 16
+            continue
+        return cnt, key, d
+
+
+def real_sph(lmax, d):
+    M = (lmax + 1) ** 2
+    y = np.zeros(M)
+    yt = np.zeros(M)
+    yp = np.zeros(M)
+    L = lib()
+    L.orc_real_sph(lmax, ctypes.c_double(d[0]), ctypes.c_double(d[1]), ctypes.c_double(d[2]), _dp(y), _dp(yt),
+                   _dp(yp))
+    return y, yt, yp
+
+
+def system_efv_par(orc: "Oracle", st, nthreads: int, cap: int = 256):
+    """Thread-parallel evaluation (liborc_fast.so) used as the CPU baseline."""
+    n = st.natoms
+    xyz = np.ascontiguousarray(st.xyz, dtype=np.float64)
+    ty = np.ascontiguousarray(st.types, dtype=np.int32)
+    box = np.ascontiguousarray(st.box, dtype=np.float64)
+    E = np.zeros(n)
+    F = np.zeros((n, 3))
+    V = np.zeros((n, 9))
+    rc = orc.L.orc_system_efv_par(ctypes.byref(orc.c), n, _dp(xyz), _ip(ty), _dp(box), cap, int(nthreads), _dp(E),
+                                  _dp(F), _dp(V))
+    if rc:
+        raise ValueError(f"oracle system_efv_par code {rc}")
+    return {"energies": E, "forces": F, "virials": V, "energy": E.sum(), "virial": V.sum(0)}

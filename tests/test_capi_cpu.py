@@ -1,0 +1,101 @@
+"""CPU checks of the C-ABI library: it loads, exports every symbol the header
+declares, and its host-side model parser agrees with the Python loader and
+reports errors with the reference's wording.  No GPU calls."""
+import ctypes
+import json
+import os
+import re
+
+import numpy as np
+import pytest
+
+from conftest import ROOT
+from jse_amd import model as M
+from jse_amd import nnap
+from jse_amd import workloads as W
+
+
+def _declared():
+    names = set()
+    for f in os.listdir(os.path.join(ROOT, "include")):
+        if f.endswith(".h"):
+            txt = open(os.path.join(ROOT, "include", f)).read()
+            names |= set(re.findall(r"\b(jse_nnap_\w+)\s*\(", txt))
+    return names
+
+
+def test_library_exports_header_symbols():
+    L = nnap.lib()
+    declared = _declared()
+    assert declared, "no declarations found"
+    for name in sorted(declared):
+        assert hasattr(L, name), f"{name} declared in include/ but not exported"
+    assert set(nnap.EXPORTS) <= declared
+
+
+def _parse(text):
+    L = nnap.lib()
+    nt = ctypes.c_int()
+    P = (ctypes.c_int * 8)()
+    S = (ctypes.c_int * 8)()
+    rc = L.jse_nnap_model_parse(text.encode(), ctypes.byref(nt), P, S)
+    return rc, nt.value, list(P)[:nt.value], list(S)[:nt.value]
+
+
+@pytest.mark.parametrize("wtype", ["default", "none", "full", "exfull"])
+def test_cpp_parser_matches_python_loader(wtype):
+    md = M.random_model(["Cu", "Zr", "Al"], 4, 5, 5.0, (8, 8), seed=2, wtype=wtype)
+    if wtype in ("full", "exfull", "none"):
+        # random_model sizes the default layout; rebuild the NN for the requested wtype
+        mdl0 = M.load_model({"models": [dict(m, nn=None) for m in md["models"]]})
+        md = M.random_model(["Cu", "Zr", "Al"], 4, 5, 5.0, (8, 8), seed=2, wtype="default")
+        for m, sp in zip(md["models"], mdl0.species):
+            m["basis"]["wtype"] = wtype
+            m["nn"]["input_dim"] = sp.S
+            rng = np.random.default_rng(1)
+            m["nn"]["hidden_weights"][0] = rng.normal(size=(8, sp.S)).tolist()
+            m["norm_mu"] = [0.0] * sp.S
+            m["norm_sigma"] = [1.0] * sp.S
+    rc, nt, P, S = _parse(json.dumps(md))
+    assert rc == 0, nnap.lib().jse_nnap_last_error()
+    mdl = M.load_model(md)
+    assert nt == 3 and P == [s.P for s in mdl.species] and S == [s.S for s in mdl.species]
+
+
+def test_parser_errors():
+    L = nnap.lib()
+    rc, *_ = _parse('{"models": [{"basis": {}}]}')
+    assert rc == 1 and b"No symbol in model" in L.jse_nnap_last_error()
+    rc, *_ = _parse('{"version": 7, "models": []}')
+    assert rc == 1 and b"Unsupported version" in L.jse_nnap_last_error()
+    rc, *_ = _parse('{"models": [{"symbol": "Cu", "basis": {"lmax": 13}}]}')
+    assert rc == 1 and b"lmax MUST be in [0, 12]" in L.jse_nnap_last_error()
+    rc, *_ = _parse("/nonexistent/model.json")
+    assert rc == 1 and b"cannot open model file" in L.jse_nnap_last_error()
+    rc, *_ = _parse('{"models": [')
+    assert rc == 1
+
+
+def test_python_loader_packing_matches_reference_layout():
+    md = M.random_model(["Cu", "Zr"], 2, 1, 4.0, (3,), seed=0)
+    md["models"][0]["norm_mu_eng"] = 0.5
+    md["models"][0]["norm_sigma_eng"] = 2.0
+    md["models"][1]["ref_eng"] = -1.25
+    mdl = M.load_model(md)
+    sp = mdl.species[1]
+    nn = md["models"][1]["nn"]
+    want = np.concatenate([np.ravel(nn["hidden_weights"][0]), nn["output_weight"], nn["hidden_biases"][0],
+                           [nn["output_bias"]]])
+    assert np.array_equal(sp.nn_param, want)
+    assert sp.norm_param[0] == 0.5 - 1.25 and sp.norm_param[1] == 2.0
+    assert sp.S == 2 * 3 * 2
+
+
+def test_workload_geometry():
+    st = W.ternary_liquid(n=2000, seed=3)
+    assert st.natoms == 2000 and np.isclose(st.box[0] ** 3 * 0.058, 2000)
+    d = st.xyz[:, None, :] - st.xyz[None, :, :]
+    d -= st.box * np.round(d / st.box)
+    r = np.sqrt((d ** 2).sum(-1)) + np.eye(2000) * 10
+    assert r.min() >= 2.0
+    assert np.bincount(st.types).tolist() == [0, 1000, 800, 200]

@@ -1,0 +1,308 @@
+"""Pins of the FP64 oracle (CPU): each check is chosen so that a plausible
+mistake in the restatement (a dropped term, a wrong sign, a transposed index,
+a wrong normalisation) fails it.  None of them re-types the oracle's own
+formula: they use closed forms, textbook identities, brute force, finite
+differences and invariances.  The reference ships no golden vectors for this
+path, so the committed fixtures in tests/golden are oracle outputs pinned by
+these checks."""
+import json
+import math
+import os
+
+import numpy as np
+import pytest
+
+from conftest import GOLDEN
+from jse_amd import model as M
+from jse_amd import workloads as W
+
+
+@pytest.fixture(scope="module")
+def O(oracle_mod):
+    oracle_mod.build()
+    return oracle_mod
+
+
+# --------------------------------------------------------------------------- wrap
+def test_wrap_edge_values(O):
+    box = np.array([10.0, 12.0, 14.0])
+    xyz = np.array([[0.0, 0.0, 0.0], [10.0, 12.0, 14.0], [-10.0, -12.0, -28.0], [25.5, -0.5, 13.999999],
+                    [3.3, 4.4, 5.5]])
+    w = O.wrap(xyz, box)
+    assert np.all(w >= 0) and np.all(w < box)
+    assert np.allclose(w[1], 0.0) and np.allclose(w[2], 0.0)
+    assert np.allclose(w[3], [5.5, 11.5, 13.999999])
+    # in-box coordinates go through x/L*L: unchanged up to one ulp
+    assert np.all(np.abs(w[4] - xyz[4]) <= np.spacing(xyz[4]))
+
+
+# ------------------------------------------------------------------- neighbours
+def _sorted_rows(cnt, key, d):
+    out = []
+    for i in range(len(cnt)):
+        o = np.argsort(key[i, :cnt[i]], kind="stable")
+        out.append((key[i, :cnt[i]][o], d[i, :cnt[i]][o]))
+    return out
+
+
+@pytest.mark.parametrize("case", ["cu256", "glass", "small_single_cell", "two_cells"])
+def test_celllist_equals_bruteforce(O, case):
+    if case == "cu256":
+        st = W.cu_fcc(seed=1)
+    elif case == "glass":
+        g = np.load(os.path.join(GOLDEN, "data_glass.npz"))
+        st = W.Structure(g["box"], g["xyz"][:1500] * 1.0, g["types"][:1500])
+        st = W.Structure(st.box, st.xyz, st.types)
+    elif case == "small_single_cell":      # L < 2 rc: one cell, 27 images, repeated j
+        st = W.Structure(np.array([9.95, 9.95, 9.95]), W.rsa(60, [9.95] * 3, 2.0, 5), np.ones(60, np.int32))
+    else:                                   # 2 cells per axis: i-1 and i+1 are the same cell
+        st = W.Structure(np.array([12.5, 12.5, 12.5]), W.rsa(100, [12.5] * 3, 2.0, 6), np.ones(100, np.int32))
+    x = O.wrap(st.xyz, st.box)
+    c1, k1, d1 = O.neighbors(x, st.box, 6.0, "celllist")
+    c2, k2, d2 = O.neighbors(x, st.box, 6.0, "bruteforce")
+    assert np.array_equal(c1, c2)
+    for (ka, da), (kb, db) in zip(_sorted_rows(c1, k1, d1), _sorted_rows(c2, k2, d2)):
+        assert np.array_equal(ka, kb)
+        assert np.array_equal(da, db)  # bit-identical FP64 displacements
+    if case == "small_single_cell":
+        j = k1[:, :c1.max()] >> 5
+        assert any(len(set(j[i, :c1[i]])) < c1[i] for i in range(len(c1)))  # multiple images of one j
+
+
+def test_fcc_shell_count(O):
+    st = W.cu_fcc(seed=1, sigma=0.0)
+    x = O.wrap(st.xyz, st.box)
+    cnt, key, d = O.neighbors(x, st.box, 6.0, "celllist")
+    assert np.all(cnt == 12 + 6 + 24 + 12 + 24)  # shells at a/sqrt2, a, a sqrt(3/2), a sqrt2, a sqrt(5/2)
+
+
+def test_reference_fixture_fcc108(O):
+    g = np.load(os.path.join(GOLDEN, "cufcc108.npz"))
+    x = O.wrap(g["xyz"], g["box"])
+    cnt, key, d = O.neighbors(x, g["box"], 3.0, "celllist")
+    assert np.all(cnt == 12)  # nearest shell at 2.553 A of a = 3.61
+
+
+# ------------------------------------------------------------- harmonics / radial
+def _real_sph_scipy(lmax, v):
+    return W._real_sph_np(lmax, np.array([v[0]]), np.array([v[1]]), np.array([v[2]]))[0]
+
+
+def test_harmonics_match_scipy_convention(O):
+    rng = np.random.default_rng(0)
+    for _ in range(20):
+        v = rng.normal(size=3)
+        for lmax in (6, 12):
+            y, _, _ = O.real_sph(lmax, v)
+            assert np.allclose(y, _real_sph_scipy(lmax, v), atol=1e-12)
+
+
+def test_harmonics_orthonormal(O):
+    lmax = 8
+    M_ = (lmax + 1) ** 2
+    xg, wg = np.polynomial.legendre.leggauss(24)
+    nphi = 40
+    G = np.zeros((M_, M_))
+    for ct, w in zip(xg, wg):
+        st = math.sqrt(1 - ct * ct)
+        for k in range(nphi):
+            ph = 2 * math.pi * k / nphi
+            y, _, _ = O.real_sph(lmax, [st * math.cos(ph), st * math.sin(ph), ct])
+            G += w * (2 * math.pi / nphi) * np.outer(y, y)
+    assert np.allclose(G, np.eye(M_), atol=1e-12)
+
+
+def test_harmonic_derivatives_finite_difference(O):
+    rng = np.random.default_rng(1)
+    lmax, h = 7, 1e-6
+    for _ in range(10):
+        th, ph = rng.uniform(0.2, 2.9), rng.uniform(-3, 3)
+        def y_at(t, p):
+            return O.real_sph(lmax, [math.sin(t) * math.cos(p), math.sin(t) * math.sin(p), math.cos(t)])[0]
+        _, yt, yp = O.real_sph(lmax, [math.sin(th) * math.cos(ph), math.sin(th) * math.sin(ph), math.cos(th)])
+        assert np.allclose(yt, (y_at(th + h, ph) - y_at(th - h, ph)) / (2 * h), atol=1e-7)
+        assert np.allclose(yp, (y_at(th, ph + h) - y_at(th, ph - h)) / (2 * h), atol=1e-7)
+
+
+def _one_species(nmax=5, lmax=6, rcut=6.0, hidden=(16, 16), seed=3):
+    return M.load_model(M.random_model(["Cu"], nmax, lmax, rcut, hidden, seed))
+
+
+def test_single_neighbor_on_axis_closed_form(O):
+    """A single neighbour on +z: A[n][l0] = fc T_n Y_l0(z) so fp[n][l] = (fc T_n)^2 P_l(1) = (fc T_n)^2."""
+    mdl = _one_species()
+    orc = O.Oracle(mdl)
+    r, rc = 2.7, 6.0
+    fp = orc.cal_fp(1, [[0.0, 0.0, r]], [1])
+    x = 1 - 2 * r / rc
+    fc = (1 - (r / rc) ** 2) ** 4
+    T = [math.cos(n * math.acos(x)) for n in range(6)]
+    want = np.array([[(fc * T[n]) ** 2 for _ in range(7)] for n in range(6)]).reshape(-1)
+    assert np.allclose(fp, want, rtol=1e-13, atol=1e-15)
+
+
+def test_descriptor_addition_theorem(O):
+    """fp[p][l] = sum_{j,k} c_j[p] c_k[p] P_l(cos gamma_jk): independent of the Y_lm convention."""
+    from scipy.special import eval_legendre
+    mdl = M.load_model(M.random_model(["Cu", "Zr", "Al"], 4, 6, 5.5, (8, 8), seed=4))
+    sp = mdl.species[1]
+    orc = O.Oracle(mdl)
+    rng = np.random.default_rng(2)
+    d = rng.uniform(-3.5, 3.5, size=(9, 3))
+    tj = rng.integers(1, 4, size=9)
+    fp = orc.cal_fp(2, d, tj).reshape(sp.P, sp.lmax + 1)
+    r = np.linalg.norm(d, axis=1)
+    keep = r < sp.rcut
+    d, tj, r = d[keep], tj[keep], r[keep]
+    x = 1 - 2 * r / sp.rcut
+    T = np.array([np.cos(n * np.arccos(x)) for n in range(sp.K)]).T
+    fc = (1 - (r / sp.rcut) ** 2) ** 4
+    c = np.einsum("jpk,jk->jp", sp.W[tj - 1], T) * fc[:, None]
+    u = d / r[:, None]
+    cosg = np.clip(u @ u.T, -1, 1)
+    for l in range(sp.lmax + 1):
+        Pl = eval_legendre(l, cosg)
+        want = np.einsum("jp,kp,jk->p", c, c, Pl)
+        assert np.allclose(fp[:, l], want, rtol=1e-11, atol=1e-12)
+
+
+def test_default_wtype_structure():
+    m = M.load_model(M.random_model(["A", "B", "C"], 3, 2, 5.0, (4,), seed=1))
+    sp = m.species[0]
+    K = 4
+    assert sp.P == 2 * K and sp.W.shape == (3, 2 * K, K)
+    for t in (1, 2, 3):
+        wt = t if t % 2 == 1 else -t
+        assert np.array_equal(sp.W[t - 1, :K], np.eye(K))
+        assert np.array_equal(sp.W[t - 1, K:], wt * np.eye(K))
+    one = M.load_model(M.random_model(["A"], 3, 2, 5.0, (4,), seed=1))
+    assert one.species[0].P == K and np.array_equal(one.species[0].W[0], np.eye(K))
+
+
+# --------------------------------------------------------------------------- MLP
+def test_mlp_against_torch_autograd(O):
+    torch = pytest.importorskip("torch")
+    md = M.random_model(["Cu", "Zr"], 4, 4, 5.0, (12, 9, 7), seed=9)
+    md["models"][0]["norm_mu_eng"] = 0.3
+    md["models"][0]["norm_sigma_eng"] = 1.7
+    md["models"][1]["ref_eng"] = -2.5
+    mdl = M.load_model(md)
+    orc = O.Oracle(mdl)
+    rng = np.random.default_rng(5)
+    for ct in (1, 2):
+        sp = mdl.species[ct - 1]
+        mu = rng.normal(size=sp.S)
+        sg = rng.uniform(0.5, 2, size=sp.S)
+        md["models"][ct - 1]["norm_mu"] = mu.tolist()
+        md["models"][ct - 1]["norm_sigma"] = sg.tolist()
+    mdl = M.load_model(md)
+    orc = O.Oracle(mdl)
+    for ct in (1, 2):
+        sp = mdl.species[ct - 1]
+        fp = rng.normal(size=sp.S)
+        e, g = orc.mlp(ct, fp)
+        x = torch.tensor(fp, dtype=torch.float64, requires_grad=True)
+        mu = torch.tensor(sp.norm_param[2:2 + sp.S])
+        sg = torch.tensor(sp.norm_param[2 + sp.S:])
+        h = (x - mu) / sg
+        nn = sp.nn_param
+        off, cols = 0, sp.S
+        Ws = []
+        for hd in sp.hidden:
+            Ws.append(torch.tensor(nn[off:off + hd * cols].reshape(hd, cols)))
+            off += hd * cols
+            cols = hd
+        wo = torch.tensor(nn[off:off + cols]); off += cols
+        for li, hd in enumerate(sp.hidden):
+            b = torch.tensor(nn[off:off + hd]); off += hd
+            h = torch.nn.functional.silu(Ws[li] @ h + b)
+        y = wo @ h + float(nn[off])
+        E = y * sp.norm_param[1] + sp.norm_param[0]
+        E.backward()
+        assert abs(float(E) - e) < 1e-12
+        assert np.allclose(x.grad.numpy(), g, rtol=1e-11, atol=1e-13)
+
+
+# ------------------------------------------------------------ forces and virials
+def _small_system(seed=3):
+    st = W.Structure(np.array([13.0, 13.5, 14.0]), W.rsa(120, [13.0, 13.5, 14.0], 2.0, seed),
+                     (np.arange(120) % 3 + 1).astype(np.int32))
+    md = M.random_model(["Cu", "Zr", "Al"], 4, 4, 5.5, (10, 10), seed=seed)
+    mdl = M.load_model(md)
+    fps = W.descriptors_np(mdl, st)
+    return st, M.load_model(M.set_normalization(md, fps))
+
+
+def test_forces_are_minus_energy_gradient(O):
+    st, mdl = _small_system()
+    orc = O.Oracle(mdl)
+    res = orc.system_efv(st)
+    h = 1e-5
+    for i in (0, 17, 61):
+        for c in range(3):
+            xp = st.xyz.copy(); xp[i, c] += h
+            xm = st.xyz.copy(); xm[i, c] -= h
+            ep = orc.system_efv(W.Structure(st.box, xp, st.types))["energy"]
+            em = orc.system_efv(W.Structure(st.box, xm, st.types))["energy"]
+            assert abs(-(ep - em) / (2 * h) - res["forces"][i, c]) < 2e-7 * max(1, abs(res["forces"][i, c]))
+
+
+def test_virial_is_minus_strain_derivative(O):
+    """V_ab = -dE/d eps_ba under x -> (1+eps) x applied to box and positions."""
+    st, mdl = _small_system(seed=4)
+    orc = O.Oracle(mdl)
+    res = orc.system_efv(st)
+    V = res["virial"]  # xx yy zz xy xz yz yx zx zy
+    comp = {(0, 0): 0, (1, 1): 1, (2, 2): 2, (0, 1): 3, (0, 2): 4, (1, 2): 5, (1, 0): 6, (2, 0): 7, (2, 1): 8}
+    h = 1e-6
+    for (a, b), k in comp.items():
+        if a != b:
+            continue  # shear strain of an orthogonal box is not representable; diagonal ones are
+        def e_at(s):
+            F = np.eye(3); F[b, a] += s
+            return orc.system_efv(W.Structure(st.box * np.diag(F), st.xyz @ F.T, st.types))["energy"]
+        dE = (e_at(h) - e_at(-h)) / (2 * h)
+        assert abs(-dE - V[k]) < 1e-6 * max(1, abs(V[k]))
+    # rotational invariance makes the total virial symmetric
+    assert abs(V[3] - V[6]) < 1e-9 and abs(V[4] - V[7]) < 1e-9 and abs(V[5] - V[8]) < 1e-9
+    assert np.allclose(res["virials"].sum(0), V)
+
+
+def test_momentum_and_invariances(O):
+    st, mdl = _small_system(seed=5)
+    orc = O.Oracle(mdl)
+    res = orc.system_efv(st)
+    assert np.abs(res["forces"].sum(0)).max() < 1e-10
+    # translation
+    t = orc.system_efv(W.Structure(st.box, st.xyz + np.array([1.3, -0.7, 5.1]), st.types))
+    assert abs(t["energy"] - res["energy"]) < 1e-9
+    # permutation of atom order
+    p = np.random.default_rng(0).permutation(st.natoms)
+    q = orc.system_efv(W.Structure(st.box, st.xyz[p], st.types[p]))
+    assert abs(q["energy"] - res["energy"]) < 1e-9
+    assert np.allclose(q["forces"], res["forces"][p], atol=1e-10)
+
+
+def test_atom_energy_rotation_invariant(O):
+    st, mdl = _small_system(seed=6)
+    orc = O.Oracle(mdl)
+    rng = np.random.default_rng(3)
+    d = rng.uniform(-3, 3, size=(14, 3))
+    tj = rng.integers(1, 4, size=14)
+    e0, g0 = orc.cal_energy_force(2, d, tj)
+    from scipy.spatial.transform import Rotation
+    R = Rotation.random(random_state=4).as_matrix()
+    e1, g1 = orc.cal_energy_force(2, d @ R.T, tj)
+    assert abs(e1 - e0) < 1e-11
+    assert np.allclose(g1, g0 @ R.T, atol=1e-11)
+
+
+# --------------------------------------------------------------- golden fixtures
+@pytest.mark.parametrize("name", ["golden_cu256", "golden_glass"])
+def test_oracle_matches_committed_fixtures(O, name):
+    g = np.load(os.path.join(GOLDEN, name + ".npz"))
+    mdl = M.load_model(json.loads(str(g["model"])))
+    res = O.Oracle(mdl).system_efv(W.Structure(g["box"], g["xyz"], g["types"]))
+    assert np.allclose(res["energies"], g["energies"], rtol=0, atol=1e-12)
+    assert np.allclose(res["forces"], g["forces"], rtol=0, atol=1e-11)
+    assert np.allclose(res["virials"], g["virials"], rtol=0, atol=1e-11)
