@@ -102,10 +102,17 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax);
  * status 4 if a row does not fit in `cap` (count[] is still written). */
 int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *key, float *d);
 
+/* Per-pair gradients of the last compute with forces: g[natoms][cap][3] =
+ * -dE_i/d(r_j - r_i) in the row order of jse_nnap_system_get_neighbors, the
+ * quantity the reference's jse_nnap_calEnergyForce returns in
+ * rGradNlDx/Dy/Dz (nnap_interface.cpp:48-89). */
+int jse_nnap_system_get_pair_gradients(jse_nnap_system s, int cap, float *g);
+
 /* Timing of `iters` compute() calls after `warmup` untimed ones, with HIP
- * events on the system's stream.  ms_step = wall time per call; phase_ms[k]
- * = average device time of phase k per call, k = 0 neighbour build, 1
- * reverse index, 2 descriptor forward, 3 MLP, 4 descriptor backward,
+ * events on the system's stream.  ms_step = device time per call of `iters`
+ * back-to-back calls; phase_ms[k] = average device time of phase k per call
+ * from a second pass of single calls, k = 0 neighbour build, 1 reverse
+ * index, 2 descriptor forward, 3 MLP, 4 descriptor backward,
  * 5 forces/virials/totals (nphase <= 6). */
 int jse_nnap_system_benchmark(jse_nnap_system s, int what, int warmup, int iters, double *ms_step, double *phase_ms,
                               int nphase);

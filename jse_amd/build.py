@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("JSE_AMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["nl.hip", "desc.hip", "mlp.hip", "mlp_bf16.hip", "force.hip", "capi.cpp", "model_host.cpp"]
+HIP_SOURCES = ["nl.hip", "desc.hip", "mlp.hip", "mlp_mfma.hip", "force.hip", "capi.cpp", "model_host.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
     # packed FP32 VALU ops are kept out of every kernel

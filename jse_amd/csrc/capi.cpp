@@ -69,6 +69,7 @@ uint16_t to_bf16(float f) {  // round to nearest even
 
 struct SpeciesDev {
     float *W = nullptr, *mu = nullptr, *sig = nullptr;
+    double *mu64 = nullptr, *sig64 = nullptr;
     float *w_fwd = nullptr, *w_bwd = nullptr, *bias = nullptr;
     uint16_t *wb_fwd = nullptr, *wb_bwd = nullptr;
 };
@@ -78,6 +79,7 @@ struct jse_nnap_model_s {
     ModelHost host;
     std::vector<SpeciesDev> dev;
     float *zeros = nullptr, *ones = nullptr;  // identity normalisation for raw descriptors
+    double *zeros64 = nullptr, *ones64 = nullptr;
     int smax = 0, pmmax = 0;
 
     void upload() {
@@ -93,6 +95,8 @@ struct jse_nnap_model_s {
             std::vector<float> sg(sp.norm.begin() + 2 + S, sp.norm.end());
             d.mu = dupload(mu);
             d.sig = dupload(sg);
+            d.mu64 = dupload(std::vector<double>(sp.norm.begin() + 2, sp.norm.begin() + 2 + S));
+            d.sig64 = dupload(std::vector<double>(sp.norm.begin() + 2 + S, sp.norm.end()));
             // forward layout: per layer [in][out], then output W; backward layout: per layer [out][in]
             std::vector<float> wf, wb, bias;
             size_t off = 0;
@@ -121,14 +125,18 @@ struct jse_nnap_model_s {
         std::vector<float> z((size_t)smax, 0.f), o((size_t)smax, 1.f);
         zeros = dupload(z);
         ones = dupload(o);
+        zeros64 = dupload(std::vector<double>((size_t)smax, 0.0));
+        ones64 = dupload(std::vector<double>((size_t)smax, 1.0));
     }
     ~jse_nnap_model_s() {
         for (auto &d : dev) {
-            dfree(d.W); dfree(d.mu); dfree(d.sig); dfree(d.w_fwd); dfree(d.w_bwd); dfree(d.bias);
+            dfree(d.W); dfree(d.mu); dfree(d.sig); dfree(d.mu64); dfree(d.sig64); dfree(d.w_fwd); dfree(d.w_bwd); dfree(d.bias);
             dfree(d.wb_fwd); dfree(d.wb_bwd);
         }
         dfree(zeros);
         dfree(ones);
+        dfree(zeros64);
+        dfree(ones64);
     }
 };
 
@@ -141,7 +149,8 @@ struct jse_nnap_system_s {
     std::vector<int> seg;  // species segment offsets into rows
     CellGrid grid{};
     hipStream_t stream = nullptr;
-    bool have_pos = false, have_nl = false;
+    bool have_pos = false, have_nl = false, pending = false;
+    int *h_overflow = nullptr;  // pinned host copy of the overflow flag
     int last_max_count = 0;
 
     double *d_xyz = nullptr, *d_xw = nullptr, *d_sxw = nullptr;
@@ -163,11 +172,11 @@ struct jse_nnap_system_s {
         dfree(d_E); dfree(d_F); dfree(d_V); dfree(d_partial); dfree(d_tot);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
+        if (h_overflow) (void)hipHostFree(h_overflow);
         if (stream) (void)hipStreamDestroy(stream);
     }
     void free_nl() {
-        dfree(nl.cnt); dfree(nl.key); dfree(nl.dx); dfree(nl.dy); dfree(nl.dz); dfree(nl.rev);
-        dfree(nl.gx); dfree(nl.gy); dfree(nl.gz);
+        dfree(nl.cnt); dfree(nl.key); dfree(nl.d); dfree(nl.rev); dfree(nl.g);
     }
     void alloc_nl(int cap) {
         free_nl();
@@ -175,9 +184,9 @@ struct jse_nnap_system_s {
         const size_t m = (size_t)n * cap;
         nl.cnt = dalloc<int>(n);
         nl.key = dalloc<int>(m);
-        nl.dx = dalloc<float>(m); nl.dy = dalloc<float>(m); nl.dz = dalloc<float>(m);
+        nl.d = dalloc<float4>(m);
         nl.rev = dalloc<int>(m);
-        nl.gx = dalloc<float>(m); nl.gy = dalloc<float>(m); nl.gz = dalloc<float>(m);
+        nl.g = dalloc<float4>(m);
     }
 
     void set_box(const double *b) {
@@ -247,6 +256,8 @@ struct jse_nnap_system_s {
         d_rank = dalloc<int>(n);
         d_sidx = dalloc<int>(n);
         d_overflow = dalloc<int>(1);
+        JNN_HIP(hipHostMalloc((void **)&h_overflow, sizeof(int), hipHostMallocDefault));
+        *h_overflow = 0;
         d_x = dalloc<float>((size_t)n * m->smax);
         d_G = dalloc<float>((size_t)n * m->smax);
         d_A = dalloc<float>((size_t)n * m->pmmax);
@@ -265,31 +276,41 @@ struct jse_nnap_system_s {
         alloc_nl(cap);
     }
 
+    // Enqueue the neighbour build (no host synchronisation: a row longer
+    // than the capacity raises a device flag that finish() checks).
     void build_nl(bool timed) {
         JNN_CHECK(have_pos, ERR_ARG, "positions not set");
-        for (int attempt = 0; attempt < 4; ++attempt) {
-            if (timed) JNN_HIP(hipEventRecord(ev[0], stream));
-            JNN_HIP(hipMemsetAsync(d_cell_cnt, 0, sizeof(int) * grid.ncell, stream));
-            JNN_HIP(hipMemsetAsync(d_overflow, 0, sizeof(int), stream));
-            launch_wrap_bin(stream, n, d_xyz, grid, d_xw, d_cell, d_cell_cnt, d_rank);
-            launch_cell_scan(stream, grid.ncell, d_cell_cnt, d_cell_start);
-            launch_cell_scatter(stream, n, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw);
-            launch_nl_build(stream, n, grid, d_xw, d_cell, d_cell_start, d_sidx, d_sxw, nl, d_overflow);
-            int ovf = 0;
-            JNN_HIP(hipMemcpyAsync(&ovf, d_overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
-            JNN_HIP(hipStreamSynchronize(stream));
-            if (ovf == 0) {
-                if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
-                launch_nl_reverse(stream, n, nl);
-                if (timed) JNN_HIP(hipEventRecord(ev[2], stream));
-                have_nl = true;
-                return;
-            }
+        if (timed) JNN_HIP(hipEventRecord(ev[0], stream));
+        launch_fill_int(stream, grid.ncell, d_cell_cnt, 0);
+        launch_fill_int(stream, 1, d_overflow, 0);
+        launch_wrap_bin(stream, n, d_xyz, grid, d_xw, d_cell, d_cell_cnt, d_rank);
+        launch_cell_scan(stream, grid.ncell, d_cell_cnt, d_cell_start);
+        launch_cell_scatter(stream, n, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw);
+        launch_nl_build(stream, n, grid, d_xw, d_cell, d_cell_start, d_sidx, d_sxw, nl, d_overflow);
+        if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
+        launch_nl_reverse(stream, n, nl);
+        if (timed) JNN_HIP(hipEventRecord(ev[2], stream));
+        JNN_HIP(hipMemcpyAsync(h_overflow, d_overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
+        have_nl = true;
+        pending = true;
+    }
+    // Waits for the enqueued work; when a neighbour row overflowed, grows the
+    // capacity and redoes the last request (neighbour build or full compute).
+    void finish() {
+        JNN_HIP(hipStreamSynchronize(stream));
+        if (!pending) return;
+        pending = false;
+        for (int attempt = 0; *h_overflow > 0; ++attempt) {
+            const int ovf = *h_overflow;
             JNN_CHECK(ovf <= 512, ERR_CAPACITY,
                       "an atom has " + std::to_string(ovf) + " neighbours within rcut; at most 512 are supported");
+            JNN_CHECK(attempt < 4, ERR_CAPACITY, "neighbour list capacity retries exhausted");
             alloc_nl(((ovf + 16 + 63) / 64) * 64);
+            if (last_what) compute(last_what, false);
+            else build_nl(false);
+            JNN_HIP(hipStreamSynchronize(stream));
+            pending = false;
         }
-        throw Error(ERR_CAPACITY, "neighbour list capacity retries exhausted");
     }
 
     SegmentArgs seg_args(int t, bool raw) const {
@@ -303,6 +324,8 @@ struct jse_nnap_system_s {
         a.W = dv.W;
         a.mu = raw ? model->zeros : dv.mu;
         a.sig = raw ? model->ones : dv.sig;
+        a.mu64 = raw ? model->zeros64 : dv.mu64;
+        a.sig64 = raw ? model->ones64 : dv.sig64;
         a.x = d_x + (size_t)seg[t] * model->smax;
         a.A = d_A + (size_t)seg[t] * model->pmmax;
         a.G = d_G + (size_t)seg[t] * model->smax;
@@ -345,8 +368,12 @@ struct jse_nnap_system_s {
             a.E = d_E;
             a.G = sa.G;
             a.want_grad = grad;
-            if (precision == JSE_NNAP_BF16_MLP) launch_mlp_bf16(stream, a, dv.wb_fwd, dv.wb_bwd);
-            else launch_mlp_fp32(stream, a);
+            if (precision == JSE_NNAP_BF16_MLP) {
+                if (!launch_mlp_mfma_bf16(stream, a))
+                    throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
+            } else if (!launch_mlp_mfma_fp32(stream, a)) {
+                launch_mlp_fp32(stream, a);
+            }
         }
         if (timed) JNN_HIP(hipEventRecord(ev[4], stream));
         if (grad) {
@@ -358,7 +385,7 @@ struct jse_nnap_system_s {
             }
         }
         if (timed) JNN_HIP(hipEventRecord(ev[5], stream));
-        if (grad) launch_gather(stream, n, d_xw, box, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0);
+        if (grad) launch_gather(stream, n, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0);
         launch_reduce(stream, n, 1, 1, d_E, d_partial, kPartials, d_tot);
         if (what & JSE_NNAP_VIRIAL) launch_reduce(stream, n, 9, 9, d_V, d_partial, kPartials, d_tot + 1);
         if (timed) JNN_HIP(hipEventRecord(ev[6], stream));
@@ -515,6 +542,7 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
     return guard([&] {
         JNN_CHECK(s, ERR_ARG, "null system");
         JNN_CHECK(s->last_what != 0, ERR_ARG, "nothing computed yet");
+        s->finish();
         const bool grad = (s->last_what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL)) != 0;
         const bool vir = (s->last_what & JSE_NNAP_VIRIAL) != 0;
         JNN_CHECK(!(forces && !grad), ERR_ARG, "forces were not computed");
@@ -538,7 +566,9 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
     return guard([&] {
         JNN_CHECK(s && (fp || s->n == 0), ERR_ARG, "null argument");
         JNN_CHECK(smax >= s->model->smax, ERR_ARG, "smax smaller than the largest descriptor");
+        s->last_what = 0;
         s->build_nl(false);
+        s->finish();
         s->desc_fwd(true);
         std::vector<float> x((size_t)s->n * s->model->smax);
         std::vector<int> rows(s->n);
@@ -561,15 +591,14 @@ int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *k
     int rc = guard([&] {
         JNN_CHECK(s && count, ERR_ARG, "null argument");
         if (!s->have_nl) s->build_nl(false);
+        s->finish();
         const int n = s->n, c = s->nl.cap;
         std::vector<int> cnt(n), k((size_t)n * c);
-        std::vector<float> dx((size_t)n * c), dy((size_t)n * c), dz((size_t)n * c);
+        std::vector<float4> dd((size_t)n * c);
         if (n) {
             JNN_HIP(hipMemcpyAsync(cnt.data(), s->nl.cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));
             JNN_HIP(hipMemcpyAsync(k.data(), s->nl.key, sizeof(int) * k.size(), hipMemcpyDeviceToHost, s->stream));
-            JNN_HIP(hipMemcpyAsync(dx.data(), s->nl.dx, sizeof(float) * k.size(), hipMemcpyDeviceToHost, s->stream));
-            JNN_HIP(hipMemcpyAsync(dy.data(), s->nl.dy, sizeof(float) * k.size(), hipMemcpyDeviceToHost, s->stream));
-            JNN_HIP(hipMemcpyAsync(dz.data(), s->nl.dz, sizeof(float) * k.size(), hipMemcpyDeviceToHost, s->stream));
+            JNN_HIP(hipMemcpyAsync(dd.data(), s->nl.d, sizeof(float4) * dd.size(), hipMemcpyDeviceToHost, s->stream));
         }
         JNN_HIP(hipStreamSynchronize(s->stream));
         bool over = false;
@@ -581,9 +610,9 @@ int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *k
                 const size_t src = (size_t)i * c + q, dst = (size_t)i * cap + q;
                 if (key) key[dst] = k[src];
                 if (d) {
-                    d[3 * dst] = dx[src];
-                    d[3 * dst + 1] = dy[src];
-                    d[3 * dst + 2] = dz[src];
+                    d[3 * dst] = dd[src].x;
+                    d[3 * dst + 1] = dd[src].y;
+                    d[3 * dst + 2] = dd[src].z;
                 }
             }
         }
@@ -592,18 +621,55 @@ int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *k
     return rc;
 }
 
+int jse_nnap_system_get_pair_gradients(jse_nnap_system s, int cap, float *g) {
+    return guard([&] {
+        JNN_CHECK(s && g, ERR_ARG, "null argument");
+        JNN_CHECK(s->last_what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL), ERR_ARG, "forces were not computed");
+        s->finish();
+        const int n = s->n, c = s->nl.cap;
+        std::vector<int> cnt(n);
+        std::vector<float4> gg((size_t)n * c);
+        if (n) {
+            JNN_HIP(hipMemcpyAsync(cnt.data(), s->nl.cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));
+            JNN_HIP(hipMemcpyAsync(gg.data(), s->nl.g, sizeof(float4) * gg.size(), hipMemcpyDeviceToHost, s->stream));
+        }
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        for (int i = 0; i < n; ++i) {
+            const int m = std::min(std::min(cnt[i], c), cap);
+            for (int q = 0; q < m; ++q) {
+                const size_t src = (size_t)i * c + q, dst = (size_t)i * cap + q;
+                g[3 * dst] = gg[src].x;
+                g[3 * dst + 1] = gg[src].y;
+                g[3 * dst + 2] = gg[src].z;
+            }
+        }
+    });
+}
+
 int jse_nnap_system_benchmark(jse_nnap_system s, int what, int warmup, int iters, double *ms_step, double *phase_ms,
                               int nphase) {
     return guard([&] {
         JNN_CHECK(s && iters > 0 && warmup >= 0, ERR_ARG, "bad benchmark arguments");
         for (int w = 0; w < warmup; ++w) s->compute(what | JSE_NNAP_ENERGY, false);
-        JNN_HIP(hipStreamSynchronize(s->stream));
-        double acc[6] = {0};
+        s->finish();
+        // pass 1: back-to-back steps, bracketed by two events on the system's stream
         hipEvent_t t0, t1;
         JNN_HIP(hipEventCreate(&t0));
         JNN_HIP(hipEventCreate(&t1));
         JNN_HIP(hipEventRecord(t0, s->stream));
-        for (int it = 0; it < iters; ++it) {
+        for (int it = 0; it < iters; ++it) s->compute(what | JSE_NNAP_ENERGY, false);
+        JNN_HIP(hipEventRecord(t1, s->stream));
+        JNN_HIP(hipEventSynchronize(t1));
+        JNN_CHECK(*s->h_overflow == 0, ERR_CAPACITY, "neighbour capacity changed during the benchmark");
+        s->pending = false;
+        float tot;
+        JNN_HIP(hipEventElapsedTime(&tot, t0, t1));
+        (void)hipEventDestroy(t0);
+        (void)hipEventDestroy(t1);
+        if (ms_step) *ms_step = tot / iters;
+        // pass 2: per-phase device time, one step at a time
+        double acc[6] = {0};
+        for (int it = 0; it < iters && phase_ms && nphase > 0; ++it) {
             s->compute(what | JSE_NNAP_ENERGY, true);
             JNN_HIP(hipEventSynchronize(s->ev[6]));
             float ms;
@@ -612,13 +678,7 @@ int jse_nnap_system_benchmark(jse_nnap_system s, int what, int warmup, int iters
                 acc[k] += ms;
             }
         }
-        JNN_HIP(hipEventRecord(t1, s->stream));
-        JNN_HIP(hipEventSynchronize(t1));
-        float tot;
-        JNN_HIP(hipEventElapsedTime(&tot, t0, t1));
-        (void)hipEventDestroy(t0);
-        (void)hipEventDestroy(t1);
-        if (ms_step) *ms_step = tot / iters;
+        s->finish();
         for (int k = 0; k < nphase && k < 6; ++k) phase_ms[k] = acc[k] / iters;
     });
 }
@@ -627,6 +687,7 @@ int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long l
     return guard([&] {
         JNN_CHECK(s, ERR_ARG, "null system");
         if (!s->have_nl) s->build_nl(false);
+        s->finish();
         const int n = s->n, c = s->nl.cap;
         std::vector<int> cnt(n), rev((size_t)n * c);
         if (n) {

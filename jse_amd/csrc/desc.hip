@@ -109,9 +109,12 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
     using SH = DescShape<LMAX, K, P, MIX>;
     constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
     constexpr int TP = SH::TP, TM = SH::TM, NMB = SH::NMB, NT = SH::NT, G = SH::G;
-    __shared__ float sY[SH::YBUF];
-    __shared__ float sC[64 * P];
-    __shared__ float sA[P * M];
+    // phase 1 (per chunk): Y and c of 64 neighbours as FP32; phase 2 (per atom): A in FP64
+    constexpr int F1 = 64 * M + 64 * P;
+    constexpr int F2 = 2 * P * M;
+    __shared__ __align__(16) float sbuf[F1 > F2 ? F1 : F2];
+    float *sY = sbuf, *sC = sbuf + 64 * M;
+    double *sA = reinterpret_cast<double *>(sbuf);
     const int lane = threadIdx.x;
     const int grp = lane / NT, tile = lane % NT;
     const int pb = tile / NMB, mb = tile % NMB;
@@ -123,11 +126,12 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
         const int i = a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
-        float acc[TP][TM];
+        // FP64 accumulators: every c*Y product of two floats is exact in FP64
+        double acc[TP][TM];
 #pragma unroll
         for (int u = 0; u < TP; ++u)
 #pragma unroll
-            for (int v = 0; v < TM; ++v) acc[u][v] = 0.f;
+            for (int v = 0; v < TM; ++v) acc[u][v] = 0.0;
 
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int j = c0 + lane;
@@ -138,7 +142,8 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
 #pragma unroll
             for (int p = 0; p < P; ++p) c[p] = 0.f;
             if (j < n) {
-                const float dx = nl.dx[base + j], dy = nl.dy[base + j], dz = nl.dz[base + j];
+                const float4 dd = nl.d[base + j];
+                const float dx = dd.x, dy = dd.y, dz = dd.z;
                 const int tj = a.types[nl.key[base + j] >> 5];
                 const PairGeom g = pair_geom(dx, dy, dz);
                 if (g.r < rc) {
@@ -154,6 +159,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
                     for (int p = 0; p < P; ++p) c[p] = fc * rnp[p];
                 }
             }
+            __syncthreads();  // the previous chunk's tiles (and the previous atom's A) are done
 #pragma unroll
             for (int m = 0; m < M; ++m) sY[lane * M + m] = y[m];
 #pragma unroll
@@ -162,49 +168,43 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
             const int nc = min(64, n - c0);
             if (tile_on) {
                 for (int jj = grp; jj < nc; jj += G) {
-                    float cv[TP], yv[TM];
+                    double cv[TP], yv[TM];
 #pragma unroll
-                    for (int u = 0; u < TP; ++u) cv[u] = (p0 + u < P) ? sC[jj * P + p0 + u] : 0.f;
+                    for (int u = 0; u < TP; ++u) cv[u] = (p0 + u < P) ? (double)sC[jj * P + p0 + u] : 0.0;
 #pragma unroll
-                    for (int v = 0; v < TM; ++v) yv[v] = (m0 + v < M) ? sY[jj * M + m0 + v] : 0.f;
+                    for (int v = 0; v < TM; ++v) yv[v] = (m0 + v < M) ? (double)sY[jj * M + m0 + v] : 0.0;
 #pragma unroll
                     for (int u = 0; u < TP; ++u)
 #pragma unroll
                         for (int v = 0; v < TM; ++v) acc[u][v] += cv[u] * yv[v];
                 }
             }
+        }
+        __syncthreads();
+        // fixed-order sum of the G partial tiles into sA (FP64), group by group
+        for (int q = 0; q < G; ++q) {
+            if (tile_on && grp == q) {
+#pragma unroll
+                for (int u = 0; u < TP; ++u)
+#pragma unroll
+                    for (int v = 0; v < TM; ++v) {
+                        const int p = p0 + u, m = m0 + v;
+                        if (p < P && m < M) sA[p * M + m] = (q == 0) ? acc[u][v] : sA[p * M + m] + acc[u][v];
+                    }
+            }
             __syncthreads();
         }
-        // fixed-order sum of the G partial tiles
-        if (tile_on) {
-#pragma unroll
-            for (int u = 0; u < TP; ++u)
-#pragma unroll
-                for (int v = 0; v < TM; ++v) {
-                    const int p = p0 + u, m = m0 + v;
-                    if (p < P && m < M) sY[(grp * P + p) * M + m] = acc[u][v];
-                }
-        }
-        __syncthreads();
-        for (int e = lane; e < P * M; e += 64) {
-            float s = 0.f;
-#pragma unroll
-            for (int q = 0; q < G; ++q) s += sY[q * P * M + e];
-            sA[e] = s;
-            a.A[(size_t)row * (P * M) + e] = s;
-        }
-        __syncthreads();
+        for (int e = lane; e < P * M; e += 64) a.A[(size_t)row * (P * M) + e] = (float)sA[e];
         for (int e = lane; e < S; e += 64) {
             const int p = e / L1, l = e % L1;
-            float s = 0.f;
+            double s = 0.0;
             for (int m = l * l; m < (l + 1) * (l + 1); ++m) {
-                const float q = sA[p * M + m];
+                const double q = sA[p * M + m];
                 s += q * q;
             }
-            const float fp = (kPi4 / (float)(2 * l + 1)) * s;
-            a.x[(size_t)row * S + e] = (fp - a.mu[e]) / a.sig[e];
+            const double fp = (4.0 * 3.14159265358979323846 / (double)(2 * l + 1)) * s;
+            a.x[(size_t)row * S + e] = (float)((fp - a.mu64[e]) / a.sig64[e]);
         }
-        __syncthreads();
     }
 }
 
@@ -231,7 +231,8 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int j = c0 + lane;
             if (j >= n) continue;
-            const float dx = nl.dx[base + j], dy = nl.dy[base + j], dz = nl.dz[base + j];
+            const float4 dd = nl.d[base + j];
+            const float dx = dd.x, dy = dd.y, dz = dd.z;
             const int tj = a.types[nl.key[base + j] >> 5];
             const PairGeom g = pair_geom(dx, dy, dz);
             float ox = 0.f, oy = 0.f, oz = 0.f;
@@ -297,9 +298,7 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
                     oy -= dEdph * (g.cp * idxy);
                 }
             }
-            nl.gx[base + j] = ox;
-            nl.gy[base + j] = oy;
-            nl.gz[base + j] = oz;
+            nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
         }
         __syncthreads();
     }

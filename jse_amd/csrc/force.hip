@@ -5,10 +5,11 @@
 // (src/nnap/java/jsex/nnap/NNAP.java:575-592, src/main/java/jse/atom/
 // IPairPotential.java:938-1006).  Here every atom gathers instead of being
 // scattered into: the incoming g_{j->i} is read through the reverse-pair slot
-// of the neighbour list, so the sums are deterministic, need no atomics and
-// are carried in FP64 with Neumaier compensation.  The displacement of the
-// incoming pair is recomputed in FP64 exactly as the neighbour build formed
-// it: dx = (x_i + s L) - x_j with s the reverse shift.
+// of the neighbour list, so the sums are deterministic and need no atomics;
+// each lane sums its slots in FP64 and the wave combines lanes in a fixed
+// butterfly, the totals use Neumaier-compensated FP64 partial sums.  The
+// virial of an incoming pair uses that pair's displacement from j's row, the
+// FP64 (x_i + s L) - x_j of the neighbour build rounded once to FP32.
 #include "kernels.h"
 
 namespace jnn {
@@ -24,49 +25,60 @@ struct Neum {
     __device__ __forceinline__ double get() const { return s + c; }
 };
 
-__global__ __launch_bounds__(256) void k_gather(int n, const double *__restrict__ xw, double bx, double by, double bz,
-                                                NeighborList nl, double *__restrict__ F, double *__restrict__ V,
+// one wavefront per atom: lane k takes slot k of the row (and k+64, ...), the
+// 12 sums are then reduced across the wave by a fixed butterfly (deterministic)
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
+    return v;
+}
+
+__global__ __launch_bounds__(256) void k_gather(int n, NeighborList nl, double *__restrict__ F, double *__restrict__ V,
                                                 int want_virial) {
-#pragma clang fp contract(off)
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    const int lane = threadIdx.x & 63;
+    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
     if (i >= n) return;
     const int ci = min(nl.cnt[i], nl.cap);
     const size_t base = (size_t)i * nl.cap;
-    Neum fx, fy, fz;
-    Neum v[9];
-    const double xi = xw[3 * i], yi = xw[3 * i + 1], zi = xw[3 * i + 2];
-    for (int k = 0; k < ci; ++k) {
-        fx.add(-(double)nl.gx[base + k]);
-        fy.add(-(double)nl.gy[base + k]);
-        fz.add(-(double)nl.gz[base + k]);
+    double f[3] = {0.0, 0.0, 0.0};
+    double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
+    for (int k = lane; k < ci; k += 64) {
+        const float4 go = nl.g[base + k];
+        f[0] -= (double)go.x;
+        f[1] -= (double)go.y;
+        f[2] -= (double)go.z;
         const int r = nl.rev[base + k];
         if (r < 0) continue;
-        const int key = nl.key[base + k];
-        const int j = key >> 5;
+        const int j = nl.key[base + k] >> 5;
         const size_t sj = (size_t)j * nl.cap + r;
-        const double gx = (double)nl.gx[sj], gy = (double)nl.gy[sj], gz = (double)nl.gz[sj];
-        fx.add(gx);
-        fy.add(gy);
-        fz.add(gz);
+        const float4 gi = nl.g[sj];
+        const double gx = (double)gi.x, gy = (double)gi.y, gz = (double)gi.z;
+        f[0] += gx;
+        f[1] += gy;
+        f[2] += gz;
         if (want_virial) {
-            const int code = 26 - (key & 31);  // shift of the pair j -> i
-            const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
-            double ax = xi, ay = yi, az = zi;
-            if (sx) ax = ax + (sx > 0 ? bx : -bx);
-            if (sy) ay = ay + (sy > 0 ? by : -by);
-            if (sz) az = az + (sz > 0 ? bz : -bz);
-            const double dx = ax - xw[3 * j], dy = ay - xw[3 * j + 1], dz = az - xw[3 * j + 2];
-            v[0].add(dx * gx); v[1].add(dy * gy); v[2].add(dz * gz);
-            v[3].add(dx * gy); v[4].add(dx * gz); v[5].add(dy * gz);
-            v[6].add(dy * gx); v[7].add(dz * gx); v[8].add(dz * gy);
+            // displacement of the pair j -> i as the neighbour build formed it (FP64, rounded once)
+            const float4 dj = nl.d[sj];
+            const double dx = (double)dj.x, dy = (double)dj.y, dz = (double)dj.z;
+            v[0] += dx * gx; v[1] += dy * gy; v[2] += dz * gz;
+            v[3] += dx * gy; v[4] += dx * gz; v[5] += dy * gz;
+            v[6] += dy * gx; v[7] += dz * gx; v[8] += dz * gy;
         }
     }
-    F[3 * i] = fx.get();
-    F[3 * i + 1] = fy.get();
-    F[3 * i + 2] = fz.get();
+#pragma unroll
+    for (int c = 0; c < 3; ++c) f[c] = wave_sum(f[c]);
     if (want_virial) {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) V[9 * (size_t)i + c] = v[c].get();
+        for (int c = 0; c < 9; ++c) v[c] = wave_sum(v[c]);
+    }
+    if (lane == 0) {
+        F[3 * i] = f[0];
+        F[3 * i + 1] = f[1];
+        F[3 * i + 2] = f[2];
+        if (want_virial) {
+#pragma unroll
+            for (int c = 0; c < 9; ++c) V[9 * (size_t)i + c] = v[c];
+        }
     }
 }
 
@@ -92,17 +104,22 @@ __global__ __launch_bounds__(kRedBlock) void k_reduce_partial(int n, int stride,
 }
 __global__ __launch_bounds__(kRedBlock) void k_reduce_final(int np, int ncomp, const double *__restrict__ partial,
                                                            double *__restrict__ out) {
-    if (threadIdx.x >= ncomp) return;
-    Neum acc;
-    for (int b = 0; b < np; ++b) acc.add(partial[(size_t)b * ncomp + threadIdx.x]);
-    out[threadIdx.x] = acc.get();
+    __shared__ double sh[kRedBlock];
+    for (int c = 0; c < ncomp; ++c) {
+        sh[threadIdx.x] = (threadIdx.x < np) ? partial[(size_t)threadIdx.x * ncomp + c] : 0.0;
+        __syncthreads();
+        for (int off = kRedBlock / 2; off > 0; off >>= 1) {
+            if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
+            __syncthreads();
+        }
+        if (threadIdx.x == 0) out[c] = sh[0];
+        __syncthreads();
+    }
 }
 
-void launch_gather(hipStream_t s, int n, const double *xw, const double *box, const NeighborList &nl, double *F,
-                   double *V, int want_virial) {
+void launch_gather(hipStream_t s, int n, const NeighborList &nl, double *F, double *V, int want_virial) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_gather, dim3((n + 255) / 256), dim3(256), 0, s, n, xw, box[0], box[1], box[2], nl, F, V,
-                       want_virial);
+    hipLaunchKernelGGL(k_gather, dim3((n + 3) / 4), dim3(256), 0, s, n, nl, F, V, want_virial);
 }
 void launch_reduce(hipStream_t s, int n, int stride, int ncomp, const double *in, double *partial, int npartial,
                    double *out) {

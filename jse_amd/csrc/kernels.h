@@ -22,12 +22,13 @@ struct NeighborList {
     int cap = 0;
     int *cnt = nullptr;     // [N] true neighbour count (may exceed cap -> overflow)
     int *key = nullptr;     // [N*cap]
-    float *dx = nullptr, *dy = nullptr, *dz = nullptr;  // [N*cap] FP32 of the FP64 displacement
+    float4 *d = nullptr;    // [N*cap] (dx, dy, dz, 0): FP32 of the FP64 displacement r_j + shift - r_i
     int *rev = nullptr;     // [N*cap] slot of the reverse pair in j's row, -1 if absent
-    float *gx = nullptr, *gy = nullptr, *gz = nullptr;  // [N*cap] g_j = -dE_i/d(r_j - r_i)
+    float4 *g = nullptr;    // [N*cap] (gx, gy, gz, 0): g_j = -dE_i/d(r_j - r_i)
 };
 
 // ---- neighbour build ----
+void launch_fill_int(hipStream_t s, int n, int *p, int v);  // kernel fill (graph-capture safe)
 void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const CellGrid &g, double *xw, int *cell,
                      int *cell_cnt, int *rank);
 void launch_cell_scan(hipStream_t s, int ncell, const int *cell_cnt, int *cell_start);
@@ -45,6 +46,7 @@ struct SegmentArgs {
     float rc;
     const float *W;         // [ntypes][P][K] FP32 mixing (used by the dense kernel)
     const float *mu, *sig;  // normalisation [S]
+    const double *mu64, *sig64;  // the same in FP64 (descriptor normalisation)
     float *x;               // [nrows][S] normalised descriptors (forward out, MLP in)
     float *A;               // [nrows][P*M]
     float *G;               // [nrows][S]  2*(4pi/(2l+1)) * dE/dfp
@@ -68,11 +70,12 @@ struct MlpArgs {
     int want_grad;
 };
 void launch_mlp_fp32(hipStream_t s, const MlpArgs &a);
-void launch_mlp_bf16(hipStream_t s, const MlpArgs &a, const uint16_t *w_bf16_fwd, const uint16_t *w_bf16_bwd);
+// MFMA kernels for 2 hidden layers of supported widths; false when the shape has no compiled kernel
+bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a);
+bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a);
 
 // ---- forces / virials / totals ----
-void launch_gather(hipStream_t s, int n, const double *xw, const double *box, const NeighborList &nl, double *F,
-                   double *V, int want_virial);
+void launch_gather(hipStream_t s, int n, const NeighborList &nl, double *F, double *V, int want_virial);
 // deterministic FP64 sums: out[c] = sum_i in[i*stride + c] for c < ncomp
 void launch_reduce(hipStream_t s, int n, int stride, int ncomp, const double *in, double *partial, int npartial,
                    double *out);

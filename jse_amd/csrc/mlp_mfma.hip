@@ -1,0 +1,343 @@
+// Per-species 2-hidden-layer MLP, forward and input-gradient backward, on
+// MFMA matrix cores (reference semantics: nn_FeedForward.hpp:8-21,131-261;
+// normalisation nnap_main.hpp:267-323).
+//
+// A workgroup of 4 wavefronts owns a tile of 64 atoms of one species (rows);
+// wave w owns rows 16w..16w+15 and every output column, so the four GEMMs of
+// one tile
+//     Z1 = X W1^T + b1, H1 = silu(Z1)       [64 x S]  x [S x H1]
+//     Z2 = H1 W2^T + b2, y = wo . silu(Z2)  [64 x H1] x [H1 x H2]
+//     gZ1 = (gZ2 W2) * silu'(Z1)            [64 x H2] x [H2 x H1]
+//     gX  = gZ1 W1                          [64 x H1] x [H1 x S]
+// keep each wave's SiLU derivatives in its own registers between forward and
+// backward (the same lane -> (row, column) map serves both), so the only
+// cross-wave traffic is the staged weights.  Activations and weights live in
+// LDS with row strides chosen against bank conflicts; the grid is persistent
+// (weights staged once per workgroup).
+//
+// Two element types share the code:
+//   float  : v_mfma_f32_16x16x4_f32   (plain FP32 path)
+//   __bf16 : v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32 accumulation); the
+//            two forward GEMMs carry each operand as a hi + lo BF16 pair
+//            (three MFMAs: hi*hi + lo*hi + hi*lo), which keeps the energy at
+//            FP32-like accuracy, while the two backward GEMMs use single BF16.
+// Lane maps (16x16 output tile, 64 lanes): A[i][k] from lane i + 16*(k/E),
+// B[k][j] from lane j + 16*(k/E), D[i][j] in lane j + 16*(i/4), register i%4,
+// with E = 1 (f32, 4 k per instruction) or 8 consecutive k per lane (bf16, 32 k).
+// SiLU, biases, the energy and the output scaling stay FP32 in both.
+#include "common.h"
+#include "kernels.h"
+
+namespace jnn {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+constexpr float kPi4m = 12.566370614359172f;
+
+__device__ __forceinline__ __bf16 to_bf16_rne(float f) {
+    uint32_t u = __float_as_uint(f);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return __builtin_bit_cast(__bf16, (uint16_t)(u >> 16));
+}
+
+template <typename T>
+struct Mma;
+
+template <>
+struct Mma<float> {
+    static constexpr int KSTEP = 4;
+    static constexpr int PAD = 2;  // row stride == 2 (mod 32) dwords
+    static constexpr bool kTransposedCopies = false;
+    static constexpr bool kSplit = false;
+    __device__ static __forceinline__ float cvt(float v) { return v; }
+    __device__ static __forceinline__ float lo(float v) { return 0.f; }
+    __device__ static __forceinline__ int lane_k(int lane) { return lane >> 4; }
+    // A and B both contiguous in k (row-major [row][k])
+    __device__ static __forceinline__ f32x4 mma(const float *a, const float *b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], c, 0, 0, 0);
+    }
+};
+
+template <>
+struct Mma<__bf16> {
+    static constexpr int KSTEP = 32;
+    static constexpr int PAD = 8;  // row stride == 4 (mod 32) dwords: conflict-free 16-byte reads
+    static constexpr bool kTransposedCopies = true;
+    // forward operands as hi + lo BF16 pairs: x w ~ xh wh + xl wh + xh wl (FP32 accumulation)
+    static constexpr bool kSplit = true;
+    __device__ static __forceinline__ __bf16 cvt(float v) { return to_bf16_rne(v); }
+    __device__ static __forceinline__ __bf16 lo(float v) { return to_bf16_rne(v - (float)to_bf16_rne(v)); }
+    __device__ static __forceinline__ int lane_k(int lane) { return 8 * (lane >> 4); }
+    __device__ static __forceinline__ f32x4 mma(const __bf16 *a, const __bf16 *b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8 *>(a),
+                                                       *reinterpret_cast<const bf16x8 *>(b), c, 0, 0, 0);
+    }
+};
+
+__device__ __forceinline__ float silu_grad(float z, float *g) {
+    const float s = 1.f / (1.f + expf(-z));
+    *g = s * (1.f + z * (1.f - s));
+    return z * s;
+}
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// S <= KP (KP a multiple of 16 and of the instruction k), H1 = 16*C1, H2 = 16*C2.
+template <typename T, int C1, int C2, int KP>
+__global__ __launch_bounds__(256) void k_mlp2_mfma(MlpArgs a) {
+    using MM = Mma<T>;
+    constexpr int H1 = 16 * C1, H2 = 16 * C2;
+    constexpr int NS = KP / 16;                                          // column tiles of gX
+    constexpr int XS = KP + MM::PAD;                                      // row stride of X, W1
+    constexpr int HK = (H1 > H2 ? H1 : H2);
+    constexpr int HKP = ((HK + MM::KSTEP - 1) / MM::KSTEP) * MM::KSTEP;   // padded hidden k
+    constexpr int HS = HKP + MM::PAD;                                     // row stride of H, W2, transposes
+    constexpr int H1P = ((H1 + MM::KSTEP - 1) / MM::KSTEP) * MM::KSTEP;  // k extent of GEMMs over H1
+    constexpr int H2P = ((H2 + MM::KSTEP - 1) / MM::KSTEP) * MM::KSTEP;  // k extent of the GEMM over H2
+    constexpr bool TC = MM::kTransposedCopies;
+    constexpr bool SP = MM::kSplit;
+    static_assert(KP % MM::KSTEP == 0 && KP % 16 == 0, "padded input dim");
+    __shared__ __align__(16) T Xs[64 * XS];
+    __shared__ __align__(16) T W1s[H1 * XS];                 // W1[j][k]
+    __shared__ __align__(16) T W2s[H2 * HS];                 // W2[j2][j1]
+    __shared__ __align__(16) T W1t[TC ? KP * HS : 1];        // W1[j][k] stored [k][j]
+    __shared__ __align__(16) T W2t[TC ? H1 * HS : 1];        // W2[j2][j1] stored [j1][j2]
+    __shared__ __align__(16) T Hs[64 * HS];
+    __shared__ __align__(16) T Xl[SP ? 64 * XS : 1];         // low halves of the forward operands
+    __shared__ __align__(16) T W1l[SP ? H1 * XS : 1];
+    __shared__ __align__(16) T W2l[SP ? H2 * HS : 1];
+    __shared__ __align__(16) T Hl[SP ? 64 * HS : 1];
+
+    const int tid = threadIdx.x;
+    const int lane = tid & 63;
+    const int lr = lane & 15;
+    const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
+    const int S = a.S;
+    const int lk = MM::lane_k(lane);
+    const float *__restrict__ W1g = a.w_out_in;                     // [H1][S]
+    const float *__restrict__ W2g = a.w_out_in + (size_t)S * H1;    // [H2][H1]
+    const float *__restrict__ b1 = a.bias;
+    const float *__restrict__ b2 = a.bias + H1;
+    const float bo = a.bias[H1 + H2];
+    const float *__restrict__ wo = a.w_in_out + (size_t)S * H1 + (size_t)H1 * H2;
+
+    for (int e = tid; e < H1 * XS; e += 256) {
+        const int j = e / XS, k = e % XS;
+        const float w = k < S ? W1g[(size_t)j * S + k] : 0.f;
+        W1s[e] = MM::cvt(w);
+        if constexpr (SP) W1l[e] = MM::lo(w);
+    }
+    for (int e = tid; e < H2 * HS; e += 256) {
+        const int j = e / HS, k = e % HS;
+        const float w = k < H1 ? W2g[(size_t)j * H1 + k] : 0.f;
+        W2s[e] = MM::cvt(w);
+        if constexpr (SP) W2l[e] = MM::lo(w);
+    }
+    if constexpr (TC) {
+        for (int e = tid; e < KP * HS; e += 256) {
+            const int k = e / HS, j = e % HS;
+            W1t[e] = MM::cvt((k < S && j < H1) ? W1g[(size_t)j * S + k] : 0.f);
+        }
+        for (int e = tid; e < H1 * HS; e += 256) {
+            const int j1 = e / HS, j2 = e % HS;
+            W2t[e] = MM::cvt(j2 < H2 ? W2g[(size_t)j2 * H1 + j1] : 0.f);
+        }
+    }
+    const int r0 = 16 * wave;
+    T *hrow_w = Hs + r0 * HS;                         // this wave's 16 rows of H
+    const T *hrow = Hs + (r0 + lr) * HS + lk;         // A-operand pointer into H
+    for (int tile = blockIdx.x; tile * 64 < a.nrows; tile += gridDim.x) {
+        const int row0 = tile * 64;
+        const int nt = min(64, a.nrows - row0);
+        __syncthreads();
+        for (int e = tid; e < 64 * XS; e += 256) {
+            const int i = e / XS, k = e % XS;
+            const float v = (i < nt && k < S) ? a.x[(size_t)(row0 + i) * S + k] : 0.f;
+            Xs[e] = MM::cvt(v);
+            if constexpr (SP) Xl[e] = MM::lo(v);
+        }
+        __syncthreads();
+        // ---- layer 1 ----
+        f32x4 acc1[C1];
+#pragma unroll
+        for (int c = 0; c < C1; ++c) acc1[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        const T *xrow = Xs + (r0 + lr) * XS + lk;
+        for (int k0 = 0; k0 < KP; k0 += MM::KSTEP) {
+#pragma unroll
+            for (int c = 0; c < C1; ++c) {
+                const int wofs = (16 * c + lr) * XS + lk + k0;
+                acc1[c] = MM::mma(xrow + k0, W1s + wofs, acc1[c]);
+                if constexpr (SP) {
+                    const int xofs = (r0 + lr) * XS + lk + k0;
+                    acc1[c] = MM::mma(Xl + xofs, W1s + wofs, acc1[c]);
+                    acc1[c] = MM::mma(Xs + xofs, W1l + wofs, acc1[c]);
+                }
+            }
+        }
+        float d1[C1][4];
+#pragma unroll
+        for (int c = 0; c < C1; ++c) {
+            const int j = 16 * c + lr;
+            const float bj = b1[j];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const float h = silu_grad(acc1[c][r] + bj, &d1[c][r]);
+                hrow_w[(4 * (lane >> 4) + r) * HS + j] = MM::cvt(h);
+                if constexpr (SP) Hl[(r0 + 4 * (lane >> 4) + r) * HS + j] = MM::lo(h);
+            }
+        }
+        if constexpr (H1P > H1) {
+            for (int e = lane; e < 16 * (H1P - H1); e += 64) {
+                hrow_w[(e / (H1P - H1)) * HS + H1 + e % (H1P - H1)] = MM::cvt(0.f);
+                if constexpr (SP) Hl[(r0 + e / (H1P - H1)) * HS + H1 + e % (H1P - H1)] = MM::cvt(0.f);
+            }
+        }
+        wave_lds_sync();
+        // ---- layer 2 ----
+        f32x4 acc2[C2];
+#pragma unroll
+        for (int c = 0; c < C2; ++c) acc2[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < H1P; k0 += MM::KSTEP) {
+#pragma unroll
+            for (int c = 0; c < C2; ++c) {
+                const int wofs = (16 * c + lr) * HS + lk + k0;
+                acc2[c] = MM::mma(hrow + k0, W2s + wofs, acc2[c]);
+                if constexpr (SP) {
+                    const int hofs = (r0 + lr) * HS + lk + k0;
+                    acc2[c] = MM::mma(Hl + hofs, W2s + wofs, acc2[c]);
+                    acc2[c] = MM::mma(Hs + hofs, W2l + wofs, acc2[c]);
+                }
+            }
+        }
+        float gz2[C2][4];
+        float ysum[4] = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int c = 0; c < C2; ++c) {
+            const int j = 16 * c + lr;
+            const float bj = b2[j], w = wo[j];
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                float dd;
+                const float h = silu_grad(acc2[c][r] + bj, &dd);
+                ysum[r] += w * h;
+                gz2[c][r] = a.sig_e * w * dd;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            float v = ysum[r];
+            v += __shfl_xor(v, 1);
+            v += __shfl_xor(v, 2);
+            v += __shfl_xor(v, 4);
+            v += __shfl_xor(v, 8);
+            ysum[r] = v;
+        }
+        if (lr == 0) {
+#pragma unroll
+            for (int r = 0; r < 4; ++r) {
+                const int i = r0 + 4 * (lane >> 4) + r;
+                if (i < nt) a.E[a.rows_atom[row0 + i]] = (double)(ysum[r] + bo) * (double)a.sig_e + (double)a.mu_e;
+            }
+        }
+        if (!a.want_grad) continue;
+        // ---- gZ1 = (gZ2 W2) * s1'  (k = j2 over H2) ----
+        wave_lds_sync();
+#pragma unroll
+        for (int c = 0; c < C2; ++c) {
+            const int j = 16 * c + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) hrow_w[(4 * (lane >> 4) + r) * HS + j] = MM::cvt(gz2[c][r]);
+        }
+        if constexpr (H2P > H2) {
+            for (int e = lane; e < 16 * (H2P - H2); e += 64)
+                hrow_w[(e / (H2P - H2)) * HS + H2 + e % (H2P - H2)] = MM::cvt(0.f);
+        }
+        wave_lds_sync();
+        f32x4 acc3[C1];
+#pragma unroll
+        for (int c = 0; c < C1; ++c) acc3[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int k0 = 0; k0 < H2P; k0 += MM::KSTEP) {
+#pragma unroll
+            for (int c = 0; c < C1; ++c) {
+                if constexpr (TC) acc3[c] = MM::mma(hrow + k0, W2t + (16 * c + lr) * HS + lk + k0, acc3[c]);
+                else acc3[c] = MM::mma(hrow + k0, W2s + (k0 + lk) * HS + 16 * c + lr, acc3[c]);
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int c = 0; c < C1; ++c) {
+            const int j = 16 * c + lr;
+#pragma unroll
+            for (int r = 0; r < 4; ++r) hrow_w[(4 * (lane >> 4) + r) * HS + j] = MM::cvt(acc3[c][r] * d1[c][r]);
+        }
+        if constexpr (H1P > H1) {
+            for (int e = lane; e < 16 * (H1P - H1); e += 64)
+                hrow_w[(e / (H1P - H1)) * HS + H1 + e % (H1P - H1)] = MM::cvt(0.f);
+        }
+        wave_lds_sync();
+        // ---- gX = gZ1 W1 (k = j over H1) ----
+#pragma unroll
+        for (int cs = 0; cs < NS; ++cs) {
+            f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+            const int n = 16 * cs + lr;
+            for (int k0 = 0; k0 < H1P; k0 += MM::KSTEP) {
+                if constexpr (TC) acc = MM::mma(hrow + k0, W1t + n * HS + lk + k0, acc);
+                else acc = MM::mma(hrow + k0, W1s + (k0 + lk) * XS + n, acc);
+            }
+            if (n < S) {
+                const int l = n % a.L1;
+                const float scale = (2.f * kPi4m / (float)(2 * l + 1)) / a.sig[n];
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int i = r0 + 4 * (lane >> 4) + r;
+                    if (i < nt) a.G[(size_t)(row0 + i) * S + n] = acc[r] * scale;
+                }
+            }
+        }
+    }
+}
+
+// ---- dispatch ----
+template <typename T, int C1, int C2, int KP>
+static void run_mlp2(hipStream_t s, const MlpArgs &a) {
+    int dev = 0, ncu = 256;
+    (void)hipGetDevice(&dev);
+    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int tiles = (a.nrows + 63) / 64;
+    const int grid = tiles < ncu * 2 ? tiles : ncu * 2;
+    hipLaunchKernelGGL((k_mlp2_mfma<T, C1, C2, KP>), dim3(grid), dim3(256), 0, s, a);
+}
+
+template <typename T>
+static bool dispatch_mlp2(hipStream_t s, const MlpArgs &a) {
+    if (a.nlayers != 2) return false;
+    const int h1 = a.hidden[0], h2 = a.hidden[1];
+    const int kp = a.S <= 64 ? 64 : (a.S <= 128 ? 128 : (a.S <= 256 ? 256 : 0));
+#define JNN_MLP_SHAPE(H1_, H2_, KP_)                    \
+    if (h1 == H1_ && h2 == H2_ && kp == KP_) {          \
+        run_mlp2<T, H1_ / 16, H2_ / 16, KP_>(s, a);     \
+        return true;                                    \
+    }
+    JNN_MLP_SHAPE(32, 32, 64)
+    JNN_MLP_SHAPE(32, 32, 128)
+    JNN_MLP_SHAPE(64, 64, 64)
+    JNN_MLP_SHAPE(64, 64, 128)
+#undef JNN_MLP_SHAPE
+    return false;
+}
+
+bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a) {
+    if (a.nrows <= 0) return true;
+    return dispatch_mlp2<float>(s, a);
+}
+bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a) {
+    if (a.nrows <= 0) return true;
+    return dispatch_mlp2<__bf16>(s, a);
+}
+
+}  // namespace jnn

@@ -190,9 +190,7 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, CellGrid g, const 
         const size_t row = (size_t)i * nl.cap;
         for (int t = lane; t < nw; t += kNlBlock) {
             nl.key[row + t] = s_key[t];
-            nl.dx[row + t] = s_d[0][t];
-            nl.dy[row + t] = s_d[1][t];
-            nl.dz[row + t] = s_d[2][t];
+            nl.d[row + t] = make_float4(s_d[0][t], s_d[1][t], s_d[2][t], 0.f);
         }
         if (lane == 0) {
             nl.cnt[i] = hits;
@@ -222,6 +220,15 @@ __global__ void k_nl_reverse(int n, NeighborList nl) {
         if (v < want) lo = mid + 1; else hi = mid - 1;
     }
     nl.rev[(size_t)i * nl.cap + k] = found;
+}
+
+__global__ void k_fill_int(int n, int *__restrict__ p, int v) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
+}
+void launch_fill_int(hipStream_t s, int n, int *p, int v) {
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, n, p, v);
 }
 
 void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const CellGrid &g, double *xw, int *cell,

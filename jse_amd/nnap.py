@@ -50,6 +50,7 @@ EXPORTS = [
     "jse_nnap_system_set_box", "jse_nnap_system_set_positions", "jse_nnap_system_set_positions_device",
     "jse_nnap_system_compute", "jse_nnap_system_get_results", "jse_nnap_system_compute_fp",
     "jse_nnap_system_get_neighbors", "jse_nnap_system_benchmark", "jse_nnap_system_nl_stats",
+    "jse_nnap_system_get_pair_gradients",
 ]
 
 
@@ -94,6 +95,7 @@ def lib():
     L.jse_nnap_system_benchmark.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                             ctypes.c_int]
     L.jse_nnap_system_nl_stats.argtypes = [vp, _c_int_p, _c_int_p, _c_ll_p, _c_ll_p]
+    L.jse_nnap_system_get_pair_gradients.argtypes = [vp, ctypes.c_int, _c_flt_p]
     _LIB = L
     return L
 
@@ -207,6 +209,11 @@ class System:
         _check(lib().jse_nnap_system_get_neighbors(self._h, cap, _ip(cnt), _ip(key),
                                                    d.ctypes.data_as(_c_flt_p)))
         return cnt, key, d
+
+    def pair_gradients(self, cap: int):
+        g = np.zeros((self.natoms, cap, 3), dtype=np.float32)
+        _check(lib().jse_nnap_system_get_pair_gradients(self._h, int(cap), g.ctypes.data_as(_c_flt_p)))
+        return g
 
     def nl_stats(self):
         c, mx, pairs, un = ctypes.c_int(), ctypes.c_int(), ctypes.c_longlong(), ctypes.c_longlong()
