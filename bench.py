@@ -82,6 +82,44 @@ def cpu_baseline(md, st, gpu_res, budget_s=12.0):
     return base, err
 
 
+def batched_c4(nrep=512, seed=17, steps=10):
+    """Batched variant: `nrep` independent replicas (log-uniform 64-4096 atoms,
+    1-3 species, 0.05-0.065 A^-3) evaluated together with the BF16 MLP and a
+    replayed step graph, against the plain path evaluated one replica per call
+    (FP32, launches issued per call, synchronised per call)."""
+    md = W.ternary_model()
+    reps = W.replicas(nrep, 64, 4096, seed)
+    natoms = sum(r.natoms for r in reps)
+    atoms = [nnap.AtomData(r.box, r.xyz, r.types) for r in reps]
+    out = {"replicas": nrep, "atoms": natoms}
+    with nnap.NNAP(md, precision=nnap.BF16_MLP) as pot:
+        b = pot.batch(atoms)
+        ms, ph = b.benchmark(nnap.ENERGY | nnap.FORCE | nnap.VIRIAL, warmup=3, iters=steps)
+        out["batched_bf16_graph"] = {"ms_per_step": ms, "atom_evals_per_s": natoms / (ms * 1e-3)}
+        b.close()
+    with nnap.NNAP(md, precision=nnap.FP32) as pot:
+        systems = []
+        for a in atoms:
+            s_ = nnap.System(pot, a.natoms, pot._types_of(a), a.box, nnap.FP32)
+            s_.set_positions(a.xyz)
+            s_.set_graphs(False)
+            systems.append(s_)
+        for s_ in systems:          # warm-up (neighbour capacities settle)
+            s_.compute()
+            s_.results(energies=False, forces=False, virials=False)
+        t0 = time.perf_counter()
+        for s_ in systems:
+            s_.compute()
+            s_.results(energies=False, forces=False, virials=False)
+        dt = time.perf_counter() - t0
+        out["plain_fp32_one_replica_per_call"] = {"ms_per_pass": dt * 1e3, "atom_evals_per_s": natoms / dt}
+        for s_ in systems:
+            s_.close()
+    out["speedup"] = out["batched_bf16_graph"]["atom_evals_per_s"] / out["plain_fp32_one_replica_per_call"][
+        "atom_evals_per_s"]
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -90,6 +128,7 @@ def main():
     ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
     ap.add_argument("--natoms", type=int, default=32000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -148,6 +187,8 @@ def main():
         base, err = cpu_baseline(md, st, res)
         line["cpu_baseline"] = base
         line["error_vs_fp64"] = err
+    if not args.no_batched:
+        line["batched_512_replicas"] = batched_c4()
     print(json.dumps(line), flush=True)
     pot.close()
 

@@ -69,6 +69,21 @@ void jse_nnap_model_destroy(jse_nnap_model m);
 int jse_nnap_system_create(jse_nnap_model m, int natoms, const int *types, const double *box, int precision,
                            jse_nnap_system *out);
 void jse_nnap_system_destroy(jse_nnap_system s);
+
+/* A batch of independent periodic replicas evaluated together (no reference
+ * counterpart: the reference evaluates one AtomData per call; its
+ * forward-flux-sampling driver runs many replicas, ForwardFluxSampling.java:
+ * 717-873).  natoms[r] atoms of replica r follow each other in `types` and in
+ * the position array; boxes[r*3 + d] is replica r's box.  Every per-atom
+ * output keeps this concatenated order; jse_nnap_system_get_replica_results
+ * gives each replica's totals, which do not depend on the other replicas in
+ * the batch. */
+int jse_nnap_batch_create(jse_nnap_model m, int nrep, const int *natoms, const int *types, const double *boxes,
+                          int precision, jse_nnap_system *out);
+int jse_nnap_system_get_replica_results(jse_nnap_system s, double *energy, double *virial);
+/* Replays each evaluation kind from a captured HIP graph (default on); 0 launches kernel by kernel. */
+int jse_nnap_system_set_graphs(jse_nnap_system s, int enable);
+/* box of every replica, [nrep][3] */
 int jse_nnap_system_set_box(jse_nnap_system s, const double *box);
 /* positions as an [natoms][3] FP64 row-major host array (the reference's
  * getMatRow layout), copied to the device; wrapping happens on the device */

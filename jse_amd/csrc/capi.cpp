@@ -142,43 +142,64 @@ struct jse_nnap_model_s {
 
 struct jse_nnap_system_s {
     jse_nnap_model model = nullptr;
-    int n = 0;
+    int n = 0;       // atoms over all replicas
+    int nrep = 0;    // replicas (independent periodic boxes)
     int precision = JSE_NNAP_FP32;
-    double box[3] = {0, 0, 0};
     std::vector<int> types;
-    std::vector<int> seg;  // species segment offsets into rows
-    CellGrid grid{};
+    std::vector<int> rep_natoms;
+    std::vector<ReplicaGrid> reps;     // host copy of the per-replica grids
+    std::vector<int> seg;              // species segment offsets into rows
+    int ncell_total = 0;
+    double rc2 = 0;
     hipStream_t stream = nullptr;
     bool have_pos = false, have_nl = false, pending = false;
-    int *h_overflow = nullptr;  // pinned host copy of the overflow flag
-    int last_max_count = 0;
+    int *h_overflow = nullptr;         // pinned host copy of the overflow flag
+    int last_what = 0;
 
     double *d_xyz = nullptr, *d_xw = nullptr, *d_sxw = nullptr;
     int *d_cell = nullptr, *d_rank = nullptr, *d_sidx = nullptr, *d_types = nullptr, *d_rows = nullptr;
+    int *d_rep_of = nullptr;
+    ReplicaGrid *d_reps = nullptr;
     int *d_cell_cnt = nullptr, *d_cell_start = nullptr;
     int ncell_alloc = 0;
     int *d_overflow = nullptr;
     NeighborList nl{};
     float *d_x = nullptr, *d_A = nullptr, *d_G = nullptr;
     double *d_E = nullptr, *d_F = nullptr, *d_V = nullptr, *d_partial = nullptr, *d_tot = nullptr;
-    static constexpr int kPartials = 256;
+    int4 *d_chunks = nullptr;
+    int *d_rep_chunk = nullptr;
+    int nchunk = 0;
+    static constexpr int kChunk = 4096;  // atoms per reduction chunk
     hipEvent_t ev[8] = {};
+    // captured step graphs, one per evaluation kind (E / E+F / E+F+V)
+    hipGraphExec_t graph_exec[8] = {};
+    bool use_graph = true;
 
     ~jse_nnap_system_s() {
         if (stream) (void)hipStreamSynchronize(stream);
+        drop_graphs();
         free_nl();
         dfree(d_xyz); dfree(d_xw); dfree(d_sxw); dfree(d_cell); dfree(d_rank); dfree(d_sidx); dfree(d_types);
-        dfree(d_rows); dfree(d_cell_cnt); dfree(d_cell_start); dfree(d_overflow); dfree(d_x); dfree(d_A); dfree(d_G);
-        dfree(d_E); dfree(d_F); dfree(d_V); dfree(d_partial); dfree(d_tot);
+        dfree(d_rows); dfree(d_rep_of); dfree(d_reps); dfree(d_cell_cnt); dfree(d_cell_start); dfree(d_overflow);
+        dfree(d_x); dfree(d_A); dfree(d_G); dfree(d_E); dfree(d_F); dfree(d_V); dfree(d_partial); dfree(d_tot);
+        dfree(d_chunks); dfree(d_rep_chunk);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (h_overflow) (void)hipHostFree(h_overflow);
         if (stream) (void)hipStreamDestroy(stream);
     }
+    void drop_graphs() {
+        for (auto &g : graph_exec)
+            if (g) {
+                (void)hipGraphExecDestroy(g);
+                g = nullptr;
+            }
+    }
     void free_nl() {
         dfree(nl.cnt); dfree(nl.key); dfree(nl.d); dfree(nl.rev); dfree(nl.g);
     }
     void alloc_nl(int cap) {
+        drop_graphs();
         free_nl();
         nl.cap = cap;
         const size_t m = (size_t)n * cap;
@@ -189,66 +210,98 @@ struct jse_nnap_system_s {
         nl.g = dalloc<float4>(m);
     }
 
-    void set_box(const double *b) {
-        for (int d = 0; d < 3; ++d) {
+    // NeighborListGetter.getProperLinkedCell for an orthogonal box (NeighborListGetter.java:952-1001)
+    void grid_of(const double *b, ReplicaGrid &g) const {
+        for (int d = 0; d < 3; ++d)
             JNN_CHECK(std::isfinite(b[d]) && b[d] > 0, ERR_ARG, "box lengths must be positive and finite");
-            box[d] = b[d];
-        }
-        // NeighborListGetter.getProperLinkedCell for an orthogonal box (NeighborListGetter.java:952-1001)
         const double rmax = model->host.rcut_max();
-        const double minbox = std::min(box[0], std::min(box[1], box[2]));
+        const double minbox = std::min(b[0], std::min(b[1], b[2]));
         JNN_CHECK(rmax <= minbox, ERR_UNSUPPORTED,
                   "rcut larger than the smallest box length (the reference's expanded cell list) is not supported");
         const int div = (int)std::floor(minbox / rmax);
         const double cell_len = minbox / (double)div;
         for (int d = 0; d < 3; ++d) {
-            int sz = (int)std::floor(box[d] / cell_len);
-            grid.size[d] = std::max(sz, div);
-            grid.cb[d] = box[d] / grid.size[d];
-            grid.box[d] = box[d];
+            const int sz = (int)std::floor(b[d] / cell_len);
+            g.size[d] = std::max(sz, div);
+            g.cb[d] = b[d] / g.size[d];
+            g.box[d] = b[d];
         }
-        grid.ncell = grid.size[0] * grid.size[1] * grid.size[2];
-        grid.rc2 = rmax * rmax;
-        if (grid.ncell + 1 > ncell_alloc) {
+        g.ncell = g.size[0] * g.size[1] * g.size[2];
+    }
+    void set_boxes(const double *boxes) {
+        std::vector<ReplicaGrid> r(nrep);
+        int coff = 0, aoff = 0;
+        for (int q = 0; q < nrep; ++q) {
+            grid_of(boxes + 3 * q, r[q]);
+            r[q].cell_off = coff;
+            r[q].atom_off = aoff;
+            r[q].natoms = rep_natoms[q];
+            coff += r[q].ncell;
+            aoff += rep_natoms[q];
+        }
+        JNN_HIP(hipStreamSynchronize(stream));
+        reps = r;
+        ncell_total = coff;
+        if (ncell_total + 1 > ncell_alloc) {
             dfree(d_cell_cnt);
             dfree(d_cell_start);
-            ncell_alloc = grid.ncell + 1;
+            ncell_alloc = ncell_total + 1;
             d_cell_cnt = dalloc<int>(ncell_alloc);
             d_cell_start = dalloc<int>(ncell_alloc + 1);
         }
+        if (!d_reps) d_reps = dalloc<ReplicaGrid>(nrep);
+        JNN_HIP(hipMemcpy(d_reps, reps.data(), sizeof(ReplicaGrid) * nrep, hipMemcpyHostToDevice));
+        drop_graphs();
         have_nl = false;
     }
 
-    void init(jse_nnap_model m, int natoms, const int *ty, const double *b, int prec) {
+    void init(jse_nnap_model m, int nr, const int *natoms_per, const int *ty, const double *boxes, int prec) {
         model = m;
-        n = natoms;
+        nrep = nr;
         precision = prec;
-        JNN_CHECK(natoms >= 0 && natoms < (1 << 26), ERR_ARG, "natoms out of range [0, 2^26)");
+        JNN_CHECK(nr >= 1, ERR_ARG, "at least one replica is required");
         JNN_CHECK(prec == JSE_NNAP_FP32 || prec == JSE_NNAP_BF16_MLP, ERR_ARG, "unknown precision");
+        long long tot = 0;
+        for (int q = 0; q < nr; ++q) {
+            JNN_CHECK(natoms_per[q] >= 0, ERR_ARG, "negative atom count");
+            tot += natoms_per[q];
+        }
+        JNN_CHECK(tot < (1 << 26), ERR_ARG, "natoms out of range [0, 2^26)");
+        n = (int)tot;
+        rep_natoms.assign(natoms_per, natoms_per + nr);
         const int nt = m->host.ntypes();
-        types.assign(ty, ty + natoms);
-        for (int i = 0; i < natoms; ++i)
+        types.assign(ty, ty + n);
+        for (int i = 0; i < n; ++i)
             JNN_CHECK(types[i] >= 1 && types[i] <= nt, ERR_TYPE,
                       "Exist type (" + std::to_string(types[i]) + ") greater than the input ntypes (" +
                           std::to_string(nt) + ")");
         // species-grouped row order (stable), as the reference's LAMMPS path regroups by type
-        std::vector<int> rows(natoms);
+        std::vector<int> rows(n);
         std::iota(rows.begin(), rows.end(), 0);
         std::stable_sort(rows.begin(), rows.end(), [&](int a, int c) { return types[a] < types[c]; });
         seg.assign(nt + 1, 0);
-        for (int i = 0; i < natoms; ++i) seg[types[i]]++;
+        for (int i = 0; i < n; ++i) seg[types[i]]++;
         for (int t = 1; t <= nt; ++t) seg[t] += seg[t - 1];
-        for (int t = 0; t < nt; ++t) {
-            const SpeciesHost &sp = m->host.species[t];
-            if (seg[t + 1] > seg[t]) {
-                // reject shapes without compiled kernels at creation time
-                JNN_CHECK(sp.lmax <= 12, ERR_UNSUPPORTED, "lmax > 12");
-            }
+        std::vector<int> rep_of(n);
+        std::vector<int4> chunks;
+        std::vector<int> rep_chunk(nr + 1, 0);
+        for (int q = 0, a = 0; q < nr; ++q) {
+            rep_chunk[q] = (int)chunks.size();
+            for (int k = 0; k < rep_natoms[q]; ++k) rep_of[a + k] = q;
+            for (int c0 = 0; c0 < rep_natoms[q]; c0 += kChunk)
+                chunks.push_back(make_int4(q, a + c0, a + std::min(c0 + kChunk, rep_natoms[q]), 0));
+            a += rep_natoms[q];
         }
+        rep_chunk[nr] = (int)chunks.size();
+        nchunk = (int)chunks.size();
+        rc2 = m->host.rcut_max() * m->host.rcut_max();
         JNN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         for (auto &e : ev) JNN_HIP(hipEventCreate(&e));
         d_types = dupload(types);
         d_rows = dupload(rows);
+        d_rep_of = dupload(rep_of);
+        d_chunks = dupload(chunks);
+        d_rep_chunk = dupload(rep_chunk);
         d_xyz = dalloc<double>((size_t)3 * n);
         d_xw = dalloc<double>((size_t)3 * n);
         d_sxw = dalloc<double>((size_t)3 * n);
@@ -264,13 +317,16 @@ struct jse_nnap_system_s {
         d_E = dalloc<double>(n);
         d_F = dalloc<double>((size_t)3 * n);
         d_V = dalloc<double>((size_t)9 * n);
-        d_partial = dalloc<double>((size_t)kPartials * 9);
-        d_tot = dalloc<double>(10);
-        set_box(b);
-        // initial row capacity from the number density, rounded up to whole wavefronts
-        const double vol = box[0] * box[1] * box[2];
+        d_partial = dalloc<double>((size_t)std::max(nchunk, 1) * 10);
+        d_tot = dalloc<double>((size_t)10 * nr);
+        set_boxes(boxes);
+        // initial row capacity from the densest replica, rounded up to whole wavefronts
+        double rho = 0.0;
+        for (int q = 0; q < nr; ++q)
+            if (rep_natoms[q] > 0)
+                rho = std::max(rho, rep_natoms[q] / (boxes[3 * q] * boxes[3 * q + 1] * boxes[3 * q + 2]));
         const double rmax = m->host.rcut_max();
-        double est = (n > 0 ? n / vol : 0.0) * 4.18879 * rmax * rmax * rmax;
+        const double est = rho * 4.18879 * rmax * rmax * rmax;
         int cap = (int)std::ceil((est * 1.5 + 16) / 64.0) * 64;
         cap = std::max(64, std::min(cap, 512));
         alloc_nl(cap);
@@ -281,18 +337,16 @@ struct jse_nnap_system_s {
     void build_nl(bool timed) {
         JNN_CHECK(have_pos, ERR_ARG, "positions not set");
         if (timed) JNN_HIP(hipEventRecord(ev[0], stream));
-        launch_fill_int(stream, grid.ncell, d_cell_cnt, 0);
+        launch_fill_int(stream, ncell_total, d_cell_cnt, 0);
         launch_fill_int(stream, 1, d_overflow, 0);
-        launch_wrap_bin(stream, n, d_xyz, grid, d_xw, d_cell, d_cell_cnt, d_rank);
-        launch_cell_scan(stream, grid.ncell, d_cell_cnt, d_cell_start);
+        launch_wrap_bin(stream, n, d_xyz, d_rep_of, d_reps, d_xw, d_cell, d_cell_cnt, d_rank);
+        launch_cell_scan(stream, ncell_total, d_cell_cnt, d_cell_start);
         launch_cell_scatter(stream, n, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw);
-        launch_nl_build(stream, n, grid, d_xw, d_cell, d_cell_start, d_sidx, d_sxw, nl, d_overflow);
+        launch_nl_build(stream, n, d_rep_of, d_reps, rc2, d_xw, d_cell, d_cell_start, d_sidx, d_sxw, nl, d_overflow);
         if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
         launch_nl_reverse(stream, n, nl);
         if (timed) JNN_HIP(hipEventRecord(ev[2], stream));
         JNN_HIP(hipMemcpyAsync(h_overflow, d_overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
-        have_nl = true;
-        pending = true;
     }
     // Waits for the enqueued work; when a neighbour row overflowed, grows the
     // capacity and redoes the last request (neighbour build or full compute).
@@ -306,10 +360,9 @@ struct jse_nnap_system_s {
                       "an atom has " + std::to_string(ovf) + " neighbours within rcut; at most 512 are supported");
             JNN_CHECK(attempt < 4, ERR_CAPACITY, "neighbour list capacity retries exhausted");
             alloc_nl(((ovf + 16 + 63) / 64) * 64);
-            if (last_what) compute(last_what, false);
+            if (last_what) enqueue(last_what, false);
             else build_nl(false);
             JNN_HIP(hipStreamSynchronize(stream));
-            pending = false;
         }
     }
 
@@ -342,7 +395,8 @@ struct jse_nnap_system_s {
         }
     }
 
-    void compute(int what, bool timed) {
+    // the whole evaluation as a sequence of launches on `stream`
+    void enqueue(int what, bool timed) {
         build_nl(timed);
         const bool grad = (what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL)) != 0;
         desc_fwd(false);
@@ -386,12 +440,45 @@ struct jse_nnap_system_s {
         }
         if (timed) JNN_HIP(hipEventRecord(ev[5], stream));
         if (grad) launch_gather(stream, n, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0);
-        launch_reduce(stream, n, 1, 1, d_E, d_partial, kPartials, d_tot);
-        if (what & JSE_NNAP_VIRIAL) launch_reduce(stream, n, 9, 9, d_V, d_partial, kPartials, d_tot + 1);
+        launch_replica_totals(stream, nchunk, d_chunks, nrep, d_rep_chunk, d_E, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0,
+                              d_partial, d_tot);
         if (timed) JNN_HIP(hipEventRecord(ev[6], stream));
-        last_what = what;
     }
-    int last_what = 0;
+
+    // One evaluation: replays the captured step graph of this kind when there
+    // is one (capturing it on first use), or enqueues the launches directly.
+    void compute(int what, bool timed) {
+        JNN_CHECK(have_pos, ERR_ARG, "positions not set");
+        last_what = what;
+        pending = true;
+        have_nl = true;
+        if (timed || !use_graph || n == 0) {
+            enqueue(what, timed);
+            return;
+        }
+        hipGraphExec_t &ge = graph_exec[what & 7];
+        if (!ge) {
+            hipGraph_t g = nullptr;
+            JNN_HIP(hipStreamBeginCapture(stream, hipStreamCaptureModeThreadLocal));
+            try {
+                enqueue(what, false);
+            } catch (...) {
+                (void)hipStreamEndCapture(stream, &g);
+                if (g) (void)hipGraphDestroy(g);
+                throw;
+            }
+            JNN_HIP(hipStreamEndCapture(stream, &g));
+            hipError_t e = hipGraphInstantiate(&ge, g, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(g);
+            if (e != hipSuccess) {
+                ge = nullptr;
+                use_graph = false;
+                enqueue(what, false);
+                return;
+            }
+        }
+        JNN_HIP(hipGraphLaunch(ge, stream));
+    }
 };
 
 extern "C" {
@@ -494,8 +581,45 @@ int jse_nnap_system_create(jse_nnap_model m, int natoms, const int *types, const
     return guard([&] {
         JNN_CHECK(m && out && box && (types || natoms == 0), ERR_ARG, "null argument");
         auto s = std::make_unique<jse_nnap_system_s>();
-        s->init(m, natoms, types, box, precision);
+        s->init(m, 1, &natoms, types, box, precision);
         *out = s.release();
+    });
+}
+
+int jse_nnap_batch_create(jse_nnap_model m, int nrep, const int *natoms, const int *types, const double *boxes,
+                          int precision, jse_nnap_system *out) {
+    return guard([&] {
+        JNN_CHECK(m && out && natoms && boxes && nrep >= 1, ERR_ARG, "null argument");
+        auto s = std::make_unique<jse_nnap_system_s>();
+        s->init(m, nrep, natoms, types, boxes, precision);
+        *out = s.release();
+    });
+}
+
+int jse_nnap_system_get_replica_results(jse_nnap_system s, double *energy, double *virial) {
+    return guard([&] {
+        JNN_CHECK(s, ERR_ARG, "null system");
+        JNN_CHECK(s->last_what != 0, ERR_ARG, "nothing computed yet");
+        s->finish();
+        const bool vir = (s->last_what & JSE_NNAP_VIRIAL) != 0;
+        JNN_CHECK(!(virial && !vir), ERR_ARG, "virials were not computed");
+        std::vector<double> tot((size_t)10 * s->nrep);
+        JNN_HIP(hipMemcpyAsync(tot.data(), s->d_tot, sizeof(double) * tot.size(), hipMemcpyDeviceToHost, s->stream));
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        for (int q = 0; q < s->nrep; ++q) {
+            if (energy) energy[q] = tot[(size_t)10 * q];
+            if (virial)
+                for (int c = 0; c < 9; ++c) virial[(size_t)9 * q + c] = tot[(size_t)10 * q + 1 + c];
+        }
+    });
+}
+
+int jse_nnap_system_set_graphs(jse_nnap_system s, int enable) {
+    return guard([&] {
+        JNN_CHECK(s, ERR_ARG, "null system");
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        s->drop_graphs();
+        s->use_graph = enable != 0;
     });
 }
 void jse_nnap_system_destroy(jse_nnap_system s) { delete s; }
@@ -503,8 +627,7 @@ void jse_nnap_system_destroy(jse_nnap_system s) { delete s; }
 int jse_nnap_system_set_box(jse_nnap_system s, const double *box) {
     return guard([&] {
         JNN_CHECK(s && box, ERR_ARG, "null argument");
-        JNN_HIP(hipStreamSynchronize(s->stream));
-        s->set_box(box);
+        s->set_boxes(box);
     });
 }
 
@@ -547,8 +670,8 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
         const bool vir = (s->last_what & JSE_NNAP_VIRIAL) != 0;
         JNN_CHECK(!(forces && !grad), ERR_ARG, "forces were not computed");
         JNN_CHECK(!((virial || virials) && !vir), ERR_ARG, "virials were not computed");
-        double tot[10] = {0};
-        JNN_HIP(hipMemcpyAsync(tot, s->d_tot, sizeof(double) * (vir ? 10 : 1), hipMemcpyDeviceToHost, s->stream));
+        std::vector<double> tot((size_t)10 * s->nrep, 0.0);
+        JNN_HIP(hipMemcpyAsync(tot.data(), s->d_tot, sizeof(double) * tot.size(), hipMemcpyDeviceToHost, s->stream));
         if (energies && s->n)
             JNN_HIP(hipMemcpyAsync(energies, s->d_E, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
         if (forces && s->n)
@@ -556,9 +679,18 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
         if (virials && s->n)
             JNN_HIP(hipMemcpyAsync(virials, s->d_V, sizeof(double) * 9 * s->n, hipMemcpyDeviceToHost, s->stream));
         JNN_HIP(hipStreamSynchronize(s->stream));
-        if (energy) *energy = tot[0];
+        // whole-system totals: replica totals added in replica order
+        if (energy) {
+            double e = 0.0;
+            for (int q = 0; q < s->nrep; ++q) e += tot[(size_t)10 * q];
+            *energy = e;
+        }
         if (virial)
-            for (int c = 0; c < 9; ++c) virial[c] = tot[1 + c];
+            for (int c = 0; c < 9; ++c) {
+                double v = 0.0;
+                for (int q = 0; q < s->nrep; ++q) v += tot[(size_t)10 * q + 1 + c];
+                virial[c] = v;
+            }
     });
 }
 
@@ -567,8 +699,10 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
         JNN_CHECK(s && (fp || s->n == 0), ERR_ARG, "null argument");
         JNN_CHECK(smax >= s->model->smax, ERR_ARG, "smax smaller than the largest descriptor");
         s->last_what = 0;
+        s->pending = true;
         s->build_nl(false);
         s->finish();
+        s->have_nl = true;
         s->desc_fwd(true);
         std::vector<float> x((size_t)s->n * s->model->smax);
         std::vector<int> rows(s->n);
@@ -590,7 +724,12 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
 int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *key, float *d) {
     int rc = guard([&] {
         JNN_CHECK(s && count, ERR_ARG, "null argument");
-        if (!s->have_nl) s->build_nl(false);
+        if (!s->have_nl) {
+            s->last_what = 0;
+            s->pending = true;
+            s->build_nl(false);
+            s->have_nl = true;
+        }
         s->finish();
         const int n = s->n, c = s->nl.cap;
         std::vector<int> cnt(n), k((size_t)n * c);
@@ -686,7 +825,12 @@ int jse_nnap_system_benchmark(jse_nnap_system s, int what, int warmup, int iters
 int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long long *pairs, long long *unmatched) {
     return guard([&] {
         JNN_CHECK(s, ERR_ARG, "null system");
-        if (!s->have_nl) s->build_nl(false);
+        if (!s->have_nl) {
+            s->last_what = 0;
+            s->pending = true;
+            s->build_nl(false);
+            s->have_nl = true;
+        }
         s->finish();
         const int n = s->n, c = s->nl.cap;
         std::vector<int> cnt(n), rev((size_t)n * c);

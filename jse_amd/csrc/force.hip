@@ -82,53 +82,50 @@ __global__ __launch_bounds__(256) void k_gather(int n, NeighborList nl, double *
     }
 }
 
-// stage 1: each block sums a contiguous slice (fixed tree order), stage 2: one block sums the partials
 constexpr int kRedBlock = 256;
-__global__ __launch_bounds__(kRedBlock) void k_reduce_partial(int n, int stride, int ncomp, const double *__restrict__ in,
-                                                             double *__restrict__ partial) {
+constexpr int kComp = 10;  // E, V[9]
+__global__ __launch_bounds__(kRedBlock) void k_rep_partial(const int4 *__restrict__ chunks, const double *__restrict__ E,
+                                                          const double *__restrict__ V, int want_virial,
+                                                          double *__restrict__ partial) {
     __shared__ double sh[kRedBlock];
-    const int per = (n + gridDim.x - 1) / gridDim.x;
-    const int b = blockIdx.x * per, e = min(b + per, n);
-    for (int c = 0; c < ncomp; ++c) {
+    const int4 ch = chunks[blockIdx.x];  // (replica, first atom, end atom, -)
+    const int nc = want_virial ? kComp : 1;
+    for (int c = 0; c < nc; ++c) {
         Neum acc;
-        for (int i = b + threadIdx.x; i < e; i += kRedBlock) acc.add(in[(size_t)i * stride + c]);
+        for (int i = ch.y + threadIdx.x; i < ch.z; i += kRedBlock) acc.add(c == 0 ? E[i] : V[9 * (size_t)i + c - 1]);
         sh[threadIdx.x] = acc.get();
         __syncthreads();
         for (int off = kRedBlock / 2; off > 0; off >>= 1) {
             if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
             __syncthreads();
         }
-        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * ncomp + c] = sh[0];
+        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * kComp + c] = sh[0];
         __syncthreads();
     }
 }
-__global__ __launch_bounds__(kRedBlock) void k_reduce_final(int np, int ncomp, const double *__restrict__ partial,
-                                                           double *__restrict__ out) {
-    __shared__ double sh[kRedBlock];
-    for (int c = 0; c < ncomp; ++c) {
-        sh[threadIdx.x] = (threadIdx.x < np) ? partial[(size_t)threadIdx.x * ncomp + c] : 0.0;
-        __syncthreads();
-        for (int off = kRedBlock / 2; off > 0; off >>= 1) {
-            if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-            __syncthreads();
-        }
-        if (threadIdx.x == 0) out[c] = sh[0];
-        __syncthreads();
-    }
+__global__ void k_rep_final(int nrep, const int *__restrict__ rep_chunk_off, const double *__restrict__ partial,
+                            int want_virial, double *__restrict__ out) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int r = t / kComp, c = t % kComp;
+    if (r >= nrep) return;
+    if (c > 0 && !want_virial) return;
+    Neum acc;
+    for (int q = rep_chunk_off[r]; q < rep_chunk_off[r + 1]; ++q) acc.add(partial[(size_t)q * kComp + c]);
+    out[(size_t)r * kComp + c] = acc.get();
 }
 
 void launch_gather(hipStream_t s, int n, const NeighborList &nl, double *F, double *V, int want_virial) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_gather, dim3((n + 3) / 4), dim3(256), 0, s, n, nl, F, V, want_virial);
 }
-void launch_reduce(hipStream_t s, int n, int stride, int ncomp, const double *in, double *partial, int npartial,
-                   double *out) {
-    if (n <= 0) {  // empty system: the final pass over zero partials writes zeros
-        hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedBlock), 0, s, 0, ncomp, partial, out);
-        return;
-    }
-    hipLaunchKernelGGL(k_reduce_partial, dim3(npartial), dim3(kRedBlock), 0, s, n, stride, ncomp, in, partial);
-    hipLaunchKernelGGL(k_reduce_final, dim3(1), dim3(kRedBlock), 0, s, npartial, ncomp, partial, out);
+void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
+                           const double *E, const double *V, int want_virial, double *partial, double *out) {
+    if (nrep <= 0) return;
+    if (nchunk > 0)
+        hipLaunchKernelGGL(k_rep_partial, dim3(nchunk), dim3(kRedBlock), 0, s, chunks, E, V, want_virial, partial);
+    const int tot = nrep * kComp;
+    hipLaunchKernelGGL(k_rep_final, dim3((tot + 255) / 256), dim3(256), 0, s, nrep, rep_chunk_off, partial,
+                       want_virial, out);
 }
 
 }  // namespace jnn

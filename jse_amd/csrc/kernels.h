@@ -7,13 +7,18 @@
 
 namespace jnn {
 
-// Periodic cell grid of an orthogonal box (reference: NeighborListGetter.java:952-1001).
-struct CellGrid {
+// Periodic cell grid of one replica's orthogonal box (reference:
+// NeighborListGetter.java:952-1001).  A system holds one or more replicas;
+// their atoms and cells are numbered consecutively.
+struct ReplicaGrid {
+    double box[3];
+    double cb[3];   // cell edge per axis = L_d / size_d
     int size[3];
     int ncell;
-    double cb[3];   // cell edge per axis = L_d / size_d
-    double box[3];
-    double rc2;     // rmax*rmax
+    int cell_off;   // first global cell of the replica
+    int atom_off;   // first atom of the replica
+    int natoms;
+    int pad_;
 };
 
 // Neighbour-list storage of one system (device pointers), rows of `cap` slots per atom,
@@ -29,13 +34,14 @@ struct NeighborList {
 
 // ---- neighbour build ----
 void launch_fill_int(hipStream_t s, int n, int *p, int v);  // kernel fill (graph-capture safe)
-void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const CellGrid &g, double *xw, int *cell,
-                     int *cell_cnt, int *rank);
+void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const int *rep_of, const ReplicaGrid *reps,
+                     double *xw, int *cell, int *cell_cnt, int *rank);
 void launch_cell_scan(hipStream_t s, int ncell, const int *cell_cnt, int *cell_start);
 void launch_cell_scatter(hipStream_t s, int n, const int *cell, const int *rank, const int *cell_start,
                          const double *xw, int *sidx, double *sxw);
-void launch_nl_build(hipStream_t s, int n, const CellGrid &g, const double *xw, const int *cell, const int *cell_start,
-                     const int *sidx, const double *sxw, NeighborList nl, int *overflow);
+void launch_nl_build(hipStream_t s, int n, const int *rep_of, const ReplicaGrid *reps, double rc2, const double *xw,
+                     const int *cell, const int *cell_start, const int *sidx, const double *sxw, NeighborList nl,
+                     int *overflow);
 void launch_nl_reverse(hipStream_t s, int n, NeighborList nl);
 
 // ---- descriptor / MLP / backward for one species segment ----
@@ -76,8 +82,11 @@ bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a);
 
 // ---- forces / virials / totals ----
 void launch_gather(hipStream_t s, int n, const NeighborList &nl, double *F, double *V, int want_virial);
-// deterministic FP64 sums: out[c] = sum_i in[i*stride + c] for c < ncomp
-void launch_reduce(hipStream_t s, int n, int stride, int ncomp, const double *in, double *partial, int npartial,
-                   double *out);
+// Per-replica totals [nrep][10] = (E, V[9]): fixed-size chunks of each
+// replica's atoms are summed (Neumaier, then a fixed tree) and the chunk sums
+// of a replica are added in order, so a replica's totals do not depend on
+// which other replicas share the system.  chunks[c] = (replica, first, end).
+void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
+                           const double *E, const double *V, int want_virial, double *partial, double *out);
 
 }  // namespace jnn

@@ -23,10 +23,12 @@ namespace jnn {
 constexpr int kNlBlock = 64;
 constexpr int kNlLds = 512;  // candidate hits staged per atom (rows longer than this overflow)
 
-__global__ void k_wrap_bin(int n, const double *__restrict__ xyz_in, CellGrid g, double *__restrict__ xw,
-                           int *__restrict__ cell, int *__restrict__ cell_cnt, int *__restrict__ rank) {
+__global__ void k_wrap_bin(int n, const double *__restrict__ xyz_in, const int *__restrict__ rep_of,
+                           const ReplicaGrid *__restrict__ reps, double *__restrict__ xw, int *__restrict__ cell,
+                           int *__restrict__ cell_cnt, int *__restrict__ rank) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
+    const ReplicaGrid &g = reps[rep_of[i]];
     int c[3];
 #pragma unroll
     for (int d = 0; d < 3; ++d) {
@@ -38,7 +40,7 @@ __global__ void k_wrap_bin(int n, const double *__restrict__ xyz_in, CellGrid g,
         ci = ci < 0 ? 0 : (ci >= g.size[d] ? g.size[d] - 1 : ci);
         c[d] = ci;
     }
-    const int cc = c[0] + g.size[0] * (c[1] + g.size[1] * c[2]);
+    const int cc = g.cell_off + c[0] + g.size[0] * (c[1] + g.size[1] * c[2]);
     cell[i] = cc;
     rank[i] = atomicAdd(&cell_cnt[cc], 1);
 }
@@ -87,7 +89,9 @@ __device__ __forceinline__ double dist2_no_fma(double dx, double dy, double dz) 
     return (a + b) + c;
 }
 
-__global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, CellGrid g, const double *__restrict__ xw,
+__global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restrict__ rep_of,
+                                                       const ReplicaGrid *__restrict__ reps, double rc2,
+                                                       const double *__restrict__ xw,
                                                        const int *__restrict__ cell, const int *__restrict__ cstart,
                                                        const int *__restrict__ sidx, const double *__restrict__ sxw,
                                                        NeighborList nl, int *__restrict__ overflow) {
@@ -97,7 +101,8 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, CellGrid g, const 
     const int lane = threadIdx.x;
     for (int i = blockIdx.x; i < n; i += gridDim.x) {
         const double xi = xw[3 * i], yi = xw[3 * i + 1], zi = xw[3 * i + 2];
-        const int c = cell[i];
+        const ReplicaGrid &g = reps[rep_of[i]];
+        const int c = cell[i] - g.cell_off;
         const int cx = c % g.size[0], cy = (c / g.size[0]) % g.size[1], cz = c / (g.size[0] * g.size[1]);
         if (lane < 27) {
             const int w = lane;
@@ -106,7 +111,7 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, CellGrid g, const 
             if (ix >= g.size[0]) { ix -= g.size[0]; sx = 1; } else if (ix < 0) { ix += g.size[0]; sx = -1; }
             if (iy >= g.size[1]) { iy -= g.size[1]; sy = 1; } else if (iy < 0) { iy += g.size[1]; sy = -1; }
             if (iz >= g.size[2]) { iz -= g.size[2]; sz = 1; } else if (iz < 0) { iz += g.size[2]; sz = -1; }
-            const int cc = ix + g.size[0] * (iy + g.size[1] * iz);
+            const int cc = g.cell_off + ix + g.size[0] * (iy + g.size[1] * iz);
             s_start[w] = cstart[cc];
             s_cnt[w] = cstart[cc + 1] - cstart[cc];
             s_code[w] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
@@ -141,7 +146,7 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, CellGrid g, const 
                 if (sz) zj = zj + (sz > 0 ? g.box[2] : -g.box[2]);
                 const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
                 const double d2 = dist2_no_fma(dx, dy, dz);
-                hit = (d2 < g.rc2) && !(j == i && code == 13);
+                hit = (d2 < rc2) && !(j == i && code == 13);
                 key = j * 32 + code;
                 fdx = (float)dx;
                 fdy = (float)dy;
@@ -231,10 +236,11 @@ void launch_fill_int(hipStream_t s, int n, int *p, int v) {
     hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, n, p, v);
 }
 
-void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const CellGrid &g, double *xw, int *cell,
-                     int *cell_cnt, int *rank) {
+void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const int *rep_of, const ReplicaGrid *reps,
+                     double *xw, int *cell, int *cell_cnt, int *rank) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_wrap_bin, dim3((n + 255) / 256), dim3(256), 0, s, n, xyz_in, g, xw, cell, cell_cnt, rank);
+    hipLaunchKernelGGL(k_wrap_bin, dim3((n + 255) / 256), dim3(256), 0, s, n, xyz_in, rep_of, reps, xw, cell, cell_cnt,
+                       rank);
 }
 void launch_cell_scan(hipStream_t s, int ncell, const int *cell_cnt, int *cell_start) {
     hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s, ncell, cell_cnt, cell_start);
@@ -244,11 +250,13 @@ void launch_cell_scatter(hipStream_t s, int n, const int *cell, const int *rank,
     if (n <= 0) return;
     hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, cell, rank, cell_start, xw, sidx, sxw);
 }
-void launch_nl_build(hipStream_t s, int n, const CellGrid &g, const double *xw, const int *cell, const int *cell_start,
-                     const int *sidx, const double *sxw, NeighborList nl, int *overflow) {
+void launch_nl_build(hipStream_t s, int n, const int *rep_of, const ReplicaGrid *reps, double rc2, const double *xw,
+                     const int *cell, const int *cell_start, const int *sidx, const double *sxw, NeighborList nl,
+                     int *overflow) {
     if (n <= 0) return;
     const int grid = n < 65536 * 4 ? n : 65536 * 4;
-    hipLaunchKernelGGL(k_nl_build, dim3(grid), dim3(kNlBlock), 0, s, n, g, xw, cell, cell_start, sidx, sxw, nl, overflow);
+    hipLaunchKernelGGL(k_nl_build, dim3(grid), dim3(kNlBlock), 0, s, n, rep_of, reps, rc2, xw, cell, cell_start, sidx,
+                       sxw, nl, overflow);
 }
 void launch_nl_reverse(hipStream_t s, int n, NeighborList nl) {
     if (n <= 0) return;

@@ -50,7 +50,8 @@ EXPORTS = [
     "jse_nnap_system_set_box", "jse_nnap_system_set_positions", "jse_nnap_system_set_positions_device",
     "jse_nnap_system_compute", "jse_nnap_system_get_results", "jse_nnap_system_compute_fp",
     "jse_nnap_system_get_neighbors", "jse_nnap_system_benchmark", "jse_nnap_system_nl_stats",
-    "jse_nnap_system_get_pair_gradients",
+    "jse_nnap_system_get_pair_gradients", "jse_nnap_batch_create", "jse_nnap_system_get_replica_results",
+    "jse_nnap_system_set_graphs",
 ]
 
 
@@ -96,6 +97,10 @@ def lib():
                                             ctypes.c_int]
     L.jse_nnap_system_nl_stats.argtypes = [vp, _c_int_p, _c_int_p, _c_ll_p, _c_ll_p]
     L.jse_nnap_system_get_pair_gradients.argtypes = [vp, ctypes.c_int, _c_flt_p]
+    L.jse_nnap_batch_create.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_dbl_p, ctypes.c_int,
+                                        ctypes.POINTER(vp)]
+    L.jse_nnap_system_get_replica_results.argtypes = [vp, _c_dbl_p, _c_dbl_p]
+    L.jse_nnap_system_set_graphs.argtypes = [vp, ctypes.c_int]
     _LIB = L
     return L
 
@@ -132,9 +137,11 @@ class AtomData:
 
 class System:
     """A device-resident configuration bound to one model: positions,
-    neighbour list and outputs stay on the GPU between evaluations."""
+    neighbour list and outputs stay on the GPU between evaluations.  With
+    ``natoms_per_replica`` it holds a batch of independent periodic replicas
+    (``box`` then has one row per replica) evaluated in the same launches."""
 
-    def __init__(self, nnap: "NNAP", natoms: int, types, box, precision: int = FP32):
+    def __init__(self, nnap: "NNAP", natoms: int, types, box, precision: int = FP32, natoms_per_replica=None):
         self.nnap = nnap
         self.natoms = int(natoms)
         self.types = np.ascontiguousarray(types, dtype=np.int32)
@@ -142,8 +149,18 @@ class System:
             raise ValueError("types must have one entry per atom")
         self.box = np.ascontiguousarray(box, dtype=np.float64)
         h = ctypes.c_void_p()
-        _check(lib().jse_nnap_system_create(nnap._h, self.natoms, _ip(self.types), _dp(self.box), int(precision),
-                                            ctypes.byref(h)))
+        if natoms_per_replica is None:
+            self.nrep = 1
+            _check(lib().jse_nnap_system_create(nnap._h, self.natoms, _ip(self.types), _dp(self.box),
+                                                int(precision), ctypes.byref(h)))
+        else:
+            per = np.ascontiguousarray(natoms_per_replica, dtype=np.int32)
+            if int(per.sum()) != self.natoms or self.box.shape != (len(per), 3):
+                raise ValueError("replica sizes / boxes do not match")
+            self.nrep = len(per)
+            self.rep_natoms = per
+            _check(lib().jse_nnap_batch_create(nnap._h, len(per), _ip(per), _ip(self.types), _dp(self.box),
+                                               int(precision), ctypes.byref(h)))
         self._h = h
 
     def close(self):
@@ -160,6 +177,10 @@ class System:
     def set_box(self, box):
         self.box = np.ascontiguousarray(box, dtype=np.float64)
         _check(lib().jse_nnap_system_set_box(self._h, _dp(self.box)))
+
+    def set_positions_device(self, ptr: int):
+        """positions from a device pointer (e.g. a torch tensor's data_ptr())"""
+        _check(lib().jse_nnap_system_set_positions_device(self._h, ctypes.c_void_p(ptr)))
 
     def set_positions(self, xyz):
         xyz = np.ascontiguousarray(xyz, dtype=np.float64)
@@ -187,6 +208,15 @@ class System:
             out["virial"] = v
             out["virials"] = V
         return out
+
+    def replica_results(self, virials=True):
+        e = np.zeros(self.nrep)
+        v = np.zeros((self.nrep, 9)) if virials else None
+        _check(lib().jse_nnap_system_get_replica_results(self._h, _dp(e), _dp(v)))
+        return (e, v) if virials else e
+
+    def set_graphs(self, enable: bool):
+        _check(lib().jse_nnap_system_set_graphs(self._h, 1 if enable else 0))
 
     def descriptors(self) -> np.ndarray:
         smax = max(s.S for s in self.nnap.model.species)
@@ -336,6 +366,15 @@ class NNAP:
         v = self.calEnergyForceVirials(atoms, energies=False, forces=False, virials=True)["virial"]
         vol = atoms.volume()
         return [-v[0] / vol, -v[1] / vol, -v[2] / vol, -v[3] / vol, -v[4] / vol, -v[5] / vol]
+
+    def batch(self, structures: Sequence[AtomData]) -> System:
+        """A device-resident batch of independent replicas (positions set)."""
+        per = [a.natoms for a in structures]
+        types = np.concatenate([self._types_of(a) for a in structures]) if structures else np.zeros(0, np.int32)
+        boxes = np.array([np.asarray(a.box, dtype=float) for a in structures])
+        b = System(self, int(sum(per)), types, boxes, self.precision, natoms_per_replica=per)
+        b.set_positions(np.concatenate([np.asarray(a.xyz, dtype=float) for a in structures]))
+        return b
 
     def calFp(self, atoms: AtomData) -> np.ndarray:
         """descriptors of every atom (jse_nnap_calFp, nnap_interface.cpp:11-24)"""
