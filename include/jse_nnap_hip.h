@@ -136,6 +136,52 @@ int jse_nnap_system_benchmark(jse_nnap_system s, int what, int warmup, int iters
  * largest row and the number of pairs without a reverse partner. */
 int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long long *pairs, long long *unmatched);
 
+/* ---- server: device-resident replicas, continuous batching ----------------
+ * No reference counterpart as one call: the reference evaluates one AtomData
+ * per call, and its forward-flux-sampling driver and LAMMPS generators step
+ * many independent replicas (ForwardFluxSampling.java:717-873,
+ * lmp/NativeLmpFullPathGenerator.java:40-188), each step an energy/force
+ * evaluation (NNAP.java:531-594).  A server keeps every replica's positions,
+ * Verlet candidate list (rcut + skin) and outputs on the device between
+ * requests, admits replicas into an arena sized from an HBM budget (evicting
+ * idle ones, least recently used first), and evaluates pending requests in
+ * batches formed as lanes free up, two batches in flight on their own streams.
+ * A replica's candidate list is rebuilt when an atom has moved more than
+ * skin/2 since the last build, and on admission. */
+typedef struct jse_nnap_server_s *jse_nnap_server;
+
+/* hbm_budget_bytes <= 0 takes half of the free device memory; max_density
+ * (atoms/A^3) sizes the neighbour rows (they grow, with a full re-admission,
+ * if exceeded). */
+int jse_nnap_server_create(jse_nnap_model model, int precision, double skin, double hbm_budget_bytes,
+                           int max_batch_atoms, int max_replicas, double max_density, jse_nnap_server *out);
+/* Registers a replica (host-side only; fixed types and orthogonal box). */
+int jse_nnap_server_add_replica(jse_nnap_server s, int natoms, const int *types, const double *box,
+                                int *replica_id);
+/* Queues an evaluation of replica `replica_id` at positions xyz[natoms][3]
+ * (copied before return).  On completion the total energy goes to *energy,
+ * forces[natoms][3] (if what & JSE_NNAP_FORCE) and the total virial[9] (if
+ * what & JSE_NNAP_VIRIAL) are written; these buffers must stay valid until
+ * the request is waited for or collected.  Requests of one replica run in
+ * submission order. */
+int jse_nnap_server_submit(jse_nnap_server s, int replica_id, const double *xyz, int what, double *energy,
+                           double *forces, double *virial, long long *ticket);
+/* Blocks until the request is done; returns its status and its latency
+ * (submission to completion, seconds).  Releases the ticket. */
+int jse_nnap_server_wait(jse_nnap_server s, long long ticket, double *latency_s);
+/* Releases up to `cap` completed requests: tickets, latencies and statuses. */
+int jse_nnap_server_collect(jse_nnap_server s, long long *tickets, double *latency_s, int *status, int cap,
+                            int *count);
+/* Blocks until no request is pending or running. */
+int jse_nnap_server_drain(jse_nnap_server s);
+/* out[0..n): completed requests, completed atom-evaluations, batches, mean
+ * batch atoms, candidate-list rebuilds, admissions, evictions, capacity
+ * growths, resident replicas, resident atoms, arena slots, arena bytes,
+ * candidate row capacity, row capacity, pending requests, device bytes. */
+int jse_nnap_server_stats(jse_nnap_server s, double *out, int n);
+/* Finishes running batches, fails queued requests, frees the server. */
+void jse_nnap_server_destroy(jse_nnap_server s);
+
 #ifdef __cplusplus
 }
 #endif

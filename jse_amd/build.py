@@ -18,7 +18,8 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("JSE_AMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["nl.hip", "desc.hip", "mlp.hip", "mlp_mfma.hip", "force.hip", "capi.cpp", "model_host.cpp"]
+HIP_SOURCES = ["nl.hip", "desc.hip", "mlp.hip", "mlp_mfma.hip", "force.hip", "serve.hip", "capi.cpp", "server.cpp",
+               "model_host.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
     # packed FP32 VALU ops are kept out of every kernel
@@ -48,7 +49,7 @@ def build(verbose: bool = False) -> None:
     objs = [o for _, o in jobs]
     out = os.path.join(HERE, "libjsenn_hip.so")
     tmp = out + ".tmp"
-    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs)
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-pthread", "-o", tmp] + objs)
     os.replace(tmp, out)
     wl = os.path.join(HERE, "libjse_workload.so")
     _run(["gcc", "-O2", "-fPIC", "-shared", "-o", wl + ".tmp", os.path.join(CSRC, "workload.c"), "-lm"])

@@ -13,52 +13,17 @@
 #include <numeric>
 #include <sstream>
 
+#include "capi_internal.h"
 #include "common.h"
 #include "kernels.h"
 
 using namespace jnn;
 
-namespace {
+namespace jnn {
 thread_local std::string g_last_error;
-
-template <class F>
-int guard(F &&f) {
-    try {
-        f();
-        g_last_error.clear();
-        return OK;
-    } catch (const Error &e) {
-        g_last_error = e.what();
-        return e.code;
-    } catch (const std::bad_alloc &) {
-        g_last_error = "host memory exhausted";
-        return ERR_MEMORY;
-    } catch (const std::exception &e) {
-        g_last_error = e.what();
-        return ERR_ARG;
-    }
 }
 
-template <class T>
-T *dalloc(size_t n) {
-    if (n == 0) n = 1;
-    void *p = nullptr;
-    hipError_t e = hipMalloc(&p, n * sizeof(T));
-    if (e != hipSuccess) throw Error(ERR_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
-    return (T *)p;
-}
-template <class T>
-void dfree(T *&p) {
-    if (p) (void)hipFree((void *)p);
-    p = nullptr;
-}
-template <class T>
-T *dupload(const std::vector<T> &v) {
-    T *p = dalloc<T>(v.size());
-    if (!v.empty()) JNN_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return p;
-}
-
+namespace {
 uint16_t to_bf16(float f) {  // round to nearest even
     uint32_t u;
     std::memcpy(&u, &f, 4);
@@ -67,78 +32,96 @@ uint16_t to_bf16(float f) {  // round to nearest even
     return (uint16_t)(u >> 16);
 }
 
-struct SpeciesDev {
-    float *W = nullptr, *mu = nullptr, *sig = nullptr;
-    double *mu64 = nullptr, *sig64 = nullptr;
-    float *w_fwd = nullptr, *w_bwd = nullptr, *bias = nullptr;
-    uint16_t *wb_fwd = nullptr, *wb_bwd = nullptr;
-};
 }  // namespace
 
-struct jse_nnap_model_s {
-    ModelHost host;
-    std::vector<SpeciesDev> dev;
-    float *zeros = nullptr, *ones = nullptr;  // identity normalisation for raw descriptors
-    double *zeros64 = nullptr, *ones64 = nullptr;
-    int smax = 0, pmmax = 0;
+void jse_nnap_model_s::upload() {
+    int nt = host.ntypes();
+    dev.resize(nt);
+    for (int t = 0; t < nt; ++t) {
+        const SpeciesHost &sp = host.species[t];
+        SpeciesDev &d = dev[t];
+        std::vector<float> W(sp.W.begin(), sp.W.end());
+        d.W = dupload(W);
+        const int S = sp.S();
+        std::vector<float> mu(sp.norm.begin() + 2, sp.norm.begin() + 2 + S);
+        std::vector<float> sg(sp.norm.begin() + 2 + S, sp.norm.end());
+        d.mu = dupload(mu);
+        d.sig = dupload(sg);
+        d.mu64 = dupload(std::vector<double>(sp.norm.begin() + 2, sp.norm.begin() + 2 + S));
+        d.sig64 = dupload(std::vector<double>(sp.norm.begin() + 2 + S, sp.norm.end()));
+        // forward layout: per layer [in][out], then output W; backward layout: per layer [out][in]
+        std::vector<float> wf, wb, bias;
+        size_t off = 0;
+        int in = S;
+        for (int h : sp.hidden) {
+            for (int i = 0; i < in; ++i)
+                for (int o = 0; o < h; ++o) wf.push_back((float)sp.nn[off + (size_t)o * in + i]);
+            for (size_t q = 0; q < (size_t)in * h; ++q) wb.push_back((float)sp.nn[off + q]);
+            off += (size_t)in * h;
+            in = h;
+        }
+        for (int i = 0; i < in; ++i) wf.push_back((float)sp.nn[off + i]);
+        off += in;
+        for (size_t q = off; q < sp.nn.size(); ++q) bias.push_back((float)sp.nn[q]);
+        d.w_fwd = dupload(wf);
+        d.w_bwd = dupload(wb);
+        d.bias = dupload(bias);
+        std::vector<uint16_t> bf(wf.size()), bb(wb.size());
+        for (size_t q = 0; q < wf.size(); ++q) bf[q] = to_bf16(wf[q]);
+        for (size_t q = 0; q < wb.size(); ++q) bb[q] = to_bf16(wb[q]);
+        d.wb_fwd = dupload(bf);
+        d.wb_bwd = dupload(bb);
+        smax = std::max(smax, S);
+        pmmax = std::max(pmmax, sp.P * sp.M());
+    }
+    std::vector<float> z((size_t)smax, 0.f), o((size_t)smax, 1.f);
+    zeros = dupload(z);
+    ones = dupload(o);
+    zeros64 = dupload(std::vector<double>((size_t)smax, 0.0));
+    ones64 = dupload(std::vector<double>((size_t)smax, 1.0));
+}
 
-    void upload() {
-        int nt = host.ntypes();
-        dev.resize(nt);
-        for (int t = 0; t < nt; ++t) {
-            const SpeciesHost &sp = host.species[t];
-            SpeciesDev &d = dev[t];
-            std::vector<float> W(sp.W.begin(), sp.W.end());
-            d.W = dupload(W);
-            const int S = sp.S();
-            std::vector<float> mu(sp.norm.begin() + 2, sp.norm.begin() + 2 + S);
-            std::vector<float> sg(sp.norm.begin() + 2 + S, sp.norm.end());
-            d.mu = dupload(mu);
-            d.sig = dupload(sg);
-            d.mu64 = dupload(std::vector<double>(sp.norm.begin() + 2, sp.norm.begin() + 2 + S));
-            d.sig64 = dupload(std::vector<double>(sp.norm.begin() + 2 + S, sp.norm.end()));
-            // forward layout: per layer [in][out], then output W; backward layout: per layer [out][in]
-            std::vector<float> wf, wb, bias;
-            size_t off = 0;
-            int in = S;
-            for (int h : sp.hidden) {
-                for (int i = 0; i < in; ++i)
-                    for (int o = 0; o < h; ++o) wf.push_back((float)sp.nn[off + (size_t)o * in + i]);
-                for (size_t q = 0; q < (size_t)in * h; ++q) wb.push_back((float)sp.nn[off + q]);
-                off += (size_t)in * h;
-                in = h;
-            }
-            for (int i = 0; i < in; ++i) wf.push_back((float)sp.nn[off + i]);
-            off += in;
-            for (size_t q = off; q < sp.nn.size(); ++q) bias.push_back((float)sp.nn[q]);
-            d.w_fwd = dupload(wf);
-            d.w_bwd = dupload(wb);
-            d.bias = dupload(bias);
-            std::vector<uint16_t> bf(wf.size()), bb(wb.size());
-            for (size_t q = 0; q < wf.size(); ++q) bf[q] = to_bf16(wf[q]);
-            for (size_t q = 0; q < wb.size(); ++q) bb[q] = to_bf16(wb[q]);
-            d.wb_fwd = dupload(bf);
-            d.wb_bwd = dupload(bb);
-            smax = std::max(smax, S);
-            pmmax = std::max(pmmax, sp.P * sp.M());
-        }
-        std::vector<float> z((size_t)smax, 0.f), o((size_t)smax, 1.f);
-        zeros = dupload(z);
-        ones = dupload(o);
-        zeros64 = dupload(std::vector<double>((size_t)smax, 0.0));
-        ones64 = dupload(std::vector<double>((size_t)smax, 1.0));
+jse_nnap_model_s::~jse_nnap_model_s() {
+    for (auto &d : dev) {
+        dfree(d.W); dfree(d.mu); dfree(d.sig); dfree(d.mu64); dfree(d.sig64); dfree(d.w_fwd); dfree(d.w_bwd); dfree(d.bias);
+        dfree(d.wb_fwd); dfree(d.wb_bwd);
     }
-    ~jse_nnap_model_s() {
-        for (auto &d : dev) {
-            dfree(d.W); dfree(d.mu); dfree(d.sig); dfree(d.mu64); dfree(d.sig64); dfree(d.w_fwd); dfree(d.w_bwd); dfree(d.bias);
-            dfree(d.wb_fwd); dfree(d.wb_bwd);
-        }
-        dfree(zeros);
-        dfree(ones);
-        dfree(zeros64);
-        dfree(ones64);
+    dfree(zeros);
+    dfree(ones);
+    dfree(zeros64);
+    dfree(ones64);
+}
+
+namespace jnn {
+void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision, int nrows, const int *rows_atom,
+                 const float *x, float *G, double *E, bool grad) {
+    const SpeciesHost &sp = m->host.species[t];
+    const SpeciesDev &dv = m->dev[t];
+    MlpArgs a{};
+    a.nrows = nrows;
+    a.S = sp.S();
+    a.L1 = sp.lmax + 1;
+    a.nlayers = (int)sp.hidden.size();
+    for (int l = 0; l < a.nlayers; ++l) a.hidden[l] = sp.hidden[l];
+    a.rows_atom = rows_atom;
+    a.x = x;
+    a.w_in_out = dv.w_fwd;
+    a.w_out_in = dv.w_bwd;
+    a.bias = dv.bias;
+    a.sig = dv.sig;
+    a.sig_e = (float)sp.norm[1];
+    a.mu_e = (float)sp.norm[0];
+    a.E = E;
+    a.G = G;
+    a.want_grad = grad;
+    if (precision == JSE_NNAP_BF16_MLP) {
+        if (!launch_mlp_mfma_bf16(s, a))
+            throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
+    } else if (!launch_mlp_mfma_fp32(s, a)) {
+        launch_mlp_fp32(s, a);
     }
-};
+}
+}  // namespace jnn
 
 struct jse_nnap_system_s {
     jse_nnap_model model = nullptr;
@@ -169,7 +152,7 @@ struct jse_nnap_system_s {
     int4 *d_chunks = nullptr;
     int *d_rep_chunk = nullptr;
     int nchunk = 0;
-    static constexpr int kChunk = 4096;  // atoms per reduction chunk
+    static constexpr int kChunk = kReduceChunk;
     hipEvent_t ev[8] = {};
     // captured step graphs, one per evaluation kind (E / E+F / E+F+V)
     hipGraphExec_t graph_exec[8] = {};
@@ -210,24 +193,7 @@ struct jse_nnap_system_s {
         nl.g = dalloc<float4>(m);
     }
 
-    // NeighborListGetter.getProperLinkedCell for an orthogonal box (NeighborListGetter.java:952-1001)
-    void grid_of(const double *b, ReplicaGrid &g) const {
-        for (int d = 0; d < 3; ++d)
-            JNN_CHECK(std::isfinite(b[d]) && b[d] > 0, ERR_ARG, "box lengths must be positive and finite");
-        const double rmax = model->host.rcut_max();
-        const double minbox = std::min(b[0], std::min(b[1], b[2]));
-        JNN_CHECK(rmax <= minbox, ERR_UNSUPPORTED,
-                  "rcut larger than the smallest box length (the reference's expanded cell list) is not supported");
-        const int div = (int)std::floor(minbox / rmax);
-        const double cell_len = minbox / (double)div;
-        for (int d = 0; d < 3; ++d) {
-            const int sz = (int)std::floor(b[d] / cell_len);
-            g.size[d] = std::max(sz, div);
-            g.cb[d] = b[d] / g.size[d];
-            g.box[d] = b[d];
-        }
-        g.ncell = g.size[0] * g.size[1] * g.size[2];
-    }
+    void grid_of(const double *b, ReplicaGrid &g) const { make_grid(b, model->host.rcut_max(), g); }
     void set_boxes(const double *boxes) {
         std::vector<ReplicaGrid> r(nrep);
         int coff = 0, aoff = 0;
@@ -236,7 +202,7 @@ struct jse_nnap_system_s {
             r[q].cell_off = coff;
             r[q].atom_off = aoff;
             r[q].natoms = rep_natoms[q];
-            coff += r[q].ncell;
+            coff += r[q].ncell + 1;   // ncell starts + the end entry
             aoff += rep_natoms[q];
         }
         JNN_HIP(hipStreamSynchronize(stream));
@@ -339,12 +305,13 @@ struct jse_nnap_system_s {
         if (timed) JNN_HIP(hipEventRecord(ev[0], stream));
         launch_fill_int(stream, ncell_total, d_cell_cnt, 0);
         launch_fill_int(stream, 1, d_overflow, 0);
-        launch_wrap_bin(stream, n, d_xyz, d_rep_of, d_reps, d_xw, d_cell, d_cell_cnt, d_rank);
-        launch_cell_scan(stream, ncell_total, d_cell_cnt, d_cell_start);
-        launch_cell_scatter(stream, n, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw);
-        launch_nl_build(stream, n, d_rep_of, d_reps, rc2, d_xw, d_cell, d_cell_start, d_sidx, d_sxw, nl, d_overflow);
+        launch_wrap_bin(stream, n, nullptr, d_xyz, d_rep_of, d_reps, nullptr, d_xw, d_cell, d_cell_cnt, d_rank);
+        launch_cell_scan(stream, nrep, nullptr, d_reps, nullptr, d_cell_cnt, d_cell_start);
+        launch_cell_scatter(stream, n, nullptr, d_rep_of, nullptr, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw);
+        launch_nl_build(stream, n, nullptr, d_rep_of, d_reps, nullptr, rc2, d_xw, d_cell, d_cell_start, d_sidx, d_sxw,
+                        nl, d_overflow);
         if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
-        launch_nl_reverse(stream, n, nl);
+        launch_nl_reverse(stream, n, nullptr, nl);
         if (timed) JNN_HIP(hipEventRecord(ev[2], stream));
         JNN_HIP(hipMemcpyAsync(h_overflow, d_overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
     }
@@ -402,32 +369,8 @@ struct jse_nnap_system_s {
         desc_fwd(false);
         if (timed) JNN_HIP(hipEventRecord(ev[3], stream));
         for (int t = 0; t < model->host.ntypes(); ++t) {
-            const SpeciesHost &sp = model->host.species[t];
-            const SpeciesDev &dv = model->dev[t];
             SegmentArgs sa = seg_args(t, false);
-            MlpArgs a{};
-            a.nrows = sa.nrows;
-            a.S = sp.S();
-            a.L1 = sp.lmax + 1;
-            a.nlayers = (int)sp.hidden.size();
-            for (int l = 0; l < a.nlayers; ++l) a.hidden[l] = sp.hidden[l];
-            a.rows_atom = sa.rows_atom;
-            a.x = sa.x;
-            a.w_in_out = dv.w_fwd;
-            a.w_out_in = dv.w_bwd;
-            a.bias = dv.bias;
-            a.sig = dv.sig;
-            a.sig_e = (float)sp.norm[1];
-            a.mu_e = (float)sp.norm[0];
-            a.E = d_E;
-            a.G = sa.G;
-            a.want_grad = grad;
-            if (precision == JSE_NNAP_BF16_MLP) {
-                if (!launch_mlp_mfma_bf16(stream, a))
-                    throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
-            } else if (!launch_mlp_mfma_fp32(stream, a)) {
-                launch_mlp_fp32(stream, a);
-            }
+            enqueue_mlp(stream, model, t, precision, sa.nrows, sa.rows_atom, sa.x, sa.G, d_E, grad);
         }
         if (timed) JNN_HIP(hipEventRecord(ev[4], stream));
         if (grad) {
@@ -439,7 +382,7 @@ struct jse_nnap_system_s {
             }
         }
         if (timed) JNN_HIP(hipEventRecord(ev[5], stream));
-        if (grad) launch_gather(stream, n, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0);
+        if (grad) launch_gather(stream, n, nullptr, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0);
         launch_replica_totals(stream, nchunk, d_chunks, nrep, d_rep_chunk, d_E, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0,
                               d_partial, d_tot);
         if (timed) JNN_HIP(hipEventRecord(ev[6], stream));

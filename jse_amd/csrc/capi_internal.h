@@ -1,0 +1,123 @@
+// Internals shared by the C-ABI translation units of libjsenn_hip: error
+// capture, device allocation helpers and the device-resident model.
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include <algorithm>
+#include <cmath>
+
+#include "../../include/jse_nnap_hip.h"
+#include "common.h"
+#include "kernels.h"
+
+namespace jnn {
+
+extern thread_local std::string g_last_error;
+
+// Runs f, mapping exceptions to a status code and the thread's last error.
+template <class F>
+int guard(F &&f) {
+    try {
+        f();
+        g_last_error.clear();
+        return OK;
+    } catch (const Error &e) {
+        g_last_error = e.what();
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        g_last_error = "host memory exhausted";
+        return ERR_MEMORY;
+    } catch (const std::exception &e) {
+        g_last_error = e.what();
+        return ERR_ARG;
+    }
+}
+
+template <class T>
+T *dalloc(size_t n) {
+    if (n == 0) n = 1;
+    void *p = nullptr;
+    hipError_t e = hipMalloc(&p, n * sizeof(T));
+    if (e != hipSuccess) throw Error(ERR_MEMORY, std::string("hipMalloc failed: ") + hipGetErrorString(e));
+    return (T *)p;
+}
+template <class T>
+void dfree(T *&p) {
+    if (p) (void)hipFree((void *)p);
+    p = nullptr;
+}
+template <class T>
+T *dupload(const std::vector<T> &v) {
+    T *p = dalloc<T>(v.size());
+    if (!v.empty()) JNN_HIP(hipMemcpy(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return p;
+}
+template <class T>
+T *halloc(size_t n) {  // pinned host memory
+    if (n == 0) n = 1;
+    void *p = nullptr;
+    hipError_t e = hipHostMalloc(&p, n * sizeof(T), hipHostMallocDefault);
+    if (e != hipSuccess) throw Error(ERR_MEMORY, std::string("hipHostMalloc failed: ") + hipGetErrorString(e));
+    return (T *)p;
+}
+template <class T>
+void hfree(T *&p) {
+    if (p) (void)hipHostFree((void *)p);
+    p = nullptr;
+}
+
+// atoms per chunk of the per-replica total reductions: a replica's totals are
+// summed over the same chunks whatever else is evaluated with it
+constexpr int kReduceChunk = 4096;
+
+// Cell grid of an orthogonal periodic box for cut-off `rcut`
+// (NeighborListGetter.getProperLinkedCell, NeighborListGetter.java:952-1001):
+// div = floor(min L / rcut) cells across the smallest edge, cell edge
+// min L / div, and per axis max(floor(L_d / edge), div) cells.
+inline void make_grid(const double *b, double rcut, ReplicaGrid &g) {
+    for (int d = 0; d < 3; ++d)
+        JNN_CHECK(std::isfinite(b[d]) && b[d] > 0, ERR_ARG, "box lengths must be positive and finite");
+    const double minbox = std::min(b[0], std::min(b[1], b[2]));
+    JNN_CHECK(rcut <= minbox, ERR_UNSUPPORTED,
+              "rcut larger than the smallest box length (the reference's expanded cell list) is not supported");
+    const int div = (int)std::floor(minbox / rcut);
+    const double cell_len = minbox / (double)div;
+    for (int d = 0; d < 3; ++d) {
+        const int sz = (int)std::floor(b[d] / cell_len);
+        g.size[d] = std::max(sz, div);
+        g.cb[d] = b[d] / g.size[d];
+        g.box[d] = b[d];
+    }
+    g.ncell = g.size[0] * g.size[1] * g.size[2];
+}
+
+struct SpeciesDev {
+    float *W = nullptr, *mu = nullptr, *sig = nullptr;
+    double *mu64 = nullptr, *sig64 = nullptr;
+    float *w_fwd = nullptr, *w_bwd = nullptr, *bias = nullptr;
+    uint16_t *wb_fwd = nullptr, *wb_bwd = nullptr;
+};
+
+}  // namespace jnn
+
+struct jse_nnap_model_s {
+    jnn::ModelHost host;
+    std::vector<jnn::SpeciesDev> dev;
+    float *zeros = nullptr, *ones = nullptr;  // identity normalisation for raw descriptors
+    double *zeros64 = nullptr, *ones64 = nullptr;
+    int smax = 0, pmmax = 0;
+
+    void upload();
+    ~jse_nnap_model_s();
+};
+
+namespace jnn {
+// Launches the MLP of species t over `nrows` rows (FP32 MFMA, generic FP32, or BF16 MFMA).
+void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision, int nrows, const int *rows_atom,
+                 const float *x, float *G, double *E, bool grad);
+}  // namespace jnn

@@ -33,11 +33,12 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_gather(int n, NeighborList nl, double *__restrict__ F, double *__restrict__ V,
-                                                int want_virial) {
+__global__ __launch_bounds__(256) void k_gather(int n, const int *__restrict__ atoms, NeighborList nl,
+                                                double *__restrict__ F, double *__restrict__ V, int want_virial) {
     const int lane = threadIdx.x & 63;
-    const int i = blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (i >= n) return;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (t >= n) return;
+    const int i = atoms ? atoms[t] : t;
     const int ci = min(nl.cnt[i], nl.cap);
     const size_t base = (size_t)i * nl.cap;
     double f[3] = {0.0, 0.0, 0.0};
@@ -114,9 +115,10 @@ __global__ void k_rep_final(int nrep, const int *__restrict__ rep_chunk_off, con
     out[(size_t)r * kComp + c] = acc.get();
 }
 
-void launch_gather(hipStream_t s, int n, const NeighborList &nl, double *F, double *V, int want_virial) {
+void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
+                   int want_virial) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_gather, dim3((n + 3) / 4), dim3(256), 0, s, n, nl, F, V, want_virial);
+    hipLaunchKernelGGL(k_gather, dim3((n + 3) / 4), dim3(256), 0, s, n, atoms, nl, F, V, want_virial);
 }
 void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
                            const double *E, const double *V, int want_virial, double *partial, double *out) {

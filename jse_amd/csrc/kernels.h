@@ -34,15 +34,21 @@ struct NeighborList {
 
 // ---- neighbour build ----
 void launch_fill_int(hipStream_t s, int n, int *p, int v);  // kernel fill (graph-capture safe)
-void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const int *rep_of, const ReplicaGrid *reps,
-                     double *xw, int *cell, int *cell_cnt, int *rank);
-void launch_cell_scan(hipStream_t s, int ncell, const int *cell_cnt, int *cell_start);
-void launch_cell_scatter(hipStream_t s, int n, const int *cell, const int *rank, const int *cell_start,
-                         const double *xw, int *sidx, double *sxw);
-void launch_nl_build(hipStream_t s, int n, const int *rep_of, const ReplicaGrid *reps, double rc2, const double *xw,
-                     const int *cell, const int *cell_start, const int *sidx, const double *sxw, NeighborList nl,
-                     int *overflow);
-void launch_nl_reverse(hipStream_t s, int n, NeighborList nl);
+// `atoms` lists the atoms to process (nullptr = atoms 0..n-1); `rep_list`
+// the replicas whose cells are scanned (nullptr = replicas 0..nrep-1);
+// `need` (nullptr = all) skips the atoms and cells of replicas whose flag is 0.
+// Each replica owns ncell + 1 entries of cell_start from its cell_off.
+// With nl.d == nullptr the build stores keys only (a Verlet candidate list).
+void launch_wrap_bin(hipStream_t s, int n, const int *atoms, const double *xyz_in, const int *rep_of,
+                     const ReplicaGrid *reps, const int *need, double *xw, int *cell, int *cell_cnt, int *rank);
+void launch_cell_scan(hipStream_t s, int nrep, const int *rep_list, const ReplicaGrid *reps, const int *need,
+                      const int *cell_cnt, int *cell_start);
+void launch_cell_scatter(hipStream_t s, int n, const int *atoms, const int *rep_of, const int *need, const int *cell,
+                         const int *rank, const int *cell_start, const double *xw, int *sidx, double *sxw);
+void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, const ReplicaGrid *reps,
+                     const int *need, double rc2, const double *xw, const int *cell, const int *cell_start,
+                     const int *sidx, const double *sxw, NeighborList nl, int *overflow);
+void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl);
 
 // ---- descriptor / MLP / backward for one species segment ----
 struct SegmentArgs {
@@ -81,12 +87,37 @@ bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a);
 bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a);
 
 // ---- forces / virials / totals ----
-void launch_gather(hipStream_t s, int n, const NeighborList &nl, double *F, double *V, int want_virial);
+void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
+                   int want_virial);
 // Per-replica totals [nrep][10] = (E, V[9]): fixed-size chunks of each
 // replica's atoms are summed (Neumaier, then a fixed tree) and the chunk sums
 // of a replica are added in order, so a replica's totals do not depend on
 // which other replicas share the system.  chunks[c] = (replica, first, end).
 void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
                            const double *E, const double *V, int want_virial, double *partial, double *out);
+
+// ---- served path (serve.hip) ----
+// One replica of a request batch: arena slots [atom_off, atom_off + natoms),
+// positions at [t_off, t_off + natoms) of the batch's position buffer.
+struct SrvRep {
+    int rid, atom_off, natoms, t_off, force, pad_[3];
+};
+// A replica admitted into the arena: its grid (with cell_off / atom_off /
+// natoms set) and where its types sit in the batch's type buffer.
+struct SrvAdmit {
+    ReplicaGrid grid;
+    int rid, types_off, pad_[2];
+};
+void launch_srv_admit(hipStream_t s, int nadm, const SrvAdmit *adm, const int *types_in, ReplicaGrid *reps,
+                      int *rep_of, int *types);
+void launch_srv_check(hipStream_t s, int nrep, const SrvRep *br, const double *pos, double *x, const double *xb,
+                      double half_skin2, int *need, int *need_out);
+void launch_srv_zero_cells(hipStream_t s, int nrep, const SrvRep *br, const ReplicaGrid *reps, const int *need,
+                           int *cell_cnt);
+void launch_srv_update(hipStream_t s, int n, const int *atoms, const int *rep_of, const int *need, const double *x,
+                       double *xb, double *xwb, double *xw);
+void launch_srv_refresh(hipStream_t s, int n, const int *atoms, const int *rep_of, const ReplicaGrid *reps,
+                        double rc2, const double *xw, const NeighborList &vl, const NeighborList &nl, int *overflow);
+void launch_srv_pack(hipStream_t s, int n, const int *atoms, const double *F, double *out);
 
 }  // namespace jnn

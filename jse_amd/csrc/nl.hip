@@ -23,11 +23,14 @@ namespace jnn {
 constexpr int kNlBlock = 64;
 constexpr int kNlLds = 512;  // candidate hits staged per atom (rows longer than this overflow)
 
-__global__ void k_wrap_bin(int n, const double *__restrict__ xyz_in, const int *__restrict__ rep_of,
-                           const ReplicaGrid *__restrict__ reps, double *__restrict__ xw, int *__restrict__ cell,
+__global__ void k_wrap_bin(int n, const int *__restrict__ atoms, const double *__restrict__ xyz_in,
+                           const int *__restrict__ rep_of, const ReplicaGrid *__restrict__ reps,
+                           const int *__restrict__ need, double *__restrict__ xw, int *__restrict__ cell,
                            int *__restrict__ cell_cnt, int *__restrict__ rank) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = atoms ? atoms[t] : t;
+    if (need && !need[rep_of[i]]) return;
     const ReplicaGrid &g = reps[rep_of[i]];
     int c[3];
 #pragma unroll
@@ -45,14 +48,24 @@ __global__ void k_wrap_bin(int n, const double *__restrict__ xyz_in, const int *
     rank[i] = atomicAdd(&cell_cnt[cc], 1);
 }
 
-// single-block exclusive scan of the cell counts
-__global__ __launch_bounds__(1024) void k_cell_scan(int ncell, const int *__restrict__ cnt, int *__restrict__ start) {
+// Exclusive scan of each replica's cell counts, one workgroup per replica:
+// start[cell_off + c] = atom_off + (atoms in the replica's cells before c), so
+// every atom gets a unique slot in its replica's slot range.
+__global__ __launch_bounds__(1024) void k_cell_scan(const int *__restrict__ rep_list, const ReplicaGrid *__restrict__ reps,
+                                                   const int *__restrict__ need, const int *__restrict__ cnt,
+                                                   int *__restrict__ start) {
     __shared__ int part[1024];
+    const int r = rep_list ? rep_list[blockIdx.x] : blockIdx.x;
+    if (need && !need[r]) return;
+    const ReplicaGrid &g = reps[r];
     const int t = threadIdx.x;
+    const int ncell = g.ncell;
+    const int *c_in = cnt + g.cell_off;
+    int *c_out = start + g.cell_off;
     const int per = (ncell + 1023) / 1024;
     const int b = t * per, e = min(b + per, ncell);
     int s = 0;
-    for (int c = b; c < e; ++c) s += cnt[c];
+    for (int c = b; c < e; ++c) s += c_in[c];
     part[t] = s;
     __syncthreads();
     for (int off = 1; off < 1024; off <<= 1) {
@@ -61,19 +74,22 @@ __global__ __launch_bounds__(1024) void k_cell_scan(int ncell, const int *__rest
         part[t] += v;
         __syncthreads();
     }
-    int run = part[t] - s;
+    int run = g.atom_off + part[t] - s;
     for (int c = b; c < e; ++c) {
-        start[c] = run;
-        run += cnt[c];
+        c_out[c] = run;
+        run += c_in[c];
     }
-    if (t == 1023) start[ncell] = part[1023];
+    if (t == 1023) c_out[ncell] = g.atom_off + part[1023];
 }
 
-__global__ void k_cell_scatter(int n, const int *__restrict__ cell, const int *__restrict__ rank,
-                               const int *__restrict__ start, const double *__restrict__ xw, int *__restrict__ sidx,
-                               double *__restrict__ sxw) {
-    const int i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= n) return;
+__global__ void k_cell_scatter(int n, const int *__restrict__ atoms, const int *__restrict__ rep_of,
+                               const int *__restrict__ need, const int *__restrict__ cell,
+                               const int *__restrict__ rank, const int *__restrict__ start,
+                               const double *__restrict__ xw, int *__restrict__ sidx, double *__restrict__ sxw) {
+    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int i = atoms ? atoms[t] : t;
+    if (need && !need[rep_of[i]]) return;
     const int slot = start[cell[i]] + rank[i];
     sidx[slot] = i;
     sxw[3 * slot] = xw[3 * i];
@@ -89,8 +105,9 @@ __device__ __forceinline__ double dist2_no_fma(double dx, double dy, double dz) 
     return (a + b) + c;
 }
 
-__global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restrict__ rep_of,
-                                                       const ReplicaGrid *__restrict__ reps, double rc2,
+__global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restrict__ atoms, const int *__restrict__ rep_of,
+                                                       const ReplicaGrid *__restrict__ reps,
+                                                       const int *__restrict__ need, double rc2,
                                                        const double *__restrict__ xw,
                                                        const int *__restrict__ cell, const int *__restrict__ cstart,
                                                        const int *__restrict__ sidx, const double *__restrict__ sxw,
@@ -99,9 +116,12 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
     __shared__ int s_key[kNlLds];
     __shared__ float s_d[3][kNlLds];
     const int lane = threadIdx.x;
-    for (int i = blockIdx.x; i < n; i += gridDim.x) {
+    for (int t_ = blockIdx.x; t_ < n; t_ += gridDim.x) {
+        const int i = atoms ? atoms[t_] : t_;
+        const int ri = rep_of[i];
+        if (need && !need[ri]) continue;   // uniform over the wave: no barrier is skipped by part of it
         const double xi = xw[3 * i], yi = xw[3 * i + 1], zi = xw[3 * i + 2];
-        const ReplicaGrid &g = reps[rep_of[i]];
+        const ReplicaGrid &g = reps[ri];
         const int c = cell[i] - g.cell_off;
         const int cx = c % g.size[0], cy = (c / g.size[0]) % g.size[1], cz = c / (g.size[0] * g.size[1]);
         if (lane < 27) {
@@ -195,7 +215,7 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
         const size_t row = (size_t)i * nl.cap;
         for (int t = lane; t < nw; t += kNlBlock) {
             nl.key[row + t] = s_key[t];
-            nl.d[row + t] = make_float4(s_d[0][t], s_d[1][t], s_d[2][t], 0.f);
+            if (nl.d) nl.d[row + t] = make_float4(s_d[0][t], s_d[1][t], s_d[2][t], 0.f);
         }
         if (lane == 0) {
             nl.cnt[i] = hits;
@@ -206,10 +226,11 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
 }
 
 // rev[i][k] = slot of (i, -shift) in j's row (binary search on the sorted keys)
-__global__ void k_nl_reverse(int n, NeighborList nl) {
+__global__ void k_nl_reverse(int n, const int *__restrict__ atoms, NeighborList nl) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = (int)(t / nl.cap), k = (int)(t % nl.cap);
-    if (i >= n) return;
+    const int ti = (int)(t / nl.cap), k = (int)(t % nl.cap);
+    if (ti >= n) return;
+    const int i = atoms ? atoms[ti] : ti;
     const int ci = min(nl.cnt[i], nl.cap);
     if (k >= ci) return;
     const int key = nl.key[(size_t)i * nl.cap + k];
@@ -236,32 +257,35 @@ void launch_fill_int(hipStream_t s, int n, int *p, int v) {
     hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, n, p, v);
 }
 
-void launch_wrap_bin(hipStream_t s, int n, const double *xyz_in, const int *rep_of, const ReplicaGrid *reps,
-                     double *xw, int *cell, int *cell_cnt, int *rank) {
+void launch_wrap_bin(hipStream_t s, int n, const int *atoms, const double *xyz_in, const int *rep_of,
+                     const ReplicaGrid *reps, const int *need, double *xw, int *cell, int *cell_cnt, int *rank) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_wrap_bin, dim3((n + 255) / 256), dim3(256), 0, s, n, xyz_in, rep_of, reps, xw, cell, cell_cnt,
-                       rank);
+    hipLaunchKernelGGL(k_wrap_bin, dim3((n + 255) / 256), dim3(256), 0, s, n, atoms, xyz_in, rep_of, reps, need, xw,
+                       cell, cell_cnt, rank);
 }
-void launch_cell_scan(hipStream_t s, int ncell, const int *cell_cnt, int *cell_start) {
-    hipLaunchKernelGGL(k_cell_scan, dim3(1), dim3(1024), 0, s, ncell, cell_cnt, cell_start);
+void launch_cell_scan(hipStream_t s, int nrep, const int *rep_list, const ReplicaGrid *reps, const int *need,
+                      const int *cell_cnt, int *cell_start) {
+    if (nrep <= 0) return;
+    hipLaunchKernelGGL(k_cell_scan, dim3(nrep), dim3(1024), 0, s, rep_list, reps, need, cell_cnt, cell_start);
 }
-void launch_cell_scatter(hipStream_t s, int n, const int *cell, const int *rank, const int *cell_start,
-                         const double *xw, int *sidx, double *sxw) {
+void launch_cell_scatter(hipStream_t s, int n, const int *atoms, const int *rep_of, const int *need, const int *cell,
+                         const int *rank, const int *cell_start, const double *xw, int *sidx, double *sxw) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, cell, rank, cell_start, xw, sidx, sxw);
+    hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, atoms, rep_of, need, cell, rank,
+                       cell_start, xw, sidx, sxw);
 }
-void launch_nl_build(hipStream_t s, int n, const int *rep_of, const ReplicaGrid *reps, double rc2, const double *xw,
-                     const int *cell, const int *cell_start, const int *sidx, const double *sxw, NeighborList nl,
-                     int *overflow) {
+void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, const ReplicaGrid *reps,
+                     const int *need, double rc2, const double *xw, const int *cell, const int *cell_start,
+                     const int *sidx, const double *sxw, NeighborList nl, int *overflow) {
     if (n <= 0) return;
     const int grid = n < 65536 * 4 ? n : 65536 * 4;
-    hipLaunchKernelGGL(k_nl_build, dim3(grid), dim3(kNlBlock), 0, s, n, rep_of, reps, rc2, xw, cell, cell_start, sidx,
-                       sxw, nl, overflow);
+    hipLaunchKernelGGL(k_nl_build, dim3(grid), dim3(kNlBlock), 0, s, n, atoms, rep_of, reps, need, rc2, xw, cell,
+                       cell_start, sidx, sxw, nl, overflow);
 }
-void launch_nl_reverse(hipStream_t s, int n, NeighborList nl) {
+void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl) {
     if (n <= 0) return;
     const long long tot = (long long)n * nl.cap;
-    hipLaunchKernelGGL(k_nl_reverse, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, nl);
+    hipLaunchKernelGGL(k_nl_reverse, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, atoms, nl);
 }
 
 }  // namespace jnn

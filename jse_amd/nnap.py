@@ -51,8 +51,14 @@ EXPORTS = [
     "jse_nnap_system_compute", "jse_nnap_system_get_results", "jse_nnap_system_compute_fp",
     "jse_nnap_system_get_neighbors", "jse_nnap_system_benchmark", "jse_nnap_system_nl_stats",
     "jse_nnap_system_get_pair_gradients", "jse_nnap_batch_create", "jse_nnap_system_get_replica_results",
-    "jse_nnap_system_set_graphs",
+    "jse_nnap_system_set_graphs", "jse_nnap_server_create", "jse_nnap_server_add_replica", "jse_nnap_server_submit",
+    "jse_nnap_server_wait", "jse_nnap_server_collect", "jse_nnap_server_drain", "jse_nnap_server_stats",
+    "jse_nnap_server_destroy",
 ]
+
+SERVER_STATS = ["completed", "atom_evals", "batches", "mean_batch_atoms", "rebuilds", "admissions", "evictions",
+                "growths", "resident_replicas", "resident_atoms", "arena_slots", "arena_bytes", "candidate_cap",
+                "row_cap", "pending", "device_bytes"]
 
 
 class NNAPError(RuntimeError):
@@ -101,6 +107,17 @@ def lib():
                                         ctypes.POINTER(vp)]
     L.jse_nnap_system_get_replica_results.argtypes = [vp, _c_dbl_p, _c_dbl_p]
     L.jse_nnap_system_set_graphs.argtypes = [vp, ctypes.c_int]
+    L.jse_nnap_server_create.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
+                                         ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)]
+    L.jse_nnap_server_add_replica.argtypes = [vp, ctypes.c_int, _c_int_p, _c_dbl_p, _c_int_p]
+    L.jse_nnap_server_submit.argtypes = [vp, ctypes.c_int, _c_dbl_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p,
+                                         _c_ll_p]
+    L.jse_nnap_server_wait.argtypes = [vp, ctypes.c_longlong, _c_dbl_p]
+    L.jse_nnap_server_collect.argtypes = [vp, _c_ll_p, _c_dbl_p, _c_int_p, ctypes.c_int, _c_int_p]
+    L.jse_nnap_server_drain.argtypes = [vp]
+    L.jse_nnap_server_stats.argtypes = [vp, _c_dbl_p, ctypes.c_int]
+    L.jse_nnap_server_destroy.argtypes = [vp]
+    L.jse_nnap_server_destroy.restype = None
     _LIB = L
     return L
 
@@ -259,6 +276,104 @@ class System:
         return ms.value, ph
 
 
+class Server:
+    """Device-resident replicas evaluated by continuous batching (see
+    jse_nnap_server_* in include/jse_nnap_hip.h).  ``submit`` returns a ticket
+    at once; ``wait(ticket)`` / ``collect()`` hand back results.  The output
+    arrays of a request are kept here until its ticket is released."""
+
+    def __init__(self, nnap: "NNAP", precision: int = BF16_MLP, skin: float = 1.0, hbm_budget_bytes: float = 2e9,
+                 max_batch_atoms: int = 65536, max_replicas: int = 4096, max_density: float = 0.08):
+        self.nnap = nnap
+        h = ctypes.c_void_p()
+        _check(lib().jse_nnap_server_create(nnap._h, int(precision), float(skin), float(hbm_budget_bytes),
+                                            int(max_batch_atoms), int(max_replicas), float(max_density),
+                                            ctypes.byref(h)))
+        self._h = h
+        self._natoms: List[int] = []
+        self._out: Dict[int, tuple] = {}
+
+    def close(self):
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().jse_nnap_server_destroy(self._h)
+            self._h = None
+            self._out.clear()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def add_replica(self, atoms: AtomData) -> int:
+        t = np.ascontiguousarray(self.nnap._types_of(atoms), dtype=np.int32)
+        box = np.ascontiguousarray(atoms.box, dtype=np.float64)
+        rid = ctypes.c_int()
+        _check(lib().jse_nnap_server_add_replica(self._h, atoms.natoms, _ip(t), _dp(box), ctypes.byref(rid)))
+        self._natoms.append(atoms.natoms)
+        return rid.value
+
+    def submit(self, rid: int, xyz, what: int = ENERGY | FORCE | VIRIAL) -> int:
+        xyz = np.ascontiguousarray(xyz, dtype=np.float64)
+        n = self._natoms[rid]
+        if xyz.shape != (n, 3):
+            raise ValueError("positions must be [natoms][3]")
+        e = np.zeros(1)
+        f = np.zeros((n, 3)) if what & FORCE else None
+        v = np.zeros(9) if what & VIRIAL else None
+        tk = ctypes.c_longlong()
+        _check(lib().jse_nnap_server_submit(self._h, int(rid), _dp(xyz), int(what), _dp(e), _dp(f), _dp(v),
+                                            ctypes.byref(tk)))
+        self._out[tk.value] = (rid, e, f, v)
+        return tk.value
+
+    def _result(self, ticket, latency):
+        rid, e, f, v = self._out.pop(ticket)
+        out = {"replica": rid, "energy": float(e[0]), "latency_s": latency}
+        if f is not None:
+            out["forces"] = f
+        if v is not None:
+            out["virial"] = v
+        return out
+
+    def wait(self, ticket: int) -> dict:
+        lat = ctypes.c_double()
+        rc = lib().jse_nnap_server_wait(self._h, int(ticket), ctypes.byref(lat))
+        if rc != 0:
+            self._out.pop(ticket, None)
+            _check(rc)
+        return self._result(ticket, lat.value)
+
+    def collect(self, cap: int = 4096) -> Dict[int, dict]:
+        """released completed requests: ticket -> result (with 'status')"""
+        tk = np.zeros(cap, dtype=np.int64)
+        lat = np.zeros(cap)
+        st = np.zeros(cap, dtype=np.int32)
+        cnt = ctypes.c_int()
+        _check(lib().jse_nnap_server_collect(self._h, tk.ctypes.data_as(_c_ll_p), _dp(lat), _ip(st), cap,
+                                             ctypes.byref(cnt)))
+        out = {}
+        for k in range(cnt.value):
+            r = self._result(int(tk[k]), float(lat[k]))
+            r["status"] = int(st[k])
+            out[int(tk[k])] = r
+        return out
+
+    def drain(self):
+        _check(lib().jse_nnap_server_drain(self._h))
+
+    def stats(self) -> Dict[str, float]:
+        v = np.zeros(len(SERVER_STATS))
+        _check(lib().jse_nnap_server_stats(self._h, _dp(v), len(v)))
+        return dict(zip(SERVER_STATS, v.tolist()))
+
+
 class NNAP:
     """jse's NNAP potential on one MI355X.  ``nthreads`` is accepted for API
     parity with the reference and ignored (the work runs on the GPU)."""
@@ -375,6 +490,10 @@ class NNAP:
         b = System(self, int(sum(per)), types, boxes, self.precision, natoms_per_replica=per)
         b.set_positions(np.concatenate([np.asarray(a.xyz, dtype=float) for a in structures]))
         return b
+
+    def server(self, **kw) -> Server:
+        """a continuous-batching server of device-resident replicas"""
+        return Server(self, **kw)
 
     def calFp(self, atoms: AtomData) -> np.ndarray:
         """descriptors of every atom (jse_nnap_calFp, nnap_interface.cpp:11-24)"""
