@@ -120,6 +120,164 @@ def batched_c4(nrep=512, seed=17, steps=10):
     return out
 
 
+def precision_c3(n=256000, seed=7, steps=10):
+    """Precision variant at 256,000 atoms (same liquid recipe, L = 164.035 A):
+    the BF16-MLP path's throughput and its energy / force / virial errors
+    against the FP64 restatement on the same inputs, beside the plain FP32
+    path's errors."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    md = W.ternary_model()
+    st = W.ternary_liquid(n=n, seed=seed)
+    ref = O.system_efv_par(O.Oracle(M.load_model(md), fast=True), st, min(16, os.cpu_count() or 1))
+    out = {"natoms": n, "box_A": float(st.box[0])}
+    for name, prec in (("bf16_mlp", nnap.BF16_MLP), ("fp32", nnap.FP32)):
+        with nnap.NNAP(md, precision=prec) as pot:
+            s_ = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+            s_.compute(nnap.ENERGY | nnap.FORCE | nnap.VIRIAL)
+            r = s_.results()
+            ms, _ = s_.benchmark(nnap.ENERGY | nnap.FORCE | nnap.VIRIAL, warmup=2, iters=steps)
+        df = r["forces"] - ref["forces"]
+        out[name] = {"ms_per_step": ms, "atom_evals_per_s": n / (ms * 1e-3),
+                     "energy_per_atom_eV": abs(r["energy"] - ref["energy"]) / n,
+                     "energy_rmse_per_atom_eV": float(np.sqrt(((r["energies"] - ref["energies"]) ** 2).mean())),
+                     "force_rms_eV_per_A": float(np.sqrt((df ** 2).mean())),
+                     "force_max_eV_per_A": float(np.abs(df).max()),
+                     "virial_per_atom_eV": float(np.abs(r["virial"] - ref["virial"]).max()) / n}
+    out["bf16_speedup_vs_fp32"] = out["bf16_mlp"]["atom_evals_per_s"] / out["fp32"]["atom_evals_per_s"]
+    return out
+
+
+def pcie_inclusive(pot, st, iters=20):
+    """Host positions in, host energies + forces + virial out, per call."""
+    a = nnap.AtomData(st.box, st.xyz, st.types)
+    s_ = pot.system(a)
+    for _ in range(3):
+        s_.set_positions(st.xyz)
+        s_.compute(nnap.ENERGY | nnap.FORCE | nnap.VIRIAL)
+        s_.results(energies=True, forces=True, virials=False)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        s_.set_positions(st.xyz)
+        s_.compute(nnap.ENERGY | nnap.FORCE | nnap.VIRIAL)
+        s_.results(energies=True, forces=True, virials=False)
+    dt = (time.perf_counter() - t0) / iters
+    return {"ms_per_call": dt * 1e3, "atom_evals_per_s": st.natoms / dt}
+
+
+def _pct(a, q):
+    return float(np.percentile(np.asarray(a), q)) if len(a) else None
+
+
+def served_c5(nrep=2000, nmin=100, nmax=20000, seed=23, sat_s=8.0, rate_s=6.0, fractions=(0.5, 0.8),
+              budget_gb=80.0, max_batch=262144):
+    """Served variant: `nrep` replicas (log-uniform nmin-nmax atoms, 1-3
+    species, 0.05-0.065 A^-3) resident on the device; every request moves a
+    replica's atoms by N(0, 0.01 A) per component and asks for E+F+virial
+    (BF16 MLP, Verlet skin 1.0 A).  Saturation throughput from a closed loop
+    with two outstanding requests per replica, then open-loop Poisson
+    arrivals (replica drawn uniformly) at fractions of the saturation
+    request rate, with p50/p99 latency (submission -> results on the host)
+    and the generator's lag behind the Poisson schedule."""
+    md = W.ternary_model()
+    reps = W.replicas(nrep, nmin, nmax, seed, workers=min(16, os.cpu_count() or 1))
+    natoms = np.array([r.natoms for r in reps])
+    rng = np.random.default_rng(seed)
+    noise = rng.normal(0.0, 0.01, size=1 << 24)
+    pos = [r.xyz.copy() for r in reps]
+
+    def step(k):
+        n3 = 3 * natoms[k]
+        o = int(rng.integers(0, noise.size - n3))
+        pos[k] += noise[o:o + n3].reshape(-1, 3)
+        return pos[k]
+
+    out = {"replicas": nrep, "resident_atoms": int(natoms.sum()), "sizes": [nmin, nmax], "skin_A": 1.0,
+           "step_sigma_A": 0.01, "precision": "bf16-mlp"}
+    with nnap.NNAP(md) as pot, pot.server(precision=nnap.BF16_MLP, skin=1.0, hbm_budget_bytes=budget_gb * 1e9,
+                                           max_batch_atoms=max_batch, max_replicas=nrep,
+                                           max_density=0.065) as srv:
+        rids = [srv.add_replica(nnap.AtomData(r.box, r.xyz, r.types)) for r in reps]
+        t0 = time.perf_counter()
+        for k in range(nrep):
+            srv.submit(rids[k], pos[k])
+        srv.drain()
+        srv.collect(nrep + 16)
+        out["admission_s"] = time.perf_counter() - t0
+        print(f"served: {nrep} replicas admitted in {out['admission_s']:.2f} s", file=sys.stderr, flush=True)
+
+        # closed loop, two requests in flight per replica
+        owner = {}
+        for k in range(nrep):
+            for _ in range(2):
+                owner[srv.submit(rids[k], step(k))] = k
+        done_atoms = 0
+        lat = []
+        t0 = time.perf_counter()
+        t_end = t0 + sat_s
+        while time.perf_counter() < t_end:
+            got = srv.collect(8192)
+            if not got:
+                time.sleep(0.0002)
+                continue
+            for tk, r in got.items():
+                k = owner.pop(tk)
+                done_atoms += natoms[k]
+                lat.append(r["latency_s"])
+                owner[srv.submit(rids[k], step(k))] = k
+        dt = time.perf_counter() - t0
+        srv.drain()
+        srv.collect(1 << 16)
+        sat_rate = done_atoms / dt
+        req_rate = len(lat) / dt
+        st = srv.stats()
+        out["saturation"] = {"atom_evals_per_s": sat_rate, "requests_per_s": req_rate, "window_s": dt,
+                             "p50_ms": _pct(lat, 50) * 1e3, "p99_ms": _pct(lat, 99) * 1e3,
+                             "mean_batch_atoms": st["mean_batch_atoms"]}
+        print(f"served: saturation {sat_rate:.3e} atom-evals/s, {req_rate:.0f} req/s", file=sys.stderr, flush=True)
+
+        # open loop: Poisson arrivals at fractions of the saturation request rate
+        out["poisson"] = []
+        for f in fractions:
+            lam = f * req_rate
+            lat, lag, atoms_done = [], [], 0
+            owner = {}
+            t0 = time.perf_counter()
+            t_end = t0 + rate_s
+            t_next = t0 + rng.exponential(1.0 / lam)
+            while True:
+                now = time.perf_counter()
+                if now >= t_end:
+                    break
+                while t_next <= now:
+                    k = int(rng.integers(0, nrep))
+                    owner[srv.submit(rids[k], step(k))] = k
+                    lag.append(time.perf_counter() - t_next)
+                    t_next += rng.exponential(1.0 / lam)
+                for tk, r in srv.collect(8192).items():
+                    atoms_done += natoms[owner.pop(tk)]
+                    lat.append(r["latency_s"])
+                wait = min(t_next, t_end) - time.perf_counter()
+                if wait > 0:
+                    time.sleep(min(wait, 0.0005))
+            srv.drain()
+            for tk, r in srv.collect(1 << 16).items():
+                atoms_done += natoms[owner.pop(tk)]
+                lat.append(r["latency_s"])
+            span = time.perf_counter() - t0
+            out["poisson"].append({"offered_fraction_of_saturation": f, "offered_requests_per_s": lam,
+                                   "requests": len(lat), "atom_evals_per_s": atoms_done / span,
+                                   "p50_ms": _pct(lat, 50) * 1e3, "p99_ms": _pct(lat, 99) * 1e3,
+                                   "generator_lag_p99_ms": _pct(lag, 99) * 1e3})
+            print(f"served: poisson {f:.2f} -> p50 {out['poisson'][-1]['p50_ms']:.2f} ms "
+                  f"p99 {out['poisson'][-1]['p99_ms']:.2f} ms", file=sys.stderr, flush=True)
+        st = srv.stats()
+        out["server"] = {k: st[k] for k in ("batches", "mean_batch_atoms", "rebuilds", "admissions", "evictions",
+                                            "growths", "resident_atoms", "candidate_cap", "row_cap")}
+        out["server"]["device_GB"] = st["device_bytes"] / 1e9
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -129,6 +287,9 @@ def main():
     ap.add_argument("--natoms", type=int, default=32000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-served", action="store_true")
+    ap.add_argument("--no-256k", action="store_true")
+    ap.add_argument("--served-replicas", type=int, default=2000)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
@@ -183,12 +344,24 @@ def main():
         "wall_s_timed": wall,
         "roofline": roof,
     }
+    line["pcie_inclusive"] = pcie_inclusive(pot, st)
     if not args.no_cpu_baseline:
         base, err = cpu_baseline(md, st, res)
         line["cpu_baseline"] = base
         line["error_vs_fp64"] = err
+    if not args.no_256k:
+        line["precision_256k"] = precision_c3()
     if not args.no_batched:
         line["batched_512_replicas"] = batched_c4()
+    if not args.no_served:
+        sv = served_c5(nrep=args.served_replicas)
+        sv["saturation"]["speedup_vs_plain_fp32_c2"] = sv["saturation"]["atom_evals_per_s"] / value
+        line["served_2000_replicas"] = sv
+        pts = sv["poisson"]
+        line["p50_ms"] = pts[-1]["p50_ms"]
+        line["p99_ms"] = pts[-1]["p99_ms"]
+        line["latency_note"] = (f"served, Poisson arrivals at {pts[-1]['offered_fraction_of_saturation']:.0%} "
+                                "of the saturation request rate, submission to results on the host")
     print(json.dumps(line), flush=True)
     pot.close()
 

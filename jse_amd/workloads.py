@@ -100,9 +100,13 @@ def ternary_liquid(n: int = 32000, rho: float = 0.058, dmin: float = 2.0, seed: 
 
 
 def replicas(count: int, nmin: int, nmax: int, seed: int, rho_range=(0.05, 0.065), dmin: float = 2.0,
-             ntypes: int = 3) -> List[Structure]:
+             ntypes: int = 3, workers: int = 1) -> List[Structure]:
+    """`count` independent replicas: N log-uniform in [nmin, nmax], density
+    uniform in `rho_range`, 1..ntypes species drawn per replica.  The random
+    draws are sequential; the per-replica placements are independent (own
+    seed), so `workers` > 1 only runs them concurrently."""
     rng = np.random.Generator(np.random.PCG64(seed))
-    out = []
+    plan = []
     for r in range(count):
         n = int(round(math.exp(rng.uniform(math.log(nmin), math.log(nmax)))))
         rho = float(rng.uniform(*rho_range))
@@ -110,10 +114,18 @@ def replicas(count: int, nmin: int, nmax: int, seed: int, rho_range=(0.05, 0.065
         chosen = np.sort(rng.choice(np.arange(1, ntypes + 1), size=k, replace=False))
         L = (n / rho) ** (1.0 / 3.0)
         box = np.array([L, L, L])
-        xyz = rsa(n, box, dmin, seed * 100003 + r)
         types = chosen[rng.integers(0, k, size=n)].astype(np.int32)
-        out.append(Structure(box, xyz, types))
-    return out
+        plan.append((n, box, seed * 100003 + r, types))
+
+    def place(p):
+        n, box, sd, types = p
+        return Structure(box, rsa(n, box, dmin, sd), types)
+
+    if workers > 1 and count > 1:
+        from concurrent.futures import ThreadPoolExecutor
+        with ThreadPoolExecutor(max_workers=workers) as ex:
+            return list(ex.map(place, plan))
+    return [place(p) for p in plan]
 
 
 # ---------------------------------------------------------------------------
