@@ -136,6 +136,19 @@ int jse_nnap_system_benchmark(jse_nnap_system s, int what, int warmup, int iters
  * largest row and the number of pairs without a reverse partner. */
 int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long long *pairs, long long *unmatched);
 
+/* Autotuner of a system's launch shapes (no reference counterpart: the
+ * reference's CUDA path uses one fixed block size, NNAPGEN's
+ * __NNAPGEN_CUDA_BLOCKSIZE__, nnap_interface_cuda.cu:456).  Times `iters`
+ * evaluations of kind `what` per candidate on the system's current data and
+ * keeps, knob by knob, the fastest: tune[0] descriptor-forward and tune[1]
+ * descriptor-backward persistent workgroups per CU (0 = one per atom),
+ * tune[2] MLP workgroups per CU, tune[3] neighbour-build workgroups per CU
+ * (0 = one per atom), tune[4] force-gather atoms per workgroup.  Writes the
+ * chosen knobs (5 ints) and the tuned device time per evaluation. */
+int jse_nnap_system_autotune(jse_nnap_system s, int what, int iters, int *tune, double *ms_step);
+/* Applies a tuning vector (e.g. a cached autotuner result); n <= 5 knobs. */
+int jse_nnap_system_set_tuning(jse_nnap_system s, const int *tune, int n);
+
 /* ---- server: device-resident replicas, continuous batching ----------------
  * No reference counterpart as one call: the reference evaluates one AtomData
  * per call, and its forward-flux-sampling driver and LAMMPS generators step

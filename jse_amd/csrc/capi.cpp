@@ -94,7 +94,7 @@ jse_nnap_model_s::~jse_nnap_model_s() {
 
 namespace jnn {
 void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision, int nrows, const int *rows_atom,
-                 const float *x, float *G, double *E, bool grad) {
+                 const float *x, float *G, double *E, bool grad, int wg_per_cu) {
     const SpeciesHost &sp = m->host.species[t];
     const SpeciesDev &dv = m->dev[t];
     MlpArgs a{};
@@ -114,6 +114,7 @@ void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision,
     a.E = E;
     a.G = G;
     a.want_grad = grad;
+    a.wg_per_cu = wg_per_cu;
     if (precision == JSE_NNAP_BF16_MLP) {
         if (!launch_mlp_mfma_bf16(s, a))
             throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
@@ -157,6 +158,7 @@ struct jse_nnap_system_s {
     // captured step graphs, one per evaluation kind (E / E+F / E+F+V)
     hipGraphExec_t graph_exec[8] = {};
     bool use_graph = true;
+    LaunchTune tune;                   // launch shapes (autotuner)
 
     ~jse_nnap_system_s() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -309,7 +311,7 @@ struct jse_nnap_system_s {
         launch_cell_scan(stream, nrep, nullptr, d_reps, nullptr, d_cell_cnt, d_cell_start);
         launch_cell_scatter(stream, n, nullptr, d_rep_of, nullptr, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw);
         launch_nl_build(stream, n, nullptr, d_rep_of, d_reps, nullptr, rc2, d_xw, d_cell, d_cell_start, d_sidx, d_sxw,
-                        nl, d_overflow);
+                        nl, d_overflow, tune.nl_wg_per_cu);
         if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
         launch_nl_reverse(stream, n, nullptr, nl);
         if (timed) JNN_HIP(hipEventRecord(ev[2], stream));
@@ -341,6 +343,7 @@ struct jse_nnap_system_s {
         a.rows_atom = d_rows + seg[t];
         a.types = d_types;
         a.rc = (float)sp.rcut;
+        a.ntypes = model->host.ntypes();
         a.W = dv.W;
         a.mu = raw ? model->zeros : dv.mu;
         a.sig = raw ? model->ones : dv.sig;
@@ -356,6 +359,7 @@ struct jse_nnap_system_s {
         for (int t = 0; t < model->host.ntypes(); ++t) {
             const SpeciesHost &sp = model->host.species[t];
             SegmentArgs a = seg_args(t, raw);
+            a.wg_per_cu = tune.desc_fwd_wg_per_cu;
             if (!launch_desc_fwd(stream, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
                 throw Error(ERR_UNSUPPORTED, "no descriptor kernel compiled for lmax=" + std::to_string(sp.lmax) +
                                                  " nmax=" + std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P));
@@ -370,22 +374,73 @@ struct jse_nnap_system_s {
         if (timed) JNN_HIP(hipEventRecord(ev[3], stream));
         for (int t = 0; t < model->host.ntypes(); ++t) {
             SegmentArgs sa = seg_args(t, false);
-            enqueue_mlp(stream, model, t, precision, sa.nrows, sa.rows_atom, sa.x, sa.G, d_E, grad);
+            enqueue_mlp(stream, model, t, precision, sa.nrows, sa.rows_atom, sa.x, sa.G, d_E, grad,
+                        tune.mlp_wg_per_cu);
         }
         if (timed) JNN_HIP(hipEventRecord(ev[4], stream));
         if (grad) {
             for (int t = 0; t < model->host.ntypes(); ++t) {
                 const SpeciesHost &sp = model->host.species[t];
                 SegmentArgs a = seg_args(t, false);
+                a.wg_per_cu = tune.desc_bwd_wg_per_cu;
                 if (!launch_desc_bwd(stream, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
                     throw Error(ERR_UNSUPPORTED, "no descriptor backward kernel compiled for this shape");
             }
         }
         if (timed) JNN_HIP(hipEventRecord(ev[5], stream));
-        if (grad) launch_gather(stream, n, nullptr, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0);
+        if (grad) launch_gather(stream, n, nullptr, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0, tune.gather_waves);
         launch_replica_totals(stream, nchunk, d_chunks, nrep, d_rep_chunk, d_E, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0,
                               d_partial, d_tot);
         if (timed) JNN_HIP(hipEventRecord(ev[6], stream));
+    }
+
+    int *tune_field(int k) {
+        switch (k) {
+            case 0: return &tune.desc_fwd_wg_per_cu;
+            case 1: return &tune.desc_bwd_wg_per_cu;
+            case 2: return &tune.mlp_wg_per_cu;
+            case 3: return &tune.nl_wg_per_cu;
+            default: return &tune.gather_waves;
+        }
+    }
+    // Device time per evaluation of `iters` back-to-back eager evaluations.
+    double time_eval(int what, int iters) {
+        enqueue(what, false);
+        JNN_HIP(hipEventRecord(ev[0], stream));
+        for (int k = 0; k < iters; ++k) enqueue(what, false);
+        JNN_HIP(hipEventRecord(ev[7], stream));
+        JNN_HIP(hipEventSynchronize(ev[7]));
+        float ms = 0.f;
+        JNN_HIP(hipEventElapsedTime(&ms, ev[0], ev[7]));
+        return (double)ms / iters;
+    }
+    // Coordinate descent over the launch-shape knobs, each candidate timed on
+    // this system's own data; keeps the fastest shape of every knob.
+    double autotune(int what, int iters) {
+        JNN_CHECK(have_pos, ERR_ARG, "positions not set");
+        JNN_CHECK(iters >= 1, ERR_ARG, "iters must be >= 1");
+        static const std::vector<int> cand[kTuneKnobs] = {
+            {0, 4, 8, 16, 32}, {0, 4, 8, 16, 32}, {1, 2, 4, 8}, {0, 4, 8, 16}, {1, 2, 4, 8}};
+        drop_graphs();
+        compute(what, true);  // settles the neighbour capacity
+        finish();
+        double best = time_eval(what, iters);
+        for (int k = 0; k < kTuneKnobs; ++k) {
+            int *f = tune_field(k);
+            int keep = *f;
+            for (int v : cand[k]) {
+                if (v == keep) continue;
+                *f = v;
+                const double t = time_eval(what, iters);
+                if (t < best) {
+                    best = t;
+                    keep = v;
+                }
+            }
+            *f = keep;
+        }
+        drop_graphs();
+        return best;
     }
 
     // One evaluation: replays the captured step graph of this kind when there
@@ -563,6 +618,24 @@ int jse_nnap_system_set_graphs(jse_nnap_system s, int enable) {
         JNN_HIP(hipStreamSynchronize(s->stream));
         s->drop_graphs();
         s->use_graph = enable != 0;
+    });
+}
+int jse_nnap_system_autotune(jse_nnap_system s, int what, int iters, int *tune, double *ms_step) {
+    return guard([&] {
+        JNN_CHECK(s, ERR_ARG, "null system");
+        const double ms = s->autotune(what, iters);
+        if (ms_step) *ms_step = ms;
+        if (tune)
+            for (int k = 0; k < kTuneKnobs; ++k) tune[k] = *s->tune_field(k);
+    });
+}
+int jse_nnap_system_set_tuning(jse_nnap_system s, const int *tune, int n) {
+    return guard([&] {
+        JNN_CHECK(s && tune && n >= 0 && n <= kTuneKnobs, ERR_ARG, "bad tuning vector");
+        for (int k = 0; k < n; ++k) JNN_CHECK(tune[k] >= 0 && tune[k] <= 64, ERR_ARG, "tuning value out of range");
+        JNN_HIP(hipStreamSynchronize(s->stream));
+        for (int k = 0; k < n; ++k) *s->tune_field(k) = tune[k];
+        s->drop_graphs();
     });
 }
 void jse_nnap_system_destroy(jse_nnap_system s) { delete s; }

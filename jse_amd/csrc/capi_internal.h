@@ -119,5 +119,5 @@ struct jse_nnap_model_s {
 namespace jnn {
 // Launches the MLP of species t over `nrows` rows (FP32 MFMA, generic FP32, or BF16 MFMA).
 void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision, int nrows, const int *rows_atom,
-                 const float *x, float *G, double *E, bool grad);
+                 const float *x, float *G, double *E, bool grad, int wg_per_cu = 0);
 }  // namespace jnn

@@ -15,10 +15,14 @@
 //
 // Backward (reference: basis_SphericalChebyshev.hpp:353-465): with
 // Gc[p][l] = 2*4pi/(2l+1) * dE/dfp[p][l] (from the MLP pass) the gradient of
-// E with respect to A is Gc[p][l(m)] * A[p][m].  One lane per neighbour:
-//   gY[m] = sum_p Gc[p][l(m)] A[p][m] c[p],  gc[p] = sum_l Gc[p][l] sum_{m in l} A[p][m] Y[m]
-// with A and Gc read at wave-uniform addresses; then the chain rule through
-// fc, Rn, W and (r, theta, phi) gives g_j = -dE_i/d(r_j - r_i).  The pole
+// E with respect to A is Gc[p][l(m)] * A[p][m].  Because c[p] = fc sum_n
+// W[t][p][n] Rn[n] is linear in Rn, the per-atom dE/dA is first taken back
+// through each species' mixing, Gt[n][m] = sum_p W[t][p][n] Gc[p][l] A[p][m]
+// (LDS, K x M per species), and one lane per neighbour of species t forms
+//   gY[m] = fc sum_n Rn[n] Gt[n][m],  gct[n] = sum_m Gt[n][m] Y[m],
+// dE/dRn[n] = fc gct[n], dE/dfc = sum_n Rn[n] gct[n]: 2KM multiply-adds per
+// pair instead of 2PM; then the chain rule through fc, Rn and
+// (r, theta, phi) gives g_j = -dE_i/d(r_j - r_i).  The pole
 // convention of the reference (dxy == 0 -> cos phi = 1, sin phi = 0, the phi
 // term dropped; basis_SphericalUtil.hpp:165-181) is kept.
 #include "common.h"
@@ -67,27 +71,6 @@ __device__ __forceinline__ void mix_rnp(const float *rn, int tj, const float *__
         }
     }
 }
-// gRn[n] = sum_p W[t][p][n] gRnp[p]
-template <int K, int P, int MIX>
-__device__ __forceinline__ void mix_rnp_back(const float *grnp, int tj, const float *__restrict__ W, float *grn) {
-    if (MIX == MIX_NONE) {
-#pragma unroll
-        for (int q = 0; q < K; ++q) grn[q] = grnp[q];
-    } else if (MIX == MIX_DEFAULT) {
-        const float wt = (tj & 1) ? (float)tj : -(float)tj;
-#pragma unroll
-        for (int q = 0; q < K; ++q) grn[q] = grnp[q] + wt * grnp[K + q];
-    } else {
-        const float *w = W + (size_t)(tj - 1) * P * K;
-#pragma unroll
-        for (int q = 0; q < K; ++q) grn[q] = 0.f;
-#pragma unroll
-        for (int p = 0; p < P; ++p)
-#pragma unroll
-            for (int q = 0; q < K; ++q) grn[q] += w[p * K + q] * grnp[p];
-    }
-}
-
 struct PairGeom {
     float r, ct, st, cp, sp, dxy;
     bool pole;
@@ -208,12 +191,23 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
     }
 }
 
+// Number of per-species dE/dA blocks the backward keeps in LDS: mixing
+// without weights (P = K, one block) or one block per species.
+template <int MIX>
+__host__ __device__ constexpr int bwd_blocks(int ntypes) {
+    return MIX == MIX_NONE ? 1 : ntypes;
+}
+
 template <int LMAX, int K, int P, int MIX>
 __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl) {
     using SH = DescShape<LMAX, K, P, MIX>;
     constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
-    __shared__ float sGA[P * M];  // dE/dA of the current atom
+    // sGt[t][n][m] = sum_p W[t][p][n] Gc[p][l(m)] A[p][m]: dE/dA taken back
+    // through species t's radial mixing, so a neighbour of species t needs
+    // K (not P) rows
+    extern __shared__ float sGt[];
     const int lane = threadIdx.x;
+    const int nblk = bwd_blocks<MIX>(a.ntypes);
     const float rc = a.rc, inv_rc = 1.f / a.rc;
     for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
         const int i = a.rows_atom[row];
@@ -221,11 +215,27 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
         const size_t base = (size_t)i * nl.cap;
         const float *__restrict__ Arow = a.A + (size_t)row * (P * M);
         const float *__restrict__ Grow = a.G + (size_t)row * S;
-        for (int e = lane; e < P * M; e += 64) {
-            const int p = e / M, m = e % M;
+        for (int e = lane; e < K * M; e += 64) {
+            const int q = e / M, m = e % M;
             int l = 0;
             while ((l + 1) * (l + 1) <= m) ++l;
-            sGA[e] = Grow[p * L1 + l] * Arow[e];
+            if (MIX == MIX_NONE) {
+                sGt[e] = Grow[q * L1 + l] * Arow[q * M + m];
+            } else if (MIX == MIX_DEFAULT) {
+                const float lo = Grow[q * L1 + l] * Arow[q * M + m];
+                const float hi = Grow[(K + q) * L1 + l] * Arow[(K + q) * M + m];
+                for (int t = 1; t <= nblk; ++t) {
+                    const float wt = (t & 1) ? (float)t : -(float)t;
+                    sGt[(t - 1) * K * M + e] = lo + wt * hi;
+                }
+            } else {
+                for (int t = 1; t <= nblk; ++t) {
+                    const float *w = a.W + (size_t)(t - 1) * P * K;
+                    float s = 0.f;
+                    for (int p = 0; p < P; ++p) s += w[p * K + q] * (Grow[p * L1 + l] * Arow[p * M + m]);
+                    sGt[(t - 1) * K * M + e] = s;
+                }
+            }
         }
         __syncthreads();
         for (int c0 = 0; c0 < n; c0 += 64) {
@@ -245,31 +255,29 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
                 const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
                 const float t2 = t * t;
                 const float fc = t2 * t2;
-                float rnp[P];
-                mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
-                // gY[m] = sum_p gA[p][m] c[p],  gc[p] = sum_m gA[p][m] Y[m]
+                const float *sG = sGt + (MIX == MIX_NONE ? 0 : (tj - 1) * K * M);
+                // gY[m] = fc sum_n Rn[n] Gt[n][m],  gct[n] = sum_m Gt[n][m] Y[m]
+                // (then dE/dRn = fc gct, dE/dfc = sum_n Rn[n] gct[n])
                 float gy[M];
 #pragma unroll
                 for (int m = 0; m < M; ++m) gy[m] = 0.f;
                 float gfc = 0.f;
-                float grnp[P];
+                float grn[K];
 #pragma unroll
-                for (int p = 0; p < P; ++p) {
-                    const float cp_ = fc * rnp[p];
+                for (int q = 0; q < K; ++q) {
+                    const float cq = fc * rn[q];
                     float s = 0.f;
 #pragma unroll
                     for (int m = 0; m < M; ++m) {
-                        const float ga = sGA[p * M + m];
-                        gy[m] += ga * cp_;
+                        const float ga = sG[q * M + m];
+                        gy[m] += ga * cq;
                         s += ga * y[m];
                     }
-                    gfc += s * rnp[p];
-                    grnp[p] = s * fc;
-                    // keep the LDS reads of one p-row from being hoisted across rows (register pressure)
+                    gfc += s * rn[q];
+                    grn[q] = s * fc;
+                    // keep the LDS reads of one row from being hoisted across rows (register pressure)
                     __builtin_amdgcn_sched_barrier(0);
                 }
-                float grn[K];
-                mix_rnp_back<K, P, MIX>(grnp, tj, a.W, grn);
                 float drn[K];
                 cheb_drn<K>(xr, inv_rc, drn);
                 float dEdr = 0.f;
@@ -307,13 +315,16 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
 // ---- dispatch table over the compiled shapes ----
 template <int LMAX, int K, int P, int MIX>
 static void run_fwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
-    const int grid = a.nrows < 262144 ? a.nrows : 262144;
+    int grid = a.nrows < 262144 ? a.nrows : 262144;
+    if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
     hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, a, nl);
 }
 template <int LMAX, int K, int P, int MIX>
 static void run_bwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
-    const int grid = a.nrows < 262144 ? a.nrows : 262144;
-    hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, a, nl);
+    int grid = a.nrows < 262144 ? a.nrows : 262144;
+    if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
+    const size_t lds = sizeof(float) * (size_t)bwd_blocks<MIX>(a.ntypes) * K * ((LMAX + 1) * (LMAX + 1));
+    hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX>), dim3(grid), dim3(64), lds, s, a, nl);
 }
 
 #define JNN_SHAPES(X)              \

@@ -33,10 +33,10 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-__global__ __launch_bounds__(256) void k_gather(int n, const int *__restrict__ atoms, NeighborList nl,
+__global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ atoms, NeighborList nl,
                                                 double *__restrict__ F, double *__restrict__ V, int want_virial) {
     const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * 4 + (threadIdx.x >> 6);
+    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (t >= n) return;
     const int i = atoms ? atoms[t] : t;
     const int ci = min(nl.cnt[i], nl.cap);
@@ -116,9 +116,10 @@ __global__ void k_rep_final(int nrep, const int *__restrict__ rep_chunk_off, con
 }
 
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
-                   int want_virial) {
+                   int want_virial, int waves) {
     if (n <= 0) return;
-    hipLaunchKernelGGL(k_gather, dim3((n + 3) / 4), dim3(256), 0, s, n, atoms, nl, F, V, want_virial);
+    const int w = (waves >= 1 && waves <= 8) ? waves : 4;
+    hipLaunchKernelGGL(k_gather, dim3((n + w - 1) / w), dim3(64 * w), 0, s, n, atoms, nl, F, V, want_virial);
 }
 void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
                            const double *E, const double *V, int want_virial, double *partial, double *out) {

@@ -7,6 +7,20 @@
 
 namespace jnn {
 
+// Compute units of the current device (cached after the first query).
+int device_cu_count();
+
+// Launch shapes of one system's kernels, chosen by the autotuner
+// (jse_nnap_system_autotune); 0 keeps a kernel's built-in default.
+struct LaunchTune {
+    int desc_fwd_wg_per_cu = 0;  // descriptor forward: persistent workgroups per CU (0: one per atom)
+    int desc_bwd_wg_per_cu = 0;  // descriptor backward: likewise
+    int mlp_wg_per_cu = 0;       // MLP: persistent workgroups per CU (0: 2)
+    int nl_wg_per_cu = 0;        // neighbour build: persistent workgroups per CU (0: one per atom)
+    int gather_waves = 0;        // force gather: atoms (waves) per workgroup (0: 4)
+};
+constexpr int kTuneKnobs = 5;
+
 // Periodic cell grid of one replica's orthogonal box (reference:
 // NeighborListGetter.java:952-1001).  A system holds one or more replicas;
 // their atoms and cells are numbered consecutively.
@@ -47,7 +61,7 @@ void launch_cell_scatter(hipStream_t s, int n, const int *atoms, const int *rep_
                          const int *rank, const int *cell_start, const double *xw, int *sidx, double *sxw);
 void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, const ReplicaGrid *reps,
                      const int *need, double rc2, const double *xw, const int *cell, const int *cell_start,
-                     const int *sidx, const double *sxw, NeighborList nl, int *overflow);
+                     const int *sidx, const double *sxw, NeighborList nl, int *overflow, int wg_per_cu = 0);
 void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl);
 
 // ---- descriptor / MLP / backward for one species segment ----
@@ -56,12 +70,14 @@ struct SegmentArgs {
     const int *rows_atom;   // atom index of each row (species-sorted order)
     const int *types;       // [N] 1-based types
     float rc;
+    int ntypes;             // species of the model (neighbour types are 1..ntypes)
     const float *W;         // [ntypes][P][K] FP32 mixing (used by the dense kernel)
     const float *mu, *sig;  // normalisation [S]
     const double *mu64, *sig64;  // the same in FP64 (descriptor normalisation)
     float *x;               // [nrows][S] normalised descriptors (forward out, MLP in)
     float *A;               // [nrows][P*M]
     float *G;               // [nrows][S]  2*(4pi/(2l+1)) * dE/dfp
+    int wg_per_cu;          // persistent grid size (0: one workgroup per row)
 };
 // returns false when no kernel is compiled for this (lmax, K, P, mix)
 bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);
@@ -80,6 +96,7 @@ struct MlpArgs {
     double *E;               // [N] per-atom energy (by atom index)
     float *G;                // [nrows][S]
     int want_grad;
+    int wg_per_cu;           // persistent grid size (0: 2 per CU)
 };
 void launch_mlp_fp32(hipStream_t s, const MlpArgs &a);
 // MFMA kernels for 2 hidden layers of supported widths; false when the shape has no compiled kernel
@@ -88,7 +105,7 @@ bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a);
 
 // ---- forces / virials / totals ----
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
-                   int want_virial);
+                   int want_virial, int waves = 0);
 // Per-replica totals [nrep][10] = (E, V[9]): fixed-size chunks of each
 // replica's atoms are summed (Neumaier, then a fixed tree) and the chunk sums
 // of a replica are added in order, so a replica's totals do not depend on

@@ -305,11 +305,10 @@ __global__ __launch_bounds__(256) void k_mlp2_mfma(MlpArgs a) {
 // ---- dispatch ----
 template <typename T, int C1, int C2, int KP>
 static void run_mlp2(hipStream_t s, const MlpArgs &a) {
-    int dev = 0, ncu = 256;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+    const int ncu = device_cu_count();
+    const int per = a.wg_per_cu > 0 ? a.wg_per_cu : 2;
     const int tiles = (a.nrows + 63) / 64;
-    const int grid = tiles < ncu * 2 ? tiles : ncu * 2;
+    const int grid = tiles < ncu * per ? tiles : ncu * per;
     hipLaunchKernelGGL((k_mlp2_mfma<T, C1, C2, KP>), dim3(grid), dim3(256), 0, s, a);
 }
 

@@ -16,6 +16,8 @@
 // scans the 27 neighbouring cells as one flattened candidate range, compacts
 // hits with a wave ballot into LDS, sorts them with a bitonic network and
 // writes one row.
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace jnn {
@@ -274,11 +276,25 @@ void launch_cell_scatter(hipStream_t s, int n, const int *atoms, const int *rep_
     hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, atoms, rep_of, need, cell, rank,
                        cell_start, xw, sidx, sxw);
 }
+int device_cu_count() {
+    static int ncu = 0;
+    if (!ncu) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess && v > 0)
+            ncu = v;
+        else
+            ncu = 256;
+    }
+    return ncu;
+}
+
 void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, const ReplicaGrid *reps,
                      const int *need, double rc2, const double *xw, const int *cell, const int *cell_start,
-                     const int *sidx, const double *sxw, NeighborList nl, int *overflow) {
+                     const int *sidx, const double *sxw, NeighborList nl, int *overflow, int wg_per_cu) {
     if (n <= 0) return;
-    const int grid = n < 65536 * 4 ? n : 65536 * 4;
+    int grid = n < 65536 * 4 ? n : 65536 * 4;
+    if (wg_per_cu > 0) grid = std::min(grid, wg_per_cu * device_cu_count());
     hipLaunchKernelGGL(k_nl_build, dim3(grid), dim3(kNlBlock), 0, s, n, atoms, rep_of, reps, need, rc2, xw, cell,
                        cell_start, sidx, sxw, nl, overflow);
 }

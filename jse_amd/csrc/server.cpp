@@ -594,6 +594,7 @@ struct jse_nnap_server_s {
         a.rows_atom = d_rows + seg[t];
         a.types = types;
         a.rc = (float)sp.rcut;
+        a.ntypes = model->host.ntypes();
         a.W = dv.W;
         a.mu = dv.mu;
         a.sig = dv.sig;

@@ -51,7 +51,7 @@ EXPORTS = [
     "jse_nnap_system_compute", "jse_nnap_system_get_results", "jse_nnap_system_compute_fp",
     "jse_nnap_system_get_neighbors", "jse_nnap_system_benchmark", "jse_nnap_system_nl_stats",
     "jse_nnap_system_get_pair_gradients", "jse_nnap_batch_create", "jse_nnap_system_get_replica_results",
-    "jse_nnap_system_set_graphs", "jse_nnap_server_create", "jse_nnap_server_add_replica", "jse_nnap_server_submit",
+    "jse_nnap_system_set_graphs", "jse_nnap_system_autotune", "jse_nnap_system_set_tuning", "jse_nnap_server_create", "jse_nnap_server_add_replica", "jse_nnap_server_submit",
     "jse_nnap_server_wait", "jse_nnap_server_collect", "jse_nnap_server_drain", "jse_nnap_server_stats",
     "jse_nnap_server_destroy",
 ]
@@ -107,6 +107,8 @@ def lib():
                                         ctypes.POINTER(vp)]
     L.jse_nnap_system_get_replica_results.argtypes = [vp, _c_dbl_p, _c_dbl_p]
     L.jse_nnap_system_set_graphs.argtypes = [vp, ctypes.c_int]
+    L.jse_nnap_system_autotune.argtypes = [vp, ctypes.c_int, ctypes.c_int, _c_int_p, _c_dbl_p]
+    L.jse_nnap_system_set_tuning.argtypes = [vp, _c_int_p, ctypes.c_int]
     L.jse_nnap_server_create.argtypes = [vp, ctypes.c_int, ctypes.c_double, ctypes.c_double, ctypes.c_int,
                                          ctypes.c_int, ctypes.c_double, ctypes.POINTER(vp)]
     L.jse_nnap_server_add_replica.argtypes = [vp, ctypes.c_int, _c_int_p, _c_dbl_p, _c_int_p]
@@ -267,6 +269,19 @@ class System:
         _check(lib().jse_nnap_system_nl_stats(self._h, ctypes.byref(c), ctypes.byref(mx), ctypes.byref(pairs),
                                               ctypes.byref(un)))
         return {"cap": c.value, "max_count": mx.value, "pairs": pairs.value, "unmatched": un.value}
+
+    TUNE_KNOBS = ["desc_fwd_wg_per_cu", "desc_bwd_wg_per_cu", "mlp_wg_per_cu", "nl_wg_per_cu", "gather_waves"]
+
+    def autotune(self, what: int = ENERGY | FORCE | VIRIAL, iters: int = 5):
+        """times candidate launch shapes on this system's data; returns (knobs, ms per evaluation)"""
+        t = np.zeros(len(self.TUNE_KNOBS), dtype=np.int32)
+        ms = ctypes.c_double()
+        _check(lib().jse_nnap_system_autotune(self._h, int(what), int(iters), _ip(t), ctypes.byref(ms)))
+        return dict(zip(self.TUNE_KNOBS, t.tolist())), ms.value
+
+    def set_tuning(self, knobs: Dict[str, int]):
+        t = np.array([int(knobs.get(k, 0)) for k in self.TUNE_KNOBS], dtype=np.int32)
+        _check(lib().jse_nnap_system_set_tuning(self._h, _ip(t), len(t)))
 
     def benchmark(self, what: int = ENERGY | FORCE | VIRIAL, warmup: int = 3, iters: int = 10):
         ms = ctypes.c_double()

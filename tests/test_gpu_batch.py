@@ -98,3 +98,29 @@ def test_graph_replay_matches_eager():
 @pytest.fixture(scope="module")
 def O(oracle_mod):
     return oracle_mod
+
+
+def test_autotuned_launch_shapes_keep_results():
+    """Launch shapes change which workgroup takes a row, never a row's
+    arithmetic: results after autotuning equal the default shapes' bit for bit."""
+    from jse_amd import autotune
+    md = W.ternary_model()
+    reps = W.replicas(24, 64, 1500, 29)
+    atoms = [nnap.AtomData(r.box, r.xyz, r.types) for r in reps]
+    what = nnap.ENERGY | nnap.FORCE | nnap.VIRIAL
+    with nnap.NNAP(md, precision=nnap.BF16_MLP) as pot:
+        b = pot.batch(atoms)
+        b.compute(what)
+        r0 = b.results()
+        knobs, ms, hit = autotune.tune(b, what, iters=2, use_cache=False)
+        print("tuned", knobs, ms)
+        # 0 = the kernel's built-in default shape kept
+        assert knobs["mlp_wg_per_cu"] in (0, 1, 2, 4, 8) and knobs["gather_waves"] in (0, 1, 2, 4, 8)
+        b.compute(what)
+        r1 = b.results()
+        for k in ("energies", "forces", "virials"):
+            assert np.array_equal(r0[k], r1[k]), k
+        # a cached tuning is applied without re-timing
+        knobs2, ms2, hit2 = autotune.tune(b, what)
+        assert hit2 and knobs2 == knobs
+        b.close()
