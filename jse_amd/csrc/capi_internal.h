@@ -71,9 +71,11 @@ void hfree(T *&p) {
     p = nullptr;
 }
 
-// atoms per chunk of the per-replica total reductions: a replica's totals are
-// summed over the same chunks whatever else is evaluated with it
-constexpr int kReduceChunk = 4096;
+// atoms per chunk of the per-replica total reductions (one workgroup of 256
+// threads x 4 atoms each): a replica's totals are summed over the same chunks
+// whatever else is evaluated with it
+constexpr int kReduceChunk = 1024;
+static_assert(kReduceChunk <= kTotalsChunkMax, "a totals chunk must fit one workgroup");
 
 // Cell grid of an orthogonal periodic box for cut-off `rcut`
 // (NeighborListGetter.getProperLinkedCell, NeighborListGetter.java:952-1001):

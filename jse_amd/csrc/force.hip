@@ -84,25 +84,43 @@ __global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ a
 }
 
 constexpr int kRedBlock = 256;
-constexpr int kComp = 10;  // E, V[9]
+constexpr int kComp = 10;      // E, V[9]
+constexpr int kPerThread = 4;  // atoms per thread: one chunk is at most kRedBlock * kPerThread atoms
+static_assert(kRedBlock * kPerThread == kTotalsChunkMax, "totals chunk size");
+// One workgroup per chunk (replica, first, end): every thread loads its (at
+// most 4) atoms' E and V at once, sums them with Neumaier compensation in
+// atom order, and the workgroup combines the 10 sums in a fixed tree.
 __global__ __launch_bounds__(kRedBlock) void k_rep_partial(const int4 *__restrict__ chunks, const double *__restrict__ E,
                                                           const double *__restrict__ V, int want_virial,
                                                           double *__restrict__ partial) {
-    __shared__ double sh[kRedBlock];
+    __shared__ double sh[kComp][kRedBlock];
     const int4 ch = chunks[blockIdx.x];  // (replica, first atom, end atom, -)
-    const int nc = want_virial ? kComp : 1;
-    for (int c = 0; c < nc; ++c) {
+    const int tid = threadIdx.x;
+    double v[kPerThread][kComp];
+#pragma unroll
+    for (int u = 0; u < kPerThread; ++u) {
+        const int i = ch.y + tid + kRedBlock * u;
+        const bool on = i < ch.z;
+        v[u][0] = on ? E[i] : 0.0;
+#pragma unroll
+        for (int c = 1; c < kComp; ++c) v[u][c] = (on && want_virial) ? V[9 * (size_t)i + c - 1] : 0.0;
+    }
+#pragma unroll
+    for (int c = 0; c < kComp; ++c) {
         Neum acc;
-        for (int i = ch.y + threadIdx.x; i < ch.z; i += kRedBlock) acc.add(c == 0 ? E[i] : V[9 * (size_t)i + c - 1]);
-        sh[threadIdx.x] = acc.get();
-        __syncthreads();
-        for (int off = kRedBlock / 2; off > 0; off >>= 1) {
-            if (threadIdx.x < off) sh[threadIdx.x] += sh[threadIdx.x + off];
-            __syncthreads();
+#pragma unroll
+        for (int u = 0; u < kPerThread; ++u) acc.add(v[u][c]);
+        sh[c][tid] = acc.get();
+    }
+    __syncthreads();
+    for (int off = kRedBlock / 2; off > 0; off >>= 1) {
+        if (tid < off) {
+#pragma unroll
+            for (int c = 0; c < kComp; ++c) sh[c][tid] += sh[c][tid + off];
         }
-        if (threadIdx.x == 0) partial[(size_t)blockIdx.x * kComp + c] = sh[0];
         __syncthreads();
     }
+    if (tid < kComp) partial[(size_t)blockIdx.x * kComp + tid] = sh[tid][0];
 }
 __global__ void k_rep_final(int nrep, const int *__restrict__ rep_chunk_off, const double *__restrict__ partial,
                             int want_virial, double *__restrict__ out) {

@@ -106,6 +106,8 @@ bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a);
 // ---- forces / virials / totals ----
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
                    int want_virial, int waves = 0);
+// Largest chunk the totals kernel sums in one workgroup.
+constexpr int kTotalsChunkMax = 1024;
 // Per-replica totals [nrep][10] = (E, V[9]): fixed-size chunks of each
 // replica's atoms are summed (Neumaier, then a fixed tree) and the chunk sums
 // of a replica are added in order, so a replica's totals do not depend on

@@ -81,6 +81,26 @@ __device__ __forceinline__ float silu_grad(float z, float *g) {
     return z * s;
 }
 
+// Stages NTOT elements with the workgroup's 256 threads: every thread first
+// issues the loads of all its ceil(NTOT/256) elements (all in flight at once),
+// then converts and stores them, so staging costs one memory latency, not one
+// per element.
+template <int NTOT, typename Ld, typename St>
+__device__ __forceinline__ void stage(int tid, Ld ld, St st) {
+    constexpr int PT = (NTOT + 255) / 256;
+    float v[PT];
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+        const int e = tid + 256 * u;
+        v[u] = e < NTOT ? ld(e) : 0.f;
+    }
+#pragma unroll
+    for (int u = 0; u < PT; ++u) {
+        const int e = tid + 256 * u;
+        if (e < NTOT) st(e, v[u]);
+    }
+}
+
 __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
     __builtin_amdgcn_wave_barrier();
@@ -126,27 +146,33 @@ __global__ __launch_bounds__(256) void k_mlp2_mfma(MlpArgs a) {
     const float bo = a.bias[H1 + H2];
     const float *__restrict__ wo = a.w_in_out + (size_t)S * H1 + (size_t)H1 * H2;
 
-    for (int e = tid; e < H1 * XS; e += 256) {
-        const int j = e / XS, k = e % XS;
-        const float w = k < S ? W1g[(size_t)j * S + k] : 0.f;
-        W1s[e] = MM::cvt(w);
-        if constexpr (SP) W1l[e] = MM::lo(w);
-    }
-    for (int e = tid; e < H2 * HS; e += 256) {
-        const int j = e / HS, k = e % HS;
-        const float w = k < H1 ? W2g[(size_t)j * H1 + k] : 0.f;
-        W2s[e] = MM::cvt(w);
-        if constexpr (SP) W2l[e] = MM::lo(w);
-    }
+    stage<H1 * XS>(
+        tid, [&](int e) { const int j = e / XS, k = e % XS; return k < S ? W1g[(size_t)j * S + k] : 0.f; },
+        [&](int e, float w) {
+            W1s[e] = MM::cvt(w);
+            if constexpr (SP) W1l[e] = MM::lo(w);
+        });
+    stage<H2 * HS>(
+        tid, [&](int e) { const int j = e / HS, k = e % HS; return k < H1 ? W2g[(size_t)j * H1 + k] : 0.f; },
+        [&](int e, float w) {
+            W2s[e] = MM::cvt(w);
+            if constexpr (SP) W2l[e] = MM::lo(w);
+        });
     if constexpr (TC) {
-        for (int e = tid; e < KP * HS; e += 256) {
-            const int k = e / HS, j = e % HS;
-            W1t[e] = MM::cvt((k < S && j < H1) ? W1g[(size_t)j * S + k] : 0.f);
-        }
-        for (int e = tid; e < H1 * HS; e += 256) {
-            const int j1 = e / HS, j2 = e % HS;
-            W2t[e] = MM::cvt(j2 < H2 ? W2g[(size_t)j2 * H1 + j1] : 0.f);
-        }
+        stage<KP * HS>(
+            tid,
+            [&](int e) {
+                const int k = e / HS, j = e % HS;
+                return (k < S && j < H1) ? W1g[(size_t)j * S + k] : 0.f;
+            },
+            [&](int e, float w) { W1t[e] = MM::cvt(w); });
+        stage<H1 * HS>(
+            tid,
+            [&](int e) {
+                const int j1 = e / HS, j2 = e % HS;
+                return j2 < H2 ? W2g[(size_t)j2 * H1 + j1] : 0.f;
+            },
+            [&](int e, float w) { W2t[e] = MM::cvt(w); });
     }
     const int r0 = 16 * wave;
     T *hrow_w = Hs + r0 * HS;                         // this wave's 16 rows of H
@@ -155,12 +181,16 @@ __global__ __launch_bounds__(256) void k_mlp2_mfma(MlpArgs a) {
         const int row0 = tile * 64;
         const int nt = min(64, a.nrows - row0);
         __syncthreads();
-        for (int e = tid; e < 64 * XS; e += 256) {
-            const int i = e / XS, k = e % XS;
-            const float v = (i < nt && k < S) ? a.x[(size_t)(row0 + i) * S + k] : 0.f;
-            Xs[e] = MM::cvt(v);
-            if constexpr (SP) Xl[e] = MM::lo(v);
-        }
+        stage<64 * XS>(
+            tid,
+            [&](int e) {
+                const int i = e / XS, k = e % XS;
+                return (i < nt && k < S) ? a.x[(size_t)(row0 + i) * S + k] : 0.f;
+            },
+            [&](int e, float v) {
+                Xs[e] = MM::cvt(v);
+                if constexpr (SP) Xl[e] = MM::lo(v);
+            });
         __syncthreads();
         // ---- layer 1 ----
         f32x4 acc1[C1];
