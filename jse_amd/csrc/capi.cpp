@@ -144,7 +144,7 @@ struct jse_nnap_system_s {
     int *d_cell = nullptr, *d_rank = nullptr, *d_sidx = nullptr, *d_types = nullptr, *d_rows = nullptr;
     int *d_rep_of = nullptr;
     ReplicaGrid *d_reps = nullptr;
-    int *d_cell_cnt = nullptr, *d_cell_start = nullptr;
+    int *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_rep = nullptr;
     int ncell_alloc = 0;
     int *d_overflow = nullptr;
     NeighborList nl{};
@@ -165,7 +165,7 @@ struct jse_nnap_system_s {
         drop_graphs();
         free_nl();
         dfree(d_xyz); dfree(d_xw); dfree(d_sxw); dfree(d_cell); dfree(d_rank); dfree(d_sidx); dfree(d_types);
-        dfree(d_rows); dfree(d_rep_of); dfree(d_reps); dfree(d_cell_cnt); dfree(d_cell_start); dfree(d_overflow);
+        dfree(d_rows); dfree(d_rep_of); dfree(d_reps); dfree(d_cell_cnt); dfree(d_cell_start); dfree(d_cell_rep); dfree(d_overflow);
         dfree(d_x); dfree(d_A); dfree(d_G); dfree(d_E); dfree(d_F); dfree(d_V); dfree(d_partial); dfree(d_tot);
         dfree(d_chunks); dfree(d_rep_chunk);
         for (auto &e : ev)
@@ -213,12 +213,18 @@ struct jse_nnap_system_s {
         if (ncell_total + 1 > ncell_alloc) {
             dfree(d_cell_cnt);
             dfree(d_cell_start);
+            dfree(d_cell_rep);
             ncell_alloc = ncell_total + 1;
             d_cell_cnt = dalloc<int>(ncell_alloc);
             d_cell_start = dalloc<int>(ncell_alloc + 1);
+            d_cell_rep = dalloc<int>(ncell_alloc);
         }
+        std::vector<int> cell_rep(ncell_total, -1);
+        for (int q = 0; q < nrep; ++q)
+            std::fill(cell_rep.begin() + r[q].cell_off, cell_rep.begin() + r[q].cell_off + r[q].ncell, q);
         if (!d_reps) d_reps = dalloc<ReplicaGrid>(nrep);
         JNN_HIP(hipMemcpy(d_reps, reps.data(), sizeof(ReplicaGrid) * nrep, hipMemcpyHostToDevice));
+        if (ncell_total) JNN_HIP(hipMemcpy(d_cell_rep, cell_rep.data(), sizeof(int) * ncell_total, hipMemcpyHostToDevice));
         drop_graphs();
         have_nl = false;
     }
@@ -310,8 +316,8 @@ struct jse_nnap_system_s {
         launch_wrap_bin(stream, n, nullptr, d_xyz, d_rep_of, d_reps, nullptr, d_xw, d_cell, d_cell_cnt, d_rank);
         launch_cell_scan(stream, nrep, nullptr, d_reps, nullptr, d_cell_cnt, d_cell_start);
         launch_cell_scatter(stream, n, nullptr, d_rep_of, nullptr, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw);
-        launch_nl_build(stream, n, nullptr, d_rep_of, d_reps, nullptr, rc2, d_xw, d_cell, d_cell_start, d_sidx, d_sxw,
-                        nl, d_overflow, tune.nl_wg_per_cu);
+        launch_nl_build_cells(stream, ncell_total, d_cell_rep, d_reps, rc2, d_cell_start, d_sidx, d_sxw, nl, d_overflow,
+                              tune.nl_wg_per_cu);
         if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
         launch_nl_reverse(stream, n, nullptr, nl);
         if (timed) JNN_HIP(hipEventRecord(ev[2], stream));

@@ -62,6 +62,12 @@ void launch_cell_scatter(hipStream_t s, int n, const int *atoms, const int *rep_
 void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, const ReplicaGrid *reps,
                      const int *need, double rc2, const double *xw, const int *cell, const int *cell_start,
                      const int *sidx, const double *sxw, NeighborList nl, int *overflow, int wg_per_cu = 0);
+// The same rows for every atom of a system, one workgroup per cell (27-cell
+// candidates staged in LDS once per cell); cell_rep[c] = replica of global
+// cell c, -1 for a replica's end entry.
+void launch_nl_build_cells(hipStream_t s, int ncell_total, const int *cell_rep, const ReplicaGrid *reps, double rc2,
+                           const int *cell_start, const int *sidx, const double *sxw, NeighborList nl, int *overflow,
+                           int wg_per_cu = 0);
 void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl);
 
 // ---- descriptor / MLP / backward for one species segment ----

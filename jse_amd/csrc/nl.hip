@@ -99,6 +99,18 @@ __global__ void k_cell_scatter(int n, const int *__restrict__ atoms, const int *
     sxw[3 * slot + 2] = xw[3 * i + 2];
 }
 
+// neighbour cell w (0..26) holding flattened candidate q: the last w with s_pref[w] <= q
+__device__ __forceinline__ int cand_cell(const int *s_pref, int q) {
+    int lo = 0, hi = 26;
+#pragma unroll
+    for (int it = 0; it < 5; ++it) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (s_pref[mid] <= q) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ double dist2_no_fma(double dx, double dy, double dz) {
 #pragma clang fp contract(off)
     const double a = dx * dx;
@@ -156,8 +168,7 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
             int key = 0;
             float fdx = 0.f, fdy = 0.f, fdz = 0.f;
             if (q < total) {
-                int w = 0;
-                while (s_pref[w + 1] <= q) ++w;
+                const int w = cand_cell(s_pref, q);
                 const int slot = s_start[w] + (q - s_pref[w]);
                 const int j = sidx[slot];
                 const int code = s_code[w];
@@ -224,6 +235,203 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
             if (hits > nl.cap) atomicMax(overflow, hits);
         }
         __syncthreads();
+    }
+}
+
+// ---- cell-tile build: one workgroup per cell ----------------------------
+// The atoms of one cell share its 27-cell neighbourhood: the workgroup stages
+// those candidates once in LDS (FP64 positions with the periodic shift
+// already added, exactly as the per-atom build forms x_j + shift, and the key
+// j*32 + code), then each wave tests one member atom at a time against the
+// staged block, compacts its hits with a ballot and sorts them (in registers
+// for <= 64 hits).  A neighbourhood larger than the staging buffer is read
+// from global memory instead (same arithmetic).
+constexpr int kCellWaves = 4;
+constexpr int kStage = 512;
+
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
+
+// ascending bitonic sort of 64 (key, d) pairs, one per lane
+__device__ __forceinline__ void wave_sort64(int &key, float &dx, float &dy, float &dz, int lane) {
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            const int ok = __shfl_xor(key, j);
+            const float ox = __shfl_xor(dx, j), oy = __shfl_xor(dy, j), oz = __shfl_xor(dz, j);
+            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
+            const bool take = keep_min ? (ok < key) : (ok > key);
+            if (take) {
+                key = ok;
+                dx = ox;
+                dy = oy;
+                dz = oz;
+            }
+        }
+    }
+}
+
+__global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
+    int ncell_total, const int *__restrict__ cell_rep, const ReplicaGrid *__restrict__ reps, double rc2,
+    const int *__restrict__ cstart, const int *__restrict__ sidx, const double *__restrict__ sxw, NeighborList nl,
+    int *__restrict__ overflow) {
+    __shared__ int s_start[27], s_cnt[27], s_code[27], s_pref[28];
+    __shared__ double s_cx[kStage], s_cy[kStage], s_cz[kStage];
+    __shared__ int s_ckey[kStage];
+    __shared__ int s_hkey[kCellWaves][kNlLds];
+    __shared__ float s_hd[kCellWaves][3][kNlLds];
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    for (int c = blockIdx.x; c < ncell_total; c += gridDim.x) {
+        const int r = cell_rep[c];
+        if (r < 0) continue;  // a replica's end entry, not a cell
+        const int cs = cstart[c], ce = cstart[c + 1];
+        if (cs == ce) continue;
+        const ReplicaGrid &g = reps[r];
+        const int lc = c - g.cell_off;
+        const int cx = lc % g.size[0], cy = (lc / g.size[0]) % g.size[1], cz = lc / (g.size[0] * g.size[1]);
+        if (tid < 27) {
+            const int w = tid;
+            int ix = cx + (w % 3) - 1, iy = cy + (w / 3) % 3 - 1, iz = cz + w / 9 - 1;
+            int sx = 0, sy = 0, sz = 0;
+            if (ix >= g.size[0]) { ix -= g.size[0]; sx = 1; } else if (ix < 0) { ix += g.size[0]; sx = -1; }
+            if (iy >= g.size[1]) { iy -= g.size[1]; sy = 1; } else if (iy < 0) { iy += g.size[1]; sy = -1; }
+            if (iz >= g.size[2]) { iz -= g.size[2]; sz = 1; } else if (iz < 0) { iz += g.size[2]; sz = -1; }
+            const int cc = g.cell_off + ix + g.size[0] * (iy + g.size[1] * iz);
+            s_start[w] = cstart[cc];
+            s_cnt[w] = cstart[cc + 1] - cstart[cc];
+            s_code[w] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+        }
+        __syncthreads();
+        if (tid == 0) {
+            int run = 0;
+            for (int w = 0; w < 27; ++w) {
+                s_pref[w] = run;
+                run += s_cnt[w];
+            }
+            s_pref[27] = run;
+        }
+        __syncthreads();
+        const int total = s_pref[27];
+        const bool staged = total <= kStage;
+        // candidate q: its key and FP64 shifted position
+        auto fetch = [&](int q, int &key, double &xj, double &yj, double &zj) {
+            const int w = cand_cell(s_pref, q);
+            const int slot = s_start[w] + (q - s_pref[w]);
+            const int code = s_code[w];
+            const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
+            xj = sxw[3 * (size_t)slot];
+            yj = sxw[3 * (size_t)slot + 1];
+            zj = sxw[3 * (size_t)slot + 2];
+            if (sx) xj = xj + (sx > 0 ? g.box[0] : -g.box[0]);
+            if (sy) yj = yj + (sy > 0 ? g.box[1] : -g.box[1]);
+            if (sz) zj = zj + (sz > 0 ? g.box[2] : -g.box[2]);
+            key = sidx[slot] * 32 + code;
+        };
+        if (staged) {
+            for (int q = tid; q < total; q += 64 * kCellWaves) {
+                int key;
+                double xj, yj, zj;
+                fetch(q, key, xj, yj, zj);
+                s_cx[q] = xj;
+                s_cy[q] = yj;
+                s_cz[q] = zj;
+                s_ckey[q] = key;
+            }
+        }
+        __syncthreads();
+        int *hkey = s_hkey[wv];
+        float *hdx = s_hd[wv][0], *hdy = s_hd[wv][1], *hdz = s_hd[wv][2];
+        for (int sl = cs + wv; sl < ce; sl += kCellWaves) {
+            const int i = sidx[sl];
+            const double xi = sxw[3 * (size_t)sl], yi = sxw[3 * (size_t)sl + 1], zi = sxw[3 * (size_t)sl + 2];
+            int hits = 0;
+            for (int q0 = 0; q0 < total; q0 += 64) {
+                const int q = q0 + lane;
+                bool hit = false;
+                int key = 0;
+                float fdx = 0.f, fdy = 0.f, fdz = 0.f;
+                if (q < total) {
+                    double xj, yj, zj;
+                    if (staged) {
+                        key = s_ckey[q];
+                        xj = s_cx[q];
+                        yj = s_cy[q];
+                        zj = s_cz[q];
+                    } else {
+                        fetch(q, key, xj, yj, zj);
+                    }
+                    const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
+                    const double d2 = dist2_no_fma(dx, dy, dz);
+                    hit = (d2 < rc2) && !((key >> 5) == i && (key & 31) == 13);
+                    fdx = (float)dx;
+                    fdy = (float)dy;
+                    fdz = (float)dz;
+                }
+                const unsigned long long mask = __ballot(hit);
+                const unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+                const int pos = hits + __popcll(below);
+                if (hit && pos < kNlLds) {
+                    hkey[pos] = key;
+                    hdx[pos] = fdx;
+                    hdy[pos] = fdy;
+                    hdz[pos] = fdz;
+                }
+                hits += __popcll(mask);
+            }
+            wave_lds_sync();
+            const int nst = hits < kNlLds ? hits : kNlLds;
+            const int nw = nst < nl.cap ? nst : nl.cap;
+            const size_t row = (size_t)i * nl.cap;
+            if (nst <= 64) {
+                int key = lane < nst ? hkey[lane] : 0x7fffffff;
+                float dx = lane < nst ? hdx[lane] : 0.f;
+                float dy = lane < nst ? hdy[lane] : 0.f;
+                float dz = lane < nst ? hdz[lane] : 0.f;
+                wave_sort64(key, dx, dy, dz, lane);
+                if (lane < nw) {
+                    nl.key[row + lane] = key;
+                    if (nl.d) nl.d[row + lane] = make_float4(dx, dy, dz, 0.f);
+                }
+            } else {
+                int p2 = 128;
+                while (p2 < nst) p2 <<= 1;
+                for (int t = nst + lane; t < p2; t += 64) hkey[t] = 0x7fffffff;
+                wave_lds_sync();
+                for (int k = 2; k <= p2; k <<= 1) {
+                    for (int jj = k >> 1; jj > 0; jj >>= 1) {
+                        for (int t = lane; t < p2; t += 64) {
+                            const int u = t ^ jj;
+                            if (u > t) {
+                                const bool up = (t & k) == 0;
+                                const int ka = hkey[t], kb = hkey[u];
+                                if ((ka > kb) == up) {
+                                    hkey[t] = kb;
+                                    hkey[u] = ka;
+                                    float tmp = hdx[t]; hdx[t] = hdx[u]; hdx[u] = tmp;
+                                    tmp = hdy[t]; hdy[t] = hdy[u]; hdy[u] = tmp;
+                                    tmp = hdz[t]; hdz[t] = hdz[u]; hdz[u] = tmp;
+                                }
+                            }
+                        }
+                        wave_lds_sync();
+                    }
+                }
+                for (int t = lane; t < nw; t += 64) {
+                    nl.key[row + t] = hkey[t];
+                    if (nl.d) nl.d[row + t] = make_float4(hdx[t], hdy[t], hdz[t], 0.f);
+                }
+            }
+            if (lane == 0) {
+                nl.cnt[i] = hits;
+                if (hits > nl.cap) atomicMax(overflow, hits);
+            }
+            wave_lds_sync();  // this wave's hit buffer is reused for its next atom
+        }
+        __syncthreads();  // the staged block is reused for the next cell
     }
 }
 
@@ -296,6 +504,15 @@ void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, 
     int grid = n < 65536 * 4 ? n : 65536 * 4;
     if (wg_per_cu > 0) grid = std::min(grid, wg_per_cu * device_cu_count());
     hipLaunchKernelGGL(k_nl_build, dim3(grid), dim3(kNlBlock), 0, s, n, atoms, rep_of, reps, need, rc2, xw, cell,
+                       cell_start, sidx, sxw, nl, overflow);
+}
+void launch_nl_build_cells(hipStream_t s, int ncell_total, const int *cell_rep, const ReplicaGrid *reps, double rc2,
+                           const int *cell_start, const int *sidx, const double *sxw, NeighborList nl, int *overflow,
+                           int wg_per_cu) {
+    if (ncell_total <= 0) return;
+    int grid = ncell_total;
+    if (wg_per_cu > 0) grid = std::min(grid, wg_per_cu * device_cu_count());
+    hipLaunchKernelGGL(k_nl_build_cells, dim3(grid), dim3(64 * kCellWaves), 0, s, ncell_total, cell_rep, reps, rc2,
                        cell_start, sidx, sxw, nl, overflow);
 }
 void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl) {
