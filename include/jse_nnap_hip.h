@@ -126,8 +126,8 @@ int jse_nnap_system_get_pair_gradients(jse_nnap_system s, int cap, float *g);
 /* Timing of `iters` compute() calls after `warmup` untimed ones, with HIP
  * events on the system's stream.  ms_step = device time per call of `iters`
  * back-to-back calls; phase_ms[k] = average device time of phase k per call
- * from a second pass of single calls, k = 0 neighbour build, 1 reverse
- * index, 2 descriptor forward, 3 MLP, 4 descriptor backward,
+ * from a second pass of single calls, k = 0 neighbour build, 1 (empty: the
+ * force gather finds reverse pairs itself), 2 descriptor forward, 3 MLP, 4 descriptor backward,
  * 5 forces/virials/totals (nphase <= 6). */
 int jse_nnap_system_benchmark(jse_nnap_system s, int what, int warmup, int iters, double *ms_step, double *phase_ms,
                               int nphase);

@@ -319,7 +319,7 @@ struct jse_nnap_system_s {
         launch_nl_build_cells(stream, ncell_total, d_cell_rep, d_reps, rc2, d_cell_start, d_sidx, d_sxw, nl, d_overflow,
                               tune.nl_wg_per_cu);
         if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
-        launch_nl_reverse(stream, n, nullptr, nl);
+        // (the force gather finds reverse pairs itself; phase 1 stays empty)
         if (timed) JNN_HIP(hipEventRecord(ev[2], stream));
         JNN_HIP(hipMemcpyAsync(h_overflow, d_overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
     }
@@ -855,6 +855,7 @@ int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long l
         }
         s->finish();
         const int n = s->n, c = s->nl.cap;
+        launch_nl_reverse(s->stream, n, nullptr, s->nl);
         std::vector<int> cnt(n), rev((size_t)n * c);
         if (n) {
             JNN_HIP(hipMemcpyAsync(cnt.data(), s->nl.cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));

@@ -5,7 +5,8 @@
 // (src/nnap/java/jsex/nnap/NNAP.java:575-592, src/main/java/jse/atom/
 // IPairPotential.java:938-1006).  Here every atom gathers instead of being
 // scattered into: the incoming g_{j->i} is read through the reverse-pair slot
-// of the neighbour list, so the sums are deterministic and need no atomics;
+// of the neighbour list (found by a binary search of j's sorted row), so the
+// sums are deterministic and need no atomics;
 // each lane sums its slots in FP64 and the wave combines lanes in a fixed
 // butterfly, the totals use Neumaier-compensated FP64 partial sums.  The
 // virial of an incoming pair uses that pair's displacement from j's row, the
@@ -48,9 +49,10 @@ __global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ a
         f[0] -= (double)go.x;
         f[1] -= (double)go.y;
         f[2] -= (double)go.z;
-        const int r = nl.rev[base + k];
+        const int key = nl.key[base + k];
+        const int r = find_reverse(nl, i, key);  // binary search in j's sorted row
         if (r < 0) continue;
-        const int j = nl.key[base + k] >> 5;
+        const int j = key >> 5;
         const size_t sj = (size_t)j * nl.cap + r;
         const float4 gi = nl.g[sj];
         const double gx = (double)gi.x, gy = (double)gi.y, gz = (double)gi.z;

@@ -42,9 +42,27 @@ struct NeighborList {
     int *cnt = nullptr;     // [N] true neighbour count (may exceed cap -> overflow)
     int *key = nullptr;     // [N*cap]
     float4 *d = nullptr;    // [N*cap] (dx, dy, dz, 0): FP32 of the FP64 displacement r_j + shift - r_i
-    int *rev = nullptr;     // [N*cap] slot of the reverse pair in j's row, -1 if absent
+    int *rev = nullptr;     // [N*cap] slot of the reverse pair in j's row, -1 if absent (statistics only)
     float4 *g = nullptr;    // [N*cap] (gx, gy, gz, 0): g_j = -dE_i/d(r_j - r_i)
 };
+
+// Slot of the reverse pair (i, -shift) of i's pair `key` in j's sorted row,
+// -1 if j's row does not hold it.
+__device__ __forceinline__ int find_reverse(const NeighborList &nl, int i, int key) {
+    const int j = key >> 5, code = key & 31;
+    const int want = i * 32 + (26 - code);
+    const int cj = min(nl.cnt[j], nl.cap);
+    const int *rj = nl.key + (size_t)j * nl.cap;
+    int lo = 0, hi = cj - 1;
+    while (lo <= hi) {
+        const int mid = (lo + hi) >> 1;
+        const int v = rj[mid];
+        if (v == want) return mid;
+        if (v < want) lo = mid + 1;
+        else hi = mid - 1;
+    }
+    return -1;
+}
 
 // ---- neighbour build ----
 void launch_fill_int(hipStream_t s, int n, int *p, int v);  // kernel fill (graph-capture safe)
@@ -83,7 +101,7 @@ struct SegmentArgs {
     float *x;               // [nrows][S] normalised descriptors (forward out, MLP in)
     float *A;               // [nrows][P*M]
     float *G;               // [nrows][S]  2*(4pi/(2l+1)) * dE/dfp
-    int wg_per_cu;          // persistent grid size (0: one workgroup per row)
+    int wg_per_cu;          // persistent workgroups per CU (0: one workgroup per row)
 };
 // returns false when no kernel is compiled for this (lmax, K, P, mix)
 bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);

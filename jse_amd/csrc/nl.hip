@@ -443,19 +443,7 @@ __global__ void k_nl_reverse(int n, const int *__restrict__ atoms, NeighborList 
     const int i = atoms ? atoms[ti] : ti;
     const int ci = min(nl.cnt[i], nl.cap);
     if (k >= ci) return;
-    const int key = nl.key[(size_t)i * nl.cap + k];
-    const int j = key >> 5, code = key & 31;
-    const int want = i * 32 + (26 - code);
-    const int cj = min(nl.cnt[j], nl.cap);
-    const int *rj = nl.key + (size_t)j * nl.cap;
-    int lo = 0, hi = cj - 1, found = -1;
-    while (lo <= hi) {
-        const int mid = (lo + hi) >> 1;
-        const int v = rj[mid];
-        if (v == want) { found = mid; break; }
-        if (v < want) lo = mid + 1; else hi = mid - 1;
-    }
-    nl.rev[(size_t)i * nl.cap + k] = found;
+    nl.rev[(size_t)i * nl.cap + k] = find_reverse(nl, i, nl.key[(size_t)i * nl.cap + k]);
 }
 
 __global__ void k_fill_int(int n, int *__restrict__ p, int v) {

@@ -166,7 +166,7 @@ struct jse_nnap_server_s {
     // ------------------------------------------------------------------ memory
     size_t slot_bytes() const {
         return sizeof(double) * (15 + 13) + sizeof(int) * 5 + sizeof(int) * (1 + cap_v) +
-               (sizeof(int) * 3 + sizeof(float4) * 2) * cap_w + sizeof(int);
+               (sizeof(int) * 2 + sizeof(float4) * 2) * cap_w + sizeof(int);
     }
     void free_arena() {
         dfree(x); dfree(xb); dfree(xwb); dfree(xw); dfree(sxw); dfree(E); dfree(F); dfree(V);
@@ -197,7 +197,6 @@ struct jse_nnap_server_s {
         nl.cnt = dalloc<int>(n);
         nl.key = dalloc<int>((size_t)n * cap_w);
         nl.d = dalloc<float4>((size_t)n * cap_w);
-        nl.rev = dalloc<int>((size_t)n * cap_w);
         nl.g = dalloc<float4>((size_t)n * cap_w);
         arena_bytes = (size_t)((double)n * slot_bytes() + (double)cells * 2 * sizeof(int));
         atom_alloc.reset(n);
@@ -558,7 +557,6 @@ struct jse_nnap_server_s {
         // rows within rc at the current positions
         launch_srv_update(s, na, d_atoms, rep_of, d_need, x, xb, xwb, xw);
         launch_srv_refresh(s, na, d_atoms, rep_of, d_reps, rc * rc, xw, vl, nl, L.d_flags + 1);
-        launch_nl_reverse(s, na, d_atoms, nl);
         // descriptor, MLP, backward per species segment of the batch rows
         for (int t = 0; t < nt; ++t) {
             const SpeciesHost &sp = model->host.species[t];
