@@ -143,6 +143,7 @@ struct jse_nnap_system_s {
     double *d_xyz = nullptr, *d_xw = nullptr, *d_sxw = nullptr;
     int *d_cell = nullptr, *d_rank = nullptr, *d_sidx = nullptr, *d_types = nullptr, *d_rows = nullptr;
     int *d_rep_of = nullptr;
+    int *d_slot_of = nullptr;  // atom -> its neighbour-list row (cell-sorted slot)
     ReplicaGrid *d_reps = nullptr;
     int *d_cell_cnt = nullptr, *d_cell_start = nullptr, *d_cell_rep = nullptr;
     int ncell_alloc = 0;
@@ -165,7 +166,7 @@ struct jse_nnap_system_s {
         drop_graphs();
         free_nl();
         dfree(d_xyz); dfree(d_xw); dfree(d_sxw); dfree(d_cell); dfree(d_rank); dfree(d_sidx); dfree(d_types);
-        dfree(d_rows); dfree(d_rep_of); dfree(d_reps); dfree(d_cell_cnt); dfree(d_cell_start); dfree(d_cell_rep); dfree(d_overflow);
+        dfree(d_rows); dfree(d_rep_of); dfree(d_slot_of); dfree(d_reps); dfree(d_cell_cnt); dfree(d_cell_start); dfree(d_cell_rep); dfree(d_overflow);
         dfree(d_x); dfree(d_A); dfree(d_G); dfree(d_E); dfree(d_F); dfree(d_V); dfree(d_partial); dfree(d_tot);
         dfree(d_chunks); dfree(d_rep_chunk);
         for (auto &e : ev)
@@ -282,6 +283,7 @@ struct jse_nnap_system_s {
         d_cell = dalloc<int>(n);
         d_rank = dalloc<int>(n);
         d_sidx = dalloc<int>(n);
+        d_slot_of = dalloc<int>(n);
         d_overflow = dalloc<int>(1);
         JNN_HIP(hipHostMalloc((void **)&h_overflow, sizeof(int), hipHostMallocDefault));
         *h_overflow = 0;
@@ -315,7 +317,8 @@ struct jse_nnap_system_s {
         launch_fill_int(stream, 1, d_overflow, 0);
         launch_wrap_bin(stream, n, nullptr, d_xyz, d_rep_of, d_reps, nullptr, d_xw, d_cell, d_cell_cnt, d_rank);
         launch_cell_scan(stream, nrep, nullptr, d_reps, nullptr, d_cell_cnt, d_cell_start);
-        launch_cell_scatter(stream, n, nullptr, d_rep_of, nullptr, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw);
+        launch_cell_scatter(stream, n, nullptr, d_rep_of, nullptr, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw,
+                            d_slot_of);
         launch_nl_build_cells(stream, ncell_total, d_cell_rep, d_reps, rc2, d_cell_start, d_sidx, d_sxw, nl, d_overflow,
                               tune.nl_wg_per_cu);
         if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
@@ -347,6 +350,7 @@ struct jse_nnap_system_s {
         SegmentArgs a{};
         a.nrows = seg[t + 1] - seg[t];
         a.rows_atom = d_rows + seg[t];
+        a.nl_of = d_slot_of;  // neighbour rows sit in cell-sorted slot order
         a.types = d_types;
         a.rc = (float)sp.rcut;
         a.ntypes = model->host.ntypes();
@@ -394,10 +398,44 @@ struct jse_nnap_system_s {
             }
         }
         if (timed) JNN_HIP(hipEventRecord(ev[5], stream));
-        if (grad) launch_gather(stream, n, nullptr, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0, tune.gather_waves);
+        if (grad)
+            launch_gather(stream, n, nullptr, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0, tune.gather_waves, d_sidx,
+                          d_slot_of);
         launch_replica_totals(stream, nchunk, d_chunks, nrep, d_rep_chunk, d_E, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0,
                               d_partial, d_tot);
         if (timed) JNN_HIP(hipEventRecord(ev[6], stream));
+    }
+
+    // The neighbour rows by atom: the device rows sit in cell-sorted slot
+    // order; perm[i*cap + q] = position of atom i's q-th entry on the device.
+    void user_rows(std::vector<int> &cnt_u, std::vector<int> &key_u, std::vector<size_t> &perm) {
+        const int c = nl.cap;
+        std::vector<int> cnt(n), k((size_t)n * c), slot_h(n);
+        if (n) {
+            JNN_HIP(hipMemcpyAsync(cnt.data(), nl.cnt, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+            JNN_HIP(hipMemcpyAsync(k.data(), nl.key, sizeof(int) * k.size(), hipMemcpyDeviceToHost, stream));
+            JNN_HIP(hipMemcpyAsync(slot_h.data(), d_slot_of, sizeof(int) * n, hipMemcpyDeviceToHost, stream));
+        }
+        JNN_HIP(hipStreamSynchronize(stream));
+        cnt_u.assign(n, 0);
+        key_u.assign((size_t)n * c, 0);
+        perm.assign((size_t)n * c, 0);
+        std::vector<std::pair<int, size_t>> row;
+        for (int i = 0; i < n; ++i) {
+            const int sl = slot_h[i];
+            cnt_u[i] = cnt[sl];
+            const int m = std::min(cnt[sl], c);
+            row.clear();
+            for (int q = 0; q < m; ++q) {
+                const size_t src = (size_t)sl * c + q;
+                row.emplace_back(k[src], src);
+            }
+            std::sort(row.begin(), row.end());
+            for (int q = 0; q < m; ++q) {
+                key_u[(size_t)i * c + q] = row[q].first;
+                perm[(size_t)i * c + q] = row[q].second;
+            }
+        }
     }
 
     int *tune_field(int k) {
@@ -754,13 +792,11 @@ int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *k
         }
         s->finish();
         const int n = s->n, c = s->nl.cap;
-        std::vector<int> cnt(n), k((size_t)n * c);
+        std::vector<int> cnt, k;
+        std::vector<size_t> perm;
+        s->user_rows(cnt, k, perm);
         std::vector<float4> dd((size_t)n * c);
-        if (n) {
-            JNN_HIP(hipMemcpyAsync(cnt.data(), s->nl.cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));
-            JNN_HIP(hipMemcpyAsync(k.data(), s->nl.key, sizeof(int) * k.size(), hipMemcpyDeviceToHost, s->stream));
-            JNN_HIP(hipMemcpyAsync(dd.data(), s->nl.d, sizeof(float4) * dd.size(), hipMemcpyDeviceToHost, s->stream));
-        }
+        if (n && d) JNN_HIP(hipMemcpyAsync(dd.data(), s->nl.d, sizeof(float4) * dd.size(), hipMemcpyDeviceToHost, s->stream));
         JNN_HIP(hipStreamSynchronize(s->stream));
         bool over = false;
         for (int i = 0; i < n; ++i) {
@@ -771,9 +807,10 @@ int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *k
                 const size_t src = (size_t)i * c + q, dst = (size_t)i * cap + q;
                 if (key) key[dst] = k[src];
                 if (d) {
-                    d[3 * dst] = dd[src].x;
-                    d[3 * dst + 1] = dd[src].y;
-                    d[3 * dst + 2] = dd[src].z;
+                    const float4 v = dd[perm[src]];
+                    d[3 * dst] = v.x;
+                    d[3 * dst + 1] = v.y;
+                    d[3 * dst + 2] = v.z;
                 }
             }
         }
@@ -788,20 +825,20 @@ int jse_nnap_system_get_pair_gradients(jse_nnap_system s, int cap, float *g) {
         JNN_CHECK(s->last_what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL), ERR_ARG, "forces were not computed");
         s->finish();
         const int n = s->n, c = s->nl.cap;
-        std::vector<int> cnt(n);
+        std::vector<int> cnt, k;
+        std::vector<size_t> perm;
+        s->user_rows(cnt, k, perm);
         std::vector<float4> gg((size_t)n * c);
-        if (n) {
-            JNN_HIP(hipMemcpyAsync(cnt.data(), s->nl.cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));
-            JNN_HIP(hipMemcpyAsync(gg.data(), s->nl.g, sizeof(float4) * gg.size(), hipMemcpyDeviceToHost, s->stream));
-        }
+        if (n) JNN_HIP(hipMemcpyAsync(gg.data(), s->nl.g, sizeof(float4) * gg.size(), hipMemcpyDeviceToHost, s->stream));
         JNN_HIP(hipStreamSynchronize(s->stream));
         for (int i = 0; i < n; ++i) {
             const int m = std::min(std::min(cnt[i], c), cap);
             for (int q = 0; q < m; ++q) {
                 const size_t src = (size_t)i * c + q, dst = (size_t)i * cap + q;
-                g[3 * dst] = gg[src].x;
-                g[3 * dst + 1] = gg[src].y;
-                g[3 * dst + 2] = gg[src].z;
+                const float4 v = gg[perm[src]];
+                g[3 * dst] = v.x;
+                g[3 * dst + 1] = v.y;
+                g[3 * dst + 2] = v.z;
             }
         }
     });
@@ -855,7 +892,7 @@ int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long l
         }
         s->finish();
         const int n = s->n, c = s->nl.cap;
-        launch_nl_reverse(s->stream, n, nullptr, s->nl);
+        launch_nl_reverse(s->stream, n, nullptr, s->nl, s->d_sidx, s->d_slot_of);
         std::vector<int> cnt(n), rev((size_t)n * c);
         if (n) {
             JNN_HIP(hipMemcpyAsync(cnt.data(), s->nl.cnt, sizeof(int) * n, hipMemcpyDeviceToHost, s->stream));

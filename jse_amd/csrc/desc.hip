@@ -106,7 +106,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
     const float rc = a.rc, inv_rc = 1.f / a.rc;
 
     for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
-        const int i = a.rows_atom[row];
+        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
         // FP64 accumulators: every c*Y product of two floats is exact in FP64
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
     const int nblk = bwd_blocks<MIX>(a.ntypes);
     const float rc = a.rc, inv_rc = 1.f / a.rc;
     for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
-        const int i = a.rows_atom[row];
+        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
         const float *__restrict__ Arow = a.A + (size_t)row * (P * M);

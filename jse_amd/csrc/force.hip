@@ -35,11 +35,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 }
 
 __global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ atoms, NeighborList nl,
-                                                double *__restrict__ F, double *__restrict__ V, int want_virial) {
+                                                double *__restrict__ F, double *__restrict__ V, int want_virial,
+                                                const int *__restrict__ out_of, const int *__restrict__ row_of) {
     const int lane = threadIdx.x & 63;
     const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (t >= n) return;
-    const int i = atoms ? atoms[t] : t;
+    const int i = atoms ? atoms[t] : t;       // row of the list
+    const int ia = out_of ? out_of[i] : i;    // its atom
     const int ci = min(nl.cnt[i], nl.cap);
     const size_t base = (size_t)i * nl.cap;
     double f[3] = {0.0, 0.0, 0.0};
@@ -50,10 +52,10 @@ __global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ a
         f[1] -= (double)go.y;
         f[2] -= (double)go.z;
         const int key = nl.key[base + k];
-        const int r = find_reverse(nl, i, key);  // binary search in j's sorted row
+        int jrow;
+        const int r = find_reverse(nl, ia, key, row_of, &jrow);  // binary search in j's sorted row
         if (r < 0) continue;
-        const int j = key >> 5;
-        const size_t sj = (size_t)j * nl.cap + r;
+        const size_t sj = (size_t)jrow * nl.cap + r;
         const float4 gi = nl.g[sj];
         const double gx = (double)gi.x, gy = (double)gi.y, gz = (double)gi.z;
         f[0] += gx;
@@ -75,12 +77,13 @@ __global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ a
         for (int c = 0; c < 9; ++c) v[c] = wave_sum(v[c]);
     }
     if (lane == 0) {
-        F[3 * i] = f[0];
-        F[3 * i + 1] = f[1];
-        F[3 * i + 2] = f[2];
+        const size_t o = (size_t)ia;
+        F[3 * o] = f[0];
+        F[3 * o + 1] = f[1];
+        F[3 * o + 2] = f[2];
         if (want_virial) {
 #pragma unroll
-            for (int c = 0; c < 9; ++c) V[9 * (size_t)i + c] = v[c];
+            for (int c = 0; c < 9; ++c) V[9 * o + c] = v[c];
         }
     }
 }
@@ -136,10 +139,11 @@ __global__ void k_rep_final(int nrep, const int *__restrict__ rep_chunk_off, con
 }
 
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
-                   int want_virial, int waves) {
+                   int want_virial, int waves, const int *out_of, const int *row_of) {
     if (n <= 0) return;
     const int w = (waves >= 1 && waves <= 8) ? waves : 4;
-    hipLaunchKernelGGL(k_gather, dim3((n + w - 1) / w), dim3(64 * w), 0, s, n, atoms, nl, F, V, want_virial);
+    hipLaunchKernelGGL(k_gather, dim3((n + w - 1) / w), dim3(64 * w), 0, s, n, atoms, nl, F, V, want_virial, out_of,
+                       row_of);
 }
 void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
                            const double *E, const double *V, int want_virial, double *partial, double *out) {

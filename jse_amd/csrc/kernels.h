@@ -46,17 +46,20 @@ struct NeighborList {
     float4 *g = nullptr;    // [N*cap] (gx, gy, gz, 0): g_j = -dE_i/d(r_j - r_i)
 };
 
-// Slot of the reverse pair (i, -shift) of i's pair `key` in j's sorted row,
-// -1 if j's row does not hold it.
-__device__ __forceinline__ int find_reverse(const NeighborList &nl, int i, int key) {
+// Slot of the reverse pair (i, -shift) of atom i's pair `key` = j*32 + code in
+// j's sorted row, -1 if j's row does not hold it.  Keys use atom numbers; row
+// j of the list is row_of[j] (nullptr: j).  *jrow receives that row.
+__device__ __forceinline__ int find_reverse(const NeighborList &nl, int i, int key, const int *row_of, int *jrow) {
     const int j = key >> 5, code = key & 31;
     const int want = i * 32 + (26 - code);
-    const int cj = min(nl.cnt[j], nl.cap);
-    const int *rj = nl.key + (size_t)j * nl.cap;
+    const int rj = row_of ? row_of[j] : j;
+    *jrow = rj;
+    const int cj = min(nl.cnt[rj], nl.cap);
+    const int *kj = nl.key + (size_t)rj * nl.cap;
     int lo = 0, hi = cj - 1;
     while (lo <= hi) {
         const int mid = (lo + hi) >> 1;
-        const int v = rj[mid];
+        const int v = kj[mid];
         if (v == want) return mid;
         if (v < want) lo = mid + 1;
         else hi = mid - 1;
@@ -75,24 +78,30 @@ void launch_wrap_bin(hipStream_t s, int n, const int *atoms, const double *xyz_i
                      const ReplicaGrid *reps, const int *need, double *xw, int *cell, int *cell_cnt, int *rank);
 void launch_cell_scan(hipStream_t s, int nrep, const int *rep_list, const ReplicaGrid *reps, const int *need,
                       const int *cell_cnt, int *cell_start);
+// Also writes slot_of[atom] when non-null.
 void launch_cell_scatter(hipStream_t s, int n, const int *atoms, const int *rep_of, const int *need, const int *cell,
-                         const int *rank, const int *cell_start, const double *xw, int *sidx, double *sxw);
+                         const int *rank, const int *cell_start, const double *xw, int *sidx, double *sxw,
+                         int *slot_of = nullptr);
 void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, const ReplicaGrid *reps,
                      const int *need, double rc2, const double *xw, const int *cell, const int *cell_start,
                      const int *sidx, const double *sxw, NeighborList nl, int *overflow, int wg_per_cu = 0);
-// The same rows for every atom of a system, one workgroup per cell (27-cell
+// The rows of every atom of a system, placed in cell-sorted slot order (row s
+// is atom sidx[s]; keys keep atom numbers j*32 + code, so each row's content
+// and order do not depend on the slot order), one workgroup per cell (27-cell
 // candidates staged in LDS once per cell); cell_rep[c] = replica of global
 // cell c, -1 for a replica's end entry.
 void launch_nl_build_cells(hipStream_t s, int ncell_total, const int *cell_rep, const ReplicaGrid *reps, double rc2,
                            const int *cell_start, const int *sidx, const double *sxw, NeighborList nl, int *overflow,
                            int wg_per_cu = 0);
-void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl);
+void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl, const int *out_of = nullptr,
+                       const int *row_of = nullptr);
 
 // ---- descriptor / MLP / backward for one species segment ----
 struct SegmentArgs {
     int nrows;              // atoms of this species
     const int *rows_atom;   // atom index of each row (species-sorted order)
-    const int *types;       // [N] 1-based types
+    const int *nl_of;       // neighbour-list row of an atom (nullptr: the atom index)
+    const int *types;       // [N] 1-based types by atom
     float rc;
     int ntypes;             // species of the model (neighbour types are 1..ntypes)
     const float *W;         // [ntypes][P][K] FP32 mixing (used by the dense kernel)
@@ -128,8 +137,10 @@ bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a);
 bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a);
 
 // ---- forces / virials / totals ----
+// Row t of the list belongs to atom out_of[t] (nullptr: t) and atom j's row is
+// row_of[j] (nullptr: j); F and V are written by atom.
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
-                   int want_virial, int waves = 0);
+                   int want_virial, int waves = 0, const int *out_of = nullptr, const int *row_of = nullptr);
 // Largest chunk the totals kernel sums in one workgroup.
 constexpr int kTotalsChunkMax = 1024;
 // Per-replica totals [nrep][10] = (E, V[9]): fixed-size chunks of each

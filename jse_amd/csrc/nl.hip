@@ -87,13 +87,15 @@ __global__ __launch_bounds__(1024) void k_cell_scan(const int *__restrict__ rep_
 __global__ void k_cell_scatter(int n, const int *__restrict__ atoms, const int *__restrict__ rep_of,
                                const int *__restrict__ need, const int *__restrict__ cell,
                                const int *__restrict__ rank, const int *__restrict__ start,
-                               const double *__restrict__ xw, int *__restrict__ sidx, double *__restrict__ sxw) {
+                               const double *__restrict__ xw, int *__restrict__ sidx, double *__restrict__ sxw,
+                               int *__restrict__ slot_of) {
     const int t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int i = atoms ? atoms[t] : t;
     if (need && !need[rep_of[i]]) return;
     const int slot = start[cell[i]] + rank[i];
     sidx[slot] = i;
+    if (slot_of) slot_of[i] = slot;
     sxw[3 * slot] = xw[3 * i];
     sxw[3 * slot + 1] = xw[3 * i + 1];
     sxw[3 * slot + 2] = xw[3 * i + 2];
@@ -239,7 +241,9 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
 }
 
 // ---- cell-tile build: one workgroup per cell ----------------------------
-// The atoms of one cell share its 27-cell neighbourhood: the workgroup stages
+// Rows are placed in cell-sorted slot order (row s is atom sidx[s]), so the
+// rows the force gather visits for an atom's neighbours sit close together in
+// memory; keys keep atom numbers.  The atoms of one cell share its 27-cell neighbourhood: the workgroup stages
 // those candidates once in LDS (FP64 positions with the periodic shift
 // already added, exactly as the per-atom build forms x_j + shift, and the key
 // j*32 + code), then each wave tests one member atom at a time against the
@@ -346,7 +350,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
         int *hkey = s_hkey[wv];
         float *hdx = s_hd[wv][0], *hdy = s_hd[wv][1], *hdz = s_hd[wv][2];
         for (int sl = cs + wv; sl < ce; sl += kCellWaves) {
-            const int i = sidx[sl];
+            const int i = sidx[sl];  // the row goes to slot sl; keys use atom numbers
             const double xi = sxw[3 * (size_t)sl], yi = sxw[3 * (size_t)sl + 1], zi = sxw[3 * (size_t)sl + 2];
             int hits = 0;
             for (int q0 = 0; q0 < total; q0 += 64) {
@@ -385,7 +389,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
             wave_lds_sync();
             const int nst = hits < kNlLds ? hits : kNlLds;
             const int nw = nst < nl.cap ? nst : nl.cap;
-            const size_t row = (size_t)i * nl.cap;
+            const size_t row = (size_t)sl * nl.cap;
             if (nst <= 64) {
                 int key = lane < nst ? hkey[lane] : 0x7fffffff;
                 float dx = lane < nst ? hdx[lane] : 0.f;
@@ -426,7 +430,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                 }
             }
             if (lane == 0) {
-                nl.cnt[i] = hits;
+                nl.cnt[sl] = hits;
                 if (hits > nl.cap) atomicMax(overflow, hits);
             }
             wave_lds_sync();  // this wave's hit buffer is reused for its next atom
@@ -436,14 +440,16 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
 }
 
 // rev[i][k] = slot of (i, -shift) in j's row (binary search on the sorted keys)
-__global__ void k_nl_reverse(int n, const int *__restrict__ atoms, NeighborList nl) {
+__global__ void k_nl_reverse(int n, const int *__restrict__ atoms, NeighborList nl, const int *__restrict__ out_of,
+                             const int *__restrict__ row_of) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const int ti = (int)(t / nl.cap), k = (int)(t % nl.cap);
     if (ti >= n) return;
-    const int i = atoms ? atoms[ti] : ti;
-    const int ci = min(nl.cnt[i], nl.cap);
+    const int r = atoms ? atoms[ti] : ti;
+    const int ci = min(nl.cnt[r], nl.cap);
     if (k >= ci) return;
-    nl.rev[(size_t)i * nl.cap + k] = find_reverse(nl, i, nl.key[(size_t)i * nl.cap + k]);
+    int jrow;
+    nl.rev[(size_t)r * nl.cap + k] = find_reverse(nl, out_of ? out_of[r] : r, nl.key[(size_t)r * nl.cap + k], row_of, &jrow);
 }
 
 __global__ void k_fill_int(int n, int *__restrict__ p, int v) {
@@ -467,10 +473,11 @@ void launch_cell_scan(hipStream_t s, int nrep, const int *rep_list, const Replic
     hipLaunchKernelGGL(k_cell_scan, dim3(nrep), dim3(1024), 0, s, rep_list, reps, need, cell_cnt, cell_start);
 }
 void launch_cell_scatter(hipStream_t s, int n, const int *atoms, const int *rep_of, const int *need, const int *cell,
-                         const int *rank, const int *cell_start, const double *xw, int *sidx, double *sxw) {
+                         const int *rank, const int *cell_start, const double *xw, int *sidx, double *sxw,
+                         int *slot_of) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_cell_scatter, dim3((n + 255) / 256), dim3(256), 0, s, n, atoms, rep_of, need, cell, rank,
-                       cell_start, xw, sidx, sxw);
+                       cell_start, xw, sidx, sxw, slot_of);
 }
 int device_cu_count() {
     static int ncu = 0;
@@ -503,10 +510,11 @@ void launch_nl_build_cells(hipStream_t s, int ncell_total, const int *cell_rep, 
     hipLaunchKernelGGL(k_nl_build_cells, dim3(grid), dim3(64 * kCellWaves), 0, s, ncell_total, cell_rep, reps, rc2,
                        cell_start, sidx, sxw, nl, overflow);
 }
-void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl) {
+void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl, const int *out_of, const int *row_of) {
     if (n <= 0) return;
     const long long tot = (long long)n * nl.cap;
-    hipLaunchKernelGGL(k_nl_reverse, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, atoms, nl);
+    hipLaunchKernelGGL(k_nl_reverse, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, n, atoms, nl, out_of,
+                       row_of);
 }
 
 }  // namespace jnn
