@@ -195,6 +195,42 @@ int jse_nnap_server_stats(jse_nnap_server s, double *out, int n);
 /* Finishes running batches, fails queued requests, frees the server. */
 void jse_nnap_server_destroy(jse_nnap_server s);
 
+/* ---- LAMMPS GPU pair hook, jse-compatible packed arguments ----------------
+ * jse binds native code as `int name(void *args)`, the declared parameters
+ * packed back to back in declaration order with no padding (pointers 8 bytes,
+ * int 4, double 8, float 4; SourceScanner.java:167-290) and loaded with
+ * dlopen/dlsym (jse_jit_SimpleJIT.c:58-119).  These four replace the
+ * reference's generated functions of the same names
+ * (nnap_interface_cuda.cu:307-483; driver NNAP.computeLammpsCuda,
+ * NNAP.java:949-1030), with flt_t = float and the same argument lists:
+ *   jse_nnap_statNeiNumLammps (int *ilist, int *numneigh, int inum, int *numneighMax)
+ *   jse_nnap_lammps2cuda      (int inum, int nlocalghost, int nlflag, int neighnumMax, double **x,
+ *                              int *type, int *ilist, int *numneigh, int **firstneigh, float *fltBuf,
+ *                              int *intBuf, float *cudaX, int *cudaType, int *cudaIlist,
+ *                              int *cudaNumneigh, int *cudaFirstneigh)
+ *   jse_nnap_computeLammpsCuda(int inum, int nlocalghost, int eflag, int vflag, int eflagAtom,
+ *                              int vflagAtom, int cvflagAtom, float *x, int *type, int *ilist,
+ *                              int *nmerges, int **mergeSorted, float **cutsq, int *numneigh,
+ *                              int *firstneigh, int *aLmpType2NNAPType, float **aFpHyperParam,
+ *                              float **aFpParam, float **aNnParam, float **aNormParam, float *f,
+ *                              float *eatom0, float *vatom0, float *vatom1, + 10 scratch pointers)
+ *   jse_nnap_cuda2lammps      (int inum, int nlocalghost, int eflag, int vflag, int eflagAtom,
+ *                              int vflagAtom, int cvflagAtom, double **f, double *engVdwl,
+ *                              double *eatom, double *virial, double **vatom, double **cvatom,
+ *                              float *fltBuf, const int *ilist, float *cudaF, float *cudaEatom0,
+ *                              float *cudaVatom0, float *cudaVatom1)
+ * The reference compiles the model's shapes into its generated library; here
+ * jse_nnap_lammps_bind (no reference counterpart) names the model once:
+ * packed (jse_nnap_model model, int precision).  The per-call parameter
+ * blocks, merge tables and scratch pointers are then not read; displacements
+ * are formed from FP64 positions and forces / virials accumulated in FP64.
+ * Return 0 on success, a status code otherwise. */
+int jse_nnap_lammps_bind(void *args);
+int jse_nnap_statNeiNumLammps(void *args);
+int jse_nnap_lammps2cuda(void *args);
+int jse_nnap_computeLammpsCuda(void *args);
+int jse_nnap_cuda2lammps(void *args);
+
 #ifdef __cplusplus
 }
 #endif

@@ -99,3 +99,27 @@ def test_workload_geometry():
     r = np.sqrt((d ** 2).sum(-1)) + np.eye(2000) * 10
     assert r.min() >= 2.0
     assert np.bincount(st.types).tolist() == [0, 1000, 800, 200]
+
+
+def _pack(*items):
+    import ctypes
+    import struct
+    fmt = "<" + "".join("i" if k == "i" else "Q" for k, _ in items)
+    return ctypes.create_string_buffer(struct.pack(fmt, *[int(v) for _, v in items]))
+
+
+def test_lammps_packed_host_entries():
+    """Host-only parts of the LAMMPS hook: the neighbour-count statistic on
+    a packed (int *ilist, int *numneigh, int inum, int *numneighMax) block,
+    and the model binding's argument checks (no device is touched)."""
+    import ctypes
+    lib = nnap.lib()
+    lib.jse_nnap_statNeiNumLammps.argtypes = [ctypes.c_void_p]
+    lib.jse_nnap_lammps_bind.argtypes = [ctypes.c_void_p]
+    ilist = np.array([2, 0, 1], dtype=np.int32)
+    numneigh = np.array([5, 9, 7, 40], dtype=np.int32)   # atom 3 is not listed
+    out = np.zeros(1, dtype=np.int32)
+    blk = _pack(("p", ilist.ctypes.data), ("p", numneigh.ctypes.data), ("i", 3), ("p", out.ctypes.data))
+    assert lib.jse_nnap_statNeiNumLammps(blk) == 0 and out[0] == 9
+    assert lib.jse_nnap_lammps_bind(_pack(("p", 0), ("i", 0))) == 1
+    assert b"null model" in lib.jse_nnap_last_error()
