@@ -183,13 +183,14 @@ def copy_calibration(nbytes, reps=5):
 
 
 def committed_traffic(kernel):
-    """HBM bytes per launch of `kernel` from the committed counter summary
-    (profiles/<kernel>_c2_traffic.md, line 'traffic_bytes_per_launch: N')."""
+    """HBM bytes that `kernel` moves per evaluation (its species launches
+    together), from the committed counter summary
+    profiles/<kernel>_c2_traffic.md (line 'traffic_bytes_per_step: N')."""
     import re
     path = os.path.join(ROOT, "profiles", f"{kernel}_c2_traffic.md")
     try:
         with open(path) as f:
-            m = re.search(r"traffic_bytes_per_launch:\s*([0-9.eE+]+)", f.read())
+            m = re.search(r"traffic_bytes_per_step:\s*([0-9.eE+]+)", f.read())
         return float(m.group(1)) if m else None
     except OSError:
         return None

@@ -38,7 +38,9 @@ __global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ a
                                                 double *__restrict__ F, double *__restrict__ V, int want_virial,
                                                 const int *__restrict__ out_of, const int *__restrict__ row_of) {
     const int lane = threadIdx.x & 63;
-    const int t = blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6);
+    // rows are in cell order: each XCD gathers one contiguous run of rows, so
+    // the neighbours' rows it reads sit in its own L2
+    const int t = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
     if (t >= n) return;
     const int i = atoms ? atoms[t] : t;       // row of the list
     const int ia = out_of ? out_of[i] : i;    // its atom
@@ -142,8 +144,8 @@ void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &n
                    int want_virial, int waves, const int *out_of, const int *row_of) {
     if (n <= 0) return;
     const int w = (waves >= 1 && waves <= 8) ? waves : 4;
-    hipLaunchKernelGGL(k_gather, dim3((n + w - 1) / w), dim3(64 * w), 0, s, n, atoms, nl, F, V, want_virial, out_of,
-                       row_of);
+    hipLaunchKernelGGL(k_gather, dim3(xcd_grid((n + w - 1) / w)), dim3(64 * w), 0, s, n, atoms, nl, F, V, want_virial,
+                       out_of, row_of);
 }
 void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
                            const double *E, const double *V, int want_virial, double *partial, double *out) {

@@ -10,6 +10,16 @@ namespace jnn {
 // Compute units of the current device (cached after the first query).
 int device_cu_count();
 
+// MI355X deals consecutive workgroups round-robin to its 8 XCDs, each with
+// its own L2.  With a grid of 8 * ceil(nblocks / 8) workgroups, this maps
+// workgroup b to logical block (b % 8) * ceil(nblocks / 8) + b / 8, so each
+// XCD takes one contiguous range of logical blocks (>= nblocks: no work).
+constexpr int kXcds = 8;
+__host__ __device__ __forceinline__ int xcd_grid(int nblocks) { return ((nblocks + kXcds - 1) / kXcds) * kXcds; }
+__device__ __forceinline__ int xcd_block(int b, int grid) {
+    return (b % kXcds) * (grid / kXcds) + b / kXcds;
+}
+
 // Launch shapes of one system's kernels, chosen by the autotuner
 // (jse_nnap_system_autotune); 0 keeps a kernel's built-in default.
 struct LaunchTune {

@@ -289,7 +289,8 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
     __shared__ int s_hkey[kCellWaves][kNlLds];
     __shared__ float s_hd[kCellWaves][3][kNlLds];
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    for (int c = blockIdx.x; c < ncell_total; c += gridDim.x) {
+    // each XCD takes a contiguous run of cells, whose 27-cell blocks overlap in its L2
+    for (int c = xcd_block(blockIdx.x, gridDim.x); c < ncell_total; c += gridDim.x) {
         const int r = cell_rep[c];
         if (r < 0) continue;  // a replica's end entry, not a cell
         const int cs = cstart[c], ce = cstart[c + 1];
@@ -507,6 +508,7 @@ void launch_nl_build_cells(hipStream_t s, int ncell_total, const int *cell_rep, 
     if (ncell_total <= 0) return;
     int grid = ncell_total;
     if (wg_per_cu > 0) grid = std::min(grid, wg_per_cu * device_cu_count());
+    grid = xcd_grid(grid);
     hipLaunchKernelGGL(k_nl_build_cells, dim3(grid), dim3(64 * kCellWaves), 0, s, ncell_total, cell_rep, reps, rc2,
                        cell_start, sidx, sxw, nl, overflow);
 }
