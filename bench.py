@@ -343,6 +343,9 @@ def main():
     res = sysm.results()
     stats = sysm.nl_stats()
 
+    ms_default, _ = sysm.benchmark(what, warmup=2, iters=5)
+    from jse_amd import autotune
+    knobs, _, _ = autotune.tune(sysm, what, use_cache=False)
     t0 = time.perf_counter()
     ms_step, phase = sysm.benchmark(what, warmup=args.warmup, iters=args.steps)
     wall = time.perf_counter() - t0
@@ -376,7 +379,8 @@ def main():
         "config": {"workload": f"ternary Cu-Zr-Al liquid, {st.natoms} atoms, rho 0.058, rcut 6, nmax 8, lmax 6, "
                                "MLP 64-64 SiLU, plain FP32 path" if prec == nnap.FP32 else
                                f"ternary Cu-Zr-Al liquid, {st.natoms} atoms, BF16 MLP",
-                   "natoms": st.natoms, "pairs": int(stats["pairs"]), "parallelism": "single GPU"},
+                   "natoms": st.natoms, "pairs": int(stats["pairs"]), "parallelism": "single GPU",
+                   "launch_shapes": knobs, "default_shapes_ms_per_step": ms_default},
         "phase_ms": phase_ms,
         "wall_s_timed": wall,
         "roofline": roof,

@@ -143,10 +143,12 @@ int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long l
  * keeps, knob by knob, the fastest: tune[0] descriptor-forward and tune[1]
  * descriptor-backward persistent workgroups per CU (0 = one per atom),
  * tune[2] MLP workgroups per CU, tune[3] neighbour-build workgroups per CU
- * (0 = one per atom), tune[4] force-gather atoms per workgroup.  Writes the
- * chosen knobs (5 ints) and the tuned device time per evaluation. */
+ * (0 = one per atom), tune[4] force-gather atoms per workgroup, tune[5] the
+ * descriptor-forward register tile (0: 6x7, 1: 9x7), tune[6] the
+ * descriptor-backward schedule (scheduling barrier every 1 / 2 / no rows).
+ * Writes the chosen knobs (7 ints) and the tuned device time per evaluation. */
 int jse_nnap_system_autotune(jse_nnap_system s, int what, int iters, int *tune, double *ms_step);
-/* Applies a tuning vector (e.g. a cached autotuner result); n <= 5 knobs. */
+/* Applies a tuning vector (e.g. a cached autotuner result); n <= 7 knobs. */
 int jse_nnap_system_set_tuning(jse_nnap_system s, const int *tune, int n);
 
 /* ---- server: device-resident replicas, continuous batching ----------------

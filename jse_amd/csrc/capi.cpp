@@ -370,6 +370,7 @@ struct jse_nnap_system_s {
             const SpeciesHost &sp = model->host.species[t];
             SegmentArgs a = seg_args(t, raw);
             a.wg_per_cu = tune.desc_fwd_wg_per_cu;
+            a.variant = tune.desc_fwd_variant;
             if (!launch_desc_fwd(stream, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
                 throw Error(ERR_UNSUPPORTED, "no descriptor kernel compiled for lmax=" + std::to_string(sp.lmax) +
                                                  " nmax=" + std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P));
@@ -393,6 +394,7 @@ struct jse_nnap_system_s {
                 const SpeciesHost &sp = model->host.species[t];
                 SegmentArgs a = seg_args(t, false);
                 a.wg_per_cu = tune.desc_bwd_wg_per_cu;
+                a.variant = tune.desc_bwd_variant;
                 if (!launch_desc_bwd(stream, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
                     throw Error(ERR_UNSUPPORTED, "no descriptor backward kernel compiled for this shape");
             }
@@ -444,7 +446,9 @@ struct jse_nnap_system_s {
             case 1: return &tune.desc_bwd_wg_per_cu;
             case 2: return &tune.mlp_wg_per_cu;
             case 3: return &tune.nl_wg_per_cu;
-            default: return &tune.gather_waves;
+            case 4: return &tune.gather_waves;
+            case 5: return &tune.desc_fwd_variant;
+            default: return &tune.desc_bwd_variant;
         }
     }
     // Device time per evaluation of `iters` back-to-back eager evaluations.
@@ -464,7 +468,7 @@ struct jse_nnap_system_s {
         JNN_CHECK(have_pos, ERR_ARG, "positions not set");
         JNN_CHECK(iters >= 1, ERR_ARG, "iters must be >= 1");
         static const std::vector<int> cand[kTuneKnobs] = {
-            {0, 4, 8, 16, 32}, {0, 4, 8, 16, 32}, {1, 2, 4, 8}, {0, 4, 8, 16}, {1, 2, 4, 8}};
+            {0, 4, 8, 16, 32}, {0, 4, 8, 16, 32}, {1, 2, 4, 8}, {0, 4, 8, 16}, {1, 2, 4, 8}, {0, 1}, {0, 1, 2}};
         drop_graphs();
         compute(what, true);  // settles the neighbour capacity
         finish();
@@ -677,6 +681,8 @@ int jse_nnap_system_set_tuning(jse_nnap_system s, const int *tune, int n) {
     return guard([&] {
         JNN_CHECK(s && tune && n >= 0 && n <= kTuneKnobs, ERR_ARG, "bad tuning vector");
         for (int k = 0; k < n; ++k) JNN_CHECK(tune[k] >= 0 && tune[k] <= 64, ERR_ARG, "tuning value out of range");
+        if (n > 5) JNN_CHECK(tune[5] <= 1, ERR_ARG, "descriptor forward variant out of range");
+        if (n > 6) JNN_CHECK(tune[6] <= 2, ERR_ARG, "descriptor backward variant out of range");
         JNN_HIP(hipStreamSynchronize(s->stream));
         for (int k = 0; k < n; ++k) *s->tune_field(k) = tune[k];
         s->drop_graphs();

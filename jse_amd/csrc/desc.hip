@@ -31,12 +31,12 @@
 
 namespace jnn {
 
-template <int LMAX, int K, int P, int MIX>
+template <int LMAX, int K, int P, int MIX, int TP0 = 6>
 struct DescShape {
     static constexpr int L1 = LMAX + 1;
     static constexpr int M = L1 * L1;
     static constexpr int S = P * L1;
-    static constexpr int TP = P >= 6 ? 6 : P;
+    static constexpr int TP = P >= TP0 ? TP0 : P;
     static constexpr int NPB = (P + TP - 1) / TP;
     static constexpr int TM = 7;
     static constexpr int NMB = (M + TM - 1) / TM;
@@ -87,9 +87,10 @@ __device__ __forceinline__ PairGeom pair_geom(float dx, float dy, float dz) {
     return g;
 }
 
-template <int LMAX, int K, int P, int MIX>
+// TP0: p-rows of each lane's register tile (tile TP0 x 7 of A)
+template <int LMAX, int K, int P, int MIX, int TP0>
 __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl) {
-    using SH = DescShape<LMAX, K, P, MIX>;
+    using SH = DescShape<LMAX, K, P, MIX, TP0>;
     constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
     constexpr int TP = SH::TP, TM = SH::TM, NMB = SH::NMB, NT = SH::NT, G = SH::G;
     // phase 1 (per chunk): Y and c of 64 neighbours as FP32; phase 2 (per atom): A in FP64
@@ -198,9 +199,12 @@ __host__ __device__ constexpr int bwd_blocks(int ntypes) {
     return MIX == MIX_NONE ? 1 : ntypes;
 }
 
-template <int LMAX, int K, int P, int MIX>
+// BAR: a scheduling barrier after every BAR rows of the per-pair contraction
+// (bounds how many rows of LDS operands are in flight; 0: none)
+template <int LMAX, int K, int P, int MIX, int BAR>
 __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl) {
     using SH = DescShape<LMAX, K, P, MIX>;
+    constexpr int BSTEP = BAR > 0 ? BAR : 1;
     constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
     // sGt[t][n][m] = sum_p W[t][p][n] Gc[p][l(m)] A[p][m]: dE/dA taken back
     // through species t's radial mixing, so a neighbour of species t needs
@@ -275,8 +279,7 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
                     }
                     gfc += s * rn[q];
                     grn[q] = s * fc;
-                    // keep the LDS reads of one row from being hoisted across rows (register pressure)
-                    __builtin_amdgcn_sched_barrier(0);
+                    if (BAR > 0 && q % BSTEP == BSTEP - 1) __builtin_amdgcn_sched_barrier(0);
                 }
                 float drn[K];
                 cheb_drn<K>(xr, inv_rc, drn);
@@ -317,14 +320,22 @@ template <int LMAX, int K, int P, int MIX>
 static void run_fwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
     int grid = a.nrows < 262144 ? a.nrows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
-    hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, a, nl);
+    if (a.variant == 1)
+        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 9>), dim3(grid), dim3(64), 0, s, a, nl);
+    else
+        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, a, nl);
 }
 template <int LMAX, int K, int P, int MIX>
 static void run_bwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
     int grid = a.nrows < 262144 ? a.nrows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
     const size_t lds = sizeof(float) * (size_t)bwd_blocks<MIX>(a.ntypes) * K * ((LMAX + 1) * (LMAX + 1));
-    hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX>), dim3(grid), dim3(64), lds, s, a, nl);
+    if (a.variant == 1)
+        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 2>), dim3(grid), dim3(64), lds, s, a, nl);
+    else if (a.variant == 2)
+        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 0>), dim3(grid), dim3(64), lds, s, a, nl);
+    else
+        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 1>), dim3(grid), dim3(64), lds, s, a, nl);
 }
 
 #define JNN_SHAPES(X)              \

@@ -28,8 +28,10 @@ struct LaunchTune {
     int mlp_wg_per_cu = 0;       // MLP: persistent workgroups per CU (0: 2)
     int nl_wg_per_cu = 0;        // neighbour build: persistent workgroups per CU (0: one per atom)
     int gather_waves = 0;        // force gather: atoms (waves) per workgroup (0: 4)
+    int desc_fwd_variant = 0;    // descriptor forward register tile: 0 = 6 x 7, 1 = 9 x 7
+    int desc_bwd_variant = 0;    // descriptor backward: scheduling barrier every 1 / 2 / no rows
 };
-constexpr int kTuneKnobs = 5;
+constexpr int kTuneKnobs = 7;
 
 // Periodic cell grid of one replica's orthogonal box (reference:
 // NeighborListGetter.java:952-1001).  A system holds one or more replicas;
@@ -121,6 +123,7 @@ struct SegmentArgs {
     float *A;               // [nrows][P*M]
     float *G;               // [nrows][S]  2*(4pi/(2l+1)) * dE/dfp
     int wg_per_cu;          // persistent workgroups per CU (0: one workgroup per row)
+    int variant;            // compiled schedule variant (autotuned; 0: default)
 };
 // returns false when no kernel is compiled for this (lmax, K, P, mix)
 bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);

@@ -271,7 +271,8 @@ class System:
                                               ctypes.byref(un)))
         return {"cap": c.value, "max_count": mx.value, "pairs": pairs.value, "unmatched": un.value}
 
-    TUNE_KNOBS = ["desc_fwd_wg_per_cu", "desc_bwd_wg_per_cu", "mlp_wg_per_cu", "nl_wg_per_cu", "gather_waves"]
+    TUNE_KNOBS = ["desc_fwd_wg_per_cu", "desc_bwd_wg_per_cu", "mlp_wg_per_cu", "nl_wg_per_cu", "gather_waves",
+                  "desc_fwd_variant", "desc_bwd_variant"]
 
     def autotune(self, what: int = ENERGY | FORCE | VIRIAL, iters: int = 5):
         """times candidate launch shapes on this system's data; returns (knobs, ms per evaluation)"""

@@ -101,8 +101,11 @@ def O(oracle_mod):
 
 
 def test_autotuned_launch_shapes_keep_results():
-    """Launch shapes change which workgroup takes a row, never a row's
-    arithmetic: results after autotuning equal the default shapes' bit for bit."""
+    """Launch shapes and the backward's schedule change which workgroup takes
+    a row and how instructions are ordered, never a row's arithmetic: with
+    the tuned knobs (forward tile at its default) results equal the default
+    shapes' bit for bit.  The alternative forward tile regroups the FP64
+    sums of A: results agree to FP64 rounding carried into FP32."""
     from jse_amd import autotune
     md = W.ternary_model()
     reps = W.replicas(24, 64, 1500, 29)
@@ -116,10 +119,17 @@ def test_autotuned_launch_shapes_keep_results():
         print("tuned", knobs, ms)
         # 0 = the kernel's built-in default shape kept
         assert knobs["mlp_wg_per_cu"] in (0, 1, 2, 4, 8) and knobs["gather_waves"] in (0, 1, 2, 4, 8)
+        assert knobs["desc_fwd_variant"] in (0, 1) and knobs["desc_bwd_variant"] in (0, 1, 2)
+        b.set_tuning(dict(knobs, desc_fwd_variant=0))
         b.compute(what)
         r1 = b.results()
         for k in ("energies", "forces", "virials"):
             assert np.array_equal(r0[k], r1[k]), k
+        b.set_tuning(dict(knobs, desc_fwd_variant=1))
+        b.compute(what)
+        r2 = b.results()
+        assert np.abs(r2["energies"] - r0["energies"]).max() <= 1e-6
+        assert np.abs(r2["forces"] - r0["forces"]).max() <= 1e-5
         # a cached tuning is applied without re-timing
         knobs2, ms2, hit2 = autotune.tune(b, what)
         assert hit2 and knobs2 == knobs
