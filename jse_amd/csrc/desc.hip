@@ -87,6 +87,26 @@ __device__ __forceinline__ PairGeom pair_geom(float dx, float dy, float dz) {
     return g;
 }
 
+// The same from hardware reciprocal square roots (about 1 ulp each): the
+// matrix-core forward's geometry.  Below FLT_MIN for dx^2 + dy^2 the pair is
+// taken as on the pole; the m != 0 harmonics there scale with sin(theta) and
+// vanish to FP32 precision.
+__device__ __forceinline__ PairGeom pair_geom_fast(float dx, float dy, float dz) {
+    PairGeom g;
+    const float q = dx * dx + dy * dy;
+    const float r2 = q + dz * dz;
+    const float ir = __builtin_amdgcn_rsqf(r2);
+    g.r = r2 * ir;
+    g.pole = !(q >= 1.17549435e-38f);
+    const float iq = g.pole ? 0.f : __builtin_amdgcn_rsqf(q);
+    g.dxy = q * iq;
+    g.ct = dz * ir;
+    g.st = g.dxy * ir;
+    g.cp = g.pole ? 1.f : dx * iq;
+    g.sp = g.pole ? 0.f : dy * iq;
+    return g;
+}
+
 // TP0: p-rows of each lane's register tile (tile TP0 x 7 of A)
 template <int LMAX, int K, int P, int MIX, int TP0>
 __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl) {
@@ -188,6 +208,143 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
             }
             const double fp = (4.0 * 3.14159265358979323846 / (double)(2 * l + 1)) * s;
             a.x[(size_t)row * S + e] = (float)((fp - a.mu64[e]) / a.sig64[e]);
+        }
+    }
+}
+
+// Forward on the matrix cores (v_mfma_f32_16x16x4_f32: FP32 products, FP32
+// accumulation).  Per atom, A^T (M x P) = sum over k-steps of
+// Y^T (16 m x 4 neighbours) * C (4 neighbours x 16 p).  Lanes first compute one
+// neighbour each: c_j goes to LDS as C^T[p][j]; Y_j stays in registers and is
+// staged into LDS one 16-row m-tile at a time as Y^T[m][j].  Each m-tile then
+// runs ceil(n/4) k-steps against the p-tiles of C, four k-steps per batch of
+// operand reads.  Lanes of padding rows (m >= M, p >= P) read one zero row of
+// LDS; lanes past the last neighbour evaluate a valid pair with c = 0.  The epilogue writes the
+// tiles to LDS as A[p][m] and forms fp and x in FP64.
+// Lane maps of the 16x16x4 instruction: A[i][k] from lane i + 16 k,
+// B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i / 4), register i % 4.
+// A row stride of 68 floats puts the 64 lanes of every operand read on 64
+// distinct banks.
+template <int LMAX, int K, int P, int MIX>
+__global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentArgs a, NeighborList nl) {
+    constexpr int L1 = LMAX + 1, M = L1 * L1, S = P * L1;
+    constexpr int NMT = (M + 15) / 16, NPT = (P + 15) / 16;
+    constexpr int MLAST = M - 16 * (NMT - 1);  // data rows of the last m-tile
+    constexpr int NS = 68;
+    constexpr int FC = (P + 1) * NS, FY = 16 * NS, FA = P * M;  // C^T rows and the zero row P
+    constexpr int F = FC + (FY > FA ? FY : FA);
+    constexpr int NE = (S + 63) / 64;
+    __shared__ __align__(16) float sbuf[F];
+    float *sC = sbuf, *sY = sbuf + FC, *sA = sbuf + FC;
+    using f4 = __attribute__((__vector_size__(4 * sizeof(float)))) float;
+    const int lane = threadIdx.x;
+    const int li = lane & 15, lk = lane >> 4;
+    const float rc = a.rc, inv_rc = 1.f / a.rc;
+    sC[P * NS + lane] = 0.f;  // the zero row
+    if (lane < NS - 64) sC[P * NS + 64 + lane] = 0.f;
+    // operand rows of this lane: C^T row pt * 16 + li (or the zero row) and, in the
+    // last m-tile, Y^T row li (or the zero row, addressed from sY)
+    int boff[NPT];
+#pragma unroll
+    for (int pt = 0; pt < NPT; ++pt) boff[pt] = (pt * 16 + li < P ? pt * 16 + li : P) * NS;
+    const int ylast = li < MLAST ? li * NS : (P * NS - FC);
+    // per-lane descriptor slots e = lane + 64 q: 4 pi / (2l + 1), mu, 1 / sigma
+    double ew[NE], emu[NE], eis[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        const int e = lane + 64 * q;
+        const int l = e % L1;
+        ew[q] = 4.0 * 3.14159265358979323846 / (double)(2 * l + 1);
+        emu[q] = e < S ? a.mu64[e] : 0.0;
+        eis[q] = e < S ? 1.0 / a.sig64[e] : 0.0;
+    }
+
+    for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
+        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        f4 acc[NMT][NPT];
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
+            for (int pt = 0; pt < NPT; ++pt) acc[mt][pt] = f4{0.f, 0.f, 0.f, 0.f};
+
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int j = c0 + lane;
+            const int jr = j < n ? j : n - 1;
+            const float4 dd = nl.d[base + jr];
+            const int tj = a.types[nl.key[base + jr] >> 5];
+            const PairGeom g = pair_geom_fast(dd.x, dd.y, dd.z);
+            float y[M];
+            real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
+            float rn[K];
+            cheb_rn<K>(1.f - 2.f * g.r * inv_rc, rn);
+            const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+            const float t2 = t * t;
+            const float fc = (j < n && g.r < rc) ? t2 * t2 : 0.f;
+            float rnp[P];
+            mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
+            const int nks = (min(64, n - c0) + 3) >> 2;
+            __syncthreads();  // the previous chunk's operand reads (and the previous atom's A) are done
+#pragma unroll
+            for (int p = 0; p < P; ++p) sC[p * NS + lane] = fc * rnp[p];
+#pragma unroll
+            for (int mt = 0; mt < NMT; ++mt) {
+                if (mt > 0) __syncthreads();  // the previous m-tile's operand reads are done
+#pragma unroll
+                for (int r = 0; r < 16; ++r)
+                    if (mt * 16 + r < M) sY[r * NS + lane] = y[mt * 16 + r];
+                __syncthreads();
+                const int aoff = mt < NMT - 1 ? li * NS : ylast;
+                int ks = 0;
+                for (; ks + 4 <= nks; ks += 4) {
+                    float av[4], bv[4][NPT];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) {
+                        const int col = 4 * (ks + u) + lk;
+                        av[u] = sY[aoff + col];
+#pragma unroll
+                        for (int pt = 0; pt < NPT; ++pt) bv[u][pt] = sC[boff[pt] + col];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int pt = 0; pt < NPT; ++pt)
+                            acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u][pt], acc[mt][pt], 0, 0, 0);
+                }
+                for (; ks < nks; ++ks) {
+                    const int col = 4 * ks + lk;
+                    const float av = sY[aoff + col];
+#pragma unroll
+                    for (int pt = 0; pt < NPT; ++pt)
+                        acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sC[boff[pt] + col], acc[mt][pt], 0, 0, 0);
+                }
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
+            for (int pt = 0; pt < NPT; ++pt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = mt * 16 + 4 * lk + r, p = pt * 16 + li;
+                    if ((mt < NMT - 1 || 4 * lk + r < MLAST) && (pt < NPT - 1 || p < P)) sA[p * M + m] = acc[mt][pt][r];
+                }
+        __syncthreads();
+        for (int e = lane; e < P * M; e += 64) a.A[(size_t)row * (P * M) + e] = sA[e];
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = lane + 64 * q;
+            if (e < S) {
+                const int p = e / L1, l = e % L1;
+                double s = 0.0;
+                for (int m = l * l; m < (l + 1) * (l + 1); ++m) {
+                    const double v = sA[p * M + m];
+                    s += v * v;
+                }
+                a.x[(size_t)row * S + e] = (float)((ew[q] * s - emu[q]) * eis[q]);
+            }
         }
     }
 }
@@ -320,10 +477,13 @@ template <int LMAX, int K, int P, int MIX>
 static void run_fwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
     int grid = a.nrows < 262144 ? a.nrows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
+    // variant 0 = matrix-core kernel, 1 / 2 = vector kernels with 6x7 / 9x7 FP64 tiles
     if (a.variant == 1)
+        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, a, nl);
+    else if (a.variant == 2)
         hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 9>), dim3(grid), dim3(64), 0, s, a, nl);
     else
-        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, a, nl);
+        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, a, nl);
 }
 template <int LMAX, int K, int P, int MIX>
 static void run_bwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {

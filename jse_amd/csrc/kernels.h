@@ -28,7 +28,8 @@ struct LaunchTune {
     int mlp_wg_per_cu = 0;       // MLP: persistent workgroups per CU (0: 2)
     int nl_wg_per_cu = 0;        // neighbour build: persistent workgroups per CU (0: one per atom)
     int gather_waves = 0;        // force gather: atoms (waves) per workgroup (0: 4)
-    int desc_fwd_variant = 0;    // descriptor forward register tile: 0 = 6 x 7, 1 = 9 x 7
+    int desc_fwd_variant = 0;    // descriptor forward: 0 = matrix cores (FP32),
+                                 // 1 / 2 = vector ALU with FP64 tiles 6x7 / 9x7
     int desc_bwd_variant = 0;    // descriptor backward: scheduling barrier every 1 / 2 / no rows
 };
 constexpr int kTuneKnobs = 7;
