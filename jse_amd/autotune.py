@@ -18,6 +18,8 @@ from typing import Dict, Tuple
 from . import nnap
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_tuned.json")
+# bumped whenever a knob's meaning or range changes, so older entries are not applied
+SCHEMA = 2
 
 
 def _p2(v: float) -> int:
@@ -29,7 +31,7 @@ def signature(system: "nnap.System", what: int) -> str:
     st = system.nl_stats()
     mean_nb = st["pairs"] / max(1, system.natoms)
     sp = ";".join(f"P{s.P}K{s.K}L{s.lmax}H{'x'.join(map(str, s.hidden))}" for s in pot.model.species)
-    return (f"prec{pot.precision}|what{what}|{sp}|atoms{_p2(system.natoms)}|reps{_p2(system.nrep)}"
+    return (f"v{SCHEMA}|prec{pot.precision}|what{what}|{sp}|atoms{_p2(system.natoms)}|reps{_p2(system.nrep)}"
             f"|nb{8 * round(mean_nb / 8)}")
 
 

@@ -319,6 +319,7 @@ struct jse_nnap_system_s {
         launch_cell_scan(stream, nrep, nullptr, d_reps, nullptr, d_cell_cnt, d_cell_start);
         launch_cell_scatter(stream, n, nullptr, d_rep_of, nullptr, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw,
                             d_slot_of);
+        nl.types = d_types;
         launch_nl_build_cells(stream, ncell_total, d_cell_rep, d_reps, rc2, d_cell_start, d_sidx, d_sxw, nl, d_overflow,
                               tune.nl_wg_per_cu);
         if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
@@ -362,6 +363,7 @@ struct jse_nnap_system_s {
         a.x = d_x + (size_t)seg[t] * model->smax;
         a.A = d_A + (size_t)seg[t] * model->pmmax;
         a.G = d_G + (size_t)seg[t] * model->smax;
+        a.fp32_acc = precision == JSE_NNAP_BF16_MLP;
         return a;
     }
 
@@ -468,7 +470,7 @@ struct jse_nnap_system_s {
         JNN_CHECK(have_pos, ERR_ARG, "positions not set");
         JNN_CHECK(iters >= 1, ERR_ARG, "iters must be >= 1");
         static const std::vector<int> cand[kTuneKnobs] = {
-            {0, 4, 8, 16, 32}, {0, 4, 8, 16, 32}, {1, 2, 4, 8}, {0, 4, 8, 16}, {1, 2, 4, 8}, {0, 1, 2}, {0, 1, 2}};
+            {0, 4, 8, 16, 32}, {0, 4, 8, 16, 32}, {1, 2, 4, 8}, {0, 4, 8, 16}, {1, 2, 4, 8}, {0, 1}, {0, 1, 2}};
         drop_graphs();
         compute(what, true);  // settles the neighbour capacity
         finish();
@@ -681,7 +683,7 @@ int jse_nnap_system_set_tuning(jse_nnap_system s, const int *tune, int n) {
     return guard([&] {
         JNN_CHECK(s && tune && n >= 0 && n <= kTuneKnobs, ERR_ARG, "bad tuning vector");
         for (int k = 0; k < n; ++k) JNN_CHECK(tune[k] >= 0 && tune[k] <= 64, ERR_ARG, "tuning value out of range");
-        if (n > 5) JNN_CHECK(tune[5] <= 2, ERR_ARG, "descriptor forward variant out of range");
+        if (n > 5) JNN_CHECK(tune[5] <= 1, ERR_ARG, "descriptor forward variant out of range");
         if (n > 6) JNN_CHECK(tune[6] <= 2, ERR_ARG, "descriptor backward variant out of range");
         JNN_HIP(hipStreamSynchronize(s->stream));
         for (int k = 0; k < n; ++k) *s->tune_field(k) = tune[k];

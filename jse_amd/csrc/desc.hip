@@ -71,39 +71,38 @@ __device__ __forceinline__ void mix_rnp(const float *rn, int tj, const float *__
         }
     }
 }
-struct PairGeom {
-    float r, ct, st, cp, sp, dxy;
+// Pair geometry from hardware reciprocal square roots (about 1 ulp each): r,
+// 1/r, cos/sin theta, cos/sin phi, dxy = |(dx, dy)| and 1/dxy.  dx and dy are
+// rescaled by 2^64 when dx^2 + dy^2 would leave the normal range; a pair with
+// dxy below 8 FLT_MIN is on the pole (cos phi = 1, sin phi = 0), the
+// reference's dxy == 0 convention (basis_SphericalUtil.hpp:165-181) with a
+// guard against denormal dxy.
+struct PairGeomR {
+    float r, ir, ct, st, cp, sp, dxy, idxy;
     bool pole;
 };
-__device__ __forceinline__ PairGeom pair_geom(float dx, float dy, float dz) {
-    PairGeom g;
-    g.r = sqrtf(dx * dx + dy * dy + dz * dz);
-    g.dxy = hypotf(dx, dy);
-    g.ct = dz / g.r;
-    g.st = g.dxy / g.r;
+__device__ __forceinline__ PairGeomR pair_geom_fast(float dx, float dy, float dz) {
+    PairGeomR g;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    g.ir = __builtin_amdgcn_rsqf(r2);
+    g.r = r2 * g.ir;
+    float sx = dx, sy = dy, unscale = 1.f, rescale = 1.f;
+    float q = dx * dx + dy * dy;
+    if (q < 0x1p-100f) {
+        sx = dx * 0x1p+64f;
+        sy = dy * 0x1p+64f;
+        unscale = 0x1p-64f;
+        rescale = 0x1p+64f;
+        q = sx * sx + sy * sy;
+    }
+    const float iq = q > 0.f ? __builtin_amdgcn_rsqf(q) : 0.f;
+    g.dxy = (q * iq) * unscale;
     g.pole = !(g.dxy >= 8.f * 1.17549435e-38f);
-    g.cp = g.pole ? 1.f : dx / g.dxy;
-    g.sp = g.pole ? 0.f : dy / g.dxy;
-    return g;
-}
-
-// The same from hardware reciprocal square roots (about 1 ulp each): the
-// matrix-core forward's geometry.  Below FLT_MIN for dx^2 + dy^2 the pair is
-// taken as on the pole; the m != 0 harmonics there scale with sin(theta) and
-// vanish to FP32 precision.
-__device__ __forceinline__ PairGeom pair_geom_fast(float dx, float dy, float dz) {
-    PairGeom g;
-    const float q = dx * dx + dy * dy;
-    const float r2 = q + dz * dz;
-    const float ir = __builtin_amdgcn_rsqf(r2);
-    g.r = r2 * ir;
-    g.pole = !(q >= 1.17549435e-38f);
-    const float iq = g.pole ? 0.f : __builtin_amdgcn_rsqf(q);
-    g.dxy = q * iq;
-    g.ct = dz * ir;
-    g.st = g.dxy * ir;
-    g.cp = g.pole ? 1.f : dx * iq;
-    g.sp = g.pole ? 0.f : dy * iq;
+    g.st = g.dxy * g.ir;
+    g.ct = dz * g.ir;
+    g.cp = g.pole ? 1.f : sx * iq;
+    g.sp = g.pole ? 0.f : sy * iq;
+    g.idxy = g.pole ? 0.f : iq * rescale;
     return g;
 }
 
@@ -125,6 +124,16 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
     const int p0 = pb * TP, m0 = mb * TM;
     const bool tile_on = grp < G;
     const float rc = a.rc, inv_rc = 1.f / a.rc;
+    // per-lane descriptor slots e = lane + 64 q: 4 pi / (2l + 1), mu, 1 / sigma
+    constexpr int NE = (S + 63) / 64;
+    double ew[NE], emu[NE], eis[NE];
+#pragma unroll
+    for (int q = 0; q < NE; ++q) {
+        const int e = lane + 64 * q;
+        ew[q] = 4.0 * 3.14159265358979323846 / (double)(2 * (e % L1) + 1);
+        emu[q] = e < S ? a.mu64[e] : 0.0;
+        eis[q] = e < S ? 1.0 / a.sig64[e] : 0.0;
+    }
 
     for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
@@ -138,36 +147,27 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
             for (int v = 0; v < TM; ++v) acc[u][v] = 0.0;
 
         for (int c0 = 0; c0 < n; c0 += 64) {
+            // lanes past the last neighbour evaluate a valid pair; phase 2 reads only
+            // the first nc rows
             const int j = c0 + lane;
+            const int jr = j < n ? j : n - 1;
+            const float4 dd = nl.d[base + jr];
+            const int tj = __float_as_int(dd.w);
+            const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
             float y[M];
-            float c[P];
-#pragma unroll
-            for (int m = 0; m < M; ++m) y[m] = 0.f;
-#pragma unroll
-            for (int p = 0; p < P; ++p) c[p] = 0.f;
-            if (j < n) {
-                const float4 dd = nl.d[base + j];
-                const float dx = dd.x, dy = dd.y, dz = dd.z;
-                const int tj = a.types[nl.key[base + j] >> 5];
-                const PairGeom g = pair_geom(dx, dy, dz);
-                if (g.r < rc) {
-                    real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
-                    float rn[K];
-                    cheb_rn<K>(1.f - 2.f * g.r * inv_rc, rn);
-                    const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-                    const float t2 = t * t;
-                    const float fc = t2 * t2;
-                    float rnp[P];
-                    mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
-#pragma unroll
-                    for (int p = 0; p < P; ++p) c[p] = fc * rnp[p];
-                }
-            }
+            real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
+            float rn[K];
+            cheb_rn<K>(1.f - 2.f * g.r * inv_rc, rn);
+            const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+            const float t2 = t * t;
+            const float fc = g.r < rc ? t2 * t2 : 0.f;  // pairs beyond this species' cutoff add nothing
+            float rnp[P];
+            mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
             __syncthreads();  // the previous chunk's tiles (and the previous atom's A) are done
 #pragma unroll
             for (int m = 0; m < M; ++m) sY[lane * M + m] = y[m];
 #pragma unroll
-            for (int p = 0; p < P; ++p) sC[lane * P + p] = c[p];
+            for (int p = 0; p < P; ++p) sC[lane * P + p] = fc * rnp[p];
             __syncthreads();
             const int nc = min(64, n - c0);
             if (tile_on) {
@@ -199,28 +199,34 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
             __syncthreads();
         }
         for (int e = lane; e < P * M; e += 64) a.A[(size_t)row * (P * M) + e] = (float)sA[e];
-        for (int e = lane; e < S; e += 64) {
-            const int p = e / L1, l = e % L1;
-            double s = 0.0;
-            for (int m = l * l; m < (l + 1) * (l + 1); ++m) {
-                const double q = sA[p * M + m];
-                s += q * q;
+#pragma unroll
+        for (int q = 0; q < NE; ++q) {
+            const int e = lane + 64 * q;
+            if (e < S) {
+                const int p = e / L1, l = e % L1;
+                double s = 0.0;
+                for (int m = l * l; m < (l + 1) * (l + 1); ++m) {
+                    const double v = sA[p * M + m];
+                    s += v * v;
+                }
+                a.x[(size_t)row * S + e] = (float)((ew[q] * s - emu[q]) * eis[q]);
             }
-            const double fp = (4.0 * 3.14159265358979323846 / (double)(2 * l + 1)) * s;
-            a.x[(size_t)row * S + e] = (float)((fp - a.mu64[e]) / a.sig64[e]);
         }
     }
 }
 
-// Forward on the matrix cores (v_mfma_f32_16x16x4_f32: FP32 products, FP32
-// accumulation).  Per atom, A^T (M x P) = sum over k-steps of
+// Forward on the matrix cores for the BF16-MLP precision mode
+// (v_mfma_f32_16x16x4_f32: FP32 products, FP32 accumulation; rounding A to
+// FP32 alone moves the normalised descriptor by ~4e-7 RMS, which that mode's
+// budget absorbs and the plain FP32 path's does not).  Per atom,
+// A^T (M x P) = sum over k-steps of
 // Y^T (16 m x 4 neighbours) * C (4 neighbours x 16 p).  Lanes first compute one
 // neighbour each: c_j goes to LDS as C^T[p][j]; Y_j stays in registers and is
 // staged into LDS one 16-row m-tile at a time as Y^T[m][j].  Each m-tile then
 // runs ceil(n/4) k-steps against the p-tiles of C, four k-steps per batch of
 // operand reads.  Lanes of padding rows (m >= M, p >= P) read one zero row of
-// LDS; lanes past the last neighbour evaluate a valid pair with c = 0.  The epilogue writes the
-// tiles to LDS as A[p][m] and forms fp and x in FP64.
+// LDS; lanes past the last neighbour evaluate a valid pair with c = 0.  The
+// epilogue writes the tiles to LDS as A[p][m] and forms fp and x in FP64.
 // Lane maps of the 16x16x4 instruction: A[i][k] from lane i + 16 k,
 // B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i / 4), register i % 4.
 // A row stride of 68 floats puts the 64 lanes of every operand read on 64
@@ -267,14 +273,14 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentArgs a, NeighborList 
 #pragma unroll
         for (int mt = 0; mt < NMT; ++mt)
 #pragma unroll
-            for (int pt = 0; pt < NPT; ++pt) acc[mt][pt] = f4{0.f, 0.f, 0.f, 0.f};
+            for (int pt = 0; pt < NPT; ++pt) acc[mt][pt] = f4{0, 0, 0, 0};
 
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int j = c0 + lane;
             const int jr = j < n ? j : n - 1;
             const float4 dd = nl.d[base + jr];
-            const int tj = a.types[nl.key[base + jr] >> 5];
-            const PairGeom g = pair_geom_fast(dd.x, dd.y, dd.z);
+            const int tj = __float_as_int(dd.w);
+            const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
             float y[M];
             real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
             float rn[K];
@@ -378,8 +384,7 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
         const float *__restrict__ Grow = a.G + (size_t)row * S;
         for (int e = lane; e < K * M; e += 64) {
             const int q = e / M, m = e % M;
-            int l = 0;
-            while ((l + 1) * (l + 1) <= m) ++l;
+            const int l = (int)__builtin_amdgcn_sqrtf((float)m + 0.5f);  // l(m) = floor(sqrt(m))
             if (MIX == MIX_NONE) {
                 sGt[e] = Grow[q * L1 + l] * Arow[q * M + m];
             } else if (MIX == MIX_DEFAULT) {
@@ -404,8 +409,8 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
             if (j >= n) continue;
             const float4 dd = nl.d[base + j];
             const float dx = dd.x, dy = dd.y, dz = dd.z;
-            const int tj = a.types[nl.key[base + j] >> 5];
-            const PairGeom g = pair_geom(dx, dy, dz);
+            const int tj = __float_as_int(dd.w);
+            const PairGeomR g = pair_geom_fast(dx, dy, dz);
             float ox = 0.f, oy = 0.f, oz = 0.f;
             if (g.r < rc) {
                 float y[M];
@@ -427,13 +432,15 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
 #pragma unroll
                 for (int q = 0; q < K; ++q) {
                     const float cq = fc * rn[q];
-                    float s = 0.f;
+                    // four interleaved partial sums: shorter FP32 accumulation chains
+                    float s4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
                     for (int m = 0; m < M; ++m) {
                         const float ga = sG[q * M + m];
                         gy[m] += ga * cq;
-                        s += ga * y[m];
+                        s4[m & 3] += ga * y[m];
                     }
+                    const float s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
                     gfc += s * rn[q];
                     grn[q] = s * fc;
                     if (BAR > 0 && q % BSTEP == BSTEP - 1) __builtin_amdgcn_sched_barrier(0);
@@ -445,23 +452,26 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
                 for (int q = 0; q < K; ++q) dEdr += grn[q] * drn[q];
                 const float dfc = -8.f * g.r * inv_rc * inv_rc * t2 * t;
                 dEdr += gfc * dfc;
-                const float dEdth = dtheta_contract<LMAX>(g.ct, g.st, g.cp, g.sp, gy);
                 float dEdph = 0.f;
 #pragma unroll
-                for (int l = 1; l < L1; ++l)
+                for (int l = 1; l < L1; ++l) {
+                    float pl = 0.f;
 #pragma unroll
                     for (int m = 1; m <= l; ++m) {
                         const int idx = l * l + l;
                         // dY_{l,m}/dphi = -m Y_{l,-m},  dY_{l,-m}/dphi = m Y_{l,m}
-                        dEdph += (float)m * (gy[idx - m] * y[idx + m] - gy[idx + m] * y[idx - m]);
+                        pl += (float)m * (gy[idx - m] * y[idx + m] - gy[idx + m] * y[idx - m]);
                     }
-                const float ir = 1.f / g.r;
+                    dEdph += pl;
+                }
+                const float dEdth = dtheta_contract<LMAX>(g.ct, g.st, g.cp, g.sp, gy);
+                const float ir = g.ir;
                 // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi))
                 ox = -(dEdr * dx * ir + dEdth * g.ct * g.cp * ir);
                 oy = -(dEdr * dy * ir + dEdth * g.ct * g.sp * ir);
                 oz = -(dEdr * dz * ir - dEdth * g.st * ir);
                 if (!g.pole) {
-                    const float idxy = 1.f / g.dxy;
+                    const float idxy = g.idxy;
                     ox -= dEdph * (-g.sp * idxy);
                     oy -= dEdph * (g.cp * idxy);
                 }
@@ -477,13 +487,14 @@ template <int LMAX, int K, int P, int MIX>
 static void run_fwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
     int grid = a.nrows < 262144 ? a.nrows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
-    // variant 0 = matrix-core kernel, 1 / 2 = vector kernels with 6x7 / 9x7 FP64 tiles
-    if (a.variant == 1)
-        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, a, nl);
-    else if (a.variant == 2)
+    // FP32 accumulation (BF16-MLP mode): the matrix-core kernel; FP64 accumulation:
+    // vector kernel with 6x7 (variant 0) or 9x7 (variant 1) register tiles
+    if (a.fp32_acc)
+        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, a, nl);
+    else if (a.variant == 1)
         hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 9>), dim3(grid), dim3(64), 0, s, a, nl);
     else
-        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, a, nl);
+        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, a, nl);
 }
 template <int LMAX, int K, int P, int MIX>
 static void run_bwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {

@@ -119,9 +119,10 @@ __device__ __forceinline__ float dtheta_contract(float ct, float st, float cp, f
         p[l] = pll;
         d[l] = dpll;
         const int idx = l * l + l;
-        acc += g[idx] * d[0];
+        float al = g[idx] * d[0];  // per-l partial sum: shorter FP32 accumulation chains
 #pragma unroll
-        for (int m = 1; m <= l; ++m) acc += kSqrt2 * d[m] * (g[idx + m] * cmv[m] + g[idx - m] * smv[m]);
+        for (int m = 1; m <= l; ++m) al += kSqrt2 * d[m] * (g[idx + m] * cmv[m] + g[idx - m] * smv[m]);
+        acc += al;
 #pragma unroll
         for (int m = 0; m <= l; ++m) {
             p2[m] = p1[m];

@@ -28,8 +28,8 @@ struct LaunchTune {
     int mlp_wg_per_cu = 0;       // MLP: persistent workgroups per CU (0: 2)
     int nl_wg_per_cu = 0;        // neighbour build: persistent workgroups per CU (0: one per atom)
     int gather_waves = 0;        // force gather: atoms (waves) per workgroup (0: 4)
-    int desc_fwd_variant = 0;    // descriptor forward: 0 = matrix cores (FP32),
-                                 // 1 / 2 = vector ALU with FP64 tiles 6x7 / 9x7
+    int desc_fwd_variant = 0;    // descriptor forward (FP64 accumulation): register tile 6x7 / 9x7;
+                                 // the BF16-MLP mode's matrix-core forward has no variants
     int desc_bwd_variant = 0;    // descriptor backward: scheduling barrier every 1 / 2 / no rows
 };
 constexpr int kTuneKnobs = 7;
@@ -54,7 +54,10 @@ struct NeighborList {
     int cap = 0;
     int *cnt = nullptr;     // [N] true neighbour count (may exceed cap -> overflow)
     int *key = nullptr;     // [N*cap]
-    float4 *d = nullptr;    // [N*cap] (dx, dy, dz, 0): FP32 of the FP64 displacement r_j + shift - r_i
+    float4 *d = nullptr;    // [N*cap] (dx, dy, dz, t_j): FP32 of the FP64 displacement r_j + shift - r_i and
+                            // the model species of j (int bits), so the descriptor kernels skip key -> type
+    const int *types = nullptr;  // [atoms] model species (1-based) of the atoms the keys name; read by
+                                 // the builders that write d
     int *rev = nullptr;     // [N*cap] slot of the reverse pair in j's row, -1 if absent (statistics only)
     float4 *g = nullptr;    // [N*cap] (gx, gy, gz, 0): g_j = -dE_i/d(r_j - r_i)
 };
@@ -125,6 +128,7 @@ struct SegmentArgs {
     float *G;               // [nrows][S]  2*(4pi/(2l+1)) * dE/dfp
     int wg_per_cu;          // persistent workgroups per CU (0: one workgroup per row)
     int variant;            // compiled schedule variant (autotuned; 0: default)
+    int fp32_acc;           // 1: the matrix-core forward accumulates in FP32 (BF16-MLP precision mode)
 };
 // returns false when no kernel is compiled for this (lmax, K, P, mix)
 bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);

@@ -240,6 +240,7 @@ int jse_nnap_computeLammpsCuda(void *args) {
             if (inum) JNN_HIP(hipMemcpyAsync(c.rows, rows.data(), sizeof(int) * inum, hipMemcpyHostToDevice, s));
             launch_lmp_types(s, nall, type, tmap, c.types);
             launch_fill_int(s, 1, c.overflow, 0);
+            c.nl.types = c.types;
             launch_lmp_rows(s, inum, ilist, numneigh, firstneigh, c.x, mh.rcut_max() * mh.rcut_max(), c.nl, c.overflow);
             JNN_HIP(hipMemcpyAsync(c.h_overflow, c.overflow, sizeof(int), hipMemcpyDeviceToHost, s));
             JNN_HIP(hipStreamSynchronize(s));
@@ -262,6 +263,7 @@ int jse_nnap_computeLammpsCuda(void *args) {
             sa.sig = dv.sig;
             sa.mu64 = dv.mu64;
             sa.sig64 = dv.sig64;
+            sa.fp32_acc = c.precision == JSE_NNAP_BF16_MLP;
             sa.x = c.xs + (size_t)seg[t] * c.model->smax;
             sa.A = c.A + (size_t)seg[t] * c.model->pmmax;
             sa.G = c.G + (size_t)seg[t] * c.model->smax;

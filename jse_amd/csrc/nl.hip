@@ -230,7 +230,7 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
         const size_t row = (size_t)i * nl.cap;
         for (int t = lane; t < nw; t += kNlBlock) {
             nl.key[row + t] = s_key[t];
-            if (nl.d) nl.d[row + t] = make_float4(s_d[0][t], s_d[1][t], s_d[2][t], 0.f);
+            if (nl.d) nl.d[row + t] = make_float4(s_d[0][t], s_d[1][t], s_d[2][t], __int_as_float(nl.types[s_key[t] >> 5]));
         }
         if (lane == 0) {
             nl.cnt[i] = hits;
@@ -399,7 +399,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                 wave_sort64(key, dx, dy, dz, lane);
                 if (lane < nw) {
                     nl.key[row + lane] = key;
-                    if (nl.d) nl.d[row + lane] = make_float4(dx, dy, dz, 0.f);
+                    if (nl.d) nl.d[row + lane] = make_float4(dx, dy, dz, __int_as_float(nl.types[key >> 5]));
                 }
             } else {
                 int p2 = 128;
@@ -427,7 +427,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                 }
                 for (int t = lane; t < nw; t += 64) {
                     nl.key[row + t] = hkey[t];
-                    if (nl.d) nl.d[row + t] = make_float4(hdx[t], hdy[t], hdz[t], 0.f);
+                    if (nl.d) nl.d[row + t] = make_float4(hdx[t], hdy[t], hdz[t], __int_as_float(nl.types[hkey[t] >> 5]));
                 }
             }
             if (lane == 0) {

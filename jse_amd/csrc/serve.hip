@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_srv_refresh(int n, const int *__restric
         const int pos = hits + __popcll(below);
         if (hit && pos < nl.cap) {
             nl.key[row + pos] = key;
-            nl.d[row + pos] = make_float4(fdx, fdy, fdz, 0.f);
+            nl.d[row + pos] = make_float4(fdx, fdy, fdz, __int_as_float(nl.types[key >> 5]));
         }
         hits += __popcll(mask);
     }

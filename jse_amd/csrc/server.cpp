@@ -552,6 +552,8 @@ struct jse_nnap_server_s {
         launch_wrap_bin(s, na, d_atoms, x, rep_of, d_reps, d_need, xw, cell, cell_cnt, rank);
         launch_cell_scan(s, nr, d_rl, d_reps, d_need, cell_cnt, cell_start);
         launch_cell_scatter(s, na, d_atoms, rep_of, d_need, cell, rank, cell_start, xw, sidx, sxw);
+        vl.types = types;
+        nl.types = types;
         launch_nl_build(s, na, d_atoms, rep_of, d_reps, d_need, rv * rv, xw, cell, cell_start, sidx, sxw, vl,
                         L.d_flags);
         // rows within rc at the current positions
@@ -601,6 +603,7 @@ struct jse_nnap_server_s {
         a.x = L.d_x + (size_t)seg[t] * model->smax;
         a.A = L.d_A + (size_t)seg[t] * model->pmmax;
         a.G = L.d_G + (size_t)seg[t] * model->smax;
+        a.fp32_acc = precision == JSE_NNAP_BF16_MLP;
         return a;
     }
 

@@ -160,6 +160,7 @@ def test_mix_with_evictions_within_bar(O):
         r = reps[k]
         ref = orc.system_efv(W.Structure(r.box, trajs[k][s], r.types))
         e, f_rms, f_max, v = _bar(res, ref, r.natoms)
+        print("replica", k, "step", s, "e %.2e f_rms %.2e f_max %.2e v %.2e" % (e, f_rms, f_max, v))
         assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-4 and v <= 1e-5, (k, s)
 
 
