@@ -9,8 +9,10 @@
 // sums are deterministic and need no atomics;
 // each lane sums its slots in FP64 and the wave combines lanes in a fixed
 // butterfly, the totals use Neumaier-compensated FP64 partial sums.  The
-// virial of an incoming pair uses that pair's displacement from j's row, the
-// FP64 (x_i + s L) - x_j of the neighbour build rounded once to FP32.
+// virial of an incoming pair j -> i takes its displacement r_i - r_j as the
+// negated displacement of the pair i -> j in i's own row (read coalesced;
+// j's row holds the same vector formed as (x_i + s L) - x_j, which can differ
+// from it in the last FP32 bit).
 #include "kernels.h"
 
 namespace jnn {
@@ -54,6 +56,7 @@ __global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ a
         f[1] -= (double)go.y;
         f[2] -= (double)go.z;
         const int key = nl.key[base + k];
+        const float4 di = want_virial ? nl.d[base + k] : make_float4(0.f, 0.f, 0.f, 0.f);
         int jrow;
         const int r = find_reverse(nl, ia, key, row_of, &jrow);  // binary search in j's sorted row
         if (r < 0) continue;
@@ -64,9 +67,8 @@ __global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ a
         f[1] += gy;
         f[2] += gz;
         if (want_virial) {
-            // displacement of the pair j -> i as the neighbour build formed it (FP64, rounded once)
-            const float4 dj = nl.d[sj];
-            const double dx = (double)dj.x, dy = (double)dj.y, dz = (double)dj.z;
+            // displacement of the pair j -> i: r_i - r_j = -(r_j - r_i)
+            const double dx = -(double)di.x, dy = -(double)di.y, dz = -(double)di.z;
             v[0] += dx * gx; v[1] += dy * gy; v[2] += dz * gz;
             v[3] += dx * gy; v[4] += dx * gz; v[5] += dy * gz;
             v[6] += dy * gx; v[7] += dz * gx; v[8] += dz * gy;
