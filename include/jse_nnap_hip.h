@@ -144,9 +144,10 @@ int jse_nnap_system_nl_stats(jse_nnap_system s, int *cap, int *max_count, long l
  * descriptor-backward persistent workgroups per CU (0 = one per atom),
  * tune[2] MLP workgroups per CU, tune[3] neighbour-build workgroups per CU
  * (0 = one per atom), tune[4] force-gather atoms per workgroup, tune[5] the
- * descriptor-forward register tile of the FP32 precision mode (0: 6x7, 1: 9x7
- * FP64 accumulators; the BF16-MLP mode runs the matrix-core forward whatever
- * this knob says), tune[6] the
+ * descriptor forward of the FP32 precision mode (0: matrix cores, FP32 sums
+ * of 16 neighbours added into FP64; 1 / 2: vector kernel with 6x7 / 9x7 FP64
+ * register tiles; the BF16-MLP mode runs the FP32-accumulating matrix-core
+ * forward whatever this knob says), tune[6] the
  * descriptor-backward schedule (scheduling barrier every 1 / 2 / no rows).
  * Writes the chosen knobs (7 ints) and the tuned device time per evaluation. */
 int jse_nnap_system_autotune(jse_nnap_system s, int what, int iters, int *tune, double *ms_step);

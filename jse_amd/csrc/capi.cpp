@@ -470,7 +470,7 @@ struct jse_nnap_system_s {
         JNN_CHECK(have_pos, ERR_ARG, "positions not set");
         JNN_CHECK(iters >= 1, ERR_ARG, "iters must be >= 1");
         static const std::vector<int> cand[kTuneKnobs] = {
-            {0, 4, 8, 16, 32}, {0, 4, 8, 16, 32}, {1, 2, 4, 8}, {0, 4, 8, 16}, {1, 2, 4, 8}, {0, 1}, {0, 1, 2}};
+            {0, 4, 8, 16, 32}, {0, 4, 8, 16, 32}, {1, 2, 4, 8}, {0, 4, 8, 16}, {1, 2, 4, 8}, {0, 1, 2}, {0, 1, 2}};
         drop_graphs();
         compute(what, true);  // settles the neighbour capacity
         finish();
@@ -683,7 +683,7 @@ int jse_nnap_system_set_tuning(jse_nnap_system s, const int *tune, int n) {
     return guard([&] {
         JNN_CHECK(s && tune && n >= 0 && n <= kTuneKnobs, ERR_ARG, "bad tuning vector");
         for (int k = 0; k < n; ++k) JNN_CHECK(tune[k] >= 0 && tune[k] <= 64, ERR_ARG, "tuning value out of range");
-        if (n > 5) JNN_CHECK(tune[5] <= 1, ERR_ARG, "descriptor forward variant out of range");
+        if (n > 5) JNN_CHECK(tune[5] <= 2, ERR_ARG, "descriptor forward variant out of range");
         if (n > 6) JNN_CHECK(tune[6] <= 2, ERR_ARG, "descriptor backward variant out of range");
         JNN_HIP(hipStreamSynchronize(s->stream));
         for (int k = 0; k < n; ++k) *s->tune_field(k) = tune[k];

@@ -215,34 +215,41 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
     }
 }
 
-// Forward on the matrix cores for the BF16-MLP precision mode
-// (v_mfma_f32_16x16x4_f32: FP32 products, FP32 accumulation; rounding A to
-// FP32 alone moves the normalised descriptor by ~4e-7 RMS, which that mode's
-// budget absorbs and the plain FP32 path's does not).  Per atom,
-// A^T (M x P) = sum over k-steps of
+// Forward on the matrix cores (v_mfma_f32_16x16x4_f32).  F64 = false (the
+// BF16-MLP mode): A accumulates in FP32 through all k-steps.  F64 = true (the
+// plain FP32 mode): every batch of four k-steps (16 neighbours) is summed in
+// FP32 and then added to FP64 accumulators, and fp is formed from the FP64 A;
+// this bounds the FP32 error to 16-term partial sums, where accumulating A in
+// FP32 throughout (and rounding it to FP32 before fp) moves the normalised
+// descriptor by ~9e-7 RMS.  Per atom, A^T (M x P) = sum over k-steps of
 // Y^T (16 m x 4 neighbours) * C (4 neighbours x 16 p).  Lanes first compute one
 // neighbour each: c_j goes to LDS as C^T[p][j]; Y_j stays in registers and is
 // staged into LDS one 16-row m-tile at a time as Y^T[m][j].  Each m-tile then
 // runs ceil(n/4) k-steps against the p-tiles of C, four k-steps per batch of
 // operand reads.  Lanes of padding rows (m >= M, p >= P) read one zero row of
 // LDS; lanes past the last neighbour evaluate a valid pair with c = 0.  The
-// epilogue writes the tiles to LDS as A[p][m] and forms fp and x in FP64.
+// epilogue writes the tiles to LDS as A[p][m] (FP64 when F64) and forms fp
+// and x in FP64.
 // Lane maps of the 16x16x4 instruction: A[i][k] from lane i + 16 k,
 // B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i / 4), register i % 4.
 // A row stride of 68 floats puts the 64 lanes of every operand read on 64
 // distinct banks.
-template <int LMAX, int K, int P, int MIX>
+template <int LMAX, int K, int P, int MIX, bool F64>
 __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentArgs a, NeighborList nl) {
+    using AT = typename std::conditional<F64, double, float>::type;  // A's precision after the k-steps
     constexpr int L1 = LMAX + 1, M = L1 * L1, S = P * L1;
     constexpr int NMT = (M + 15) / 16, NPT = (P + 15) / 16;
     constexpr int MLAST = M - 16 * (NMT - 1);  // data rows of the last m-tile
     constexpr int NS = 68;
-    constexpr int FC = (P + 1) * NS, FY = 16 * NS, FA = P * M;  // C^T rows and the zero row P
+    constexpr int FC = (P + 1) * NS, FY = 16 * NS, FA = P * M * (F64 ? 2 : 1);  // C^T rows, zero row P
     constexpr int F = FC + (FY > FA ? FY : FA);
     constexpr int NE = (S + 63) / 64;
+    static_assert(FC % 2 == 0, "FP64 A in LDS needs 8-byte alignment");
     __shared__ __align__(16) float sbuf[F];
-    float *sC = sbuf, *sY = sbuf + FC, *sA = sbuf + FC;
+    float *sC = sbuf, *sY = sbuf + FC;
+    AT *sA = reinterpret_cast<AT *>(sbuf + FC);
     using f4 = __attribute__((__vector_size__(4 * sizeof(float)))) float;
+    constexpr f4 kZero = {0.f, 0.f, 0.f, 0.f};
     const int lane = threadIdx.x;
     const int li = lane & 15, lk = lane >> 4;
     const float rc = a.rc, inv_rc = 1.f / a.rc;
@@ -269,11 +276,21 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentArgs a, NeighborList 
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
+        // F64: acc holds one batch's FP32 partial sum and a64 the FP64 totals
         f4 acc[NMT][NPT];
+        AT a64[F64 ? NMT : 1][F64 ? NPT : 1][4];
 #pragma unroll
         for (int mt = 0; mt < NMT; ++mt)
 #pragma unroll
-            for (int pt = 0; pt < NPT; ++pt) acc[mt][pt] = f4{0, 0, 0, 0};
+            for (int pt = 0; pt < NPT; ++pt) acc[mt][pt] = kZero;
+        if constexpr (F64) {
+#pragma unroll
+            for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
+                for (int pt = 0; pt < NPT; ++pt)
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) a64[mt][pt][r] = 0.0;
+        }
 
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int j = c0 + lane;
@@ -316,14 +333,30 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentArgs a, NeighborList 
                     for (int u = 0; u < 4; ++u)
 #pragma unroll
                         for (int pt = 0; pt < NPT; ++pt)
-                            acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u][pt], acc[mt][pt], 0, 0, 0);
-                }
-                for (; ks < nks; ++ks) {
-                    const int col = 4 * ks + lk;
-                    const float av = sY[aoff + col];
+                            acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                av[u], bv[u][pt], (F64 && u == 0) ? kZero : acc[mt][pt], 0, 0, 0);
+                    if constexpr (F64) {
 #pragma unroll
-                    for (int pt = 0; pt < NPT; ++pt)
-                        acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sC[boff[pt] + col], acc[mt][pt], 0, 0, 0);
+                        for (int pt = 0; pt < NPT; ++pt)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) a64[mt][pt][r] += (double)acc[mt][pt][r];
+                    }
+                }
+                if (ks < nks) {  // the last (shorter) batch
+                    for (int u = 0; ks < nks; ++ks, ++u) {
+                        const int col = 4 * ks + lk;
+                        const float av = sY[aoff + col];
+#pragma unroll
+                        for (int pt = 0; pt < NPT; ++pt)
+                            acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
+                                av, sC[boff[pt] + col], (F64 && u == 0) ? kZero : acc[mt][pt], 0, 0, 0);
+                    }
+                    if constexpr (F64) {
+#pragma unroll
+                        for (int pt = 0; pt < NPT; ++pt)
+#pragma unroll
+                            for (int r = 0; r < 4; ++r) a64[mt][pt][r] += (double)acc[mt][pt][r];
+                    }
                 }
             }
         }
@@ -335,10 +368,13 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentArgs a, NeighborList 
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int m = mt * 16 + 4 * lk + r, p = pt * 16 + li;
-                    if ((mt < NMT - 1 || 4 * lk + r < MLAST) && (pt < NPT - 1 || p < P)) sA[p * M + m] = acc[mt][pt][r];
+                    if ((mt < NMT - 1 || 4 * lk + r < MLAST) && (pt < NPT - 1 || p < P)) {
+                        if constexpr (F64) sA[p * M + m] = a64[mt][pt][r];
+                        else sA[p * M + m] = acc[mt][pt][r];
+                    }
                 }
         __syncthreads();
-        for (int e = lane; e < P * M; e += 64) a.A[(size_t)row * (P * M) + e] = sA[e];
+        for (int e = lane; e < P * M; e += 64) a.A[(size_t)row * (P * M) + e] = (float)sA[e];
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
             const int e = lane + 64 * q;
@@ -487,14 +523,17 @@ template <int LMAX, int K, int P, int MIX>
 static void run_fwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
     int grid = a.nrows < 262144 ? a.nrows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
-    // FP32 accumulation (BF16-MLP mode): the matrix-core kernel; FP64 accumulation:
-    // vector kernel with 6x7 (variant 0) or 9x7 (variant 1) register tiles
+    // BF16-MLP mode: the matrix-core kernel with FP32 accumulation.  Plain FP32 mode:
+    // variant 0 = matrix cores with FP64 totals per 16 neighbours; 1 / 2 = vector kernel
+    // with 6x7 / 9x7 FP64 register tiles
     if (a.fp32_acc)
-        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, a, nl);
+        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX, false>), dim3(grid), dim3(64), 0, s, a, nl);
     else if (a.variant == 1)
+        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, a, nl);
+    else if (a.variant == 2)
         hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 9>), dim3(grid), dim3(64), 0, s, a, nl);
     else
-        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, a, nl);
+        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX, true>), dim3(grid), dim3(64), 0, s, a, nl);
 }
 template <int LMAX, int K, int P, int MIX>
 static void run_bwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {

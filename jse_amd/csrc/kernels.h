@@ -28,8 +28,9 @@ struct LaunchTune {
     int mlp_wg_per_cu = 0;       // MLP: persistent workgroups per CU (0: 2)
     int nl_wg_per_cu = 0;        // neighbour build: persistent workgroups per CU (0: one per atom)
     int gather_waves = 0;        // force gather: atoms (waves) per workgroup (0: 4)
-    int desc_fwd_variant = 0;    // descriptor forward (FP64 accumulation): register tile 6x7 / 9x7;
-                                 // the BF16-MLP mode's matrix-core forward has no variants
+    int desc_fwd_variant = 0;    // descriptor forward, plain FP32 mode: 0 = matrix cores with FP64 totals,
+                                 // 1 / 2 = vector kernel with 6x7 / 9x7 FP64 register tiles; the BF16-MLP
+                                 // mode's matrix-core forward has no variants
     int desc_bwd_variant = 0;    // descriptor backward: scheduling barrier every 1 / 2 / no rows
 };
 constexpr int kTuneKnobs = 7;

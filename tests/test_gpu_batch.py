@@ -105,9 +105,10 @@ def test_autotuned_launch_shapes_keep_results():
     a row and how instructions are ordered, never a row's arithmetic: in the
     BF16-MLP mode (whose matrix-core forward ignores the forward-tile knob)
     the tuned knobs reproduce the default shapes' results bit for bit.  In
-    FP32 mode the two forward register tiles sum the same exact FP64
-    products in different orders: results agree to FP64 rounding carried
-    through the FP32 outputs."""
+    FP32 mode the forward variants sum the same FP32 products in different
+    orders (16-neighbour FP32 partial sums added into FP64 on the matrix
+    cores; FP64 register tiles on the vector ALU): results agree to within
+    the FP32 bar."""
     from jse_amd import autotune
     md = W.ternary_model()
     reps = W.replicas(24, 64, 1500, 29)
@@ -121,7 +122,7 @@ def test_autotuned_launch_shapes_keep_results():
         print("tuned", knobs, ms)
         # 0 = the kernel's built-in default shape kept
         assert knobs["mlp_wg_per_cu"] in (0, 1, 2, 4, 8) and knobs["gather_waves"] in (0, 1, 2, 4, 8)
-        assert knobs["desc_fwd_variant"] in (0, 1) and knobs["desc_bwd_variant"] in (0, 1, 2)
+        assert knobs["desc_fwd_variant"] in (0, 1, 2) and knobs["desc_bwd_variant"] in (0, 1, 2)
         b.set_tuning(knobs)
         b.compute(what)
         r1 = b.results()
@@ -137,13 +138,13 @@ def test_autotuned_launch_shapes_keep_results():
         b.compute(what)
         r0 = b.results()
         n = len(r0["energies"])
-        for fv in (1,):
+        for fv in (1, 2):
             b.set_tuning({"desc_fwd_variant": fv})
             b.compute(what)
             r2 = b.results()
             de = np.abs(r2["energies"] - r0["energies"])
             df = np.abs(r2["forces"] - r0["forces"])
             print("forward variant", fv, "max |dE_i|", de.max(), "max |dF|", df.max())
-            assert abs(r2["energy"] - r0["energy"]) / n <= 1e-9
-            assert de.max() <= 1e-6 and df.max() <= 1e-5
+            assert abs(r2["energy"] - r0["energy"]) / n <= 1e-6
+            assert de.max() <= 2e-5 and df.max() <= 2e-4
         b.close()
