@@ -93,8 +93,8 @@ jse_nnap_model_s::~jse_nnap_model_s() {
 }
 
 namespace jnn {
-void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision, int nrows, const int *rows_atom,
-                 const float *x, float *G, double *E, bool grad, int wg_per_cu) {
+MlpArgs mlp_args(const jse_nnap_model_s *m, int t, int nrows, const int *rows_atom, const float *x, float *G, double *E,
+                 bool grad, int wg_per_cu) {
     const SpeciesHost &sp = m->host.species[t];
     const SpeciesDev &dv = m->dev[t];
     MlpArgs a{};
@@ -115,11 +115,29 @@ void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision,
     a.G = G;
     a.want_grad = grad;
     a.wg_per_cu = wg_per_cu;
+    return a;
+}
+void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision, int nrows, const int *rows_atom,
+                 const float *x, float *G, double *E, bool grad, int wg_per_cu) {
+    const MlpArgs a = mlp_args(m, t, nrows, rows_atom, x, G, E, grad, wg_per_cu);
     if (precision == JSE_NNAP_BF16_MLP) {
         if (!launch_mlp_mfma_bf16(s, a))
             throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
     } else if (!launch_mlp_mfma_fp32(s, a)) {
         launch_mlp_fp32(s, a);
+    }
+}
+void enqueue_mlp_all(hipStream_t s, const jse_nnap_model_s *m, int precision, const MlpArgs *args, int nsp) {
+    if (launch_mlp_mfma_multi(s, args, nsp, precision == JSE_NNAP_BF16_MLP)) return;
+    for (int t = 0; t < nsp; ++t) {
+        const MlpArgs &a = args[t];
+        if (a.nrows <= 0) continue;
+        if (precision == JSE_NNAP_BF16_MLP) {
+            if (!launch_mlp_mfma_bf16(s, a))
+                throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
+        } else if (!launch_mlp_mfma_fp32(s, a)) {
+            launch_mlp_fp32(s, a);
+        }
     }
 }
 }  // namespace jnn
@@ -385,10 +403,13 @@ struct jse_nnap_system_s {
         const bool grad = (what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL)) != 0;
         desc_fwd(false);
         if (timed) JNN_HIP(hipEventRecord(ev[3], stream));
-        for (int t = 0; t < model->host.ntypes(); ++t) {
-            SegmentArgs sa = seg_args(t, false);
-            enqueue_mlp(stream, model, t, precision, sa.nrows, sa.rows_atom, sa.x, sa.G, d_E, grad,
-                        tune.mlp_wg_per_cu);
+        {
+            std::vector<MlpArgs> ma;
+            for (int t = 0; t < model->host.ntypes(); ++t) {
+                SegmentArgs sa = seg_args(t, false);
+                ma.push_back(mlp_args(model, t, sa.nrows, sa.rows_atom, sa.x, sa.G, d_E, grad, tune.mlp_wg_per_cu));
+            }
+            enqueue_mlp_all(stream, model, precision, ma.data(), (int)ma.size());
         }
         if (timed) JNN_HIP(hipEventRecord(ev[4], stream));
         if (grad) {

@@ -122,4 +122,9 @@ namespace jnn {
 // Launches the MLP of species t over `nrows` rows (FP32 MFMA, generic FP32, or BF16 MFMA).
 void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision, int nrows, const int *rows_atom,
                  const float *x, float *G, double *E, bool grad, int wg_per_cu = 0);
+// The MlpArgs of species t's rows.
+MlpArgs mlp_args(const jse_nnap_model_s *m, int t, int nrows, const int *rows_atom, const float *x, float *G, double *E,
+                 bool grad, int wg_per_cu = 0);
+// Every species' MLP: one launch when their shapes allow it, else one per species.
+void enqueue_mlp_all(hipStream_t s, const jse_nnap_model_s *m, int precision, const MlpArgs *args, int nsp);
 }  // namespace jnn

@@ -154,6 +154,17 @@ void launch_mlp_fp32(hipStream_t s, const MlpArgs &a);
 // MFMA kernels for 2 hidden layers of supported widths; false when the shape has no compiled kernel
 bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a);
 bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a);
+// All species in one launch: the persistent workgroups are split among the
+// species in proportion to their row tiles (each stages its species' weights
+// once).  False when the species' MLP shapes differ, there are more than
+// kMlpMaxSpecies of them or the shape has no compiled kernel.
+constexpr int kMlpMaxSpecies = 8;
+struct MlpMulti {
+    MlpArgs a[kMlpMaxSpecies];
+    int wg0[kMlpMaxSpecies + 1];  // first workgroup of each species, then the total
+    int nsp;
+};
+bool launch_mlp_mfma_multi(hipStream_t s, const MlpArgs *a, int nsp, bool bf16);
 
 // ---- forces / virials / totals ----
 // Row t of the list belongs to atom out_of[t] (nullptr: t) and atom j's row is

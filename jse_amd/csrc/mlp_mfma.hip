@@ -108,8 +108,9 @@ __device__ __forceinline__ void wave_lds_sync() {
 }
 
 // S <= KP (KP a multiple of 16 and of the instruction k), H1 = 16*C1, H2 = 16*C2.
+// Workgroup lb of lg takes row tiles lb, lb + lg, ...
 template <typename T, int C1, int C2, int KP>
-__global__ __launch_bounds__(256) void k_mlp2_mfma(MlpArgs a) {
+__device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     using MM = Mma<T>;
     constexpr int H1 = 16 * C1, H2 = 16 * C2;
     constexpr int NS = KP / 16;                                          // column tiles of gX
@@ -177,7 +178,7 @@ __global__ __launch_bounds__(256) void k_mlp2_mfma(MlpArgs a) {
     const int r0 = 16 * wave;
     T *hrow_w = Hs + r0 * HS;                         // this wave's 16 rows of H
     const T *hrow = Hs + (r0 + lr) * HS + lk;         // A-operand pointer into H
-    for (int tile = blockIdx.x; tile * 64 < a.nrows; tile += gridDim.x) {
+    for (int tile = lb; tile * 64 < a.nrows; tile += lg) {
         const int row0 = tile * 64;
         const int nt = min(64, a.nrows - row0);
         __syncthreads();
@@ -332,6 +333,19 @@ __global__ __launch_bounds__(256) void k_mlp2_mfma(MlpArgs a) {
     }
 }
 
+template <typename T, int C1, int C2, int KP>
+__global__ __launch_bounds__(256) void k_mlp2_mfma(MlpArgs a) {
+    mlp2_tiles<T, C1, C2, KP>(a, blockIdx.x, gridDim.x);
+}
+// one launch for all species: workgroups wg0[s] .. wg0[s+1]-1 serve species s
+template <typename T, int C1, int C2, int KP>
+__global__ __launch_bounds__(256) void k_mlp2_mfma_multi(MlpMulti m) {
+    const int b = blockIdx.x;
+    int sp = 0;
+    while (sp + 1 < m.nsp && b >= m.wg0[sp + 1]) ++sp;
+    mlp2_tiles<T, C1, C2, KP>(m.a[sp], b - m.wg0[sp], m.wg0[sp + 1] - m.wg0[sp]);
+}
+
 // ---- dispatch ----
 template <typename T, int C1, int C2, int KP>
 static void run_mlp2(hipStream_t s, const MlpArgs &a) {
@@ -358,6 +372,59 @@ static bool dispatch_mlp2(hipStream_t s, const MlpArgs &a) {
     JNN_MLP_SHAPE(64, 64, 128)
 #undef JNN_MLP_SHAPE
     return false;
+}
+
+template <typename T, int C1, int C2, int KP>
+static void run_mlp2_multi(hipStream_t s, const MlpMulti &m) {
+    hipLaunchKernelGGL((k_mlp2_mfma_multi<T, C1, C2, KP>), dim3(m.wg0[m.nsp]), dim3(256), 0, s, m);
+}
+template <typename T>
+static bool dispatch_mlp2_multi(hipStream_t s, const MlpMulti &m) {
+    const MlpArgs &a = m.a[0];
+    const int h1 = a.hidden[0], h2 = a.hidden[1];
+    const int kp = a.S <= 64 ? 64 : (a.S <= 128 ? 128 : (a.S <= 256 ? 256 : 0));
+#define JNN_MLP_SHAPE(H1_, H2_, KP_)                        \
+    if (h1 == H1_ && h2 == H2_ && kp == KP_) {              \
+        run_mlp2_multi<T, H1_ / 16, H2_ / 16, KP_>(s, m);   \
+        return true;                                        \
+    }
+    JNN_MLP_SHAPE(32, 32, 64)
+    JNN_MLP_SHAPE(32, 32, 128)
+    JNN_MLP_SHAPE(64, 64, 64)
+    JNN_MLP_SHAPE(64, 64, 128)
+#undef JNN_MLP_SHAPE
+    return false;
+}
+
+bool launch_mlp_mfma_multi(hipStream_t s, const MlpArgs *a, int nsp, bool bf16) {
+    if (nsp <= 0 || nsp > kMlpMaxSpecies) return false;
+    MlpMulti m{};
+    int tiles_tot = 0, first = -1;
+    for (int t = 0; t < nsp; ++t) {
+        if (a[t].nrows <= 0) continue;
+        if (first < 0) first = t;
+        const MlpArgs &f = a[first];
+        if (a[t].nlayers != 2 || a[t].S != f.S || a[t].hidden[0] != f.hidden[0] || a[t].hidden[1] != f.hidden[1])
+            return false;
+        tiles_tot += (a[t].nrows + 63) / 64;
+    }
+    if (first < 0) return true;  // no rows at all
+    const int per = a[first].wg_per_cu > 0 ? a[first].wg_per_cu : 2;
+    const int total = device_cu_count() * per;
+    // workgroups in proportion to each species' tiles, at least one, at most its tiles
+    int wg = 0;
+    for (int t = 0; t < nsp; ++t) {
+        if (a[t].nrows <= 0) continue;
+        const int tiles = (a[t].nrows + 63) / 64;
+        int w = (int)(((long long)total * tiles + tiles_tot - 1) / tiles_tot);
+        w = w < 1 ? 1 : (w > tiles ? tiles : w);
+        m.a[m.nsp] = a[t];
+        m.wg0[m.nsp] = wg;
+        ++m.nsp;
+        wg += w;
+    }
+    m.wg0[m.nsp] = wg;
+    return bf16 ? dispatch_mlp2_multi<__bf16>(s, m) : dispatch_mlp2_multi<float>(s, m);
 }
 
 bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a) {

@@ -566,9 +566,13 @@ struct jse_nnap_server_s {
             if (a.nrows && !launch_desc_fwd(s, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
                 throw Error(ERR_UNSUPPORTED, "no descriptor kernel compiled for this species shape");
         }
-        for (int t = 0; t < nt; ++t) {
-            SegmentArgs a = seg_args(L, t, seg, d_rows);
-            if (a.nrows) enqueue_mlp(s, model, t, precision, a.nrows, a.rows_atom, a.x, a.G, E, true);
+        {
+            std::vector<MlpArgs> ma;
+            for (int t = 0; t < nt; ++t) {
+                SegmentArgs a = seg_args(L, t, seg, d_rows);
+                ma.push_back(mlp_args(model, t, a.nrows, a.rows_atom, a.x, a.G, E, true));
+            }
+            enqueue_mlp_all(s, model, precision, ma.data(), nt);
         }
         for (int t = 0; t < nt; ++t) {
             const SpeciesHost &sp = model->host.species[t];
