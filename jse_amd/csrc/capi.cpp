@@ -127,6 +127,52 @@ void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision,
         launch_mlp_fp32(s, a);
     }
 }
+void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *segs, bool backward,
+                  const NeighborList &nl) {
+    const int nt = m->host.ntypes();
+    auto same_shape = [&](int t, int u) {
+        const SpeciesHost &a = m->host.species[t], &b = m->host.species[u];
+        return a.lmax == b.lmax && a.K() == b.K() && a.P == b.P && a.mix == b.mix;
+    };
+    int first = -1;
+    bool one = nt <= kSegMax;
+    for (int t = 0; t < nt; ++t) {
+        if (segs[t].nrows <= 0) continue;
+        if (first < 0) first = t;
+        else if (!same_shape(t, first)) one = false;
+    }
+    if (first < 0) return;
+    auto fail = [&](int t) {
+        const SpeciesHost &sp = m->host.species[t];
+        throw Error(ERR_UNSUPPORTED, std::string("no descriptor ") + (backward ? "backward " : "") +
+                                         "kernel compiled for lmax=" + std::to_string(sp.lmax) + " nmax=" +
+                                         std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P));
+    };
+    if (one) {
+        SegmentSet set{};
+        int rows = 0;
+        for (int t = 0; t < nt; ++t) {
+            if (segs[t].nrows <= 0) continue;
+            set.a[set.nseg] = segs[t];
+            set.row0[set.nseg] = rows;
+            rows += segs[t].nrows;
+            ++set.nseg;
+        }
+        set.row0[set.nseg] = rows;
+        const SpeciesHost &sp = m->host.species[first];
+        const bool ok = backward ? launch_desc_bwd_set(s, sp.lmax, sp.K(), sp.P, sp.mix, set, nl)
+                                 : launch_desc_fwd_set(s, sp.lmax, sp.K(), sp.P, sp.mix, set, nl);
+        if (!ok) fail(first);
+        return;
+    }
+    for (int t = 0; t < nt; ++t) {
+        if (segs[t].nrows <= 0) continue;
+        const SpeciesHost &sp = m->host.species[t];
+        const bool ok = backward ? launch_desc_bwd(s, sp.lmax, sp.K(), sp.P, sp.mix, segs[t], nl)
+                                 : launch_desc_fwd(s, sp.lmax, sp.K(), sp.P, sp.mix, segs[t], nl);
+        if (!ok) fail(t);
+    }
+}
 void enqueue_mlp_all(hipStream_t s, const jse_nnap_model_s *m, int precision, const MlpArgs *args, int nsp) {
     if (launch_mlp_mfma_multi(s, args, nsp, precision == JSE_NNAP_BF16_MLP)) return;
     for (int t = 0; t < nsp; ++t) {
@@ -386,15 +432,14 @@ struct jse_nnap_system_s {
     }
 
     void desc_fwd(bool raw) {
+        std::vector<SegmentArgs> segs;
         for (int t = 0; t < model->host.ntypes(); ++t) {
-            const SpeciesHost &sp = model->host.species[t];
             SegmentArgs a = seg_args(t, raw);
             a.wg_per_cu = tune.desc_fwd_wg_per_cu;
             a.variant = tune.desc_fwd_variant;
-            if (!launch_desc_fwd(stream, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
-                throw Error(ERR_UNSUPPORTED, "no descriptor kernel compiled for lmax=" + std::to_string(sp.lmax) +
-                                                 " nmax=" + std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P));
+            segs.push_back(a);
         }
+        enqueue_desc(stream, model, segs.data(), false, nl);
     }
 
     // the whole evaluation as a sequence of launches on `stream`
@@ -413,14 +458,14 @@ struct jse_nnap_system_s {
         }
         if (timed) JNN_HIP(hipEventRecord(ev[4], stream));
         if (grad) {
+            std::vector<SegmentArgs> segs;
             for (int t = 0; t < model->host.ntypes(); ++t) {
-                const SpeciesHost &sp = model->host.species[t];
                 SegmentArgs a = seg_args(t, false);
                 a.wg_per_cu = tune.desc_bwd_wg_per_cu;
                 a.variant = tune.desc_bwd_variant;
-                if (!launch_desc_bwd(stream, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
-                    throw Error(ERR_UNSUPPORTED, "no descriptor backward kernel compiled for this shape");
+                segs.push_back(a);
             }
+            enqueue_desc(stream, model, segs.data(), true, nl);
         }
         if (timed) JNN_HIP(hipEventRecord(ev[5], stream));
         if (grad)

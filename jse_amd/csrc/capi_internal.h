@@ -125,6 +125,11 @@ void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision,
 // The MlpArgs of species t's rows.
 MlpArgs mlp_args(const jse_nnap_model_s *m, int t, int nrows, const int *rows_atom, const float *x, float *G, double *E,
                  bool grad, int wg_per_cu = 0);
+// Every species' descriptor forward (or backward): one launch over all segments
+// when they share one descriptor shape, else one per species.  segs[t] is
+// species t's segment (nrows 0: none).
+void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *segs, bool backward,
+                  const NeighborList &nl);
 // Every species' MLP: one launch when their shapes allow it, else one per species.
 void enqueue_mlp_all(hipStream_t s, const jse_nnap_model_s *m, int precision, const MlpArgs *args, int nsp);
 }  // namespace jnn

@@ -108,7 +108,7 @@ __device__ __forceinline__ PairGeomR pair_geom_fast(float dx, float dy, float dz
 
 // TP0: p-rows of each lane's register tile (tile TP0 x 7 of A)
 template <int LMAX, int K, int P, int MIX, int TP0>
-__global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl) {
+__global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl) {
     using SH = DescShape<LMAX, K, P, MIX, TP0>;
     constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
     constexpr int TP = SH::TP, TM = SH::TM, NMB = SH::NMB, NT = SH::NT, G = SH::G;
@@ -123,19 +123,29 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
     const int pb = tile / NMB, mb = tile % NMB;
     const int p0 = pb * TP, m0 = mb * TM;
     const bool tile_on = grp < G;
-    const float rc = a.rc, inv_rc = 1.f / a.rc;
-    // per-lane descriptor slots e = lane + 64 q: 4 pi / (2l + 1), mu, 1 / sigma
+    // per-lane descriptor slots e = lane + 64 q: 4 pi / (2l + 1), mu, 1 / sigma of the current segment
     constexpr int NE = (S + 63) / 64;
     double ew[NE], emu[NE], eis[NE];
-#pragma unroll
-    for (int q = 0; q < NE; ++q) {
-        const int e = lane + 64 * q;
-        ew[q] = 4.0 * 3.14159265358979323846 / (double)(2 * (e % L1) + 1);
-        emu[q] = e < S ? a.mu64[e] : 0.0;
-        eis[q] = e < S ? 1.0 / a.sig64[e] : 0.0;
-    }
+    float rc = 0.f, inv_rc = 0.f;
+    int cur = -1;
 
-    for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
+    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+        int sg = 0;  // the row's segment (species)
+        while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
+        const SegmentArgs &a = set.a[sg];
+        const int row = gr - set.row0[sg];
+        if (sg != cur) {
+            cur = sg;
+            rc = a.rc;
+            inv_rc = 1.f / a.rc;
+#pragma unroll
+            for (int q = 0; q < NE; ++q) {
+                const int e = lane + 64 * q;
+                ew[q] = 4.0 * 3.14159265358979323846 / (double)(2 * (e % L1) + 1);
+                emu[q] = e < S ? a.mu64[e] : 0.0;
+                eis[q] = e < S ? 1.0 / a.sig64[e] : 0.0;
+            }
+        }
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
@@ -235,7 +245,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentArgs a, NeighborList nl)
 // A row stride of 68 floats puts the 64 lanes of every operand read on 64
 // distinct banks.
 template <int LMAX, int K, int P, int MIX, bool F64>
-__global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentArgs a, NeighborList nl) {
+__global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList nl) {
     using AT = typename std::conditional<F64, double, float>::type;  // A's precision after the k-steps
     constexpr int L1 = LMAX + 1, M = L1 * L1, S = P * L1;
     constexpr int NMT = (M + 15) / 16, NPT = (P + 15) / 16;
@@ -252,7 +262,6 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentArgs a, NeighborList 
     constexpr f4 kZero = {0.f, 0.f, 0.f, 0.f};
     const int lane = threadIdx.x;
     const int li = lane & 15, lk = lane >> 4;
-    const float rc = a.rc, inv_rc = 1.f / a.rc;
     sC[P * NS + lane] = 0.f;  // the zero row
     if (lane < NS - 64) sC[P * NS + 64 + lane] = 0.f;
     // operand rows of this lane: C^T row pt * 16 + li (or the zero row) and, in the
@@ -261,18 +270,28 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentArgs a, NeighborList 
 #pragma unroll
     for (int pt = 0; pt < NPT; ++pt) boff[pt] = (pt * 16 + li < P ? pt * 16 + li : P) * NS;
     const int ylast = li < MLAST ? li * NS : (P * NS - FC);
-    // per-lane descriptor slots e = lane + 64 q: 4 pi / (2l + 1), mu, 1 / sigma
+    // per-lane descriptor slots e = lane + 64 q: 4 pi / (2l + 1), mu, 1 / sigma of the current segment
     double ew[NE], emu[NE], eis[NE];
-#pragma unroll
-    for (int q = 0; q < NE; ++q) {
-        const int e = lane + 64 * q;
-        const int l = e % L1;
-        ew[q] = 4.0 * 3.14159265358979323846 / (double)(2 * l + 1);
-        emu[q] = e < S ? a.mu64[e] : 0.0;
-        eis[q] = e < S ? 1.0 / a.sig64[e] : 0.0;
-    }
+    float rc = 0.f, inv_rc = 0.f;
+    int cur = -1;
 
-    for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
+    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+        int sg = 0;  // the row's segment (species)
+        while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
+        const SegmentArgs &a = set.a[sg];
+        const int row = gr - set.row0[sg];
+        if (sg != cur) {
+            cur = sg;
+            rc = a.rc;
+            inv_rc = 1.f / a.rc;
+#pragma unroll
+            for (int q = 0; q < NE; ++q) {
+                const int e = lane + 64 * q;
+                ew[q] = 4.0 * 3.14159265358979323846 / (double)(2 * (e % L1) + 1);
+                emu[q] = e < S ? a.mu64[e] : 0.0;
+                eis[q] = e < S ? 1.0 / a.sig64[e] : 0.0;
+            }
+        }
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
@@ -401,7 +420,7 @@ __host__ __device__ constexpr int bwd_blocks(int ntypes) {
 // BAR: a scheduling barrier after every BAR rows of the per-pair contraction
 // (bounds how many rows of LDS operands are in flight; 0: none)
 template <int LMAX, int K, int P, int MIX, int BAR>
-__global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl) {
+__global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl) {
     using SH = DescShape<LMAX, K, P, MIX>;
     constexpr int BSTEP = BAR > 0 ? BAR : 1;
     constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
@@ -410,9 +429,13 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
     // K (not P) rows
     extern __shared__ float sGt[];
     const int lane = threadIdx.x;
-    const int nblk = bwd_blocks<MIX>(a.ntypes);
-    const float rc = a.rc, inv_rc = 1.f / a.rc;
-    for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
+    const int nblk = bwd_blocks<MIX>(set.a[0].ntypes);
+    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+        int sg = 0;  // the row's segment (species)
+        while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
+        const SegmentArgs &a = set.a[sg];
+        const int row = gr - set.row0[sg];
+        const float rc = a.rc, inv_rc = 1.f / a.rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
@@ -520,32 +543,36 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentArgs a, NeighborList nl)
 
 // ---- dispatch table over the compiled shapes ----
 template <int LMAX, int K, int P, int MIX>
-static void run_fwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
-    int grid = a.nrows < 262144 ? a.nrows : 262144;
+static void run_fwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl) {
+    const SegmentArgs &a = set.a[0];  // launch shape, variant and precision are the set's
+    const int rows = set.row0[set.nseg];
+    int grid = rows < 262144 ? rows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
     // BF16-MLP mode: the matrix-core kernel with FP32 accumulation.  Plain FP32 mode:
     // variant 0 = matrix cores with FP64 totals per 16 neighbours; 1 / 2 = vector kernel
     // with 6x7 / 9x7 FP64 register tiles
     if (a.fp32_acc)
-        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX, false>), dim3(grid), dim3(64), 0, s, a, nl);
+        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX, false>), dim3(grid), dim3(64), 0, s, set, nl);
     else if (a.variant == 1)
-        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, a, nl);
+        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, set, nl);
     else if (a.variant == 2)
-        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 9>), dim3(grid), dim3(64), 0, s, a, nl);
+        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 9>), dim3(grid), dim3(64), 0, s, set, nl);
     else
-        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX, true>), dim3(grid), dim3(64), 0, s, a, nl);
+        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX, true>), dim3(grid), dim3(64), 0, s, set, nl);
 }
 template <int LMAX, int K, int P, int MIX>
-static void run_bwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl) {
-    int grid = a.nrows < 262144 ? a.nrows : 262144;
+static void run_bwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl) {
+    const SegmentArgs &a = set.a[0];
+    const int rows = set.row0[set.nseg];
+    int grid = rows < 262144 ? rows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
     const size_t lds = sizeof(float) * (size_t)bwd_blocks<MIX>(a.ntypes) * K * ((LMAX + 1) * (LMAX + 1));
     if (a.variant == 1)
-        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 2>), dim3(grid), dim3(64), lds, s, a, nl);
+        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 2>), dim3(grid), dim3(64), lds, s, set, nl);
     else if (a.variant == 2)
-        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 0>), dim3(grid), dim3(64), lds, s, a, nl);
+        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 0>), dim3(grid), dim3(64), lds, s, set, nl);
     else
-        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 1>), dim3(grid), dim3(64), lds, s, a, nl);
+        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 1>), dim3(grid), dim3(64), lds, s, set, nl);
 }
 
 #define JNN_SHAPES(X)              \
@@ -558,27 +585,43 @@ static void run_bwd(hipStream_t s, const SegmentArgs &a, const NeighborList &nl)
     X(4, 6, 6, MIX_NONE)           \
     X(4, 6, 12, MIX_DEFAULT)
 
-bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl) {
-    if (a.nrows <= 0) return true;
+bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
+                         const NeighborList &nl) {
+    if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
 #define X(L_, K_, P_, M_)                                              \
     if (lmax == L_ && K == K_ && P == P_ && mix == M_) {              \
-        run_fwd<L_, K_, P_, M_>(s, a, nl);                            \
+        run_fwd<L_, K_, P_, M_>(s, set, nl);                          \
         return true;                                                   \
     }
     JNN_SHAPES(X)
 #undef X
     return false;
 }
-bool launch_desc_bwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl) {
-    if (a.nrows <= 0) return true;
+bool launch_desc_bwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
+                         const NeighborList &nl) {
+    if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
 #define X(L_, K_, P_, M_)                                              \
     if (lmax == L_ && K == K_ && P == P_ && mix == M_) {              \
-        run_bwd<L_, K_, P_, M_>(s, a, nl);                            \
+        run_bwd<L_, K_, P_, M_>(s, set, nl);                          \
         return true;                                                   \
     }
     JNN_SHAPES(X)
 #undef X
     return false;
+}
+static SegmentSet single(const SegmentArgs &a) {
+    SegmentSet set{};
+    set.a[0] = a;
+    set.row0[0] = 0;
+    set.row0[1] = a.nrows;
+    set.nseg = 1;
+    return set;
+}
+bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl) {
+    return launch_desc_fwd_set(s, lmax, K, P, mix, single(a), nl);
+}
+bool launch_desc_bwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl) {
+    return launch_desc_bwd_set(s, lmax, K, P, mix, single(a), nl);
 }
 
 }  // namespace jnn

@@ -134,6 +134,19 @@ struct SegmentArgs {
 // returns false when no kernel is compiled for this (lmax, K, P, mix)
 bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);
 bool launch_desc_bwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);
+// Several species segments of one descriptor shape in one launch (no tail per
+// species): rows row0[k] .. row0[k+1]-1 of the launch are row (r - row0[k]) of
+// segment k.  Launch shape, variant and precision come from a[0].
+constexpr int kSegMax = 8;
+struct SegmentSet {
+    SegmentArgs a[kSegMax];
+    int row0[kSegMax + 1];
+    int nseg;
+};
+bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
+                         const NeighborList &nl);
+bool launch_desc_bwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
+                         const NeighborList &nl);
 
 struct MlpArgs {
     int nrows, S, L1, nlayers;

@@ -559,26 +559,18 @@ struct jse_nnap_server_s {
         // rows within rc at the current positions
         launch_srv_update(s, na, d_atoms, rep_of, d_need, x, xb, xwb, xw);
         launch_srv_refresh(s, na, d_atoms, rep_of, d_reps, rc * rc, xw, vl, nl, L.d_flags + 1);
-        // descriptor, MLP, backward per species segment of the batch rows
-        for (int t = 0; t < nt; ++t) {
-            const SpeciesHost &sp = model->host.species[t];
-            SegmentArgs a = seg_args(L, t, seg, d_rows);
-            if (a.nrows && !launch_desc_fwd(s, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
-                throw Error(ERR_UNSUPPORTED, "no descriptor kernel compiled for this species shape");
-        }
+        // descriptor, MLP, backward over the species segments of the batch rows
         {
+            std::vector<SegmentArgs> segs;
             std::vector<MlpArgs> ma;
             for (int t = 0; t < nt; ++t) {
-                SegmentArgs a = seg_args(L, t, seg, d_rows);
+                segs.push_back(seg_args(L, t, seg, d_rows));
+                const SegmentArgs &a = segs.back();
                 ma.push_back(mlp_args(model, t, a.nrows, a.rows_atom, a.x, a.G, E, true));
             }
+            enqueue_desc(s, model, segs.data(), false, nl);
             enqueue_mlp_all(s, model, precision, ma.data(), nt);
-        }
-        for (int t = 0; t < nt; ++t) {
-            const SpeciesHost &sp = model->host.species[t];
-            SegmentArgs a = seg_args(L, t, seg, d_rows);
-            if (a.nrows && !launch_desc_bwd(s, sp.lmax, sp.K(), sp.P, sp.mix, a, nl))
-                throw Error(ERR_UNSUPPORTED, "no descriptor backward kernel compiled for this species shape");
+            enqueue_desc(s, model, segs.data(), true, nl);
         }
         launch_gather(s, na, d_atoms, nl, F, V, 1);
         launch_replica_totals(s, c, d_ch, nr, d_rch, E, V, 1, L.d_partial, L.d_tot);
