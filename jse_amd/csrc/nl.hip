@@ -14,8 +14,9 @@
 // Layout: positions wrapped into SoA-free AoS FP64 [N][3]; atoms counting-
 // sorted by cell so a cell's members are contiguous; one wavefront per atom
 // scans the 27 neighbouring cells as one flattened candidate range, compacts
-// hits with a wave ballot into LDS, sorts them with a bitonic network and
-// writes one row.
+// hits with a wave ballot into LDS and writes each hit straight to its slot of
+// the sorted row: keys are unique, so the slot is the number of smaller keys
+// among the atom's hits (counted against the LDS hit buffer).
 #include <algorithm>
 
 #include "kernels.h"
@@ -199,38 +200,17 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
             hits += __popcll(mask);
         }
         const int nst = hits < kNlLds ? hits : kNlLds;
-        int p2 = 1;
-        while (p2 < nst) p2 <<= 1;
-        for (int t = nst + lane; t < p2; t += kNlBlock) s_key[t] = 0x7fffffff;
         __syncthreads();
-        // bitonic sort of (key, d) by key
-        for (int k = 2; k <= p2; k <<= 1) {
-            for (int jj = k >> 1; jj > 0; jj >>= 1) {
-                for (int t = lane; t < p2; t += kNlBlock) {
-                    const int u = t ^ jj;
-                    if (u > t) {
-                        const bool up = (t & k) == 0;
-                        const int ka = s_key[t], kb = s_key[u];
-                        if ((ka > kb) == up) {
-                            s_key[t] = kb;
-                            s_key[u] = ka;
-#pragma unroll
-                            for (int d = 0; d < 3; ++d) {
-                                const float tmp = s_d[d][t];
-                                s_d[d][t] = s_d[d][u];
-                                s_d[d][u] = tmp;
-                            }
-                        }
-                    }
-                }
-                __syncthreads();
-            }
-        }
-        const int nw = nst < nl.cap ? nst : nl.cap;
+        // keys are unique: a hit's slot in the sorted row is the number of smaller keys
         const size_t row = (size_t)i * nl.cap;
-        for (int t = lane; t < nw; t += kNlBlock) {
-            nl.key[row + t] = s_key[t];
-            if (nl.d) nl.d[row + t] = make_float4(s_d[0][t], s_d[1][t], s_d[2][t], __int_as_float(nl.types[s_key[t] >> 5]));
+        for (int t = lane; t < nst; t += kNlBlock) {
+            const int kt = s_key[t];
+            int rank = 0;
+            for (int u = 0; u < nst; ++u) rank += s_key[u] < kt ? 1 : 0;
+            if (rank < nl.cap) {
+                nl.key[row + rank] = kt;
+                if (nl.d) nl.d[row + rank] = make_float4(s_d[0][t], s_d[1][t], s_d[2][t], __int_as_float(nl.types[kt >> 5]));
+            }
         }
         if (lane == 0) {
             nl.cnt[i] = hits;
@@ -247,8 +227,8 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
 // those candidates once in LDS (FP64 positions with the periodic shift
 // already added, exactly as the per-atom build forms x_j + shift, and the key
 // j*32 + code), then each wave tests one member atom at a time against the
-// staged block, compacts its hits with a ballot and sorts them (in registers
-// for <= 64 hits).  A neighbourhood larger than the staging buffer is read
+// staged block, compacts its hits with a ballot and writes each hit to its
+// rank in the row.  A neighbourhood larger than the staging buffer is read
 // from global memory instead (same arithmetic).
 constexpr int kCellWaves = 4;
 constexpr int kStage = 512;
@@ -259,25 +239,6 @@ __device__ __forceinline__ void wave_lds_sync() {
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
 }
 
-// ascending bitonic sort of 64 (key, d) pairs, one per lane
-__device__ __forceinline__ void wave_sort64(int &key, float &dx, float &dy, float &dz, int lane) {
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            const int ok = __shfl_xor(key, j);
-            const float ox = __shfl_xor(dx, j), oy = __shfl_xor(dy, j), oz = __shfl_xor(dz, j);
-            const bool keep_min = ((lane & j) == 0) == ((lane & k) == 0);
-            const bool take = keep_min ? (ok < key) : (ok > key);
-            if (take) {
-                key = ok;
-                dx = ox;
-                dy = oy;
-                dz = oz;
-            }
-        }
-    }
-}
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
     int ncell_total, const int *__restrict__ cell_rep, const ReplicaGrid *__restrict__ reps, double rc2,
@@ -388,46 +349,17 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                 hits += __popcll(mask);
             }
             wave_lds_sync();
+            // keys are unique: a hit's slot in the sorted row is the number of smaller
+            // keys, counted against the whole hit buffer (broadcast reads)
             const int nst = hits < kNlLds ? hits : kNlLds;
-            const int nw = nst < nl.cap ? nst : nl.cap;
             const size_t row = (size_t)sl * nl.cap;
-            if (nst <= 64) {
-                int key = lane < nst ? hkey[lane] : 0x7fffffff;
-                float dx = lane < nst ? hdx[lane] : 0.f;
-                float dy = lane < nst ? hdy[lane] : 0.f;
-                float dz = lane < nst ? hdz[lane] : 0.f;
-                wave_sort64(key, dx, dy, dz, lane);
-                if (lane < nw) {
-                    nl.key[row + lane] = key;
-                    if (nl.d) nl.d[row + lane] = make_float4(dx, dy, dz, __int_as_float(nl.types[key >> 5]));
-                }
-            } else {
-                int p2 = 128;
-                while (p2 < nst) p2 <<= 1;
-                for (int t = nst + lane; t < p2; t += 64) hkey[t] = 0x7fffffff;
-                wave_lds_sync();
-                for (int k = 2; k <= p2; k <<= 1) {
-                    for (int jj = k >> 1; jj > 0; jj >>= 1) {
-                        for (int t = lane; t < p2; t += 64) {
-                            const int u = t ^ jj;
-                            if (u > t) {
-                                const bool up = (t & k) == 0;
-                                const int ka = hkey[t], kb = hkey[u];
-                                if ((ka > kb) == up) {
-                                    hkey[t] = kb;
-                                    hkey[u] = ka;
-                                    float tmp = hdx[t]; hdx[t] = hdx[u]; hdx[u] = tmp;
-                                    tmp = hdy[t]; hdy[t] = hdy[u]; hdy[u] = tmp;
-                                    tmp = hdz[t]; hdz[t] = hdz[u]; hdz[u] = tmp;
-                                }
-                            }
-                        }
-                        wave_lds_sync();
-                    }
-                }
-                for (int t = lane; t < nw; t += 64) {
-                    nl.key[row + t] = hkey[t];
-                    if (nl.d) nl.d[row + t] = make_float4(hdx[t], hdy[t], hdz[t], __int_as_float(nl.types[hkey[t] >> 5]));
+            for (int t = lane; t < nst; t += 64) {
+                const int kt = hkey[t];
+                int rank = 0;
+                for (int u = 0; u < nst; ++u) rank += hkey[u] < kt ? 1 : 0;
+                if (rank < nl.cap) {
+                    nl.key[row + rank] = kt;
+                    if (nl.d) nl.d[row + rank] = make_float4(hdx[t], hdy[t], hdz[t], __int_as_float(nl.types[kt >> 5]));
                 }
             }
             if (lane == 0) {
