@@ -215,6 +215,7 @@ struct jse_nnap_system_s {
     NeighborList nl{};
     float *d_x = nullptr, *d_A = nullptr, *d_G = nullptr;
     double *d_E = nullptr, *d_F = nullptr, *d_V = nullptr, *d_partial = nullptr, *d_tot = nullptr;
+    int *d_arrive = nullptr;  // per-replica chunk arrivals of the totals kernel (zero between calls)
     int4 *d_chunks = nullptr;
     int *d_rep_chunk = nullptr;
     int nchunk = 0;
@@ -231,7 +232,7 @@ struct jse_nnap_system_s {
         free_nl();
         dfree(d_xyz); dfree(d_xw); dfree(d_sxw); dfree(d_cell); dfree(d_rank); dfree(d_sidx); dfree(d_types);
         dfree(d_rows); dfree(d_rep_of); dfree(d_slot_of); dfree(d_reps); dfree(d_cell_cnt); dfree(d_cell_start); dfree(d_cell_rep); dfree(d_overflow);
-        dfree(d_x); dfree(d_A); dfree(d_G); dfree(d_E); dfree(d_F); dfree(d_V); dfree(d_partial); dfree(d_tot);
+        dfree(d_x); dfree(d_A); dfree(d_G); dfree(d_E); dfree(d_F); dfree(d_V); dfree(d_partial); dfree(d_tot); dfree(d_arrive);
         dfree(d_chunks); dfree(d_rep_chunk);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
@@ -281,6 +282,8 @@ struct jse_nnap_system_s {
             dfree(d_cell_rep);
             ncell_alloc = ncell_total + 1;
             d_cell_cnt = dalloc<int>(ncell_alloc);
+            // zero once: every build's scan clears the counts it consumed
+            JNN_HIP(hipMemset(d_cell_cnt, 0, sizeof(int) * ncell_alloc));
             d_cell_start = dalloc<int>(ncell_alloc + 1);
             d_cell_rep = dalloc<int>(ncell_alloc);
         }
@@ -327,7 +330,8 @@ struct jse_nnap_system_s {
         for (int q = 0, a = 0; q < nr; ++q) {
             rep_chunk[q] = (int)chunks.size();
             for (int k = 0; k < rep_natoms[q]; ++k) rep_of[a + k] = q;
-            for (int c0 = 0; c0 < rep_natoms[q]; c0 += kChunk)
+            // at least one chunk (possibly empty) per replica, so every totals row is written
+            for (int c0 = 0; c0 == 0 || c0 < rep_natoms[q]; c0 += kChunk)
                 chunks.push_back(make_int4(q, a + c0, a + std::min(c0 + kChunk, rep_natoms[q]), 0));
             a += rep_natoms[q];
         }
@@ -359,6 +363,8 @@ struct jse_nnap_system_s {
         d_V = dalloc<double>((size_t)9 * n);
         d_partial = dalloc<double>((size_t)std::max(nchunk, 1) * 10);
         d_tot = dalloc<double>((size_t)10 * nr);
+        d_arrive = dalloc<int>(nr);
+        JNN_HIP(hipMemset(d_arrive, 0, sizeof(int) * nr));
         set_boxes(boxes);
         // initial row capacity from the densest replica, rounded up to whole wavefronts
         double rho = 0.0;
@@ -377,10 +383,10 @@ struct jse_nnap_system_s {
     void build_nl(bool timed) {
         JNN_CHECK(have_pos, ERR_ARG, "positions not set");
         if (timed) JNN_HIP(hipEventRecord(ev[0], stream));
-        launch_fill_int(stream, ncell_total, d_cell_cnt, 0);
-        launch_fill_int(stream, 1, d_overflow, 0);
+        // the cell counts start at zero (cleared by the previous build's scan); the
+        // scan also clears the overflow flag before the cell-tile build can raise it
         launch_wrap_bin(stream, n, nullptr, d_xyz, d_rep_of, d_reps, nullptr, d_xw, d_cell, d_cell_cnt, d_rank);
-        launch_cell_scan(stream, nrep, nullptr, d_reps, nullptr, d_cell_cnt, d_cell_start);
+        launch_cell_scan(stream, nrep, nullptr, d_reps, nullptr, d_cell_cnt, d_cell_start, true, d_overflow);
         launch_cell_scatter(stream, n, nullptr, d_rep_of, nullptr, d_cell, d_rank, d_cell_start, d_xw, d_sidx, d_sxw,
                             d_slot_of);
         nl.types = d_types;
@@ -472,7 +478,7 @@ struct jse_nnap_system_s {
             launch_gather(stream, n, nullptr, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0, tune.gather_waves, d_sidx,
                           d_slot_of);
         launch_replica_totals(stream, nchunk, d_chunks, nrep, d_rep_chunk, d_E, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0,
-                              d_partial, d_tot);
+                              d_partial, d_tot, d_arrive);
         if (timed) JNN_HIP(hipEventRecord(ev[6], stream));
     }
 

@@ -98,11 +98,20 @@ constexpr int kPerThread = 4;  // atoms per thread: one chunk is at most kRedBlo
 static_assert(kRedBlock * kPerThread == kTotalsChunkMax, "totals chunk size");
 // One workgroup per chunk (replica, first, end): every thread loads its (at
 // most 4) atoms' E and V at once, sums them with Neumaier compensation in
-// atom order, and the workgroup combines the 10 sums in a fixed tree.
-__global__ __launch_bounds__(kRedBlock) void k_rep_partial(const int4 *__restrict__ chunks, const double *__restrict__ E,
-                                                          const double *__restrict__ V, int want_virial,
-                                                          double *__restrict__ partial) {
+// atom order, and the workgroup combines the 10 sums in a fixed tree into the
+// chunk's partial.  The last chunk of a replica to finish (an arrival count
+// per replica) then sums that replica's partials in chunk order, Neumaier
+// again, and resets the count for the next evaluation; the order of every
+// sum is fixed, so the totals do not depend on which chunk finishes last.
+// Replicas without atoms have no chunk: their totals keep the zeros they
+// were allocated with.
+__global__ __launch_bounds__(kRedBlock) void k_rep_totals(const int4 *__restrict__ chunks,
+                                                         const int *__restrict__ rep_chunk_off,
+                                                         const double *__restrict__ E, const double *__restrict__ V,
+                                                         int want_virial, double *__restrict__ partial,
+                                                         double *__restrict__ out, int *__restrict__ arrivals) {
     __shared__ double sh[kComp][kRedBlock];
+    __shared__ int s_last;
     const int4 ch = chunks[blockIdx.x];  // (replica, first atom, end atom, -)
     const int tid = threadIdx.x;
     double v[kPerThread][kComp];
@@ -129,17 +138,21 @@ __global__ __launch_bounds__(kRedBlock) void k_rep_partial(const int4 *__restric
         }
         __syncthreads();
     }
+    const int r = ch.x;
+    const int q0 = rep_chunk_off[r], q1 = rep_chunk_off[r + 1];
     if (tid < kComp) partial[(size_t)blockIdx.x * kComp + tid] = sh[tid][0];
-}
-__global__ void k_rep_final(int nrep, const int *__restrict__ rep_chunk_off, const double *__restrict__ partial,
-                            int want_virial, double *__restrict__ out) {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
-    const int r = t / kComp, c = t % kComp;
-    if (r >= nrep) return;
-    if (c > 0 && !want_virial) return;
-    Neum acc;
-    for (int q = rep_chunk_off[r]; q < rep_chunk_off[r + 1]; ++q) acc.add(partial[(size_t)q * kComp + c]);
-    out[(size_t)r * kComp + c] = acc.get();
+    __threadfence();  // the partial is visible device-wide before this chunk's arrival counts
+    __syncthreads();
+    if (tid == 0) s_last = atomicAdd(arrivals + r, 1) == q1 - q0 - 1;
+    __syncthreads();
+    if (!s_last) return;
+    __threadfence();  // every earlier chunk's partial is read after its arrival
+    if (tid < kComp && (tid == 0 || want_virial)) {
+        Neum acc;
+        for (int q = q0; q < q1; ++q) acc.add(partial[(size_t)q * kComp + tid]);
+        out[(size_t)r * kComp + tid] = acc.get();
+    }
+    if (tid == 0) arrivals[r] = 0;
 }
 
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
@@ -150,13 +163,11 @@ void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &n
                        out_of, row_of);
 }
 void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
-                           const double *E, const double *V, int want_virial, double *partial, double *out) {
-    if (nrep <= 0) return;
-    if (nchunk > 0)
-        hipLaunchKernelGGL(k_rep_partial, dim3(nchunk), dim3(kRedBlock), 0, s, chunks, E, V, want_virial, partial);
-    const int tot = nrep * kComp;
-    hipLaunchKernelGGL(k_rep_final, dim3((tot + 255) / 256), dim3(256), 0, s, nrep, rep_chunk_off, partial,
-                       want_virial, out);
+                           const double *E, const double *V, int want_virial, double *partial, double *out,
+                           int *arrivals) {
+    if (nrep <= 0 || nchunk <= 0) return;
+    hipLaunchKernelGGL(k_rep_totals, dim3(nchunk), dim3(kRedBlock), 0, s, chunks, rep_chunk_off, E, V, want_virial,
+                       partial, out, arrivals);
 }
 
 }  // namespace jnn

@@ -93,8 +93,10 @@ void launch_fill_int(hipStream_t s, int n, int *p, int v);  // kernel fill (grap
 // With nl.d == nullptr the build stores keys only (a Verlet candidate list).
 void launch_wrap_bin(hipStream_t s, int n, const int *atoms, const double *xyz_in, const int *rep_of,
                      const ReplicaGrid *reps, const int *need, double *xw, int *cell, int *cell_cnt, int *rank);
+// With `clear` the scan zeroes the counts it consumed (the next binning starts
+// from zero without a fill) and *clear_flag when non-null.
 void launch_cell_scan(hipStream_t s, int nrep, const int *rep_list, const ReplicaGrid *reps, const int *need,
-                      const int *cell_cnt, int *cell_start);
+                      int *cell_cnt, int *cell_start, bool clear = false, int *clear_flag = nullptr);
 // Also writes slot_of[atom] when non-null.
 void launch_cell_scatter(hipStream_t s, int n, const int *atoms, const int *rep_of, const int *need, const int *cell,
                          const int *rank, const int *cell_start, const double *xw, int *sidx, double *sxw,
@@ -190,8 +192,12 @@ constexpr int kTotalsChunkMax = 1024;
 // replica's atoms are summed (Neumaier, then a fixed tree) and the chunk sums
 // of a replica are added in order, so a replica's totals do not depend on
 // which other replicas share the system.  chunks[c] = (replica, first, end).
+// One launch: the last chunk of each replica to finish adds the chunk sums.
+// arrivals[nrep] must be zero before the first call (each call leaves it
+// zero); out rows of replicas without atoms are not written.
 void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
-                           const double *E, const double *V, int want_virial, double *partial, double *out);
+                           const double *E, const double *V, int want_virial, double *partial, double *out,
+                           int *arrivals);
 
 // ---- served path (serve.hip) ----
 // One replica of a request batch: arena slots [atom_off, atom_off + natoms),

@@ -54,16 +54,19 @@ __global__ void k_wrap_bin(int n, const int *__restrict__ atoms, const double *_
 // Exclusive scan of each replica's cell counts, one workgroup per replica:
 // start[cell_off + c] = atom_off + (atoms in the replica's cells before c), so
 // every atom gets a unique slot in its replica's slot range.
+// With `clear`, the counts are zeroed once consumed (ready for the next
+// binning) and *clear_flag (nullable) is zeroed by workgroup 0.
 __global__ __launch_bounds__(1024) void k_cell_scan(const int *__restrict__ rep_list, const ReplicaGrid *__restrict__ reps,
-                                                   const int *__restrict__ need, const int *__restrict__ cnt,
-                                                   int *__restrict__ start) {
+                                                   const int *__restrict__ need, int *__restrict__ cnt,
+                                                   int *__restrict__ start, int clear, int *__restrict__ clear_flag) {
     __shared__ int part[1024];
     const int r = rep_list ? rep_list[blockIdx.x] : blockIdx.x;
     if (need && !need[r]) return;
     const ReplicaGrid &g = reps[r];
     const int t = threadIdx.x;
     const int ncell = g.ncell;
-    const int *c_in = cnt + g.cell_off;
+    if (clear && clear_flag && blockIdx.x == 0 && threadIdx.x == 0) *clear_flag = 0;
+    int *c_in = cnt + g.cell_off;
     int *c_out = start + g.cell_off;
     const int per = (ncell + 1023) / 1024;
     const int b = t * per, e = min(b + per, ncell);
@@ -81,6 +84,7 @@ __global__ __launch_bounds__(1024) void k_cell_scan(const int *__restrict__ rep_
     for (int c = b; c < e; ++c) {
         c_out[c] = run;
         run += c_in[c];
+        if (clear) c_in[c] = 0;
     }
     if (t == 1023) c_out[ncell] = g.atom_off + part[1023];
 }
@@ -401,9 +405,10 @@ void launch_wrap_bin(hipStream_t s, int n, const int *atoms, const double *xyz_i
                        cell, cell_cnt, rank);
 }
 void launch_cell_scan(hipStream_t s, int nrep, const int *rep_list, const ReplicaGrid *reps, const int *need,
-                      const int *cell_cnt, int *cell_start) {
+                      int *cell_cnt, int *cell_start, bool clear, int *clear_flag) {
     if (nrep <= 0) return;
-    hipLaunchKernelGGL(k_cell_scan, dim3(nrep), dim3(1024), 0, s, rep_list, reps, need, cell_cnt, cell_start);
+    hipLaunchKernelGGL(k_cell_scan, dim3(nrep), dim3(1024), 0, s, rep_list, reps, need, cell_cnt, cell_start,
+                       clear ? 1 : 0, clear_flag);
 }
 void launch_cell_scatter(hipStream_t s, int n, const int *atoms, const int *rep_of, const int *need, const int *cell,
                          const int *rank, const int *cell_start, const double *xw, int *sidx, double *sxw,

@@ -111,6 +111,7 @@ struct Lane {
     // device
     char *d_plan = nullptr;
     double *d_pos = nullptr, *d_Fout = nullptr, *d_partial = nullptr, *d_tot = nullptr;
+    int *d_arrive = nullptr;  // per batch replica: chunk arrivals of the totals kernel
     float *d_x = nullptr, *d_A = nullptr, *d_G = nullptr;
     int *d_flags = nullptr;  // [0] candidate-row overflow, [1] row overflow, [2 + b] rebuilt flag of batch replica b
     // pinned host
@@ -223,6 +224,8 @@ struct jse_nnap_server_s {
         L.d_Fout = dalloc<double>(3 * nb);
         L.d_partial = dalloc<double>((size_t)max_chunks * 10);
         L.d_tot = dalloc<double>((size_t)max_batch_reps * 10);
+        L.d_arrive = dalloc<int>((size_t)max_batch_reps);
+        JNN_HIP(hipMemset(L.d_arrive, 0, sizeof(int) * (size_t)max_batch_reps));
         L.d_x = dalloc<float>(nb * model->smax);
         L.d_G = dalloc<float>(nb * model->smax);
         L.d_A = dalloc<float>(nb * model->pmmax);
@@ -235,7 +238,7 @@ struct jse_nnap_server_s {
     }
     void free_lane(Lane &L) {
         if (L.stream) (void)hipStreamSynchronize(L.stream);
-        dfree(L.d_plan); dfree(L.d_pos); dfree(L.d_Fout); dfree(L.d_partial); dfree(L.d_tot);
+        dfree(L.d_plan); dfree(L.d_pos); dfree(L.d_Fout); dfree(L.d_partial); dfree(L.d_tot); dfree(L.d_arrive);
         dfree(L.d_x); dfree(L.d_G); dfree(L.d_A); dfree(L.d_flags);
         hfree(L.h_plan); hfree(L.h_pos); hfree(L.h_Fout); hfree(L.h_tot); hfree(L.h_flags);
         if (L.done) (void)hipEventDestroy(L.done);
@@ -465,7 +468,7 @@ struct jse_nnap_server_s {
             const Replica &R = reps[sel[b]->rid];
             L.t_off[b] = na;
             na += R.natoms;
-            nch += (R.natoms + kReduceChunk - 1) / kReduceChunk;
+            nch += std::max(1, (R.natoms + kReduceChunk - 1) / kReduceChunk);
             if (admitted[b]) {
                 nadm++;
                 ntyp += R.natoms;
@@ -516,7 +519,9 @@ struct jse_nnap_server_s {
             br[b] = r;
             rl[b] = q->rid;
             rch[b] = c;
-            for (int c0 = 0; c0 < n; c0 += kReduceChunk) ch[c++] = make_int4(b, a0 + c0, a0 + std::min(c0 + kReduceChunk, n), 0);
+            // at least one chunk (possibly empty) per replica, so every totals row is written
+            for (int c0 = 0; c0 == 0 || c0 < n; c0 += kReduceChunk)
+                ch[c++] = make_int4(b, a0 + c0, a0 + std::min(c0 + kReduceChunk, n), 0);
             if (admitted[b]) {
                 SrvAdmit A{};
                 A.grid = R.grid;
@@ -573,7 +578,7 @@ struct jse_nnap_server_s {
             enqueue_desc(s, model, segs.data(), true, nl);
         }
         launch_gather(s, na, d_atoms, nl, F, V, 1);
-        launch_replica_totals(s, c, d_ch, nr, d_rch, E, V, 1, L.d_partial, L.d_tot);
+        launch_replica_totals(s, c, d_ch, nr, d_rch, E, V, 1, L.d_partial, L.d_tot, L.d_arrive);
         launch_srv_pack(s, na, d_atoms, F, L.d_Fout);
         JNN_HIP(hipMemcpyAsync(L.h_Fout, L.d_Fout, sizeof(double) * 3 * na, hipMemcpyDeviceToHost, s));
         JNN_HIP(hipMemcpyAsync(L.h_tot, L.d_tot, sizeof(double) * 10 * nr, hipMemcpyDeviceToHost, s));
