@@ -430,7 +430,36 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl
     extern __shared__ float sGt[];
     const int lane = threadIdx.x;
     const int nblk = bwd_blocks<MIX>(set.a[0].ntypes);
-    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+    const int total = set.row0[set.nseg];
+    // The next row's Gc and A inputs of the dE/dA fold (mixing without weights or the
+    // default mixing) are loaded while this row's pairs are worked on: pg[it][h], pa[it][h]
+    // for slot e = lane + 64 it and half h (p = q or K + q).
+    constexpr bool PF = MIX != MIX_DENSE;
+    constexpr int NIT = (K * M + 63) / 64;
+    constexpr int NH = MIX == MIX_DEFAULT ? 2 : 1;
+    float pg[PF ? NIT : 1][NH], pa[PF ? NIT : 1][NH];
+    auto prefetch = [&](int gr2) {
+        if (!PF || gr2 >= total) return;
+        int s2 = 0;
+        while (s2 + 1 < set.nseg && gr2 >= set.row0[s2 + 1]) ++s2;
+        const SegmentArgs &b = set.a[s2];
+        const int row2 = gr2 - set.row0[s2];
+        const float *__restrict__ A2 = b.A + (size_t)row2 * (P * M);
+        const float *__restrict__ G2 = b.G + (size_t)row2 * S;
+#pragma unroll
+        for (int it = 0; it < (PF ? NIT : 1); ++it) {
+            const int e = lane + 64 * it;
+            const int q = e / M, m = e % M;
+            const int l = (int)__builtin_amdgcn_sqrtf((float)m + 0.5f);  // l(m) = floor(sqrt(m))
+#pragma unroll
+            for (int h = 0; h < NH; ++h) {
+                pg[it][h] = e < K * M ? G2[(h * K + q) * L1 + l] : 0.f;
+                pa[it][h] = e < K * M ? A2[(h * K + q) * M + m] : 0.f;
+            }
+        }
+    };
+    prefetch(blockIdx.x);
+    for (int gr = blockIdx.x; gr < total; gr += gridDim.x) {
         int sg = 0;  // the row's segment (species)
         while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
         const SegmentArgs &a = set.a[sg];
@@ -441,19 +470,28 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl
         const size_t base = (size_t)i * nl.cap;
         const float *__restrict__ Arow = a.A + (size_t)row * (P * M);
         const float *__restrict__ Grow = a.G + (size_t)row * S;
-        for (int e = lane; e < K * M; e += 64) {
+        if constexpr (PF) {
+#pragma unroll
+            for (int it = 0; it < NIT; ++it) {
+                const int e = lane + 64 * it;
+                if (e < K * M) {
+                    if (MIX == MIX_NONE) {
+                        sGt[e] = pg[it][0] * pa[it][0];
+                    } else {
+                        const float lo = pg[it][0] * pa[it][0];
+                        const float hi = pg[it][NH - 1] * pa[it][NH - 1];
+                        for (int t = 1; t <= nblk; ++t) {
+                            const float wt = (t & 1) ? (float)t : -(float)t;
+                            sGt[(t - 1) * K * M + e] = lo + wt * hi;
+                        }
+                    }
+                }
+            }
+        }
+        for (int e = lane; !PF && e < K * M; e += 64) {
             const int q = e / M, m = e % M;
             const int l = (int)__builtin_amdgcn_sqrtf((float)m + 0.5f);  // l(m) = floor(sqrt(m))
-            if (MIX == MIX_NONE) {
-                sGt[e] = Grow[q * L1 + l] * Arow[q * M + m];
-            } else if (MIX == MIX_DEFAULT) {
-                const float lo = Grow[q * L1 + l] * Arow[q * M + m];
-                const float hi = Grow[(K + q) * L1 + l] * Arow[(K + q) * M + m];
-                for (int t = 1; t <= nblk; ++t) {
-                    const float wt = (t & 1) ? (float)t : -(float)t;
-                    sGt[(t - 1) * K * M + e] = lo + wt * hi;
-                }
-            } else {
+            {
                 for (int t = 1; t <= nblk; ++t) {
                     const float *w = a.W + (size_t)(t - 1) * P * K;
                     float s = 0.f;
@@ -463,10 +501,12 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl
             }
         }
         __syncthreads();
+        if (n == 0) prefetch(gr + gridDim.x);
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int j = c0 + lane;
+            const float4 dd = nl.d[base + (j < n ? j : n - 1)];
+            if (c0 == 0) prefetch(gr + gridDim.x);  // issued after this chunk's displacements
             if (j >= n) continue;
-            const float4 dd = nl.d[base + j];
             const float dx = dd.x, dy = dd.y, dz = dd.z;
             const int tj = __float_as_int(dd.w);
             const PairGeomR g = pair_geom_fast(dx, dy, dz);
