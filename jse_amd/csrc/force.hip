@@ -8,7 +8,8 @@
 // of the neighbour list (found by a binary search of j's sorted row), so the
 // sums are deterministic and need no atomics;
 // each lane sums its slots in FP64 and the wave combines lanes in a fixed
-// butterfly, the totals use Neumaier-compensated FP64 partial sums.  The
+// order on the vector ALU, the totals use Neumaier-compensated FP64 partial
+// sums.  The
 // virial of an incoming pair j -> i takes its displacement r_i - r_j as the
 // negated displacement of the pair i -> j in i's own row (read coalesced;
 // j's row holds the same vector formed as (x_i + s L) - x_j, which can differ
@@ -28,12 +29,38 @@ struct Neum {
     __device__ __forceinline__ double get() const { return s + c; }
 };
 
-// one wavefront per atom: lane k takes slot k of the row (and k+64, ...), the
-// 12 sums are then reduced across the wave by a fixed butterfly (deterministic)
+// One wavefront per atom: lane k takes slot k of the row (and k+64, ...), and
+// the 12 sums are reduced across the wave in a fixed order on the vector ALU
+// (no LDS traffic): DPP rotations within each row of 16 lanes (every lane then
+// holds its row's sum), then lane 0 adds rows 1-3 read with readlane.  The
+// result is valid in lane 0.
+__device__ __forceinline__ double dpp_ror(double v, int ctrl_ror) {
+    const long long b = __double_as_longlong(v);
+    int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+    switch (ctrl_ror) {  // row_ror:n is DPP control 0x120 + n
+        case 8: lo = __builtin_amdgcn_update_dpp(0, lo, 0x128, 0xf, 0xf, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x128, 0xf, 0xf, false); break;
+        case 4: lo = __builtin_amdgcn_update_dpp(0, lo, 0x124, 0xf, 0xf, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x124, 0xf, 0xf, false); break;
+        case 2: lo = __builtin_amdgcn_update_dpp(0, lo, 0x122, 0xf, 0xf, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x122, 0xf, 0xf, false); break;
+        default: lo = __builtin_amdgcn_update_dpp(0, lo, 0x121, 0xf, 0xf, false);
+                 hi = __builtin_amdgcn_update_dpp(0, hi, 0x121, 0xf, 0xf, false); break;
+    }
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double lane_of(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
 __device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int m = 32; m >= 1; m >>= 1) v += __shfl_xor(v, m);
-    return v;
+    v += dpp_ror(v, 8);
+    v += dpp_ror(v, 4);
+    v += dpp_ror(v, 2);
+    v += dpp_ror(v, 1);
+    return ((v + lane_of(v, 16)) + lane_of(v, 32)) + lane_of(v, 48);
 }
 
 __global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ atoms, NeighborList nl,
