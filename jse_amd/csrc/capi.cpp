@@ -252,6 +252,7 @@ struct jse_nnap_system_s {
     void alloc_nl(int cap) {
         drop_graphs();
         free_nl();
+        JNN_CHECK(cap > 0 && cap % 4 == 0, ERR_ARG, "neighbour capacity must be a positive multiple of 4");
         nl.cap = cap;
         const size_t m = (size_t)n * cap;
         nl.cnt = dalloc<int>(n);
@@ -380,7 +381,7 @@ struct jse_nnap_system_s {
 
     // Enqueue the neighbour build (no host synchronisation: a row longer
     // than the capacity raises a device flag that finish() checks).
-    void build_nl(bool timed) {
+    void build_nl(bool timed, bool copy_flag = true) {
         JNN_CHECK(have_pos, ERR_ARG, "positions not set");
         if (timed) JNN_HIP(hipEventRecord(ev[0], stream));
         // the cell counts start at zero (cleared by the previous build's scan); the
@@ -395,6 +396,11 @@ struct jse_nnap_system_s {
         if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
         // (the force gather finds reverse pairs itself; phase 1 stays empty)
         if (timed) JNN_HIP(hipEventRecord(ev[2], stream));
+        if (copy_flag) copy_overflow();
+    }
+    // the overflow flag to the host (finish() reads it); a full evaluation copies it
+    // after its last kernel, so no copy sits between the neighbour build and the descriptor
+    void copy_overflow() {
         JNN_HIP(hipMemcpyAsync(h_overflow, d_overflow, sizeof(int), hipMemcpyDeviceToHost, stream));
     }
     // Waits for the enqueued work; when a neighbour row overflowed, grows the
@@ -450,7 +456,7 @@ struct jse_nnap_system_s {
 
     // the whole evaluation as a sequence of launches on `stream`
     void enqueue(int what, bool timed) {
-        build_nl(timed);
+        build_nl(timed, false);
         const bool grad = (what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL)) != 0;
         desc_fwd(false);
         if (timed) JNN_HIP(hipEventRecord(ev[3], stream));
@@ -475,11 +481,11 @@ struct jse_nnap_system_s {
         }
         if (timed) JNN_HIP(hipEventRecord(ev[5], stream));
         if (grad)
-            launch_gather(stream, n, nullptr, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0, tune.gather_waves, d_sidx,
-                          d_slot_of);
+            launch_gather(stream, n, nullptr, nl, d_F, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0, tune.gather_waves, d_sidx);
         launch_replica_totals(stream, nchunk, d_chunks, nrep, d_rep_chunk, d_E, d_V, (what & JSE_NNAP_VIRIAL) ? 1 : 0,
                               d_partial, d_tot, d_arrive);
         if (timed) JNN_HIP(hipEventRecord(ev[6], stream));
+        copy_overflow();
     }
 
     // The neighbour rows by atom: the device rows sit in cell-sorted slot

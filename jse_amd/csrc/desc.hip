@@ -162,7 +162,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl
             const int j = c0 + lane;
             const int jr = j < n ? j : n - 1;
             const float4 dd = nl.d[base + jr];
-            const int tj = __float_as_int(dd.w);
+            const int tj = tag_species(dd.w);
             const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
             float y[M];
             real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
@@ -315,7 +315,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
             const int j = c0 + lane;
             const int jr = j < n ? j : n - 1;
             const float4 dd = nl.d[base + jr];
-            const int tj = __float_as_int(dd.w);
+            const int tj = tag_species(dd.w);
             const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
             float y[M];
             real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
@@ -508,7 +508,7 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl
             if (c0 == 0) prefetch(gr + gridDim.x);  // issued after this chunk's displacements
             if (j >= n) continue;
             const float dx = dd.x, dy = dd.y, dz = dd.z;
-            const int tj = __float_as_int(dd.w);
+            const int tj = tag_species(dd.w);
             const PairGeomR g = pair_geom_fast(dx, dy, dz);
             float ox = 0.f, oy = 0.f, oz = 0.f;
             if (g.r < rc) {

@@ -4,16 +4,15 @@
 //   F[i] -= g, F[j] += g, V[j] += (dx gx, dy gy, dz gz, dx gy, dx gz, dy gz, dy gx, dz gx, dz gy)
 // (src/nnap/java/jsex/nnap/NNAP.java:575-592, src/main/java/jse/atom/
 // IPairPotential.java:938-1006).  Here every atom gathers instead of being
-// scattered into: the incoming g_{j->i} is read through the reverse-pair slot
-// of the neighbour list (found by a binary search of j's sorted row), so the
-// sums are deterministic and need no atomics;
-// each lane sums its slots in FP64 and the wave combines lanes in a fixed
-// order on the vector ALU, the totals use Neumaier-compensated FP64 partial
-// sums.  The
-// virial of an incoming pair j -> i takes its displacement r_i - r_j as the
-// negated displacement of the pair i -> j in i's own row (read coalesced;
-// j's row holds the same vector formed as (x_i + s L) - x_j, which can differ
-// from it in the last FP32 bit).
+// scattered into, so the sums are deterministic and need no atomics: the
+// incoming g_{j->i} sits at the reverse pair's slot of j's sorted row, found
+// by a binary search (j's row comes with the pair, in the pair tag of d.w).
+// Each lane sums in FP64 and the lanes of an atom are combined in a fixed
+// order on the vector ALU; the totals use
+// Neumaier-compensated FP64 partial sums.  The virial of an incoming pair
+// j -> i takes its displacement r_i - r_j as the negated displacement of the
+// pair i -> j in i's own row (j's row holds the same vector formed as
+// (x_i + s L) - x_j, which can differ from it in the last FP32 bit).
 #include "kernels.h"
 
 namespace jnn {
@@ -29,15 +28,12 @@ struct Neum {
     __device__ __forceinline__ double get() const { return s + c; }
 };
 
-// One wavefront per atom: lane k takes slot k of the row (and k+64, ...), and
-// the 12 sums are reduced across the wave in a fixed order on the vector ALU
-// (no LDS traffic): DPP rotations within each row of 16 lanes (every lane then
-// holds its row's sum), then lane 0 adds rows 1-3 read with readlane.  The
-// result is valid in lane 0.
+// DPP row rotation of a double (row_ror:n is DPP control 0x120 + n): lane l of
+// each row of 16 lanes receives lane (l + n) mod 16 of the same row.
 __device__ __forceinline__ double dpp_ror(double v, int ctrl_ror) {
     const long long b = __double_as_longlong(v);
     int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
-    switch (ctrl_ror) {  // row_ror:n is DPP control 0x120 + n
+    switch (ctrl_ror) {
         case 8: lo = __builtin_amdgcn_update_dpp(0, lo, 0x128, 0xf, 0xf, false);
                 hi = __builtin_amdgcn_update_dpp(0, hi, 0x128, 0xf, 0xf, false); break;
         case 4: lo = __builtin_amdgcn_update_dpp(0, lo, 0x124, 0xf, 0xf, false);
@@ -49,65 +45,126 @@ __device__ __forceinline__ double dpp_ror(double v, int ctrl_ror) {
     }
     return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
-__device__ __forceinline__ double lane_of(double v, int l) {
-    const long long b = __double_as_longlong(v);
-    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
-    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
-    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
-__device__ __forceinline__ double wave_sum(double v) {
+// Sum over each row of 16 lanes in a fixed order (DPP row rotations); every
+// lane of the row gets it.
+__device__ __forceinline__ double row16_sum(double v) {
     v += dpp_ror(v, 8);
     v += dpp_ror(v, 4);
     v += dpp_ror(v, 2);
     v += dpp_ror(v, 1);
-    return ((v + lane_of(v, 16)) + lane_of(v, 32)) + lane_of(v, 48);
+    return v;
 }
 
-__global__ __launch_bounds__(512) void k_gather(int n, const int *__restrict__ atoms, NeighborList nl,
-                                                double *__restrict__ F, double *__restrict__ V, int want_virial,
-                                                const int *__restrict__ out_of, const int *__restrict__ row_of) {
-    const int lane = threadIdx.x & 63;
+constexpr int kGatherMaxWaves = 8;
+constexpr int kGatherSlots = 4;  // pairs per lane and pass: 16 lanes x 4 = 64 pairs of an atom
+
+// Sixteen lanes per atom (a DPP row), four atoms per wavefront.  In each pass
+// over 64 pairs of the atom's row, lane l takes pairs l, l + 16, l + 32 and
+// l + 48: their own g (F_i -= g) and four binary searches of the neighbours'
+// sorted rows for the reverse pairs, run in lockstep so their loads overlap;
+// j's row comes with the pair (the tag in d.w).  The 12 sums of the row of
+// lanes are then combined in a fixed order with DPP row rotations.
+__global__ __launch_bounds__(64 * kGatherMaxWaves) void k_gather(int n, const int *__restrict__ atoms,
+                                                                 NeighborList nl, double *__restrict__ F,
+                                                                 double *__restrict__ V, int want_virial,
+                                                                 const int *__restrict__ out_of) {
+    constexpr int U = kGatherSlots;
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int grp = lane >> 4, lid = lane & 15;
     // rows are in cell order: each XCD gathers one contiguous run of rows, so
     // the neighbours' rows it reads sit in its own L2
-    const int t = xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + (threadIdx.x >> 6);
-    if (t >= n) return;
-    const int i = atoms ? atoms[t] : t;       // row of the list
-    const int ia = out_of ? out_of[i] : i;    // its atom
-    const int ci = min(nl.cnt[i], nl.cap);
-    const size_t base = (size_t)i * nl.cap;
+    const int t = (xcd_block(blockIdx.x, gridDim.x) * (blockDim.x >> 6) + wv) * 4 + grp;
+    const bool act = t < n;  // no early return: every lane takes part in the row sums
+    const int i = act ? (atoms ? atoms[t] : t) : 0;  // row of the list
+    const int ia = act && out_of ? out_of[i] : i;    // its atom
+    const int cap = nl.cap;
+    const int ci = act ? min(nl.cnt[i], cap) : 0;
+    const size_t base = (size_t)i * cap;
     double f[3] = {0.0, 0.0, 0.0};
     double v[9] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
-    for (int k = lane; k < ci; k += 64) {
-        const float4 go = nl.g[base + k];
-        f[0] -= (double)go.x;
-        f[1] -= (double)go.y;
-        f[2] -= (double)go.z;
-        const int key = nl.key[base + k];
-        const float4 di = want_virial ? nl.d[base + k] : make_float4(0.f, 0.f, 0.f, 0.f);
-        int jrow;
-        const int r = find_reverse(nl, ia, key, row_of, &jrow);  // binary search in j's sorted row
-        if (r < 0) continue;
-        const size_t sj = (size_t)jrow * nl.cap + r;
-        const float4 gi = nl.g[sj];
-        const double gx = (double)gi.x, gy = (double)gi.y, gz = (double)gi.z;
-        f[0] += gx;
-        f[1] += gy;
-        f[2] += gz;
-        if (want_virial) {
-            // displacement of the pair j -> i: r_i - r_j = -(r_j - r_i)
-            const double dx = -(double)di.x, dy = -(double)di.y, dz = -(double)di.z;
-            v[0] += dx * gx; v[1] += dy * gy; v[2] += dz * gz;
-            v[3] += dx * gy; v[4] += dx * gz; v[5] += dy * gz;
-            v[6] += dy * gx; v[7] += dz * gx; v[8] += dz * gy;
+    for (int k0 = 0; k0 < ci; k0 += 16 * U) {
+        bool on[U], live[U];
+        int want[U], jr[U], lo[U], hi[U], res[U];
+        float dx[U], dy[U], dz[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int k = k0 + lid + 16 * u;
+            on[u] = k < ci;
+            jr[u] = 0;
+            want[u] = 0;
+            dx[u] = dy[u] = dz[u] = 0.f;
+            if (on[u]) {
+                const float4 go = nl.g[base + k];
+                f[0] -= (double)go.x;
+                f[1] -= (double)go.y;
+                f[2] -= (double)go.z;
+                const float4 dk = nl.d[base + k];
+                dx[u] = dk.x;
+                dy[u] = dk.y;
+                dz[u] = dk.z;
+                jr[u] = tag_row(dk.w);
+                want[u] = ia * 32 + (26 - (nl.key[base + k] & 31));
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            lo[u] = 0;
+            hi[u] = on[u] ? min(nl.cnt[jr[u]], cap) - 1 : -1;
+            res[u] = -1;
+            live[u] = hi[u] >= 0;
+        }
+        // the four searches advance together until every one has ended
+        while (true) {
+            bool any = false;
+#pragma unroll
+            for (int u = 0; u < U; ++u) any = any || live[u];
+            if (!__any(any)) break;
+            int mid[U], key[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                mid[u] = (lo[u] + hi[u]) >> 1;
+                key[u] = live[u] ? nl.key[(size_t)jr[u] * cap + mid[u]] : 0;
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (!live[u]) continue;
+                if (key[u] == want[u]) {
+                    res[u] = mid[u];
+                    live[u] = false;
+                } else {
+                    if (key[u] < want[u]) lo[u] = mid[u] + 1;
+                    else hi[u] = mid[u] - 1;
+                    live[u] = lo[u] <= hi[u];
+                }
+            }
+        }
+        float4 gi[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            gi[u] = res[u] >= 0 ? nl.g[(size_t)jr[u] * cap + res[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (res[u] < 0) continue;
+            const double gx = (double)gi[u].x, gy = (double)gi[u].y, gz = (double)gi[u].z;
+            f[0] += gx;
+            f[1] += gy;
+            f[2] += gz;
+            if (want_virial) {
+                // displacement of the pair j -> i: r_i - r_j = -(r_j - r_i)
+                const double ex = -(double)dx[u], ey = -(double)dy[u], ez = -(double)dz[u];
+                v[0] += ex * gx; v[1] += ey * gy; v[2] += ez * gz;
+                v[3] += ex * gy; v[4] += ex * gz; v[5] += ey * gz;
+                v[6] += ey * gx; v[7] += ez * gx; v[8] += ez * gy;
+            }
         }
     }
 #pragma unroll
-    for (int c = 0; c < 3; ++c) f[c] = wave_sum(f[c]);
+    for (int c = 0; c < 3; ++c) f[c] = row16_sum(f[c]);
     if (want_virial) {
 #pragma unroll
-        for (int c = 0; c < 9; ++c) v[c] = wave_sum(v[c]);
+        for (int c = 0; c < 9; ++c) v[c] = row16_sum(v[c]);
     }
-    if (lane == 0) {
+    if (act && lid == 0) {
         const size_t o = (size_t)ia;
         F[3 * o] = f[0];
         F[3 * o + 1] = f[1];
@@ -174,20 +231,31 @@ __global__ __launch_bounds__(kRedBlock) void k_rep_totals(const int4 *__restrict
     __syncthreads();
     if (!s_last) return;
     __threadfence();  // every earlier chunk's partial is read after its arrival
-    if (tid < kComp && (tid == 0 || want_virial)) {
-        Neum acc;
-        for (int q = q0; q < q1; ++q) acc.add(partial[(size_t)q * kComp + tid]);
-        out[(size_t)r * kComp + tid] = acc.get();
+    // the partials (written from other XCDs) are loaded in parallel, up to one per thread
+    // at a time, and summed from LDS in chunk order
+    Neum acc;
+    for (int q = q0; q < q1; q += kRedBlock) {
+        const int nq = min(kRedBlock, q1 - q);
+        __syncthreads();
+        if (tid < nq) {
+#pragma unroll
+            for (int c = 0; c < kComp; ++c) sh[c][tid] = partial[(size_t)(q + tid) * kComp + c];
+        }
+        __syncthreads();
+        if (tid < kComp)
+            for (int u = 0; u < nq; ++u) acc.add(sh[tid][u]);
     }
+    if (tid < kComp && (tid == 0 || want_virial)) out[(size_t)r * kComp + tid] = acc.get();
     if (tid == 0) arrivals[r] = 0;
 }
 
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
-                   int want_virial, int waves, const int *out_of, const int *row_of) {
+                   int want_virial, int waves, const int *out_of) {
     if (n <= 0) return;
-    const int w = (waves >= 1 && waves <= 8) ? waves : 4;
-    hipLaunchKernelGGL(k_gather, dim3(xcd_grid((n + w - 1) / w)), dim3(64 * w), 0, s, n, atoms, nl, F, V, want_virial,
-                       out_of, row_of);
+    const int w = (waves >= 1 && waves <= kGatherMaxWaves) ? waves : 4;
+    const int atoms_per_wg = 4 * w;  // four atoms per wavefront
+    hipLaunchKernelGGL(k_gather, dim3(xcd_grid((n + atoms_per_wg - 1) / atoms_per_wg)), dim3(64 * w), 0, s, n, atoms,
+                       nl, F, V, want_virial, out_of);
 }
 void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
                            const double *E, const double *V, int want_virial, double *partial, double *out,

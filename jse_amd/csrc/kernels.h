@@ -55,13 +55,30 @@ struct NeighborList {
     int cap = 0;
     int *cnt = nullptr;     // [N] true neighbour count (may exceed cap -> overflow)
     int *key = nullptr;     // [N*cap]
-    float4 *d = nullptr;    // [N*cap] (dx, dy, dz, t_j): FP32 of the FP64 displacement r_j + shift - r_i and
-                            // the model species of j (int bits), so the descriptor kernels skip key -> type
+    float4 *d = nullptr;    // [N*cap] (dx, dy, dz, tag): FP32 of the FP64 displacement r_j + shift - r_i and
+                            // the pair tag of j (pair_tag: its model species and its own row), so the
+                            // descriptor kernels skip key -> type and the force gather key -> row
     const int *types = nullptr;  // [atoms] model species (1-based) of the atoms the keys name; read by
                                  // the builders that write d
     int *rev = nullptr;     // [N*cap] slot of the reverse pair in j's row, -1 if absent (statistics only)
     float4 *g = nullptr;    // [N*cap] (gx, gy, gz, 0): g_j = -dE_i/d(r_j - r_i)
 };
+
+// Pair tag, carried as the int bits of d.w: bits 0-3 the model species of j
+// (1-based), bits 4-30 the row of the list that holds j's own neighbours
+// (0 where j has no row, e.g. a ghost atom).  Rows are limited to kTagRows.
+constexpr int kTagRows = 1 << 27;
+__device__ __forceinline__ float pair_tag(int jrow, int tj) { return __int_as_float((jrow << 4) | tj); }
+__device__ __forceinline__ int tag_species(float w) { return __float_as_int(w) & 15; }
+__device__ __forceinline__ int tag_row(float w) { return __float_as_int(w) >> 4; }
+
+// Orders one wavefront's LDS writes before its own later LDS reads of other
+// lanes' data (no workgroup barrier).
+__device__ __forceinline__ void wave_lds_sync() {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+}
 
 // Slot of the reverse pair (i, -shift) of atom i's pair `key` = j*32 + code in
 // j's sorted row, -1 if j's row does not hold it.  Keys use atom numbers; row
@@ -182,10 +199,11 @@ struct MlpMulti {
 bool launch_mlp_mfma_multi(hipStream_t s, const MlpArgs *a, int nsp, bool bf16);
 
 // ---- forces / virials / totals ----
-// Row t of the list belongs to atom out_of[t] (nullptr: t) and atom j's row is
-// row_of[j] (nullptr: j); F and V are written by atom.
+// Row t of the list belongs to atom out_of[t] (nullptr: t); atom j's row is
+// the row in the pair tag of d.w.  F and V are written by atom.  The rows'
+// capacity must be a multiple of 4 (keys are read 16 bytes at a time).
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
-                   int want_virial, int waves = 0, const int *out_of = nullptr, const int *row_of = nullptr);
+                   int want_virial, int waves = 0, const int *out_of = nullptr);
 // Largest chunk the totals kernel sums in one workgroup.
 constexpr int kTotalsChunkMax = 1024;
 // Per-replica totals [nrep][10] = (E, V[9]): fixed-size chunks of each

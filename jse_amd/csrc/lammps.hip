@@ -39,7 +39,7 @@ __global__ void k_lmp_rows(int inum, const int *__restrict__ ilist, const int *_
         if (dist2_no_fma(dx, dy, dz) < rc2) {
             if (c < nl.cap) {
                 nl.key[row + c] = j * 32 + 13;
-                nl.d[row + c] = make_float4((float)dx, (float)dy, (float)dz, __int_as_float(nl.types[j]));
+                nl.d[row + c] = make_float4((float)dx, (float)dy, (float)dz, pair_tag(0, nl.types[j]));  // species only
             }
             ++c;
         }

@@ -101,12 +101,6 @@ __device__ __forceinline__ void stage(int tid, Ld ld, St st) {
     }
 }
 
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
-
 // S <= KP (KP a multiple of 16 and of the instruction k), H1 = 16*C1, H2 = 16*C2.
 // Workgroup lb of lg takes row tiles lb, lb + lg, ...
 template <typename T, int C1, int C2, int KP>

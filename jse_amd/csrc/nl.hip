@@ -213,7 +213,8 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
             for (int u = 0; u < nst; ++u) rank += s_key[u] < kt ? 1 : 0;
             if (rank < nl.cap) {
                 nl.key[row + rank] = kt;
-                if (nl.d) nl.d[row + rank] = make_float4(s_d[0][t], s_d[1][t], s_d[2][t], __int_as_float(nl.types[kt >> 5]));
+                if (nl.d)  // rows by atom: j's row is j
+                    nl.d[row + rank] = make_float4(s_d[0][t], s_d[1][t], s_d[2][t], pair_tag(kt >> 5, nl.types[kt >> 5]));
             }
         }
         if (lane == 0) {
@@ -230,18 +231,13 @@ __global__ __launch_bounds__(kNlBlock) void k_nl_build(int n, const int *__restr
 // memory; keys keep atom numbers.  The atoms of one cell share its 27-cell neighbourhood: the workgroup stages
 // those candidates once in LDS (FP64 positions with the periodic shift
 // already added, exactly as the per-atom build forms x_j + shift, and the key
-// j*32 + code), then each wave tests one member atom at a time against the
-// staged block, compacts its hits with a ballot and writes each hit to its
-// rank in the row.  A neighbourhood larger than the staging buffer is read
+// j*32 + code, and slot), then each wave tests one member atom at a time
+// against the staged block, compacts its hits (key, candidate index) with a
+// ballot and writes each hit to its rank in the row, its displacement formed
+// again from the staged candidate by the same FP64 arithmetic.  A neighbourhood larger than the staging buffer is read
 // from global memory instead (same arithmetic).
 constexpr int kCellWaves = 4;
 constexpr int kStage = 512;
-
-__device__ __forceinline__ void wave_lds_sync() {
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
-}
 
 
 __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
@@ -250,9 +246,8 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
     int *__restrict__ overflow) {
     __shared__ int s_start[27], s_cnt[27], s_code[27], s_pref[28];
     __shared__ double s_cx[kStage], s_cy[kStage], s_cz[kStage];
-    __shared__ int s_ckey[kStage];
-    __shared__ int s_hkey[kCellWaves][kNlLds];
-    __shared__ float s_hd[kCellWaves][3][kNlLds];
+    __shared__ int s_ckey[kStage], s_cslot[kStage];
+    __shared__ int s_hkey[kCellWaves][kNlLds], s_hq[kCellWaves][kNlLds];  // hits: key, candidate index
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // each XCD takes a contiguous run of cells, whose 27-cell blocks overlap in its L2
     for (int c = xcd_block(blockIdx.x, gridDim.x); c < ncell_total; c += gridDim.x) {
@@ -287,10 +282,10 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
         __syncthreads();
         const int total = s_pref[27];
         const bool staged = total <= kStage;
-        // candidate q: its key and FP64 shifted position
-        auto fetch = [&](int q, int &key, double &xj, double &yj, double &zj) {
+        // candidate q: its key, slot (= its own row) and FP64 shifted position
+        auto fetch = [&](int q, int &key, int &slot, double &xj, double &yj, double &zj) {
             const int w = cand_cell(s_pref, q);
-            const int slot = s_start[w] + (q - s_pref[w]);
+            slot = s_start[w] + (q - s_pref[w]);
             const int code = s_code[w];
             const int sx = code % 3 - 1, sy = (code / 3) % 3 - 1, sz = code / 9 - 1;
             xj = sxw[3 * (size_t)slot];
@@ -303,52 +298,54 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
         };
         if (staged) {
             for (int q = tid; q < total; q += 64 * kCellWaves) {
-                int key;
+                int key, slot;
                 double xj, yj, zj;
-                fetch(q, key, xj, yj, zj);
+                fetch(q, key, slot, xj, yj, zj);
                 s_cx[q] = xj;
                 s_cy[q] = yj;
                 s_cz[q] = zj;
                 s_ckey[q] = key;
+                s_cslot[q] = slot;
             }
         }
         __syncthreads();
-        int *hkey = s_hkey[wv];
-        float *hdx = s_hd[wv][0], *hdy = s_hd[wv][1], *hdz = s_hd[wv][2];
+        int *hkey = s_hkey[wv], *hq = s_hq[wv];
         for (int sl = cs + wv; sl < ce; sl += kCellWaves) {
             const int i = sidx[sl];  // the row goes to slot sl; keys use atom numbers
             const double xi = sxw[3 * (size_t)sl], yi = sxw[3 * (size_t)sl + 1], zi = sxw[3 * (size_t)sl + 2];
+            // candidate q: key, slot and displacement from atom i (the same FP64 arithmetic wherever it is formed)
+            auto cand = [&](int q, int &key, int &slot, double &dx, double &dy, double &dz) {
+                double xj, yj, zj;
+                if (staged) {
+                    key = s_ckey[q];
+                    slot = s_cslot[q];
+                    xj = s_cx[q];
+                    yj = s_cy[q];
+                    zj = s_cz[q];
+                } else {
+                    fetch(q, key, slot, xj, yj, zj);
+                }
+                dx = xj - xi;
+                dy = yj - yi;
+                dz = zj - zi;
+            };
             int hits = 0;
             for (int q0 = 0; q0 < total; q0 += 64) {
                 const int q = q0 + lane;
                 bool hit = false;
                 int key = 0;
-                float fdx = 0.f, fdy = 0.f, fdz = 0.f;
                 if (q < total) {
-                    double xj, yj, zj;
-                    if (staged) {
-                        key = s_ckey[q];
-                        xj = s_cx[q];
-                        yj = s_cy[q];
-                        zj = s_cz[q];
-                    } else {
-                        fetch(q, key, xj, yj, zj);
-                    }
-                    const double dx = xj - xi, dy = yj - yi, dz = zj - zi;
-                    const double d2 = dist2_no_fma(dx, dy, dz);
-                    hit = (d2 < rc2) && !((key >> 5) == i && (key & 31) == 13);
-                    fdx = (float)dx;
-                    fdy = (float)dy;
-                    fdz = (float)dz;
+                    int slot;
+                    double dx, dy, dz;
+                    cand(q, key, slot, dx, dy, dz);
+                    hit = (dist2_no_fma(dx, dy, dz) < rc2) && !((key >> 5) == i && (key & 31) == 13);
                 }
                 const unsigned long long mask = __ballot(hit);
                 const unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
                 const int pos = hits + __popcll(below);
                 if (hit && pos < kNlLds) {
                     hkey[pos] = key;
-                    hdx[pos] = fdx;
-                    hdy[pos] = fdy;
-                    hdz[pos] = fdz;
+                    hq[pos] = q;
                 }
                 hits += __popcll(mask);
             }
@@ -363,7 +360,13 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                 for (int u = 0; u < nst; ++u) rank += hkey[u] < kt ? 1 : 0;
                 if (rank < nl.cap) {
                     nl.key[row + rank] = kt;
-                    if (nl.d) nl.d[row + rank] = make_float4(hdx[t], hdy[t], hdz[t], __int_as_float(nl.types[kt >> 5]));
+                    if (nl.d) {
+                        int key, slot;
+                        double dx, dy, dz;
+                        cand(hq[t], key, slot, dx, dy, dz);
+                        // j's own row is its slot
+                        nl.d[row + rank] = make_float4((float)dx, (float)dy, (float)dz, pair_tag(slot, nl.types[kt >> 5]));
+                    }
                 }
             }
             if (lane == 0) {

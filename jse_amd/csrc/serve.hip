@@ -126,7 +126,7 @@ __global__ __launch_bounds__(256) void k_srv_refresh(int n, const int *__restric
         const int pos = hits + __popcll(below);
         if (hit && pos < nl.cap) {
             nl.key[row + pos] = key;
-            nl.d[row + pos] = make_float4(fdx, fdy, fdz, __int_as_float(nl.types[key >> 5]));
+            nl.d[row + pos] = make_float4(fdx, fdy, fdz, pair_tag(key >> 5, nl.types[key >> 5]));  // row j: atom j
         }
         hits += __popcll(mask);
     }
