@@ -416,18 +416,23 @@ template <int MIX>
 __host__ __device__ constexpr int bwd_blocks(int ntypes) {
     return MIX == MIX_NONE ? 1 : ntypes;
 }
+// LDS row length of a block (K rows of M values): M rounded up to 4, so each
+// lane reads its block 16 bytes at a time with immediate offsets
+__host__ __device__ constexpr int bwd_row(int m) { return (m + 3) & ~3; }
 
 // BAR: a scheduling barrier after every BAR rows of the per-pair contraction
 // (bounds how many rows of LDS operands are in flight; 0: none)
 template <int LMAX, int K, int P, int MIX, int BAR>
-__global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl) {
+__global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList nl) {  // two waves per SIMD
     using SH = DescShape<LMAX, K, P, MIX>;
     constexpr int BSTEP = BAR > 0 ? BAR : 1;
     constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
+    constexpr int MP = bwd_row(M), MB = MP / 4;
     // sGt[t][n][m] = sum_p W[t][p][n] Gc[p][l(m)] A[p][m]: dE/dA taken back
     // through species t's radial mixing, so a neighbour of species t needs
-    // K (not P) rows
-    extern __shared__ float sGt[];
+    // K (not P) rows; rows of MP floats (m >= M unused)
+    extern __shared__ float4 sGt4[];
+    float *sGt = reinterpret_cast<float *>(sGt4);
     const int lane = threadIdx.x;
     const int nblk = bwd_blocks<MIX>(set.a[0].ntypes);
     const int total = set.row0[set.nseg];
@@ -474,15 +479,16 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl
 #pragma unroll
             for (int it = 0; it < NIT; ++it) {
                 const int e = lane + 64 * it;
+                const int w = e + (MP - M) * (e / M);  // (q, m) -> q MP + m
                 if (e < K * M) {
                     if (MIX == MIX_NONE) {
-                        sGt[e] = pg[it][0] * pa[it][0];
+                        sGt[w] = pg[it][0] * pa[it][0];
                     } else {
                         const float lo = pg[it][0] * pa[it][0];
                         const float hi = pg[it][NH - 1] * pa[it][NH - 1];
                         for (int t = 1; t <= nblk; ++t) {
                             const float wt = (t & 1) ? (float)t : -(float)t;
-                            sGt[(t - 1) * K * M + e] = lo + wt * hi;
+                            sGt[(t - 1) * K * MP + w] = lo + wt * hi;
                         }
                     }
                 }
@@ -496,7 +502,7 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl
                     const float *w = a.W + (size_t)(t - 1) * P * K;
                     float s = 0.f;
                     for (int p = 0; p < P; ++p) s += w[p * K + q] * (Grow[p * L1 + l] * Arow[p * M + m]);
-                    sGt[(t - 1) * K * M + e] = s;
+                    sGt[(t - 1) * K * MP + q * MP + m] = s;
                 }
             }
         }
@@ -520,7 +526,11 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl
                 const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
                 const float t2 = t * t;
                 const float fc = t2 * t2;
-                const float *sG = sGt + (MIX == MIX_NONE ? 0 : (tj - 1) * K * M);
+                int blk = MIX == MIX_NONE ? 0 : (tj - 1) * K * MB;
+                // opaque to the compiler, so every read below is this base plus an immediate
+                // offset (folding the -1 into the reads would make the offsets negative)
+                __asm__("" : "+v"(blk));
+                const float4 *sG = sGt4 + blk;
                 // gY[m] = fc sum_n Rn[n] Gt[n][m],  gct[n] = sum_m Gt[n][m] Y[m]
                 // (then dE/dRn = fc gct, dE/dfc = sum_n Rn[n] gct[n])
                 float gy[M];
@@ -534,10 +544,17 @@ __global__ __launch_bounds__(64) void k_desc_bwd(SegmentSet set, NeighborList nl
                     // four interleaved partial sums: shorter FP32 accumulation chains
                     float s4[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-                    for (int m = 0; m < M; ++m) {
-                        const float ga = sG[q * M + m];
-                        gy[m] += ga * cq;
-                        s4[m & 3] += ga * y[m];
+                    for (int mb = 0; mb < MB; ++mb) {
+                        const float4 g4 = sG[q * MB + mb];
+                        const float ga[4] = {g4.x, g4.y, g4.z, g4.w};
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            const int m = 4 * mb + c;
+                            if (m < M) {
+                                gy[m] += ga[c] * cq;
+                                s4[c] += ga[c] * y[m];
+                            }
+                        }
                     }
                     const float s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
                     gfc += s * rn[q];
@@ -606,7 +623,7 @@ static void run_bwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl
     const int rows = set.row0[set.nseg];
     int grid = rows < 262144 ? rows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
-    const size_t lds = sizeof(float) * (size_t)bwd_blocks<MIX>(a.ntypes) * K * ((LMAX + 1) * (LMAX + 1));
+    const size_t lds = sizeof(float) * (size_t)bwd_blocks<MIX>(a.ntypes) * K * bwd_row((LMAX + 1) * (LMAX + 1));
     if (a.variant == 1)
         hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 2>), dim3(grid), dim3(64), lds, s, set, nl);
     else if (a.variant == 2)
