@@ -25,6 +25,9 @@ HIP_FLAGS = [
     # packed FP32 VALU ops are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
+# per-source additions: the descriptor forward's matrix-core accumulators live in
+# ordinary VGPRs, so its FP64 totals read them without AGPR-to-VGPR copies
+FILE_FLAGS = {"desc.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _run(cmd):
@@ -40,7 +43,8 @@ def build(verbose: bool = False) -> None:
     for src in HIP_SOURCES:
         obj = os.path.join(OBJ, src.replace(".", "_") + ".o")
         lang = ["-x", "hip"]
-        jobs.append(([HIPCC] + HIP_FLAGS + lang + ["-c", os.path.join(CSRC, src), "-o", obj], obj))
+        jobs.append(([HIPCC] + HIP_FLAGS + FILE_FLAGS.get(src, []) + lang + ["-c", os.path.join(CSRC, src), "-o", obj],
+                     obj))
     workers = min(len(jobs), int(os.environ.get("MAX_JOBS", "8")))
     with cf.ThreadPoolExecutor(max_workers=workers) as ex:
         for err in ex.map(lambda j: _run(j[0]), jobs):

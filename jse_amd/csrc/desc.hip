@@ -244,12 +244,18 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl
 // B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i / 4), register i % 4.
 // A row stride of 68 floats puts the 64 lanes of every operand read on 64
 // distinct banks.
+// With the default mixing (c[K + n] = w_t c[n]) and at most 8 data rows in
+// the last m-tile, that tile also carries the rows w_t Y[m] below its data
+// rows: against the columns c[n] of p-tile 0 they give A[K + n][m], so the
+// last m-tile needs no other p-tile.
 template <int LMAX, int K, int P, int MIX, bool F64>
 __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList nl) {
     using AT = typename std::conditional<F64, double, float>::type;  // A's precision after the k-steps
     constexpr int L1 = LMAX + 1, M = L1 * L1, S = P * L1;
     constexpr int NMT = (M + 15) / 16, NPT = (P + 15) / 16;
     constexpr int MLAST = M - 16 * (NMT - 1);  // data rows of the last m-tile
+    constexpr bool WROWS = MIX == MIX_DEFAULT && P == 2 * K && K <= 16 && 2 * MLAST <= 16;
+    constexpr int YLAST = WROWS ? 2 * MLAST : MLAST;  // operand rows of the last m-tile
     constexpr int NS = 68;
     constexpr int FC = (P + 1) * NS, FY = 16 * NS, FA = P * M * (F64 ? 2 : 1);  // C^T rows, zero row P
     constexpr int F = FC + (FY > FA ? FY : FA);
@@ -269,7 +275,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
     int boff[NPT];
 #pragma unroll
     for (int pt = 0; pt < NPT; ++pt) boff[pt] = (pt * 16 + li < P ? pt * 16 + li : P) * NS;
-    const int ylast = li < MLAST ? li * NS : (P * NS - FC);
+    const int ylast = li < YLAST ? li * NS : (P * NS - FC);
     // per-lane descriptor slots e = lane + 64 q: 4 pi / (2l + 1), mu, 1 / sigma of the current segment
     double ew[NE], emu[NE], eis[NE];
     float rc = 0.f, inv_rc = 0.f;
@@ -327,6 +333,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
             float rnp[P];
             mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
             const int nks = (min(64, n - c0) + 3) >> 2;
+            const float wt = (tj & 1) ? (float)tj : -(float)tj;  // the default mixing's weight
             __syncthreads();  // the previous chunk's operand reads (and the previous atom's A) are done
 #pragma unroll
             for (int p = 0; p < P; ++p) sC[p * NS + lane] = fc * rnp[p];
@@ -334,8 +341,12 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
             for (int mt = 0; mt < NMT; ++mt) {
                 if (mt > 0) __syncthreads();  // the previous m-tile's operand reads are done
 #pragma unroll
-                for (int r = 0; r < 16; ++r)
+                for (int r = 0; r < 16; ++r) {
                     if (mt * 16 + r < M) sY[r * NS + lane] = y[mt * 16 + r];
+                    else if (WROWS && mt == NMT - 1 && r < YLAST) sY[r * NS + lane] = wt * y[mt * 16 + r - MLAST];
+                }
+                // p-tiles this m-tile runs (a constant once the m-tile loop is unrolled)
+                const int npt = (WROWS && mt == NMT - 1) ? 1 : NPT;
                 __syncthreads();
                 const int aoff = mt < NMT - 1 ? li * NS : ylast;
                 int ks = 0;
@@ -346,17 +357,17 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
                         const int col = 4 * (ks + u) + lk;
                         av[u] = sY[aoff + col];
 #pragma unroll
-                        for (int pt = 0; pt < NPT; ++pt) bv[u][pt] = sC[boff[pt] + col];
+                        for (int pt = 0; pt < npt; ++pt) bv[u][pt] = sC[boff[pt] + col];
                     }
 #pragma unroll
                     for (int u = 0; u < 4; ++u)
 #pragma unroll
-                        for (int pt = 0; pt < NPT; ++pt)
+                        for (int pt = 0; pt < npt; ++pt)
                             acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
                                 av[u], bv[u][pt], (F64 && u == 0) ? kZero : acc[mt][pt], 0, 0, 0);
                     if constexpr (F64) {
 #pragma unroll
-                        for (int pt = 0; pt < NPT; ++pt)
+                        for (int pt = 0; pt < npt; ++pt)
 #pragma unroll
                             for (int r = 0; r < 4; ++r) a64[mt][pt][r] += (double)acc[mt][pt][r];
                     }
@@ -366,13 +377,13 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
                         const int col = 4 * ks + lk;
                         const float av = sY[aoff + col];
 #pragma unroll
-                        for (int pt = 0; pt < NPT; ++pt)
+                        for (int pt = 0; pt < npt; ++pt)
                             acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
                                 av, sC[boff[pt] + col], (F64 && u == 0) ? kZero : acc[mt][pt], 0, 0, 0);
                     }
                     if constexpr (F64) {
 #pragma unroll
-                        for (int pt = 0; pt < NPT; ++pt)
+                        for (int pt = 0; pt < npt; ++pt)
 #pragma unroll
                             for (int r = 0; r < 4; ++r) a64[mt][pt][r] += (double)acc[mt][pt][r];
                     }
@@ -386,8 +397,18 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
             for (int pt = 0; pt < NPT; ++pt)
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int m = mt * 16 + 4 * lk + r, p = pt * 16 + li;
-                    if ((mt < NMT - 1 || 4 * lk + r < MLAST) && (pt < NPT - 1 || p < P)) {
+                    const int i = 4 * lk + r;  // row of the tile
+                    int m = mt * 16 + i, p = pt * 16 + li;
+                    bool keep = (mt < NMT - 1 || i < MLAST) && (pt < NPT - 1 || p < P);
+                    if (WROWS && mt == NMT - 1) {
+                        // data rows give A[n][m] for n < K, the w_t Y rows A[K + n][m]
+                        keep = pt == 0 && li < K && i < 2 * MLAST;
+                        if (i >= MLAST) {
+                            m -= MLAST;
+                            p += K;
+                        }
+                    }
+                    if (keep) {
                         if constexpr (F64) sA[p * M + m] = a64[mt][pt][r];
                         else sA[p * M + m] = acc[mt][pt][r];
                     }
