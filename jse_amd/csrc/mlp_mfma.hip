@@ -11,9 +11,11 @@
 //     gX  = gZ1 W1                          [64 x H1] x [H1 x S]
 // keep each wave's SiLU derivatives in its own registers between forward and
 // backward (the same lane -> (row, column) map serves both), so the only
-// cross-wave traffic is the staged weights.  Activations and weights live in
-// LDS with row strides chosen against bank conflicts; the grid is persistent
-// (weights staged once per workgroup).
+// cross-wave traffic is the staged weights.  X goes from memory straight into
+// the registers of the first product's A operand (the next tile's X is loaded
+// while the current tile's other three products run); hidden activations and
+// weights live in LDS with row strides chosen against bank conflicts.  The
+// grid is persistent (weights staged once per workgroup).
 //
 // Two element types share the code:
 //   float  : v_mfma_f32_16x16x4_f32   (plain FP32 path)
@@ -47,9 +49,11 @@ struct Mma;
 template <>
 struct Mma<float> {
     static constexpr int KSTEP = 4;
+    static constexpr int KPL = 1;  // k values per lane and instruction
     static constexpr int PAD = 2;  // row stride == 2 (mod 32) dwords
     static constexpr bool kTransposedCopies = false;
     static constexpr bool kSplit = false;
+    using AV = float;  // an A operand held in registers
     __device__ static __forceinline__ float cvt(float v) { return v; }
     __device__ static __forceinline__ float lo(float v) { return 0.f; }
     __device__ static __forceinline__ int lane_k(int lane) { return lane >> 4; }
@@ -57,12 +61,19 @@ struct Mma<float> {
     __device__ static __forceinline__ f32x4 mma(const float *a, const float *b, f32x4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x4f32(a[0], b[0], c, 0, 0, 0);
     }
+    __device__ static __forceinline__ AV a_hi(const float *x) { return x[0]; }
+    __device__ static __forceinline__ AV a_lo(const float *x) { return 0.f; }
+    __device__ static __forceinline__ f32x4 mma_r(AV a, const float *b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b[0], c, 0, 0, 0);
+    }
 };
 
 template <>
 struct Mma<__bf16> {
     static constexpr int KSTEP = 32;
+    static constexpr int KPL = 8;
     static constexpr int PAD = 8;  // row stride == 4 (mod 32) dwords: conflict-free 16-byte reads
+    using AV = bf16x8;
     static constexpr bool kTransposedCopies = true;
     // forward operands as hi + lo BF16 pairs: x w ~ xh wh + xl wh + xh wl (FP32 accumulation)
     static constexpr bool kSplit = true;
@@ -72,6 +83,21 @@ struct Mma<__bf16> {
     __device__ static __forceinline__ f32x4 mma(const __bf16 *a, const __bf16 *b, f32x4 c) {
         return __builtin_amdgcn_mfma_f32_16x16x32_bf16(*reinterpret_cast<const bf16x8 *>(a),
                                                        *reinterpret_cast<const bf16x8 *>(b), c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ AV a_hi(const float *x) {
+        AV v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = cvt(x[e]);
+        return v;
+    }
+    __device__ static __forceinline__ AV a_lo(const float *x) {
+        AV v;
+#pragma unroll
+        for (int e = 0; e < 8; ++e) v[e] = lo(x[e]);
+        return v;
+    }
+    __device__ static __forceinline__ f32x4 mma_r(AV a, const __bf16 *b, f32x4 c) {
+        return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, *reinterpret_cast<const bf16x8 *>(b), c, 0, 0, 0);
     }
 };
 
@@ -117,14 +143,13 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     constexpr bool TC = MM::kTransposedCopies;
     constexpr bool SP = MM::kSplit;
     static_assert(KP % MM::KSTEP == 0 && KP % 16 == 0, "padded input dim");
-    __shared__ __align__(16) T Xs[64 * XS];
+    constexpr int NX = KP / MM::KSTEP * MM::KPL;                          // X values per lane (layer-1 A operand)
     __shared__ __align__(16) T W1s[H1 * XS];                 // W1[j][k]
     __shared__ __align__(16) T W2s[H2 * HS];                 // W2[j2][j1]
     __shared__ __align__(16) T W1t[TC ? KP * HS : 1];        // W1[j][k] stored [k][j]
     __shared__ __align__(16) T W2t[TC ? H1 * HS : 1];        // W2[j2][j1] stored [j1][j2]
     __shared__ __align__(16) T Hs[64 * HS];
-    __shared__ __align__(16) T Xl[SP ? 64 * XS : 1];         // low halves of the forward operands
-    __shared__ __align__(16) T W1l[SP ? H1 * XS : 1];
+    __shared__ __align__(16) T W1l[SP ? H1 * XS : 1];        // low halves of the forward operands
     __shared__ __align__(16) T W2l[SP ? H2 * HS : 1];
     __shared__ __align__(16) T Hl[SP ? 64 * HS : 1];
 
@@ -169,46 +194,75 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
             },
             [&](int e, float w) { W2t[e] = MM::cvt(w); });
     }
+    __syncthreads();  // the weights are staged; below, each wave touches only its own rows of H
     const int r0 = 16 * wave;
     T *hrow_w = Hs + r0 * HS;                         // this wave's 16 rows of H
     const T *hrow = Hs + (r0 + lr) * HS + lk;         // A-operand pointer into H
+    // per-lane constants of the output columns this lane holds
+    float b1r[C1], b2r[C2], wor[C2], gscale[NS];
+#pragma unroll
+    for (int c = 0; c < C1; ++c) b1r[c] = b1[16 * c + lr];
+#pragma unroll
+    for (int c = 0; c < C2; ++c) {
+        b2r[c] = b2[16 * c + lr];
+        wor[c] = wo[16 * c + lr];
+    }
+#pragma unroll
+    for (int cs = 0; cs < NS; ++cs) {
+        const int n = 16 * cs + lr;
+        gscale[cs] = n < S ? (2.f * kPi4m / (float)(2 * (n % a.L1) + 1)) / a.sig[n] : 0.f;
+    }
+    // X of a tile straight from memory into the layer-1 A operand: value u of
+    // this lane is X[row r0 + lr][(u / KPL) * KSTEP + lk + u % KPL]
+    float xr[NX];
+    auto load_x = [&](int t) {
+        const int i = t * 64 + r0 + lr;
+        const bool ok = i < a.nrows;
+        const float *__restrict__ xp = a.x + (size_t)(ok ? i : 0) * S;
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int k = (u / MM::KPL) * MM::KSTEP + lk + u % MM::KPL;
+            xr[u] = (ok && k < S) ? xp[k] : 0.f;
+        }
+    };
+    if (lb * 64 < a.nrows) load_x(lb);
     for (int tile = lb; tile * 64 < a.nrows; tile += lg) {
         const int row0 = tile * 64;
         const int nt = min(64, a.nrows - row0);
-        __syncthreads();
-        stage<64 * XS>(
-            tid,
-            [&](int e) {
-                const int i = e / XS, k = e % XS;
-                return (i < nt && k < S) ? a.x[(size_t)(row0 + i) * S + k] : 0.f;
-            },
-            [&](int e, float v) {
-                Xs[e] = MM::cvt(v);
-                if constexpr (SP) Xl[e] = MM::lo(v);
-            });
-        __syncthreads();
+        // atoms of the rows whose energies this lane writes (loaded before the next tile's X)
+        int atom_r[4];
+#pragma unroll
+        for (int r = 0; r < 4; ++r) {
+            const int i = r0 + 4 * (lane >> 4) + r;
+            atom_r[r] = (lr == 0 && i < nt) ? a.rows_atom[row0 + i] : 0;
+        }
         // ---- layer 1 ----
         f32x4 acc1[C1];
 #pragma unroll
         for (int c = 0; c < C1; ++c) acc1[c] = f32x4{0.f, 0.f, 0.f, 0.f};
-        const T *xrow = Xs + (r0 + lr) * XS + lk;
-        for (int k0 = 0; k0 < KP; k0 += MM::KSTEP) {
+#pragma unroll
+        for (int ks = 0; ks < KP / MM::KSTEP; ++ks) {
+            const int k0 = ks * MM::KSTEP;
+            const typename MM::AV xh = MM::a_hi(xr + ks * MM::KPL);
+            typename MM::AV xl{};
+            if constexpr (SP) xl = MM::a_lo(xr + ks * MM::KPL);
 #pragma unroll
             for (int c = 0; c < C1; ++c) {
                 const int wofs = (16 * c + lr) * XS + lk + k0;
-                acc1[c] = MM::mma(xrow + k0, W1s + wofs, acc1[c]);
+                acc1[c] = MM::mma_r(xh, W1s + wofs, acc1[c]);
                 if constexpr (SP) {
-                    const int xofs = (r0 + lr) * XS + lk + k0;
-                    acc1[c] = MM::mma(Xl + xofs, W1s + wofs, acc1[c]);
-                    acc1[c] = MM::mma(Xs + xofs, W1l + wofs, acc1[c]);
+                    acc1[c] = MM::mma_r(xl, W1s + wofs, acc1[c]);
+                    acc1[c] = MM::mma_r(xh, W1l + wofs, acc1[c]);
                 }
             }
         }
+        // the next tile's X is loaded while this tile's other three products run
+        if ((tile + lg) * 64 < a.nrows) load_x(tile + lg);
         float d1[C1][4];
 #pragma unroll
         for (int c = 0; c < C1; ++c) {
             const int j = 16 * c + lr;
-            const float bj = b1[j];
+            const float bj = b1r[c];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const float h = silu_grad(acc1[c][r] + bj, &d1[c][r]);
@@ -243,8 +297,7 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         float ysum[4] = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
         for (int c = 0; c < C2; ++c) {
-            const int j = 16 * c + lr;
-            const float bj = b2[j], w = wo[j];
+            const float bj = b2r[c], w = wor[c];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 float dd;
@@ -266,7 +319,7 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
                 const int i = r0 + 4 * (lane >> 4) + r;
-                if (i < nt) a.E[a.rows_atom[row0 + i]] = (double)(ysum[r] + bo) * (double)a.sig_e + (double)a.mu_e;
+                if (i < nt) a.E[atom_r[r]] = (double)(ysum[r] + bo) * (double)a.sig_e + (double)a.mu_e;
             }
         }
         if (!a.want_grad) continue;
@@ -315,8 +368,7 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
                 else acc = MM::mma(hrow + k0, W1s + (k0 + lk) * XS + n, acc);
             }
             if (n < S) {
-                const int l = n % a.L1;
-                const float scale = (2.f * kPi4m / (float)(2 * l + 1)) / a.sig[n];
+                const float scale = gscale[cs];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
                     const int i = r0 + 4 * (lane >> 4) + r;
