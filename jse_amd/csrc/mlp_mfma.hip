@@ -166,6 +166,20 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     const float bo = a.bias[H1 + H2];
     const float *__restrict__ wo = a.w_in_out + (size_t)S * H1 + (size_t)H1 * H2;
 
+    // X of a tile straight from memory into the layer-1 A operand: value u of
+    // this lane is X[row 16 wave + lr][(u / KPL) * KSTEP + lk + u % KPL]
+    float xr[NX];
+    auto load_x = [&](int t) {
+        const int i = t * 64 + 16 * wave + lr;
+        const bool ok = i < a.nrows;
+        const float *__restrict__ xp = a.x + (size_t)(ok ? i : 0) * S;
+#pragma unroll
+        for (int u = 0; u < NX; ++u) {
+            const int k = (u / MM::KPL) * MM::KSTEP + lk + u % MM::KPL;
+            xr[u] = (ok && k < S) ? xp[k] : 0.f;
+        }
+    };
+    if (lb * 64 < a.nrows) load_x(lb);  // in flight while the weights are staged
     stage<H1 * XS>(
         tid, [&](int e) { const int j = e / XS, k = e % XS; return k < S ? W1g[(size_t)j * S + k] : 0.f; },
         [&](int e, float w) {
@@ -212,20 +226,6 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         const int n = 16 * cs + lr;
         gscale[cs] = n < S ? (2.f * kPi4m / (float)(2 * (n % a.L1) + 1)) / a.sig[n] : 0.f;
     }
-    // X of a tile straight from memory into the layer-1 A operand: value u of
-    // this lane is X[row r0 + lr][(u / KPL) * KSTEP + lk + u % KPL]
-    float xr[NX];
-    auto load_x = [&](int t) {
-        const int i = t * 64 + r0 + lr;
-        const bool ok = i < a.nrows;
-        const float *__restrict__ xp = a.x + (size_t)(ok ? i : 0) * S;
-#pragma unroll
-        for (int u = 0; u < NX; ++u) {
-            const int k = (u / MM::KPL) * MM::KSTEP + lk + u % MM::KPL;
-            xr[u] = (ok && k < S) ? xp[k] : 0.f;
-        }
-    };
-    if (lb * 64 < a.nrows) load_x(lb);
     for (int tile = lb; tile * 64 < a.nrows; tile += lg) {
         const int row0 = tile * 64;
         const int nt = min(64, a.nrows - row0);
