@@ -484,18 +484,39 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
             }
         }
     };
-    prefetch(blockIdx.x);
-    for (int gr = blockIdx.x; gr < total; gr += gridDim.x) {
+    // Each wave works on one contiguous range of rows as a single stream of
+    // pairs: a 64-lane chunk takes the rest of row A and, when that is short of
+    // 64, the first pairs of the next row B, whose fold sits in the other of two
+    // LDS slots.  The fold inputs of the next row are loaded after the
+    // displacements of the following chunk (or right before its fold).
+    const int slot_f4 = nblk * K * MB;  // float4 per slot
+    const int per = (total + (int)gridDim.x - 1) / (int)gridDim.x;
+    const int rbeg = min(total, (int)blockIdx.x * per), rend = min(total, rbeg + per);
+    struct RowRef {
+        int gr, nlrow, n;
+        float rc, inv_rc;
+    };
+    auto row_ref = [&](int gr) {
+        RowRef r{gr, 0, 0, 1.f, 1.f};
+        if (gr >= rend) return r;
         int sg = 0;  // the row's segment (species)
         while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
         const SegmentArgs &a = set.a[sg];
         const int row = gr - set.row0[sg];
-        const float rc = a.rc, inv_rc = 1.f / a.rc;
-        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
-        const int n = min(nl.cnt[i], nl.cap);
-        const size_t base = (size_t)i * nl.cap;
-        const float *__restrict__ Arow = a.A + (size_t)row * (P * M);
-        const float *__restrict__ Grow = a.G + (size_t)row * S;
+        r.nlrow = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
+        r.n = min(nl.cnt[r.nlrow], nl.cap);
+        r.rc = a.rc;
+        r.inv_rc = 1.f / a.rc;
+        return r;
+    };
+    auto next_row = [&](int gr) {  // the first row at or after gr with pairs (gr == rend: none)
+        RowRef r = row_ref(gr);
+        while (r.gr < rend && r.n == 0) r = row_ref(r.gr + 1);
+        return r;
+    };
+    // sGt of row r into LDS slot `slot` (with PF from the prefetched inputs of r)
+    auto fold = [&](const RowRef &r, int slot) {
+        float *sG = sGt + (size_t)slot * slot_f4 * 4;
         if constexpr (PF) {
 #pragma unroll
             for (int it = 0; it < NIT; ++it) {
@@ -503,119 +524,182 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
                 const int w = e + (MP - M) * (e / M);  // (q, m) -> q MP + m
                 if (e < K * M) {
                     if (MIX == MIX_NONE) {
-                        sGt[w] = pg[it][0] * pa[it][0];
+                        sG[w] = pg[it][0] * pa[it][0];
                     } else {
                         const float lo = pg[it][0] * pa[it][0];
                         const float hi = pg[it][NH - 1] * pa[it][NH - 1];
                         for (int t = 1; t <= nblk; ++t) {
                             const float wt = (t & 1) ? (float)t : -(float)t;
-                            sGt[(t - 1) * K * MP + w] = lo + wt * hi;
+                            sG[(t - 1) * K * MP + w] = lo + wt * hi;
                         }
                     }
                 }
             }
-        }
-        for (int e = lane; !PF && e < K * M; e += 64) {
-            const int q = e / M, m = e % M;
-            const int l = (int)__builtin_amdgcn_sqrtf((float)m + 0.5f);  // l(m) = floor(sqrt(m))
-            {
+        } else {
+            int sg = 0;
+            while (sg + 1 < set.nseg && r.gr >= set.row0[sg + 1]) ++sg;
+            const SegmentArgs &a = set.a[sg];
+            const int row = r.gr - set.row0[sg];
+            const float *__restrict__ Arow = a.A + (size_t)row * (P * M);
+            const float *__restrict__ Grow = a.G + (size_t)row * S;
+            for (int e = lane; e < K * M; e += 64) {
+                const int q = e / M, m = e % M;
+                const int l = (int)__builtin_amdgcn_sqrtf((float)m + 0.5f);  // l(m) = floor(sqrt(m))
                 for (int t = 1; t <= nblk; ++t) {
                     const float *w = a.W + (size_t)(t - 1) * P * K;
-                    float s = 0.f;
-                    for (int p = 0; p < P; ++p) s += w[p * K + q] * (Grow[p * L1 + l] * Arow[p * M + m]);
-                    sGt[(t - 1) * K * MP + q * MP + m] = s;
+                    float sacc = 0.f;
+                    for (int p = 0; p < P; ++p) sacc += w[p * K + q] * (Grow[p * L1 + l] * Arow[p * M + m]);
+                    sG[(t - 1) * K * MP + q * MP + m] = sacc;
                 }
             }
         }
-        __syncthreads();
-        if (n == 0) prefetch(gr + gridDim.x);
-        for (int c0 = 0; c0 < n; c0 += 64) {
-            const int j = c0 + lane;
-            const float4 dd = nl.d[base + (j < n ? j : n - 1)];
-            if (c0 == 0) prefetch(gr + gridDim.x);  // issued after this chunk's displacements
-            if (j >= n) continue;
-            const float dx = dd.x, dy = dd.y, dz = dd.z;
-            const int tj = tag_species(dd.w);
-            const PairGeomR g = pair_geom_fast(dx, dy, dz);
-            float ox = 0.f, oy = 0.f, oz = 0.f;
-            if (g.r < rc) {
-                float y[M];
-                real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
-                const float xr = 1.f - 2.f * g.r * inv_rc;
-                float rn[K];
-                cheb_rn<K>(xr, rn);
-                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-                const float t2 = t * t;
-                const float fc = t2 * t2;
-                int blk = MIX == MIX_NONE ? 0 : (tj - 1) * K * MB;
-                // opaque to the compiler, so every read below is this base plus an immediate
-                // offset (folding the -1 into the reads would make the offsets negative)
-                __asm__("" : "+v"(blk));
-                const float4 *sG = sGt4 + blk;
-                // gY[m] = fc sum_n Rn[n] Gt[n][m],  gct[n] = sum_m Gt[n][m] Y[m]
-                // (then dE/dRn = fc gct, dE/dfc = sum_n Rn[n] gct[n])
-                float gy[M];
-#pragma unroll
-                for (int m = 0; m < M; ++m) gy[m] = 0.f;
-                float gfc = 0.f;
-                float grn[K];
-#pragma unroll
-                for (int q = 0; q < K; ++q) {
-                    const float cq = fc * rn[q];
-                    // four interleaved partial sums: shorter FP32 accumulation chains
-                    float s4[4] = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-                    for (int mb = 0; mb < MB; ++mb) {
-                        const float4 g4 = sG[q * MB + mb];
-                        const float ga[4] = {g4.x, g4.y, g4.z, g4.w};
-#pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            const int m = 4 * mb + c;
-                            if (m < M) {
-                                gy[m] += ga[c] * cq;
-                                s4[c] += ga[c] * y[m];
+    };
+    RowRef A = next_row(rbeg);
+    if (A.gr >= rend) return;
+    prefetch(A.gr);
+    fold(A, 0);
+    RowRef N = next_row(A.gr + 1);  // the next row to fold
+    int pf = N.gr;                   // row whose inputs are still to be loaded (rend: none)
+    RowRef B = A;
+    int kA = 0, slotA = 0, slotB = 1;
+    bool haveB = false;
+    __syncthreads();
+    while (true) {
+        const int remA = A.n - kA;
+        if (remA < 64 && !haveB && N.gr < rend) {  // row B joins this chunk
+            if (pf == N.gr) {
+                prefetch(N.gr);
+                pf = rend;
+            }
+            B = N;
+            slotB = slotA ^ 1;
+            __syncthreads();  // the last reads of that slot are done
+            fold(B, slotB);
+            __syncthreads();
+            haveB = true;
+            N = next_row(B.gr + 1);
+            pf = N.gr;
+        }
+        const int cntA = min(remA, 64);
+        const int takeB = haveB ? min(64 - cntA, B.n) : 0;
+        {
+            const bool inA = lane < cntA;
+            const bool on = inA || lane < cntA + takeB;
+            const int j = inA ? kA + lane : lane - cntA;
+            const int slot = inA ? slotA : slotB;
+            const float rc = inA ? A.rc : B.rc, inv_rc = inA ? A.inv_rc : B.inv_rc;
+            const size_t base = (size_t)(inA ? A.nlrow : B.nlrow) * nl.cap;
+            const float4 dd = on ? nl.d[base + j] : make_float4(1.f, 0.f, 0.f, 0.f);
+            if (pf < rend) {  // issued after this chunk's displacements
+                prefetch(pf);
+                pf = rend;
+            }
+            if (on) {
+                const float dx = dd.x, dy = dd.y, dz = dd.z;
+                const int tj = tag_species(dd.w);
+                const PairGeomR g = pair_geom_fast(dx, dy, dz);
+                float ox = 0.f, oy = 0.f, oz = 0.f;
+                if (g.r < rc) {
+                    float y[M];
+                    real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
+                    const float xr = 1.f - 2.f * g.r * inv_rc;
+                    float rn[K];
+                    cheb_rn<K>(xr, rn);
+                    const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+                    const float t2 = t * t;
+                    const float fc = t2 * t2;
+                    int blk = slot * slot_f4 + (MIX == MIX_NONE ? 0 : (tj - 1) * K * MB);
+                    // opaque to the compiler, so every read below is this base plus an immediate
+                    // offset (folding the -1 into the reads would make the offsets negative)
+                    __asm__("" : "+v"(blk));
+                    const float4 *sG = sGt4 + blk;
+                    // gY[m] = fc sum_n Rn[n] Gt[n][m],  gct[n] = sum_m Gt[n][m] Y[m]
+                    // (then dE/dRn = fc gct, dE/dfc = sum_n Rn[n] gct[n])
+                    float gy[M];
+    #pragma unroll
+                    for (int m = 0; m < M; ++m) gy[m] = 0.f;
+                    float gfc = 0.f;
+                    float grn[K];
+    #pragma unroll
+                    for (int q = 0; q < K; ++q) {
+                        const float cq = fc * rn[q];
+                        // four interleaved partial sums: shorter FP32 accumulation chains
+                        float s4[4] = {0.f, 0.f, 0.f, 0.f};
+    #pragma unroll
+                        for (int mb = 0; mb < MB; ++mb) {
+                            const float4 g4 = sG[q * MB + mb];
+                            const float ga[4] = {g4.x, g4.y, g4.z, g4.w};
+    #pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const int m = 4 * mb + c;
+                                if (m < M) {
+                                    gy[m] += ga[c] * cq;
+                                    s4[c] += ga[c] * y[m];
+                                }
                             }
                         }
+                        const float s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+                        gfc += s * rn[q];
+                        grn[q] = s * fc;
+                        if (BAR > 0 && q % BSTEP == BSTEP - 1) __builtin_amdgcn_sched_barrier(0);
                     }
-                    const float s = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-                    gfc += s * rn[q];
-                    grn[q] = s * fc;
-                    if (BAR > 0 && q % BSTEP == BSTEP - 1) __builtin_amdgcn_sched_barrier(0);
-                }
-                float drn[K];
-                cheb_drn<K>(xr, inv_rc, drn);
-                float dEdr = 0.f;
-#pragma unroll
-                for (int q = 0; q < K; ++q) dEdr += grn[q] * drn[q];
-                const float dfc = -8.f * g.r * inv_rc * inv_rc * t2 * t;
-                dEdr += gfc * dfc;
-                float dEdph = 0.f;
-#pragma unroll
-                for (int l = 1; l < L1; ++l) {
-                    float pl = 0.f;
-#pragma unroll
-                    for (int m = 1; m <= l; ++m) {
-                        const int idx = l * l + l;
-                        // dY_{l,m}/dphi = -m Y_{l,-m},  dY_{l,-m}/dphi = m Y_{l,m}
-                        pl += (float)m * (gy[idx - m] * y[idx + m] - gy[idx + m] * y[idx - m]);
+                    float drn[K];
+                    cheb_drn<K>(xr, inv_rc, drn);
+                    float dEdr = 0.f;
+    #pragma unroll
+                    for (int q = 0; q < K; ++q) dEdr += grn[q] * drn[q];
+                    const float dfc = -8.f * g.r * inv_rc * inv_rc * t2 * t;
+                    dEdr += gfc * dfc;
+                    float dEdph = 0.f;
+    #pragma unroll
+                    for (int l = 1; l < L1; ++l) {
+                        float pl = 0.f;
+    #pragma unroll
+                        for (int m = 1; m <= l; ++m) {
+                            const int idx = l * l + l;
+                            // dY_{l,m}/dphi = -m Y_{l,-m},  dY_{l,-m}/dphi = m Y_{l,m}
+                            pl += (float)m * (gy[idx - m] * y[idx + m] - gy[idx + m] * y[idx - m]);
+                        }
+                        dEdph += pl;
                     }
-                    dEdph += pl;
+                    const float dEdth = dtheta_contract<LMAX>(g.ct, g.st, g.cp, g.sp, gy);
+                    const float ir = g.ir;
+                    // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi))
+                    ox = -(dEdr * dx * ir + dEdth * g.ct * g.cp * ir);
+                    oy = -(dEdr * dy * ir + dEdth * g.ct * g.sp * ir);
+                    oz = -(dEdr * dz * ir - dEdth * g.st * ir);
+                    if (!g.pole) {
+                        const float idxy = g.idxy;
+                        ox -= dEdph * (-g.sp * idxy);
+                        oy -= dEdph * (g.cp * idxy);
+                    }
                 }
-                const float dEdth = dtheta_contract<LMAX>(g.ct, g.st, g.cp, g.sp, gy);
-                const float ir = g.ir;
-                // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi))
-                ox = -(dEdr * dx * ir + dEdth * g.ct * g.cp * ir);
-                oy = -(dEdr * dy * ir + dEdth * g.ct * g.sp * ir);
-                oz = -(dEdr * dz * ir - dEdth * g.st * ir);
-                if (!g.pole) {
-                    const float idxy = g.idxy;
-                    ox -= dEdph * (-g.sp * idxy);
-                    oy -= dEdph * (g.cp * idxy);
-                }
+                nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
             }
-            nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
         }
+        kA += cntA;
+        if (kA < A.n) continue;
+        if (haveB) {  // A is done: B carries on
+            A = B;
+            slotA = slotB;
+            kA = takeB;
+            haveB = false;
+            if (kA < A.n) continue;
+        }
+        // the next row starts a chunk of its own
+        if (N.gr >= rend) break;
+        if (pf == N.gr) {
+            prefetch(N.gr);
+            pf = rend;
+        }
+        A = N;
+        slotA ^= 1;
+        kA = 0;
         __syncthreads();
+        fold(A, slotA);
+        __syncthreads();
+        N = next_row(A.gr + 1);
+        pf = N.gr;
     }
 }
 
@@ -644,7 +728,8 @@ static void run_bwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl
     const int rows = set.row0[set.nseg];
     int grid = rows < 262144 ? rows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
-    const size_t lds = sizeof(float) * (size_t)bwd_blocks<MIX>(a.ntypes) * K * bwd_row((LMAX + 1) * (LMAX + 1));
+    // two slots: the rows a chunk of pairs can straddle
+    const size_t lds = 2 * sizeof(float) * (size_t)bwd_blocks<MIX>(a.ntypes) * K * bwd_row((LMAX + 1) * (LMAX + 1));
     if (a.variant == 1)
         hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 2>), dim3(grid), dim3(64), lds, s, set, nl);
     else if (a.variant == 2)
