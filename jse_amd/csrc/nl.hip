@@ -246,7 +246,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
     int *__restrict__ overflow) {
     __shared__ int s_start[27], s_cnt[27], s_code[27], s_pref[28];
     __shared__ double s_cx[kStage], s_cy[kStage], s_cz[kStage];
-    __shared__ int s_ckey[kStage], s_cslot[kStage];
+    __shared__ int s_ckey[kStage], s_cslot[kStage], s_ctype[kStage];
     __shared__ int s_hkey[kCellWaves][kNlLds], s_hq[kCellWaves][kNlLds];  // hits: key, candidate index
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
     // each XCD takes a contiguous run of cells, whose 27-cell blocks overlap in its L2
@@ -258,26 +258,30 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
         const ReplicaGrid &g = reps[r];
         const int lc = c - g.cell_off;
         const int cx = lc % g.size[0], cy = (lc / g.size[0]) % g.size[1], cz = lc / (g.size[0] * g.size[1]);
-        if (tid < 27) {
-            const int w = tid;
-            int ix = cx + (w % 3) - 1, iy = cy + (w / 3) % 3 - 1, iz = cz + w / 9 - 1;
-            int sx = 0, sy = 0, sz = 0;
-            if (ix >= g.size[0]) { ix -= g.size[0]; sx = 1; } else if (ix < 0) { ix += g.size[0]; sx = -1; }
-            if (iy >= g.size[1]) { iy -= g.size[1]; sy = 1; } else if (iy < 0) { iy += g.size[1]; sy = -1; }
-            if (iz >= g.size[2]) { iz -= g.size[2]; sz = 1; } else if (iz < 0) { iz += g.size[2]; sz = -1; }
-            const int cc = g.cell_off + ix + g.size[0] * (iy + g.size[1] * iz);
-            s_start[w] = cstart[cc];
-            s_cnt[w] = cstart[cc + 1] - cstart[cc];
-            s_code[w] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-        }
-        __syncthreads();
-        if (tid == 0) {
-            int run = 0;
-            for (int w = 0; w < 27; ++w) {
-                s_pref[w] = run;
-                run += s_cnt[w];
+        if (tid < 64) {  // wave 0: the 27 neighbour cells and their prefix (a shuffle scan)
+            int cnt = 0;
+            if (tid < 27) {
+                const int w = tid;
+                int ix = cx + (w % 3) - 1, iy = cy + (w / 3) % 3 - 1, iz = cz + w / 9 - 1;
+                int sx = 0, sy = 0, sz = 0;
+                if (ix >= g.size[0]) { ix -= g.size[0]; sx = 1; } else if (ix < 0) { ix += g.size[0]; sx = -1; }
+                if (iy >= g.size[1]) { iy -= g.size[1]; sy = 1; } else if (iy < 0) { iy += g.size[1]; sy = -1; }
+                if (iz >= g.size[2]) { iz -= g.size[2]; sz = 1; } else if (iz < 0) { iz += g.size[2]; sz = -1; }
+                const int cc = g.cell_off + ix + g.size[0] * (iy + g.size[1] * iz);
+                const int st = cstart[cc];
+                cnt = cstart[cc + 1] - st;
+                s_start[w] = st;
+                s_cnt[w] = cnt;
+                s_code[w] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
             }
-            s_pref[27] = run;
+            int incl = cnt;
+#pragma unroll
+            for (int off = 1; off < 32; off <<= 1) {
+                const int u = __shfl_up(incl, off);
+                if (tid >= off) incl += u;
+            }
+            if (tid < 27) s_pref[tid] = incl - cnt;
+            if (tid == 26) s_pref[27] = incl;
         }
         __syncthreads();
         const int total = s_pref[27];
@@ -306,13 +310,27 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                 s_cz[q] = zj;
                 s_ckey[q] = key;
                 s_cslot[q] = slot;
+                s_ctype[q] = nl.types[key >> 5];
             }
         }
         __syncthreads();
         int *hkey = s_hkey[wv], *hq = s_hq[wv];
         for (int sl = cs + wv; sl < ce; sl += kCellWaves) {
-            const int i = sidx[sl];  // the row goes to slot sl; keys use atom numbers
-            const double xi = sxw[3 * (size_t)sl], yi = sxw[3 * (size_t)sl + 1], zi = sxw[3 * (size_t)sl + 2];
+            // the atom itself: from the staged centre cell (no shift), else from memory
+            int i;  // the row goes to slot sl; keys use atom numbers
+            double xi, yi, zi;
+            if (staged) {
+                const int q = s_pref[13] + (sl - cs);
+                i = s_ckey[q] >> 5;
+                xi = s_cx[q];
+                yi = s_cy[q];
+                zi = s_cz[q];
+            } else {
+                i = sidx[sl];
+                xi = sxw[3 * (size_t)sl];
+                yi = sxw[3 * (size_t)sl + 1];
+                zi = sxw[3 * (size_t)sl + 2];
+            }
             // candidate q: key, slot and displacement from atom i (the same FP64 arithmetic wherever it is formed)
             auto cand = [&](int q, int &key, int &slot, double &dx, double &dy, double &dz) {
                 double xj, yj, zj;
@@ -351,21 +369,29 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
             }
             wave_lds_sync();
             // keys are unique: a hit's slot in the sorted row is the number of smaller
-            // keys, counted against the whole hit buffer (broadcast reads)
+            // keys.  Up to 64 hits each lane holds its hit's key and counts against the
+            // others read lane by lane (no LDS round trips); longer rows count against
+            // the whole hit buffer (broadcast reads).
             const int nst = hits < kNlLds ? hits : kNlLds;
             const size_t row = (size_t)sl * nl.cap;
             for (int t = lane; t < nst; t += 64) {
                 const int kt = hkey[t];
                 int rank = 0;
-                for (int u = 0; u < nst; ++u) rank += hkey[u] < kt ? 1 : 0;
+                if (nst <= 64) {
+                    for (int u = 0; u < nst; ++u) rank += __builtin_amdgcn_readlane(kt, u) < kt ? 1 : 0;
+                } else {
+                    for (int u = 0; u < nst; ++u) rank += hkey[u] < kt ? 1 : 0;
+                }
                 if (rank < nl.cap) {
                     nl.key[row + rank] = kt;
                     if (nl.d) {
+                        const int q = hq[t];
                         int key, slot;
                         double dx, dy, dz;
-                        cand(hq[t], key, slot, dx, dy, dz);
+                        cand(q, key, slot, dx, dy, dz);
+                        const int tj = staged ? s_ctype[q] : nl.types[kt >> 5];
                         // j's own row is its slot
-                        nl.d[row + rank] = make_float4((float)dx, (float)dy, (float)dz, pair_tag(slot, nl.types[kt >> 5]));
+                        nl.d[row + rank] = make_float4((float)dx, (float)dy, (float)dz, pair_tag(slot, tj));
                     }
                 }
             }
