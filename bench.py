@@ -189,6 +189,60 @@ def pcie_inclusive(pot, st, iters=20):
     return {"ms_per_call": dt * 1e3, "atom_evals_per_s": st.natoms / dt}
 
 
+def gpumd_hook(pot, st, iters=20):
+    """The GPUMD hook (jse_nnap_computeGPUMD, packed arguments) on the same
+    configuration: GPUMD's device-resident full neighbour list (column-major,
+    FP32 minimum-image displacements within rc + 0.5 A, built once), forces,
+    per-atom virials and energies added into FP64 device arrays.  Each call is
+    synchronous, as the reference's; the time per call includes the hook's
+    species grouping (a copy of the centres' types) and capacity check."""
+    import ctypes
+    import struct
+    import torch
+    from scipy.spatial import cKDTree
+    lib = nnap.lib()
+    for name in ("jse_nnap_gpumd_bind", "jse_nnap_computeGPUMD"):
+        getattr(lib, name).argtypes = [ctypes.c_void_p]
+    L = np.asarray(st.box, float)
+    x = st.xyz - np.floor(st.xyz / L) * L
+    n = st.natoms
+    rg = pot.rcutMax() + 0.5
+    pairs = cKDTree(x, boxsize=L).query_pairs(rg, output_type="ndarray")
+    i_all = np.concatenate([pairs[:, 0], pairs[:, 1]])
+    j_all = np.concatenate([pairs[:, 1], pairs[:, 0]])
+    order = np.lexsort((j_all, i_all))
+    i_all, j_all = i_all[order], j_all[order]
+    num = np.bincount(i_all, minlength=n).astype(np.int32)
+    slot = np.arange(len(i_all)) - np.repeat(np.cumsum(num) - num, num)
+    mn = int(num.max())
+    nbr = np.zeros((mn, n), dtype=np.int32)
+    nbr[slot, i_all] = j_all
+    dv = x[j_all] - x[i_all]
+    dv -= np.round(dv / L) * L
+    d = np.zeros((3, mn, n), dtype=np.float32)
+    for c in range(3):
+        d[c, slot, i_all] = dv[:, c]
+    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
+    t_num, t_nbr, t_type = dev(num), dev(nbr), dev((st.types - 1).astype(np.int32))
+    t_d = [dev(d[c]) for c in range(3)]
+    out = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(3)]
+    vir = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
+    pe = torch.zeros(n, dtype=torch.float64, device="cuda")
+    assert lib.jse_nnap_gpumd_bind(ctypes.create_string_buffer(struct.pack("<Qi", pot._h.value, nnap.FP32))) == 0
+    ptrs = [t_num.data_ptr(), t_nbr.data_ptr()] + [t.data_ptr() for t in t_d] + [t_type.data_ptr()] + [0] * 7 + \
+        [t.data_ptr() for t in out] + [vir.data_ptr(), pe.data_ptr()] + [0] * 9
+    blk = ctypes.create_string_buffer(struct.pack("<iii" + "Q" * len(ptrs), n, 0, n, *ptrs))
+    torch.cuda.synchronize()
+    for _ in range(3):
+        assert lib.jse_nnap_computeGPUMD(blk) == 0, lib.jse_nnap_last_error()
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        assert lib.jse_nnap_computeGPUMD(blk) == 0, lib.jse_nnap_last_error()
+    dt = (time.perf_counter() - t0) / iters
+    return {"ms_per_call": dt * 1e3, "atom_evals_per_s": n / dt, "pairs_listed": int(num.sum()),
+            "note": "device-resident GPUMD neighbour list; rows grouped by species on each call"}
+
+
 def copy_calibration(nbytes, reps=5):
     """A plain device-to-device copy of `nbytes` (torch's copy kernel), the
     reference point that scales the FETCH_SIZE / WRITE_SIZE counters."""
@@ -413,6 +467,7 @@ def main():
         "roofline": roof,
     }
     line["pcie_inclusive"] = pcie_inclusive(pot, st)
+    line["gpumd_hook"] = gpumd_hook(pot, st)
     if not args.no_cpu_baseline:
         base, err = cpu_baseline(md, st, res)
         line["cpu_baseline"] = base

@@ -236,6 +236,27 @@ int jse_nnap_lammps2cuda(void *args);
 int jse_nnap_computeLammpsCuda(void *args);
 int jse_nnap_cuda2lammps(void *args);
 
+/* ---- GPUMD hook, jse-compatible packed arguments --------------------------
+ * Replaces the reference's generated jse_nnap_computeGPUMD
+ * (nnap_interface_cuda.cu:485-518; driver NNAP.computeGPUMD,
+ * NNAP.java:1047-1083), same argument list with flt_t = float:
+ *   (int number_of_particles, int N1, int N2, const int *g_neighbor_number,
+ *    const int *g_neighbor_list, const float *nl_dx, const float *nl_dy,
+ *    const float *nl_dz, const int *g_type, const int *nmerges,
+ *    const int **mergeSorted, const float **cutsq, float **aFpHyperParam,
+ *    float **aFpParam, float **aNnParam, float **aNormParam, double *g_fx,
+ *    double *g_fy, double *g_fz, double *g_virial, double *g_potential,
+ *    + 9 scratch pointers)
+ * All arrays are GPUMD's device arrays: the neighbour list column-major
+ * [jj * number_of_particles + i] with FP32 displacements r_j - r_i, 0-based
+ * types; centres are particles N1 .. N2-1, and forces, per-atom virials
+ * ([9][number_of_particles], on the neighbour) and potentials are added into.
+ * jse_nnap_gpumd_bind (no reference counterpart) names the model once:
+ * packed (jse_nnap_model model, int precision); the parameter blocks, merge
+ * tables and scratch pointers are then not read.  Returns 0 on success. */
+int jse_nnap_gpumd_bind(void *args);
+int jse_nnap_computeGPUMD(void *args);
+
 #ifdef __cplusplus
 }
 #endif

@@ -132,4 +132,9 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
                   const NeighborList &nl);
 // Every species' MLP: one launch when their shapes allow it, else one per species.
 void enqueue_mlp_all(hipStream_t s, const jse_nnap_model_s *m, int precision, const MlpArgs *args, int nsp);
+// Descriptor forward, MLP and descriptor backward of a plugin hook's rows:
+// the rows of species t are rows[seg[t] .. seg[t+1]-1] (list rows, in the
+// order of nl); E receives one energy per list row.
+void enqueue_hook_rows(hipStream_t s, jse_nnap_model model, int precision, const int *seg, const int *rows,
+                       const int *types, float *xs, float *A, float *G, double *E, const NeighborList &nl);
 }  // namespace jnn

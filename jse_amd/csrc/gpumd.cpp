@@ -1,0 +1,183 @@
+// The reference's GPUMD hook as a jse-compatible packed-argument entry point
+// (jse_nnap_computeGPUMD, nnap_interface_cuda.cu:485-518, driven by
+// NNAP.computeGPUMD, NNAP.java:1047-1083).  jse binds native functions as
+// `int name(void *args)` with the declared parameters packed back to back in
+// declaration order, no padding (SourceScanner.java:167-290).
+//
+// GPUMD's arrays are all on the device: a full neighbour list, column-major
+// [jj * N + i], with FP32 displacements and 0-based types, and FP64 outputs
+// that the hook adds into (forces, per-atom virials [9][N], potential).  As
+// with the LAMMPS hook, the model is named once (jse_nnap_gpumd_bind) and the
+// reference's per-call parameter blocks, merge tables and scratch buffers are
+// accepted and not read.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <mutex>
+#include <vector>
+
+#include "capi_internal.h"
+#include "common.h"
+#include "kernels.h"
+
+using namespace jnn;
+
+namespace jnn {
+void launch_gpumd_types(hipStream_t s, int n, const int *g_type, int *out);
+void launch_gpumd_rows(hipStream_t s, int N, int N1, int N2, const int *num, const int *list, const float *nl_dx,
+                       const float *nl_dy, const float *nl_dz, float rc2, const NeighborList &nl, int *overflow);
+void launch_gpumd_scatter(hipStream_t s, int N, int N1, int nrow, const NeighborList &nl, const double *E, double *fx,
+                          double *fy, double *fz, double *vir, double *pot);
+}  // namespace jnn
+
+namespace {
+
+struct Args {
+    const char *p;
+    explicit Args(void *a) : p((const char *)a) {}
+    template <class T>
+    T get() {
+        T v;
+        std::memcpy(&v, p, sizeof(T));
+        p += sizeof(T);
+        return v;
+    }
+};
+
+// Device state of the hook between calls (one potential per process).
+struct GpumdContext {
+    jse_nnap_model model = nullptr;
+    int precision = JSE_NNAP_FP32;
+    hipStream_t stream = nullptr;
+    int cap = 64;
+    size_t n_alloc = 0, row_alloc = 0, row_n_alloc = 0;
+    int *types = nullptr, *rows = nullptr, *overflow = nullptr, *h_overflow = nullptr;
+    float *xs = nullptr, *A = nullptr, *G = nullptr;
+    double *E = nullptr;
+    NeighborList nl{};
+
+    void ensure(int n, int nrow, int cap_) {
+        if (!stream) JNN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
+        if (!h_overflow) h_overflow = halloc<int>(1);
+        if (!overflow) overflow = dalloc<int>(1);
+        if ((size_t)n > n_alloc) {
+            dfree(types);
+            n_alloc = (size_t)n + n / 4 + 64;
+            types = dalloc<int>(n_alloc);
+        }
+        const size_t need = (size_t)nrow * cap_;
+        if ((size_t)nrow > row_n_alloc || need > row_alloc) {
+            dfree(rows); dfree(xs); dfree(A); dfree(G); dfree(E);
+            dfree(nl.cnt); dfree(nl.key); dfree(nl.d); dfree(nl.g);
+            row_n_alloc = std::max(row_n_alloc, (size_t)nrow + nrow / 4 + 64);
+            row_alloc = row_n_alloc * cap_;
+            rows = dalloc<int>(row_n_alloc);
+            xs = dalloc<float>(row_n_alloc * model->smax);
+            G = dalloc<float>(row_n_alloc * model->smax);
+            A = dalloc<float>(row_n_alloc * model->pmmax);
+            E = dalloc<double>(row_n_alloc);
+            nl.cnt = dalloc<int>(row_n_alloc);
+            nl.key = dalloc<int>(row_alloc);
+            nl.d = dalloc<float4>(row_alloc);
+            nl.g = dalloc<float4>(row_alloc);
+        }
+        nl.cap = cap_;
+    }
+};
+GpumdContext g_gpumd;
+std::mutex g_gpumd_mu;
+
+}  // namespace
+
+extern "C" {
+
+// Binds the model evaluated by the hook: packed (jse_nnap_model model, int precision).
+int jse_nnap_gpumd_bind(void *args) {
+    return guard([&] {
+        JNN_CHECK(args, ERR_ARG, "null argument block");
+        Args a(args);
+        const jse_nnap_model m = a.get<jse_nnap_model>();
+        const int prec = a.get<int>();
+        JNN_CHECK(m, ERR_ARG, "null model");
+        JNN_CHECK(prec == JSE_NNAP_FP32 || prec == JSE_NNAP_BF16_MLP, ERR_ARG, "unknown precision");
+        std::lock_guard<std::mutex> lk(g_gpumd_mu);
+        g_gpumd.model = m;
+        g_gpumd.precision = prec;
+    });
+}
+
+// nnap_interface_cuda.cu:485-493: (int number_of_particles, int N1, int N2, const int *g_neighbor_number,
+// const int *g_neighbor_list, const float *nl_dx, const float *nl_dy, const float *nl_dz, const int *g_type,
+// const int *nmerges, const int **mergeSorted, const flt_t **cutsq, flt_t **aFpHyperParam, flt_t **aFpParam,
+// flt_t **aNnParam, flt_t **aNormParam, double *g_fx, double *g_fy, double *g_fz, double *g_virial,
+// double *g_potential, then 9 scratch buffers), flt_t = float.  Centres are the particles N1 .. N2-1;
+// forces, virials (on the neighbour) and energies are added into GPUMD's arrays.
+int jse_nnap_computeGPUMD(void *args) {
+    return guard([&] {
+        JNN_CHECK(args, ERR_ARG, "null argument block");
+        Args a(args);
+        const int N = a.get<int>(), N1 = a.get<int>(), N2 = a.get<int>();
+        const int *num = a.get<const int *>();
+        const int *list = a.get<const int *>();
+        const float *nl_dx = a.get<const float *>();
+        const float *nl_dy = a.get<const float *>();
+        const float *nl_dz = a.get<const float *>();
+        const int *g_type = a.get<const int *>();
+        (void)a.get<const int *>();     // nmerges
+        (void)a.get<const int **>();    // mergeSorted
+        (void)a.get<const float **>();  // cutsq
+        for (int k = 0; k < 4; ++k) (void)a.get<float **>();  // parameter blocks (the bound model is used)
+        double *g_fx = a.get<double *>();
+        double *g_fy = a.get<double *>();
+        double *g_fz = a.get<double *>();
+        double *g_virial = a.get<double *>();
+        double *g_potential = a.get<double *>();
+        JNN_CHECK(N >= 0 && N1 >= 0 && N2 >= N1 && N2 <= N, ERR_ARG, "bad GPUMD particle range");
+        JNN_CHECK(N < (1 << 26), ERR_ARG, "number_of_particles out of range [0, 2^26)");
+        std::lock_guard<std::mutex> lk(g_gpumd_mu);
+        GpumdContext &c = g_gpumd;
+        JNN_CHECK(c.model, ERR_ARG, "no model bound (jse_nnap_gpumd_bind)");
+        const int nrow = N2 - N1;
+        if (nrow == 0) return;
+        // GPUMD's inputs come from its own stream(s): wait for them, as the reference's
+        // legacy-default-stream launches do
+        JNN_HIP(hipDeviceSynchronize());
+        const ModelHost &mh = c.model->host;
+        const int nt = mh.ntypes();
+        // species-grouped rows (host) from the centres' types
+        std::vector<int> ht(nrow);
+        JNN_HIP(hipMemcpy(ht.data(), g_type + N1, sizeof(int) * nrow, hipMemcpyDeviceToHost));
+        std::vector<int> seg(nt + 1, 0), rows(nrow);
+        for (int r = 0; r < nrow; ++r) {
+            const int t = ht[r] + 1;
+            JNN_CHECK(t >= 1 && t <= nt, ERR_TYPE, "GPUMD type outside the model's types");
+            seg[t]++;
+        }
+        for (int t = 1; t <= nt; ++t) seg[t] += seg[t - 1];
+        {
+            std::vector<int> fill(seg.begin(), seg.end() - 1);
+            for (int r = 0; r < nrow; ++r) rows[fill[ht[r]]++] = r;
+        }
+        hipStream_t s = c.stream;
+        const float rc = (float)mh.rcut_max();
+        for (int attempt = 0;; ++attempt) {
+            c.ensure(N, nrow, c.cap);
+            JNN_HIP(hipMemcpyAsync(c.rows, rows.data(), sizeof(int) * nrow, hipMemcpyHostToDevice, s));
+            launch_gpumd_types(s, N, g_type, c.types);
+            launch_fill_int(s, 1, c.overflow, 0);
+            c.nl.types = c.types;
+            launch_gpumd_rows(s, N, N1, N2, num, list, nl_dx, nl_dy, nl_dz, rc * rc, c.nl, c.overflow);
+            JNN_HIP(hipMemcpyAsync(c.h_overflow, c.overflow, sizeof(int), hipMemcpyDeviceToHost, s));
+            JNN_HIP(hipStreamSynchronize(s));
+            if (*c.h_overflow <= c.nl.cap) break;
+            JNN_CHECK(attempt < 2 && *c.h_overflow <= 4096, ERR_CAPACITY, "neighbour row capacity exceeded");
+            c.cap = ((*c.h_overflow + 31) / 32) * 32;
+        }
+        enqueue_hook_rows(s, c.model, c.precision, seg.data(), c.rows, c.types, c.xs, c.A, c.G, c.E, c.nl);
+        launch_gpumd_scatter(s, N, N1, nrow, c.nl, c.E, g_fx, g_fy, g_fz, g_virial, g_potential);
+        JNN_HIP(hipGetLastError());
+        JNN_HIP(hipStreamSynchronize(s));
+    });
+}
+
+}  // extern "C"

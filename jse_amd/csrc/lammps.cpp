@@ -104,6 +104,39 @@ std::mutex g_lmp_mu;
 
 }  // namespace
 
+namespace jnn {
+void enqueue_hook_rows(hipStream_t s, jse_nnap_model model, int precision, const int *seg, const int *rows,
+                       const int *types, float *xs, float *A, float *G, double *E, const NeighborList &nl) {
+    const ModelHost &mh = model->host;
+    const int nt = mh.ntypes();
+    for (int t = 0; t < nt; ++t) {
+        const SpeciesHost &sp = mh.species[t];
+        const SpeciesDev &dv = model->dev[t];
+        SegmentArgs sa{};
+        sa.nrows = seg[t + 1] - seg[t];
+        if (!sa.nrows) continue;
+        sa.rows_atom = rows + seg[t];
+        sa.types = types;
+        sa.rc = (float)sp.rcut;
+        sa.ntypes = nt;
+        sa.W = dv.W;
+        sa.mu = dv.mu;
+        sa.sig = dv.sig;
+        sa.mu64 = dv.mu64;
+        sa.sig64 = dv.sig64;
+        sa.fp32_acc = precision == JSE_NNAP_BF16_MLP;
+        sa.x = xs + (size_t)seg[t] * model->smax;
+        sa.A = A + (size_t)seg[t] * model->pmmax;
+        sa.G = G + (size_t)seg[t] * model->smax;
+        if (!launch_desc_fwd(s, sp.lmax, sp.K(), sp.P, sp.mix, sa, nl))
+            throw Error(ERR_UNSUPPORTED, "no descriptor kernel compiled for this species shape");
+        enqueue_mlp(s, model, t, precision, sa.nrows, sa.rows_atom, sa.x, sa.G, E, true);
+        if (!launch_desc_bwd(s, sp.lmax, sp.K(), sp.P, sp.mix, sa, nl))
+            throw Error(ERR_UNSUPPORTED, "no descriptor backward kernel compiled for this species shape");
+    }
+}
+}  // namespace jnn
+
 extern "C" {
 
 // Binds the model evaluated by the hook: packed (jse_nnap_model model, int precision).
@@ -234,7 +267,6 @@ int jse_nnap_computeLammpsCuda(void *args) {
             for (int ii = 0; ii < inum; ++ii) rows[fill[hmap[c.h_type[c.h_ilist[ii]]] - 1]++] = ii;
         }
         hipStream_t s = c.stream;
-        const bool grad = true;
         for (int attempt = 0;; ++attempt) {
             c.ensure(nall, inum, c.cap);
             if (inum) JNN_HIP(hipMemcpyAsync(c.rows, rows.data(), sizeof(int) * inum, hipMemcpyHostToDevice, s));
@@ -248,31 +280,7 @@ int jse_nnap_computeLammpsCuda(void *args) {
             JNN_CHECK(attempt < 2 && *c.h_overflow <= 4096, ERR_CAPACITY, "neighbour row capacity exceeded");
             c.cap = ((*c.h_overflow + 31) / 32) * 32;
         }
-        for (int t = 0; t < nt; ++t) {
-            const SpeciesHost &sp = mh.species[t];
-            const SpeciesDev &dv = c.model->dev[t];
-            SegmentArgs sa{};
-            sa.nrows = seg[t + 1] - seg[t];
-            if (!sa.nrows) continue;
-            sa.rows_atom = c.rows + seg[t];
-            sa.types = c.types;
-            sa.rc = (float)sp.rcut;
-            sa.ntypes = nt;
-            sa.W = dv.W;
-            sa.mu = dv.mu;
-            sa.sig = dv.sig;
-            sa.mu64 = dv.mu64;
-            sa.sig64 = dv.sig64;
-            sa.fp32_acc = c.precision == JSE_NNAP_BF16_MLP;
-            sa.x = c.xs + (size_t)seg[t] * c.model->smax;
-            sa.A = c.A + (size_t)seg[t] * c.model->pmmax;
-            sa.G = c.G + (size_t)seg[t] * c.model->smax;
-            if (!launch_desc_fwd(s, sp.lmax, sp.K(), sp.P, sp.mix, sa, c.nl))
-                throw Error(ERR_UNSUPPORTED, "no descriptor kernel compiled for this species shape");
-            enqueue_mlp(s, c.model, t, c.precision, sa.nrows, sa.rows_atom, sa.x, sa.G, c.E, grad);
-            if (!launch_desc_bwd(s, sp.lmax, sp.K(), sp.P, sp.mix, sa, c.nl))
-                throw Error(ERR_UNSUPPORTED, "no descriptor backward kernel compiled for this species shape");
-        }
+        enqueue_hook_rows(s, c.model, c.precision, seg.data(), c.rows, c.types, c.xs, c.A, c.G, c.E, c.nl);
         launch_zero_double(s, 3 * (size_t)nall, c.F);
         launch_zero_double(s, 9 * (size_t)nall, c.V1);
         launch_lmp_scatter(s, inum, nall, ilist, c.nl, vflag, vflag_atom, cvflag_atom, c.F, c.V0, c.V1);
