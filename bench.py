@@ -198,7 +198,6 @@ def gpumd_hook(pot, st, iters=20):
     species grouping (a copy of the centres' types) and capacity check."""
     import ctypes
     import struct
-    import torch
     from scipy.spatial import cKDTree
     lib = nnap.lib()
     for name in ("jse_nnap_gpumd_bind", "jse_nnap_computeGPUMD"):
@@ -222,23 +221,32 @@ def gpumd_hook(pot, st, iters=20):
     d = np.zeros((3, mn, n), dtype=np.float32)
     for c in range(3):
         d[c, slot, i_all] = dv[:, c]
-    dev = lambda a: torch.from_numpy(np.ascontiguousarray(a)).cuda()
-    t_num, t_nbr, t_type = dev(num), dev(nbr), dev((st.types - 1).astype(np.int32))
-    t_d = [dev(d[c]) for c in range(3)]
-    out = [torch.zeros(n, dtype=torch.float64, device="cuda") for _ in range(3)]
-    vir = torch.zeros(9 * n, dtype=torch.float64, device="cuda")
-    pe = torch.zeros(n, dtype=torch.float64, device="cuda")
+    # device buffers from the HIP runtime the library itself uses (as GPUMD's would be)
+    hip = ctypes.CDLL("libamdhip64.so.7", mode=ctypes.RTLD_GLOBAL)
+    hip.hipMalloc.argtypes = [ctypes.POINTER(ctypes.c_void_p), ctypes.c_size_t]
+    hip.hipFree.argtypes = [ctypes.c_void_p]
+    hip.hipMemcpy.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int]
+    bufs = []
+
+    def dev(a):
+        a = np.ascontiguousarray(a)
+        p = ctypes.c_void_p()
+        assert hip.hipMalloc(ctypes.byref(p), max(1, a.nbytes)) == 0
+        assert hip.hipMemcpy(p, a.ctypes.data, a.nbytes, 1) == 0  # host to device
+        bufs.append(p)
+        return p.value
+    ptrs = [dev(num), dev(nbr)] + [dev(d[c]) for c in range(3)] + [dev((st.types - 1).astype(np.int32))] + [0] * 7 + \
+        [dev(np.zeros(n)) for _ in range(3)] + [dev(np.zeros(9 * n)), dev(np.zeros(n))] + [0] * 9
     assert lib.jse_nnap_gpumd_bind(ctypes.create_string_buffer(struct.pack("<Qi", pot._h.value, nnap.FP32))) == 0
-    ptrs = [t_num.data_ptr(), t_nbr.data_ptr()] + [t.data_ptr() for t in t_d] + [t_type.data_ptr()] + [0] * 7 + \
-        [t.data_ptr() for t in out] + [vir.data_ptr(), pe.data_ptr()] + [0] * 9
     blk = ctypes.create_string_buffer(struct.pack("<iii" + "Q" * len(ptrs), n, 0, n, *ptrs))
-    torch.cuda.synchronize()
     for _ in range(3):
         assert lib.jse_nnap_computeGPUMD(blk) == 0, lib.jse_nnap_last_error()
     t0 = time.perf_counter()
     for _ in range(iters):
         assert lib.jse_nnap_computeGPUMD(blk) == 0, lib.jse_nnap_last_error()
     dt = (time.perf_counter() - t0) / iters
+    for p in bufs:
+        hip.hipFree(p)
     return {"ms_per_call": dt * 1e3, "atom_evals_per_s": n / dt, "pairs_listed": int(num.sum()),
             "note": "device-resident GPUMD neighbour list; rows grouped by species on each call"}
 

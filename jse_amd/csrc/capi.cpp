@@ -425,6 +425,8 @@ struct jse_nnap_system_s {
         const SpeciesHost &sp = model->host.species[t];
         const SpeciesDev &dv = model->dev[t];
         SegmentArgs a{};
+        a.self_t = t + 1;
+        a.mirror = model->host.species[t].mirror;
         a.nrows = seg[t + 1] - seg[t];
         a.rows_atom = d_rows + seg[t];
         a.nl_of = d_slot_of;  // neighbour rows sit in cell-sorted slot order

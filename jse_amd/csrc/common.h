@@ -56,6 +56,8 @@ struct SpeciesHost {
     int M() const { return (lmax + 1) * (lmax + 1); }
     int S() const { return P * (lmax + 1); }
     int mix = MIX_DENSE;              // kernel specialisation for W
+    int mirror = 0;                   // mirror basis: for this species' centres, neighbour species
+                                      // `mirror` and this species swap roles (0: none)
 };
 
 struct ModelHost {

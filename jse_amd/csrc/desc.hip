@@ -162,7 +162,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl
             const int j = c0 + lane;
             const int jr = j < n ? j : n - 1;
             const float4 dd = nl.d[base + jr];
-            const int tj = tag_species(dd.w);
+            const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
             const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
             float y[M];
             real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
@@ -321,7 +321,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
             const int j = c0 + lane;
             const int jr = j < n ? j : n - 1;
             const float4 dd = nl.d[base + jr];
-            const int tj = tag_species(dd.w);
+            const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
             const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
             float y[M];
             real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
@@ -495,9 +495,10 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
     struct RowRef {
         int gr, nlrow, n;
         float rc, inv_rc;
+        int self_t, mirror;
     };
     auto row_ref = [&](int gr) {
-        RowRef r{gr, 0, 0, 1.f, 1.f};
+        RowRef r{gr, 0, 0, 1.f, 1.f, 0, 0};
         if (gr >= rend) return r;
         int sg = 0;  // the row's segment (species)
         while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
@@ -507,6 +508,8 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
         r.n = min(nl.cnt[r.nlrow], nl.cap);
         r.rc = a.rc;
         r.inv_rc = 1.f / a.rc;
+        r.self_t = a.self_t;
+        r.mirror = a.mirror;
         return r;
     };
     auto next_row = [&](int gr) {  // the first row at or after gr with pairs (gr == rend: none)
@@ -588,6 +591,7 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
             const int j = inA ? kA + lane : lane - cntA;
             const int slot = inA ? slotA : slotB;
             const float rc = inA ? A.rc : B.rc, inv_rc = inA ? A.inv_rc : B.inv_rc;
+            const int self_t = inA ? A.self_t : B.self_t, mirror = inA ? A.mirror : B.mirror;
             const size_t base = (size_t)(inA ? A.nlrow : B.nlrow) * nl.cap;
             const float4 dd = on ? nl.d[base + j] : make_float4(1.f, 0.f, 0.f, 0.f);
             if (pf < rend) {  // issued after this chunk's displacements
@@ -596,7 +600,7 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
             }
             if (on) {
                 const float dx = dd.x, dy = dd.y, dz = dd.z;
-                const int tj = tag_species(dd.w);
+                const int tj = role_species(tag_species(dd.w), self_t, mirror);
                 const PairGeomR g = pair_geom_fast(dx, dy, dz);
                 float ox = 0.f, oy = 0.f, oz = 0.f;
                 if (g.r < rc) {

@@ -149,7 +149,14 @@ struct SegmentArgs {
     int wg_per_cu;          // persistent workgroups per CU (0: one workgroup per row)
     int variant;            // compiled schedule variant (autotuned; 0: default)
     int fp32_acc;           // 1: the matrix-core forward accumulates in FP32 (BF16-MLP precision mode)
+    int self_t;             // this segment's species (1-based)
+    int mirror;             // mirror basis: neighbours of species `mirror` and of self_t swap roles (0: none)
 };
+// A neighbour's species as the centre's basis sees it (MirrorBasis: the mirrored
+// species and the centre's own swap roles; nnap_main.hpp:40-47)
+__device__ __forceinline__ int role_species(int tj, int self_t, int mirror) {
+    return mirror == 0 ? tj : (tj == mirror ? self_t : (tj == self_t ? mirror : tj));
+}
 // returns false when no kernel is compiled for this (lmax, K, P, mix)
 bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);
 bool launch_desc_bwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);

@@ -113,6 +113,8 @@ void enqueue_hook_rows(hipStream_t s, jse_nnap_model model, int precision, const
         const SpeciesHost &sp = mh.species[t];
         const SpeciesDev &dv = model->dev[t];
         SegmentArgs sa{};
+        sa.self_t = t + 1;
+        sa.mirror = sp.mirror;
         sa.nrows = seg[t + 1] - seg[t];
         if (!sa.nrows) continue;
         sa.rows_atom = rows + seg[t];

@@ -169,8 +169,14 @@ ModelHost parse_model_json(const std::string &text) {
         if (!have_m && m.get("norm_mu_eng")) { mu_e = num(m.get("norm_mu_eng"), 0.0); have_m = true; }
     }
     ModelHost out;
-    for (auto &m : models->arr) {
-        SpeciesHost sp;
+    out.species.resize(ntypes);
+    // pass 1 (Basis.java:22-50): symbols and the species' own bases; a mirror or
+    // share entry only records the species whose basis it takes
+    std::vector<int> ref(ntypes, 0);
+    std::vector<char> is_mirror(ntypes, 0);
+    for (int i = 0; i < ntypes; ++i) {
+        const JVal &m = models->arr[i];
+        SpeciesHost &sp = out.species[i];
         const JVal *sym = m.get("symbol");
         JNN_CHECK(sym && sym->kind == JVal::STR, ERR_ARG, "No symbol in model");
         sp.symbol = sym->str;
@@ -180,8 +186,20 @@ ModelHost parse_model_json(const std::string &text) {
         const JVal *b = m.get("basis");
         if (!b) b = &dflt_basis;
         if (auto t = b->get("type")) {
-            JNN_CHECK(t->kind == JVal::STR && (t->str == "spherical_chebyshev" || t->str == "sph_cheby"), ERR_UNSUPPORTED,
-                      "Unsupported basis type: " + (t->kind == JVal::STR ? t->str : std::string("?")));
+            JNN_CHECK(t->kind == JVal::STR, ERR_UNSUPPORTED, "Unsupported basis type: ?");
+            const bool mir = t->str == "mirror" || t->str == "mirror_basis";
+            if (mir || t->str == "share" || t->str == "shared_basis") {
+                // MirrorBasis.java:61-66, SharedBasis.java:61-66: key `mirror` / `share`
+                const JVal *k = b->get(mir ? "mirror" : "share");
+                JNN_CHECK(k, ERR_ARG, std::string("Key `") + (mir ? "mirror" : "share") + "` required for " +
+                                          (mir ? "mirror_basis" : "shared_basis"));
+                ref[i] = (int)num(k, 0);
+                JNN_CHECK(ref[i] >= 1 && ref[i] <= ntypes, ERR_ARG, "mirror/share type out of range");
+                is_mirror[i] = mir;
+                continue;
+            }
+            JNN_CHECK(t->str == "spherical_chebyshev" || t->str == "sph_cheby", ERR_UNSUPPORTED,
+                      "Unsupported basis type: " + t->str);
         }
         sp.nmax = (int)num(b->get("nmax"), 5);
         sp.lmax = (int)num(b->get("lmax"), 6);
@@ -262,6 +280,26 @@ ModelHost parse_model_json(const std::string &text) {
                     }
                 }
         }
+    }
+    // pass 2 (Basis.java:51-66): mirror / share take the named species' basis
+    for (int i = 0; i < ntypes; ++i) {
+        if (!ref[i]) continue;
+        JNN_CHECK(!ref[ref[i] - 1], ERR_UNSUPPORTED, "a mirror/share basis of a mirror/share basis is not supported");
+        SpeciesHost &sp = out.species[i];
+        const SpeciesHost &src = out.species[ref[i] - 1];
+        sp.nmax = src.nmax;
+        sp.lmax = src.lmax;
+        sp.P = src.P;
+        sp.wtype = src.wtype;
+        sp.typed_wtype = src.typed_wtype;
+        sp.rcut = src.rcut;
+        sp.W = src.W;
+        sp.mirror = is_mirror[i] ? ref[i] : 0;
+    }
+    // pass 3: networks and normalisation
+    for (int i = 0; i < ntypes; ++i) {
+        const JVal &m = models->arr[i];
+        SpeciesHost &sp = out.species[i];
         const int S = sp.S();
         JVal dflt_nn;
         dflt_nn.kind = JVal::OBJ;
@@ -325,7 +363,6 @@ ModelHost parse_model_json(const std::string &text) {
         sp.norm = {mu_e + ref, sig_e};
         sp.norm.insert(sp.norm.end(), mu.begin(), mu.end());
         sp.norm.insert(sp.norm.end(), sg.begin(), sg.end());
-        out.species.push_back(std::move(sp));
     }
     finalize_species(out);
     return out;

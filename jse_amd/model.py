@@ -14,8 +14,10 @@ repository root):
   ``norm_mu_eng``/``norm_sigma_eng`` taken from the first species that has
   them (``NNAP.java:272-306``).
 
-Only ``spherical_chebyshev`` bases with ``l3max = l4max = 0`` are accepted,
-and every error is raised as ``ValueError`` with the reference's wording
+Bases are ``spherical_chebyshev`` with ``l3max = l4max = 0``, or ``mirror`` /
+``share`` entries that take another species' basis (``basis/MirrorBasis.java``,
+``basis/SharedBasis.java``; a mirror also swaps the roles of the two species
+among the neighbours of its centres).  Every error is raised as ``ValueError`` with the reference's wording
 where it has one (the reference raises ``IllegalArgumentException``).
 """
 from __future__ import annotations
@@ -49,6 +51,7 @@ class Species:
     hidden: List[int]
     nn_param: np.ndarray
     norm_param: np.ndarray
+    mirror: int = 0     # mirror basis: neighbour species `mirror` and this one swap roles (0: none)
 
     @property
     def K(self) -> int:
@@ -188,15 +191,28 @@ def load_model(src) -> Model:
             norm_mu_eng = m.get("norm_mu_eng")
     sig_e = 1.0 if norm_sigma_eng is None else float(norm_sigma_eng)
     mu_e = 0.0 if norm_mu_eng is None else float(norm_mu_eng)
-    out: List[Species] = []
-    for m in models:
-        sym = m.get("symbol")
-        if sym is None:
+    # pass 1 (Basis.java:22-50): the species' own bases; mirror / share entries
+    # only record the species whose basis they take
+    bases = [None] * ntypes
+    ref = [0] * ntypes
+    is_mirror = [False] * ntypes
+    for i, m in enumerate(models):
+        if m.get("symbol") is None:
             raise ValueError("No symbol in model")
         if m.get("torch") is not None:
             raise ValueError("torch model is invalid now.")
         b = m.get("basis") or {"type": "spherical_chebyshev"}
         btype = str(b.get("type", "spherical_chebyshev"))
+        if btype in ("mirror", "mirror_basis", "share", "shared_basis"):
+            mir = btype.startswith("mirror")
+            key = "mirror" if mir else "share"
+            if b.get(key) is None:  # MirrorBasis.java:61-66, SharedBasis.java:61-66
+                raise ValueError(f"Key `{key}` required for {'mirror_basis' if mir else 'shared_basis'}")
+            ref[i] = int(b[key])
+            if not 1 <= ref[i] <= ntypes:
+                raise ValueError("mirror/share type out of range")
+            is_mirror[i] = mir
+            continue
         if btype not in ("spherical_chebyshev", "sph_cheby"):
             raise ValueError(f"Unsupported basis type: {btype}")
         nmax = int(_get(b, "nmax", default=DEFAULT_NMAX))
@@ -221,6 +237,18 @@ def load_model(src) -> Model:
         if wtype == WTYPE_RFUSE and rfuse_w is None:
             raise ValueError("Input rfuse_weight MUST NOT be null when wtype=='rfuse'")
         typed, P, W = _mixing(wtype, ntypes, nmax, fuse_w, rfuse_w, rfuse_scale, ws)
+        bases[i] = (nmax, lmax, rcut, wtype, typed, P, W)
+    # pass 2 (Basis.java:51-66): mirror / share take the named species' basis
+    for i in range(ntypes):
+        if ref[i]:
+            if ref[ref[i] - 1]:
+                raise ValueError("a mirror/share basis of a mirror/share basis is not supported")
+            bases[i] = bases[ref[i] - 1]
+    # pass 3: networks and normalisation
+    out: List[Species] = []
+    for i, m in enumerate(models):
+        sym = m.get("symbol")
+        nmax, lmax, rcut, wtype, typed, P, W = bases[i]
         S = P * (lmax + 1)
         nn = m.get("nn") or {}
         if str(nn.get("type", "feed_forward")) != "feed_forward":
@@ -264,7 +292,8 @@ def load_model(src) -> Model:
         if mu.shape != (S,) or sg.shape != (S,):
             raise ValueError("norm_mu/norm_sigma size mismatch")
         norm = np.concatenate([[mu_e + ref_eng, sig_e], mu, sg])
-        out.append(Species(str(sym), nmax, lmax, rcut, wtype, typed, P, W, hidden, nn_param, norm))
+        out.append(Species(str(sym), nmax, lmax, rcut, wtype, typed, P, W, hidden, nn_param, norm,
+                           ref[i] if is_mirror[i] else 0))
     return Model(out, info.get("units"), info)
 
 

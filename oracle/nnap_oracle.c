@@ -54,6 +54,7 @@ typedef struct {
     const double *const *fp_param;
     const double *const *nn_param;
     const double *const *norm_param;
+    const int *mirror;        /* [ntypes] mirror basis: the mirrored species, 0 none (NULL: none at all) */
 } orc_model;
 
 /* ------------------------------------------------------------------ */
@@ -551,34 +552,54 @@ static void descriptor_backward(const sp_view *v, int lmax, const double *dx, co
     }
 }
 
+/* A mirror basis (MirrorBasis.java; nnap_main.hpp:40-47,73-79): for the
+ * centres of species ctype, neighbours of the mirrored species and of ctype
+ * swap roles while the basis is evaluated.  Returns the types to use (ty
+ * itself, or buf holding the swapped copy). */
+static const int *role_types(const orc_model *m, int ctype, const int *ty, int nn, int *buf) {
+    const int mir = m->mirror ? m->mirror[ctype - 1] : 0;
+    if (!mir) return ty;
+    for (int k = 0; k < nn; ++k) buf[k] = ty[k] == mir ? ctype : (ty[k] == ctype ? mir : ty[k]);
+    return buf;
+}
+
 /* jse_nnap_calFp restated (nnap_interface.cpp:11-24) */
 int orc_cal_fp(const orc_model *m, int ctype, const double *dx, const double *dy, const double *dz,
-               const int *ty, int nn, double *fp) {
+               const int *ty_in, int nn, double *fp) {
     if (ctype < 1 || ctype > m->ntypes) return 1;
+    int *buf = (int *)malloc(sizeof(int) * (size_t)(nn > 0 ? nn : 1));
+    const int *ty = role_types(m, ctype, ty_in, nn, buf);
     sp_view v = species(m, ctype);
     double A[ORC_MAX_P * ORC_MAX_M];
     descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
+    free(buf);
     return 0;
 }
 /* jse_nnap_calEnergy restated (nnap_interface.cpp:26-46) */
 int orc_cal_energy(const orc_model *m, int ctype, const double *dx, const double *dy, const double *dz,
-                   const int *ty, int nn, double *e) {
+                   const int *ty_in, int nn, double *e) {
     if (ctype < 1 || ctype > m->ntypes) return 1;
+    int *buf = (int *)malloc(sizeof(int) * (size_t)(nn > 0 ? nn : 1));
+    const int *ty = role_types(m, ctype, ty_in, nn, buf);
     sp_view v = species(m, ctype);
     double A[ORC_MAX_P * ORC_MAX_M], fp[ORC_MAX_P * (ORC_MAX_LMAX + 1)];
     descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
     *e = mlp(&v, fp, NULL);
+    free(buf);
     return 0;
 }
 /* jse_nnap_calEnergyForce restated (nnap_interface.cpp:48-89): g_j = -dE_i/d(r_j - r_i) */
 int orc_cal_energy_force(const orc_model *m, int ctype, const double *dx, const double *dy, const double *dz,
-                         const int *ty, int nn, double *e, double *gx, double *gy, double *gz) {
+                         const int *ty_in, int nn, double *e, double *gx, double *gy, double *gz) {
     if (ctype < 1 || ctype > m->ntypes) return 1;
+    int *buf = (int *)malloc(sizeof(int) * (size_t)(nn > 0 ? nn : 1));
+    const int *ty = role_types(m, ctype, ty_in, nn, buf);
     sp_view v = species(m, ctype);
     double A[ORC_MAX_P * ORC_MAX_M], fp[ORC_MAX_P * (ORC_MAX_LMAX + 1)], gfp[ORC_MAX_P * (ORC_MAX_LMAX + 1)];
     descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
     *e = mlp(&v, fp, gfp);
     descriptor_backward(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, gfp, gx, gy, gz);
+    free(buf);
     return 0;
 }
 /* the MLP alone on a given fp vector (for the MLP pins) */

@@ -30,6 +30,7 @@ class OrcModel(ctypes.Structure):
         ("fp_param", ctypes.POINTER(_c_dbl_p)),
         ("nn_param", ctypes.POINTER(_c_dbl_p)),
         ("norm_param", ctypes.POINTER(_c_dbl_p)),
+        ("mirror", _c_int_p),
     ]
 
 
@@ -85,8 +86,9 @@ class Oracle:
         self._Wp = (_c_dbl_p * nt)(*[_dp(a) for a in self._W])
         self._nnp = (_c_dbl_p * nt)(*[_dp(a) for a in self._nn])
         self._normp = (_c_dbl_p * nt)(*[_dp(a) for a in self._norm])
+        self._mirror = np.array([getattr(s, "mirror", 0) for s in sp], dtype=np.int32)
         self.c = OrcModel(nt, _ip(self._nmax), _ip(self._lmax), _ip(self._P), _dp(self._rc), _ip(self._nl),
-                          _ip(self._hid), self._Wp, self._nnp, self._normp)
+                          _ip(self._hid), self._Wp, self._nnp, self._normp, _ip(self._mirror))
 
     # --- per-atom entry points (jse_nnap_calFp / calEnergy / calEnergyForce) ---
     def cal_fp(self, ctype, d, tj):

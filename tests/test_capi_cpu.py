@@ -62,6 +62,30 @@ def test_cpp_parser_matches_python_loader(wtype):
     assert nt == 3 and P == [s.P for s in mdl.species] and S == [s.S for s in mdl.species]
 
 
+@pytest.mark.parametrize("kind", ["share", "mirror"])
+def test_cpp_parser_share_mirror(kind):
+    """mirror / share entries take another species' basis (Basis.java:22-67):
+    both loaders agree on the sizes, and a mirror or share of a mirror or share
+    is rejected by both."""
+    md = M.random_model(["Cu", "Zr", "Al"], 4, 5, 5.0, (8, 8), seed=3)
+    md["models"][0]["basis"]["rcut"] = 4.5
+    md["models"][2]["basis"] = {"type": kind, kind: 1}
+    rc, nt, P, S = _parse(json.dumps(md))
+    assert rc == 0, nnap.lib().jse_nnap_last_error()
+    mdl = M.load_model(md)
+    assert P == [s.P for s in mdl.species] and S == [s.S for s in mdl.species]
+    assert mdl.species[2].rcut == 4.5 and mdl.species[2].mirror == (1 if kind == "mirror" else 0)
+    md["models"][1]["basis"] = {"type": kind, kind: 3}
+    rc, _, _, _ = _parse(json.dumps(md))
+    assert rc != 0
+    with pytest.raises(ValueError):
+        M.load_model(md)
+    md["models"][1]["basis"] = {"type": kind}
+    assert _parse(json.dumps(md))[0] != 0
+    with pytest.raises(ValueError, match="required"):
+        M.load_model(md)
+
+
 def test_parser_errors():
     L = nnap.lib()
     rc, *_ = _parse('{"models": [{"basis": {}}]}')

@@ -1,7 +1,8 @@
 """GPU parity of the other compiled descriptor shapes against the FP64 oracle
 (through the C-ABI): dense radial mixing (a two-species model with wtype
 "full", P = 2K, each species' channels in its own block) and lmax = 4 with
-nmax = 5 (one species: no mixing; three species: the default mixing).
+nmax = 5 (one species: no mixing; three species: the default mixing), and
+mirror / share bases.
 Plain FP32 bar: 1e-6 eV/atom, 1e-5 eV/A RMS forces, 1e-5 eV/atom virial."""
 import numpy as np
 import pytest
@@ -54,4 +55,20 @@ def test_lmax4_nmax5(O, symbols):
     st = _liquid(symbols, 1500, 22)
     md = M.random_model(symbols, nmax=5, lmax=4, rcut=6.0, hidden=(32, 32), seed=32)
     md = W._normalized(md, st, 300, 32)
+    _check(O, md, st)
+
+
+def test_mirror_and_share_bases(O):
+    """mirror / share bases (MirrorBasis.java, SharedBasis.java): species 1
+    with its own rcut, species 2 mirroring it (neighbours of species 1 and 2
+    swap roles for species-2 centres), species 3 sharing it; each keeps its own
+    network and normalisation."""
+    st = _liquid(["Cu", "Zr", "Al"], 1500, 23)
+    md = M.random_model(["Cu", "Zr", "Al"], nmax=8, lmax=6, rcut=6.0, hidden=(64, 64), seed=33)
+    md["models"][0]["basis"]["rcut"] = 5.5
+    md["models"][1]["basis"] = {"type": "mirror", "mirror": 1}
+    md["models"][2]["basis"] = {"type": "share", "share": 1}
+    mdl = M.load_model(md)
+    assert [s.rcut for s in mdl.species] == [5.5, 5.5, 5.5] and [s.mirror for s in mdl.species] == [0, 1, 0]
+    md = W._normalized(md, st, 300, 33)
     _check(O, md, st)

@@ -306,3 +306,55 @@ def test_oracle_matches_committed_fixtures(O, name):
     assert np.allclose(res["energies"], g["energies"], rtol=0, atol=1e-12)
     assert np.allclose(res["forces"], g["forces"], rtol=0, atol=1e-11)
     assert np.allclose(res["virials"], g["virials"], rtol=0, atol=1e-11)
+
+
+def _share_mirror_model(kind):
+    """Ternary model whose species 1 has its own rcut (5.5 A) and whose species 2
+    takes species 1's basis as `kind` ("share" or "mirror")."""
+    md = M.random_model(["Cu", "Zr", "Al"], nmax=4, lmax=3, rcut=6.0, hidden=(8, 8), seed=41)
+    md["models"][0]["basis"]["rcut"] = 5.5
+    md["models"][1]["basis"] = {"type": kind, kind: 1}
+    return md
+
+
+def test_share_basis_equals_copied_basis(O):
+    """SharedBasis: a species with `share: s` evaluates species s's basis with
+    its own network and normalisation, exactly as an explicit copy of that
+    basis in its own entry."""
+    st = W.ternary_liquid(n=300, seed=9)
+    md = _share_mirror_model("share")
+    mc = json.loads(json.dumps(md))
+    mc["models"][1]["basis"] = dict(mc["models"][0]["basis"])
+    a = O.Oracle(M.load_model(md)).system_efv(st)
+    b = O.Oracle(M.load_model(mc)).system_efv(st)
+    assert M.load_model(md).species[1].rcut == 5.5
+    for k in ("energies", "forces", "virials"):
+        assert np.array_equal(a[k], b[k])
+
+
+def test_mirror_basis_is_a_species_swap_symmetry(O):
+    """MirrorBasis: for centres of species 2 with `mirror: 1`, neighbours of
+    species 1 and 2 swap roles in species 1's basis.  With species 2 also
+    carrying species 1's network and normalisation, relabelling every atom
+    1 <-> 2 leaves each atom's energy and the forces unchanged (an Ising-like
+    symmetry; two species, so no third species' basis sees the relabelling).
+    The same model with `share` instead (no swap) breaks it, so the test sees
+    a missing or misplaced swap."""
+    liq = W.ternary_liquid(n=300, seed=10)
+    st = W.Structure(liq.box, liq.xyz, (((liq.types - 1) % 2) + 1).astype(np.int32))  # two species
+    st2 = W.Structure(st.box, st.xyz, (3 - st.types).astype(np.int32))
+    for kind, symmetric in (("mirror", True), ("share", False)):
+        md = M.random_model(["Cu", "Zr"], nmax=4, lmax=3, rcut=6.0, hidden=(8, 8), seed=42)
+        md["models"][0]["basis"]["rcut"] = 5.5
+        md["models"][1]["basis"] = {"type": kind, kind: 1}
+        md["models"][1]["nn"] = md["models"][0]["nn"]
+        for key in ("norm_mu", "norm_sigma", "ref_eng"):
+            if key in md["models"][0]:
+                md["models"][1][key] = md["models"][0][key]
+        orc = O.Oracle(M.load_model(md))
+        a, b = orc.system_efv(st), orc.system_efv(st2)
+        if symmetric:
+            assert np.allclose(a["energies"], b["energies"], rtol=0, atol=1e-12)
+            assert np.allclose(a["forces"], b["forces"], rtol=0, atol=1e-11)
+        else:
+            assert np.abs(a["energies"] - b["energies"]).max() > 1e-6
