@@ -67,7 +67,8 @@ constexpr int kGatherSlots = 4;  // pairs per lane and pass: 16 lanes x 4 = 64 p
 __global__ __launch_bounds__(64 * kGatherMaxWaves) void k_gather(int n, const int *__restrict__ atoms,
                                                                  NeighborList nl, double *__restrict__ F,
                                                                  double *__restrict__ V, int want_virial,
-                                                                 const int *__restrict__ out_of) {
+                                                                 const int *__restrict__ out_of,
+                                                                 int *__restrict__ unmatched) {
     constexpr int U = kGatherSlots;
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int grp = lane >> 4, lid = lane & 15;
@@ -140,8 +141,10 @@ __global__ __launch_bounds__(64 * kGatherMaxWaves) void k_gather(int n, const in
         }
         float4 gi[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+        for (int u = 0; u < U; ++u) {
             gi[u] = res[u] >= 0 ? nl.g[(size_t)jr[u] * cap + res[u]] : make_float4(0.f, 0.f, 0.f, 0.f);
+            if (unmatched && on[u] && res[u] < 0) atomicAdd(unmatched, 1);  // j's row lacks the pair
+        }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (res[u] < 0) continue;
@@ -250,12 +253,12 @@ __global__ __launch_bounds__(kRedBlock) void k_rep_totals(const int4 *__restrict
 }
 
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
-                   int want_virial, int waves, const int *out_of) {
+                   int want_virial, int waves, const int *out_of, int *unmatched) {
     if (n <= 0) return;
     const int w = (waves >= 1 && waves <= kGatherMaxWaves) ? waves : 4;
     const int atoms_per_wg = 4 * w;  // four atoms per wavefront
     hipLaunchKernelGGL(k_gather, dim3(xcd_grid((n + atoms_per_wg - 1) / atoms_per_wg)), dim3(64 * w), 0, s, n, atoms,
-                       nl, F, V, want_virial, out_of);
+                       nl, F, V, want_virial, out_of, unmatched);
 }
 void launch_replica_totals(hipStream_t s, int nchunk, const int4 *chunks, int nrep, const int *rep_chunk_off,
                            const double *E, const double *V, int want_virial, double *partial, double *out,

@@ -10,6 +10,14 @@
 // with the LAMMPS hook, the model is named once (jse_nnap_gpumd_bind) and the
 // reference's per-call parameter blocks, merge tables and scratch buffers are
 // accepted and not read.
+//
+// The reference scatters every pair's force and virial with FP64 atomics.
+// Rows are sorted by key, so results do not depend on the order of GPUMD's
+// list.  When every particle is a centre and the list is symmetric (each pair
+// listed from both ends, once), the plain path's atomic-free force gather
+// produces per-atom results that are then added to GPUMD's arrays; a pair
+// without its reverse, or a neighbour listed twice, sends the call to the
+// atomic scatter instead.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -25,7 +33,11 @@ using namespace jnn;
 namespace jnn {
 void launch_gpumd_types(hipStream_t s, int n, const int *g_type, int *out);
 void launch_gpumd_rows(hipStream_t s, int N, int N1, int N2, const int *num, const int *list, const float *nl_dx,
-                       const float *nl_dy, const float *nl_dz, float rc2, const NeighborList &nl, int *overflow);
+                       const float *nl_dy, const float *nl_dz, float rc2, int tag_rows, const NeighborList &nl,
+                       int *overflow);
+void launch_gpumd_sort(hipStream_t s, int nrow, const NeighborList &nl, int *key2, float4 *d2, int *flag);
+void launch_gpumd_add(hipStream_t s, int N, const double *F, const double *V, const double *E, double *fx, double *fy,
+                      double *fz, double *vir, double *pot);
 void launch_gpumd_scatter(hipStream_t s, int N, int N1, int nrow, const NeighborList &nl, const double *E, double *fx,
                           double *fy, double *fz, double *vir, double *pot);
 }  // namespace jnn
@@ -52,23 +64,30 @@ struct GpumdContext {
     int cap = 64;
     size_t n_alloc = 0, row_alloc = 0, row_n_alloc = 0;
     int *types = nullptr, *rows = nullptr, *overflow = nullptr, *h_overflow = nullptr;
+    int *flag = nullptr, *h_flag = nullptr;  // unmatched reverse pairs or repeated neighbours
     float *xs = nullptr, *A = nullptr, *G = nullptr;
-    double *E = nullptr;
+    double *E = nullptr, *F = nullptr, *V = nullptr;
+    int *key2 = nullptr;
+    float4 *d2 = nullptr;
     NeighborList nl{};
 
     void ensure(int n, int nrow, int cap_) {
         if (!stream) JNN_HIP(hipStreamCreateWithFlags(&stream, hipStreamNonBlocking));
         if (!h_overflow) h_overflow = halloc<int>(1);
         if (!overflow) overflow = dalloc<int>(1);
+        if (!h_flag) h_flag = halloc<int>(1);
+        if (!flag) flag = dalloc<int>(1);
         if ((size_t)n > n_alloc) {
-            dfree(types);
+            dfree(types); dfree(F); dfree(V);
             n_alloc = (size_t)n + n / 4 + 64;
             types = dalloc<int>(n_alloc);
+            F = dalloc<double>(3 * n_alloc);
+            V = dalloc<double>(9 * n_alloc);
         }
         const size_t need = (size_t)nrow * cap_;
         if ((size_t)nrow > row_n_alloc || need > row_alloc) {
             dfree(rows); dfree(xs); dfree(A); dfree(G); dfree(E);
-            dfree(nl.cnt); dfree(nl.key); dfree(nl.d); dfree(nl.g);
+            dfree(nl.cnt); dfree(nl.key); dfree(nl.d); dfree(nl.g); dfree(key2); dfree(d2);
             row_n_alloc = std::max(row_n_alloc, (size_t)nrow + nrow / 4 + 64);
             row_alloc = row_n_alloc * cap_;
             rows = dalloc<int>(row_n_alloc);
@@ -80,6 +99,8 @@ struct GpumdContext {
             nl.key = dalloc<int>(row_alloc);
             nl.d = dalloc<float4>(row_alloc);
             nl.g = dalloc<float4>(row_alloc);
+            key2 = dalloc<int>(row_alloc);
+            d2 = dalloc<float4>(row_alloc);
         }
         nl.cap = cap_;
     }
@@ -160,21 +181,38 @@ int jse_nnap_computeGPUMD(void *args) {
         }
         hipStream_t s = c.stream;
         const float rc = (float)mh.rcut_max();
+        const bool all = N1 == 0 && N2 == N;  // every particle a centre: rows by particle, gather possible
         for (int attempt = 0;; ++attempt) {
             c.ensure(N, nrow, c.cap);
             JNN_HIP(hipMemcpyAsync(c.rows, rows.data(), sizeof(int) * nrow, hipMemcpyHostToDevice, s));
             launch_gpumd_types(s, N, g_type, c.types);
             launch_fill_int(s, 1, c.overflow, 0);
+            launch_fill_int(s, 1, c.flag, 0);
             c.nl.types = c.types;
-            launch_gpumd_rows(s, N, N1, N2, num, list, nl_dx, nl_dy, nl_dz, rc * rc, c.nl, c.overflow);
+            launch_gpumd_rows(s, N, N1, N2, num, list, nl_dx, nl_dy, nl_dz, rc * rc, all ? 1 : 0, c.nl, c.overflow);
             JNN_HIP(hipMemcpyAsync(c.h_overflow, c.overflow, sizeof(int), hipMemcpyDeviceToHost, s));
             JNN_HIP(hipStreamSynchronize(s));
             if (*c.h_overflow <= c.nl.cap) break;
             JNN_CHECK(attempt < 2 && *c.h_overflow <= 4096, ERR_CAPACITY, "neighbour row capacity exceeded");
             c.cap = ((*c.h_overflow + 31) / 32) * 32;
         }
+        // rows sorted by key: the results do not depend on the order of GPUMD's list,
+        // and the gather's reverse search needs it
+        launch_gpumd_sort(s, nrow, c.nl, c.key2, c.d2, c.flag);
+        std::swap(c.nl.key, c.key2);
+        std::swap(c.nl.d, c.d2);
         enqueue_hook_rows(s, c.model, c.precision, seg.data(), c.rows, c.types, c.xs, c.A, c.G, c.E, c.nl);
-        launch_gpumd_scatter(s, N, N1, nrow, c.nl, c.E, g_fx, g_fy, g_fz, g_virial, g_potential);
+        bool gathered = false;
+        if (all) {
+            launch_gather(s, N, nullptr, c.nl, c.F, c.V, 1, 0, nullptr, c.flag);
+            JNN_HIP(hipMemcpyAsync(c.h_flag, c.flag, sizeof(int), hipMemcpyDeviceToHost, s));
+            JNN_HIP(hipStreamSynchronize(s));
+            if (*c.h_flag == 0) {
+                launch_gpumd_add(s, N, c.F, c.V, c.E, g_fx, g_fy, g_fz, g_virial, g_potential);
+                gathered = true;
+            }
+        }
+        if (!gathered) launch_gpumd_scatter(s, N, N1, nrow, c.nl, c.E, g_fx, g_fy, g_fz, g_virial, g_potential);
         JNN_HIP(hipGetLastError());
         JNN_HIP(hipStreamSynchronize(s));
     });

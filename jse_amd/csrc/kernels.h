@@ -209,8 +209,9 @@ bool launch_mlp_mfma_multi(hipStream_t s, const MlpArgs *a, int nsp, bool bf16);
 // Row t of the list belongs to atom out_of[t] (nullptr: t); atom j's row is
 // the row in the pair tag of d.w.  F and V are written by atom.  The rows'
 // capacity must be a multiple of 4 (keys are read 16 bytes at a time).
+// `unmatched` (optional) counts pairs whose reverse pair j's row lacks.
 void launch_gather(hipStream_t s, int n, const int *atoms, const NeighborList &nl, double *F, double *V,
-                   int want_virial, int waves = 0, const int *out_of = nullptr);
+                   int want_virial, int waves = 0, const int *out_of = nullptr, int *unmatched = nullptr);
 // Largest chunk the totals kernel sums in one workgroup.
 constexpr int kTotalsChunkMax = 1024;
 // Per-replica totals [nrep][10] = (E, V[9]): fixed-size chunks of each

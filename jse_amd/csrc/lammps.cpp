@@ -109,14 +109,16 @@ void enqueue_hook_rows(hipStream_t s, jse_nnap_model model, int precision, const
                        const int *types, float *xs, float *A, float *G, double *E, const NeighborList &nl) {
     const ModelHost &mh = model->host;
     const int nt = mh.ntypes();
+    std::vector<SegmentArgs> segs(nt);
+    std::vector<MlpArgs> mlps;
     for (int t = 0; t < nt; ++t) {
         const SpeciesHost &sp = mh.species[t];
         const SpeciesDev &dv = model->dev[t];
-        SegmentArgs sa{};
+        SegmentArgs &sa = segs[t];
+        sa = SegmentArgs{};
         sa.self_t = t + 1;
         sa.mirror = sp.mirror;
         sa.nrows = seg[t + 1] - seg[t];
-        if (!sa.nrows) continue;
         sa.rows_atom = rows + seg[t];
         sa.types = types;
         sa.rc = (float)sp.rcut;
@@ -130,12 +132,16 @@ void enqueue_hook_rows(hipStream_t s, jse_nnap_model model, int precision, const
         sa.x = xs + (size_t)seg[t] * model->smax;
         sa.A = A + (size_t)seg[t] * model->pmmax;
         sa.G = G + (size_t)seg[t] * model->smax;
-        if (!launch_desc_fwd(s, sp.lmax, sp.K(), sp.P, sp.mix, sa, nl))
-            throw Error(ERR_UNSUPPORTED, "no descriptor kernel compiled for this species shape");
-        enqueue_mlp(s, model, t, precision, sa.nrows, sa.rows_atom, sa.x, sa.G, E, true);
-        if (!launch_desc_bwd(s, sp.lmax, sp.K(), sp.P, sp.mix, sa, nl))
-            throw Error(ERR_UNSUPPORTED, "no descriptor backward kernel compiled for this species shape");
+        mlps.push_back(mlp_args(model, t, sa.nrows, sa.rows_atom, sa.x, sa.G, E, true));
     }
+    // one launch per kernel for all species of one descriptor shape, persistent
+    // workgroups as the autotuner picks them for a 32,000-atom liquid (the hooks
+    // are not tuned per call)
+    for (auto &sa : segs) sa.wg_per_cu = 32;
+    enqueue_desc(s, model, segs.data(), false, nl);
+    enqueue_mlp_all(s, model, precision, mlps.data(), nt);
+    for (auto &sa : segs) sa.wg_per_cu = 16;
+    enqueue_desc(s, model, segs.data(), true, nl);
 }
 }  // namespace jnn
 

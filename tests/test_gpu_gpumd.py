@@ -7,8 +7,11 @@ device, column-major [jj * N + i], with FP32 minimum-image displacements
 r_j - r_i (here within rc + 0.5 A, in a shuffled order per atom) and 0-based
 types.  The hook adds forces, per-atom virials (on the neighbour) and
 per-atom energies into GPUMD's FP64 arrays; they must meet the FP32 bar
-against the FP64 oracle and agree with the plain path.  Two calls over the
-centre ranges [0, N/2) and [N/2, N) must add up to one call over [0, N)."""
+against the FP64 oracle and agree with the plain path.  A call over all
+particles with a symmetric list goes through the atomic-free gather, calls
+over part of the range through the reference's atomic scatter: two calls
+over [0, N/2) and [N/2, N) must add up to one call over [0, N), and a list
+with a missing reverse pair must fall back to the scatter."""
 import ctypes
 
 import numpy as np
@@ -60,11 +63,17 @@ def call(lib, N, N1, N2, dev, out):
     assert rc_ == 0, lib.jse_nnap_last_error()
 
 
-def run_hook(pot, st, precision, ranges, seed=0):
+def run_hook(pot, st, precision, ranges, seed=0, drop=None):
+    """drop = (i, k): remove entry k of atom i's list (its pair then lacks the reverse)"""
     lib = nnap.lib()
     for name in ("jse_nnap_gpumd_bind", "jse_nnap_computeGPUMD"):
         getattr(lib, name).argtypes = [ctypes.c_void_p]
     num, nbr, d = gpumd_list(st, pot.rcutMax() + 0.5, seed)
+    if drop is not None:
+        i, k = drop
+        last = num[i] - 1
+        nbr[k, i], d[:, k, i] = nbr[last, i], d[:, last, i]
+        num[i] -= 1
     n = st.natoms
     dev = {"num": DevBuf(num), "nbr": DevBuf(nbr.ravel()), "dx": DevBuf(d[0].ravel()), "dy": DevBuf(d[1].ravel()),
            "dz": DevBuf(d[2].ravel()), "type": DevBuf((st.types - 1).astype(np.int32))}
@@ -99,6 +108,23 @@ def test_gpumd_hook_matches_oracle_and_plain_path(O):
     # the plain path (FP64 displacements, gathered forces)
     assert np.abs(got["forces"] - plain["forces"]).max() <= 1e-4
     assert np.abs(got["virials"] - plain["virials"]).max() <= 1e-4
-    # split centre ranges add up to the whole (FP64 atomics: order-level differences only)
+    # split centre ranges (the atomic scatter) add up to the whole (the gather):
+    # summation-order differences only
     for k in ("energies", "forces", "virials"):
         assert np.abs(split[k] - got[k]).max() <= 1e-9
+
+
+def test_gpumd_hook_asymmetric_list_falls_back_to_scatter():
+    """A list in which one pair lacks its reverse cannot go through the
+    gather: the whole-range call must detect it and scatter, matching the
+    split-range calls (which always scatter) on the same list."""
+    st = W.ternary_liquid(n=1000, seed=6)
+    md = W.ternary_model()
+    n = st.natoms
+    with nnap.NNAP(md) as pot:
+        whole = run_hook(pot, st, nnap.FP32, [(0, n)], seed=4, drop=(17, 0))
+        split = run_hook(pot, st, nnap.FP32, [(0, n // 3), (n // 3, n)], seed=4, drop=(17, 0))
+        sym = run_hook(pot, st, nnap.FP32, [(0, n)], seed=4)
+    for k in ("energies", "forces", "virials"):
+        assert np.abs(whole[k] - split[k]).max() <= 1e-9
+    assert np.abs(whole["forces"] - sym["forces"]).max() > 1e-6  # the dropped pair mattered
