@@ -25,9 +25,11 @@ HIP_FLAGS = [
     # packed FP32 VALU ops are kept out of every kernel
     "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
 ]
-# per-source additions: the descriptor forward's matrix-core accumulators live in
-# ordinary VGPRs, so its FP64 totals read them without AGPR-to-VGPR copies
-FILE_FLAGS = {"desc.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
+# per-source additions: the matrix-core accumulators of the descriptor forward
+# and the MLP live in ordinary VGPRs, so the code that reads them (FP64 totals,
+# SiLU epilogues) needs no AGPR-to-VGPR copies
+FILE_FLAGS = {"desc.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"],
+              "mlp_mfma.hip": ["-mllvm", "-amdgpu-mfma-vgpr-form"]}
 
 
 def _run(cmd):
