@@ -231,17 +231,73 @@ ModelHost parse_model_json(const std::string &text) {
             throw Error(ERR_ARG, "Input fuse_weight MUST NOT be null when wtype=='fuse' or 'exfuse'");
         if (wtype == WT_RFUSE && rfuse_w.empty())
             throw Error(ERR_ARG, "Input rfuse_weight MUST NOT be null when wtype=='rfuse'");
-        int typed = wtype;
-        if (ntypes == 1 && (wtype == WT_EXFULL || wtype == WT_FULL || wtype == WT_NONE || wtype == WT_DEFAULT))
-            typed = WT_NONE;
+        const int K = sp.K();
+        // base mixing W[t][q][n] of a non-rfuse wtype (sizeN rows), WTypeBasis.java:202-304
+        auto base_mixing = [&](int typed_, int sn_, int fuse_) {
+            std::vector<double> Wb((size_t)ntypes * sn_ * K, 0.0);
+            auto at = [&](int t, int q, int n) -> double & { return Wb[((size_t)(t - 1) * sn_ + q) * K + n]; };
+            for (int t = 1; t <= ntypes; ++t)
+                for (int n = 0; n < K; ++n) {
+                    switch (typed_) {
+                        case WT_NONE: at(t, n, n) = 1.0; break;
+                        case WT_DEFAULT: at(t, n, n) = 1.0; at(t, K + n, n) = (t & 1) ? (double)t : -(double)t; break;
+                        case WT_FULL: at(t, (t - 1) * K + n, n) = 1.0; break;
+                        case WT_EXFULL: at(t, n, n) = 1.0; at(t, t * K + n, n) = 1.0; break;
+                        case WT_FUSE:
+                            for (int k = 0; k < fuse_; ++k) at(t, k * K + n, n) = fuse_w[(size_t)(t - 1) * fuse_ + k];
+                            break;
+                        case WT_EXFUSE:
+                            at(t, n, n) = 1.0;
+                            for (int k = 0; k < fuse_; ++k) at(t, (k + 1) * K + n, n) = fuse_w[(size_t)(t - 1) * fuse_ + k];
+                            break;
+                        default: throw Error(ERR_ARG, "unsupported wtype");
+                    }
+                }
+            return Wb;
+        };
+        auto typed_of = [&](int wt_) {
+            return (ntypes == 1 && (wt_ == WT_EXFULL || wt_ == WT_FULL || wt_ == WT_NONE || wt_ == WT_DEFAULT)) ? WT_NONE
+                                                                                                            : wt_;
+        };
+        // an older file's post-fuse weight becomes the rfuse weight it equals, rfuse[t] = post . W_base[t]
+        // (SphericalChebyshev.java:81-94; WTypeBasis.getPostFuseWeight_ / postFuse2RFuse_, :391-540)
+        const bool post_keys = b->get("post_fuse_size") || b->get("post_fuse_weight");
+        if (wtype == WT_RFUSE) {
+            JNN_CHECK(!post_keys, ERR_ARG, "Key `post_fuse_size` or `post_fuse_weight` is invalid for wtype=='rfuse'");
+        } else {
+            const JVal *pf = b->get("post_fuse");
+            const bool off = pf && pf->kind == JVal::BOOL && !pf->b;
+            if (!off && (pf || post_keys)) {
+                const std::vector<double> post = vec(b->get("post_fuse_weight"));
+                JNN_CHECK(!post.empty(), ERR_ARG,
+                          "Key `post_fuse_weight` always required for post_fuse, training with post_fuse is invalid now");
+                const int fz = (wtype == WT_FUSE || wtype == WT_EXFUSE) ? (int)fuse_w.size() / ntypes : 0;
+                const int sn0 = size_n(wtype, ntypes, sp.nmax, fz);
+                if (auto ps = b->get("post_fuse_size"))
+                    JNN_CHECK(post.size() == (size_t)num(ps, 0) * sn0, ERR_ARG, "Size of post fuse weight mismatch");
+                const int pp = (int)post.size() / sn0;
+                const std::vector<double> Wb = base_mixing(typed_of(wtype), sn0, fz);
+                rfuse_w.assign((size_t)ntypes * pp * K, 0.0);
+                for (int t = 0; t < ntypes; ++t)
+                    for (int p = 0; p < pp; ++p)
+                        for (int q = 0; q < sn0; ++q) {
+                            const double w = post[(size_t)p * sn0 + q];
+                            if (w == 0.0) continue;
+                            for (int n = 0; n < K; ++n)
+                                rfuse_w[((size_t)t * pp + p) * K + n] += w * Wb[((size_t)t * sn0 + q) * K + n];
+                        }
+                rfuse_scale = num(b->get("post_fuse_scale"), 1.0);
+                wtype = WT_RFUSE;
+                sp.wtype = wtype;
+            }
+        }
+        int typed = typed_of(wtype);
         sp.typed_wtype = typed;
         const int fuse = (wtype == WT_FUSE || wtype == WT_EXFUSE) ? (int)fuse_w.size() / ntypes : 0;
         const int sn = size_n(wtype, ntypes, sp.nmax, fuse);
-        const int K = sp.K();
         sp.P = rfuse_w.empty() ? sn : (int)rfuse_w.size() / sn;
         JNN_CHECK(sp.P >= 1 && sp.P <= 128, ERR_ARG, "basis size_np out of range");
         sp.W.assign((size_t)ntypes * sp.P * K, 0.0);
-        auto Wat = [&](int t, int p, int n) -> double & { return sp.W[((size_t)(t - 1) * sp.P + p) * K + n]; };
         if (!rfuse_w.empty()) {
             JNN_CHECK(rfuse_w.size() == (size_t)sp.P * sn, ERR_ARG, "Size of rfuse weight mismatch");
             std::vector<double> rw = rfuse_w;
@@ -257,28 +313,7 @@ ModelHost parse_model_json(const std::string &text) {
             }
             for (size_t q = 0; q < rw.size(); ++q) sp.W[q] = rfuse_scale * rw[q];
         } else {
-            for (int t = 1; t <= ntypes; ++t)
-                for (int n = 0; n < K; ++n) {
-                    switch (typed) {
-                        case WT_NONE: Wat(t, n, n) = 1.0; break;
-                        case WT_DEFAULT: {
-                            double wt = (t & 1) ? (double)t : -(double)t;
-                            Wat(t, n, n) = 1.0;
-                            Wat(t, K + n, n) = wt;
-                            break;
-                        }
-                        case WT_FULL: Wat(t, (t - 1) * K + n, n) = 1.0; break;
-                        case WT_EXFULL: Wat(t, n, n) = 1.0; Wat(t, t * K + n, n) = 1.0; break;
-                        case WT_FUSE:
-                            for (int k = 0; k < fuse; ++k) Wat(t, k * K + n, n) = fuse_w[(size_t)(t - 1) * fuse + k];
-                            break;
-                        case WT_EXFUSE:
-                            Wat(t, n, n) = 1.0;
-                            for (int k = 0; k < fuse; ++k) Wat(t, (k + 1) * K + n, n) = fuse_w[(size_t)(t - 1) * fuse + k];
-                            break;
-                        default: throw Error(ERR_ARG, "unsupported wtype");
-                    }
-                }
+            sp.W = base_mixing(typed, sn, fuse);
         }
     }
     // pass 2 (Basis.java:51-66): mirror / share take the named species' basis

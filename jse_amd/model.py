@@ -160,6 +160,33 @@ def _mixing(wtype: int, ntypes: int, nmax: int, fuse_w, rfuse_w, rfuse_scale: fl
     return typed, P, W
 
 
+def _post_fuse(b, wtype: int, ntypes: int, nmax: int, fuse_w):
+    """The rfuse weight [ntypes][P][K] (flat) equal to a basis' post-fuse weight
+    (WTypeBasis.getPostFuseWeight_ and postFuse2RFuse_, WTypeBasis.java:
+    391-540): post[P][sizeN] applied after the base mixing, so
+    rfuse[t] = post . W_base[t].  None without post-fuse keys."""
+    has = any(b.get(k) is not None for k in ("post_fuse", "post_fuse_size", "post_fuse_weight"))
+    if wtype == WTYPE_RFUSE:
+        if b.get("post_fuse_size") is not None or b.get("post_fuse_weight") is not None:
+            raise ValueError("Key `post_fuse_size` or `post_fuse_weight` is invalid for wtype=='rfuse'")
+        return None
+    flag = b.get("post_fuse")
+    if (flag is not None and not flag) or not has:
+        return None
+    if b.get("post_fuse_weight") is None:
+        raise ValueError("Key `post_fuse_weight` always required for post_fuse, training with post_fuse is invalid now")
+    post = np.asarray(b["post_fuse_weight"], dtype=float)
+    if post.ndim != 1:
+        raise ValueError("post_fuse_weight must be a flat list")
+    fuse_size = 0 if wtype not in (WTYPE_FUSE, WTYPE_EXFUSE) else len(fuse_w) // ntypes
+    sn = _size_n(wtype, ntypes, nmax, fuse_size)
+    if b.get("post_fuse_size") is not None and len(post) != int(b["post_fuse_size"]) * sn:
+        raise ValueError("Size of post fuse weight mismatch")
+    _, _, Wb = _mixing(wtype, ntypes, nmax, fuse_w, None, 1.0, False)   # [ntypes][sizeN][K]
+    P = len(post) // sn
+    return np.einsum("pq,tqn->tpn", post.reshape(P, sn), Wb).ravel()
+
+
 def load_model(src) -> Model:
     """Parse a model (path to .json/.yaml/.yml, JSON text, or a dict) as
     ``new NNAP(path)`` does (NNAP.java:129-307, 402-414)."""
@@ -236,6 +263,12 @@ def load_model(src) -> Model:
             raise ValueError("Input fuse_weight MUST NOT be null when wtype=='fuse' or 'exfuse'")
         if wtype == WTYPE_RFUSE and rfuse_w is None:
             raise ValueError("Input rfuse_weight MUST NOT be null when wtype=='rfuse'")
+        post = _post_fuse(b, wtype, ntypes, nmax, fuse_w)
+        if post is not None:
+            # an older file's post-fuse weight becomes the rfuse weight it equals
+            # (SphericalChebyshev.java:81-94, WTypeBasis.postFuse2RFuse_)
+            rfuse_w, wtype = post, WTYPE_RFUSE
+            rfuse_scale = float(b.get("post_fuse_scale", 1.0) or 1.0)
         typed, P, W = _mixing(wtype, ntypes, nmax, fuse_w, rfuse_w, rfuse_scale, ws)
         bases[i] = (nmax, lmax, rcut, wtype, typed, P, W)
     # pass 2 (Basis.java:51-66): mirror / share take the named species' basis

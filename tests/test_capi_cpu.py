@@ -86,6 +86,30 @@ def test_cpp_parser_share_mirror(kind):
         M.load_model(md)
 
 
+def test_cpp_parser_post_fuse():
+    """A post-fuse weight (P rows over the base mixing's sizeN) gives P mixed
+    channels in both loaders; with wtype rfuse its keys are rejected."""
+    md = M.random_model(["Cu", "Zr", "Al"], 4, 5, 5.0, (8, 8), seed=4)
+    sn = M.load_model(md).species[0].P
+    P = 7
+    rng = np.random.default_rng(5)
+    for m in md["models"]:
+        m["basis"]["post_fuse_weight"] = rng.normal(size=P * sn).tolist()
+        m["basis"]["post_fuse_scale"] = 0.5
+        m["nn"]["input_dim"] = P * 6
+        m["nn"]["hidden_weights"][0] = rng.normal(size=(8, P * 6)).tolist()
+        m["norm_mu"] = [0.0] * (P * 6)
+        m["norm_sigma"] = [1.0] * (P * 6)
+    rc, nt, Ps, S = _parse(json.dumps(md))
+    assert rc == 0, nnap.lib().jse_nnap_last_error()
+    mdl = M.load_model(md)
+    assert Ps == [P] * 3 == [s.P for s in mdl.species] and S == [s.S for s in mdl.species]
+    md["models"][0]["basis"]["wtype"] = "rfuse"
+    assert _parse(json.dumps(md))[0] != 0
+    with pytest.raises(ValueError):
+        M.load_model(md)
+
+
 def test_parser_errors():
     L = nnap.lib()
     rc, *_ = _parse('{"models": [{"basis": {}}]}')

@@ -1,8 +1,8 @@
 """GPU parity of the other compiled descriptor shapes against the FP64 oracle
 (through the C-ABI): dense radial mixing (a two-species model with wtype
 "full", P = 2K, each species' channels in its own block) and lmax = 4 with
-nmax = 5 (one species: no mixing; three species: the default mixing), and
-mirror / share bases.
+nmax = 5 (one species: no mixing; three species: the default mixing),
+mirror / share bases and a post-fuse weight.
 Plain FP32 bar: 1e-6 eV/atom, 1e-5 eV/A RMS forces, 1e-5 eV/atom virial."""
 import numpy as np
 import pytest
@@ -71,4 +71,20 @@ def test_mirror_and_share_bases(O):
     mdl = M.load_model(md)
     assert [s.rcut for s in mdl.species] == [5.5, 5.5, 5.5] and [s.mirror for s in mdl.species] == [0, 1, 0]
     md = W._normalized(md, st, 300, 33)
+    _check(O, md, st)
+
+
+def test_post_fuse_basis(O):
+    """An older file's post-fuse weight (18 x 18 over the default mixing):
+    the C++ loader's rfuse weight, through the dense-mixing kernels, against
+    the oracle built from the Python loader's."""
+    st = _liquid(["Cu", "Zr", "Al"], 1500, 24)
+    md = M.random_model(["Cu", "Zr", "Al"], nmax=8, lmax=6, rcut=6.0, hidden=(64, 64), seed=34)
+    rng = np.random.default_rng(34)
+    for m in md["models"]:
+        m["basis"]["post_fuse_weight"] = (rng.normal(size=18 * 18) / 4).tolist()
+        m["basis"]["post_fuse_scale"] = 0.75
+    mdl = M.load_model(md)
+    assert [s.P for s in mdl.species] == [18] * 3 and mdl.species[0].wtype == M.WTYPE_RFUSE
+    md = W._normalized(md, st, 300, 34)
     _check(O, md, st)

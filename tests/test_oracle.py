@@ -358,3 +358,35 @@ def test_mirror_basis_is_a_species_swap_symmetry(O):
             assert np.allclose(a["forces"], b["forces"], rtol=0, atol=1e-11)
         else:
             assert np.abs(a["energies"] - b["energies"]).max() > 1e-6
+
+
+def test_post_fuse_is_applied_after_the_base_mixing(O):
+    """An older file's post-fuse weight post[P][sizeN] acts after the base
+    mixing (WTypeBasis.postFuse2RFuse_): Rnp' = post . Rnp.  With post a
+    permutation (a 3-cycle among others, so post != post^T) and the network's
+    input columns and normalisation permuted to match, every energy equals the
+    plain model's; a transposed application would not."""
+    st = W.ternary_liquid(n=300, seed=12)
+    md = M.random_model(["Cu", "Zr", "Al"], nmax=4, lmax=3, rcut=6.0, hidden=(8, 8), seed=43)
+    md = W._normalized(md, st, 100, 43)
+    sp = M.load_model(md).species[0]
+    P, L1 = sp.P, sp.lmax + 1
+    perm = np.roll(np.arange(P), 1)
+    perm[[2, 3, 4]] = perm[[3, 4, 2]]
+    post = np.zeros((P, P))
+    post[np.arange(P), perm] = 1.0          # Rnp'[p] = Rnp[perm[p]]
+    assert not np.array_equal(post, post.T)
+    cols = (perm[:, None] * L1 + np.arange(L1)[None, :]).ravel()  # x'[p L1 + l] = x[perm[p] L1 + l]
+    mp = json.loads(json.dumps(md))
+    for m in mp["models"]:
+        m["basis"]["post_fuse_weight"] = post.ravel().tolist()
+        m["basis"]["post_fuse_size"] = P
+        w = np.asarray(m["nn"]["hidden_weights"][0])
+        m["nn"]["hidden_weights"][0] = w[:, cols].tolist()
+        m["norm_mu"] = np.asarray(m["norm_mu"])[cols].tolist()
+        m["norm_sigma"] = np.asarray(m["norm_sigma"])[cols].tolist()
+    assert M.load_model(mp).species[0].wtype == M.WTYPE_RFUSE
+    a = O.Oracle(M.load_model(md)).system_efv(st)
+    b = O.Oracle(M.load_model(mp)).system_efv(st)
+    assert np.abs(a["energies"] - b["energies"]).max() <= 1e-12
+    assert np.abs(a["forces"] - b["forces"]).max() <= 1e-11
