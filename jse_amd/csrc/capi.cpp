@@ -101,6 +101,7 @@ MlpArgs mlp_args(const jse_nnap_model_s *m, int t, int nrows, const int *rows_at
     a.nrows = nrows;
     a.S = sp.S();
     a.L1 = sp.lmax + 1;
+    a.radial = sp.radial ? 1 : 0;
     a.nlayers = (int)sp.hidden.size();
     for (int l = 0; l < a.nlayers; ++l) a.hidden[l] = sp.hidden[l];
     a.rows_atom = rows_atom;
@@ -132,7 +133,7 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
     const int nt = m->host.ntypes();
     auto same_shape = [&](int t, int u) {
         const SpeciesHost &a = m->host.species[t], &b = m->host.species[u];
-        return a.lmax == b.lmax && a.K() == b.K() && a.P == b.P && a.mix == b.mix;
+        return a.radial == b.radial && a.lmax == b.lmax && a.K() == b.K() && a.P == b.P && a.mix == b.mix;
     };
     int first = -1;
     bool one = nt <= kSegMax;
@@ -160,16 +161,18 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
         }
         set.row0[set.nseg] = rows;
         const SpeciesHost &sp = m->host.species[first];
-        const bool ok = backward ? launch_desc_bwd_set(s, sp.lmax, sp.K(), sp.P, sp.mix, set, nl)
-                                 : launch_desc_fwd_set(s, sp.lmax, sp.K(), sp.P, sp.mix, set, nl);
+        const int lm = sp.radial ? kRadialLmax : sp.lmax;
+        const bool ok = backward ? launch_desc_bwd_set(s, lm, sp.K(), sp.P, sp.mix, set, nl)
+                                 : launch_desc_fwd_set(s, lm, sp.K(), sp.P, sp.mix, set, nl);
         if (!ok) fail(first);
         return;
     }
     for (int t = 0; t < nt; ++t) {
         if (segs[t].nrows <= 0) continue;
         const SpeciesHost &sp = m->host.species[t];
-        const bool ok = backward ? launch_desc_bwd(s, sp.lmax, sp.K(), sp.P, sp.mix, segs[t], nl)
-                                 : launch_desc_fwd(s, sp.lmax, sp.K(), sp.P, sp.mix, segs[t], nl);
+        const int lm = sp.radial ? kRadialLmax : sp.lmax;
+        const bool ok = backward ? launch_desc_bwd(s, lm, sp.K(), sp.P, sp.mix, segs[t], nl)
+                                 : launch_desc_fwd(s, lm, sp.K(), sp.P, sp.mix, segs[t], nl);
         if (!ok) fail(t);
     }
 }

@@ -58,6 +58,7 @@ struct SpeciesHost {
     int mix = MIX_DENSE;              // kernel specialisation for W
     int mirror = 0;                   // mirror basis: for this species' centres, neighbour species
                                       // `mirror` and this species swap roles (0: none)
+    bool radial = false;              // radial-only chebyshev basis: fp[p] = sum_j fc Rnp[p] (lmax 0 sizes)
 };
 
 struct ModelHost {

@@ -707,6 +707,179 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
     }
 }
 
+// ---- radial-only chebyshev basis (Chebyshev.java; basis_Chebyshev.hpp:9-80) ----
+// fp[p] = sum_j fc(r) Rnp[p], Rnp = W[type_j] Rn: no angular part, so the
+// descriptor rows have S = P and no A.  One wavefront per atom: each lane sums
+// its neighbours' fc Rnp[p] in FP64 and the lanes are combined in a fixed
+// order (DPP row rotations, then readlane across the four rows).
+
+__device__ __forceinline__ double dpp_row_ror(double v, int n) {
+    const long long b = __double_as_longlong(v);
+    int lo = (int)(b & 0xffffffffll), hi = (int)(b >> 32);
+    switch (n) {  // row_ror:n is DPP control 0x120 + n
+        case 8: lo = __builtin_amdgcn_update_dpp(0, lo, 0x128, 0xf, 0xf, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x128, 0xf, 0xf, false); break;
+        case 4: lo = __builtin_amdgcn_update_dpp(0, lo, 0x124, 0xf, 0xf, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x124, 0xf, 0xf, false); break;
+        case 2: lo = __builtin_amdgcn_update_dpp(0, lo, 0x122, 0xf, 0xf, false);
+                hi = __builtin_amdgcn_update_dpp(0, hi, 0x122, 0xf, 0xf, false); break;
+        default: lo = __builtin_amdgcn_update_dpp(0, lo, 0x121, 0xf, 0xf, false);
+                 hi = __builtin_amdgcn_update_dpp(0, hi, 0x121, 0xf, 0xf, false); break;
+    }
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+__device__ __forceinline__ double readlane_f64(double v, int l) {
+    const long long b = __double_as_longlong(v);
+    const int lo = __builtin_amdgcn_readlane((int)(b & 0xffffffffll), l);
+    const int hi = __builtin_amdgcn_readlane((int)(b >> 32), l);
+    return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+// the wave's sum, the same in every lane
+__device__ __forceinline__ double wave_sum_f64(double v) {
+    v += dpp_row_ror(v, 8);
+    v += dpp_row_ror(v, 4);
+    v += dpp_row_ror(v, 2);
+    v += dpp_row_ror(v, 1);
+    return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
+}
+
+template <int K, int P, int MIX>
+__global__ __launch_bounds__(64) void k_rad_fwd(SegmentSet set, NeighborList nl) {
+    const int lane = threadIdx.x;
+    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+        int sg = 0;  // the row's segment (species)
+        while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
+        const SegmentArgs &a = set.a[sg];
+        const int row = gr - set.row0[sg];
+        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        double acc[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) acc[p] = 0.0;
+        for (int j = lane; j < n; j += 64) {
+            const float4 dd = nl.d[base + j];
+            const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
+            const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+            if (g.r < rc) {
+                float rn[K];
+                cheb_rn<K>(1.f - 2.f * g.r * inv_rc, rn);
+                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+                const float t2 = t * t;
+                const float fc = t2 * t2;
+                float rnp[P];
+                mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
+#pragma unroll
+                for (int p = 0; p < P; ++p) acc[p] += (double)(fc * rnp[p]);
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < P; ++p) {
+            const double fp = wave_sum_f64(acc[p]);
+            if (lane == p) a.x[(size_t)row * P + p] = (float)((fp - a.mu64[p]) * (1.0 / a.sig64[p]));
+        }
+    }
+}
+
+// Lane per pair: with G[p] = dE/dfp[p] of the atom (the MLP's gradient),
+// dE/dfc = sum_p G[p] Rnp[p] and dE/dRn[n] = fc sum_p G[p] W[t][p][n], so
+// g_j = -(sum_n dE/dRn[n] dRn/dr + dE/dfc dfc/dr) d / r.
+template <int K, int P, int MIX>
+__global__ __launch_bounds__(64) void k_rad_bwd(SegmentSet set, NeighborList nl) {
+    const int lane = threadIdx.x;
+    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+        int sg = 0;
+        while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
+        const SegmentArgs &a = set.a[sg];
+        const int row = gr - set.row0[sg];
+        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        float gp[P];
+#pragma unroll
+        for (int p = 0; p < P; ++p) gp[p] = a.G[(size_t)row * P + p];
+        for (int j = lane; j < n; j += 64) {
+            const float4 dd = nl.d[base + j];
+            const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
+            const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+            float ox = 0.f, oy = 0.f, oz = 0.f;
+            if (g.r < rc) {
+                const float xr = 1.f - 2.f * g.r * inv_rc;
+                float rn[K], drn[K];
+                cheb_rn<K>(xr, rn);
+                cheb_drn<K>(xr, inv_rc, drn);
+                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+                const float t2 = t * t;
+                const float fc = t2 * t2;
+                float rnp[P];
+                mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
+                float gfc = 0.f;
+#pragma unroll
+                for (int p = 0; p < P; ++p) gfc += gp[p] * rnp[p];
+                // gw[n] = sum_p G[p] W[t][p][n]
+                float gw[K];
+                if (MIX == MIX_NONE) {
+#pragma unroll
+                    for (int q = 0; q < K; ++q) gw[q] = gp[q];
+                } else if (MIX == MIX_DEFAULT) {
+                    const float wt = (tj & 1) ? (float)tj : -(float)tj;
+#pragma unroll
+                    for (int q = 0; q < K; ++q) gw[q] = gp[q] + wt * gp[K + q];
+                } else {
+                    const float *w = a.W + (size_t)(tj - 1) * P * K;
+#pragma unroll
+                    for (int q = 0; q < K; ++q) {
+                        float sacc = 0.f;
+#pragma unroll
+                        for (int p = 0; p < P; ++p) sacc += gp[p] * w[p * K + q];
+                        gw[q] = sacc;
+                    }
+                }
+                float dEdr = 0.f;
+#pragma unroll
+                for (int q = 0; q < K; ++q) dEdr += gw[q] * drn[q];
+                dEdr *= fc;
+                dEdr += gfc * (-8.f * g.r * inv_rc * inv_rc * t2 * t);
+                ox = -dEdr * dd.x * g.ir;
+                oy = -dEdr * dd.y * g.ir;
+                oz = -dEdr * dd.z * g.ir;
+            }
+            nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
+        }
+    }
+}
+
+template <int K, int P, int MIX>
+static void run_rad(hipStream_t s, const SegmentSet &set, const NeighborList &nl, bool backward) {
+    const SegmentArgs &a = set.a[0];
+    const int rows = set.row0[set.nseg];
+    int grid = rows < 262144 ? rows : 262144;
+    if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
+    if (backward) hipLaunchKernelGGL((k_rad_bwd<K, P, MIX>), dim3(grid), dim3(64), 0, s, set, nl);
+    else hipLaunchKernelGGL((k_rad_fwd<K, P, MIX>), dim3(grid), dim3(64), 0, s, set, nl);
+}
+#define JNN_RAD_SHAPES(X) \
+    X(6, 6, MIX_NONE)      \
+    X(6, 12, MIX_DEFAULT)  \
+    X(6, 12, MIX_DENSE)    \
+    X(9, 9, MIX_NONE)      \
+    X(9, 18, MIX_DEFAULT)  \
+    X(9, 18, MIX_DENSE)
+static bool launch_rad_set(hipStream_t s, int K, int P, int mix, const SegmentSet &set, const NeighborList &nl,
+                           bool backward) {
+    if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
+#define X(K_, P_, M_)                                  \
+    if (K == K_ && P == P_ && mix == M_) {            \
+        run_rad<K_, P_, M_>(s, set, nl, backward);    \
+        return true;                                   \
+    }
+    JNN_RAD_SHAPES(X)
+#undef X
+    return false;
+}
+
 // ---- dispatch table over the compiled shapes ----
 template <int LMAX, int K, int P, int MIX>
 static void run_fwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl) {
@@ -755,6 +928,7 @@ static void run_bwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl
 bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
                          const NeighborList &nl) {
     if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
+    if (lmax == kRadialLmax) return launch_rad_set(s, K, P, mix, set, nl, false);
 #define X(L_, K_, P_, M_)                                              \
     if (lmax == L_ && K == K_ && P == P_ && mix == M_) {              \
         run_fwd<L_, K_, P_, M_>(s, set, nl);                          \
@@ -767,6 +941,7 @@ bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const S
 bool launch_desc_bwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
                          const NeighborList &nl) {
     if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
+    if (lmax == kRadialLmax) return launch_rad_set(s, K, P, mix, set, nl, true);
 #define X(L_, K_, P_, M_)                                              \
     if (lmax == L_ && K == K_ && P == P_ && mix == M_) {              \
         run_bwd<L_, K_, P_, M_>(s, set, nl);                          \

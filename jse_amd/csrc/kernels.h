@@ -157,7 +157,10 @@ struct SegmentArgs {
 __device__ __forceinline__ int role_species(int tj, int self_t, int mirror) {
     return mirror == 0 ? tj : (tj == mirror ? self_t : (tj == self_t ? mirror : tj));
 }
-// returns false when no kernel is compiled for this (lmax, K, P, mix)
+// returns false when no kernel is compiled for this (lmax, K, P, mix); lmax =
+// kRadialLmax selects the radial-only chebyshev basis (fp[p] = sum_j fc Rnp[p],
+// rows of S = P and no A)
+constexpr int kRadialLmax = -1;
 bool launch_desc_fwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);
 bool launch_desc_bwd(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl);
 // Several species segments of one descriptor shape in one launch (no tail per
@@ -188,6 +191,7 @@ struct MlpArgs {
     float *G;                // [nrows][S]
     int want_grad;
     int wg_per_cu;           // persistent grid size (0: 2 per CU)
+    int radial;              // radial-only basis: G = dE/dfp / sigma (no L2 power-spectrum factor)
 };
 void launch_mlp_fp32(hipStream_t s, const MlpArgs &a);
 // MFMA kernels for 2 hidden layers of supported widths; false when the shape has no compiled kernel

@@ -136,8 +136,9 @@ __global__ __launch_bounds__(256) void k_mlp_fp32(MlpArgs a) {
                 const int r = idx / S, e = idx % S;
                 if (r < nt) {
                     const int l = e % a.L1;
-                    a.G[(size_t)(row0 + r) * S + e] =
-                        smem[e * kMlpTile + r] / a.sig[e] * (2.f * kPi4f / (float)(2 * l + 1));
+                    // times 2 4pi/(2l+1) for the L2 power spectrum's backward (not for a radial basis)
+                    const float l2 = a.radial ? 1.f : 2.f * kPi4f / (float)(2 * l + 1);
+                    a.G[(size_t)(row0 + r) * S + e] = smem[e * kMlpTile + r] / a.sig[e] * l2;
                 }
             }
         }

@@ -224,7 +224,9 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
 #pragma unroll
     for (int cs = 0; cs < NS; ++cs) {
         const int n = 16 * cs + lr;
-        gscale[cs] = n < S ? (2.f * kPi4m / (float)(2 * (n % a.L1) + 1)) / a.sig[n] : 0.f;
+        // G = dE/dx / sigma, times 2 4pi/(2l+1) for the L2 power spectrum's backward
+        const float l2 = a.radial ? 1.f : 2.f * kPi4m / (float)(2 * (n % a.L1) + 1);
+        gscale[cs] = n < S ? l2 / a.sig[n] : 0.f;
     }
     for (int tile = lb; tile * 64 < a.nrows; tile += lg) {
         const int row0 = tile * 64;

@@ -198,14 +198,16 @@ ModelHost parse_model_json(const std::string &text) {
                 is_mirror[i] = mir;
                 continue;
             }
-            JNN_CHECK(t->str == "spherical_chebyshev" || t->str == "sph_cheby", ERR_UNSUPPORTED,
+            // Chebyshev.java: the radial-only basis (no angular part)
+            sp.radial = t->str == "chebyshev" || t->str == "cheby";
+            JNN_CHECK(sp.radial || t->str == "spherical_chebyshev" || t->str == "sph_cheby", ERR_UNSUPPORTED,
                       "Unsupported basis type: " + t->str);
         }
         sp.nmax = (int)num(b->get("nmax"), 5);
-        sp.lmax = (int)num(b->get("lmax"), 6);
+        sp.lmax = sp.radial ? 0 : (int)num(b->get("lmax"), 6);
         JNN_CHECK(sp.lmax >= 0 && sp.lmax <= 12, ERR_ARG, "Input lmax MUST be in [0, 12], input: " + std::to_string(sp.lmax));
         JNN_CHECK(sp.nmax >= 0 && sp.nmax < 32, ERR_ARG, "Input nmax out of range");
-        JNN_CHECK((int)num(b->get("l3max"), 0) == 0 && (int)num(b->get("l4max"), 0) == 0, ERR_UNSUPPORTED,
+        JNN_CHECK(sp.radial || ((int)num(b->get("l3max"), 0) == 0 && (int)num(b->get("l4max"), 0) == 0), ERR_UNSUPPORTED,
                   "l3max/l4max > 0 are not supported by this build");
         sp.rcut = num(b->get("rcut"), 6.0);
         JNN_CHECK(sp.rcut > 0, ERR_ARG, "rcut must be positive");
@@ -329,6 +331,7 @@ ModelHost parse_model_json(const std::string &text) {
         sp.typed_wtype = src.typed_wtype;
         sp.rcut = src.rcut;
         sp.W = src.W;
+        sp.radial = src.radial;
         sp.mirror = is_mirror[i] ? ref[i] : 0;
     }
     // pass 3: networks and normalisation

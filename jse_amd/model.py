@@ -14,7 +14,8 @@ repository root):
   ``norm_mu_eng``/``norm_sigma_eng`` taken from the first species that has
   them (``NNAP.java:272-306``).
 
-Bases are ``spherical_chebyshev`` with ``l3max = l4max = 0``, or ``mirror`` /
+Bases are ``spherical_chebyshev`` with ``l3max = l4max = 0``, the radial-only
+``chebyshev`` (``basis/Chebyshev.java``; descriptor sizes as lmax = 0), or ``mirror`` /
 ``share`` entries that take another species' basis (``basis/MirrorBasis.java``,
 ``basis/SharedBasis.java``; a mirror also swaps the roles of the two species
 among the neighbours of its centres).  Every error is raised as ``ValueError`` with the reference's wording
@@ -52,6 +53,7 @@ class Species:
     nn_param: np.ndarray
     norm_param: np.ndarray
     mirror: int = 0     # mirror basis: neighbour species `mirror` and this one swap roles (0: none)
+    radial: bool = False  # radial-only chebyshev basis: fp[p] = sum_j fc Rnp[p] (sizes as lmax = 0)
 
     @property
     def K(self) -> int:
@@ -240,12 +242,13 @@ def load_model(src) -> Model:
                 raise ValueError("mirror/share type out of range")
             is_mirror[i] = mir
             continue
-        if btype not in ("spherical_chebyshev", "sph_cheby"):
+        radial = btype in ("chebyshev", "cheby")  # Chebyshev.java: no angular part
+        if not radial and btype not in ("spherical_chebyshev", "sph_cheby"):
             raise ValueError(f"Unsupported basis type: {btype}")
         nmax = int(_get(b, "nmax", default=DEFAULT_NMAX))
-        lmax = int(_get(b, "lmax", default=DEFAULT_LMAX))
-        l3 = int(_get(b, "l3max", default=0))
-        l4 = int(_get(b, "l4max", default=0))
+        lmax = 0 if radial else int(_get(b, "lmax", default=DEFAULT_LMAX))
+        l3 = 0 if radial else int(_get(b, "l3max", default=0))
+        l4 = 0 if radial else int(_get(b, "l4max", default=0))
         if lmax < 0 or lmax > 12:
             raise ValueError(f"Input lmax MUST be in [0, 12], input: {lmax}")
         if l3 != 0 or l4 != 0:
@@ -270,7 +273,7 @@ def load_model(src) -> Model:
             rfuse_w, wtype = post, WTYPE_RFUSE
             rfuse_scale = float(b.get("post_fuse_scale", 1.0) or 1.0)
         typed, P, W = _mixing(wtype, ntypes, nmax, fuse_w, rfuse_w, rfuse_scale, ws)
-        bases[i] = (nmax, lmax, rcut, wtype, typed, P, W)
+        bases[i] = (nmax, lmax, rcut, wtype, typed, P, W, radial)
     # pass 2 (Basis.java:51-66): mirror / share take the named species' basis
     for i in range(ntypes):
         if ref[i]:
@@ -281,7 +284,7 @@ def load_model(src) -> Model:
     out: List[Species] = []
     for i, m in enumerate(models):
         sym = m.get("symbol")
-        nmax, lmax, rcut, wtype, typed, P, W = bases[i]
+        nmax, lmax, rcut, wtype, typed, P, W, radial = bases[i]
         S = P * (lmax + 1)
         nn = m.get("nn") or {}
         if str(nn.get("type", "feed_forward")) != "feed_forward":
@@ -326,7 +329,7 @@ def load_model(src) -> Model:
             raise ValueError("norm_mu/norm_sigma size mismatch")
         norm = np.concatenate([[mu_e + ref_eng, sig_e], mu, sg])
         out.append(Species(str(sym), nmax, lmax, rcut, wtype, typed, P, W, hidden, nn_param, norm,
-                           ref[i] if is_mirror[i] else 0))
+                           ref[i] if is_mirror[i] else 0, radial))
     return Model(out, info.get("units"), info)
 
 

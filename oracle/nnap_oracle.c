@@ -55,6 +55,7 @@ typedef struct {
     const double *const *nn_param;
     const double *const *norm_param;
     const int *mirror;        /* [ntypes] mirror basis: the mirrored species, 0 none (NULL: none at all) */
+    const int *radial;        /* [ntypes] 1: radial-only chebyshev basis (lmax 0 sizes), NULL: none */
 } orc_model;
 
 /* ------------------------------------------------------------------ */
@@ -373,7 +374,7 @@ int orc_real_sph(int lmax, double dx, double dy, double dz, double *y, double *y
 /* Per-atom model pieces                                               */
 /* ------------------------------------------------------------------ */
 typedef struct {
-    int K, P, M, L1, S, nl;
+    int K, P, M, L1, S, nl, radial;
     double rc;
     const double *W;
     const int *h;
@@ -394,7 +395,56 @@ static sp_view species(const orc_model *m, int ctype) {
     v.h = m->hidden + t * ORC_MAX_LAYERS;
     v.nn = m->nn_param[t];
     v.norm = m->norm_param[t];
+    v.radial = m->radial ? m->radial[t] : 0;
     return v;
+}
+
+/* Radial-only chebyshev basis (basis_Chebyshev.hpp:9-33, Chebyshev.java):
+ * fp[p] = sum_j fc * Rnp[p], Rnp = W[type_j] Rn. */
+static void descriptor_radial(const sp_view *v, const double *dx, const double *dy, const double *dz,
+                              const int *ty, int nn, double *fp) {
+    double rn[ORC_MAX_K];
+    for (int p = 0; p < v->P; ++p) fp[p] = 0.0;
+    for (int j = 0; j < nn; ++j) {
+        double r = sqrt(dx[j] * dx[j] + dy[j] * dy[j] + dz[j] * dz[j]);
+        if (r >= v->rc) continue;
+        cal_rn(v->K - 1, r, v->rc, rn);
+        double fc = cal_fc(r, v->rc);
+        const double *W = v->W + (size_t)(ty[j] - 1) * v->P * v->K;
+        for (int p = 0; p < v->P; ++p) {
+            double rnp = 0.0;
+            for (int q = 0; q < v->K; ++q) rnp += rn[q] * W[p * v->K + q];
+            fp[p] += fc * rnp;
+        }
+    }
+}
+/* its backward (basis_Chebyshev.hpp:35-80): g_j = -dE_i/d(dx_j) */
+static void descriptor_radial_backward(const sp_view *v, const double *dx, const double *dy, const double *dz,
+                                       const int *ty, int nn, const double *gfp, double *gx, double *gy,
+                                       double *gz) {
+    double rn[ORC_MAX_K], rng[ORC_MAX_K], grn[ORC_MAX_K];
+    for (int j = 0; j < nn; ++j) {
+        gx[j] = gy[j] = gz[j] = 0.0;
+        double r = sqrt(dx[j] * dx[j] + dy[j] * dy[j] + dz[j] * dz[j]);
+        if (r >= v->rc) continue;
+        cal_rn(v->K - 1, r, v->rc, rn);
+        double fc = cal_fc(r, v->rc);
+        const double *W = v->W + (size_t)(ty[j] - 1) * v->P * v->K;
+        double gfc = 0.0;
+        for (int q = 0; q < v->K; ++q) grn[q] = 0.0;
+        for (int p = 0; p < v->P; ++p) {
+            double rnp = 0.0;
+            for (int q = 0; q < v->K; ++q) rnp += rn[q] * W[p * v->K + q];
+            gfc += gfp[p] * rnp;
+            for (int q = 0; q < v->K; ++q) grn[q] += gfp[p] * fc * W[p * v->K + q];
+        }
+        cal_rn_grad(v->K - 1, r, v->rc, rng);
+        double gr = gfc * cal_fc_grad(r, v->rc);
+        for (int q = 0; q < v->K; ++q) gr += grn[q] * rng[q];
+        gx[j] = gr * dx[j];
+        gy[j] = gr * dy[j];
+        gz[j] = gr * dz[j];
+    }
 }
 
 /* A[p][lm] = sum_j fc*Rnp[p]*Y[lm], Rnp = W[type_j] Rn;  fp[p*(lmax+1)+l] = 4pi/(2l+1) sum_m A^2 */
@@ -571,7 +621,8 @@ int orc_cal_fp(const orc_model *m, int ctype, const double *dx, const double *dy
     const int *ty = role_types(m, ctype, ty_in, nn, buf);
     sp_view v = species(m, ctype);
     double A[ORC_MAX_P * ORC_MAX_M];
-    descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
+    if (v.radial) descriptor_radial(&v, dx, dy, dz, ty, nn, fp);
+    else descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
     free(buf);
     return 0;
 }
@@ -583,7 +634,8 @@ int orc_cal_energy(const orc_model *m, int ctype, const double *dx, const double
     const int *ty = role_types(m, ctype, ty_in, nn, buf);
     sp_view v = species(m, ctype);
     double A[ORC_MAX_P * ORC_MAX_M], fp[ORC_MAX_P * (ORC_MAX_LMAX + 1)];
-    descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
+    if (v.radial) descriptor_radial(&v, dx, dy, dz, ty, nn, fp);
+    else descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
     *e = mlp(&v, fp, NULL);
     free(buf);
     return 0;
@@ -596,9 +648,15 @@ int orc_cal_energy_force(const orc_model *m, int ctype, const double *dx, const 
     const int *ty = role_types(m, ctype, ty_in, nn, buf);
     sp_view v = species(m, ctype);
     double A[ORC_MAX_P * ORC_MAX_M], fp[ORC_MAX_P * (ORC_MAX_LMAX + 1)], gfp[ORC_MAX_P * (ORC_MAX_LMAX + 1)];
-    descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
-    *e = mlp(&v, fp, gfp);
-    descriptor_backward(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, gfp, gx, gy, gz);
+    if (v.radial) {
+        descriptor_radial(&v, dx, dy, dz, ty, nn, fp);
+        *e = mlp(&v, fp, gfp);
+        descriptor_radial_backward(&v, dx, dy, dz, ty, nn, gfp, gx, gy, gz);
+    } else {
+        descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
+        *e = mlp(&v, fp, gfp);
+        descriptor_backward(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, gfp, gx, gy, gz);
+    }
     free(buf);
     return 0;
 }

@@ -31,6 +31,7 @@ class OrcModel(ctypes.Structure):
         ("nn_param", ctypes.POINTER(_c_dbl_p)),
         ("norm_param", ctypes.POINTER(_c_dbl_p)),
         ("mirror", _c_int_p),
+        ("radial", _c_int_p),
     ]
 
 
@@ -87,8 +88,9 @@ class Oracle:
         self._nnp = (_c_dbl_p * nt)(*[_dp(a) for a in self._nn])
         self._normp = (_c_dbl_p * nt)(*[_dp(a) for a in self._norm])
         self._mirror = np.array([getattr(s, "mirror", 0) for s in sp], dtype=np.int32)
+        self._radial = np.array([1 if getattr(s, "radial", False) else 0 for s in sp], dtype=np.int32)
         self.c = OrcModel(nt, _ip(self._nmax), _ip(self._lmax), _ip(self._P), _dp(self._rc), _ip(self._nl),
-                          _ip(self._hid), self._Wp, self._nnp, self._normp, _ip(self._mirror))
+                          _ip(self._hid), self._Wp, self._nnp, self._normp, _ip(self._mirror), _ip(self._radial))
 
     # --- per-atom entry points (jse_nnap_calFp / calEnergy / calEnergyForce) ---
     def cal_fp(self, ctype, d, tj):

@@ -110,6 +110,18 @@ def test_cpp_parser_post_fuse():
         M.load_model(md)
 
 
+def test_cpp_parser_radial_basis():
+    """The radial-only `chebyshev` basis (no lmax; S = P) in both loaders."""
+    md = M.random_model(["Cu", "Zr", "Al"], 4, 0, 5.0, (8, 8), seed=6)
+    for m in md["models"]:
+        b = m["basis"]
+        m["basis"] = {"type": "chebyshev", "nmax": b["nmax"], "rcut": b["rcut"], "wtype": b["wtype"]}
+    rc, nt, P, S = _parse(json.dumps(md))
+    assert rc == 0, nnap.lib().jse_nnap_last_error()
+    mdl = M.load_model(md)
+    assert P == S == [s.P for s in mdl.species] == [10] * 3 and all(s.radial for s in mdl.species)
+
+
 def test_parser_errors():
     L = nnap.lib()
     rc, *_ = _parse('{"models": [{"basis": {}}]}')

@@ -2,7 +2,8 @@
 (through the C-ABI): dense radial mixing (a two-species model with wtype
 "full", P = 2K, each species' channels in its own block) and lmax = 4 with
 nmax = 5 (one species: no mixing; three species: the default mixing),
-mirror / share bases and a post-fuse weight.
+mirror / share bases, a post-fuse weight and the radial-only chebyshev
+basis.
 Plain FP32 bar: 1e-6 eV/atom, 1e-5 eV/A RMS forces, 1e-5 eV/atom virial."""
 import numpy as np
 import pytest
@@ -87,4 +88,18 @@ def test_post_fuse_basis(O):
     mdl = M.load_model(md)
     assert [s.P for s in mdl.species] == [18] * 3 and mdl.species[0].wtype == M.WTYPE_RFUSE
     md = W._normalized(md, st, 300, 34)
+    _check(O, md, st)
+
+
+@pytest.mark.parametrize("symbols,nmax", [(["Cu", "Zr", "Al"], 8), (["Cu"], 5)])
+def test_radial_chebyshev_basis(O, symbols, nmax):
+    """The radial-only `chebyshev` basis (Chebyshev.java): fp[p] = sum_j fc Rnp[p];
+    three species under the default mixing, one without mixing."""
+    st = _liquid(symbols, 1500, 25)
+    md = M.random_model(symbols, nmax=nmax, lmax=0, rcut=6.0, hidden=(32, 32), seed=35)
+    for m in md["models"]:
+        b = m["basis"]
+        m["basis"] = {"type": "chebyshev", "nmax": b["nmax"], "rcut": b["rcut"], "wtype": b["wtype"]}
+    assert all(s.radial for s in M.load_model(md).species)
+    md = W._normalized(md, st, 300, 35)
     _check(O, md, st)

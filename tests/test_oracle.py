@@ -390,3 +390,53 @@ def test_post_fuse_is_applied_after_the_base_mixing(O):
     b = O.Oracle(M.load_model(mp)).system_efv(st)
     assert np.abs(a["energies"] - b["energies"]).max() <= 1e-12
     assert np.abs(a["forces"] - b["forces"]).max() <= 1e-11
+
+
+def _radial_model(symbols, nmax, rcut, hidden, seed, wtype="default"):
+    """A model whose bases are the radial-only `chebyshev` type (Chebyshev.java)."""
+    md = M.random_model(symbols, nmax=nmax, lmax=0, rcut=rcut, hidden=hidden, seed=seed, wtype=wtype)
+    for m in md["models"]:
+        b = m["basis"]
+        m["basis"] = {"type": "chebyshev", "nmax": b["nmax"], "rcut": b["rcut"], "wtype": b["wtype"]}
+    return md
+
+
+def test_radial_basis_closed_form(O):
+    """Chebyshev basis: fp[p] = sum_j fc(r_j) Rnp_j[p] with Rnp = W[type_j] T_n(1 - 2r/rc)
+    (basis_Chebyshev.hpp:9-33), checked against numpy's Chebyshev series for two
+    neighbours of different species under the default mixing [I; w_t I]."""
+    from numpy.polynomial import chebyshev as C
+    md = _radial_model(["Cu", "Zr"], 5, 6.0, (8, 8), seed=44)
+    mdl = M.load_model(md)
+    sp = mdl.species[0]
+    assert sp.radial and sp.S == sp.P == 12
+    orc = O.Oracle(mdl)
+    d = np.array([[1.1, -2.0, 0.7], [0.3, 2.5, -3.1]])
+    tj = [1, 2]
+    fp = orc.cal_fp(1, d, tj)
+    want = np.zeros(12)
+    for dv, t in zip(d, tj):
+        r = np.linalg.norm(dv)
+        fc = (1 - (r / 6.0) ** 2) ** 4
+        T = np.array([C.chebval(1 - 2 * r / 6.0, np.eye(6)[n]) for n in range(6)])
+        wt = 1.0 if t == 1 else -2.0
+        want += fc * np.concatenate([T, wt * T])
+    assert np.allclose(fp, want, rtol=1e-13, atol=1e-14)
+
+
+def test_radial_basis_forces_are_minus_energy_gradient(O):
+    """The radial basis' backward (basis_Chebyshev.hpp:35-80) against central
+    finite differences of the total energy."""
+    st = W.ternary_liquid(n=200, seed=13)
+    md = _radial_model(["Cu", "Zr", "Al"], 6, 6.0, (16, 16), seed=45)
+    md = W._normalized(md, st, 100, 45)
+    orc = O.Oracle(M.load_model(md))
+    res = orc.system_efv(st)
+    h = 1e-5
+    for i in (0, 17, 61):
+        for c in range(3):
+            xp = st.xyz.copy(); xp[i, c] += h
+            xm = st.xyz.copy(); xm[i, c] -= h
+            ep = orc.system_efv(W.Structure(st.box, xp, st.types))["energy"]
+            em = orc.system_efv(W.Structure(st.box, xm, st.types))["energy"]
+            assert abs(-(ep - em) / (2 * h) - res["forces"][i, c]) < 2e-7 * max(1, abs(res["forces"][i, c]))
