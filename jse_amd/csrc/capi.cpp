@@ -124,8 +124,8 @@ void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision,
     if (precision == JSE_NNAP_BF16_MLP) {
         if (!launch_mlp_mfma_bf16(s, a))
             throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
-    } else if (!launch_mlp_mfma_fp32(s, a)) {
-        launch_mlp_fp32(s, a);
+    } else if (!launch_mlp_mfma_fp32(s, a) && !launch_mlp_fp32(s, a)) {
+        throw Error(ERR_UNSUPPORTED, kMlpFp32Limit);
     }
 }
 void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *segs, bool backward,
@@ -146,7 +146,7 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
     auto fail = [&](int t) {
         const SpeciesHost &sp = m->host.species[t];
         throw Error(ERR_UNSUPPORTED, std::string("no descriptor ") + (backward ? "backward " : "") +
-                                         "kernel compiled for lmax=" + std::to_string(sp.lmax) + " nmax=" +
+                                         "kernel for lmax=" + std::to_string(sp.lmax) + " nmax=" +
                                          std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P));
     };
     if (one) {
@@ -184,8 +184,8 @@ void enqueue_mlp_all(hipStream_t s, const jse_nnap_model_s *m, int precision, co
         if (precision == JSE_NNAP_BF16_MLP) {
             if (!launch_mlp_mfma_bf16(s, a))
                 throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
-        } else if (!launch_mlp_mfma_fp32(s, a)) {
-            launch_mlp_fp32(s, a);
+        } else if (!launch_mlp_mfma_fp32(s, a) && !launch_mlp_fp32(s, a)) {
+            throw Error(ERR_UNSUPPORTED, kMlpFp32Limit);
         }
     }
 }

@@ -71,41 +71,6 @@ __device__ __forceinline__ void mix_rnp(const float *rn, int tj, const float *__
         }
     }
 }
-// Pair geometry from hardware reciprocal square roots (about 1 ulp each): r,
-// 1/r, cos/sin theta, cos/sin phi, dxy = |(dx, dy)| and 1/dxy.  dx and dy are
-// rescaled by 2^64 when dx^2 + dy^2 would leave the normal range; a pair with
-// dxy below 8 FLT_MIN is on the pole (cos phi = 1, sin phi = 0), the
-// reference's dxy == 0 convention (basis_SphericalUtil.hpp:165-181) with a
-// guard against denormal dxy.
-struct PairGeomR {
-    float r, ir, ct, st, cp, sp, dxy, idxy;
-    bool pole;
-};
-__device__ __forceinline__ PairGeomR pair_geom_fast(float dx, float dy, float dz) {
-    PairGeomR g;
-    const float r2 = dx * dx + dy * dy + dz * dz;
-    g.ir = __builtin_amdgcn_rsqf(r2);
-    g.r = r2 * g.ir;
-    float sx = dx, sy = dy, unscale = 1.f, rescale = 1.f;
-    float q = dx * dx + dy * dy;
-    if (q < 0x1p-100f) {
-        sx = dx * 0x1p+64f;
-        sy = dy * 0x1p+64f;
-        unscale = 0x1p-64f;
-        rescale = 0x1p+64f;
-        q = sx * sx + sy * sy;
-    }
-    const float iq = q > 0.f ? __builtin_amdgcn_rsqf(q) : 0.f;
-    g.dxy = (q * iq) * unscale;
-    g.pole = !(g.dxy >= 8.f * 1.17549435e-38f);
-    g.st = g.dxy * g.ir;
-    g.ct = dz * g.ir;
-    g.cp = g.pole ? 1.f : sx * iq;
-    g.sp = g.pole ? 0.f : sy * iq;
-    g.idxy = g.pole ? 0.f : iq * rescale;
-    return g;
-}
-
 // TP0: p-rows of each lane's register tile (tile TP0 x 7 of A)
 template <int LMAX, int K, int P, int MIX, int TP0>
 __global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl) {
@@ -880,7 +845,7 @@ static bool launch_rad_set(hipStream_t s, int K, int P, int mix, const SegmentSe
     return false;
 }
 
-// ---- dispatch table over the compiled shapes ----
+// ---- dispatch table over the tiled shapes (other shapes: the kernels of desc_gen.hip) ----
 template <int LMAX, int K, int P, int MIX>
 static void run_fwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl) {
     const SegmentArgs &a = set.a[0];  // launch shape, variant and precision are the set's
@@ -928,7 +893,8 @@ static void run_bwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl
 bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
                          const NeighborList &nl) {
     if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
-    if (lmax == kRadialLmax) return launch_rad_set(s, K, P, mix, set, nl, false);
+    if (lmax == kRadialLmax)
+        return launch_rad_set(s, K, P, mix, set, nl, false) || launch_desc_gen_set(s, lmax, K, P, set, nl, false);
 #define X(L_, K_, P_, M_)                                              \
     if (lmax == L_ && K == K_ && P == P_ && mix == M_) {              \
         run_fwd<L_, K_, P_, M_>(s, set, nl);                          \
@@ -936,12 +902,13 @@ bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const S
     }
     JNN_SHAPES(X)
 #undef X
-    return false;
+    return launch_desc_gen_set(s, lmax, K, P, set, nl, false);
 }
 bool launch_desc_bwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
                          const NeighborList &nl) {
     if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
-    if (lmax == kRadialLmax) return launch_rad_set(s, K, P, mix, set, nl, true);
+    if (lmax == kRadialLmax)
+        return launch_rad_set(s, K, P, mix, set, nl, true) || launch_desc_gen_set(s, lmax, K, P, set, nl, true);
 #define X(L_, K_, P_, M_)                                              \
     if (lmax == L_ && K == K_ && P == P_ && mix == M_) {              \
         run_bwd<L_, K_, P_, M_>(s, set, nl);                          \
@@ -949,7 +916,7 @@ bool launch_desc_bwd_set(hipStream_t s, int lmax, int K, int P, int mix, const S
     }
     JNN_SHAPES(X)
 #undef X
-    return false;
+    return launch_desc_gen_set(s, lmax, K, P, set, nl, true);
 }
 static SegmentSet single(const SegmentArgs &a) {
     SegmentSet set{};

@@ -1,0 +1,375 @@
+// Descriptor kernels for every basis shape the reference can load that has no
+// tiled kernel of its own: spherical-Chebyshev with any lmax in [0, 12], nmax
+// in [0, 20] (SphericalChebyshev.java:32, WTypeBasis.java:56) and any radial
+// mixing (all wtypes go through the dense W[t][P][K] of
+// WTypeBasis.updateParameters_, WTypeBasis.java:202-304), and the radial-only
+// chebyshev basis with any nmax and mixing.  The quantities are those of the
+// tiled kernels (basis_SphericalChebyshev.hpp:264-465, basis_Chebyshev.hpp:
+// 9-80); only the work layout differs:
+//
+// Forward, one wavefront per atom: chunks of `ch` neighbours, one lane each,
+// put c_j[p] = fc W[t_j] Rn and Y_j into LDS; each lane then owns the
+// elements e = lane + 64 k of A (P x M) and adds sum_j c_j[p] Y_j[m] over the
+// chunk, neighbours in list order, into FP64 accumulators in LDS (every
+// c * Y product of two floats is exact in FP64).  fp and x are formed in FP64.
+//
+// Backward, one lane per neighbour: dE/dA[p][m] = Gc[p][l(m)] A[p][m] goes to
+// LDS once per atom; each lane runs over p with s_p = sum_m dE/dA[p][m] Y[m]
+// (dE/dc_p) and gY[m] += dE/dA[p][m] c_p, then dE/dfc = sum_p s_p Rnp[p] and
+// dE/dRn[n] = fc sum_p s_p W[t][p][n], and the chain rule of the tiled
+// backward gives g_j = -dE_i/d(r_j - r_i).
+#include <hip/hip_runtime.h>
+
+#include "common.h"
+#include "device_math.h"
+#include "kernels.h"
+
+namespace jnn {
+namespace {
+
+constexpr int kGenKMax = 21;        // nmax <= 20
+constexpr int kGenPMax = 128;       // mixing rows handled here
+constexpr size_t kGenLds = 160 * 1024;
+
+__device__ __forceinline__ int seg_of(const SegmentSet &set, int gr) {
+    int sg = 0;
+    while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
+    return sg;
+}
+
+// c[p] = fc * sum_{n < K} w[p][n] rn[n] for one mixing row (w = W[t] + p K)
+__device__ __forceinline__ float mix_row(const float *__restrict__ w, int K, const float *rn) {
+    float s = 0.f;
+#pragma unroll
+    for (int q = 0; q < kGenKMax; ++q)
+        if (q < K) s += w[q] * rn[q];
+    return s;
+}
+
+// LDS: A accumulators (FP64, P*M), then Y (ch x MS floats, MS = M rounded up to
+// odd so the lanes' row writes fall on distinct banks), then c (ch x P floats).
+template <int LMAX>
+__global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl, int K, int P, int ch) {
+    constexpr int L1 = LMAX + 1, M = L1 * L1, MS = M | 1;
+    extern __shared__ double gen_smem[];
+    double *sA = gen_smem;
+    float *sY = reinterpret_cast<float *>(sA + (size_t)P * M);
+    float *sC = sY + ch * MS;
+    const int lane = threadIdx.x;
+    const int PM = P * M, S = P * L1;
+    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+        const int sg = seg_of(set, gr);
+        const SegmentArgs &a = set.a[sg];
+        const int row = gr - set.row0[sg];
+        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        __syncthreads();  // the previous atom's epilogue has read sA
+        for (int e = lane; e < PM; e += 64) sA[e] = 0.0;
+        for (int c0 = 0; c0 < n; c0 += ch) {
+            const int nc = min(ch, n - c0);
+            __syncthreads();  // the previous chunk's sums have read sY / sC
+            if (lane < nc) {
+                const float4 dd = nl.d[base + c0 + lane];
+                const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
+                const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+                float y[M];
+                real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
+#pragma unroll
+                for (int m = 0; m < M; ++m) sY[lane * MS + m] = y[m];
+                float rn[kGenKMax];
+                cheb_rn<kGenKMax>(1.f - 2.f * g.r * inv_rc, rn);
+                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+                const float t2 = t * t;
+                const float fc = g.r < rc ? t2 * t2 : 0.f;  // pairs beyond this species' cutoff add nothing
+                const float *w = a.W + (size_t)(tj - 1) * P * K;
+                for (int p = 0; p < P; ++p) sC[lane * P + p] = fc * mix_row(w + p * K, K, rn);
+            }
+            __syncthreads();
+            for (int e = lane; e < PM; e += 64) {
+                const int p = e / M, m = e - p * M;
+                double acc = 0.0;
+                for (int j = 0; j < nc; ++j) acc += (double)sC[j * P + p] * (double)sY[j * MS + m];
+                sA[e] += acc;
+            }
+        }
+        __syncthreads();
+        for (int e = lane; e < PM; e += 64) a.A[(size_t)row * PM + e] = (float)sA[e];
+        for (int e = lane; e < S; e += 64) {
+            const int p = e / L1, l = e - p * L1;
+            double s = 0.0;
+            for (int m = l * l; m < (l + 1) * (l + 1); ++m) {
+                const double v = sA[p * M + m];
+                s += v * v;
+            }
+            const double w4 = 4.0 * 3.14159265358979323846 / (double)(2 * l + 1);
+            a.x[(size_t)row * S + e] = (float)((w4 * s - a.mu64[e]) * (1.0 / a.sig64[e]));
+        }
+    }
+}
+
+// LDS: dE/dA (P*M floats) of the current atom.
+template <int LMAX>
+__global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl, int K, int P) {
+    constexpr int L1 = LMAX + 1, M = L1 * L1;
+    extern __shared__ double gen_smem[];
+    float *sGA = reinterpret_cast<float *>(gen_smem);
+    const int lane = threadIdx.x;
+    const int PM = P * M, S = P * L1;
+    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+        const int sg = seg_of(set, gr);
+        const SegmentArgs &a = set.a[sg];
+        const int row = gr - set.row0[sg];
+        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        __syncthreads();  // the previous atom's pairs have read sGA
+        for (int e = lane; e < PM; e += 64) {
+            const int p = e / M, m = e - p * M;
+            int l = 0;
+            while ((l + 1) * (l + 1) <= m) ++l;
+            sGA[e] = a.G[(size_t)row * S + p * L1 + l] * a.A[(size_t)row * PM + e];
+        }
+        __syncthreads();
+        for (int j = lane; j < n; j += 64) {
+            const float4 dd = nl.d[base + j];
+            const float dx = dd.x, dy = dd.y, dz = dd.z;
+            const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
+            const PairGeomR g = pair_geom_fast(dx, dy, dz);
+            float ox = 0.f, oy = 0.f, oz = 0.f;
+            if (g.r < rc) {
+                float y[M];
+                real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
+                const float xr = 1.f - 2.f * g.r * inv_rc;
+                float rn[kGenKMax];
+                cheb_rn<kGenKMax>(xr, rn);
+                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+                const float t2 = t * t;
+                const float fc = t2 * t2;
+                const float *w = a.W + (size_t)(tj - 1) * P * K;
+                float gy[M];
+#pragma unroll
+                for (int m = 0; m < M; ++m) gy[m] = 0.f;
+                float grn[kGenKMax];
+#pragma unroll
+                for (int q = 0; q < kGenKMax; ++q) grn[q] = 0.f;
+                float gfc = 0.f;
+                for (int p = 0; p < P; ++p) {
+                    const float *wp = w + p * K;
+                    const float rnp = mix_row(wp, K, rn);
+                    const float cp = fc * rnp;
+                    const float *ga = sGA + p * M;
+                    float s4[4] = {0.f, 0.f, 0.f, 0.f};  // shorter FP32 accumulation chains
+#pragma unroll
+                    for (int m = 0; m < M; ++m) {
+                        gy[m] += ga[m] * cp;
+                        s4[m & 3] += ga[m] * y[m];
+                    }
+                    const float sp = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+                    gfc += sp * rnp;
+                    const float sf = sp * fc;
+#pragma unroll
+                    for (int q = 0; q < kGenKMax; ++q)
+                        if (q < K) grn[q] += wp[q] * sf;
+                }
+                float drn[kGenKMax];
+                cheb_drn<kGenKMax>(xr, inv_rc, drn);
+                float dEdr = 0.f;
+#pragma unroll
+                for (int q = 0; q < kGenKMax; ++q)
+                    if (q < K) dEdr += grn[q] * drn[q];
+                dEdr += gfc * (-8.f * g.r * inv_rc * inv_rc * t2 * t);
+                float dEdph = 0.f;
+#pragma unroll
+                for (int l = 1; l < L1; ++l) {
+                    float pl = 0.f;
+#pragma unroll
+                    for (int m = 1; m <= l; ++m) {
+                        const int idx = l * l + l;
+                        // dY_{l,m}/dphi = -m Y_{l,-m},  dY_{l,-m}/dphi = m Y_{l,m}
+                        pl += (float)m * (gy[idx - m] * y[idx + m] - gy[idx + m] * y[idx - m]);
+                    }
+                    dEdph += pl;
+                }
+                const float dEdth = dtheta_contract<LMAX>(g.ct, g.st, g.cp, g.sp, gy);
+                const float ir = g.ir;
+                // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi))
+                ox = -(dEdr * dx * ir + dEdth * g.ct * g.cp * ir);
+                oy = -(dEdr * dy * ir + dEdth * g.ct * g.sp * ir);
+                oz = -(dEdr * dz * ir - dEdth * g.st * ir);
+                if (!g.pole) {
+                    ox -= dEdph * (-g.sp * g.idxy);
+                    oy -= dEdph * (g.cp * g.idxy);
+                }
+            }
+            nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
+        }
+    }
+}
+
+// Radial-only basis, forward: chunks of 64 neighbours put c_j[p] = fc Rnp[p]
+// into LDS; lane p (and p + 64) sums them in FP64, neighbours in list order.
+__global__ __launch_bounds__(64) void k_gen_rad_fwd(SegmentSet set, NeighborList nl, int K, int P) {
+    __shared__ float sC[64 * kGenPMax];
+    const int lane = threadIdx.x;
+    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+        const int sg = seg_of(set, gr);
+        const SegmentArgs &a = set.a[sg];
+        const int row = gr - set.row0[sg];
+        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        double acc0 = 0.0, acc1 = 0.0;
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int nc = min(64, n - c0);
+            __syncthreads();
+            if (lane < nc) {
+                const float4 dd = nl.d[base + c0 + lane];
+                const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
+                const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+                float rn[kGenKMax];
+                cheb_rn<kGenKMax>(1.f - 2.f * g.r * inv_rc, rn);
+                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+                const float t2 = t * t;
+                const float fc = g.r < rc ? t2 * t2 : 0.f;
+                const float *w = a.W + (size_t)(tj - 1) * P * K;
+                for (int p = 0; p < P; ++p) sC[lane * P + p] = fc * mix_row(w + p * K, K, rn);
+            }
+            __syncthreads();
+            for (int j = 0; j < nc; ++j) {
+                if (lane < P) acc0 += (double)sC[j * P + lane];
+                if (lane + 64 < P) acc1 += (double)sC[j * P + lane + 64];
+            }
+        }
+        if (lane < P) a.x[(size_t)row * P + lane] = (float)((acc0 - a.mu64[lane]) * (1.0 / a.sig64[lane]));
+        if (lane + 64 < P)
+            a.x[(size_t)row * P + lane + 64] = (float)((acc1 - a.mu64[lane + 64]) * (1.0 / a.sig64[lane + 64]));
+    }
+}
+
+// Radial-only basis, backward: lane per pair, dE/dfc = sum_p G[p] Rnp[p] and
+// dE/dRn[n] = fc sum_p G[p] W[t][p][n] with G = dE/dfp of the atom.
+__global__ __launch_bounds__(64) void k_gen_rad_bwd(SegmentSet set, NeighborList nl, int K, int P) {
+    __shared__ float sG[kGenPMax];
+    const int lane = threadIdx.x;
+    for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
+        const int sg = seg_of(set, gr);
+        const SegmentArgs &a = set.a[sg];
+        const int row = gr - set.row0[sg];
+        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        __syncthreads();
+        for (int p = lane; p < P; p += 64) sG[p] = a.G[(size_t)row * P + p];
+        __syncthreads();
+        for (int j = lane; j < n; j += 64) {
+            const float4 dd = nl.d[base + j];
+            const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
+            const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+            float ox = 0.f, oy = 0.f, oz = 0.f;
+            if (g.r < rc) {
+                const float xr = 1.f - 2.f * g.r * inv_rc;
+                float rn[kGenKMax], drn[kGenKMax];
+                cheb_rn<kGenKMax>(xr, rn);
+                cheb_drn<kGenKMax>(xr, inv_rc, drn);
+                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
+                const float t2 = t * t;
+                const float fc = t2 * t2;
+                const float *w = a.W + (size_t)(tj - 1) * P * K;
+                float gw[kGenKMax];
+#pragma unroll
+                for (int q = 0; q < kGenKMax; ++q) gw[q] = 0.f;
+                float gfc = 0.f;
+                for (int p = 0; p < P; ++p) {
+                    const float *wp = w + p * K;
+                    const float gp = sG[p];
+                    gfc += gp * mix_row(wp, K, rn);
+#pragma unroll
+                    for (int q = 0; q < kGenKMax; ++q)
+                        if (q < K) gw[q] += gp * wp[q];
+                }
+                float dEdr = 0.f;
+#pragma unroll
+                for (int q = 0; q < kGenKMax; ++q)
+                    if (q < K) dEdr += gw[q] * drn[q];
+                dEdr *= fc;
+                dEdr += gfc * (-8.f * g.r * inv_rc * inv_rc * t2 * t);
+                ox = -dEdr * dd.x * g.ir;
+                oy = -dEdr * dd.y * g.ir;
+                oz = -dEdr * dd.z * g.ir;
+            }
+            nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
+        }
+    }
+}
+
+int gen_grid(const SegmentSet &set) {
+    const int rows = set.row0[set.nseg];
+    return rows < 262144 ? rows : 262144;
+}
+
+// LDS of the forward for a chunk of ch neighbours
+size_t gen_fwd_lds(int M, int P, int ch) {
+    return sizeof(double) * (size_t)P * M + sizeof(float) * (size_t)ch * ((M | 1) + P);
+}
+
+template <int LMAX>
+bool run_gen(hipStream_t s, int K, int P, const SegmentSet &set, const NeighborList &nl, bool backward) {
+    constexpr int M = (LMAX + 1) * (LMAX + 1);
+    static bool attr_set = false;
+    if (!attr_set) {
+        JNN_HIP(hipFuncSetAttribute((const void *)k_gen_fwd<LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kGenLds));
+        JNN_HIP(hipFuncSetAttribute((const void *)k_gen_bwd<LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kGenLds));
+        attr_set = true;
+    }
+    const int grid = gen_grid(set);
+    if (backward) {
+        const size_t lds = sizeof(float) * (size_t)P * M;
+        if (lds > kGenLds) return false;
+        hipLaunchKernelGGL((k_gen_bwd<LMAX>), dim3(grid), dim3(64), lds, s, set, nl, K, P);
+        return true;
+    }
+    int ch = 64;  // the largest chunk that fits
+    while (ch > 8 && gen_fwd_lds(M, P, ch) > kGenLds) ch /= 2;
+    if (gen_fwd_lds(M, P, ch) > kGenLds) return false;
+    hipLaunchKernelGGL((k_gen_fwd<LMAX>), dim3(grid), dim3(64), gen_fwd_lds(M, P, ch), s, set, nl, K, P, ch);
+    return true;
+}
+
+}  // namespace
+
+bool launch_desc_gen_set(hipStream_t s, int lmax, int K, int P, const SegmentSet &set, const NeighborList &nl,
+                         bool backward) {
+    if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
+    if (K < 1 || K > kGenKMax || P < 1 || P > kGenPMax) return false;
+    if (lmax == kRadialLmax) {
+        const int grid = gen_grid(set);
+        if (backward) hipLaunchKernelGGL(k_gen_rad_bwd, dim3(grid), dim3(64), 0, s, set, nl, K, P);
+        else hipLaunchKernelGGL(k_gen_rad_fwd, dim3(grid), dim3(64), 0, s, set, nl, K, P);
+        return true;
+    }
+    switch (lmax) {
+        case 0: return run_gen<0>(s, K, P, set, nl, backward);
+        case 1: return run_gen<1>(s, K, P, set, nl, backward);
+        case 2: return run_gen<2>(s, K, P, set, nl, backward);
+        case 3: return run_gen<3>(s, K, P, set, nl, backward);
+        case 4: return run_gen<4>(s, K, P, set, nl, backward);
+        case 5: return run_gen<5>(s, K, P, set, nl, backward);
+        case 6: return run_gen<6>(s, K, P, set, nl, backward);
+        case 7: return run_gen<7>(s, K, P, set, nl, backward);
+        case 8: return run_gen<8>(s, K, P, set, nl, backward);
+        case 9: return run_gen<9>(s, K, P, set, nl, backward);
+        case 10: return run_gen<10>(s, K, P, set, nl, backward);
+        case 11: return run_gen<11>(s, K, P, set, nl, backward);
+        case 12: return run_gen<12>(s, K, P, set, nl, backward);
+        default: return false;
+    }
+}
+
+}  // namespace jnn

@@ -134,6 +134,41 @@ __device__ __forceinline__ float dtheta_contract(float ct, float st, float cp, f
     return acc;
 }
 
+// Pair geometry from hardware reciprocal square roots (about 1 ulp each): r,
+// 1/r, cos/sin theta, cos/sin phi, dxy = |(dx, dy)| and 1/dxy.  dx and dy are
+// rescaled by 2^64 when dx^2 + dy^2 would leave the normal range; a pair with
+// dxy below 8 FLT_MIN is on the pole (cos phi = 1, sin phi = 0), the
+// reference's dxy == 0 convention (basis_SphericalUtil.hpp:165-181) with a
+// guard against denormal dxy.
+struct PairGeomR {
+    float r, ir, ct, st, cp, sp, dxy, idxy;
+    bool pole;
+};
+__device__ __forceinline__ PairGeomR pair_geom_fast(float dx, float dy, float dz) {
+    PairGeomR g;
+    const float r2 = dx * dx + dy * dy + dz * dz;
+    g.ir = __builtin_amdgcn_rsqf(r2);
+    g.r = r2 * g.ir;
+    float sx = dx, sy = dy, unscale = 1.f, rescale = 1.f;
+    float q = dx * dx + dy * dy;
+    if (q < 0x1p-100f) {
+        sx = dx * 0x1p+64f;
+        sy = dy * 0x1p+64f;
+        unscale = 0x1p-64f;
+        rescale = 0x1p+64f;
+        q = sx * sx + sy * sy;
+    }
+    const float iq = q > 0.f ? __builtin_amdgcn_rsqf(q) : 0.f;
+    g.dxy = (q * iq) * unscale;
+    g.pole = !(g.dxy >= 8.f * 1.17549435e-38f);
+    g.st = g.dxy * g.ir;
+    g.ct = dz * g.ir;
+    g.cp = g.pole ? 1.f : sx * iq;
+    g.sp = g.pole ? 0.f : sy * iq;
+    g.idxy = g.pole ? 0.f : iq * rescale;
+    return g;
+}
+
 // Chebyshev radial basis Rn = T_n(1 - 2r/rc), n = 0..K-1
 template <int K>
 __device__ __forceinline__ void cheb_rn(float x, float *rn) {

@@ -176,6 +176,11 @@ bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const S
                          const NeighborList &nl);
 bool launch_desc_bwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
                          const NeighborList &nl);
+// The untiled kernels for the shapes above without a tiled kernel: lmax in
+// [0, 12] or kRadialLmax, K <= 21, P <= 128, any mixing (through the dense W);
+// false when the shape is outside these limits or its A does not fit in LDS.
+bool launch_desc_gen_set(hipStream_t s, int lmax, int K, int P, const SegmentSet &set, const NeighborList &nl,
+                         bool backward);
 
 struct MlpArgs {
     int nrows, S, L1, nlayers;
@@ -193,7 +198,11 @@ struct MlpArgs {
     int wg_per_cu;           // persistent grid size (0: 2 per CU)
     int radial;              // radial-only basis: G = dE/dfp / sigma (no L2 power-spectrum factor)
 };
-void launch_mlp_fp32(hipStream_t s, const MlpArgs &a);
+// Any layer count and widths; false when a 64-row tile of the input and the
+// hidden activations (and their SiLU derivatives) exceeds 160 KB of LDS:
+// S + 2 * sum(hidden) > 640.
+bool launch_mlp_fp32(hipStream_t s, const MlpArgs &a);
+constexpr const char *kMlpFp32Limit = "the FP32 MLP needs input_dim + 2 * sum(hidden_dims) <= 640";
 // MFMA kernels for 2 hidden layers of supported widths; false when the shape has no compiled kernel
 bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a);
 bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a);

@@ -152,9 +152,10 @@ static size_t mlp_lds_bytes(const MlpArgs &a) {
     return f * sizeof(float);
 }
 
-void launch_mlp_fp32(hipStream_t s, const MlpArgs &a) {
-    if (a.nrows <= 0) return;
+bool launch_mlp_fp32(hipStream_t s, const MlpArgs &a) {
+    if (a.nrows <= 0) return true;
     const size_t lds = mlp_lds_bytes(a);
+    if (lds > 160 * 1024) return false;
     static bool attr_set = false;
     if (!attr_set) {
         (void)hipFuncSetAttribute((const void *)k_mlp_fp32, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
@@ -162,6 +163,7 @@ void launch_mlp_fp32(hipStream_t s, const MlpArgs &a) {
     }
     const int tiles = (a.nrows + kMlpTile - 1) / kMlpTile;
     hipLaunchKernelGGL(k_mlp_fp32, dim3(tiles), dim3(256), lds, s, a);
+    return true;
 }
 
 }  // namespace jnn

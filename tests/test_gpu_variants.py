@@ -1,9 +1,10 @@
-"""GPU parity of the other compiled descriptor shapes against the FP64 oracle
+"""GPU parity of the other descriptor shapes against the FP64 oracle
 (through the C-ABI): dense radial mixing (a two-species model with wtype
 "full", P = 2K, each species' channels in its own block) and lmax = 4 with
 nmax = 5 (one species: no mixing; three species: the default mixing),
-mirror / share bases, a post-fuse weight and the radial-only chebyshev
-basis.
+mirror / share bases, a post-fuse weight, the radial-only chebyshev basis,
+and shapes without a tiled kernel (the untiled kernels: lmax 0-12, nmax up
+to 20, any wtype).
 Plain FP32 bar: 1e-6 eV/atom, 1e-5 eV/A RMS forces, 1e-5 eV/atom virial."""
 import numpy as np
 import pytest
@@ -102,4 +103,30 @@ def test_radial_chebyshev_basis(O, symbols, nmax):
         m["basis"] = {"type": "chebyshev", "nmax": b["nmax"], "rcut": b["rcut"], "wtype": b["wtype"]}
     assert all(s.radial for s in M.load_model(md).species)
     md = W._normalized(md, st, 300, 35)
+    _check(O, md, st)
+
+
+@pytest.mark.parametrize("symbols,nmax,lmax,wtype,hidden", [
+    (["Cu"], 4, 3, "none", (32, 32)),                  # P = 5, S = 20
+    (["Cu", "Zr", "Al"], 5, 8, "exfull", (32, 32)),    # P = 24, S = 216 (the vector MLP)
+    (["Cu", "Zr"], 20, 2, "full", (64, 64)),           # K = 21 (nmax at its limit), P = 42
+    (["Cu"], 3, 12, "none", (32, 32)),                 # lmax at its limit, M = 169
+    (["Cu", "Zr", "Al"], 7, 0, "default", (32, 32)),   # lmax = 0: fp = 4 pi A_00^2
+])
+def test_untiled_shapes(O, symbols, nmax, lmax, wtype, hidden):
+    st = _liquid(symbols, 1000, 26)
+    md = M.random_model(symbols, nmax=nmax, lmax=lmax, rcut=6.0, hidden=hidden, seed=36, wtype=wtype)
+    md = W._normalized(md, st, 200, 36)
+    _check(O, md, st)
+
+
+def test_untiled_radial_shape(O):
+    """Radial-only basis with wtype "exfull" over two species: K = 13, P = 39."""
+    st = _liquid(["Cu", "Zr"], 1000, 27)
+    md = M.random_model(["Cu", "Zr"], nmax=12, lmax=0, rcut=6.0, hidden=(32, 32), seed=37, wtype="exfull")
+    for m in md["models"]:
+        b = m["basis"]
+        m["basis"] = {"type": "chebyshev", "nmax": b["nmax"], "rcut": b["rcut"], "wtype": b["wtype"]}
+    assert [s.P for s in M.load_model(md).species] == [39, 39]
+    md = W._normalized(md, st, 200, 37)
     _check(O, md, st)

@@ -179,6 +179,70 @@ def test_default_wtype_structure():
     assert one.species[0].P == K and np.array_equal(one.species[0].W[0], np.eye(K))
 
 
+def test_full_and_exfull_wtype_structure():
+    """WTypeBasis.updateParameters_ (WTypeBasis.java:265-283): "full" puts
+    species t's channels in block t (P = ntypes K); "exfull" adds a shared
+    block 0 in front ((ntypes + 1) K)."""
+    K = 5
+    full = M.load_model(M.random_model(["A", "B", "C"], 4, 2, 5.0, (4,), seed=1, wtype="full")).species[0]
+    ex = M.load_model(M.random_model(["A", "B", "C"], 4, 2, 5.0, (4,), seed=1, wtype="exfull")).species[0]
+    assert full.W.shape == (3, 3 * K, K) and ex.W.shape == (3, 4 * K, K)
+    for t in (1, 2, 3):
+        want = np.zeros((3 * K, K))
+        want[(t - 1) * K:t * K] = np.eye(K)
+        assert np.array_equal(full.W[t - 1], want)
+        want = np.zeros((4 * K, K))
+        want[:K] = np.eye(K)
+        want[t * K:(t + 1) * K] = np.eye(K)
+        assert np.array_equal(ex.W[t - 1], want)
+
+
+def test_addition_theorem_at_the_shape_limits(O):
+    """lmax = 12 and nmax = 20 (the loaders' limits) with the exfull mixing:
+    the closed form sum_{j,k} c_j c_k P_l(cos gamma_jk) still holds."""
+    from scipy.special import eval_legendre
+    mdl = M.load_model(M.random_model(["Cu", "Zr"], 20, 12, 6.0, (4, 4), seed=5, wtype="exfull"))
+    sp = mdl.species[0]
+    assert sp.K == 21 and sp.P == 63 and sp.lmax == 12
+    orc = O.Oracle(mdl)
+    rng = np.random.default_rng(3)
+    d = rng.uniform(-4.0, 4.0, size=(12, 3))
+    tj = rng.integers(1, 3, size=12)
+    fp = orc.cal_fp(1, d, tj).reshape(sp.P, sp.lmax + 1)
+    r = np.linalg.norm(d, axis=1)
+    keep = r < sp.rcut
+    d, tj, r = d[keep], tj[keep], r[keep]
+    x = 1 - 2 * r / sp.rcut
+    T = np.array([np.cos(n * np.arccos(x)) for n in range(sp.K)]).T
+    fc = (1 - (r / sp.rcut) ** 2) ** 4
+    c = np.einsum("jpk,jk->jp", sp.W[tj - 1], T) * fc[:, None]
+    u = d / r[:, None]
+    cosg = np.clip(u @ u.T, -1, 1)
+    for l in range(sp.lmax + 1):
+        want = np.einsum("jp,kp,jk->p", c, c, eval_legendre(l, cosg))
+        assert np.allclose(fp[:, l], want, rtol=1e-10, atol=1e-12)
+
+
+def test_forces_at_the_shape_limits_are_minus_energy_gradient(O):
+    """Finite differences of the total energy at lmax = 12 and at nmax = 20
+    (exfull mixing, two species)."""
+    for nmax, lmax, wtype in ((3, 12, "default"), (20, 2, "exfull")):
+        st = W.Structure(np.array([12.5, 12.5, 13.0]), W.rsa(60, [12.5, 12.5, 13.0], 2.0, 9),
+                         (np.arange(60) % 2 + 1).astype(np.int32))
+        md = M.random_model(["Cu", "Zr"], nmax, lmax, 5.5, (8, 8), seed=9, wtype=wtype)
+        md = M.set_normalization(md, W.descriptors_np(M.load_model(md), st))
+        orc = O.Oracle(M.load_model(md))
+        res = orc.system_efv(st)
+        h = 1e-5
+        for i in (0, 31):
+            for c in range(3):
+                xp = st.xyz.copy(); xp[i, c] += h
+                xm = st.xyz.copy(); xm[i, c] -= h
+                ep = orc.system_efv(W.Structure(st.box, xp, st.types))["energy"]
+                em = orc.system_efv(W.Structure(st.box, xm, st.types))["energy"]
+                assert abs(-(ep - em) / (2 * h) - res["forces"][i, c]) < 2e-7 * max(1, abs(res["forces"][i, c]))
+
+
 # --------------------------------------------------------------------------- MLP
 def test_mlp_against_torch_autograd(O):
     torch = pytest.importorskip("torch")
