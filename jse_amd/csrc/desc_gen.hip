@@ -8,16 +8,19 @@
 // 9-80); only the work layout differs:
 //
 // Forward, one wavefront per atom: chunks of `ch` neighbours, one lane each,
-// put c_j[p] = fc W[t_j] Rn and Y_j into LDS; each lane then owns the
-// elements e = lane + 64 k of A (P x M) and adds sum_j c_j[p] Y_j[m] over the
-// chunk, neighbours in list order, into FP64 accumulators in LDS (every
-// c * Y product of two floats is exact in FP64).  fp and x are formed in FP64.
+// put c_j[p] = fc W[t_j] Rn and Y_j into LDS (rows padded with zeros to a
+// multiple of 4); A (P x M) is cut into 4 x 4 tiles, which the lanes take
+// round-robin: per neighbour a tile costs two 16-byte LDS reads and 16 FP64
+// multiply-adds (every c * Y product of two floats is exact in FP64), summed
+// over the chunk in list order in registers and then added into the FP64
+// copy of A in LDS.  fp and x are formed in FP64.
 //
 // Backward, one lane per neighbour: dE/dA[p][m] = Gc[p][l(m)] A[p][m] goes to
-// LDS once per atom; each lane runs over p with s_p = sum_m dE/dA[p][m] Y[m]
-// (dE/dc_p) and gY[m] += dE/dA[p][m] c_p, then dE/dfc = sum_p s_p Rnp[p] and
-// dE/dRn[n] = fc sum_p s_p W[t][p][n], and the chain rule of the tiled
-// backward gives g_j = -dE_i/d(r_j - r_i).
+// LDS once per atom (rows padded to a multiple of 4, read 16 bytes at a time
+// by every lane at once); each lane runs over p with s_p = sum_m dE/dA[p][m]
+// Y[m] (dE/dc_p) and gY[m] += dE/dA[p][m] c_p, then dE/dfc = sum_p s_p
+// Rnp[p] and dE/dRn[n] = fc sum_p s_p W[t][p][n], and the chain rule of the
+// tiled backward gives g_j = -dE_i/d(r_j - r_i).
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -46,21 +49,41 @@ __device__ __forceinline__ float mix_row(const float *__restrict__ w, int K, con
     return s;
 }
 
-// LDS: A accumulators (FP64, P*M), then Y (ch x MS floats, MS = M rounded up to
-// odd so the lanes' row writes fall on distinct banks), then c (ch x P floats).
+constexpr int kTile = 4;  // A tile edge of the forward
+__host__ __device__ constexpr int pad4(int v) { return (v + 3) & ~3; }
+
+// The centre species' mixing W[ntypes][P][K] is copied into LDS whenever the
+// workgroup's rows move to another segment.
+__device__ __forceinline__ void stage_w(const SegmentArgs &a, int wsz, float *sW) {
+    __syncthreads();  // every lane is done with the previous segment's copy
+    for (int e = threadIdx.x; e < wsz; e += 64) sW[e] = a.W[e];
+}
+
+// LDS: A accumulators (FP64, P*M), then Y (ch x MS floats), c (ch x PS
+// floats), MS and PS the row lengths M and P padded to a multiple of 4, and W.
 template <int LMAX>
 __global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl, int K, int P, int ch) {
-    constexpr int L1 = LMAX + 1, M = L1 * L1, MS = M | 1;
-    extern __shared__ double gen_smem[];
+    constexpr int L1 = LMAX + 1, M = L1 * L1, MS = pad4(M);
+    constexpr int NMB = MS / kTile;
+    extern __shared__ __align__(16) double gen_smem[];
     double *sA = gen_smem;
-    float *sY = reinterpret_cast<float *>(sA + (size_t)P * M);
+    const int PS = pad4(P);
+    float *sY = reinterpret_cast<float *>(sA + pad4(P * M));
     float *sC = sY + ch * MS;
+    float *sW = sC + ch * PS;
     const int lane = threadIdx.x;
     const int PM = P * M, S = P * L1;
+    const int ntile = (PS / kTile) * NMB;
+    const int wsz = set.a[0].ntypes * P * K;
+    int cur = -1;
     for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
         const int sg = seg_of(set, gr);
         const SegmentArgs &a = set.a[sg];
         const int row = gr - set.row0[sg];
+        if (sg != cur) {
+            stage_w(a, wsz, sW);
+            cur = sg;
+        }
         const float rc = a.rc, inv_rc = 1.f / a.rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
@@ -69,7 +92,7 @@ __global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl,
         for (int e = lane; e < PM; e += 64) sA[e] = 0.0;
         for (int c0 = 0; c0 < n; c0 += ch) {
             const int nc = min(ch, n - c0);
-            __syncthreads();  // the previous chunk's sums have read sY / sC
+            __syncthreads();  // the previous chunk's tiles have read sY / sC (and sA is cleared)
             if (lane < nc) {
                 const float4 dd = nl.d[base + c0 + lane];
                 const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
@@ -77,21 +100,39 @@ __global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl,
                 float y[M];
                 real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
 #pragma unroll
-                for (int m = 0; m < M; ++m) sY[lane * MS + m] = y[m];
+                for (int m = 0; m < MS; ++m) sY[lane * MS + m] = m < M ? y[m] : 0.f;
                 float rn[kGenKMax];
                 cheb_rn<kGenKMax>(1.f - 2.f * g.r * inv_rc, rn);
                 const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
                 const float t2 = t * t;
                 const float fc = g.r < rc ? t2 * t2 : 0.f;  // pairs beyond this species' cutoff add nothing
-                const float *w = a.W + (size_t)(tj - 1) * P * K;
-                for (int p = 0; p < P; ++p) sC[lane * P + p] = fc * mix_row(w + p * K, K, rn);
+                const float *w = sW + (tj - 1) * P * K;
+                for (int p = 0; p < PS; ++p) sC[lane * PS + p] = p < P ? fc * mix_row(w + p * K, K, rn) : 0.f;
             }
             __syncthreads();
-            for (int e = lane; e < PM; e += 64) {
-                const int p = e / M, m = e - p * M;
-                double acc = 0.0;
-                for (int j = 0; j < nc; ++j) acc += (double)sC[j * P + p] * (double)sY[j * MS + m];
-                sA[e] += acc;
+            for (int tl = lane; tl < ntile; tl += 64) {
+                const int p0 = (tl / NMB) * kTile, m0 = (tl % NMB) * kTile;
+                double acc[kTile][kTile];
+#pragma unroll
+                for (int u = 0; u < kTile; ++u)
+#pragma unroll
+                    for (int v = 0; v < kTile; ++v) acc[u][v] = 0.0;
+#pragma unroll 2
+                for (int j = 0; j < nc; ++j) {
+                    const float4 c4 = *reinterpret_cast<const float4 *>(sC + j * PS + p0);
+                    const float4 y4 = *reinterpret_cast<const float4 *>(sY + j * MS + m0);
+                    const double cv[kTile] = {c4.x, c4.y, c4.z, c4.w};
+                    const double yv[kTile] = {y4.x, y4.y, y4.z, y4.w};
+#pragma unroll
+                    for (int u = 0; u < kTile; ++u)
+#pragma unroll
+                        for (int v = 0; v < kTile; ++v) acc[u][v] += cv[u] * yv[v];
+                }
+#pragma unroll
+                for (int u = 0; u < kTile; ++u)
+#pragma unroll
+                    for (int v = 0; v < kTile; ++v)
+                        if (p0 + u < P && m0 + v < M) sA[(p0 + u) * M + m0 + v] += acc[u][v];
             }
         }
         __syncthreads();
@@ -109,28 +150,36 @@ __global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl,
     }
 }
 
-// LDS: dE/dA (P*M floats) of the current atom.
+// LDS: dE/dA of the current atom, P rows of MP floats (M padded to a
+// multiple of 4), then W.
 template <int LMAX>
 __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl, int K, int P) {
-    constexpr int L1 = LMAX + 1, M = L1 * L1;
-    extern __shared__ double gen_smem[];
+    constexpr int L1 = LMAX + 1, M = L1 * L1, MP = pad4(M);
+    extern __shared__ __align__(16) double gen_smem[];
     float *sGA = reinterpret_cast<float *>(gen_smem);
+    float *sW = sGA + P * MP;
     const int lane = threadIdx.x;
     const int PM = P * M, S = P * L1;
+    const int wsz = set.a[0].ntypes * P * K;
+    int cur = -1;
     for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
         const int sg = seg_of(set, gr);
         const SegmentArgs &a = set.a[sg];
         const int row = gr - set.row0[sg];
+        if (sg != cur) {
+            stage_w(a, wsz, sW);
+            cur = sg;
+        }
         const float rc = a.rc, inv_rc = 1.f / a.rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
         __syncthreads();  // the previous atom's pairs have read sGA
-        for (int e = lane; e < PM; e += 64) {
-            const int p = e / M, m = e - p * M;
+        for (int e = lane; e < P * MP; e += 64) {
+            const int p = e / MP, m = e - p * MP;
             int l = 0;
             while ((l + 1) * (l + 1) <= m) ++l;
-            sGA[e] = a.G[(size_t)row * S + p * L1 + l] * a.A[(size_t)row * PM + e];
+            sGA[e] = m < M ? a.G[(size_t)row * S + p * L1 + l] * a.A[(size_t)row * PM + p * M + m] : 0.f;
         }
         __syncthreads();
         for (int j = lane; j < n; j += 64) {
@@ -148,7 +197,7 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
                 const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
                 const float t2 = t * t;
                 const float fc = t2 * t2;
-                const float *w = a.W + (size_t)(tj - 1) * P * K;
+                const float *w = sW + (tj - 1) * P * K;
                 float gy[M];
 #pragma unroll
                 for (int m = 0; m < M; ++m) gy[m] = 0.f;
@@ -160,12 +209,20 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
                     const float *wp = w + p * K;
                     const float rnp = mix_row(wp, K, rn);
                     const float cp = fc * rnp;
-                    const float *ga = sGA + p * M;
+                    const float4 *ga4 = reinterpret_cast<const float4 *>(sGA + p * MP);
                     float s4[4] = {0.f, 0.f, 0.f, 0.f};  // shorter FP32 accumulation chains
 #pragma unroll
-                    for (int m = 0; m < M; ++m) {
-                        gy[m] += ga[m] * cp;
-                        s4[m & 3] += ga[m] * y[m];
+                    for (int mb = 0; mb < MP / 4; ++mb) {
+                        const float4 g4 = ga4[mb];
+                        const float ga[4] = {g4.x, g4.y, g4.z, g4.w};
+#pragma unroll
+                        for (int c = 0; c < 4; ++c) {
+                            const int m = 4 * mb + c;
+                            if (m < M) {
+                                gy[m] += ga[c] * cp;
+                                s4[c] += ga[c] * y[m];
+                            }
+                        }
                     }
                     const float sp = (s4[0] + s4[1]) + (s4[2] + s4[3]);
                     gfc += sp * rnp;
@@ -211,13 +268,22 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
 
 // Radial-only basis, forward: chunks of 64 neighbours put c_j[p] = fc Rnp[p]
 // into LDS; lane p (and p + 64) sums them in FP64, neighbours in list order.
+// LDS: c (64 x P floats), then W.
 __global__ __launch_bounds__(64) void k_gen_rad_fwd(SegmentSet set, NeighborList nl, int K, int P) {
-    __shared__ float sC[64 * kGenPMax];
+    extern __shared__ __align__(16) double gen_smem[];
+    float *sC = reinterpret_cast<float *>(gen_smem);
+    float *sW = sC + 64 * P;
     const int lane = threadIdx.x;
+    const int wsz = set.a[0].ntypes * P * K;
+    int cur = -1;
     for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
         const int sg = seg_of(set, gr);
         const SegmentArgs &a = set.a[sg];
         const int row = gr - set.row0[sg];
+        if (sg != cur) {
+            stage_w(a, wsz, sW);
+            cur = sg;
+        }
         const float rc = a.rc, inv_rc = 1.f / a.rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
@@ -235,7 +301,7 @@ __global__ __launch_bounds__(64) void k_gen_rad_fwd(SegmentSet set, NeighborList
                 const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
                 const float t2 = t * t;
                 const float fc = g.r < rc ? t2 * t2 : 0.f;
-                const float *w = a.W + (size_t)(tj - 1) * P * K;
+                const float *w = sW + (tj - 1) * P * K;
                 for (int p = 0; p < P; ++p) sC[lane * P + p] = fc * mix_row(w + p * K, K, rn);
             }
             __syncthreads();
@@ -252,13 +318,22 @@ __global__ __launch_bounds__(64) void k_gen_rad_fwd(SegmentSet set, NeighborList
 
 // Radial-only basis, backward: lane per pair, dE/dfc = sum_p G[p] Rnp[p] and
 // dE/dRn[n] = fc sum_p G[p] W[t][p][n] with G = dE/dfp of the atom.
+// LDS: G (P floats), then W.
 __global__ __launch_bounds__(64) void k_gen_rad_bwd(SegmentSet set, NeighborList nl, int K, int P) {
-    __shared__ float sG[kGenPMax];
+    extern __shared__ __align__(16) double gen_smem[];
+    float *sG = reinterpret_cast<float *>(gen_smem);
+    float *sW = sG + P;
     const int lane = threadIdx.x;
+    const int wsz = set.a[0].ntypes * P * K;
+    int cur = -1;
     for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
         const int sg = seg_of(set, gr);
         const SegmentArgs &a = set.a[sg];
         const int row = gr - set.row0[sg];
+        if (sg != cur) {
+            stage_w(a, wsz, sW);
+            cur = sg;
+        }
         const float rc = a.rc, inv_rc = 1.f / a.rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
@@ -279,7 +354,7 @@ __global__ __launch_bounds__(64) void k_gen_rad_bwd(SegmentSet set, NeighborList
                 const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
                 const float t2 = t * t;
                 const float fc = t2 * t2;
-                const float *w = a.W + (size_t)(tj - 1) * P * K;
+                const float *w = sW + (tj - 1) * P * K;
                 float gw[kGenKMax];
 #pragma unroll
                 for (int q = 0; q < kGenKMax; ++q) gw[q] = 0.f;
@@ -312,13 +387,14 @@ int gen_grid(const SegmentSet &set) {
     return rows < 262144 ? rows : 262144;
 }
 
-// LDS of the forward for a chunk of ch neighbours
-size_t gen_fwd_lds(int M, int P, int ch) {
-    return sizeof(double) * (size_t)P * M + sizeof(float) * (size_t)ch * ((M | 1) + P);
+// LDS of the forward for a chunk of ch neighbours, wsz floats of W
+size_t gen_fwd_lds(int M, int P, int ch, int wsz) {
+    return sizeof(double) * (size_t)pad4(P * M) + sizeof(float) * ((size_t)ch * (pad4(M) + pad4(P)) + wsz);
 }
 
 template <int LMAX>
 bool run_gen(hipStream_t s, int K, int P, const SegmentSet &set, const NeighborList &nl, bool backward) {
+    const int wsz = set.a[0].ntypes * P * K;
     constexpr int M = (LMAX + 1) * (LMAX + 1);
     static bool attr_set = false;
     if (!attr_set) {
@@ -330,15 +406,15 @@ bool run_gen(hipStream_t s, int K, int P, const SegmentSet &set, const NeighborL
     }
     const int grid = gen_grid(set);
     if (backward) {
-        const size_t lds = sizeof(float) * (size_t)P * M;
+        const size_t lds = sizeof(float) * ((size_t)P * pad4(M) + wsz);
         if (lds > kGenLds) return false;
         hipLaunchKernelGGL((k_gen_bwd<LMAX>), dim3(grid), dim3(64), lds, s, set, nl, K, P);
         return true;
     }
     int ch = 64;  // the largest chunk that fits
-    while (ch > 8 && gen_fwd_lds(M, P, ch) > kGenLds) ch /= 2;
-    if (gen_fwd_lds(M, P, ch) > kGenLds) return false;
-    hipLaunchKernelGGL((k_gen_fwd<LMAX>), dim3(grid), dim3(64), gen_fwd_lds(M, P, ch), s, set, nl, K, P, ch);
+    while (ch > 8 && gen_fwd_lds(M, P, ch, wsz) > kGenLds) ch /= 2;
+    if (gen_fwd_lds(M, P, ch, wsz) > kGenLds) return false;
+    hipLaunchKernelGGL((k_gen_fwd<LMAX>), dim3(grid), dim3(64), gen_fwd_lds(M, P, ch, wsz), s, set, nl, K, P, ch);
     return true;
 }
 
@@ -349,9 +425,20 @@ bool launch_desc_gen_set(hipStream_t s, int lmax, int K, int P, const SegmentSet
     if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
     if (K < 1 || K > kGenKMax || P < 1 || P > kGenPMax) return false;
     if (lmax == kRadialLmax) {
+        static bool attr_set = false;
+        if (!attr_set) {
+            JNN_HIP(hipFuncSetAttribute((const void *)k_gen_rad_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)kGenLds));
+            JNN_HIP(hipFuncSetAttribute((const void *)k_gen_rad_bwd, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                        (int)kGenLds));
+            attr_set = true;
+        }
+        const size_t wsz = (size_t)set.a[0].ntypes * P * K;
+        const size_t lds = sizeof(float) * ((backward ? (size_t)P : 64 * (size_t)P) + wsz);
+        if (lds > kGenLds) return false;
         const int grid = gen_grid(set);
-        if (backward) hipLaunchKernelGGL(k_gen_rad_bwd, dim3(grid), dim3(64), 0, s, set, nl, K, P);
-        else hipLaunchKernelGGL(k_gen_rad_fwd, dim3(grid), dim3(64), 0, s, set, nl, K, P);
+        if (backward) hipLaunchKernelGGL(k_gen_rad_bwd, dim3(grid), dim3(64), lds, s, set, nl, K, P);
+        else hipLaunchKernelGGL(k_gen_rad_fwd, dim3(grid), dim3(64), lds, s, set, nl, K, P);
         return true;
     }
     switch (lmax) {

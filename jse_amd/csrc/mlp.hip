@@ -59,6 +59,8 @@ __global__ __launch_bounds__(256) void k_mlp_fp32(MlpArgs a) {
                 float acc[kUB];
 #pragma unroll
                 for (int u = 0; u < kUB; ++u) acc[u] = (o0 + u < out) ? bs[o0 + u] : 0.f;
+                // unrolled so that several rows of weight reads are in flight at once
+#pragma unroll 4
                 for (int i = 0; i < in; ++i) {
                     const float v = smem[ain + i * kMlpTile + lane];
                     const float *w = wf + (size_t)i * out + o0;
@@ -116,6 +118,7 @@ __global__ __launch_bounds__(256) void k_mlp_fp32(MlpArgs a) {
                     float acc[kUB];
 #pragma unroll
                     for (int u = 0; u < kUB; ++u) acc[u] = 0.f;
+#pragma unroll 4
                     for (int o = 0; o < out; ++o) {
                         const float v = smem[gout + o * kMlpTile + lane];
                         const float *w = wb + (size_t)o * inl + i0;

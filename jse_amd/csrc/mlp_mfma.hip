@@ -27,6 +27,8 @@
 // B[k][j] from lane j + 16*(k/E), D[i][j] in lane j + 16*(i/4), register i%4,
 // with E = 1 (f32, 4 k per instruction) or 8 consecutive k per lane (bf16, 32 k).
 // SiLU, biases, the energy and the output scaling stay FP32 in both.
+#include <type_traits>
+
 #include "common.h"
 #include "kernels.h"
 
@@ -408,7 +410,7 @@ template <typename T>
 static bool dispatch_mlp2(hipStream_t s, const MlpArgs &a) {
     if (a.nlayers != 2) return false;
     const int h1 = a.hidden[0], h2 = a.hidden[1];
-    const int kp = a.S <= 64 ? 64 : (a.S <= 128 ? 128 : (a.S <= 256 ? 256 : 0));
+    const int kp = a.S <= 64 ? 64 : (a.S <= 128 ? 128 : (a.S <= 256 ? 256 : (a.S <= 384 ? 384 : 0)));
 #define JNN_MLP_SHAPE(H1_, H2_, KP_)                    \
     if (h1 == H1_ && h2 == H2_ && kp == KP_) {          \
         run_mlp2<T, H1_ / 16, H2_ / 16, KP_>(s, a);     \
@@ -418,6 +420,12 @@ static bool dispatch_mlp2(hipStream_t s, const MlpArgs &a) {
     JNN_MLP_SHAPE(32, 32, 128)
     JNN_MLP_SHAPE(64, 64, 64)
     JNN_MLP_SHAPE(64, 64, 128)
+    if constexpr (std::is_same<T, float>::value) {  // wider inputs (FP32 only: the BF16 tiles' LDS would not fit)
+        JNN_MLP_SHAPE(32, 32, 256)
+        JNN_MLP_SHAPE(32, 32, 384)
+        JNN_MLP_SHAPE(64, 64, 256)
+        JNN_MLP_SHAPE(64, 64, 384)
+    }
 #undef JNN_MLP_SHAPE
     return false;
 }
@@ -430,7 +438,7 @@ template <typename T>
 static bool dispatch_mlp2_multi(hipStream_t s, const MlpMulti &m) {
     const MlpArgs &a = m.a[0];
     const int h1 = a.hidden[0], h2 = a.hidden[1];
-    const int kp = a.S <= 64 ? 64 : (a.S <= 128 ? 128 : (a.S <= 256 ? 256 : 0));
+    const int kp = a.S <= 64 ? 64 : (a.S <= 128 ? 128 : (a.S <= 256 ? 256 : (a.S <= 384 ? 384 : 0)));
 #define JNN_MLP_SHAPE(H1_, H2_, KP_)                        \
     if (h1 == H1_ && h2 == H2_ && kp == KP_) {              \
         run_mlp2_multi<T, H1_ / 16, H2_ / 16, KP_>(s, m);   \
@@ -440,6 +448,12 @@ static bool dispatch_mlp2_multi(hipStream_t s, const MlpMulti &m) {
     JNN_MLP_SHAPE(32, 32, 128)
     JNN_MLP_SHAPE(64, 64, 64)
     JNN_MLP_SHAPE(64, 64, 128)
+    if constexpr (std::is_same<T, float>::value) {  // wider inputs (FP32 only: the BF16 tiles' LDS would not fit)
+        JNN_MLP_SHAPE(32, 32, 256)
+        JNN_MLP_SHAPE(32, 32, 384)
+        JNN_MLP_SHAPE(64, 64, 256)
+        JNN_MLP_SHAPE(64, 64, 384)
+    }
 #undef JNN_MLP_SHAPE
     return false;
 }

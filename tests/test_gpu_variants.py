@@ -130,3 +130,19 @@ def test_untiled_radial_shape(O):
     assert [s.P for s in M.load_model(md).species] == [39, 39]
     md = W._normalized(md, st, 200, 37)
     _check(O, md, st)
+
+
+@pytest.mark.parametrize("wtype,fuse_size", [("fuse", 3), ("exfuse", 2)])
+def test_fuse_wtypes(O, wtype, fuse_size):
+    """fuse / exfuse mixing (P = 3K here): the network is drawn for the full
+    mixing of three species (also P = 3K), then the basis switches to the fuse
+    weights."""
+    st = _liquid(["Cu", "Zr", "Al"], 1000, 28)
+    md = M.random_model(["Cu", "Zr", "Al"], nmax=6, lmax=5, rcut=6.0, hidden=(32, 32), seed=38, wtype="full")
+    rng = np.random.default_rng(38)
+    for m in md["models"]:
+        m["basis"]["wtype"] = wtype
+        m["basis"]["fuse_weight"] = rng.normal(size=3 * fuse_size).tolist()
+    assert [s.P for s in M.load_model(md).species] == [21] * 3
+    md = W._normalized(md, st, 200, 38)
+    _check(O, md, st)

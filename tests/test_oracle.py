@@ -197,6 +197,36 @@ def test_full_and_exfull_wtype_structure():
         assert np.array_equal(ex.W[t - 1], want)
 
 
+def test_fuse_and_exfuse_wtype_structure():
+    """WTypeBasis.updateParameters_ (WTypeBasis.java:239-264): "fuse" puts
+    fuse_weight[t][k] on the diagonal of block k (P = fuse_size K); "exfuse"
+    adds an identity block 0 and shifts the fuse blocks by one.  Both are
+    drawn with P = 2K (the network of the full mixing of two species)."""
+    K = 4
+    def sp(wtype, fw):
+        md = M.random_model(["A", "B"], 3, 2, 5.0, (4,), seed=1, wtype="full")
+        for m in md["models"]:
+            m["basis"]["wtype"] = wtype
+            m["basis"]["fuse_weight"] = fw
+        return M.load_model(md).species[0]
+    fw = [0.5, -1.25, 2.0, 0.75]   # [type][k], fuse_size 2
+    fu = sp("fuse", fw)
+    assert fu.W.shape == (2, 2 * K, K)
+    for t in (1, 2):
+        want = np.zeros((2 * K, K))
+        for k in range(2):
+            want[k * K:(k + 1) * K] = fw[(t - 1) * 2 + k] * np.eye(K)
+        assert np.array_equal(fu.W[t - 1], want)
+    fw1 = [-1.5, 0.25]             # fuse_size 1
+    ex = sp("exfuse", fw1)
+    assert ex.W.shape == (2, 2 * K, K)
+    for t in (1, 2):
+        want = np.zeros((2 * K, K))
+        want[:K] = np.eye(K)
+        want[K:] = fw1[t - 1] * np.eye(K)
+        assert np.array_equal(ex.W[t - 1], want)
+
+
 def test_addition_theorem_at_the_shape_limits(O):
     """lmax = 12 and nmax = 20 (the loaders' limits) with the exfull mixing:
     the closed form sum_{j,k} c_j c_k P_l(cos gamma_jk) still holds."""
