@@ -16,11 +16,13 @@
 // copy of A in LDS.  fp and x are formed in FP64.
 //
 // Backward, one lane per neighbour: dE/dA[p][m] = Gc[p][l(m)] A[p][m] goes to
-// LDS once per atom (rows padded to a multiple of 4, read 16 bytes at a time
-// by every lane at once); each lane runs over p with s_p = sum_m dE/dA[p][m]
-// Y[m] (dE/dc_p) and gY[m] += dE/dA[p][m] c_p, then dE/dfc = sum_p s_p
-// Rnp[p] and dE/dRn[n] = fc sum_p s_p W[t][p][n], and the chain rule of the
-// tiled backward gives g_j = -dE_i/d(r_j - r_i).
+// LDS once per atom and is folded through every species' mixing, Gt[t][n][m]
+// = sum_p W[t][p][n] dE/dA[p][m] (c_j is linear in Rn), as in the tiled
+// backward; a lane then forms gY[m] = fc sum_n Rn[n] Gt[n][m] and gct[n] =
+// sum_m Gt[n][m] Y[m] (2KM multiply-adds per pair for any P, rows read 16
+// bytes at a time by every lane at once), dE/dRn = fc gct, dE/dfc = sum_n
+// Rn[n] gct[n], and the chain rule of the tiled backward gives g_j =
+// -dE_i/d(r_j - r_i).
 #include <hip/hip_runtime.h>
 
 #include "common.h"
@@ -150,17 +152,20 @@ __global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl,
     }
 }
 
-// LDS: dE/dA of the current atom, P rows of MP floats (M padded to a
-// multiple of 4), then W.
+// LDS: dE/dA of the current atom (P rows of MP floats, M padded to a multiple
+// of 4), its fold through each species' mixing, Gt[t][n][m] = sum_p W[t][p][n]
+// dE/dA[p][m] (ntypes x K rows of MP floats), then W.
 template <int LMAX>
 __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl, int K, int P) {
     constexpr int L1 = LMAX + 1, M = L1 * L1, MP = pad4(M);
     extern __shared__ __align__(16) double gen_smem[];
+    const int nt = set.a[0].ntypes;
     float *sGA = reinterpret_cast<float *>(gen_smem);
-    float *sW = sGA + P * MP;
+    float *sGt = sGA + P * MP;
+    float *sW = sGt + nt * K * MP;
     const int lane = threadIdx.x;
     const int PM = P * M, S = P * L1;
-    const int wsz = set.a[0].ntypes * P * K;
+    const int wsz = nt * P * K;
     int cur = -1;
     for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
         const int sg = seg_of(set, gr);
@@ -174,12 +179,21 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
-        __syncthreads();  // the previous atom's pairs have read sGA
+        __syncthreads();  // the previous atom's pairs have read sGt
         for (int e = lane; e < P * MP; e += 64) {
             const int p = e / MP, m = e - p * MP;
             int l = 0;
             while ((l + 1) * (l + 1) <= m) ++l;
             sGA[e] = m < M ? a.G[(size_t)row * S + p * L1 + l] * a.A[(size_t)row * PM + p * M + m] : 0.f;
+        }
+        __syncthreads();
+        for (int e = lane; e < nt * K * MP; e += 64) {
+            const int tq = e / MP, m = e - tq * MP;  // tq = t K + n
+            const int t = tq / K, q = tq - t * K;
+            const float *w = sW + t * P * K + q;
+            float acc = 0.f;
+            for (int p = 0; p < P; ++p) acc += w[p * K] * sGA[p * MP + m];
+            sGt[e] = acc;
         }
         __syncthreads();
         for (int j = lane; j < n; j += 64) {
@@ -197,39 +211,38 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
                 const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
                 const float t2 = t * t;
                 const float fc = t2 * t2;
-                const float *w = sW + (tj - 1) * P * K;
+                const float *gt = sGt + (tj - 1) * K * MP;
+                // gY[m] = fc sum_n Rn[n] Gt[n][m],  gct[n] = sum_m Gt[n][m] Y[m]
+                // (then dE/dRn = fc gct, dE/dfc = sum_n Rn[n] gct[n])
                 float gy[M];
 #pragma unroll
                 for (int m = 0; m < M; ++m) gy[m] = 0.f;
                 float grn[kGenKMax];
-#pragma unroll
-                for (int q = 0; q < kGenKMax; ++q) grn[q] = 0.f;
                 float gfc = 0.f;
-                for (int p = 0; p < P; ++p) {
-                    const float *wp = w + p * K;
-                    const float rnp = mix_row(wp, K, rn);
-                    const float cp = fc * rnp;
-                    const float4 *ga4 = reinterpret_cast<const float4 *>(sGA + p * MP);
-                    float s4[4] = {0.f, 0.f, 0.f, 0.f};  // shorter FP32 accumulation chains
 #pragma unroll
-                    for (int mb = 0; mb < MP / 4; ++mb) {
-                        const float4 g4 = ga4[mb];
-                        const float ga[4] = {g4.x, g4.y, g4.z, g4.w};
+                for (int q = 0; q < kGenKMax; ++q) {
+                    grn[q] = 0.f;
+                    if (q < K) {
+                        const float4 *g4 = reinterpret_cast<const float4 *>(gt + q * MP);
+                        const float cq = fc * rn[q];
+                        float s4[4] = {0.f, 0.f, 0.f, 0.f};  // shorter FP32 accumulation chains
 #pragma unroll
-                        for (int c = 0; c < 4; ++c) {
-                            const int m = 4 * mb + c;
-                            if (m < M) {
-                                gy[m] += ga[c] * cp;
-                                s4[c] += ga[c] * y[m];
+                        for (int mb = 0; mb < MP / 4; ++mb) {
+                            const float4 v4 = g4[mb];
+                            const float ga[4] = {v4.x, v4.y, v4.z, v4.w};
+#pragma unroll
+                            for (int c = 0; c < 4; ++c) {
+                                const int m = 4 * mb + c;
+                                if (m < M) {
+                                    gy[m] += ga[c] * cq;
+                                    s4[c] += ga[c] * y[m];
+                                }
                             }
                         }
+                        const float sq = (s4[0] + s4[1]) + (s4[2] + s4[3]);
+                        gfc += sq * rn[q];
+                        grn[q] = sq * fc;
                     }
-                    const float sp = (s4[0] + s4[1]) + (s4[2] + s4[3]);
-                    gfc += sp * rnp;
-                    const float sf = sp * fc;
-#pragma unroll
-                    for (int q = 0; q < kGenKMax; ++q)
-                        if (q < K) grn[q] += wp[q] * sf;
                 }
                 float drn[kGenKMax];
                 cheb_drn<kGenKMax>(xr, inv_rc, drn);
@@ -406,7 +419,7 @@ bool run_gen(hipStream_t s, int K, int P, const SegmentSet &set, const NeighborL
     }
     const int grid = gen_grid(set);
     if (backward) {
-        const size_t lds = sizeof(float) * ((size_t)P * pad4(M) + wsz);
+        const size_t lds = sizeof(float) * ((size_t)(P + set.a[0].ntypes * K) * pad4(M) + wsz);
         if (lds > kGenLds) return false;
         hipLaunchKernelGGL((k_gen_bwd<LMAX>), dim3(grid), dim3(64), lds, s, set, nl, K, P);
         return true;
