@@ -148,3 +148,43 @@ def test_autotuned_launch_shapes_keep_results():
             assert abs(r2["energy"] - r0["energy"]) / n <= 1e-6
             assert de.max() <= 2e-5 and df.max() <= 2e-4
         b.close()
+
+
+def test_untiled_shape_in_captured_graphs(O):
+    """A model without tiled kernels (nmax 8, lmax 7, exfull mixing) in the
+    batched path: graph replays equal kernel-by-kernel evaluation, and the
+    replicas meet the FP32 bar against the oracle."""
+    reps = _reps(6, 64, 500, seed=14)
+    md = M.random_model(["Cu", "Zr", "Al"], nmax=8, lmax=7, rcut=6.0, hidden=(32, 32), seed=44, wtype="exfull")
+    md = W._normalized(md, reps[-1], 200, 44)
+    pot = nnap.NNAP(md)
+    b = pot.batch([_atoms(r) for r in reps])
+    eager = nnap.System(pot, b.natoms, b.types, b.box, nnap.FP32, natoms_per_replica=b.rep_natoms)
+    eager.set_graphs(False)
+    xyz = np.concatenate([r.xyz for r in reps])
+    rng = np.random.default_rng(1)
+    for step in range(3):
+        moved = xyz + (rng.normal(0.0, 0.01, size=xyz.shape) if step else 0.0)
+        b.set_positions(moved)
+        eager.set_positions(moved)
+        b.compute()
+        eager.compute()
+        rg, re_ = b.results(), eager.results()
+        for k in ("energies", "forces", "virials"):
+            assert np.array_equal(rg[k], re_[k]), f"step {step} {k}"
+    b.set_positions(xyz)
+    b.compute()
+    full = b.results()
+    e_rep, v_rep = b.replica_results()
+    eager.close()
+    b.close()
+    pot.close()
+    orc = O.Oracle(M.load_model(md))
+    off = 0
+    for k, r in enumerate(reps):
+        ref = orc.system_efv(r)
+        sl = slice(off, off + r.natoms)
+        assert abs(e_rep[k] - ref["energy"]) / r.natoms <= 1e-6
+        assert np.sqrt(((full["forces"][sl] - ref["forces"]) ** 2).mean()) <= 1e-5
+        assert np.abs(v_rep[k] - ref["virial"]).max() / r.natoms <= 1e-5
+        off += r.natoms
