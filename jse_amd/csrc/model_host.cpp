@@ -206,7 +206,8 @@ ModelHost parse_model_json(const std::string &text) {
         sp.nmax = (int)num(b->get("nmax"), 5);
         sp.lmax = sp.radial ? 0 : (int)num(b->get("lmax"), 6);
         JNN_CHECK(sp.lmax >= 0 && sp.lmax <= 12, ERR_ARG, "Input lmax MUST be in [0, 12], input: " + std::to_string(sp.lmax));
-        JNN_CHECK(sp.nmax >= 0 && sp.nmax < 32, ERR_ARG, "Input nmax out of range");
+        JNN_CHECK(sp.nmax >= 0 && sp.nmax <= 20, ERR_ARG,  // WTypeBasis.java:56
+                  "Input nmax MUST be in [0, 20], input: " + std::to_string(sp.nmax));
         JNN_CHECK(sp.radial || ((int)num(b->get("l3max"), 0) == 0 && (int)num(b->get("l4max"), 0) == 0), ERR_UNSUPPORTED,
                   "l3max/l4max > 0 are not supported by this build");
         sp.rcut = num(b->get("rcut"), 6.0);

@@ -251,6 +251,8 @@ def load_model(src) -> Model:
         l4 = 0 if radial else int(_get(b, "l4max", default=0))
         if lmax < 0 or lmax > 12:
             raise ValueError(f"Input lmax MUST be in [0, 12], input: {lmax}")
+        if nmax < 0 or nmax > 20:   # WTypeBasis.java:56
+            raise ValueError(f"Input nmax MUST be in [0, 20], input: {nmax}")
         if l3 != 0 or l4 != 0:
             raise ValueError("l3max/l4max > 0 are not supported by this build")
         rcut = float(_get(b, "rcut", default=DEFAULT_RCUT))

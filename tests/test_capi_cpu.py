@@ -130,6 +130,8 @@ def test_parser_errors():
     assert rc == 1 and b"Unsupported version" in L.jse_nnap_last_error()
     rc, *_ = _parse('{"models": [{"symbol": "Cu", "basis": {"lmax": 13}}]}')
     assert rc == 1 and b"lmax MUST be in [0, 12]" in L.jse_nnap_last_error()
+    rc, *_ = _parse('{"models": [{"symbol": "Cu", "basis": {"nmax": 21}}]}')
+    assert rc == 1 and b"nmax MUST be in [0, 20]" in L.jse_nnap_last_error()
     rc, *_ = _parse("/nonexistent/model.json")
     assert rc == 1 and b"cannot open model file" in L.jse_nnap_last_error()
     rc, *_ = _parse('{"models": [')
