@@ -263,6 +263,29 @@ def copy_calibration(nbytes, reps=5):
     print(json.dumps({"copy_bytes": x.numel() * 4, "reps": reps}), flush=True)
 
 
+def untiled_shapes(natoms, steps, warmup):
+    """Phase times (ms) of E+F+virial on the ternary liquid for model shapes
+    without tiled descriptor kernels, beside the tiled nmax 8, lmax 6,
+    default-mixing shape; networks 64-64, seeded random weights, default
+    launch shapes (no autotuning)."""
+    st = W.ternary_liquid(n=natoms, seed=7)
+    sample = W.ternary_liquid(n=2000, seed=7)
+    names = ["neighbor_build", "reverse_index", "desc_fwd", "mlp", "desc_bwd", "force_gather"]
+    what = nnap.ENERGY | nnap.FORCE | nnap.VIRIAL
+    for nmax, lmax, wtype in ((8, 6, "default"), (8, 6, "full"), (8, 6, "exfull"), (5, 8, "default")):
+        md = M.random_model(["Cu", "Zr", "Al"], nmax=nmax, lmax=lmax, rcut=6.0, hidden=(64, 64), seed=13,
+                            wtype=wtype)
+        md = W._normalized(md, sample, 300, 13)
+        with nnap.NNAP(md) as pot:
+            sysm = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+            sysm.compute(what)
+            ms, ph = sysm.benchmark(what, warmup=warmup, iters=steps)
+            sp = pot.model.species[0]
+            print(json.dumps({"nmax": nmax, "lmax": lmax, "wtype": wtype, "P": sp.P, "ms_per_step": ms,
+                              "atom_evals_per_s": natoms / (ms * 1e-3),
+                              "phase_ms": {k: float(v) for k, v in zip(names, ph)}}), flush=True)
+
+
 def committed_traffic(kernel):
     """HBM bytes that `kernel` moves per evaluation (its species launches
     together), from the committed counter summary
@@ -405,6 +428,9 @@ def main():
     ap.add_argument("--knobs", type=str, default="",
                     help="JSON launch shapes for the 32,000-atom system instead of autotuning it "
                          "(profiling runs reuse the shapes a full run chose)")
+    ap.add_argument("--shapes", action="store_true",
+                    help="only time model shapes of the untiled descriptor kernels on the 32,000-atom liquid "
+                         "(one JSON line per shape, the tiled default shape first)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -415,6 +441,9 @@ def main():
         sys.exit(2)
     if args.copy_bytes:
         copy_calibration(args.copy_bytes)
+        return
+    if args.shapes:
+        untiled_shapes(args.natoms, args.steps, args.warmup)
         return
 
     md = W.ternary_model()
