@@ -42,7 +42,7 @@ __device__ __forceinline__ int seg_of(const SegmentSet &set, int gr) {
     return sg;
 }
 
-// c[p] = fc * sum_{n < K} w[p][n] rn[n] for one mixing row (w = W[t] + p K)
+// Rnp[p] = sum_{n < K} w[p][n] rn[n] for one mixing row (w = W[t] + p K)
 __device__ __forceinline__ float mix_row(const float *__restrict__ w, int K, const float *rn) {
     float s = 0.f;
 #pragma unroll
@@ -409,14 +409,15 @@ template <int LMAX>
 bool run_gen(hipStream_t s, int K, int P, const SegmentSet &set, const NeighborList &nl, bool backward) {
     const int wsz = set.a[0].ntypes * P * K;
     constexpr int M = (LMAX + 1) * (LMAX + 1);
-    static bool attr_set = false;
-    if (!attr_set) {
+    // once per process (thread-safe static initialisation; retried if it throws)
+    static const bool attr_set = [] {
         JNN_HIP(hipFuncSetAttribute((const void *)k_gen_fwd<LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)kGenLds));
         JNN_HIP(hipFuncSetAttribute((const void *)k_gen_bwd<LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)kGenLds));
-        attr_set = true;
-    }
+        return true;
+    }();
+    (void)attr_set;
     const int grid = gen_grid(set);
     if (backward) {
         const size_t lds = sizeof(float) * ((size_t)(P + set.a[0].ntypes * K) * pad4(M) + wsz);
@@ -438,14 +439,14 @@ bool launch_desc_gen_set(hipStream_t s, int lmax, int K, int P, const SegmentSet
     if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
     if (K < 1 || K > kGenKMax || P < 1 || P > kGenPMax) return false;
     if (lmax == kRadialLmax) {
-        static bool attr_set = false;
-        if (!attr_set) {
+        static const bool attr_set = [] {
             JNN_HIP(hipFuncSetAttribute((const void *)k_gen_rad_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)kGenLds));
             JNN_HIP(hipFuncSetAttribute((const void *)k_gen_rad_bwd, hipFuncAttributeMaxDynamicSharedMemorySize,
                                         (int)kGenLds));
-            attr_set = true;
-        }
+            return true;
+        }();
+        (void)attr_set;
         const size_t wsz = (size_t)set.a[0].ntypes * P * K;
         const size_t lds = sizeof(float) * ((backward ? (size_t)P : 64 * (size_t)P) + wsz);
         if (lds > kGenLds) return false;

@@ -10,6 +10,7 @@
 // output units and reads the weights at wave-uniform addresses (scalar loads),
 // so every LDS read feeds 8 FMAs.  The output is Gc = 2*4pi/(2l+1) dE/dfp in
 // the [row][S] layout the descriptor backward reads.
+#include "common.h"
 #include "kernels.h"
 
 namespace jnn {
@@ -59,8 +60,6 @@ __global__ __launch_bounds__(256) void k_mlp_fp32(MlpArgs a) {
                 float acc[kUB];
 #pragma unroll
                 for (int u = 0; u < kUB; ++u) acc[u] = (o0 + u < out) ? bs[o0 + u] : 0.f;
-                // unrolled so that several rows of weight reads are in flight at once
-#pragma unroll 4
                 for (int i = 0; i < in; ++i) {
                     const float v = smem[ain + i * kMlpTile + lane];
                     const float *w = wf + (size_t)i * out + o0;
@@ -118,7 +117,6 @@ __global__ __launch_bounds__(256) void k_mlp_fp32(MlpArgs a) {
                     float acc[kUB];
 #pragma unroll
                     for (int u = 0; u < kUB; ++u) acc[u] = 0.f;
-#pragma unroll 4
                     for (int o = 0; o < out; ++o) {
                         const float v = smem[gout + o * kMlpTile + lane];
                         const float *w = wb + (size_t)o * inl + i0;
@@ -159,11 +157,11 @@ bool launch_mlp_fp32(hipStream_t s, const MlpArgs &a) {
     if (a.nrows <= 0) return true;
     const size_t lds = mlp_lds_bytes(a);
     if (lds > 160 * 1024) return false;
-    static bool attr_set = false;
-    if (!attr_set) {
-        (void)hipFuncSetAttribute((const void *)k_mlp_fp32, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-        attr_set = true;
-    }
+    static const bool attr_set = [] {  // once per process (thread-safe static initialisation)
+        JNN_HIP(hipFuncSetAttribute((const void *)k_mlp_fp32, hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024));
+        return true;
+    }();
+    (void)attr_set;
     const int tiles = (a.nrows + kMlpTile - 1) / kMlpTile;
     hipLaunchKernelGGL(k_mlp_fp32, dim3(tiles), dim3(256), lds, s, a);
     return true;

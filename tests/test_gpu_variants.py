@@ -3,8 +3,8 @@
 "full", P = 2K, each species' channels in its own block) and lmax = 4 with
 nmax = 5 (one species: no mixing; three species: the default mixing),
 mirror / share bases, a post-fuse weight, the radial-only chebyshev basis,
-and shapes without a tiled kernel (the untiled kernels: lmax 0-12, nmax up
-to 20, any wtype).
+shapes without a tiled kernel (the untiled kernels: lmax 0-12, nmax up to
+20, any wtype), and a network without matrix-core tiles (the vector MLP).
 Plain FP32 bar: 1e-6 eV/atom, 1e-5 eV/A RMS forces, 1e-5 eV/atom virial."""
 import numpy as np
 import pytest
@@ -145,4 +145,13 @@ def test_fuse_wtypes(O, wtype, fuse_size):
         m["basis"]["fuse_weight"] = rng.normal(size=3 * fuse_size).tolist()
     assert [s.P for s in M.load_model(md).species] == [21] * 3
     md = W._normalized(md, st, 200, 38)
+    _check(O, md, st)
+
+
+def test_network_without_matrix_core_tiles(O):
+    """Three hidden layers of widths 48-24-16 (no matrix-core tiles): the FP32
+    vector MLP, behind the tiled descriptor of the benchmark shape."""
+    st = _liquid(["Cu", "Zr", "Al"], 1000, 29)
+    md = M.random_model(["Cu", "Zr", "Al"], nmax=8, lmax=6, rcut=6.0, hidden=(48, 24, 16), seed=39)
+    md = W._normalized(md, st, 200, 39)
     _check(O, md, st)
