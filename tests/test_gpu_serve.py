@@ -231,3 +231,25 @@ def test_server_errors():
         # the server still serves after the rejected calls
         res = srv.wait(srv.submit(rid, st.xyz))
         assert np.isfinite(res["energy"])
+
+
+def test_served_untiled_shape_matches_oracle(O):
+    """A model without tiled descriptor kernels (nmax 6, lmax 5, full mixing
+    of three species) served by continuous batching: every result meets the
+    FP32 bar of the oracle."""
+    reps = _replica_set(3, 150, 1200, 81)
+    md = M.random_model(["Cu", "Zr", "Al"], nmax=6, lmax=5, rcut=6.0, hidden=(32, 32), seed=45, wtype="full")
+    md = W._normalized(md, max(reps, key=lambda r: r.natoms), 200, 45)
+    orc = O.Oracle(M.load_model(md))
+    with nnap.NNAP(md) as pot, pot.server(precision=nnap.FP32, max_batch_atoms=8192) as srv:
+        rids = [srv.add_replica(nnap.AtomData(r.box, r.xyz, r.types)) for r in reps]
+        trajs = [_trajectory(r, 3, 0.01, 500 + k) for k, r in enumerate(reps)]
+        for s in range(3):
+            tks = [srv.submit(rid, trajs[k][s]) for k, rid in enumerate(rids)]
+            for k, tk in enumerate(tks):
+                res = srv.wait(tk)
+                r = reps[k]
+                ref = orc.system_efv(W.Structure(r.box, trajs[k][s], r.types))
+                e, f_rms, f_max, v = _bar(res, ref, r.natoms)
+                print(f"step {s} replica {k} n={r.natoms}: e {e:.2e} f_rms {f_rms:.2e} v {v:.2e}", flush=True)
+                assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-4 and v <= 1e-5
