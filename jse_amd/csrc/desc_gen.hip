@@ -109,6 +109,7 @@ __global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl,
                 const float t2 = t * t;
                 const float fc = g.r < rc ? t2 * t2 : 0.f;  // pairs beyond this species' cutoff add nothing
                 const float *w = sW + (tj - 1) * P * K;
+#pragma unroll 4
                 for (int p = 0; p < PS; ++p) sC[lane * PS + p] = p < P ? fc * mix_row(w + p * K, K, rn) : 0.f;
             }
             __syncthreads();
@@ -192,6 +193,7 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
             const int t = tq / K, q = tq - t * K;
             const float *w = sW + t * P * K + q;
             float acc = 0.f;
+#pragma unroll 8
             for (int p = 0; p < P; ++p) acc += w[p * K] * sGA[p * MP + m];
             sGt[e] = acc;
         }
