@@ -7,9 +7,9 @@
 // tiled kernels (basis_SphericalChebyshev.hpp:264-465, basis_Chebyshev.hpp:
 // 9-80); only the work layout differs:
 //
-// Forward, one wavefront per atom: chunks of `ch` neighbours, one lane each,
+// Forward, four wavefronts per atom: chunks of `ch` neighbours, one lane each,
 // put c_j[p] = fc W[t_j] Rn and Y_j into LDS (rows padded with zeros to a
-// multiple of 4); A (P x M) is cut into 4 x 4 tiles, which the lanes take
+// multiple of 4); A (P x M) is cut into 4 x 4 tiles, which the 256 lanes take
 // round-robin: per neighbour a tile costs two 16-byte LDS reads and 16 FP64
 // multiply-adds (every c * Y product of two floats is exact in FP64), summed
 // over the chunk in list order in registers and then added into the FP64
@@ -58,13 +58,14 @@ __host__ __device__ constexpr int pad4(int v) { return (v + 3) & ~3; }
 // workgroup's rows move to another segment.
 __device__ __forceinline__ void stage_w(const SegmentArgs &a, int wsz, float *sW) {
     __syncthreads();  // every lane is done with the previous segment's copy
-    for (int e = threadIdx.x; e < wsz; e += 64) sW[e] = a.W[e];
+    for (int e = threadIdx.x; e < wsz; e += blockDim.x) sW[e] = a.W[e];
 }
 
 // LDS: A accumulators (FP64, P*M), then Y (ch x MS floats), c (ch x PS
 // floats), MS and PS the row lengths M and P padded to a multiple of 4, and W.
+constexpr int kFwdThreads = 256;  // four wavefronts per atom in the forward
 template <int LMAX>
-__global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl, int K, int P, int ch) {
+__global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, NeighborList nl, int K, int P, int ch) {
     constexpr int L1 = LMAX + 1, M = L1 * L1, MS = pad4(M);
     constexpr int NMB = MS / kTile;
     extern __shared__ __align__(16) double gen_smem[];
@@ -91,7 +92,7 @@ __global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl,
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
         __syncthreads();  // the previous atom's epilogue has read sA
-        for (int e = lane; e < PM; e += 64) sA[e] = 0.0;
+        for (int e = lane; e < PM; e += kFwdThreads) sA[e] = 0.0;
         for (int c0 = 0; c0 < n; c0 += ch) {
             const int nc = min(ch, n - c0);
             __syncthreads();  // the previous chunk's tiles have read sY / sC (and sA is cleared)
@@ -113,7 +114,7 @@ __global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl,
                 for (int p = 0; p < PS; ++p) sC[lane * PS + p] = p < P ? fc * mix_row(w + p * K, K, rn) : 0.f;
             }
             __syncthreads();
-            for (int tl = lane; tl < ntile; tl += 64) {
+            for (int tl = lane; tl < ntile; tl += kFwdThreads) {
                 const int p0 = (tl / NMB) * kTile, m0 = (tl % NMB) * kTile;
                 double acc[kTile][kTile];
 #pragma unroll
@@ -139,8 +140,8 @@ __global__ __launch_bounds__(64) void k_gen_fwd(SegmentSet set, NeighborList nl,
             }
         }
         __syncthreads();
-        for (int e = lane; e < PM; e += 64) a.A[(size_t)row * PM + e] = (float)sA[e];
-        for (int e = lane; e < S; e += 64) {
+        for (int e = lane; e < PM; e += kFwdThreads) a.A[(size_t)row * PM + e] = (float)sA[e];
+        for (int e = lane; e < S; e += kFwdThreads) {
             const int p = e / L1, l = e - p * L1;
             double s = 0.0;
             for (int m = l * l; m < (l + 1) * (l + 1); ++m) {
@@ -430,7 +431,8 @@ bool run_gen(hipStream_t s, int K, int P, const SegmentSet &set, const NeighborL
     int ch = 64;  // the largest chunk that fits
     while (ch > 8 && gen_fwd_lds(M, P, ch, wsz) > kGenLds) ch /= 2;
     if (gen_fwd_lds(M, P, ch, wsz) > kGenLds) return false;
-    hipLaunchKernelGGL((k_gen_fwd<LMAX>), dim3(grid), dim3(64), gen_fwd_lds(M, P, ch, wsz), s, set, nl, K, P, ch);
+    hipLaunchKernelGGL((k_gen_fwd<LMAX>), dim3(grid), dim3(kFwdThreads), gen_fwd_lds(M, P, ch, wsz), s, set, nl, K, P,
+                       ch);
     return true;
 }
 
