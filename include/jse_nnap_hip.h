@@ -45,6 +45,15 @@ int jse_nnap_model_load(const char *json_or_path, jse_nnap_model *out);
  * the radial channel count P and the descriptor length S.  No device use. */
 int jse_nnap_model_parse(const char *json_or_path, int *ntypes, int *size_np, int *fp_size);
 
+/* Host-only: the packed parameter blocks of species `ctype` (1-based) as the
+ * loader builds them from a model file, i.e. the contents of the reference's
+ * mFpParam / mNnParam / mNormParam entries for that species (NNAP.java:
+ * 226-307), in the layouts of jse_nnap_model_create below; hidden[] receives
+ * nlayers widths and *mirror the mirrored species (0: none).  Any output may
+ * be NULL; the arrays must hold the sizes jse_nnap_model_parse implies. */
+int jse_nnap_model_parse_blocks(const char *json_or_path, int ctype, double *fp_param, double *nn_param,
+                                double *norm_param, int *nlayers, int *hidden, int *mirror);
+
 /* Replaces the packed parameter blocks the reference hands to its native
  * code (NNAP.java:226-307): per species t (0-based here):
  *   fp_param[t]   = W[ntypes][size_np[t]][nmax[t]+1]      (WTypeBasis.java:202-304)

@@ -612,6 +612,22 @@ struct jse_nnap_system_s {
     }
 };
 
+namespace {
+// The model text: the argument itself when it starts with '{', else the file it names.
+std::string model_text(const char *json_or_path) {
+    std::string text(json_or_path);
+    size_t k = text.find_first_not_of(" \t\r\n");
+    if (k == std::string::npos || text[k] != '{') {
+        std::ifstream f(text);
+        JNN_CHECK(f.good(), ERR_ARG, "cannot open model file: " + text);
+        std::stringstream ss;
+        ss << f.rdbuf();
+        text = ss.str();
+    }
+    return text;
+}
+}  // namespace
+
 extern "C" {
 
 const char *jse_nnap_last_error(void) { return g_last_error.c_str(); }
@@ -619,17 +635,8 @@ const char *jse_nnap_last_error(void) { return g_last_error.c_str(); }
 int jse_nnap_model_load(const char *json_or_path, jse_nnap_model *out) {
     return guard([&] {
         JNN_CHECK(json_or_path && out, ERR_ARG, "null argument");
-        std::string text(json_or_path);
-        size_t k = text.find_first_not_of(" \t\r\n");
-        if (k == std::string::npos || text[k] != '{') {
-            std::ifstream f(text);
-            JNN_CHECK(f.good(), ERR_ARG, "cannot open model file: " + text);
-            std::stringstream ss;
-            ss << f.rdbuf();
-            text = ss.str();
-        }
         auto m = std::make_unique<jse_nnap_model_s>();
-        m->host = parse_model_json(text);
+        m->host = parse_model_json(model_text(json_or_path));
         m->upload();
         *out = m.release();
     });
@@ -638,21 +645,28 @@ int jse_nnap_model_load(const char *json_or_path, jse_nnap_model *out) {
 int jse_nnap_model_parse(const char *json_or_path, int *ntypes, int *size_np, int *fp_size) {
     return guard([&] {
         JNN_CHECK(json_or_path && ntypes, ERR_ARG, "null argument");
-        std::string text(json_or_path);
-        size_t k = text.find_first_not_of(" \t\r\n");
-        if (k == std::string::npos || text[k] != '{') {
-            std::ifstream f(text);
-            JNN_CHECK(f.good(), ERR_ARG, "cannot open model file: " + text);
-            std::stringstream ss;
-            ss << f.rdbuf();
-            text = ss.str();
-        }
-        ModelHost h = parse_model_json(text);
+        ModelHost h = parse_model_json(model_text(json_or_path));
         *ntypes = h.ntypes();
         for (int t = 0; t < h.ntypes(); ++t) {
             if (size_np) size_np[t] = h.species[t].P;
             if (fp_size) fp_size[t] = h.species[t].S();
         }
+    });
+}
+
+int jse_nnap_model_parse_blocks(const char *json_or_path, int ctype, double *fp_param, double *nn_param,
+                                double *norm_param, int *nlayers, int *hidden, int *mirror) {
+    return guard([&] {
+        JNN_CHECK(json_or_path, ERR_ARG, "null argument");
+        ModelHost h = parse_model_json(model_text(json_or_path));
+        JNN_CHECK(ctype >= 1 && ctype <= h.ntypes(), ERR_ARG, "ctype out of range");
+        const SpeciesHost &sp = h.species[ctype - 1];
+        if (fp_param) std::copy(sp.W.begin(), sp.W.end(), fp_param);
+        if (nn_param) std::copy(sp.nn.begin(), sp.nn.end(), nn_param);
+        if (norm_param) std::copy(sp.norm.begin(), sp.norm.end(), norm_param);
+        if (nlayers) *nlayers = (int)sp.hidden.size();
+        if (hidden) std::copy(sp.hidden.begin(), sp.hidden.end(), hidden);
+        if (mirror) *mirror = sp.mirror;
     });
 }
 

@@ -3,6 +3,7 @@
 // NNAP constructor (src/nnap/java/jsex/nnap/NNAP.java:129-307), the basis
 // loader (basis/SphericalChebyshev.java:60-103, basis/WTypeBasis.java:202-354)
 // and FeedForward.load (nn/FeedForward.java:131-221).
+#include <algorithm>
 #include <cmath>
 #include <cstdlib>
 #include <map>
@@ -136,6 +137,10 @@ const JVal *first(const JVal &m, std::initializer_list<const char *> ks) {
     return nullptr;
 }
 
+// weight standardisation of the rfuse weight: each row's norm starts at this
+// epsilon (WTypeBasis.java:20, used at :216)
+constexpr double kWsEps = 1e-5;
+
 constexpr int WT_DEFAULT = 0, WT_NONE = -1, WT_FULL = 2, WT_EXFULL = 3, WT_FUSE = 4, WT_RFUSE = 5, WT_EXFUSE = 6;
 
 int size_n(int wtype, int ntypes, int nmax, int fuse) {  // WTypeBasis.java:331-354
@@ -226,14 +231,59 @@ ModelHost parse_model_json(const std::string &text) {
             }
         }
         sp.wtype = wtype;
-        std::vector<double> fuse_w = vec(b->get("fuse_weight"));
-        std::vector<double> rfuse_w = vec(b->get("rfuse_weight"));
+        JNN_CHECK(wtype == WT_DEFAULT || wtype == WT_NONE || (wtype >= WT_FULL && wtype <= WT_EXFUSE), ERR_ARG,
+                  "Input wtype MUST be in {-1, 0, 2, 3, 4, 5, 6}, input: " + std::to_string(wtype));
+        // the fuse weight is read for the fuse wtypes only and the rfuse weight for
+        // rfuse only; other wtypes ignore those keys (WTypeBasis.java:365-410)
+        std::vector<double> fuse_w, rfuse_w;
+        if (wtype == WT_FUSE || wtype == WT_EXFUSE) {
+            const JVal *fw = b->get("fuse_weight");
+            const JVal *fs = b->get("fuse_size");
+            if (fw) {
+                JNN_CHECK(fw->kind == JVal::ARR && !fw->arr.empty(), ERR_ARG, "model JSON: fuse_weight must be a non-empty array");
+                const bool nested = fw->arr[0].kind == JVal::ARR;
+                if (!fs && nested) {
+                    // an older file: the inner lists are the columns of the [ntypes][fuse] matrix
+                    const size_t ncol = fw->arr.size(), nrow = fw->arr[0].arr.size();
+                    fuse_w.assign(nrow * ncol, 0.0);
+                    for (size_t c = 0; c < ncol; ++c) {
+                        std::vector<double> col = vec(&fw->arr[c]);
+                        JNN_CHECK(col.size() == nrow, ERR_ARG, "Size of fuse weight mismatch");
+                        for (size_t r = 0; r < nrow; ++r) fuse_w[r * ncol + c] = col[r];
+                    }
+                } else if (nested) {
+                    for (auto &row : fw->arr) {
+                        std::vector<double> rr = vec(&row);
+                        JNN_CHECK((int)rr.size() == (int)num(fs, 0), ERR_ARG, "Column number of fuse weight mismatch");
+                        fuse_w.insert(fuse_w.end(), rr.begin(), rr.end());
+                    }
+                } else {
+                    fuse_w = vec(fw);
+                    if (fs) JNN_CHECK((long)fuse_w.size() == (long)num(fs, 0) * ntypes, ERR_ARG, "Size of fuse weight mismatch");
+                }
+            } else {
+                JNN_CHECK(fs, ERR_ARG, "Key `fuse_weight` or `fuse_size` required for fuse wtype");
+                fuse_w.assign((size_t)num(fs, 0) * ntypes, 0.0);
+            }
+            JNN_CHECK(!fuse_w.empty() && fuse_w.size() % ntypes == 0, ERR_ARG, "Size of fuse weight mismatch");
+        }
+        const int fuse0 = (wtype == WT_FUSE || wtype == WT_EXFUSE) ? (int)fuse_w.size() / ntypes : 0;
+        if (wtype == WT_RFUSE) {
+            JNN_CHECK(!b->get("post_fuse_size") && !b->get("post_fuse_weight"), ERR_ARG,
+                      "Key `post_fuse_size` or `post_fuse_weight` is invalid for wtype=='rfuse'");
+            const int sn0 = size_n(wtype, ntypes, sp.nmax, 0);
+            const JVal *rs = b->get("rfuse_size");
+            if (const JVal *rw = b->get("rfuse_weight")) {
+                rfuse_w = vec(rw);
+                if (rs) JNN_CHECK((long)rfuse_w.size() == (long)num(rs, 0) * sn0, ERR_ARG, "Size of rfuse weight mismatch");
+            } else {
+                JNN_CHECK(rs, ERR_ARG, "Key `rfuse_weight` or `rfuse_size` required for wtype=='rfuse'");
+                rfuse_w.assign((size_t)num(rs, 0) * sn0, 0.0);
+            }
+            JNN_CHECK(!rfuse_w.empty() && rfuse_w.size() % sn0 == 0, ERR_ARG, "Size of rfuse weight mismatch");
+        }
         double rfuse_scale = num(b->get("rfuse_scale"), 1.0);
         bool ws = num(first(*b, {"weight_standardization", "ws"}), 0.0) != 0.0;
-        if ((wtype == WT_FUSE || wtype == WT_EXFUSE) && fuse_w.empty())
-            throw Error(ERR_ARG, "Input fuse_weight MUST NOT be null when wtype=='fuse' or 'exfuse'");
-        if (wtype == WT_RFUSE && rfuse_w.empty())
-            throw Error(ERR_ARG, "Input rfuse_weight MUST NOT be null when wtype=='rfuse'");
         const int K = sp.K();
         // base mixing W[t][q][n] of a non-rfuse wtype (sizeN rows), WTypeBasis.java:202-304
         auto base_mixing = [&](int typed_, int sn_, int fuse_) {
@@ -274,7 +324,7 @@ ModelHost parse_model_json(const std::string &text) {
                 const std::vector<double> post = vec(b->get("post_fuse_weight"));
                 JNN_CHECK(!post.empty(), ERR_ARG,
                           "Key `post_fuse_weight` always required for post_fuse, training with post_fuse is invalid now");
-                const int fz = (wtype == WT_FUSE || wtype == WT_EXFUSE) ? (int)fuse_w.size() / ntypes : 0;
+                const int fz = fuse0;
                 const int sn0 = size_n(wtype, ntypes, sp.nmax, fz);
                 if (auto ps = b->get("post_fuse_size"))
                     JNN_CHECK(post.size() == (size_t)num(ps, 0) * sn0, ERR_ARG, "Size of post fuse weight mismatch");
@@ -292,6 +342,7 @@ ModelHost parse_model_json(const std::string &text) {
                 rfuse_scale = num(b->get("post_fuse_scale"), 1.0);
                 wtype = WT_RFUSE;
                 sp.wtype = wtype;
+                fuse_w.clear();
             }
         }
         int typed = typed_of(wtype);
@@ -306,7 +357,7 @@ ModelHost parse_model_json(const std::string &text) {
             std::vector<double> rw = rfuse_w;
             if (ws) {
                 for (int p = 0; p < sp.P; ++p) {
-                    double norm = 1e-8;
+                    double norm = kWsEps;
                     for (int t = 1; t <= ntypes; ++t)
                         for (int n = 0; n < K; ++n) norm += std::fabs(rfuse_w[((size_t)(t - 1) * sp.P + p) * K + n]);
                     norm /= sn;
@@ -335,17 +386,32 @@ ModelHost parse_model_json(const std::string &text) {
         sp.radial = src.radial;
         sp.mirror = is_mirror[i] ? ref[i] : 0;
     }
-    // pass 3: networks and normalisation
+    // pass 3 (NeuralNetwork.java:24-65): the networks.  feed_forward entries
+    // first; then a mirror species takes the full network of the species it
+    // mirrors (SharedFeedForward.full, its `nn` must be absent) and a
+    // shared_feed_forward entry takes another species' network with its own
+    // leading rows per layer (SharedFeedForward.java:163-198, 235-353)
+    std::vector<char> shared_nn(ntypes, 0);
     for (int i = 0; i < ntypes; ++i) {
         const JVal &m = models->arr[i];
         SpeciesHost &sp = out.species[i];
         const int S = sp.S();
+        if (is_mirror[i]) {
+            JNN_CHECK(!m.get("nn"), ERR_ARG, "nn data in mirror ModelInfo MUST be empty");
+            continue;
+        }
         JVal dflt_nn;
         dflt_nn.kind = JVal::OBJ;
         const JVal *nn = m.get("nn");
         if (!nn) nn = &dflt_nn;
-        if (auto t = nn->get("type"))
-            JNN_CHECK(t->kind == JVal::STR && t->str == "feed_forward", ERR_UNSUPPORTED, "Unsupported nn type");
+        if (auto t = nn->get("type")) {
+            JNN_CHECK(t->kind == JVal::STR, ERR_UNSUPPORTED, "Unsupported nn type: ?");
+            if (t->str == "shared_feed_forward") {
+                shared_nn[i] = 1;
+                continue;
+            }
+            JNN_CHECK(t->str == "feed_forward", ERR_UNSUPPORTED, "Unsupported nn type: " + t->str);
+        }
         if (auto d = nn->get("input_dim"))
             JNN_CHECK((int)num(d, 0) == S, ERR_ARG,
                       "Input dimension mismatch: " + std::to_string((int)num(d, 0)) + " vs " + std::to_string(S));
@@ -393,13 +459,113 @@ ModelHost parse_model_json(const std::string &text) {
         sp.nn.insert(sp.nn.end(), ow.begin(), ow.end());
         sp.nn.insert(sp.nn.end(), hb.begin(), hb.end());
         sp.nn.push_back(ob);
-        double ref = num(m.get("ref_eng"), 0.0);
+    }
+    // the species whose own feed_forward network another one takes
+    auto base_of = [&](int i, int b) -> const SpeciesHost & {
+        JNN_CHECK(b >= 1 && b <= ntypes && !is_mirror[b - 1] && !shared_nn[b - 1], ERR_ARG,
+                  "species " + std::to_string(i + 1) + " takes the network of species " + std::to_string(b) +
+                      ", which has no feed_forward network of its own");
+        return out.species[b - 1];
+    };
+    for (int i = 0; i < ntypes; ++i) {
+        SpeciesHost &sp = out.species[i];
+        if (is_mirror[i]) {
+            const SpeciesHost &bs = base_of(i, ref[i]);
+            sp.hidden = bs.hidden;
+            sp.nn = bs.nn;
+            continue;
+        }
+        if (!shared_nn[i]) continue;
+        const JVal &nn = *models->arr[i].get("nn");
+        const JVal *sh = nn.get("share");
+        JNN_CHECK(sh, ERR_ARG, "Key `share` required for shared_feed_forward");
+        const SpeciesHost &bs = base_of(i, (int)num(sh, 0));
+        JNN_CHECK(!nn.get("shared_flags"), ERR_ARG, "shared in layer nn is invalid now");
+        const int S = sp.S();
+        if (auto d = nn.get("input_dim"))
+            JNN_CHECK((int)num(d, 0) == S, ERR_ARG,
+                      "Input dimension mismatch: " + std::to_string((int)num(d, 0)) + " vs " + std::to_string(S));
+        JNN_CHECK(bs.S() == S, ERR_ARG, "Input dimensions mismatch for shared nn");
+        const JVal *shd = first(nn, {"shared_hidden_dims", "shared_nnarch"});
+        JNN_CHECK(shd, ERR_ARG, "Key `shared_hidden_dims` required for shared_feed_forward");
+        std::vector<double> shv = vec(shd);
+        const int L = (int)bs.hidden.size();
+        if ((int)shv.size() == L) shv.push_back(0.0);  // the output layer defaults to not shared
+        JNN_CHECK((int)shv.size() == L + 1, ERR_ARG,
+                  "The number of hidden layers mismatch: " + std::to_string((int)shv.size() - 1) + " vs " + std::to_string(L));
+        // start from the base network; each layer's first hidden - shared rows are this species' own
+        sp.hidden = bs.hidden;
+        sp.nn = bs.nn;
+        size_t hw_size = 0, hb_size = 0;
+        {
+            int cols = S;
+            for (int h : bs.hidden) { hw_size += (size_t)h * cols; hb_size += h; cols = h; }
+        }
+        const size_t ow_off = hw_size, hb_off = hw_size + bs.hidden.back(), ob_off = hb_off + hb_size;
+        const JVal *hws = nn.get("hidden_weights");
+        const JVal *hbs = nn.get("hidden_biases");
+        if (hws) JNN_CHECK(hws->kind == JVal::ARR && (int)hws->arr.size() == L, ERR_ARG, "The number of hidden weights mismatch");
+        if (hbs) JNN_CHECK(hbs->kind == JVal::ARR && (int)hbs->arr.size() == L, ERR_ARG, "The number of hidden biases mismatch");
+        size_t woff = 0, boff = 0;
+        int cols = S;
+        for (int li = 0; li < L; ++li) {
+            const int h = bs.hidden[li], shared = (int)shv[li];
+            JNN_CHECK(shared >= 0 && shared <= h, ERR_ARG,
+                      "invalid shared dimension: " + std::to_string(shared) + " vs " + std::to_string(h));
+            const int own = h - shared;
+            if (hws) {
+                const JVal &rows = hws->arr[li];
+                JNN_CHECK(rows.kind == JVal::ARR && (int)rows.arr.size() == own, ERR_ARG,
+                          "nrows of hidden weight '" + std::to_string(li) + "' mismatch");
+                for (int r = 0; r < own; ++r) {
+                    std::vector<double> rr = vec(&rows.arr[r]);
+                    JNN_CHECK((int)rr.size() == cols, ERR_ARG, "ncols of hidden weight '" + std::to_string(li) + "' mismatch");
+                    std::copy(rr.begin(), rr.end(), sp.nn.begin() + woff + (size_t)r * cols);
+                }
+            } else {
+                std::fill(sp.nn.begin() + woff, sp.nn.begin() + woff + (size_t)own * cols, 0.0);
+            }
+            if (hbs) {
+                std::vector<double> bb = vec(&hbs->arr[li]);
+                JNN_CHECK((int)bb.size() == own, ERR_ARG, "Size of hidden bias '" + std::to_string(li) + "' mismatch");
+                std::copy(bb.begin(), bb.end(), sp.nn.begin() + hb_off + boff);
+            } else {
+                std::fill(sp.nn.begin() + hb_off + boff, sp.nn.begin() + hb_off + boff + own, 0.0);
+            }
+            woff += (size_t)h * cols;
+            boff += h;
+            cols = h;
+        }
+        const int oshared = (int)shv[L];
+        JNN_CHECK(oshared == 0 || oshared == 1, ERR_ARG, "invalid shared dimension: " + std::to_string(oshared) + " vs 1");
+        if (!oshared) {
+            std::vector<double> ow = vec(nn.get("output_weight"));
+            if (ow.empty()) ow.assign((size_t)cols, 0.0);
+            JNN_CHECK((int)ow.size() == cols, ERR_ARG, "Size of output weight mismatch");
+            std::copy(ow.begin(), ow.end(), sp.nn.begin() + ow_off);
+            sp.nn[ob_off] = num(nn.get("output_bias"), 0.0);
+        }
+    }
+    // normalisation (NNAP.java:272-306): a mirror species must not carry its
+    // own and uses that of the species it mirrors
+    for (int i = 0; i < ntypes; ++i) {
+        SpeciesHost &sp = out.species[i];
+        const int S = sp.S();
+        const JVal *mm = &models->arr[i];
+        if (is_mirror[i]) {
+            JNN_CHECK(!mm->get("ref_eng"), ERR_ARG, "ref_eng in mirror_basis MUST be empty");
+            JNN_CHECK(!first(*mm, {"norm_vec", "norm_sigma", "norm_mu"}), ERR_ARG,
+                      "norm_vec/norm_sigma/norm_mu in mirror_basis MUST be empty");
+            mm = &models->arr[ref[i] - 1];
+        }
+        const JVal &m = *mm;
+        double refe = num(m.get("ref_eng"), 0.0);
         std::vector<double> mu = vec(m.get("norm_mu"));
         std::vector<double> sg = vec(first(m, {"norm_sigma", "norm_vec"}));
         if (mu.empty()) mu.assign((size_t)S, 0.0);
         if (sg.empty()) sg.assign((size_t)S, 1.0);
         JNN_CHECK((int)mu.size() == S && (int)sg.size() == S, ERR_ARG, "norm_mu/norm_sigma size mismatch");
-        sp.norm = {mu_e + ref, sig_e};
+        sp.norm = {mu_e + refe, sig_e};
         sp.norm.insert(sp.norm.end(), mu.begin(), mu.end());
         sp.norm.insert(sp.norm.end(), sg.begin(), sg.end());
     }

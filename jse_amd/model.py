@@ -37,6 +37,7 @@ WTYPE_DEFAULT, WTYPE_NONE, WTYPE_FULL, WTYPE_EXFULL, WTYPE_FUSE, WTYPE_RFUSE, WT
 
 DEFAULT_NMAX, DEFAULT_LMAX, DEFAULT_RCUT = 5, 6, 6.0  # SphericalChebyshev.java:18-22
 DEFAULT_HIDDEN = (32, 32)  # FeedForward.java:28
+WS_EPS = 1e-5  # weight standardisation epsilon of the rfuse weight (WTypeBasis.java:20, used at :216)
 
 
 @dataclass
@@ -131,7 +132,7 @@ def _mixing(wtype: int, ntypes: int, nmax: int, fuse_w, rfuse_w, rfuse_scale: fl
         if ws:
             out = np.empty_like(rw)
             for p in range(P):
-                norm = 1e-8 + np.abs(rw[:, p, :]).sum()   # WS_EPS
+                norm = WS_EPS + np.abs(rw[:, p, :]).sum()
                 norm /= size_n
                 out[:, p, :] = rw[:, p, :] / norm
             rw = out
@@ -162,6 +163,61 @@ def _mixing(wtype: int, ntypes: int, nmax: int, fuse_w, rfuse_w, rfuse_scale: fl
     return typed, P, W
 
 
+def _fuse_weight(b, wtype: int, ntypes: int):
+    """WTypeBasis.getFuseWeight_ (WTypeBasis.java:365-389): read for the fuse
+    wtypes only (other wtypes ignore the key); a nested list without
+    `fuse_size` is an older file whose inner lists are the columns of the
+    [ntypes][fuse] matrix; `fuse_size` alone gives zeros."""
+    if wtype not in (WTYPE_FUSE, WTYPE_EXFUSE):
+        return None
+    fs = b.get("fuse_size")
+    fw = b.get("fuse_weight")
+    if fw is not None:
+        if len(fw) == 0:
+            raise ValueError("Size of fuse weight mismatch")
+        nested = isinstance(fw[0], (list, tuple))
+        if fs is None and nested:
+            out = np.asarray(fw, dtype=float).T.ravel()
+        elif nested:
+            mat = np.asarray(fw, dtype=float)
+            if mat.ndim != 2 or mat.shape[1] != int(fs):
+                raise ValueError("Column number of fuse weight mismatch")
+            out = mat.ravel()
+        else:
+            out = np.asarray(fw, dtype=float)
+            if fs is not None and len(out) != int(fs) * ntypes:
+                raise ValueError("Size of fuse weight mismatch")
+    else:
+        if fs is None:
+            raise ValueError("Key `fuse_weight` or `fuse_size` required for fuse wtype")
+        out = np.zeros(int(fs) * ntypes)
+    if len(out) == 0 or len(out) % ntypes:
+        raise ValueError("Size of fuse weight mismatch")
+    return out
+
+
+def _rfuse_weight(b, wtype: int, size_n: int):
+    """WTypeBasis.getRFuseWeight_ (WTypeBasis.java:391-410): read for rfuse
+    only; `rfuse_size` alone gives zeros."""
+    if wtype != WTYPE_RFUSE:
+        return None
+    if b.get("post_fuse_size") is not None or b.get("post_fuse_weight") is not None:
+        raise ValueError("Key `post_fuse_size` or `post_fuse_weight` is invalid for wtype=='rfuse'")
+    rs = b.get("rfuse_size")
+    rw = b.get("rfuse_weight")
+    if rw is not None:
+        out = np.asarray(rw, dtype=float)
+        if rs is not None and len(out) != int(rs) * size_n:
+            raise ValueError("Size of rfuse weight mismatch")
+    else:
+        if rs is None:
+            raise ValueError("Key `rfuse_weight` or `rfuse_size` required for wtype=='rfuse'")
+        out = np.zeros(int(rs) * size_n)
+    if len(out) == 0 or len(out) % size_n:
+        raise ValueError("Size of rfuse weight mismatch")
+    return out
+
+
 def _post_fuse(b, wtype: int, ntypes: int, nmax: int, fuse_w):
     """The rfuse weight [ntypes][P][K] (flat) equal to a basis' post-fuse weight
     (WTypeBasis.getPostFuseWeight_ and postFuse2RFuse_, WTypeBasis.java:
@@ -187,6 +243,120 @@ def _post_fuse(b, wtype: int, ntypes: int, nmax: int, fuse_w):
     _, _, Wb = _mixing(wtype, ntypes, nmax, fuse_w, None, 1.0, False)   # [ntypes][sizeN][K]
     P = len(post) // sn
     return np.einsum("pq,tqn->tpn", post.reshape(P, sn), Wb).ravel()
+
+
+def _feed_forward(nn: dict, S: int):
+    """FeedForward.load (FeedForward.java:131-221): (hidden dims, packed
+    [hidden W | output W | hidden b | output b])."""
+    in_dim = nn.get("input_dim")
+    if in_dim is not None and int(in_dim) != S:
+        raise ValueError(f"Input dimension mismatch: {int(in_dim)} vs {S}")
+    hidden = [int(h) for h in (_get(nn, "hidden_dims", "nnarch", default=list(DEFAULT_HIDDEN)))]
+    if len(hidden) == 0:
+        raise ValueError("At least one hidden layer is required")
+    hw_list, hb_list = [], []
+    hws = nn.get("hidden_weights")
+    hbs = nn.get("hidden_biases")
+    if hws is not None and len(hws) != len(hidden):
+        raise ValueError(f"The number of hidden weights mismatch: {len(hws)} vs {len(hidden)}")
+    if hbs is not None and len(hbs) != len(hidden):
+        raise ValueError(f"The number of hidden biases mismatch: {len(hbs)} vs {len(hidden)}")
+    cols = S
+    for li, h in enumerate(hidden):
+        if hws is not None:
+            w = np.asarray(hws[li], dtype=float)
+            if w.shape != (h, cols):
+                raise ValueError(f"shape of hidden weight '{li}' mismatch: {w.shape} vs {(h, cols)}")
+        else:
+            w = np.zeros((h, cols))
+        hw_list.append(w.reshape(-1))
+        if hbs is not None:
+            bb = np.asarray(hbs[li], dtype=float)
+            if bb.shape != (h,):
+                raise ValueError(f"Size of hidden bias '{li}' mismatch: {bb.shape[0]} vs {h}")
+        else:
+            bb = np.zeros(h)
+        hb_list.append(bb)
+        cols = h
+    ow = np.asarray(nn.get("output_weight", np.zeros(hidden[-1])), dtype=float)
+    if ow.shape != (hidden[-1],):
+        raise ValueError(f"Size of output weight mismatch: {ow.shape[0]} vs {hidden[-1]}")
+    ob = float(nn.get("output_bias", 0.0) or 0.0)
+    return hidden, np.concatenate(hw_list + [ow] + hb_list + [np.array([ob])])
+
+
+def _shared_feed_forward(nn: dict, S: int, base_of, i: int, bases):
+    """SharedFeedForward.load (SharedFeedForward.java:235-353) as the packed
+    block its updateParameters_ (:163-198) fills: the base species' network
+    with each hidden layer's first (hidden - shared) rows, and the output layer
+    unless shared, taken from this entry."""
+    if nn.get("share") is None:
+        raise ValueError("Key `share` required for shared_feed_forward")
+    b = int(nn["share"])
+    hidden, base = base_of(i, b)
+    if nn.get("shared_flags") is not None:
+        raise ValueError("shared in layer nn is invalid now")
+    in_dim = nn.get("input_dim")
+    if in_dim is not None and int(in_dim) != S:
+        raise ValueError(f"Input dimension mismatch: {int(in_dim)} vs {S}")
+    nb, lb = bases[b - 1][5], bases[b - 1][1]
+    if nb * (lb + 1) != S:
+        raise ValueError("Input dimensions mismatch for shared nn")
+    shd = _get(nn, "shared_hidden_dims", "shared_nnarch")
+    if shd is None:
+        raise ValueError("Key `shared_hidden_dims` required for shared_feed_forward")
+    shd = [int(v) for v in shd]
+    L = len(hidden)
+    if len(shd) == L:
+        shd.append(0)   # the output layer defaults to not shared
+    if len(shd) != L + 1:
+        raise ValueError(f"The number of hidden layers mismatch: {len(shd) - 1} vs {L}")
+    out = base.copy()
+    hw_size = sum(h * c for h, c in zip(hidden, [S] + hidden[:-1]))
+    hb_off = hw_size + hidden[-1]
+    ob_off = hb_off + sum(hidden)
+    hws = nn.get("hidden_weights")
+    hbs = nn.get("hidden_biases")
+    if hws is not None and len(hws) != L:
+        raise ValueError(f"The number of hidden weights mismatch: {len(hws)} vs {L}")
+    if hbs is not None and len(hbs) != L:
+        raise ValueError(f"The number of hidden biases mismatch: {len(hbs)} vs {L}")
+    woff = boff = 0
+    cols = S
+    for li, h in enumerate(hidden):
+        sh = shd[li]
+        if sh < 0 or sh > h:
+            raise ValueError(f"invalid shared dimension: {sh} vs {h}")
+        own = h - sh
+        if hws is not None:
+            rows = hws[li]
+            if len(rows) != own:
+                raise ValueError(f"nrows of hidden weight '{li}' mismatch: {len(rows)} vs {own}")
+            w = np.asarray(rows, dtype=float).reshape(own, -1) if own else np.zeros((0, cols))
+            if own and w.shape[1] != cols:
+                raise ValueError(f"ncols of hidden weight '{li}' mismatch: {w.shape[1]} vs {cols}")
+            out[woff:woff + own * cols] = w.ravel()
+        else:
+            out[woff:woff + own * cols] = 0.0
+        if hbs is not None:
+            bb = np.asarray(hbs[li], dtype=float)
+            if bb.shape != (own,):
+                raise ValueError(f"Size of hidden bias '{li}' mismatch: {bb.shape[0]} vs {own}")
+            out[hb_off + boff:hb_off + boff + own] = bb
+        else:
+            out[hb_off + boff:hb_off + boff + own] = 0.0
+        woff += h * cols
+        boff += h
+        cols = h
+    if shd[L] not in (0, 1):
+        raise ValueError(f"invalid shared dimension: {shd[L]} vs 1")
+    if shd[L] == 0:
+        ow = np.asarray(nn.get("output_weight", np.zeros(cols)), dtype=float)
+        if ow.shape != (cols,):
+            raise ValueError(f"Size of output weight mismatch: {ow.shape[0]} vs {cols}")
+        out[hw_size:hw_size + cols] = ow
+        out[ob_off] = float(nn.get("output_bias", 0.0) or 0.0)
+    return list(hidden), out
 
 
 def load_model(src) -> Model:
@@ -260,19 +430,17 @@ def load_model(src) -> Model:
         wtype = int(wt) if isinstance(wt, (int, float)) else WTYPES.get(str(wt))
         if wtype is None:
             raise ValueError(f"Input wtype MUST be in {{-1, 0, 2, 3, 4, 5, 6}}, input: {wt}")
-        fuse_w = b.get("fuse_weight")
-        rfuse_w = b.get("rfuse_weight")
+        if wtype not in WTYPES.values():
+            raise ValueError(f"Input wtype MUST be in {{-1, 0, 2, 3, 4, 5, 6}}, input: {wtype}")
+        fuse_w = _fuse_weight(b, wtype, ntypes)
+        rfuse_w = _rfuse_weight(b, wtype, _size_n(wtype, ntypes, nmax, 0))
         rfuse_scale = float(b.get("rfuse_scale", 1.0) or 1.0)
         ws = bool(_get(b, "weight_standardization", "ws", default=False))
-        if wtype in (WTYPE_FUSE, WTYPE_EXFUSE) and fuse_w is None:
-            raise ValueError("Input fuse_weight MUST NOT be null when wtype=='fuse' or 'exfuse'")
-        if wtype == WTYPE_RFUSE and rfuse_w is None:
-            raise ValueError("Input rfuse_weight MUST NOT be null when wtype=='rfuse'")
         post = _post_fuse(b, wtype, ntypes, nmax, fuse_w)
         if post is not None:
             # an older file's post-fuse weight becomes the rfuse weight it equals
             # (SphericalChebyshev.java:81-94, WTypeBasis.postFuse2RFuse_)
-            rfuse_w, wtype = post, WTYPE_RFUSE
+            rfuse_w, wtype, fuse_w = post, WTYPE_RFUSE, None
             rfuse_scale = float(b.get("post_fuse_scale", 1.0) or 1.0)
         typed, P, W = _mixing(wtype, ntypes, nmax, fuse_w, rfuse_w, rfuse_scale, ws)
         bases[i] = (nmax, lmax, rcut, wtype, typed, P, W, radial)
@@ -282,55 +450,65 @@ def load_model(src) -> Model:
             if ref[ref[i] - 1]:
                 raise ValueError("a mirror/share basis of a mirror/share basis is not supported")
             bases[i] = bases[ref[i] - 1]
-    # pass 3: networks and normalisation
+    # pass 3 (NeuralNetwork.java:24-65): the networks.  feed_forward entries
+    # first; then a mirror species takes the full network of the species it
+    # mirrors (SharedFeedForward.full; its `nn` must be absent) and a
+    # shared_feed_forward entry takes another species' network with its own
+    # leading rows per layer (SharedFeedForward.java:163-198, 235-353)
+    nets = [None] * ntypes      # (hidden, nn_param)
+    shared_nn = [False] * ntypes
+    for i, m in enumerate(models):
+        nmax, lmax, rcut, wtype, typed, P, W, radial = bases[i]
+        S = P * (lmax + 1)
+        if is_mirror[i]:
+            if m.get("nn") is not None:
+                raise ValueError("nn data in mirror ModelInfo MUST be empty")
+            continue
+        nn = m.get("nn") or {}
+        ntype = str(nn.get("type", "feed_forward"))
+        if ntype == "shared_feed_forward":
+            shared_nn[i] = True
+            continue
+        if ntype != "feed_forward":
+            raise ValueError(f"Unsupported nn type: {nn.get('type')}")
+        nets[i] = _feed_forward(nn, S)
+
+    def base_of(i, b):
+        if not (1 <= b <= ntypes) or is_mirror[b - 1] or shared_nn[b - 1]:
+            raise ValueError(f"species {i + 1} takes the network of species {b}, "
+                             "which has no feed_forward network of its own")
+        return nets[b - 1]
+
+    for i, m in enumerate(models):
+        if is_mirror[i]:
+            nets[i] = base_of(i, ref[i])
+        elif shared_nn[i]:
+            nmax, lmax, rcut, wtype, typed, P, W, radial = bases[i]
+            nets[i] = _shared_feed_forward(m["nn"], P * (lmax + 1), base_of, i, bases)
+    # normalisation (NNAP.java:272-306): a mirror species must not carry its
+    # own and uses that of the species it mirrors
     out: List[Species] = []
     for i, m in enumerate(models):
         sym = m.get("symbol")
         nmax, lmax, rcut, wtype, typed, P, W, radial = bases[i]
         S = P * (lmax + 1)
-        nn = m.get("nn") or {}
-        if str(nn.get("type", "feed_forward")) != "feed_forward":
-            raise ValueError(f"Unsupported nn type: {nn.get('type')}")
-        in_dim = nn.get("input_dim")
-        if in_dim is not None and int(in_dim) != S:
-            raise ValueError(f"Input dimension mismatch: {int(in_dim)} vs {S}")
-        hidden = [int(h) for h in (_get(nn, "hidden_dims", "nnarch", default=list(DEFAULT_HIDDEN)))]
-        if len(hidden) == 0:
-            raise ValueError("At least one hidden layer is required")
-        hw_list, hb_list = [], []
-        hws = nn.get("hidden_weights")
-        hbs = nn.get("hidden_biases")
-        cols = S
-        for li, h in enumerate(hidden):
-            if hws is not None:
-                w = np.asarray(hws[li], dtype=float)
-                if w.shape != (h, cols):
-                    raise ValueError(f"shape of hidden weight '{li}' mismatch: {w.shape} vs {(h, cols)}")
-            else:
-                w = np.zeros((h, cols))
-            hw_list.append(w.reshape(-1))
-            if hbs is not None:
-                bb = np.asarray(hbs[li], dtype=float)
-                if bb.shape != (h,):
-                    raise ValueError(f"Size of hidden bias '{li}' mismatch: {bb.shape[0]} vs {h}")
-            else:
-                bb = np.zeros(h)
-            hb_list.append(bb)
-            cols = h
-        ow = np.asarray(nn.get("output_weight", np.zeros(hidden[-1])), dtype=float)
-        if ow.shape != (hidden[-1],):
-            raise ValueError(f"Size of output weight mismatch: {ow.shape[0]} vs {hidden[-1]}")
-        ob = float(nn.get("output_bias", 0.0) or 0.0)
-        nn_param = np.concatenate(hw_list + [ow] + hb_list + [np.array([ob])])
-        ref_eng = float(m.get("ref_eng", 0.0) or 0.0)
-        nsig = _get(m, "norm_sigma", "norm_vec")
-        nmu = m.get("norm_mu")
+        src = m
+        if is_mirror[i]:
+            if m.get("ref_eng") is not None:
+                raise ValueError("ref_eng in mirror_basis MUST be empty")
+            if _get(m, "norm_vec", "norm_sigma", "norm_mu") is not None:
+                raise ValueError("norm_vec/norm_sigma/norm_mu in mirror_basis MUST be empty")
+            src = models[ref[i] - 1]
+        ref_eng = float(src.get("ref_eng", 0.0) or 0.0)
+        nsig = _get(src, "norm_sigma", "norm_vec")
+        nmu = src.get("norm_mu")
         mu = np.zeros(S) if nmu is None else np.asarray(nmu, dtype=float)
         sg = np.ones(S) if nsig is None else np.asarray(nsig, dtype=float)
         if mu.shape != (S,) or sg.shape != (S,):
             raise ValueError("norm_mu/norm_sigma size mismatch")
         norm = np.concatenate([[mu_e + ref_eng, sig_e], mu, sg])
-        out.append(Species(str(sym), nmax, lmax, rcut, wtype, typed, P, W, hidden, nn_param, norm,
+        hidden, nn_param = nets[i]
+        out.append(Species(str(sym), nmax, lmax, rcut, wtype, typed, P, W, list(hidden), nn_param.copy(), norm,
                            ref[i] if is_mirror[i] else 0, radial))
     return Model(out, info.get("units"), info)
 

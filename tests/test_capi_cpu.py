@@ -70,6 +70,8 @@ def test_cpp_parser_share_mirror(kind):
     md = M.random_model(["Cu", "Zr", "Al"], 4, 5, 5.0, (8, 8), seed=3)
     md["models"][0]["basis"]["rcut"] = 4.5
     md["models"][2]["basis"] = {"type": kind, kind: 1}
+    if kind == "mirror":  # a mirror entry has no network or normalisation of its own
+        md["models"][2] = {"symbol": "Al", "basis": md["models"][2]["basis"]}
     rc, nt, P, S = _parse(json.dumps(md))
     assert rc == 0, nnap.lib().jse_nnap_last_error()
     mdl = M.load_model(md)
@@ -185,3 +187,179 @@ def test_lammps_packed_host_entries():
     assert lib.jse_nnap_statNeiNumLammps(blk) == 0 and out[0] == 9
     assert lib.jse_nnap_lammps_bind(_pack(("p", 0), ("i", 0))) == 1
     assert b"null model" in lib.jse_nnap_last_error()
+
+
+def _blocks(text, ctype, sp):
+    """The C++ loader's packed blocks of species `ctype`, sized from the
+    Python loader's species `sp`."""
+    L = nnap.lib()
+    nt = 0
+    fp = np.zeros(sp.W.size)
+    nn = np.zeros(sp.nn_param.size)
+    norm = np.zeros(sp.norm_param.size)
+    nl = ctypes.c_int()
+    hid = (ctypes.c_int * 8)()
+    mir = ctypes.c_int()
+    dp = lambda a: a.ctypes.data_as(ctypes.POINTER(ctypes.c_double))  # noqa: E731
+    rc = L.jse_nnap_model_parse_blocks(text.encode(), ctype, dp(fp), dp(nn), dp(norm), ctypes.byref(nl), hid,
+                                       ctypes.byref(mir))
+    assert rc == 0, nnap.lib().jse_nnap_last_error()
+    return fp, nn, norm, list(hid)[:nl.value], mir.value
+
+
+def _both_equal(md):
+    """Both loaders build identical blocks for every species of `md`."""
+    mdl = M.load_model(md)
+    text = json.dumps(md)
+    for t, sp in enumerate(mdl.species):
+        fp, nn, norm, hid, mir = _blocks(text, t + 1, sp)
+        assert np.array_equal(fp, sp.W.ravel()), f"species {t + 1} fp_param"
+        assert np.array_equal(nn, sp.nn_param), f"species {t + 1} nn_param"
+        assert np.array_equal(norm, sp.norm_param), f"species {t + 1} norm_param"
+        assert hid == list(sp.hidden) and mir == sp.mirror
+    return mdl
+
+
+def _mirror_model():
+    """Cu/Zr/Al with Zr a mirror of Cu in the reference's own shape: the mirror
+    entry carries no nn, ref_eng or normalisation (NNAP.java:281-288,
+    NeuralNetwork.java:53-56)."""
+    md = M.random_model(["Cu", "Zr", "Al"], 4, 5, 5.0, (8, 8), seed=5)
+    rng = np.random.default_rng(5)
+    for m in md["models"]:
+        S = m["nn"]["input_dim"]
+        m["norm_mu"] = rng.normal(size=S).tolist()
+        m["norm_sigma"] = (1.0 + rng.random(S)).tolist()
+        m["ref_eng"] = float(rng.normal())
+    md["models"][1] = {"symbol": "Zr", "basis": {"type": "mirror", "mirror": 1}}
+    return md
+
+
+def test_mirror_species_takes_the_mirrored_network_and_normalisation():
+    """A mirror species evaluates with the full network (SharedFeedForward.full)
+    and the ref_eng / norm_mu / norm_sigma of the species it mirrors, in both
+    loaders."""
+    md = _mirror_model()
+    mdl = _both_equal(md)
+    a, b = mdl.species[0], mdl.species[1]
+    assert b.mirror == 1 and b.hidden == a.hidden
+    assert np.array_equal(b.nn_param, a.nn_param) and np.array_equal(b.norm_param, a.norm_param)
+    assert not np.array_equal(mdl.species[2].nn_param[:10], a.nn_param[:10])
+
+
+@pytest.mark.parametrize("key,msg", [("nn", "nn data in mirror"), ("ref_eng", "ref_eng in mirror"),
+                                     ("norm_mu", "norm_vec/norm_sigma/norm_mu in mirror"),
+                                     ("norm_vec", "norm_vec/norm_sigma/norm_mu in mirror")])
+def test_mirror_species_rejects_its_own_network_or_normalisation(key, msg):
+    md = _mirror_model()
+    S = md["models"][0]["nn"]["input_dim"]
+    md["models"][1][key] = md["models"][0]["nn"] if key == "nn" else ([0.5] * S if key != "ref_eng" else 0.1)
+    rc = _parse(json.dumps(md))[0]
+    assert rc == 1 and msg in nnap.lib().jse_nnap_last_error().decode()
+    with pytest.raises(ValueError, match=msg.split()[0]):
+        M.load_model(md)
+
+
+def test_shared_feed_forward_network():
+    """shared_feed_forward (SharedFeedForward.java:163-198,235-353): species 3
+    takes species 1's network; per hidden layer its first (hidden - shared)
+    rows and biases are its own, the rest are species 1's; the output layer is
+    its own unless shared.  Fully shared it equals species 1's network."""
+    md = M.random_model(["Cu", "Zr", "Al"], 4, 5, 5.0, (8, 6), seed=6)
+    S = md["models"][0]["nn"]["input_dim"]
+    rng = np.random.default_rng(6)
+    own_w = [rng.normal(size=(3, S)).tolist(), rng.normal(size=(6, 8)).tolist()]
+    own_b = [rng.normal(size=3).tolist(), rng.normal(size=6).tolist()]
+    md["models"][2]["nn"] = {"type": "shared_feed_forward", "share": 1, "shared_hidden_dims": [5, 0],
+                             "hidden_weights": own_w, "hidden_biases": own_b,
+                             "output_weight": rng.normal(size=6).tolist(), "output_bias": 0.25}
+    mdl = _both_equal(md)
+    base, sh = mdl.species[0].nn_param, mdl.species[2].nn_param
+    w1 = sh[:8 * S].reshape(8, S)
+    assert np.array_equal(w1[:3], np.asarray(own_w[0])) and np.array_equal(w1[3:], base[:8 * S].reshape(8, S)[3:])
+    w2 = sh[8 * S:8 * S + 48].reshape(6, 8)
+    assert np.array_equal(w2, np.asarray(own_w[1]))
+    hb = 8 * S + 48 + 6
+    assert np.array_equal(sh[hb:hb + 3], own_b[0]) and np.array_equal(sh[hb + 3:hb + 8], base[hb + 3:hb + 8])
+    assert sh[-1] == 0.25
+    md["models"][2]["nn"] = {"type": "shared_feed_forward", "share": 1, "shared_hidden_dims": [8, 6, 1]}
+    mdl = _both_equal(md)
+    assert np.array_equal(mdl.species[2].nn_param, mdl.species[0].nn_param)
+    md["models"][2]["nn"] = {"type": "shared_feed_forward", "share": 1}
+    assert _parse(json.dumps(md))[0] == 1
+    with pytest.raises(ValueError, match="shared_hidden_dims"):
+        M.load_model(md)
+
+
+def test_rfuse_weight_is_ignored_for_other_wtypes():
+    """The reference reads rfuse_weight only for wtype rfuse and fuse_weight only
+    for fuse/exfuse (WTypeBasis.java:365-410): a stray key of the wrong size
+    changes nothing, in either loader."""
+    md = M.random_model(["Cu", "Zr"], 4, 5, 5.0, (8, 8), seed=7, wtype="default")
+    for m in md["models"]:
+        m["basis"]["wtype"] = "exfull"
+        m["nn"] = None
+        m.pop("norm_mu", None)
+        m.pop("norm_sigma", None)
+    plain = _both_equal(md)
+    for m in md["models"]:
+        m["basis"]["rfuse_weight"] = [1.0] * (3 * 15 * 5)
+        m["basis"]["fuse_weight"] = [2.0] * 7
+    stray = _both_equal(md)
+    for a, b in zip(plain.species, stray.species):
+        assert a.P == b.P == 15 and np.array_equal(a.W, b.W)
+
+
+def test_weight_standardization_epsilon():
+    """Weight standardisation divides each rfuse row by (WS_EPS + sum |w|) / sizeN
+    with WS_EPS = 1e-5 (WTypeBasis.java:20,210-226): a row whose only non-zero
+    weight is w = 1e-5 becomes scale * w * sizeN / (1e-5 + w) = scale * sizeN / 2."""
+    md = M.random_model(["Cu", "Zr"], 3, 2, 5.0, (8, 8), seed=8, wtype="default")
+    K, P, sn = 4, 3, 2 * 4
+    rw = np.zeros((2, P, K))
+    rw[1, 0, 2] = 1e-5                       # row 0: one small weight
+    rw[:, 1, :] = np.arange(2 * K).reshape(2, K) - 3.0   # row 1: ordinary weights
+    rw[0, 2, 0] = 4.0                        # row 2: one large weight
+    for m in md["models"]:
+        m["basis"].update(wtype="rfuse", rfuse_weight=rw.ravel().tolist(), rfuse_scale=0.5,
+                          weight_standardization=True)
+        m["nn"] = None
+        m.pop("norm_mu", None)
+        m.pop("norm_sigma", None)
+    mdl = _both_equal(md)
+    W = mdl.species[0].W
+    assert W.shape == (2, P, K)
+    assert abs(W[1, 0, 2] - 0.5 * sn / 2) <= 1e-12 * sn
+    assert abs(W[0, 2, 0] - 0.5 * 4.0 * sn / (1e-5 + 4.0)) <= 1e-12
+    norm1 = (1e-5 + np.abs(rw[:, 1, :]).sum()) / sn
+    assert np.allclose(W[:, 1, :], 0.5 * rw[:, 1, :] / norm1, rtol=1e-14, atol=0)
+
+
+@pytest.mark.parametrize("form", ["flat", "rows", "old_columns", "size_only"])
+def test_fuse_weight_formats(form):
+    """fuse_weight as a flat [ntypes*fuse] list (with fuse_size), as rows of a
+    [ntypes][fuse] matrix, as an older file's columns (no fuse_size), or zeros
+    from fuse_size alone (WTypeBasis.getFuseWeight_, :365-389); both loaders
+    give the same mixing."""
+    md = M.random_model(["Cu", "Zr", "Al"], 3, 2, 5.0, (8, 8), seed=9, wtype="default")
+    fw = np.arange(1, 3 * 2 + 1, dtype=float).reshape(3, 2) / 4     # [ntypes][fuse]
+    for m in md["models"]:
+        b = m["basis"]
+        b["wtype"] = "fuse"
+        if form == "flat":
+            b.update(fuse_size=2, fuse_weight=fw.ravel().tolist())
+        elif form == "rows":
+            b.update(fuse_size=2, fuse_weight=fw.tolist())
+        elif form == "old_columns":
+            b.update(fuse_weight=fw.T.tolist())
+        else:
+            b.update(fuse_size=2)
+        m["nn"] = None
+        m.pop("norm_mu", None)
+        m.pop("norm_sigma", None)
+    mdl = _both_equal(md)
+    W = mdl.species[0].W                    # [t][k*K + n][n] = fw[t][k]
+    want = fw if form != "size_only" else np.zeros_like(fw)
+    for t in range(3):
+        for k in range(2):
+            assert W[t, k * 4 + 1, 1] == want[t, k]

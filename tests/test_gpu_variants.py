@@ -63,16 +63,25 @@ def test_lmax4_nmax5(O, symbols):
 def test_mirror_and_share_bases(O):
     """mirror / share bases (MirrorBasis.java, SharedBasis.java): species 1
     with its own rcut, species 2 mirroring it (neighbours of species 1 and 2
-    swap roles for species-2 centres), species 3 sharing it; each keeps its own
-    network and normalisation."""
+    swap roles for species-2 centres; its entry has no network or
+    normalisation and uses species 1's, NNAP.java:281-288), species 3 sharing
+    it with its own network (a shared_feed_forward over species 1's: three
+    rows per layer of its own, SharedFeedForward.java:163-198) and
+    normalisation."""
     st = _liquid(["Cu", "Zr", "Al"], 1500, 23)
     md = M.random_model(["Cu", "Zr", "Al"], nmax=8, lmax=6, rcut=6.0, hidden=(64, 64), seed=33)
     md["models"][0]["basis"]["rcut"] = 5.5
-    md["models"][1]["basis"] = {"type": "mirror", "mirror": 1}
     md["models"][2]["basis"] = {"type": "share", "share": 1}
+    md = W._normalized(md, st, 300, 33)
+    md["models"][1] = {"symbol": "Zr", "basis": {"type": "mirror", "mirror": 1}}
+    own = md["models"][2]["nn"]
+    md["models"][2]["nn"] = {"type": "shared_feed_forward", "share": 1, "shared_hidden_dims": [61, 61],
+                             "hidden_weights": [w[:3] for w in own["hidden_weights"]],
+                             "hidden_biases": [b[:3] for b in own["hidden_biases"]],
+                             "output_weight": own["output_weight"], "output_bias": own["output_bias"]}
     mdl = M.load_model(md)
     assert [s.rcut for s in mdl.species] == [5.5, 5.5, 5.5] and [s.mirror for s in mdl.species] == [0, 1, 0]
-    md = W._normalized(md, st, 300, 33)
+    assert np.array_equal(mdl.species[1].nn_param, mdl.species[0].nn_param)
     _check(O, md, st)
 
 

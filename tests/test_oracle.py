@@ -441,10 +441,15 @@ def test_mirror_basis_is_a_species_swap_symmetry(O):
         md = M.random_model(["Cu", "Zr"], nmax=4, lmax=3, rcut=6.0, hidden=(8, 8), seed=42)
         md["models"][0]["basis"]["rcut"] = 5.5
         md["models"][1]["basis"] = {"type": kind, kind: 1}
-        md["models"][1]["nn"] = md["models"][0]["nn"]
-        for key in ("norm_mu", "norm_sigma", "ref_eng"):
-            if key in md["models"][0]:
-                md["models"][1][key] = md["models"][0][key]
+        if kind == "mirror":
+            # the reference's mirror entry: no network or normalisation of its own,
+            # species 1's are used (NNAP.java:281-288, NeuralNetwork.java:53-56)
+            md["models"][1] = {"symbol": "Zr", "basis": md["models"][1]["basis"]}
+        else:
+            md["models"][1]["nn"] = md["models"][0]["nn"]
+            for key in ("norm_mu", "norm_sigma", "ref_eng"):
+                if key in md["models"][0]:
+                    md["models"][1][key] = md["models"][0][key]
         orc = O.Oracle(M.load_model(md))
         a, b = orc.system_efv(st), orc.system_efv(st2)
         if symmetric:
