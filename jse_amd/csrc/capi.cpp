@@ -42,6 +42,7 @@ void jse_nnap_model_s::upload() {
         SpeciesDev &d = dev[t];
         std::vector<float> W(sp.W.begin(), sp.W.end());
         d.W = dupload(W);
+        d.W64 = dupload(sp.W);
         const int S = sp.S();
         std::vector<float> mu(sp.norm.begin() + 2, sp.norm.begin() + 2 + S);
         std::vector<float> sg(sp.norm.begin() + 2 + S, sp.norm.end());
@@ -66,6 +67,24 @@ void jse_nnap_model_s::upload() {
         d.w_fwd = dupload(wf);
         d.w_bwd = dupload(wb);
         d.bias = dupload(bias);
+        {  // the same layouts in FP64 (the plain mode's MLP)
+            std::vector<double> wf64, wb64, bias64;
+            size_t o = 0;
+            int ci = S;
+            for (int h : sp.hidden) {
+                for (int i = 0; i < ci; ++i)
+                    for (int q = 0; q < h; ++q) wf64.push_back(sp.nn[o + (size_t)q * ci + i]);
+                for (size_t q = 0; q < (size_t)ci * h; ++q) wb64.push_back(sp.nn[o + q]);
+                o += (size_t)ci * h;
+                ci = h;
+            }
+            for (int i = 0; i < ci; ++i) wf64.push_back(sp.nn[o + i]);
+            o += ci;
+            for (size_t q = o; q < sp.nn.size(); ++q) bias64.push_back(sp.nn[q]);
+            d.w_fwd64 = dupload(wf64);
+            d.w_bwd64 = dupload(wb64);
+            d.bias64 = dupload(bias64);
+        }
         std::vector<uint16_t> bf(wf.size()), bb(wb.size());
         for (size_t q = 0; q < wf.size(); ++q) bf[q] = to_bf16(wf[q]);
         for (size_t q = 0; q < wb.size(); ++q) bb[q] = to_bf16(wb[q]);
@@ -83,7 +102,7 @@ void jse_nnap_model_s::upload() {
 
 jse_nnap_model_s::~jse_nnap_model_s() {
     for (auto &d : dev) {
-        dfree(d.W); dfree(d.mu); dfree(d.sig); dfree(d.mu64); dfree(d.sig64); dfree(d.w_fwd); dfree(d.w_bwd); dfree(d.bias);
+        dfree(d.W); dfree(d.W64); dfree(d.mu); dfree(d.sig); dfree(d.mu64); dfree(d.sig64); dfree(d.w_fwd); dfree(d.w_bwd); dfree(d.bias); dfree(d.w_fwd64); dfree(d.w_bwd64); dfree(d.bias64);
         dfree(d.wb_fwd); dfree(d.wb_bwd);
     }
     dfree(zeros);
@@ -93,39 +112,75 @@ jse_nnap_model_s::~jse_nnap_model_s() {
 }
 
 namespace jnn {
-MlpArgs mlp_args(const jse_nnap_model_s *m, int t, int nrows, const int *rows_atom, const float *x, float *G, double *E,
-                 bool grad, int wg_per_cu) {
+void check_model_shapes(const jse_nnap_model_s *m, int precision) {
+    const int nt = m->host.ntypes();
+    for (int t = 0; t < nt; ++t) {
+        const SpeciesHost &sp = m->host.species[t];
+        const int lm = sp.radial ? kRadialLmax : sp.lmax;
+        if (desc_tiled(lm, sp.K(), sp.P, sp.mix)) continue;
+        const std::string why = desc_gen_limit(lm, sp.K(), sp.P, nt, precision_acc(precision));
+        JNN_CHECK(why.empty(), ERR_UNSUPPORTED,
+                  "species " + std::to_string(t + 1) + " (lmax=" + std::to_string(sp.lmax) + " nmax=" +
+                      std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P) + ", " + std::to_string(nt) +
+                      " species): " + why);
+    }
+}
+SegmentArgs species_segment(const jse_nnap_model_s *m, int t, int precision, const int *types) {
+    const SpeciesHost &sp = m->host.species[t];
+    const SpeciesDev &dv = m->dev[t];
+    SegmentArgs a{};
+    a.self_t = t + 1;
+    a.mirror = sp.mirror;
+    a.types = types;
+    a.rc = (float)sp.rcut;
+    a.rc64 = sp.rcut;
+    a.ntypes = m->host.ntypes();
+    a.W = dv.W;
+    a.W64 = dv.W64;
+    a.mu = dv.mu;
+    a.sig = dv.sig;
+    a.mu64 = dv.mu64;
+    a.sig64 = dv.sig64;
+    a.acc = precision_acc(precision) ? 1 : 0;
+    a.fp32_acc = a.acc ? 0 : 1;
+    return a;
+}
+MlpArgs mlp_args(const jse_nnap_model_s *m, int t, const SegmentArgs &sa, double *E, bool grad, int wg_per_cu) {
     const SpeciesHost &sp = m->host.species[t];
     const SpeciesDev &dv = m->dev[t];
     MlpArgs a{};
-    a.nrows = nrows;
+    a.nrows = sa.nrows;
     a.S = sp.S();
     a.L1 = sp.lmax + 1;
     a.radial = sp.radial ? 1 : 0;
     a.nlayers = (int)sp.hidden.size();
     for (int l = 0; l < a.nlayers; ++l) a.hidden[l] = sp.hidden[l];
-    a.rows_atom = rows_atom;
-    a.x = x;
+    a.rows_atom = sa.rows_atom;
+    a.x = sa.x;
+    a.x64 = sa.x64;
     a.w_in_out = dv.w_fwd;
     a.w_out_in = dv.w_bwd;
     a.bias = dv.bias;
     a.sig = dv.sig;
-    a.sig_e = (float)sp.norm[1];
-    a.mu_e = (float)sp.norm[0];
+    a.w_in_out64 = dv.w_fwd64;
+    a.w_out_in64 = dv.w_bwd64;
+    a.bias64 = dv.bias64;
+    a.sig64 = dv.sig64;
+    a.sig_e = sp.norm[1];
+    a.mu_e = sp.norm[0];
     a.E = E;
-    a.G = G;
+    a.G = sa.G;
     a.want_grad = grad;
     a.wg_per_cu = wg_per_cu;
     return a;
 }
-void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision, int nrows, const int *rows_atom,
-                 const float *x, float *G, double *E, bool grad, int wg_per_cu) {
-    const MlpArgs a = mlp_args(m, t, nrows, rows_atom, x, G, E, grad, wg_per_cu);
-    if (precision == JSE_NNAP_BF16_MLP) {
-        if (!launch_mlp_mfma_bf16(s, a))
-            throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
-    } else if (!launch_mlp_mfma_fp32(s, a) && !launch_mlp_fp32(s, a)) {
-        throw Error(ERR_UNSUPPORTED, kMlpFp32Limit);
+void set_segment_x(SegmentArgs &a, float *xbuf, size_t row0, int smax) {
+    if (a.acc) {
+        a.x = nullptr;
+        a.x64 = reinterpret_cast<double *>(xbuf) + row0 * smax;
+    } else {
+        a.x = xbuf + row0 * smax;
+        a.x64 = nullptr;
     }
 }
 void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *segs, bool backward,
@@ -177,13 +232,17 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
     }
 }
 void enqueue_mlp_all(hipStream_t s, const jse_nnap_model_s *m, int precision, const MlpArgs *args, int nsp) {
-    if (launch_mlp_mfma_multi(s, args, nsp, precision == JSE_NNAP_BF16_MLP)) return;
+    const int mode = precision == JSE_NNAP_BF16_MLP ? kMlpBf16 : (precision_acc(precision) ? kMlpFp64 : kMlpFp32);
+    if (launch_mlp_mfma_multi(s, args, nsp, mode)) return;
     for (int t = 0; t < nsp; ++t) {
         const MlpArgs &a = args[t];
         if (a.nrows <= 0) continue;
-        if (precision == JSE_NNAP_BF16_MLP) {
-            if (!launch_mlp_mfma_bf16(s, a))
-                throw Error(ERR_UNSUPPORTED, "the BF16 MLP supports 2 hidden layers of widths 32-32 or 64-64");
+        if (mode == kMlpBf16) {
+            // networks without BF16 tiles (2 hidden layers of 32 or 64) run in FP32, inside the BF16 budget
+            if (!launch_mlp_mfma_bf16(s, a) && !launch_mlp_mfma_fp32(s, a) && !launch_mlp_fp32(s, a))
+                throw Error(ERR_UNSUPPORTED, kMlpFp32Limit);
+        } else if (mode == kMlpFp64) {
+            if (!launch_mlp_mfma_fp64(s, a) && !launch_mlp_fp64(s, a)) throw Error(ERR_UNSUPPORTED, kMlpFp64Limit);
         } else if (!launch_mlp_mfma_fp32(s, a) && !launch_mlp_fp32(s, a)) {
             throw Error(ERR_UNSUPPORTED, kMlpFp32Limit);
         }
@@ -250,7 +309,7 @@ struct jse_nnap_system_s {
             }
     }
     void free_nl() {
-        dfree(nl.cnt); dfree(nl.key); dfree(nl.d); dfree(nl.rev); dfree(nl.g);
+        dfree(nl.cnt); dfree(nl.key); dfree(nl.d); dfree(nl.rev); dfree(nl.g); dfree(nl.dlo);
     }
     void alloc_nl(int cap) {
         drop_graphs();
@@ -263,6 +322,7 @@ struct jse_nnap_system_s {
         nl.d = dalloc<float4>(m);
         nl.rev = dalloc<int>(m);
         nl.g = dalloc<float4>(m);
+        if (precision_acc(precision)) nl.dlo = dalloc<float4>(m);
     }
 
     void grid_of(const double *b, ReplicaGrid &g) const { make_grid(b, model->host.rcut_max(), g); }
@@ -306,7 +366,8 @@ struct jse_nnap_system_s {
         nrep = nr;
         precision = prec;
         JNN_CHECK(nr >= 1, ERR_ARG, "at least one replica is required");
-        JNN_CHECK(prec == JSE_NNAP_FP32 || prec == JSE_NNAP_BF16_MLP, ERR_ARG, "unknown precision");
+        check_precision(prec);
+        check_model_shapes(m, prec);
         long long tot = 0;
         for (int q = 0; q < nr; ++q) {
             JNN_CHECK(natoms_per[q] >= 0, ERR_ARG, "negative atom count");
@@ -359,7 +420,7 @@ struct jse_nnap_system_s {
         d_overflow = dalloc<int>(1);
         JNN_HIP(hipHostMalloc((void **)&h_overflow, sizeof(int), hipHostMallocDefault));
         *h_overflow = 0;
-        d_x = dalloc<float>((size_t)n * m->smax);
+        d_x = dalloc<float>((size_t)n * m->smax * (precision_acc(prec) ? 2 : 1));  // FP64 rows in the plain mode
         d_G = dalloc<float>((size_t)n * m->smax);
         d_A = dalloc<float>((size_t)n * m->pmmax);
         d_E = dalloc<double>(n);
@@ -425,26 +486,19 @@ struct jse_nnap_system_s {
     }
 
     SegmentArgs seg_args(int t, bool raw) const {
-        const SpeciesHost &sp = model->host.species[t];
-        const SpeciesDev &dv = model->dev[t];
-        SegmentArgs a{};
-        a.self_t = t + 1;
-        a.mirror = model->host.species[t].mirror;
+        SegmentArgs a = species_segment(model, t, precision, d_types);
         a.nrows = seg[t + 1] - seg[t];
         a.rows_atom = d_rows + seg[t];
         a.nl_of = d_slot_of;  // neighbour rows sit in cell-sorted slot order
-        a.types = d_types;
-        a.rc = (float)sp.rcut;
-        a.ntypes = model->host.ntypes();
-        a.W = dv.W;
-        a.mu = raw ? model->zeros : dv.mu;
-        a.sig = raw ? model->ones : dv.sig;
-        a.mu64 = raw ? model->zeros64 : dv.mu64;
-        a.sig64 = raw ? model->ones64 : dv.sig64;
-        a.x = d_x + (size_t)seg[t] * model->smax;
+        if (raw) {
+            a.mu = model->zeros;
+            a.sig = model->ones;
+            a.mu64 = model->zeros64;
+            a.sig64 = model->ones64;
+        }
+        set_segment_x(a, d_x, (size_t)seg[t], model->smax);
         a.A = d_A + (size_t)seg[t] * model->pmmax;
         a.G = d_G + (size_t)seg[t] * model->smax;
-        a.fp32_acc = precision == JSE_NNAP_BF16_MLP;
         return a;
     }
 
@@ -469,7 +523,7 @@ struct jse_nnap_system_s {
             std::vector<MlpArgs> ma;
             for (int t = 0; t < model->host.ntypes(); ++t) {
                 SegmentArgs sa = seg_args(t, false);
-                ma.push_back(mlp_args(model, t, sa.nrows, sa.rows_atom, sa.x, sa.G, d_E, grad, tune.mlp_wg_per_cu));
+                ma.push_back(mlp_args(model, t, sa, d_E, grad, tune.mlp_wg_per_cu));
             }
             enqueue_mlp_all(stream, model, precision, ma.data(), (int)ma.size());
         }
@@ -553,7 +607,7 @@ struct jse_nnap_system_s {
         JNN_CHECK(have_pos, ERR_ARG, "positions not set");
         JNN_CHECK(iters >= 1, ERR_ARG, "iters must be >= 1");
         static const std::vector<int> cand[kTuneKnobs] = {
-            {0, 4, 8, 16, 32}, {0, 4, 8, 16, 32}, {1, 2, 4, 8}, {0, 4, 8, 16}, {1, 2, 4, 8}, {0, 1, 2}, {0, 1, 2}};
+            {0, 4, 8, 16, 32}, {0, 4, 8, 16, 32}, {1, 2, 4, 8}, {0, 4, 8, 16}, {1, 2, 4, 8}, {0}, {0, 1, 2}};
         drop_graphs();
         compute(what, true);  // settles the neighbour capacity
         finish();
@@ -637,6 +691,7 @@ int jse_nnap_model_load(const char *json_or_path, jse_nnap_model *out) {
         JNN_CHECK(json_or_path && out, ERR_ARG, "null argument");
         auto m = std::make_unique<jse_nnap_model_s>();
         m->host = parse_model_json(model_text(json_or_path));
+        check_model_shapes(m.get(), JSE_NNAP_FP32_FAST);  // shapes no precision mode can run
         m->upload();
         *out = m.release();
     });
@@ -700,6 +755,7 @@ int jse_nnap_model_create(int ntypes, const int *nmax, const int *lmax, const in
             m->host.species.push_back(std::move(sp));
         }
         finalize_species(m->host);
+        check_model_shapes(m.get(), JSE_NNAP_FP32_FAST);
         m->upload();
         *out = m.release();
     });
@@ -869,18 +925,22 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
         s->finish();
         s->have_nl = true;
         s->desc_fwd(true);
-        std::vector<float> x((size_t)s->n * s->model->smax);
+        // FP64 rows in the plain mode, FP32 rows otherwise
+        const bool f64 = precision_acc(s->precision);
+        std::vector<float> x((size_t)s->n * s->model->smax * (f64 ? 2 : 1));
         std::vector<int> rows(s->n);
         if (s->n) {
             JNN_HIP(hipMemcpyAsync(x.data(), s->d_x, sizeof(float) * x.size(), hipMemcpyDeviceToHost, s->stream));
             JNN_HIP(hipMemcpyAsync(rows.data(), s->d_rows, sizeof(int) * s->n, hipMemcpyDeviceToHost, s->stream));
         }
         JNN_HIP(hipStreamSynchronize(s->stream));
+        const double *x64 = reinterpret_cast<const double *>(x.data());
         for (int t = 0; t < s->model->host.ntypes(); ++t) {
             const int S = s->model->host.species[t].S();
             for (int r = s->seg[t]; r < s->seg[t + 1]; ++r) {
                 double *o = fp + (size_t)rows[r] * smax;
-                for (int e = 0; e < smax; ++e) o[e] = e < S ? (double)x[(size_t)r * s->model->smax + e] : 0.0;
+                const size_t b = (size_t)r * s->model->smax;
+                for (int e = 0; e < smax; ++e) o[e] = e < S ? (f64 ? x64[b + e] : (double)x[b + e]) : 0.0;
             }
         }
     });

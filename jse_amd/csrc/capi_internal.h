@@ -100,8 +100,9 @@ inline void make_grid(const double *b, double rcut, ReplicaGrid &g) {
 
 struct SpeciesDev {
     float *W = nullptr, *mu = nullptr, *sig = nullptr;
-    double *mu64 = nullptr, *sig64 = nullptr;
+    double *W64 = nullptr, *mu64 = nullptr, *sig64 = nullptr;
     float *w_fwd = nullptr, *w_bwd = nullptr, *bias = nullptr;
+    double *w_fwd64 = nullptr, *w_bwd64 = nullptr, *bias64 = nullptr;
     uint16_t *wb_fwd = nullptr, *wb_bwd = nullptr;
 };
 
@@ -119,12 +120,24 @@ struct jse_nnap_model_s {
 };
 
 namespace jnn {
-// Launches the MLP of species t over `nrows` rows (FP32 MFMA, generic FP32, or BF16 MFMA).
-void enqueue_mlp(hipStream_t s, const jse_nnap_model_s *m, int t, int precision, int nrows, const int *rows_atom,
-                 const float *x, float *G, double *E, bool grad, int wg_per_cu = 0);
-// The MlpArgs of species t's rows.
-MlpArgs mlp_args(const jse_nnap_model_s *m, int t, int nrows, const int *rows_atom, const float *x, float *G, double *E,
-                 bool grad, int wg_per_cu = 0);
+// Precision modes (JSE_NNAP_FP32 / BF16_MLP / FP32_FAST): a system's precision is checked once.
+inline void check_precision(int p) {
+    JNN_CHECK(p == JSE_NNAP_FP32 || p == JSE_NNAP_BF16_MLP || p == JSE_NNAP_FP32_FAST, ERR_ARG, "unknown precision");
+}
+// FP64 pair geometry, harmonics and A (the plain mode)
+inline bool precision_acc(int p) { return p == JSE_NNAP_FP32; }
+// Throws status 2 naming the limit when a species' descriptor shape has no
+// kernel that runs it in this precision mode.
+void check_model_shapes(const jse_nnap_model_s *m, int precision);
+// The descriptor fields of species t's segment shared by every caller (rows,
+// outputs and launch shape are the caller's).
+SegmentArgs species_segment(const jse_nnap_model_s *m, int t, int precision, const int *types);
+// The MlpArgs of species t's segment (its rows, descriptors and gradients).
+MlpArgs mlp_args(const jse_nnap_model_s *m, int t, const SegmentArgs &sa, double *E, bool grad, int wg_per_cu = 0);
+// Points the segment's descriptor rows at row0 of a buffer of smax values per
+// row: FP64 values (the buffer holds 2 smax floats per row) in the plain
+// mode, FP32 otherwise.
+void set_segment_x(SegmentArgs &a, float *xbuf, size_t row0, int smax);
 // Every species' descriptor forward (or backward): one launch over all segments
 // when they share one descriptor shape, else one per species.  segs[t] is
 // species t's segment (nrows 0: none).

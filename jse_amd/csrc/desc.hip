@@ -48,50 +48,67 @@ struct DescShape {
 };
 
 // c[p] = fc * sum_n W[t][p][n] Rn[n] for the three mixing specialisations
-template <int K, int P, int MIX>
-__device__ __forceinline__ void mix_rnp(const float *rn, int tj, const float *__restrict__ W, float *rnp) {
+template <int K, int P, int MIX, typename T>
+__device__ __forceinline__ void mix_rnp(const T *rn, int tj, const T *__restrict__ W, T *rnp) {
     if (MIX == MIX_NONE) {
 #pragma unroll
         for (int p = 0; p < P; ++p) rnp[p] = rn[p];
     } else if (MIX == MIX_DEFAULT) {
-        const float wt = (tj & 1) ? (float)tj : -(float)tj;
+        const T wt = (tj & 1) ? (T)tj : -(T)tj;
 #pragma unroll
         for (int q = 0; q < K; ++q) {
             rnp[q] = rn[q];
             rnp[K + q] = wt * rn[q];
         }
     } else {
-        const float *w = W + (size_t)(tj - 1) * P * K;
+        const T *w = W + (size_t)(tj - 1) * P * K;
 #pragma unroll
         for (int p = 0; p < P; ++p) {
-            float s = 0.f;
+            T s = (T)0;
 #pragma unroll
             for (int q = 0; q < K; ++q) s += w[p * K + q] * rn[q];
             rnp[p] = s;
         }
     }
 }
-// TP0: p-rows of each lane's register tile (tile TP0 x 7 of A)
-template <int LMAX, int K, int P, int MIX, int TP0>
-__global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl) {
-    using SH = DescShape<LMAX, K, P, MIX, TP0>;
-    constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
-    constexpr int TP = SH::TP, TM = SH::TM, NMB = SH::NMB, NT = SH::NT, G = SH::G;
-    // phase 1 (per chunk): Y and c of 64 neighbours as FP32; phase 2 (per atom): A in FP64
-    constexpr int F1 = 64 * M + 64 * P;
-    constexpr int F2 = 2 * P * M;
-    __shared__ __align__(16) float sbuf[F1 > F2 ? F1 : F2];
-    float *sY = sbuf, *sC = sbuf + 64 * M;
-    double *sA = reinterpret_cast<double *>(sbuf);
-    const int lane = threadIdx.x;
-    const int grp = lane / NT, tile = lane % NT;
-    const int pb = tile / NMB, mb = tile % NMB;
-    const int p0 = pb * TP, m0 = mb * TM;
-    const bool tile_on = grp < G;
-    // per-lane descriptor slots e = lane + 64 q: 4 pi / (2l + 1), mu, 1 / sigma of the current segment
+// The plain mode's forward on the FP64 matrix cores (v_mfma_f64_16x16x4_f64).
+// Each lane evaluates one neighbour from its FP64 displacement (d + dlo): the
+// pair geometry, Y, Rn, fc and c in FP64.  c goes to LDS as C^T[p][j]; Y stays
+// in registers and is staged one 16-row m-tile at a time as Y^T[m][j]; each
+// m-tile runs ceil(n/4) k-steps of A^T += Y^T(16 m x 4 neighbours) C(4
+// neighbours x 16 p) against the p-tiles of C, all in FP64.  The epilogue
+// writes A[p][m] (FP64) to LDS, stores A as FP32 for the backward and forms fp
+// and x in FP64.  Lane maps of the FP64 instruction: A[i][k] from lane
+// i + 16 k, B[k][j] from lane j + 16 k (as in the FP32 one), and D[i][j] in
+// lane j + 16 (i % 4), element i / 4 (the FP32 one has lane j + 16 (i / 4),
+// element i % 4).  Rows of 65 doubles put the 16 lanes of each
+// quarter-wave of an operand read on distinct bank pairs.  With the default
+// mixing the last m-tile carries the rows w_t Y[m] as in the FP32 kernel.
+template <int LMAX, int K, int P, int MIX>
+__global__ __launch_bounds__(64) void k_desc_fwd_acc(SegmentSet set, NeighborList nl) {
+    constexpr int L1 = LMAX + 1, M = L1 * L1, S = P * L1;
+    constexpr int NMT = (M + 15) / 16, NPT = (P + 15) / 16;
+    constexpr int MLAST = M - 16 * (NMT - 1);  // data rows of the last m-tile
+    constexpr bool WROWS = MIX == MIX_DEFAULT && P == 2 * K && K <= 16 && 2 * MLAST <= 16;
+    constexpr int YLAST = WROWS ? 2 * MLAST : MLAST;  // operand rows of the last m-tile
+    constexpr int NS = 65;
+    constexpr int FC = (P + 1) * NS, FY = 16 * NS, FA = P * M;  // C^T rows, zero row P
+    constexpr int F = FC + (FY > FA ? FY : FA);
     constexpr int NE = (S + 63) / 64;
+    __shared__ double sbuf[F];
+    double *sC = sbuf, *sY = sbuf + FC, *sA = sbuf + FC;
+    using d4 = __attribute__((__vector_size__(4 * sizeof(double)))) double;
+    constexpr d4 kZero = {0.0, 0.0, 0.0, 0.0};
+    const int lane = threadIdx.x;
+    const int li = lane & 15, lk = lane >> 4;
+    sC[P * NS + lane] = 0.0;  // the zero row
+    if (lane < NS - 64) sC[P * NS + 64 + lane] = 0.0;
+    int boff[NPT];
+#pragma unroll
+    for (int pt = 0; pt < NPT; ++pt) boff[pt] = (pt * 16 + li < P ? pt * 16 + li : P) * NS;
+    const int ylast = li < YLAST ? li * NS : (P * NS - FC);
     double ew[NE], emu[NE], eis[NE];
-    float rc = 0.f, inv_rc = 0.f;
+    double rc = 0.0, inv_rc = 0.0;
     int cur = -1;
 
     for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
@@ -101,8 +118,8 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl
         const int row = gr - set.row0[sg];
         if (sg != cur) {
             cur = sg;
-            rc = a.rc;
-            inv_rc = 1.f / a.rc;
+            rc = a.rc64;
+            inv_rc = 1.0 / a.rc64;
 #pragma unroll
             for (int q = 0; q < NE; ++q) {
                 const int e = lane + 64 * q;
@@ -114,65 +131,90 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
-        // FP64 accumulators: every c*Y product of two floats is exact in FP64
-        double acc[TP][TM];
+        d4 acc[NMT][NPT];
 #pragma unroll
-        for (int u = 0; u < TP; ++u)
+        for (int mt = 0; mt < NMT; ++mt)
 #pragma unroll
-            for (int v = 0; v < TM; ++v) acc[u][v] = 0.0;
+            for (int pt = 0; pt < NPT; ++pt) acc[mt][pt] = kZero;
 
         for (int c0 = 0; c0 < n; c0 += 64) {
-            // lanes past the last neighbour evaluate a valid pair; phase 2 reads only
-            // the first nc rows
+            // lanes past the last neighbour evaluate a valid pair with c = 0
             const int j = c0 + lane;
             const int jr = j < n ? j : n - 1;
             const float4 dd = nl.d[base + jr];
             const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
-            const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
-            float y[M];
+            const PairGeomD g = pair_geom64(dd, pair_lo(nl, base + jr));
+            double y[M];
             real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
-            float rn[K];
-            cheb_rn<K>(1.f - 2.f * g.r * inv_rc, rn);
-            const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-            const float t2 = t * t;
-            const float fc = g.r < rc ? t2 * t2 : 0.f;  // pairs beyond this species' cutoff add nothing
-            float rnp[P];
-            mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
-            __syncthreads();  // the previous chunk's tiles (and the previous atom's A) are done
+            double rn[K];
+            cheb_rn<K>(1.0 - 2.0 * g.r * inv_rc, rn);
+            const double t = 1.0 - (g.r * inv_rc) * (g.r * inv_rc);
+            const double t2 = t * t;
+            const double fc = (j < n && g.r < rc) ? t2 * t2 : 0.0;
+            double rnp[P];
+            mix_rnp<K, P, MIX>(rn, tj, a.W64, rnp);
+            const int nks = (min(64, n - c0) + 3) >> 2;
+            const double wt = (tj & 1) ? (double)tj : -(double)tj;  // the default mixing's weight
+            __syncthreads();  // the previous chunk's operand reads (and the previous atom's A) are done
 #pragma unroll
-            for (int m = 0; m < M; ++m) sY[lane * M + m] = y[m];
+            for (int p = 0; p < P; ++p) sC[p * NS + lane] = fc * rnp[p];
 #pragma unroll
-            for (int p = 0; p < P; ++p) sC[lane * P + p] = fc * rnp[p];
-            __syncthreads();
-            const int nc = min(64, n - c0);
-            if (tile_on) {
-                for (int jj = grp; jj < nc; jj += G) {
-                    double cv[TP], yv[TM];
+            for (int mt = 0; mt < NMT; ++mt) {
+                if (mt > 0) __syncthreads();  // the previous m-tile's operand reads are done
 #pragma unroll
-                    for (int u = 0; u < TP; ++u) cv[u] = (p0 + u < P) ? (double)sC[jj * P + p0 + u] : 0.0;
+                for (int r = 0; r < 16; ++r) {
+                    if (mt * 16 + r < M) sY[r * NS + lane] = y[mt * 16 + r];
+                    else if (WROWS && mt == NMT - 1 && r < YLAST) sY[r * NS + lane] = wt * y[mt * 16 + r - MLAST];
+                }
+                const int npt = (WROWS && mt == NMT - 1) ? 1 : NPT;
+                __syncthreads();
+                const int aoff = mt < NMT - 1 ? li * NS : ylast;
+                int ks = 0;
+                for (; ks + 4 <= nks; ks += 4) {
+                    double av[4], bv[4][NPT];
 #pragma unroll
-                    for (int v = 0; v < TM; ++v) yv[v] = (m0 + v < M) ? (double)sY[jj * M + m0 + v] : 0.0;
+                    for (int u = 0; u < 4; ++u) {
+                        const int col = 4 * (ks + u) + lk;
+                        av[u] = sY[aoff + col];
 #pragma unroll
-                    for (int u = 0; u < TP; ++u)
+                        for (int pt = 0; pt < npt; ++pt) bv[u][pt] = sC[boff[pt] + col];
+                    }
 #pragma unroll
-                        for (int v = 0; v < TM; ++v) acc[u][v] += cv[u] * yv[v];
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int pt = 0; pt < npt; ++pt)
+                            acc[mt][pt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av[u], bv[u][pt], acc[mt][pt], 0, 0, 0);
+                }
+                for (; ks < nks; ++ks) {  // the last (shorter) batch
+                    const int col = 4 * ks + lk;
+                    const double av = sY[aoff + col];
+#pragma unroll
+                    for (int pt = 0; pt < npt; ++pt)
+                        acc[mt][pt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sC[boff[pt] + col], acc[mt][pt], 0, 0, 0);
                 }
             }
         }
         __syncthreads();
-        // fixed-order sum of the G partial tiles into sA (FP64), group by group
-        for (int q = 0; q < G; ++q) {
-            if (tile_on && grp == q) {
 #pragma unroll
-                for (int u = 0; u < TP; ++u)
+        for (int mt = 0; mt < NMT; ++mt)
 #pragma unroll
-                    for (int v = 0; v < TM; ++v) {
-                        const int p = p0 + u, m = m0 + v;
-                        if (p < P && m < M) sA[p * M + m] = (q == 0) ? acc[u][v] : sA[p * M + m] + acc[u][v];
+            for (int pt = 0; pt < NPT; ++pt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int ti = 4 * r + lk;  // row of the tile
+                    int m = mt * 16 + ti, p = pt * 16 + li;
+                    bool keep = (mt < NMT - 1 || ti < MLAST) && (pt < NPT - 1 || p < P);
+                    if (WROWS && mt == NMT - 1) {
+                        // data rows give A[n][m] for n < K, the w_t Y rows A[K + n][m]
+                        keep = pt == 0 && li < K && ti < 2 * MLAST;
+                        if (ti >= MLAST) {
+                            m -= MLAST;
+                            p += K;
+                        }
                     }
-            }
-            __syncthreads();
-        }
+                    if (keep) sA[p * M + m] = acc[mt][pt][r];
+                }
+        __syncthreads();
         for (int e = lane; e < P * M; e += 64) a.A[(size_t)row * (P * M) + e] = (float)sA[e];
 #pragma unroll
         for (int q = 0; q < NE; ++q) {
@@ -184,27 +226,22 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl
                     const double v = sA[p * M + m];
                     s += v * v;
                 }
-                a.x[(size_t)row * S + e] = (float)((ew[q] * s - emu[q]) * eis[q]);
+                store_x(a, (size_t)row * S + e, ((ew[q] * s - emu[q]) * eis[q]));
             }
         }
     }
 }
 
-// Forward on the matrix cores (v_mfma_f32_16x16x4_f32).  F64 = false (the
-// BF16-MLP mode): A accumulates in FP32 through all k-steps.  F64 = true (the
-// plain FP32 mode): every batch of four k-steps (16 neighbours) is summed in
-// FP32 and then added to FP64 accumulators, and fp is formed from the FP64 A;
-// this bounds the FP32 error to 16-term partial sums, where accumulating A in
-// FP32 throughout (and rounding it to FP32 before fp) moves the normalised
-// descriptor by ~9e-7 RMS.  Per atom, A^T (M x P) = sum over k-steps of
+// Forward of the FP32-geometry modes on the matrix cores
+// (v_mfma_f32_16x16x4_f32), A accumulated in FP32.  Per atom, A^T (M x P) =
+// sum over k-steps of
 // Y^T (16 m x 4 neighbours) * C (4 neighbours x 16 p).  Lanes first compute one
 // neighbour each: c_j goes to LDS as C^T[p][j]; Y_j stays in registers and is
 // staged into LDS one 16-row m-tile at a time as Y^T[m][j].  Each m-tile then
 // runs ceil(n/4) k-steps against the p-tiles of C, four k-steps per batch of
 // operand reads.  Lanes of padding rows (m >= M, p >= P) read one zero row of
 // LDS; lanes past the last neighbour evaluate a valid pair with c = 0.  The
-// epilogue writes the tiles to LDS as A[p][m] (FP64 when F64) and forms fp
-// and x in FP64.
+// epilogue writes the tiles to LDS as A[p][m] and forms fp and x in FP64.
 // Lane maps of the 16x16x4 instruction: A[i][k] from lane i + 16 k,
 // B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i / 4), register i % 4.
 // A row stride of 68 floats puts the 64 lanes of every operand read on 64
@@ -213,19 +250,18 @@ __global__ __launch_bounds__(64) void k_desc_fwd(SegmentSet set, NeighborList nl
 // the last m-tile, that tile also carries the rows w_t Y[m] below its data
 // rows: against the columns c[n] of p-tile 0 they give A[K + n][m], so the
 // last m-tile needs no other p-tile.
-template <int LMAX, int K, int P, int MIX, bool F64>
+template <int LMAX, int K, int P, int MIX>
 __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList nl) {
-    using AT = typename std::conditional<F64, double, float>::type;  // A's precision after the k-steps
+    using AT = float;
     constexpr int L1 = LMAX + 1, M = L1 * L1, S = P * L1;
     constexpr int NMT = (M + 15) / 16, NPT = (P + 15) / 16;
     constexpr int MLAST = M - 16 * (NMT - 1);  // data rows of the last m-tile
     constexpr bool WROWS = MIX == MIX_DEFAULT && P == 2 * K && K <= 16 && 2 * MLAST <= 16;
     constexpr int YLAST = WROWS ? 2 * MLAST : MLAST;  // operand rows of the last m-tile
     constexpr int NS = 68;
-    constexpr int FC = (P + 1) * NS, FY = 16 * NS, FA = P * M * (F64 ? 2 : 1);  // C^T rows, zero row P
+    constexpr int FC = (P + 1) * NS, FY = 16 * NS, FA = P * M;  // C^T rows, zero row P
     constexpr int F = FC + (FY > FA ? FY : FA);
     constexpr int NE = (S + 63) / 64;
-    static_assert(FC % 2 == 0, "FP64 A in LDS needs 8-byte alignment");
     __shared__ __align__(16) float sbuf[F];
     float *sC = sbuf, *sY = sbuf + FC;
     AT *sA = reinterpret_cast<AT *>(sbuf + FC);
@@ -266,21 +302,11 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
-        // F64: acc holds one batch's FP32 partial sum and a64 the FP64 totals
         f4 acc[NMT][NPT];
-        AT a64[F64 ? NMT : 1][F64 ? NPT : 1][4];
 #pragma unroll
         for (int mt = 0; mt < NMT; ++mt)
 #pragma unroll
             for (int pt = 0; pt < NPT; ++pt) acc[mt][pt] = kZero;
-        if constexpr (F64) {
-#pragma unroll
-            for (int mt = 0; mt < NMT; ++mt)
-#pragma unroll
-                for (int pt = 0; pt < NPT; ++pt)
-#pragma unroll
-                    for (int r = 0; r < 4; ++r) a64[mt][pt][r] = 0.0;
-        }
 
         for (int c0 = 0; c0 < n; c0 += 64) {
             const int j = c0 + lane;
@@ -328,30 +354,14 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
                     for (int u = 0; u < 4; ++u)
 #pragma unroll
                         for (int pt = 0; pt < npt; ++pt)
-                            acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                                av[u], bv[u][pt], (F64 && u == 0) ? kZero : acc[mt][pt], 0, 0, 0);
-                    if constexpr (F64) {
-#pragma unroll
-                        for (int pt = 0; pt < npt; ++pt)
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) a64[mt][pt][r] += (double)acc[mt][pt][r];
-                    }
+                            acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av[u], bv[u][pt], acc[mt][pt], 0, 0, 0);
                 }
-                if (ks < nks) {  // the last (shorter) batch
-                    for (int u = 0; ks < nks; ++ks, ++u) {
-                        const int col = 4 * ks + lk;
-                        const float av = sY[aoff + col];
+                for (; ks < nks; ++ks) {  // the last (shorter) batch
+                    const int col = 4 * ks + lk;
+                    const float av = sY[aoff + col];
 #pragma unroll
-                        for (int pt = 0; pt < npt; ++pt)
-                            acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(
-                                av, sC[boff[pt] + col], (F64 && u == 0) ? kZero : acc[mt][pt], 0, 0, 0);
-                    }
-                    if constexpr (F64) {
-#pragma unroll
-                        for (int pt = 0; pt < npt; ++pt)
-#pragma unroll
-                            for (int r = 0; r < 4; ++r) a64[mt][pt][r] += (double)acc[mt][pt][r];
-                    }
+                    for (int pt = 0; pt < npt; ++pt)
+                        acc[mt][pt] = __builtin_amdgcn_mfma_f32_16x16x4f32(av, sC[boff[pt] + col], acc[mt][pt], 0, 0, 0);
                 }
             }
         }
@@ -373,10 +383,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
                             p += K;
                         }
                     }
-                    if (keep) {
-                        if constexpr (F64) sA[p * M + m] = a64[mt][pt][r];
-                        else sA[p * M + m] = acc[mt][pt][r];
-                    }
+                    if (keep) sA[p * M + m] = acc[mt][pt][r];
                 }
         __syncthreads();
         for (int e = lane; e < P * M; e += 64) a.A[(size_t)row * (P * M) + e] = (float)sA[e];
@@ -390,7 +397,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
                     const double v = sA[p * M + m];
                     s += v * v;
                 }
-                a.x[(size_t)row * S + e] = (float)((ew[q] * s - emu[q]) * eis[q]);
+                store_x(a, (size_t)row * S + e, ((ew[q] * s - emu[q]) * eis[q]));
             }
         }
     }
@@ -408,8 +415,13 @@ __host__ __device__ constexpr int bwd_row(int m) { return (m + 3) & ~3; }
 
 // BAR: a scheduling barrier after every BAR rows of the per-pair contraction
 // (bounds how many rows of LDS operands are in flight; 0: none)
-template <int LMAX, int K, int P, int MIX, int BAR>
+// ACC (the plain mode): each pair's geometry, Y, Rn, fc and their derivatives
+// are evaluated in FP64 from the FP64 displacement (d + dlo); Y, Rn and fc are
+// rounded to FP32 for the dE/dA contraction, which stays FP32; the radial and
+// theta derivatives and the final chain rule are FP64.
+template <int LMAX, int K, int P, int MIX, int BAR, bool ACC>
 __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList nl) {  // two waves per SIMD
+    using GT = typename std::conditional<ACC, double, float>::type;  // geometry precision
     using SH = DescShape<LMAX, K, P, MIX>;
     constexpr int BSTEP = BAR > 0 ? BAR : 1;
     constexpr int M = SH::M, S = SH::S, L1 = SH::L1;
@@ -459,11 +471,11 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
     const int rbeg = min(total, (int)blockIdx.x * per), rend = min(total, rbeg + per);
     struct RowRef {
         int gr, nlrow, n;
-        float rc, inv_rc;
+        GT rc, inv_rc;
         int self_t, mirror;
     };
     auto row_ref = [&](int gr) {
-        RowRef r{gr, 0, 0, 1.f, 1.f, 0, 0};
+        RowRef r{gr, 0, 0, (GT)1, (GT)1, 0, 0};
         if (gr >= rend) return r;
         int sg = 0;  // the row's segment (species)
         while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
@@ -471,8 +483,13 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
         const int row = gr - set.row0[sg];
         r.nlrow = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         r.n = min(nl.cnt[r.nlrow], nl.cap);
-        r.rc = a.rc;
-        r.inv_rc = 1.f / a.rc;
+        if constexpr (ACC) {
+            r.rc = a.rc64;
+            r.inv_rc = 1.0 / a.rc64;
+        } else {
+            r.rc = a.rc;
+            r.inv_rc = 1.f / a.rc;
+        }
         r.self_t = a.self_t;
         r.mirror = a.mirror;
         return r;
@@ -555,7 +572,7 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
             const bool on = inA || lane < cntA + takeB;
             const int j = inA ? kA + lane : lane - cntA;
             const int slot = inA ? slotA : slotB;
-            const float rc = inA ? A.rc : B.rc, inv_rc = inA ? A.inv_rc : B.inv_rc;
+            const GT rc = inA ? A.rc : B.rc, inv_rc = inA ? A.inv_rc : B.inv_rc;
             const int self_t = inA ? A.self_t : B.self_t, mirror = inA ? A.mirror : B.mirror;
             const size_t base = (size_t)(inA ? A.nlrow : B.nlrow) * nl.cap;
             const float4 dd = on ? nl.d[base + j] : make_float4(1.f, 0.f, 0.f, 0.f);
@@ -564,19 +581,33 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
                 pf = rend;
             }
             if (on) {
-                const float dx = dd.x, dy = dd.y, dz = dd.z;
                 const int tj = role_species(tag_species(dd.w), self_t, mirror);
-                const PairGeomR g = pair_geom_fast(dx, dy, dz);
+                // pair geometry in GT: r, 1/r, cos/sin theta, cos/sin phi, 1/dxy, d
+                GT gr_, gir, gct, gst, gcp, gsp, gidxy, gdx, gdy, gdz;
+                bool gpole;
+                if constexpr (ACC) {
+                    const PairGeomD g = pair_geom64(dd, pair_lo(nl, base + j));
+                    gr_ = g.r; gir = g.ir; gct = g.ct; gst = g.st; gcp = g.cp; gsp = g.sp; gidxy = g.idxy;
+                    gdx = g.dx; gdy = g.dy; gdz = g.dz; gpole = g.pole;
+                } else {
+                    const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+                    gr_ = g.r; gir = g.ir; gct = g.ct; gst = g.st; gcp = g.cp; gsp = g.sp; gidxy = g.idxy;
+                    gdx = dd.x; gdy = dd.y; gdz = dd.z; gpole = g.pole;
+                }
                 float ox = 0.f, oy = 0.f, oz = 0.f;
-                if (g.r < rc) {
+                if (gr_ < rc) {
                     float y[M];
-                    real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
-                    const float xr = 1.f - 2.f * g.r * inv_rc;
+                    if constexpr (ACC) real_sph_rows<LMAX, double, float>(gct, gst, gcp, gsp, y);
+                    else real_sph<LMAX>(gct, gst, gcp, gsp, y);
+                    const GT xr = (GT)1 - (GT)2 * gr_ * inv_rc;
+                    GT rnt[K];
+                    cheb_rn<K>(xr, rnt);
                     float rn[K];
-                    cheb_rn<K>(xr, rn);
-                    const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-                    const float t2 = t * t;
-                    const float fc = t2 * t2;
+    #pragma unroll
+                    for (int q = 0; q < K; ++q) rn[q] = (float)rnt[q];
+                    const GT t = (GT)1 - (gr_ * inv_rc) * (gr_ * inv_rc);
+                    const GT t2 = t * t;
+                    const float fc = (float)(t2 * t2);
                     int blk = slot * slot_f4 + (MIX == MIX_NONE ? 0 : (tj - 1) * K * MB);
                     // opaque to the compiler, so every read below is this base plus an immediate
                     // offset (folding the -1 into the reads would make the offsets negative)
@@ -612,13 +643,13 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
                         grn[q] = s * fc;
                         if (BAR > 0 && q % BSTEP == BSTEP - 1) __builtin_amdgcn_sched_barrier(0);
                     }
-                    float drn[K];
+                    GT drn[K];
                     cheb_drn<K>(xr, inv_rc, drn);
-                    float dEdr = 0.f;
+                    GT dEdr = (GT)0;
     #pragma unroll
-                    for (int q = 0; q < K; ++q) dEdr += grn[q] * drn[q];
-                    const float dfc = -8.f * g.r * inv_rc * inv_rc * t2 * t;
-                    dEdr += gfc * dfc;
+                    for (int q = 0; q < K; ++q) dEdr += (GT)grn[q] * drn[q];
+                    const GT dfc = (GT)(-8) * gr_ * inv_rc * inv_rc * t2 * t;
+                    dEdr += (GT)gfc * dfc;
                     float dEdph = 0.f;
     #pragma unroll
                     for (int l = 1; l < L1; ++l) {
@@ -631,17 +662,19 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
                         }
                         dEdph += pl;
                     }
-                    const float dEdth = dtheta_contract<LMAX>(g.ct, g.st, g.cp, g.sp, gy);
-                    const float ir = g.ir;
+                    const GT dEdth = dtheta_contract<LMAX>(gct, gst, gcp, gsp, gy);
+                    const GT ir = gir;
                     // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi))
-                    ox = -(dEdr * dx * ir + dEdth * g.ct * g.cp * ir);
-                    oy = -(dEdr * dy * ir + dEdth * g.ct * g.sp * ir);
-                    oz = -(dEdr * dz * ir - dEdth * g.st * ir);
-                    if (!g.pole) {
-                        const float idxy = g.idxy;
-                        ox -= dEdph * (-g.sp * idxy);
-                        oy -= dEdph * (g.cp * idxy);
+                    GT tx = -(dEdr * gdx * ir + dEdth * gct * gcp * ir);
+                    GT ty = -(dEdr * gdy * ir + dEdth * gct * gsp * ir);
+                    const GT tz = -(dEdr * gdz * ir - dEdth * gst * ir);
+                    if (!gpole) {
+                        tx -= (GT)dEdph * (-gsp * gidxy);
+                        ty -= (GT)dEdph * (gcp * gidxy);
                     }
+                    ox = (float)tx;
+                    oy = (float)ty;
+                    oz = (float)tz;
                 }
                 nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
             }
@@ -708,33 +741,40 @@ __device__ __forceinline__ double wave_sum_f64(double v) {
     return ((readlane_f64(v, 0) + readlane_f64(v, 16)) + readlane_f64(v, 32)) + readlane_f64(v, 48);
 }
 
-template <int K, int P, int MIX>
+// ACC (the plain mode): the displacement d + dlo and every pair term in FP64.
+template <int K, int P, int MIX, bool ACC>
 __global__ __launch_bounds__(64) void k_rad_fwd(SegmentSet set, NeighborList nl) {
+    using GT = typename std::conditional<ACC, double, float>::type;  // pair arithmetic
     const int lane = threadIdx.x;
     for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
         int sg = 0;  // the row's segment (species)
         while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
         const SegmentArgs &a = set.a[sg];
         const int row = gr - set.row0[sg];
-        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const GT rc = ACC ? (GT)a.rc64 : (GT)a.rc, inv_rc = (GT)1 / rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
+        const GT *W = nullptr;
+        if constexpr (ACC) W = a.W64;
+        else W = a.W;
         double acc[P];
 #pragma unroll
         for (int p = 0; p < P; ++p) acc[p] = 0.0;
         for (int j = lane; j < n; j += 64) {
             const float4 dd = nl.d[base + j];
             const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
-            const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
-            if (g.r < rc) {
-                float rn[K];
-                cheb_rn<K>(1.f - 2.f * g.r * inv_rc, rn);
-                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-                const float t2 = t * t;
-                const float fc = t2 * t2;
-                float rnp[P];
-                mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
+            GT r;
+            if constexpr (ACC) r = pair_geom64(dd, pair_lo(nl, base + j)).r;
+            else r = pair_geom_fast(dd.x, dd.y, dd.z).r;
+            if (r < rc) {
+                GT rn[K];
+                cheb_rn<K>((GT)1 - (GT)2 * r * inv_rc, rn);
+                const GT t = (GT)1 - (r * inv_rc) * (r * inv_rc);
+                const GT t2 = t * t;
+                const GT fc = t2 * t2;
+                GT rnp[P];
+                mix_rnp<K, P, MIX>(rn, tj, W, rnp);
 #pragma unroll
                 for (int p = 0; p < P; ++p) acc[p] += (double)(fc * rnp[p]);
             }
@@ -742,7 +782,7 @@ __global__ __launch_bounds__(64) void k_rad_fwd(SegmentSet set, NeighborList nl)
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const double fp = wave_sum_f64(acc[p]);
-            if (lane == p) a.x[(size_t)row * P + p] = (float)((fp - a.mu64[p]) * (1.0 / a.sig64[p]));
+            if (lane == p) store_x(a, (size_t)row * P + p, ((fp - a.mu64[p]) * (1.0 / a.sig64[p])));
         }
     }
 }
@@ -750,66 +790,77 @@ __global__ __launch_bounds__(64) void k_rad_fwd(SegmentSet set, NeighborList nl)
 // Lane per pair: with G[p] = dE/dfp[p] of the atom (the MLP's gradient),
 // dE/dfc = sum_p G[p] Rnp[p] and dE/dRn[n] = fc sum_p G[p] W[t][p][n], so
 // g_j = -(sum_n dE/dRn[n] dRn/dr + dE/dfc dfc/dr) d / r.
-template <int K, int P, int MIX>
+template <int K, int P, int MIX, bool ACC>
 __global__ __launch_bounds__(64) void k_rad_bwd(SegmentSet set, NeighborList nl) {
+    using GT = typename std::conditional<ACC, double, float>::type;  // pair arithmetic
     const int lane = threadIdx.x;
     for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
         int sg = 0;
         while (sg + 1 < set.nseg && gr >= set.row0[sg + 1]) ++sg;
         const SegmentArgs &a = set.a[sg];
         const int row = gr - set.row0[sg];
-        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const GT rc = ACC ? (GT)a.rc64 : (GT)a.rc, inv_rc = (GT)1 / rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
-        float gp[P];
+        const GT *W = nullptr;
+        if constexpr (ACC) W = a.W64;
+        else W = a.W;
+        GT gp[P];
 #pragma unroll
-        for (int p = 0; p < P; ++p) gp[p] = a.G[(size_t)row * P + p];
+        for (int p = 0; p < P; ++p) gp[p] = (GT)a.G[(size_t)row * P + p];
         for (int j = lane; j < n; j += 64) {
             const float4 dd = nl.d[base + j];
             const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
-            const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+            GT r, ir, dx, dy, dz;
+            if constexpr (ACC) {
+                const PairGeomD g = pair_geom64(dd, pair_lo(nl, base + j));
+                r = g.r; ir = g.ir; dx = g.dx; dy = g.dy; dz = g.dz;
+            } else {
+                const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+                r = g.r; ir = g.ir; dx = dd.x; dy = dd.y; dz = dd.z;
+            }
             float ox = 0.f, oy = 0.f, oz = 0.f;
-            if (g.r < rc) {
-                const float xr = 1.f - 2.f * g.r * inv_rc;
-                float rn[K], drn[K];
+            if (r < rc) {
+                const GT xr = (GT)1 - (GT)2 * r * inv_rc;
+                GT rn[K], drn[K];
                 cheb_rn<K>(xr, rn);
                 cheb_drn<K>(xr, inv_rc, drn);
-                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-                const float t2 = t * t;
-                const float fc = t2 * t2;
-                float rnp[P];
-                mix_rnp<K, P, MIX>(rn, tj, a.W, rnp);
-                float gfc = 0.f;
+                const GT t = (GT)1 - (r * inv_rc) * (r * inv_rc);
+                const GT t2 = t * t;
+                const GT fc = t2 * t2;
+                GT rnp[P];
+                mix_rnp<K, P, MIX>(rn, tj, W, rnp);
+                GT gfc = (GT)0;
 #pragma unroll
                 for (int p = 0; p < P; ++p) gfc += gp[p] * rnp[p];
                 // gw[n] = sum_p G[p] W[t][p][n]
-                float gw[K];
+                GT gw[K];
                 if (MIX == MIX_NONE) {
 #pragma unroll
                     for (int q = 0; q < K; ++q) gw[q] = gp[q];
                 } else if (MIX == MIX_DEFAULT) {
-                    const float wt = (tj & 1) ? (float)tj : -(float)tj;
+                    const GT wt = (tj & 1) ? (GT)tj : -(GT)tj;
 #pragma unroll
                     for (int q = 0; q < K; ++q) gw[q] = gp[q] + wt * gp[K + q];
                 } else {
-                    const float *w = a.W + (size_t)(tj - 1) * P * K;
+                    const GT *w = W + (size_t)(tj - 1) * P * K;
 #pragma unroll
                     for (int q = 0; q < K; ++q) {
-                        float sacc = 0.f;
+                        GT sacc = (GT)0;
 #pragma unroll
                         for (int p = 0; p < P; ++p) sacc += gp[p] * w[p * K + q];
                         gw[q] = sacc;
                     }
                 }
-                float dEdr = 0.f;
+                GT dEdr = (GT)0;
 #pragma unroll
                 for (int q = 0; q < K; ++q) dEdr += gw[q] * drn[q];
                 dEdr *= fc;
-                dEdr += gfc * (-8.f * g.r * inv_rc * inv_rc * t2 * t);
-                ox = -dEdr * dd.x * g.ir;
-                oy = -dEdr * dd.y * g.ir;
-                oz = -dEdr * dd.z * g.ir;
+                dEdr += gfc * ((GT)(-8) * r * inv_rc * inv_rc * t2 * t);
+                ox = (float)(-dEdr * dx * ir);
+                oy = (float)(-dEdr * dy * ir);
+                oz = (float)(-dEdr * dz * ir);
             }
             nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
         }
@@ -822,8 +873,13 @@ static void run_rad(hipStream_t s, const SegmentSet &set, const NeighborList &nl
     const int rows = set.row0[set.nseg];
     int grid = rows < 262144 ? rows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
-    if (backward) hipLaunchKernelGGL((k_rad_bwd<K, P, MIX>), dim3(grid), dim3(64), 0, s, set, nl);
-    else hipLaunchKernelGGL((k_rad_fwd<K, P, MIX>), dim3(grid), dim3(64), 0, s, set, nl);
+    if (a.acc) {
+        if (backward) hipLaunchKernelGGL((k_rad_bwd<K, P, MIX, true>), dim3(grid), dim3(64), 0, s, set, nl);
+        else hipLaunchKernelGGL((k_rad_fwd<K, P, MIX, true>), dim3(grid), dim3(64), 0, s, set, nl);
+    } else {
+        if (backward) hipLaunchKernelGGL((k_rad_bwd<K, P, MIX, false>), dim3(grid), dim3(64), 0, s, set, nl);
+        else hipLaunchKernelGGL((k_rad_fwd<K, P, MIX, false>), dim3(grid), dim3(64), 0, s, set, nl);
+    }
 }
 #define JNN_RAD_SHAPES(X) \
     X(6, 6, MIX_NONE)      \
@@ -852,17 +908,12 @@ static void run_fwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl
     const int rows = set.row0[set.nseg];
     int grid = rows < 262144 ? rows : 262144;
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
-    // BF16-MLP mode: the matrix-core kernel with FP32 accumulation.  Plain FP32 mode:
-    // variant 0 = matrix cores with FP64 totals per 16 neighbours; 1 / 2 = vector kernel
-    // with 6x7 / 9x7 FP64 register tiles
-    if (a.fp32_acc)
-        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX, false>), dim3(grid), dim3(64), 0, s, set, nl);
-    else if (a.variant == 1)
-        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 6>), dim3(grid), dim3(64), 0, s, set, nl);
-    else if (a.variant == 2)
-        hipLaunchKernelGGL((k_desc_fwd<LMAX, K, P, MIX, 9>), dim3(grid), dim3(64), 0, s, set, nl);
+    // plain mode: FP64 geometry and A on the FP64 matrix cores; the FP32-geometry
+    // modes: FP32 matrix cores with FP32 accumulation
+    if (a.acc)
+        hipLaunchKernelGGL((k_desc_fwd_acc<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, set, nl);
     else
-        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX, true>), dim3(grid), dim3(64), 0, s, set, nl);
+        hipLaunchKernelGGL((k_desc_fwd_mf<LMAX, K, P, MIX>), dim3(grid), dim3(64), 0, s, set, nl);
 }
 template <int LMAX, int K, int P, int MIX>
 static void run_bwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl) {
@@ -872,12 +923,21 @@ static void run_bwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl
     if (a.wg_per_cu > 0 && grid > a.wg_per_cu * device_cu_count()) grid = a.wg_per_cu * device_cu_count();
     // two slots: the rows a chunk of pairs can straddle
     const size_t lds = 2 * sizeof(float) * (size_t)bwd_blocks<MIX>(a.ntypes) * K * bwd_row((LMAX + 1) * (LMAX + 1));
-    if (a.variant == 1)
-        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 2>), dim3(grid), dim3(64), lds, s, set, nl);
-    else if (a.variant == 2)
-        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 0>), dim3(grid), dim3(64), lds, s, set, nl);
-    else
-        hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 1>), dim3(grid), dim3(64), lds, s, set, nl);
+    if (a.acc) {
+        if (a.variant == 1)
+            hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 2, true>), dim3(grid), dim3(64), lds, s, set, nl);
+        else if (a.variant == 2)
+            hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 0, true>), dim3(grid), dim3(64), lds, s, set, nl);
+        else
+            hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 1, true>), dim3(grid), dim3(64), lds, s, set, nl);
+    } else {
+        if (a.variant == 1)
+            hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 2, false>), dim3(grid), dim3(64), lds, s, set, nl);
+        else if (a.variant == 2)
+            hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 0, false>), dim3(grid), dim3(64), lds, s, set, nl);
+        else
+            hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 1, false>), dim3(grid), dim3(64), lds, s, set, nl);
+    }
 }
 
 #define JNN_SHAPES(X)              \
@@ -917,6 +977,19 @@ bool launch_desc_bwd_set(hipStream_t s, int lmax, int K, int P, int mix, const S
     JNN_SHAPES(X)
 #undef X
     return launch_desc_gen_set(s, lmax, K, P, set, nl, true);
+}
+bool desc_tiled(int lmax, int K, int P, int mix) {
+#define X(L_, K_, P_, M_) \
+    if (lmax == L_ && K == K_ && P == P_ && mix == M_) return true;
+    JNN_SHAPES(X)
+#undef X
+    if (lmax == kRadialLmax) {
+#define X(K_, P_, M_) \
+    if (K == K_ && P == P_ && mix == M_) return true;
+        JNN_RAD_SHAPES(X)
+#undef X
+    }
+    return false;
 }
 static SegmentSet single(const SegmentArgs &a) {
     SegmentSet set{};

@@ -43,8 +43,9 @@ __device__ __forceinline__ int seg_of(const SegmentSet &set, int gr) {
 }
 
 // Rnp[p] = sum_{n < K} w[p][n] rn[n] for one mixing row (w = W[t] + p K)
-__device__ __forceinline__ float mix_row(const float *__restrict__ w, int K, const float *rn) {
-    float s = 0.f;
+template <typename T>
+__device__ __forceinline__ T mix_row(const T *__restrict__ w, int K, const T *rn) {
+    T s = (T)0;
 #pragma unroll
     for (int q = 0; q < kGenKMax; ++q)
         if (q < K) s += w[q] * rn[q];
@@ -60,20 +61,28 @@ __device__ __forceinline__ void stage_w(const SegmentArgs &a, int wsz, float *sW
     __syncthreads();  // every lane is done with the previous segment's copy
     for (int e = threadIdx.x; e < wsz; e += blockDim.x) sW[e] = a.W[e];
 }
+__device__ __forceinline__ void stage_w(const SegmentArgs &a, int wsz, double *sW) {
+    __syncthreads();
+    for (int e = threadIdx.x; e < wsz; e += blockDim.x) sW[e] = a.W64[e];
+}
 
-// LDS: A accumulators (FP64, P*M), then Y (ch x MS floats), c (ch x PS
-// floats), MS and PS the row lengths M and P padded to a multiple of 4, and W.
+// LDS: A accumulators (FP64, P*M), then Y (ch x MS values), c (ch x PS
+// values), MS and PS the row lengths M and P padded to a multiple of 4, and
+// W.  ACC (the plain mode): the displacement d + dlo, the pair geometry, Y,
+// Rn, fc, c and W in FP64 (Y written to LDS by the row recurrence, not held
+// in registers); otherwise FP32 values whose products are exact in FP64.
 constexpr int kFwdThreads = 256;  // four wavefronts per atom in the forward
-template <int LMAX>
+template <int LMAX, bool ACC>
 __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, NeighborList nl, int K, int P, int ch) {
+    using ST = typename std::conditional<ACC, double, float>::type;  // staged values
     constexpr int L1 = LMAX + 1, M = L1 * L1, MS = pad4(M);
     constexpr int NMB = MS / kTile;
     extern __shared__ __align__(16) double gen_smem[];
     double *sA = gen_smem;
     const int PS = pad4(P);
-    float *sY = reinterpret_cast<float *>(sA + pad4(P * M));
-    float *sC = sY + ch * MS;
-    float *sW = sC + ch * PS;
+    ST *sY = reinterpret_cast<ST *>(sA + pad4(P * M));
+    ST *sC = sY + ch * MS;
+    ST *sW = sC + ch * PS;
     const int lane = threadIdx.x;
     const int PM = P * M, S = P * L1;
     const int ntile = (PS / kTile) * NMB;
@@ -87,7 +96,7 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
             stage_w(a, wsz, sW);
             cur = sg;
         }
-        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const ST rc = ACC ? (ST)a.rc64 : (ST)a.rc, inv_rc = (ST)1 / rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
@@ -99,19 +108,29 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
             if (lane < nc) {
                 const float4 dd = nl.d[base + c0 + lane];
                 const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
-                const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
-                float y[M];
-                real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
+                ST r;
+                if constexpr (ACC) {
+                    const PairGeomD g = pair_geom64(dd, pair_lo(nl, base + c0 + lane));
+                    r = g.r;
+                    real_sph_rows<LMAX, double, double>(g.ct, g.st, g.cp, g.sp, sY + lane * MS);
 #pragma unroll
-                for (int m = 0; m < MS; ++m) sY[lane * MS + m] = m < M ? y[m] : 0.f;
-                float rn[kGenKMax];
-                cheb_rn<kGenKMax>(1.f - 2.f * g.r * inv_rc, rn);
-                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-                const float t2 = t * t;
-                const float fc = g.r < rc ? t2 * t2 : 0.f;  // pairs beyond this species' cutoff add nothing
-                const float *w = sW + (tj - 1) * P * K;
+                    for (int m = M; m < MS; ++m) sY[lane * MS + m] = 0.0;
+                } else {
+                    const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+                    r = g.r;
+                    float y[M];
+                    real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
+#pragma unroll
+                    for (int m = 0; m < MS; ++m) sY[lane * MS + m] = m < M ? y[m] : 0.f;
+                }
+                ST rn[kGenKMax];
+                cheb_rn<kGenKMax>((ST)1 - (ST)2 * r * inv_rc, rn);
+                const ST t = (ST)1 - (r * inv_rc) * (r * inv_rc);
+                const ST t2 = t * t;
+                const ST fc = r < rc ? t2 * t2 : (ST)0;  // pairs beyond this species' cutoff add nothing
+                const ST *w = sW + (tj - 1) * P * K;
 #pragma unroll 4
-                for (int p = 0; p < PS; ++p) sC[lane * PS + p] = p < P ? fc * mix_row(w + p * K, K, rn) : 0.f;
+                for (int p = 0; p < PS; ++p) sC[lane * PS + p] = p < P ? fc * mix_row(w + p * K, K, rn) : (ST)0;
             }
             __syncthreads();
             for (int tl = lane; tl < ntile; tl += kFwdThreads) {
@@ -123,10 +142,20 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
                     for (int v = 0; v < kTile; ++v) acc[u][v] = 0.0;
 #pragma unroll 2
                 for (int j = 0; j < nc; ++j) {
-                    const float4 c4 = *reinterpret_cast<const float4 *>(sC + j * PS + p0);
-                    const float4 y4 = *reinterpret_cast<const float4 *>(sY + j * MS + m0);
-                    const double cv[kTile] = {c4.x, c4.y, c4.z, c4.w};
-                    const double yv[kTile] = {y4.x, y4.y, y4.z, y4.w};
+                    double cv[kTile], yv[kTile];
+                    if constexpr (ACC) {
+                        const double2 c0v = *reinterpret_cast<const double2 *>(sC + j * PS + p0);
+                        const double2 c1v = *reinterpret_cast<const double2 *>(sC + j * PS + p0 + 2);
+                        const double2 y0v = *reinterpret_cast<const double2 *>(sY + j * MS + m0);
+                        const double2 y1v = *reinterpret_cast<const double2 *>(sY + j * MS + m0 + 2);
+                        cv[0] = c0v.x; cv[1] = c0v.y; cv[2] = c1v.x; cv[3] = c1v.y;
+                        yv[0] = y0v.x; yv[1] = y0v.y; yv[2] = y1v.x; yv[3] = y1v.y;
+                    } else {
+                        const float4 c4 = *reinterpret_cast<const float4 *>(sC + j * PS + p0);
+                        const float4 y4 = *reinterpret_cast<const float4 *>(sY + j * MS + m0);
+                        cv[0] = c4.x; cv[1] = c4.y; cv[2] = c4.z; cv[3] = c4.w;
+                        yv[0] = y4.x; yv[1] = y4.y; yv[2] = y4.z; yv[3] = y4.w;
+                    }
 #pragma unroll
                     for (int u = 0; u < kTile; ++u)
 #pragma unroll
@@ -149,7 +178,7 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
                 s += v * v;
             }
             const double w4 = 4.0 * 3.14159265358979323846 / (double)(2 * l + 1);
-            a.x[(size_t)row * S + e] = (float)((w4 * s - a.mu64[e]) * (1.0 / a.sig64[e]));
+            store_x(a, (size_t)row * S + e, ((w4 * s - a.mu64[e]) * (1.0 / a.sig64[e])));
         }
     }
 }
@@ -157,8 +186,11 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
 // LDS: dE/dA of the current atom (P rows of MP floats, M padded to a multiple
 // of 4), its fold through each species' mixing, Gt[t][n][m] = sum_p W[t][p][n]
 // dE/dA[p][m] (ntypes x K rows of MP floats), then W.
-template <int LMAX>
+// ACC (the plain mode): the pair geometry, Y, Rn, fc and the derivatives in
+// FP64 from d + dlo; Y, Rn and fc rounded to FP32 for the contraction.
+template <int LMAX, bool ACC>
 __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl, int K, int P) {
+    using GT = typename std::conditional<ACC, double, float>::type;  // geometry precision
     constexpr int L1 = LMAX + 1, M = L1 * L1, MP = pad4(M);
     extern __shared__ __align__(16) double gen_smem[];
     const int nt = set.a[0].ntypes;
@@ -177,7 +209,7 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
             stage_w(a, wsz, sW);
             cur = sg;
         }
-        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const GT rc = ACC ? (GT)a.rc64 : (GT)a.rc, inv_rc = (GT)1 / rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
@@ -201,19 +233,32 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
         __syncthreads();
         for (int j = lane; j < n; j += 64) {
             const float4 dd = nl.d[base + j];
-            const float dx = dd.x, dy = dd.y, dz = dd.z;
             const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
-            const PairGeomR g = pair_geom_fast(dx, dy, dz);
+            GT gr_, gir, gct, gst, gcp, gsp, gidxy, gdx, gdy, gdz;
+            bool gpole;
+            if constexpr (ACC) {
+                const PairGeomD g = pair_geom64(dd, pair_lo(nl, base + j));
+                gr_ = g.r; gir = g.ir; gct = g.ct; gst = g.st; gcp = g.cp; gsp = g.sp; gidxy = g.idxy;
+                gdx = g.dx; gdy = g.dy; gdz = g.dz; gpole = g.pole;
+            } else {
+                const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+                gr_ = g.r; gir = g.ir; gct = g.ct; gst = g.st; gcp = g.cp; gsp = g.sp; gidxy = g.idxy;
+                gdx = dd.x; gdy = dd.y; gdz = dd.z; gpole = g.pole;
+            }
             float ox = 0.f, oy = 0.f, oz = 0.f;
-            if (g.r < rc) {
+            if (gr_ < rc) {
                 float y[M];
-                real_sph<LMAX>(g.ct, g.st, g.cp, g.sp, y);
-                const float xr = 1.f - 2.f * g.r * inv_rc;
+                if constexpr (ACC) real_sph_rows<LMAX, double, float>(gct, gst, gcp, gsp, y);
+                else real_sph<LMAX>(gct, gst, gcp, gsp, y);
+                const GT xr = (GT)1 - (GT)2 * gr_ * inv_rc;
+                GT rnt[kGenKMax];
+                cheb_rn<kGenKMax>(xr, rnt);
                 float rn[kGenKMax];
-                cheb_rn<kGenKMax>(xr, rn);
-                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-                const float t2 = t * t;
-                const float fc = t2 * t2;
+#pragma unroll
+                for (int q = 0; q < kGenKMax; ++q) rn[q] = (float)rnt[q];
+                const GT t = (GT)1 - (gr_ * inv_rc) * (gr_ * inv_rc);
+                const GT t2 = t * t;
+                const float fc = (float)(t2 * t2);
                 const float *gt = sGt + (tj - 1) * K * MP;
                 // gY[m] = fc sum_n Rn[n] Gt[n][m],  gct[n] = sum_m Gt[n][m] Y[m]
                 // (then dE/dRn = fc gct, dE/dfc = sum_n Rn[n] gct[n])
@@ -247,13 +292,13 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
                         grn[q] = sq * fc;
                     }
                 }
-                float drn[kGenKMax];
+                GT drn[kGenKMax];
                 cheb_drn<kGenKMax>(xr, inv_rc, drn);
-                float dEdr = 0.f;
+                GT dEdr = (GT)0;
 #pragma unroll
                 for (int q = 0; q < kGenKMax; ++q)
-                    if (q < K) dEdr += grn[q] * drn[q];
-                dEdr += gfc * (-8.f * g.r * inv_rc * inv_rc * t2 * t);
+                    if (q < K) dEdr += (GT)grn[q] * drn[q];
+                dEdr += (GT)gfc * ((GT)(-8) * gr_ * inv_rc * inv_rc * t2 * t);
                 float dEdph = 0.f;
 #pragma unroll
                 for (int l = 1; l < L1; ++l) {
@@ -266,16 +311,19 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
                     }
                     dEdph += pl;
                 }
-                const float dEdth = dtheta_contract<LMAX>(g.ct, g.st, g.cp, g.sp, gy);
-                const float ir = g.ir;
+                const GT dEdth = dtheta_contract<LMAX>(gct, gst, gcp, gsp, gy);
+                const GT ir = gir;
                 // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi))
-                ox = -(dEdr * dx * ir + dEdth * g.ct * g.cp * ir);
-                oy = -(dEdr * dy * ir + dEdth * g.ct * g.sp * ir);
-                oz = -(dEdr * dz * ir - dEdth * g.st * ir);
-                if (!g.pole) {
-                    ox -= dEdph * (-g.sp * g.idxy);
-                    oy -= dEdph * (g.cp * g.idxy);
+                GT tx = -(dEdr * gdx * ir + dEdth * gct * gcp * ir);
+                GT ty = -(dEdr * gdy * ir + dEdth * gct * gsp * ir);
+                const GT tz = -(dEdr * gdz * ir - dEdth * gst * ir);
+                if (!gpole) {
+                    tx -= (GT)dEdph * (-gsp * gidxy);
+                    ty -= (GT)dEdph * (gcp * gidxy);
                 }
+                ox = (float)tx;
+                oy = (float)ty;
+                oz = (float)tz;
             }
             nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
         }
@@ -285,10 +333,12 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
 // Radial-only basis, forward: chunks of 64 neighbours put c_j[p] = fc Rnp[p]
 // into LDS; lane p (and p + 64) sums them in FP64, neighbours in list order.
 // LDS: c (64 x P floats), then W.
+template <bool ACC>
 __global__ __launch_bounds__(64) void k_gen_rad_fwd(SegmentSet set, NeighborList nl, int K, int P) {
+    using ST = typename std::conditional<ACC, double, float>::type;  // pair arithmetic and staged values
     extern __shared__ __align__(16) double gen_smem[];
-    float *sC = reinterpret_cast<float *>(gen_smem);
-    float *sW = sC + 64 * P;
+    ST *sC = reinterpret_cast<ST *>(gen_smem);
+    ST *sW = sC + 64 * P;
     const int lane = threadIdx.x;
     const int wsz = set.a[0].ntypes * P * K;
     int cur = -1;
@@ -300,7 +350,7 @@ __global__ __launch_bounds__(64) void k_gen_rad_fwd(SegmentSet set, NeighborList
             stage_w(a, wsz, sW);
             cur = sg;
         }
-        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const ST rc = ACC ? (ST)a.rc64 : (ST)a.rc, inv_rc = (ST)1 / rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
@@ -311,13 +361,15 @@ __global__ __launch_bounds__(64) void k_gen_rad_fwd(SegmentSet set, NeighborList
             if (lane < nc) {
                 const float4 dd = nl.d[base + c0 + lane];
                 const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
-                const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
-                float rn[kGenKMax];
-                cheb_rn<kGenKMax>(1.f - 2.f * g.r * inv_rc, rn);
-                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-                const float t2 = t * t;
-                const float fc = g.r < rc ? t2 * t2 : 0.f;
-                const float *w = sW + (tj - 1) * P * K;
+                ST r;
+                if constexpr (ACC) r = pair_geom64(dd, pair_lo(nl, base + c0 + lane)).r;
+                else r = pair_geom_fast(dd.x, dd.y, dd.z).r;
+                ST rn[kGenKMax];
+                cheb_rn<kGenKMax>((ST)1 - (ST)2 * r * inv_rc, rn);
+                const ST t = (ST)1 - (r * inv_rc) * (r * inv_rc);
+                const ST t2 = t * t;
+                const ST fc = r < rc ? t2 * t2 : (ST)0;
+                const ST *w = sW + (tj - 1) * P * K;
                 for (int p = 0; p < P; ++p) sC[lane * P + p] = fc * mix_row(w + p * K, K, rn);
             }
             __syncthreads();
@@ -326,19 +378,21 @@ __global__ __launch_bounds__(64) void k_gen_rad_fwd(SegmentSet set, NeighborList
                 if (lane + 64 < P) acc1 += (double)sC[j * P + lane + 64];
             }
         }
-        if (lane < P) a.x[(size_t)row * P + lane] = (float)((acc0 - a.mu64[lane]) * (1.0 / a.sig64[lane]));
+        if (lane < P) store_x(a, (size_t)row * P + lane, ((acc0 - a.mu64[lane]) * (1.0 / a.sig64[lane])));
         if (lane + 64 < P)
-            a.x[(size_t)row * P + lane + 64] = (float)((acc1 - a.mu64[lane + 64]) * (1.0 / a.sig64[lane + 64]));
+            store_x(a, (size_t)row * P + lane + 64, ((acc1 - a.mu64[lane + 64]) * (1.0 / a.sig64[lane + 64])));
     }
 }
 
 // Radial-only basis, backward: lane per pair, dE/dfc = sum_p G[p] Rnp[p] and
 // dE/dRn[n] = fc sum_p G[p] W[t][p][n] with G = dE/dfp of the atom.
 // LDS: G (P floats), then W.
+template <bool ACC>
 __global__ __launch_bounds__(64) void k_gen_rad_bwd(SegmentSet set, NeighborList nl, int K, int P) {
+    using ST = typename std::conditional<ACC, double, float>::type;  // pair arithmetic and staged values
     extern __shared__ __align__(16) double gen_smem[];
-    float *sG = reinterpret_cast<float *>(gen_smem);
-    float *sW = sG + P;
+    ST *sW = reinterpret_cast<ST *>(gen_smem);
+    float *sG = reinterpret_cast<float *>(sW + set.a[0].ntypes * P * K);
     const int lane = threadIdx.x;
     const int wsz = set.a[0].ntypes * P * K;
     int cur = -1;
@@ -350,7 +404,7 @@ __global__ __launch_bounds__(64) void k_gen_rad_bwd(SegmentSet set, NeighborList
             stage_w(a, wsz, sW);
             cur = sg;
         }
-        const float rc = a.rc, inv_rc = 1.f / a.rc;
+        const ST rc = ACC ? (ST)a.rc64 : (ST)a.rc, inv_rc = (ST)1 / rc;
         const int i = a.nl_of ? a.nl_of[a.rows_atom[row]] : a.rows_atom[row];
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
@@ -360,38 +414,45 @@ __global__ __launch_bounds__(64) void k_gen_rad_bwd(SegmentSet set, NeighborList
         for (int j = lane; j < n; j += 64) {
             const float4 dd = nl.d[base + j];
             const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
-            const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+            ST r, ir, dx, dy, dz;
+            if constexpr (ACC) {
+                const PairGeomD g = pair_geom64(dd, pair_lo(nl, base + j));
+                r = g.r; ir = g.ir; dx = g.dx; dy = g.dy; dz = g.dz;
+            } else {
+                const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
+                r = g.r; ir = g.ir; dx = dd.x; dy = dd.y; dz = dd.z;
+            }
             float ox = 0.f, oy = 0.f, oz = 0.f;
-            if (g.r < rc) {
-                const float xr = 1.f - 2.f * g.r * inv_rc;
-                float rn[kGenKMax], drn[kGenKMax];
+            if (r < rc) {
+                const ST xr = (ST)1 - (ST)2 * r * inv_rc;
+                ST rn[kGenKMax], drn[kGenKMax];
                 cheb_rn<kGenKMax>(xr, rn);
                 cheb_drn<kGenKMax>(xr, inv_rc, drn);
-                const float t = 1.f - (g.r * inv_rc) * (g.r * inv_rc);
-                const float t2 = t * t;
-                const float fc = t2 * t2;
-                const float *w = sW + (tj - 1) * P * K;
-                float gw[kGenKMax];
+                const ST t = (ST)1 - (r * inv_rc) * (r * inv_rc);
+                const ST t2 = t * t;
+                const ST fc = t2 * t2;
+                const ST *w = sW + (tj - 1) * P * K;
+                ST gw[kGenKMax];
 #pragma unroll
-                for (int q = 0; q < kGenKMax; ++q) gw[q] = 0.f;
-                float gfc = 0.f;
+                for (int q = 0; q < kGenKMax; ++q) gw[q] = (ST)0;
+                ST gfc = (ST)0;
                 for (int p = 0; p < P; ++p) {
-                    const float *wp = w + p * K;
-                    const float gp = sG[p];
+                    const ST *wp = w + p * K;
+                    const ST gp = (ST)sG[p];
                     gfc += gp * mix_row(wp, K, rn);
 #pragma unroll
                     for (int q = 0; q < kGenKMax; ++q)
                         if (q < K) gw[q] += gp * wp[q];
                 }
-                float dEdr = 0.f;
+                ST dEdr = (ST)0;
 #pragma unroll
                 for (int q = 0; q < kGenKMax; ++q)
                     if (q < K) dEdr += gw[q] * drn[q];
                 dEdr *= fc;
-                dEdr += gfc * (-8.f * g.r * inv_rc * inv_rc * t2 * t);
-                ox = -dEdr * dd.x * g.ir;
-                oy = -dEdr * dd.y * g.ir;
-                oz = -dEdr * dd.z * g.ir;
+                dEdr += gfc * ((ST)(-8) * r * inv_rc * inv_rc * t2 * t);
+                ox = (float)(-dEdr * dx * ir);
+                oy = (float)(-dEdr * dy * ir);
+                oz = (float)(-dEdr * dz * ir);
             }
             nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
         }
@@ -403,37 +464,87 @@ int gen_grid(const SegmentSet &set) {
     return rows < 262144 ? rows : 262144;
 }
 
-// LDS of the forward for a chunk of ch neighbours, wsz floats of W
-size_t gen_fwd_lds(int M, int P, int ch, int wsz) {
-    return sizeof(double) * (size_t)pad4(P * M) + sizeof(float) * ((size_t)ch * (pad4(M) + pad4(P)) + wsz);
+// LDS of the forward for a chunk of ch neighbours and wsz mixing weights; v
+// is the size of a staged value (4, or 8 in the plain mode)
+size_t gen_fwd_lds(int M, int P, int ch, int wsz, size_t v) {
+    return sizeof(double) * (size_t)pad4(P * M) + v * ((size_t)ch * (pad4(M) + pad4(P)) + wsz);
+}
+size_t gen_bwd_lds(int M, int P, int K, int nt) {
+    return sizeof(float) * ((size_t)(P + nt * K) * pad4(M) + (size_t)nt * P * K);
+}
+// the largest forward chunk (64, halved down to 8) that fits, 0 if none
+int gen_fwd_chunk(int M, int P, int wsz, size_t v) {
+    int ch = 64;
+    while (ch > 8 && gen_fwd_lds(M, P, ch, wsz, v) > kGenLds) ch /= 2;
+    return gen_fwd_lds(M, P, ch, wsz, v) > kGenLds ? 0 : ch;
 }
 
-template <int LMAX>
+template <int LMAX, bool ACC>
 bool run_gen(hipStream_t s, int K, int P, const SegmentSet &set, const NeighborList &nl, bool backward) {
-    const int wsz = set.a[0].ntypes * P * K;
+    const int nt = set.a[0].ntypes;
+    const int wsz = nt * P * K;
     constexpr int M = (LMAX + 1) * (LMAX + 1);
     // once per process (thread-safe static initialisation; retried if it throws)
     static const bool attr_set = [] {
-        JNN_HIP(hipFuncSetAttribute((const void *)k_gen_fwd<LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        JNN_HIP(hipFuncSetAttribute((const void *)k_gen_fwd<LMAX, ACC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)kGenLds));
-        JNN_HIP(hipFuncSetAttribute((const void *)k_gen_bwd<LMAX>, hipFuncAttributeMaxDynamicSharedMemorySize,
+        JNN_HIP(hipFuncSetAttribute((const void *)k_gen_bwd<LMAX, ACC>, hipFuncAttributeMaxDynamicSharedMemorySize,
                                     (int)kGenLds));
         return true;
     }();
     (void)attr_set;
     const int grid = gen_grid(set);
     if (backward) {
-        const size_t lds = sizeof(float) * ((size_t)(P + set.a[0].ntypes * K) * pad4(M) + wsz);
+        const size_t lds = gen_bwd_lds(M, P, K, nt);
         if (lds > kGenLds) return false;
-        hipLaunchKernelGGL((k_gen_bwd<LMAX>), dim3(grid), dim3(64), lds, s, set, nl, K, P);
+        hipLaunchKernelGGL((k_gen_bwd<LMAX, ACC>), dim3(grid), dim3(64), lds, s, set, nl, K, P);
         return true;
     }
-    int ch = 64;  // the largest chunk that fits
-    while (ch > 8 && gen_fwd_lds(M, P, ch, wsz) > kGenLds) ch /= 2;
-    if (gen_fwd_lds(M, P, ch, wsz) > kGenLds) return false;
-    hipLaunchKernelGGL((k_gen_fwd<LMAX>), dim3(grid), dim3(kFwdThreads), gen_fwd_lds(M, P, ch, wsz), s, set, nl, K, P,
-                       ch);
+    const size_t v = ACC ? sizeof(double) : sizeof(float);
+    const int ch = gen_fwd_chunk(M, P, wsz, v);
+    if (!ch) return false;
+    hipLaunchKernelGGL((k_gen_fwd<LMAX, ACC>), dim3(grid), dim3(kFwdThreads), gen_fwd_lds(M, P, ch, wsz, v), s, set,
+                       nl, K, P, ch);
     return true;
+}
+template <bool ACC>
+bool run_gen_rad(hipStream_t s, int K, int P, const SegmentSet &set, const NeighborList &nl, bool backward) {
+    static const bool attr_set = [] {
+        JNN_HIP(hipFuncSetAttribute((const void *)k_gen_rad_fwd<ACC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kGenLds));
+        JNN_HIP(hipFuncSetAttribute((const void *)k_gen_rad_bwd<ACC>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    (int)kGenLds));
+        return true;
+    }();
+    (void)attr_set;
+    const size_t v = ACC ? sizeof(double) : sizeof(float);
+    const size_t wsz = (size_t)set.a[0].ntypes * P * K;
+    const size_t lds = backward ? v * wsz + sizeof(float) * P : v * (64 * (size_t)P + wsz);
+    if (lds > kGenLds) return false;
+    const int grid = gen_grid(set);
+    if (backward) hipLaunchKernelGGL((k_gen_rad_bwd<ACC>), dim3(grid), dim3(64), lds, s, set, nl, K, P);
+    else hipLaunchKernelGGL((k_gen_rad_fwd<ACC>), dim3(grid), dim3(64), lds, s, set, nl, K, P);
+    return true;
+}
+template <bool ACC>
+bool launch_gen(hipStream_t s, int lmax, int K, int P, const SegmentSet &set, const NeighborList &nl, bool backward) {
+    switch (lmax) {
+        case kRadialLmax: return run_gen_rad<ACC>(s, K, P, set, nl, backward);
+        case 0: return run_gen<0, ACC>(s, K, P, set, nl, backward);
+        case 1: return run_gen<1, ACC>(s, K, P, set, nl, backward);
+        case 2: return run_gen<2, ACC>(s, K, P, set, nl, backward);
+        case 3: return run_gen<3, ACC>(s, K, P, set, nl, backward);
+        case 4: return run_gen<4, ACC>(s, K, P, set, nl, backward);
+        case 5: return run_gen<5, ACC>(s, K, P, set, nl, backward);
+        case 6: return run_gen<6, ACC>(s, K, P, set, nl, backward);
+        case 7: return run_gen<7, ACC>(s, K, P, set, nl, backward);
+        case 8: return run_gen<8, ACC>(s, K, P, set, nl, backward);
+        case 9: return run_gen<9, ACC>(s, K, P, set, nl, backward);
+        case 10: return run_gen<10, ACC>(s, K, P, set, nl, backward);
+        case 11: return run_gen<11, ACC>(s, K, P, set, nl, backward);
+        case 12: return run_gen<12, ACC>(s, K, P, set, nl, backward);
+        default: return false;
+    }
 }
 
 }  // namespace
@@ -442,39 +553,27 @@ bool launch_desc_gen_set(hipStream_t s, int lmax, int K, int P, const SegmentSet
                          bool backward) {
     if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
     if (K < 1 || K > kGenKMax || P < 1 || P > kGenPMax) return false;
+    return set.a[0].acc ? launch_gen<true>(s, lmax, K, P, set, nl, backward)
+                        : launch_gen<false>(s, lmax, K, P, set, nl, backward);
+}
+
+std::string desc_gen_limit(int lmax, int K, int P, int ntypes, bool acc) {
+    if (K < 1 || K > kGenKMax || P < 1 || P > kGenPMax)
+        return "nmax + 1 <= " + std::to_string(kGenKMax) + " and size_np <= " + std::to_string(kGenPMax);
+    const size_t v = acc ? sizeof(double) : sizeof(float);
+    const size_t wsz = (size_t)ntypes * P * K;
+    const size_t cap = kGenLds;
     if (lmax == kRadialLmax) {
-        static const bool attr_set = [] {
-            JNN_HIP(hipFuncSetAttribute((const void *)k_gen_rad_fwd, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)kGenLds));
-            JNN_HIP(hipFuncSetAttribute((const void *)k_gen_rad_bwd, hipFuncAttributeMaxDynamicSharedMemorySize,
-                                        (int)kGenLds));
-            return true;
-        }();
-        (void)attr_set;
-        const size_t wsz = (size_t)set.a[0].ntypes * P * K;
-        const size_t lds = sizeof(float) * ((backward ? (size_t)P : 64 * (size_t)P) + wsz);
-        if (lds > kGenLds) return false;
-        const int grid = gen_grid(set);
-        if (backward) hipLaunchKernelGGL(k_gen_rad_bwd, dim3(grid), dim3(64), lds, s, set, nl, K, P);
-        else hipLaunchKernelGGL(k_gen_rad_fwd, dim3(grid), dim3(64), lds, s, set, nl, K, P);
-        return true;
+        if (v * (64 * (size_t)P + wsz) > cap || v * wsz + sizeof(float) * P > cap)
+            return "the radial descriptor kernels need (64 size_np + ntypes size_np (nmax + 1)) values within 160 KB of LDS";
+        return "";
     }
-    switch (lmax) {
-        case 0: return run_gen<0>(s, K, P, set, nl, backward);
-        case 1: return run_gen<1>(s, K, P, set, nl, backward);
-        case 2: return run_gen<2>(s, K, P, set, nl, backward);
-        case 3: return run_gen<3>(s, K, P, set, nl, backward);
-        case 4: return run_gen<4>(s, K, P, set, nl, backward);
-        case 5: return run_gen<5>(s, K, P, set, nl, backward);
-        case 6: return run_gen<6>(s, K, P, set, nl, backward);
-        case 7: return run_gen<7>(s, K, P, set, nl, backward);
-        case 8: return run_gen<8>(s, K, P, set, nl, backward);
-        case 9: return run_gen<9>(s, K, P, set, nl, backward);
-        case 10: return run_gen<10>(s, K, P, set, nl, backward);
-        case 11: return run_gen<11>(s, K, P, set, nl, backward);
-        case 12: return run_gen<12>(s, K, P, set, nl, backward);
-        default: return false;
-    }
+    const int M = (lmax + 1) * (lmax + 1);
+    if (!gen_fwd_chunk(M, P, (int)wsz, v))
+        return "the descriptor forward needs 8 P M + 8 (M + size_np) + ntypes size_np (nmax + 1) values within 160 KB of LDS";
+    if (gen_bwd_lds(M, P, K, ntypes) > cap)
+        return "the descriptor backward needs (size_np + ntypes (nmax + 1)) (lmax + 1)^2 + ntypes size_np (nmax + 1) floats within 160 KB of LDS";
+    return "";
 }
 
 }  // namespace jnn

@@ -1,4 +1,5 @@
-// Device-side building blocks of the spherical-Chebyshev descriptor (FP32):
+// Device-side building blocks of the spherical-Chebyshev descriptor (FP32,
+// and FP64 where a template parameter says so):
 // radial Chebyshev basis, smooth cutoff, real spherical harmonics and their
 // theta-derivatives.  The functions define the same quantities as the
 // reference's nnap_util.hpp:226-271 and basis_SphericalUtil.hpp:12-182; the
@@ -19,109 +20,164 @@ constexpr double csqrt(double x) {
 
 constexpr int kMaxL = 12;
 
+template <typename T>
 struct ShTables {
     // P_lm = A[l][m] (cos t P_{l-1,m} + B[l][m] P_{l-2,m}) for m <= l-2
-    float A[kMaxL + 1][kMaxL + 1];
-    float B[kMaxL + 1][kMaxL + 1];
-    float D1[kMaxL + 1];  // P_{l,l-1} = D1[l] cos t P_{l-1,l-1},  D1 = sqrt(2l+1)
-    float D2[kMaxL + 1];  // P_{l,l}   = D2[l] sin t P_{l-1,l-1},  D2 = -sqrt((2l+1)/(2l))  (Condon-Shortley)
+    T A[kMaxL + 1][kMaxL + 1];
+    T B[kMaxL + 1][kMaxL + 1];
+    T D1[kMaxL + 1];  // P_{l,l-1} = D1[l] cos t P_{l-1,l-1},  D1 = sqrt(2l+1)
+    T D2[kMaxL + 1];  // P_{l,l}   = D2[l] sin t P_{l-1,l-1},  D2 = -sqrt((2l+1)/(2l))  (Condon-Shortley)
     constexpr ShTables() : A(), B(), D1(), D2() {
         for (int l = 0; l <= kMaxL; ++l) {
-            D1[l] = (float)csqrt(2.0 * l + 1.0);
-            D2[l] = l > 0 ? (float)(-csqrt((2.0 * l + 1.0) / (2.0 * l))) : 0.f;
+            D1[l] = (T)csqrt(2.0 * l + 1.0);
+            D2[l] = l > 0 ? (T)(-csqrt((2.0 * l + 1.0) / (2.0 * l))) : (T)0;
             for (int m = 0; m <= kMaxL; ++m) {
                 if (l >= 2 && m <= l - 2) {
-                    A[l][m] = (float)csqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
-                    B[l][m] = (float)(-csqrt((((double)(l - 1) * (l - 1)) - (double)m * m) /
-                                             (4.0 * (l - 1) * (l - 1) - 1.0)));
+                    A[l][m] = (T)csqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
+                    B[l][m] = (T)(-csqrt((((double)(l - 1) * (l - 1)) - (double)m * m) /
+                                         (4.0 * (l - 1) * (l - 1) - 1.0)));
                 } else {
-                    A[l][m] = 0.f;
-                    B[l][m] = 0.f;
+                    A[l][m] = (T)0;
+                    B[l][m] = (T)0;
                 }
             }
         }
     }
 };
-static constexpr ShTables kSh{};
+static constexpr ShTables<float> kSh{};
+static constexpr ShTables<double> kShD{};
+template <typename T>
+__device__ __forceinline__ constexpr const ShTables<T> &sh_tables() {
+    if constexpr (sizeof(T) == 8) return kShD;
+    else return kSh;
+}
 
 constexpr float kInvSqrt4Pi = 0.28209479177387814f;
 constexpr float kSqrt2 = 1.4142135623730951f;
 constexpr float kPi4 = 12.566370614359172f;
+constexpr double kInvSqrt4PiD = 0.28209479177387814347;
+constexpr double kSqrt2D = 1.41421356237309504880;
 
-// Normalised associated Legendre P_lm(cos t) for m >= 0, stored at l*l+l+m.
-template <int LMAX>
-__device__ __forceinline__ void legendre(float ct, float st, float *y) {
-    float pll = kInvSqrt4Pi;
+// Normalised associated Legendre P_lm(cos t) for m >= 0, stored at l*l+l+m
+// (T = float or double: the arithmetic precision).
+template <int LMAX, typename T>
+__device__ __forceinline__ void legendre(T ct, T st, T *y) {
+    const ShTables<T> &sh = sh_tables<T>();
+    T pll = (T)kInvSqrt4PiD;
     y[0] = pll;
 #pragma unroll
     for (int l = 1; l <= LMAX; ++l) {
         const int idx = l * l + l;
 #pragma unroll
         for (int m = 0; m <= l - 2; ++m)
-            y[idx + m] = kSh.A[l][m] * (ct * y[(l - 1) * (l - 1) + (l - 1) + m] + kSh.B[l][m] * y[(l - 2) * (l - 2) + (l - 2) + m]);
-        y[idx + l - 1] = kSh.D1[l] * ct * pll;
-        pll *= kSh.D2[l] * st;
+            y[idx + m] = sh.A[l][m] * (ct * y[(l - 1) * (l - 1) + (l - 1) + m] + sh.B[l][m] * y[(l - 2) * (l - 2) + (l - 2) + m]);
+        y[idx + l - 1] = sh.D1[l] * ct * pll;
+        pll *= sh.D2[l] * st;
         y[idx + l] = pll;
     }
 }
 
 // Real orthonormal harmonics at index l*l+l+m: m>0 -> sqrt2 P cos(m phi), m<0 -> sqrt2 P sin(|m| phi).
-template <int LMAX>
-__device__ __forceinline__ void real_sph(float ct, float st, float cp, float sp, float *y) {
+template <int LMAX, typename T>
+__device__ __forceinline__ void real_sph(T ct, T st, T cp, T sp, T *y) {
     legendre<LMAX>(ct, st, y);
-    float cm = 1.f, sm = 0.f;
+    T cm = (T)1, sm = (T)0;
 #pragma unroll
     for (int m = 1; m <= LMAX; ++m) {
-        const float c2 = cm * cp - sm * sp;
-        const float s2 = sm * cp + cm * sp;
+        const T c2 = cm * cp - sm * sp;
+        const T s2 = sm * cp + cm * sp;
         cm = c2;
         sm = s2;
 #pragma unroll
         for (int l = m; l <= LMAX; ++l) {
             const int idx = l * l + l;
-            const float p = kSqrt2 * y[idx + m];
+            const T p = (T)kSqrt2D * y[idx + m];
             y[idx + m] = p * cm;
             y[idx - m] = p * sm;
         }
     }
 }
 
-// sum_{lm} g[lm] * dY_lm/dtheta, with dP/dtheta from the differentiated recurrence.
-template <int LMAX>
-__device__ __forceinline__ float dtheta_contract(float ct, float st, float cp, float sp, const float *g) {
-    // rows of P and dP for l-1 and l-2 (m = 0..l)
-    float p1[LMAX + 1] = {}, d1[LMAX + 1] = {}, p2[LMAX + 1] = {}, d2[LMAX + 1] = {};
-    float cmv[LMAX + 1], smv[LMAX + 1];
-    cmv[0] = 1.f;
-    smv[0] = 0.f;
+// The same harmonics evaluated in T and stored as O (FP64 arithmetic, FP32
+// storage: no FP64 copy of the whole set is held).  The recurrence keeps only
+// the rows l-1 and l-2 of P.
+template <int LMAX, typename T, typename O>
+__device__ __forceinline__ void real_sph_rows(T ct, T st, T cp, T sp, O *y) {
+    const ShTables<T> &sh = sh_tables<T>();
+    T cmv[LMAX + 1], smv[LMAX + 1];
+    cmv[0] = (T)1;
+    smv[0] = (T)0;
 #pragma unroll
     for (int m = 1; m <= LMAX; ++m) {
         cmv[m] = cmv[m - 1] * cp - smv[m - 1] * sp;
         smv[m] = smv[m - 1] * cp + cmv[m - 1] * sp;
     }
-    float acc = 0.f;
-    float pll = kInvSqrt4Pi, dpll = 0.f;
+    T p1[LMAX + 1] = {}, p2[LMAX + 1] = {};
+    T pll = (T)kInvSqrt4PiD;
     p1[0] = pll;
-    d1[0] = 0.f;
+    y[0] = (O)pll;
 #pragma unroll
     for (int l = 1; l <= LMAX; ++l) {
-        float p[LMAX + 1] = {}, d[LMAX + 1] = {};
+        T p[LMAX + 1] = {};
+#pragma unroll
+        for (int m = 0; m <= l - 2; ++m) p[m] = sh.A[l][m] * (ct * p1[m] + sh.B[l][m] * p2[m]);
+        p[l - 1] = sh.D1[l] * ct * pll;
+        pll *= sh.D2[l] * st;
+        p[l] = pll;
+        const int idx = l * l + l;
+        y[idx] = (O)p[0];
+#pragma unroll
+        for (int m = 1; m <= l; ++m) {
+            const T q = (T)kSqrt2D * p[m];
+            y[idx + m] = (O)(q * cmv[m]);
+            y[idx - m] = (O)(q * smv[m]);
+        }
+#pragma unroll
+        for (int m = 0; m <= l; ++m) {
+            p2[m] = p1[m];
+            p1[m] = p[m];
+        }
+    }
+}
+
+// sum_{lm} g[lm] * dY_lm/dtheta, with dP/dtheta from the differentiated
+// recurrence, evaluated in T (g may be stored in another type).
+template <int LMAX, typename T, typename G>
+__device__ __forceinline__ T dtheta_contract(T ct, T st, T cp, T sp, const G *g) {
+    const ShTables<T> &sh = sh_tables<T>();
+    // rows of P and dP for l-1 and l-2 (m = 0..l)
+    T p1[LMAX + 1] = {}, d1[LMAX + 1] = {}, p2[LMAX + 1] = {}, d2[LMAX + 1] = {};
+    T cmv[LMAX + 1], smv[LMAX + 1];
+    cmv[0] = (T)1;
+    smv[0] = (T)0;
+#pragma unroll
+    for (int m = 1; m <= LMAX; ++m) {
+        cmv[m] = cmv[m - 1] * cp - smv[m - 1] * sp;
+        smv[m] = smv[m - 1] * cp + cmv[m - 1] * sp;
+    }
+    T acc = (T)0;
+    T pll = (T)kInvSqrt4PiD, dpll = (T)0;
+    p1[0] = pll;
+    d1[0] = (T)0;
+#pragma unroll
+    for (int l = 1; l <= LMAX; ++l) {
+        T p[LMAX + 1] = {}, d[LMAX + 1] = {};
 #pragma unroll
         for (int m = 0; m <= l - 2; ++m) {
-            p[m] = kSh.A[l][m] * (ct * p1[m] + kSh.B[l][m] * p2[m]);
-            d[m] = kSh.A[l][m] * (ct * d1[m] - st * p1[m] + kSh.B[l][m] * d2[m]);
+            p[m] = sh.A[l][m] * (ct * p1[m] + sh.B[l][m] * p2[m]);
+            d[m] = sh.A[l][m] * (ct * d1[m] - st * p1[m] + sh.B[l][m] * d2[m]);
         }
-        p[l - 1] = kSh.D1[l] * ct * pll;
-        d[l - 1] = kSh.D1[l] * (ct * dpll - st * pll);
-        const float npll = kSh.D2[l] * st * pll;
-        dpll = kSh.D2[l] * (st * dpll + ct * pll);
+        p[l - 1] = sh.D1[l] * ct * pll;
+        d[l - 1] = sh.D1[l] * (ct * dpll - st * pll);
+        const T npll = sh.D2[l] * st * pll;
+        dpll = sh.D2[l] * (st * dpll + ct * pll);
         pll = npll;
         p[l] = pll;
         d[l] = dpll;
         const int idx = l * l + l;
-        float al = g[idx] * d[0];  // per-l partial sum: shorter FP32 accumulation chains
+        T al = (T)g[idx] * d[0];  // per-l partial sum: shorter accumulation chains
 #pragma unroll
-        for (int m = 1; m <= l; ++m) al += kSqrt2 * d[m] * (g[idx + m] * cmv[m] + g[idx - m] * smv[m]);
+        for (int m = 1; m <= l; ++m) al += (T)kSqrt2D * d[m] * ((T)g[idx + m] * cmv[m] + (T)g[idx - m] * smv[m]);
         acc += al;
 #pragma unroll
         for (int m = 0; m <= l; ++m) {
@@ -169,25 +225,60 @@ __device__ __forceinline__ PairGeomR pair_geom_fast(float dx, float dy, float dz
     return g;
 }
 
+// 1/sqrt(x) in FP64: the hardware estimate and one Newton step (the estimate
+// alone is not to FP64 accuracy).
+__device__ __forceinline__ double rsq64(double x) {
+    double y = __builtin_amdgcn_rsq(x);
+    const double h = 0.5 * x * y;
+    const double e = __builtin_fma(-h, y, 0.5);
+    return __builtin_fma(y, e, y);
+}
+// The pair geometry of pair_geom_fast in FP64, from the displacement held as
+// the sum of two floats (hi + lo, exact in FP64).  A pair with dxy below
+// 8 DBL_MIN is on the pole; no displacement reaches the FP64 underflow range.
+struct PairGeomD {
+    double dx, dy, dz, r, ir, ct, st, cp, sp, idxy;
+    bool pole;
+};
+__device__ __forceinline__ PairGeomD pair_geom64(float4 hi, float4 lo) {
+    PairGeomD g;
+    g.dx = (double)hi.x + (double)lo.x;
+    g.dy = (double)hi.y + (double)lo.y;
+    g.dz = (double)hi.z + (double)lo.z;
+    const double q = g.dx * g.dx + g.dy * g.dy;
+    const double r2 = q + g.dz * g.dz;
+    g.ir = rsq64(r2);
+    g.r = r2 * g.ir;
+    const double iq = q > 0.0 ? rsq64(q) : 0.0;
+    const double dxy = q * iq;
+    g.pole = !(dxy >= 8.0 * 2.2250738585072014e-308);
+    g.st = dxy * g.ir;
+    g.ct = g.dz * g.ir;
+    g.cp = g.pole ? 1.0 : g.dx * iq;
+    g.sp = g.pole ? 0.0 : g.dy * iq;
+    g.idxy = g.pole ? 0.0 : iq;
+    return g;
+}
+
 // Chebyshev radial basis Rn = T_n(1 - 2r/rc), n = 0..K-1
-template <int K>
-__device__ __forceinline__ void cheb_rn(float x, float *rn) {
-    rn[0] = 1.f;
+template <int K, typename T>
+__device__ __forceinline__ void cheb_rn(T x, T *rn) {
+    rn[0] = (T)1;
     if (K > 1) rn[1] = x;
 #pragma unroll
-    for (int q = 2; q < K; ++q) rn[q] = 2.f * x * rn[q - 1] - rn[q - 2];
+    for (int q = 2; q < K; ++q) rn[q] = (T)2 * x * rn[q - 1] - rn[q - 2];
 }
 // dRn/dr = -(2/rc) n U_{n-1}(x)
-template <int K>
-__device__ __forceinline__ void cheb_drn(float x, float inv_rc, float *drn) {
-    float u[K > 1 ? K - 1 : 1];
-    if (K > 1) u[0] = 1.f;
-    if (K > 2) u[1] = 2.f * x;
+template <int K, typename T>
+__device__ __forceinline__ void cheb_drn(T x, T inv_rc, T *drn) {
+    T u[K > 1 ? K - 1 : 1];
+    if (K > 1) u[0] = (T)1;
+    if (K > 2) u[1] = (T)2 * x;
 #pragma unroll
-    for (int q = 2; q < K - 1; ++q) u[q] = 2.f * x * u[q - 1] - u[q - 2];
-    drn[0] = 0.f;
+    for (int q = 2; q < K - 1; ++q) u[q] = (T)2 * x * u[q - 1] - u[q - 2];
+    drn[0] = (T)0;
 #pragma unroll
-    for (int q = 1; q < K; ++q) drn[q] = -2.f * inv_rc * (float)q * u[q - 1];
+    for (int q = 1; q < K; ++q) drn[q] = (T)(-2) * inv_rc * (T)q * u[q - 1];
 }
 
 }  // namespace jnn

@@ -86,23 +86,24 @@ __global__ __launch_bounds__(64 * kGatherMaxWaves) void k_gather(int n, const in
     for (int k0 = 0; k0 < ci; k0 += 16 * U) {
         bool on[U], live[U];
         int want[U], jr[U], lo[U], hi[U], res[U];
-        float dx[U], dy[U], dz[U];
+        double dx[U], dy[U], dz[U];  // d, plus dlo where the list keeps FP64 displacements
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = k0 + lid + 16 * u;
             on[u] = k < ci;
             jr[u] = 0;
             want[u] = 0;
-            dx[u] = dy[u] = dz[u] = 0.f;
+            dx[u] = dy[u] = dz[u] = 0.0;
             if (on[u]) {
                 const float4 go = nl.g[base + k];
                 f[0] -= (double)go.x;
                 f[1] -= (double)go.y;
                 f[2] -= (double)go.z;
                 const float4 dk = nl.d[base + k];
-                dx[u] = dk.x;
-                dy[u] = dk.y;
-                dz[u] = dk.z;
+                const float4 dl = (want_virial && nl.dlo) ? nl.dlo[base + k] : make_float4(0.f, 0.f, 0.f, 0.f);
+                dx[u] = (double)dk.x + (double)dl.x;
+                dy[u] = (double)dk.y + (double)dl.y;
+                dz[u] = (double)dk.z + (double)dl.z;
                 jr[u] = tag_row(dk.w);
                 want[u] = ia * 32 + (26 - (nl.key[base + k] & 31));
             }
@@ -154,7 +155,7 @@ __global__ __launch_bounds__(64 * kGatherMaxWaves) void k_gather(int n, const in
             f[2] += gz;
             if (want_virial) {
                 // displacement of the pair j -> i: r_i - r_j = -(r_j - r_i)
-                const double ex = -(double)dx[u], ey = -(double)dy[u], ez = -(double)dz[u];
+                const double ex = -dx[u], ey = -dy[u], ez = -dz[u];
                 v[0] += ex * gx; v[1] += ey * gy; v[2] += ez * gz;
                 v[3] += ex * gy; v[4] += ex * gz; v[5] += ey * gz;
                 v[6] += ey * gx; v[7] += ez * gx; v[8] += ez * gy;

@@ -91,7 +91,7 @@ struct GpumdContext {
             row_n_alloc = std::max(row_n_alloc, (size_t)nrow + nrow / 4 + 64);
             row_alloc = row_n_alloc * cap_;
             rows = dalloc<int>(row_n_alloc);
-            xs = dalloc<float>(row_n_alloc * model->smax);
+            xs = dalloc<float>(row_n_alloc * model->smax * 2);  // room for FP64 rows (the plain mode)
             G = dalloc<float>(row_n_alloc * model->smax);
             A = dalloc<float>(row_n_alloc * model->pmmax);
             E = dalloc<double>(row_n_alloc);
@@ -120,7 +120,8 @@ int jse_nnap_gpumd_bind(void *args) {
         const jse_nnap_model m = a.get<jse_nnap_model>();
         const int prec = a.get<int>();
         JNN_CHECK(m, ERR_ARG, "null model");
-        JNN_CHECK(prec == JSE_NNAP_FP32 || prec == JSE_NNAP_BF16_MLP, ERR_ARG, "unknown precision");
+        check_precision(prec);
+        check_model_shapes(m, prec);
         std::lock_guard<std::mutex> lk(g_gpumd_mu);
         g_gpumd.model = m;
         g_gpumd.precision = prec;

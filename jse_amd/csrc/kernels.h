@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <string>
 
 namespace jnn {
 
@@ -28,9 +29,7 @@ struct LaunchTune {
     int mlp_wg_per_cu = 0;       // MLP: persistent workgroups per CU (0: 2)
     int nl_wg_per_cu = 0;        // neighbour build: persistent workgroups per CU (0: one per atom)
     int gather_waves = 0;        // force gather: atoms (waves) per workgroup (0: 4)
-    int desc_fwd_variant = 0;    // descriptor forward, plain FP32 mode: 0 = matrix cores with FP64 totals,
-                                 // 1 / 2 = vector kernel with 6x7 / 9x7 FP64 register tiles; the BF16-MLP
-                                 // mode's matrix-core forward has no variants
+    int desc_fwd_variant = 0;    // descriptor forward: reserved (one compiled schedule per precision mode)
     int desc_bwd_variant = 0;    // descriptor backward: scheduling barrier every 1 / 2 / no rows
 };
 constexpr int kTuneKnobs = 7;
@@ -62,7 +61,14 @@ struct NeighborList {
                                  // the builders that write d
     int *rev = nullptr;     // [N*cap] slot of the reverse pair in j's row, -1 if absent (statistics only)
     float4 *g = nullptr;    // [N*cap] (gx, gy, gz, 0): g_j = -dE_i/d(r_j - r_i)
+    float4 *dlo = nullptr;  // [N*cap] (dx, dy, dz, 0) of the FP64 displacement minus d's FP32 rounding, so
+                            // d + dlo holds the FP64 displacement to 48 bits (nullptr: FP32 only, the
+                            // accurate kernels then take lo = 0)
 };
+// The FP64 displacement of list slot k (d + dlo, or d alone without dlo).
+__device__ __forceinline__ float4 pair_lo(const NeighborList &nl, size_t k) {
+    return nl.dlo ? nl.dlo[k] : make_float4(0.f, 0.f, 0.f, 0.f);
+}
 
 // Pair tag, carried as the int bits of d.w: bits 0-3 the model species of j
 // (1-based), bits 4-30 the row of the list that holds j's own neighbours
@@ -139,19 +145,28 @@ struct SegmentArgs {
     const int *nl_of;       // neighbour-list row of an atom (nullptr: the atom index)
     const int *types;       // [N] 1-based types by atom
     float rc;
+    double rc64;            // the same cut-off in FP64 (the plain mode's kernels)
     int ntypes;             // species of the model (neighbour types are 1..ntypes)
     const float *W;         // [ntypes][P][K] FP32 mixing (used by the dense kernel)
     const float *mu, *sig;  // normalisation [S]
     const double *mu64, *sig64;  // the same in FP64 (descriptor normalisation)
     float *x;               // [nrows][S] normalised descriptors (forward out, MLP in)
+    double *x64;            // the same in FP64 (the plain mode writes these instead)
     float *A;               // [nrows][P*M]
     float *G;               // [nrows][S]  2*(4pi/(2l+1)) * dE/dfp
     int wg_per_cu;          // persistent workgroups per CU (0: one workgroup per row)
     int variant;            // compiled schedule variant (autotuned; 0: default)
-    int fp32_acc;           // 1: the matrix-core forward accumulates in FP32 (BF16-MLP precision mode)
+    int fp32_acc;           // 1: the matrix-core forward accumulates in FP32 (FP32 geometry modes)
+    int acc;                // 1: pair geometry, harmonics, radial basis and A in FP64 (the plain mode)
+    const double *W64;      // [ntypes][P][K] FP64 mixing (the plain mode's forward)
     int self_t;             // this segment's species (1-based)
     int mirror;             // mirror basis: neighbours of species `mirror` and of self_t swap roles (0: none)
 };
+// Stores a normalised descriptor value: FP64 rows in the plain mode, else FP32.
+__device__ __forceinline__ void store_x(const SegmentArgs &a, size_t idx, double v) {
+    if (a.x64) a.x64[idx] = v;
+    else a.x[idx] = (float)v;
+}
 // A neighbour's species as the centre's basis sees it (MirrorBasis: the mirrored
 // species and the centre's own swap roles; nnap_main.hpp:40-47)
 __device__ __forceinline__ int role_species(int tj, int self_t, int mirror) {
@@ -181,6 +196,11 @@ bool launch_desc_bwd_set(hipStream_t s, int lmax, int K, int P, int mix, const S
 // false when the shape is outside these limits or its A does not fit in LDS.
 bool launch_desc_gen_set(hipStream_t s, int lmax, int K, int P, const SegmentSet &set, const NeighborList &nl,
                          bool backward);
+// true when (lmax, K, P, mix) has a tiled kernel (lmax = kRadialLmax: radial)
+bool desc_tiled(int lmax, int K, int P, int mix);
+// Why the untiled kernels cannot run a shape (its LDS needs exceed 160 KB;
+// `acc`: the plain mode's FP64 staging), or "" when they can.
+std::string desc_gen_limit(int lmax, int K, int P, int ntypes, bool acc);
 
 struct MlpArgs {
     int nrows, S, L1, nlayers;
@@ -191,21 +211,29 @@ struct MlpArgs {
     const float *w_out_in;   // per layer [out][in] (backward layout)
     const float *bias;       // hidden biases, then output bias
     const float *sig;        // [S]
-    float sig_e, mu_e;
+    double sig_e, mu_e;
+    // the FP64 MLP of the plain mode reads these instead of x, the weights and sigma
+    const double *x64;       // [nrows][S]
+    const double *w_in_out64, *w_out_in64, *bias64, *sig64;
     double *E;               // [N] per-atom energy (by atom index)
     float *G;                // [nrows][S]
     int want_grad;
     int wg_per_cu;           // persistent grid size (0: 2 per CU)
     int radial;              // radial-only basis: G = dE/dfp / sigma (no L2 power-spectrum factor)
 };
-// Any layer count and widths; false when a 64-row tile of the input and the
+// Any layer count and widths; false when a 32-row tile of the input and the
 // hidden activations (and their SiLU derivatives) exceeds 160 KB of LDS:
-// S + 2 * sum(hidden) > 640.
+// S + 2 * sum(hidden) > 1280 (FP32) or 640 (FP64).
 bool launch_mlp_fp32(hipStream_t s, const MlpArgs &a);
-constexpr const char *kMlpFp32Limit = "the FP32 MLP needs input_dim + 2 * sum(hidden_dims) <= 640";
+constexpr const char *kMlpFp32Limit = "the FP32 MLP needs input_dim + 2 * sum(hidden_dims) <= 1280";
 // MFMA kernels for 2 hidden layers of supported widths; false when the shape has no compiled kernel
 bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a);
 bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a);
+bool launch_mlp_mfma_fp64(hipStream_t s, const MlpArgs &a);
+// FP64 vector MLP of the plain mode: any depth with S + 2 * sum(hidden) <= 640
+bool launch_mlp_fp64(hipStream_t s, const MlpArgs &a);
+constexpr const char *kMlpFp64Limit = "the FP64 MLP needs input_dim + 2 * sum(hidden_dims) <= 640";
+enum { kMlpFp32 = 0, kMlpBf16 = 1, kMlpFp64 = 2 };  // element types of the MLP kernels
 // All species in one launch: the persistent workgroups are split among the
 // species in proportion to their row tiles (each stages its species' weights
 // once).  False when the species' MLP shapes differ, there are more than
@@ -216,7 +244,7 @@ struct MlpMulti {
     int wg0[kMlpMaxSpecies + 1];  // first workgroup of each species, then the total
     int nsp;
 };
-bool launch_mlp_mfma_multi(hipStream_t s, const MlpArgs *a, int nsp, bool bf16);
+bool launch_mlp_mfma_multi(hipStream_t s, const MlpArgs *a, int nsp, int mode);
 
 // ---- forces / virials / totals ----
 // Row t of the list belongs to atom out_of[t] (nullptr: t); atom j's row is

@@ -86,7 +86,7 @@ struct LmpContext {
             inum_alloc = std::max(inum_alloc, (size_t)inum_ + inum_ / 4 + 64);
             row_alloc = inum_alloc * cap_;
             rows = dalloc<int>(inum_alloc);
-            xs = dalloc<float>(inum_alloc * model->smax);
+            xs = dalloc<float>(inum_alloc * model->smax * 2);  // room for FP64 rows (the plain mode)
             G = dalloc<float>(inum_alloc * model->smax);
             A = dalloc<float>(inum_alloc * model->pmmax);
             E = dalloc<double>(inum_alloc);
@@ -112,27 +112,14 @@ void enqueue_hook_rows(hipStream_t s, jse_nnap_model model, int precision, const
     std::vector<SegmentArgs> segs(nt);
     std::vector<MlpArgs> mlps;
     for (int t = 0; t < nt; ++t) {
-        const SpeciesHost &sp = mh.species[t];
-        const SpeciesDev &dv = model->dev[t];
         SegmentArgs &sa = segs[t];
-        sa = SegmentArgs{};
-        sa.self_t = t + 1;
-        sa.mirror = sp.mirror;
+        sa = species_segment(model, t, precision, types);
         sa.nrows = seg[t + 1] - seg[t];
         sa.rows_atom = rows + seg[t];
-        sa.types = types;
-        sa.rc = (float)sp.rcut;
-        sa.ntypes = nt;
-        sa.W = dv.W;
-        sa.mu = dv.mu;
-        sa.sig = dv.sig;
-        sa.mu64 = dv.mu64;
-        sa.sig64 = dv.sig64;
-        sa.fp32_acc = precision == JSE_NNAP_BF16_MLP;
-        sa.x = xs + (size_t)seg[t] * model->smax;
+        set_segment_x(sa, xs, (size_t)seg[t], model->smax);
         sa.A = A + (size_t)seg[t] * model->pmmax;
         sa.G = G + (size_t)seg[t] * model->smax;
-        mlps.push_back(mlp_args(model, t, sa.nrows, sa.rows_atom, sa.x, sa.G, E, true));
+        mlps.push_back(mlp_args(model, t, sa, E, true));
     }
     // one launch per kernel for all species of one descriptor shape, persistent
     // workgroups as the autotuner picks them for a 32,000-atom liquid (the hooks
@@ -155,7 +142,8 @@ int jse_nnap_lammps_bind(void *args) {
         const jse_nnap_model m = a.get<jse_nnap_model>();
         const int prec = a.get<int>();
         JNN_CHECK(m, ERR_ARG, "null model");
-        JNN_CHECK(prec == JSE_NNAP_FP32 || prec == JSE_NNAP_BF16_MLP, ERR_ARG, "unknown precision");
+        check_precision(prec);
+        check_model_shapes(m, prec);
         std::lock_guard<std::mutex> lk(g_lmp_mu);
         g_lmp.model = m;
         g_lmp.precision = prec;

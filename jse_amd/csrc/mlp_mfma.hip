@@ -17,16 +17,20 @@
 // weights live in LDS with row strides chosen against bank conflicts.  The
 // grid is persistent (weights staged once per workgroup).
 //
-// Two element types share the code:
-//   float  : v_mfma_f32_16x16x4_f32   (plain FP32 path)
+// Three element types share the code:
+//   double : v_mfma_f64_16x16x4_f64   (the plain mode: FP64 inputs, weights,
+//            activations and SiLU; its error stays far below the force bar)
+//   float  : v_mfma_f32_16x16x4_f32   (the FP32-geometry mode)
 //   __bf16 : v_mfma_f32_16x16x32_bf16 (BF16 operands, FP32 accumulation); the
 //            two forward GEMMs carry each operand as a hi + lo BF16 pair
 //            (three MFMAs: hi*hi + lo*hi + hi*lo), which keeps the energy at
 //            FP32-like accuracy, while the two backward GEMMs use single BF16.
 // Lane maps (16x16 output tile, 64 lanes): A[i][k] from lane i + 16*(k/E),
-// B[k][j] from lane j + 16*(k/E), D[i][j] in lane j + 16*(i/4), register i%4,
-// with E = 1 (f32, 4 k per instruction) or 8 consecutive k per lane (bf16, 32 k).
-// SiLU, biases, the energy and the output scaling stay FP32 in both.
+// B[k][j] from lane j + 16*(k/E), with E = 1 (f64 and f32, 4 k per
+// instruction) or 8 consecutive k per lane (bf16, 32 k); D[i][j] in lane
+// j + 16*(i/4), element i%4 (f32, bf16) or lane j + 16*(i%4), element i/4
+// (f64).  SiLU, biases, the energy and the output scaling are FP64 in the
+// FP64 kernel and FP32 in the other two.
 #include <type_traits>
 
 #include "common.h"
@@ -35,6 +39,7 @@
 namespace jnn {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef double f64x4 __attribute__((ext_vector_type(4)));
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 
 constexpr float kPi4m = 12.566370614359172f;
@@ -49,6 +54,31 @@ template <typename T>
 struct Mma;
 
 template <>
+struct Mma<double> {
+    static constexpr int KSTEP = 4;
+    static constexpr int KPL = 1;
+    static constexpr int PAD = 4;  // row stride == 8 (mod 64) dwords
+    static constexpr bool kTransposedCopies = false;
+    static constexpr bool kSplit = false;
+    using AV = double;
+    using CT = double;  // epilogue arithmetic
+    using ACC = f64x4;
+    __device__ static __forceinline__ double cvt(double v) { return v; }
+    __device__ static __forceinline__ double lo(double v) { return 0.0; }
+    __device__ static __forceinline__ int lane_k(int lane) { return lane >> 4; }
+    // tile row of accumulator element r of this lane
+    __device__ static __forceinline__ int row(int lane, int r) { return 4 * r + (lane >> 4); }
+    __device__ static __forceinline__ f64x4 mma(const double *a, const double *b, f64x4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a[0], b[0], c, 0, 0, 0);
+    }
+    __device__ static __forceinline__ AV a_hi(const double *x) { return x[0]; }
+    __device__ static __forceinline__ AV a_lo(const double *x) { return 0.0; }
+    __device__ static __forceinline__ f64x4 mma_r(AV a, const double *b, f64x4 c) {
+        return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b[0], c, 0, 0, 0);
+    }
+};
+
+template <>
 struct Mma<float> {
     static constexpr int KSTEP = 4;
     static constexpr int KPL = 1;  // k values per lane and instruction
@@ -56,6 +86,9 @@ struct Mma<float> {
     static constexpr bool kTransposedCopies = false;
     static constexpr bool kSplit = false;
     using AV = float;  // an A operand held in registers
+    using CT = float;
+    using ACC = f32x4;
+    __device__ static __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
     __device__ static __forceinline__ float cvt(float v) { return v; }
     __device__ static __forceinline__ float lo(float v) { return 0.f; }
     __device__ static __forceinline__ int lane_k(int lane) { return lane >> 4; }
@@ -76,6 +109,9 @@ struct Mma<__bf16> {
     static constexpr int KPL = 8;
     static constexpr int PAD = 8;  // row stride == 4 (mod 32) dwords: conflict-free 16-byte reads
     using AV = bf16x8;
+    using CT = float;
+    using ACC = f32x4;
+    __device__ static __forceinline__ int row(int lane, int r) { return 4 * (lane >> 4) + r; }
     static constexpr bool kTransposedCopies = true;
     // forward operands as hi + lo BF16 pairs: x w ~ xh wh + xl wh + xh wl (FP32 accumulation)
     static constexpr bool kSplit = true;
@@ -108,19 +144,24 @@ __device__ __forceinline__ float silu_grad(float z, float *g) {
     *g = s * (1.f + z * (1.f - s));
     return z * s;
 }
+__device__ __forceinline__ double silu_grad(double z, double *g) {
+    const double s = 1.0 / (1.0 + exp(-z));
+    *g = s * (1.0 + z * (1.0 - s));
+    return z * s;
+}
 
 // Stages NTOT elements with the workgroup's 256 threads: every thread first
 // issues the loads of all its ceil(NTOT/256) elements (all in flight at once),
 // then converts and stores them, so staging costs one memory latency, not one
 // per element.
-template <int NTOT, typename Ld, typename St>
+template <int NTOT, typename V, typename Ld, typename St>
 __device__ __forceinline__ void stage(int tid, Ld ld, St st) {
     constexpr int PT = (NTOT + 255) / 256;
-    float v[PT];
+    V v[PT];
 #pragma unroll
     for (int u = 0; u < PT; ++u) {
         const int e = tid + 256 * u;
-        v[u] = e < NTOT ? ld(e) : 0.f;
+        v[u] = e < NTOT ? ld(e) : (V)0;
     }
 #pragma unroll
     for (int u = 0; u < PT; ++u) {
@@ -134,6 +175,9 @@ __device__ __forceinline__ void stage(int tid, Ld ld, St st) {
 template <typename T, int C1, int C2, int KP>
 __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     using MM = Mma<T>;
+    using CT = typename MM::CT;    // epilogue arithmetic (and the weights' source type)
+    using ACC = typename MM::ACC;
+    constexpr bool F64 = sizeof(CT) == 8;
     constexpr int H1 = 16 * C1, H2 = 16 * C2;
     constexpr int NS = KP / 16;                                          // column tiles of gX
     constexpr int XS = KP + MM::PAD;                                      // row stride of X, W1
@@ -161,61 +205,74 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     const int wave = __builtin_amdgcn_readfirstlane(tid >> 6);
     const int S = a.S;
     const int lk = MM::lane_k(lane);
-    const float *__restrict__ W1g = a.w_out_in;                     // [H1][S]
-    const float *__restrict__ W2g = a.w_out_in + (size_t)S * H1;    // [H2][H1]
-    const float *__restrict__ b1 = a.bias;
-    const float *__restrict__ b2 = a.bias + H1;
-    const float bo = a.bias[H1 + H2];
-    const float *__restrict__ wo = a.w_in_out + (size_t)S * H1 + (size_t)H1 * H2;
+    const CT *w_out_in, *w_in_out, *bias;
+    if constexpr (F64) {
+        w_out_in = a.w_out_in64;
+        w_in_out = a.w_in_out64;
+        bias = a.bias64;
+    } else {
+        w_out_in = a.w_out_in;
+        w_in_out = a.w_in_out;
+        bias = a.bias;
+    }
+    const CT *__restrict__ W1g = w_out_in;                     // [H1][S]
+    const CT *__restrict__ W2g = w_out_in + (size_t)S * H1;    // [H2][H1]
+    const CT *__restrict__ b1 = bias;
+    const CT *__restrict__ b2 = bias + H1;
+    const CT bo = bias[H1 + H2];
+    const CT *__restrict__ wo = w_in_out + (size_t)S * H1 + (size_t)H1 * H2;
 
     // X of a tile straight from memory into the layer-1 A operand: value u of
     // this lane is X[row 16 wave + lr][(u / KPL) * KSTEP + lk + u % KPL]
-    float xr[NX];
+    CT xr[NX];
+    const CT *xsrc;
+    if constexpr (F64) xsrc = a.x64;
+    else xsrc = a.x;
     auto load_x = [&](int t) {
         const int i = t * 64 + 16 * wave + lr;
         const bool ok = i < a.nrows;
-        const float *__restrict__ xp = a.x + (size_t)(ok ? i : 0) * S;
+        const CT *__restrict__ xp = xsrc + (size_t)(ok ? i : 0) * S;
 #pragma unroll
         for (int u = 0; u < NX; ++u) {
             const int k = (u / MM::KPL) * MM::KSTEP + lk + u % MM::KPL;
-            xr[u] = (ok && k < S) ? xp[k] : 0.f;
+            xr[u] = (ok && k < S) ? xp[k] : (CT)0;
         }
     };
     if (lb * 64 < a.nrows) load_x(lb);  // in flight while the weights are staged
-    stage<H1 * XS>(
-        tid, [&](int e) { const int j = e / XS, k = e % XS; return k < S ? W1g[(size_t)j * S + k] : 0.f; },
-        [&](int e, float w) {
+    stage<H1 * XS, CT>(
+        tid, [&](int e) { const int j = e / XS, k = e % XS; return k < S ? W1g[(size_t)j * S + k] : (CT)0; },
+        [&](int e, CT w) {
             W1s[e] = MM::cvt(w);
             if constexpr (SP) W1l[e] = MM::lo(w);
         });
-    stage<H2 * HS>(
-        tid, [&](int e) { const int j = e / HS, k = e % HS; return k < H1 ? W2g[(size_t)j * H1 + k] : 0.f; },
-        [&](int e, float w) {
+    stage<H2 * HS, CT>(
+        tid, [&](int e) { const int j = e / HS, k = e % HS; return k < H1 ? W2g[(size_t)j * H1 + k] : (CT)0; },
+        [&](int e, CT w) {
             W2s[e] = MM::cvt(w);
             if constexpr (SP) W2l[e] = MM::lo(w);
         });
     if constexpr (TC) {
-        stage<KP * HS>(
+        stage<KP * HS, CT>(
             tid,
             [&](int e) {
                 const int k = e / HS, j = e % HS;
-                return (k < S && j < H1) ? W1g[(size_t)j * S + k] : 0.f;
+                return (k < S && j < H1) ? W1g[(size_t)j * S + k] : (CT)0;
             },
-            [&](int e, float w) { W1t[e] = MM::cvt(w); });
-        stage<H1 * HS>(
+            [&](int e, CT w) { W1t[e] = MM::cvt(w); });
+        stage<H1 * HS, CT>(
             tid,
             [&](int e) {
                 const int j1 = e / HS, j2 = e % HS;
-                return j2 < H2 ? W2g[(size_t)j2 * H1 + j1] : 0.f;
+                return j2 < H2 ? W2g[(size_t)j2 * H1 + j1] : (CT)0;
             },
-            [&](int e, float w) { W2t[e] = MM::cvt(w); });
+            [&](int e, CT w) { W2t[e] = MM::cvt(w); });
     }
     __syncthreads();  // the weights are staged; below, each wave touches only its own rows of H
     const int r0 = 16 * wave;
     T *hrow_w = Hs + r0 * HS;                         // this wave's 16 rows of H
     const T *hrow = Hs + (r0 + lr) * HS + lk;         // A-operand pointer into H
     // per-lane constants of the output columns this lane holds
-    float b1r[C1], b2r[C2], wor[C2], gscale[NS];
+    CT b1r[C1], b2r[C2], wor[C2], gscale[NS];
 #pragma unroll
     for (int c = 0; c < C1; ++c) b1r[c] = b1[16 * c + lr];
 #pragma unroll
@@ -227,8 +284,13 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     for (int cs = 0; cs < NS; ++cs) {
         const int n = 16 * cs + lr;
         // G = dE/dx / sigma, times 2 4pi/(2l+1) for the L2 power spectrum's backward
-        const float l2 = a.radial ? 1.f : 2.f * kPi4m / (float)(2 * (n % a.L1) + 1);
-        gscale[cs] = n < S ? l2 / a.sig[n] : 0.f;
+        if constexpr (F64) {
+            const double l2 = a.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * (n % a.L1) + 1);
+            gscale[cs] = n < S ? l2 / a.sig64[n] : 0.0;
+        } else {
+            const float l2 = a.radial ? 1.f : 2.f * kPi4m / (float)(2 * (n % a.L1) + 1);
+            gscale[cs] = n < S ? l2 / a.sig[n] : 0.f;
+        }
     }
     for (int tile = lb; tile * 64 < a.nrows; tile += lg) {
         const int row0 = tile * 64;
@@ -237,13 +299,13 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         int atom_r[4];
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            const int i = r0 + 4 * (lane >> 4) + r;
+            const int i = r0 + MM::row(lane, r);
             atom_r[r] = (lr == 0 && i < nt) ? a.rows_atom[row0 + i] : 0;
         }
         // ---- layer 1 ----
-        f32x4 acc1[C1];
+        ACC acc1[C1];
 #pragma unroll
-        for (int c = 0; c < C1; ++c) acc1[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < C1; ++c) acc1[c] = ACC{};
 #pragma unroll
         for (int ks = 0; ks < KP / MM::KSTEP; ++ks) {
             const int k0 = ks * MM::KSTEP;
@@ -262,16 +324,16 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         }
         // the next tile's X is loaded while this tile's other three products run
         if ((tile + lg) * 64 < a.nrows) load_x(tile + lg);
-        float d1[C1][4];
+        CT d1[C1][4];
 #pragma unroll
         for (int c = 0; c < C1; ++c) {
             const int j = 16 * c + lr;
-            const float bj = b1r[c];
+            const CT bj = b1r[c];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const float h = silu_grad(acc1[c][r] + bj, &d1[c][r]);
-                hrow_w[(4 * (lane >> 4) + r) * HS + j] = MM::cvt(h);
-                if constexpr (SP) Hl[(r0 + 4 * (lane >> 4) + r) * HS + j] = MM::lo(h);
+                const CT h = silu_grad((CT)(acc1[c][r] + bj), &d1[c][r]);
+                hrow_w[MM::row(lane, r) * HS + j] = MM::cvt(h);
+                if constexpr (SP) Hl[(r0 + MM::row(lane, r)) * HS + j] = MM::lo(h);
             }
         }
         if constexpr (H1P > H1) {
@@ -282,9 +344,9 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         }
         wave_lds_sync();
         // ---- layer 2 ----
-        f32x4 acc2[C2];
+        ACC acc2[C2];
 #pragma unroll
-        for (int c = 0; c < C2; ++c) acc2[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < C2; ++c) acc2[c] = ACC{};
         for (int k0 = 0; k0 < H1P; k0 += MM::KSTEP) {
 #pragma unroll
             for (int c = 0; c < C2; ++c) {
@@ -297,22 +359,23 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
                 }
             }
         }
-        float gz2[C2][4];
-        float ysum[4] = {0.f, 0.f, 0.f, 0.f};
+        CT gz2[C2][4];
+        CT ysum[4] = {(CT)0, (CT)0, (CT)0, (CT)0};
+        const CT sig_e = (CT)a.sig_e;
 #pragma unroll
         for (int c = 0; c < C2; ++c) {
-            const float bj = b2r[c], w = wor[c];
+            const CT bj = b2r[c], w = wor[c];
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                float dd;
-                const float h = silu_grad(acc2[c][r] + bj, &dd);
+                CT dd;
+                const CT h = silu_grad((CT)(acc2[c][r] + bj), &dd);
                 ysum[r] += w * h;
-                gz2[c][r] = a.sig_e * w * dd;
+                gz2[c][r] = sig_e * w * dd;
             }
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            float v = ysum[r];
+            CT v = ysum[r];
             v += __shfl_xor(v, 1);
             v += __shfl_xor(v, 2);
             v += __shfl_xor(v, 4);
@@ -322,8 +385,8 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         if (lr == 0) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const int i = r0 + 4 * (lane >> 4) + r;
-                if (i < nt) a.E[atom_r[r]] = (double)(ysum[r] + bo) * (double)a.sig_e + (double)a.mu_e;
+                const int i = r0 + MM::row(lane, r);
+                if (i < nt) a.E[atom_r[r]] = (double)(ysum[r] + bo) * a.sig_e + a.mu_e;
             }
         }
         if (!a.want_grad) continue;
@@ -333,16 +396,16 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         for (int c = 0; c < C2; ++c) {
             const int j = 16 * c + lr;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) hrow_w[(4 * (lane >> 4) + r) * HS + j] = MM::cvt(gz2[c][r]);
+            for (int r = 0; r < 4; ++r) hrow_w[MM::row(lane, r) * HS + j] = MM::cvt(gz2[c][r]);
         }
         if constexpr (H2P > H2) {
             for (int e = lane; e < 16 * (H2P - H2); e += 64)
                 hrow_w[(e / (H2P - H2)) * HS + H2 + e % (H2P - H2)] = MM::cvt(0.f);
         }
         wave_lds_sync();
-        f32x4 acc3[C1];
+        ACC acc3[C1];
 #pragma unroll
-        for (int c = 0; c < C1; ++c) acc3[c] = f32x4{0.f, 0.f, 0.f, 0.f};
+        for (int c = 0; c < C1; ++c) acc3[c] = ACC{};
         for (int k0 = 0; k0 < H2P; k0 += MM::KSTEP) {
 #pragma unroll
             for (int c = 0; c < C1; ++c) {
@@ -355,7 +418,7 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         for (int c = 0; c < C1; ++c) {
             const int j = 16 * c + lr;
 #pragma unroll
-            for (int r = 0; r < 4; ++r) hrow_w[(4 * (lane >> 4) + r) * HS + j] = MM::cvt(acc3[c][r] * d1[c][r]);
+            for (int r = 0; r < 4; ++r) hrow_w[MM::row(lane, r) * HS + j] = MM::cvt((CT)acc3[c][r] * d1[c][r]);
         }
         if constexpr (H1P > H1) {
             for (int e = lane; e < 16 * (H1P - H1); e += 64)
@@ -365,18 +428,18 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         // ---- gX = gZ1 W1 (k = j over H1) ----
 #pragma unroll
         for (int cs = 0; cs < NS; ++cs) {
-            f32x4 acc = f32x4{0.f, 0.f, 0.f, 0.f};
+            ACC acc = ACC{};
             const int n = 16 * cs + lr;
             for (int k0 = 0; k0 < H1P; k0 += MM::KSTEP) {
                 if constexpr (TC) acc = MM::mma(hrow + k0, W1t + n * HS + lk + k0, acc);
                 else acc = MM::mma(hrow + k0, W1s + (k0 + lk) * XS + n, acc);
             }
             if (n < S) {
-                const float scale = gscale[cs];
+                const CT scale = gscale[cs];
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int i = r0 + 4 * (lane >> 4) + r;
-                    if (i < nt) a.G[(size_t)(row0 + i) * S + n] = acc[r] * scale;
+                    const int i = r0 + MM::row(lane, r);
+                    if (i < nt) a.G[(size_t)(row0 + i) * S + n] = (float)((CT)acc[r] * scale);
                 }
             }
         }
@@ -420,9 +483,9 @@ static bool dispatch_mlp2(hipStream_t s, const MlpArgs &a) {
     JNN_MLP_SHAPE(32, 32, 128)
     JNN_MLP_SHAPE(64, 64, 64)
     JNN_MLP_SHAPE(64, 64, 128)
-    if constexpr (std::is_same<T, float>::value) {  // wider inputs (FP32 only: the BF16 tiles' LDS would not fit)
-        JNN_MLP_SHAPE(32, 32, 256)
-        JNN_MLP_SHAPE(32, 32, 384)
+    if constexpr (std::is_same<T, float>::value) {  // wider inputs, FP32 only (the BF16 transposed copies
+        JNN_MLP_SHAPE(32, 32, 256)                   // and the FP64 weights would not fit in LDS, and the
+        JNN_MLP_SHAPE(32, 32, 384)                   // FP64 X operands not in registers)
         JNN_MLP_SHAPE(64, 64, 256)
         JNN_MLP_SHAPE(64, 64, 384)
     }
@@ -448,7 +511,7 @@ static bool dispatch_mlp2_multi(hipStream_t s, const MlpMulti &m) {
     JNN_MLP_SHAPE(32, 32, 128)
     JNN_MLP_SHAPE(64, 64, 64)
     JNN_MLP_SHAPE(64, 64, 128)
-    if constexpr (std::is_same<T, float>::value) {  // wider inputs (FP32 only: the BF16 tiles' LDS would not fit)
+    if constexpr (std::is_same<T, float>::value) {
         JNN_MLP_SHAPE(32, 32, 256)
         JNN_MLP_SHAPE(32, 32, 384)
         JNN_MLP_SHAPE(64, 64, 256)
@@ -458,7 +521,7 @@ static bool dispatch_mlp2_multi(hipStream_t s, const MlpMulti &m) {
     return false;
 }
 
-bool launch_mlp_mfma_multi(hipStream_t s, const MlpArgs *a, int nsp, bool bf16) {
+bool launch_mlp_mfma_multi(hipStream_t s, const MlpArgs *a, int nsp, int mode) {
     if (nsp <= 0 || nsp > kMlpMaxSpecies) return false;
     MlpMulti m{};
     int tiles_tot = 0, first = -1;
@@ -486,7 +549,9 @@ bool launch_mlp_mfma_multi(hipStream_t s, const MlpArgs *a, int nsp, bool bf16) 
         wg += w;
     }
     m.wg0[m.nsp] = wg;
-    return bf16 ? dispatch_mlp2_multi<__bf16>(s, m) : dispatch_mlp2_multi<float>(s, m);
+    if (mode == kMlpBf16) return dispatch_mlp2_multi<__bf16>(s, m);
+    if (mode == kMlpFp64) return dispatch_mlp2_multi<double>(s, m);
+    return dispatch_mlp2_multi<float>(s, m);
 }
 
 bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a) {
@@ -496,6 +561,10 @@ bool launch_mlp_mfma_fp32(hipStream_t s, const MlpArgs &a) {
 bool launch_mlp_mfma_bf16(hipStream_t s, const MlpArgs &a) {
     if (a.nrows <= 0) return true;
     return dispatch_mlp2<__bf16>(s, a);
+}
+bool launch_mlp_mfma_fp64(hipStream_t s, const MlpArgs &a) {
+    if (a.nrows <= 0) return true;
+    return dispatch_mlp2<double>(s, a);
 }
 
 }  // namespace jnn

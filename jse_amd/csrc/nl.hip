@@ -19,6 +19,7 @@
 // among the atom's hits (counted against the LDS hit buffer).
 #include <algorithm>
 
+#include "common.h"
 #include "kernels.h"
 
 namespace jnn {
@@ -391,7 +392,11 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                         cand(q, key, slot, dx, dy, dz);
                         const int tj = staged ? s_ctype[q] : nl.types[kt >> 5];
                         // j's own row is its slot
-                        nl.d[row + rank] = make_float4((float)dx, (float)dy, (float)dz, pair_tag(slot, tj));
+                        const float fx = (float)dx, fy = (float)dy, fz = (float)dz;
+                        nl.d[row + rank] = make_float4(fx, fy, fz, pair_tag(slot, tj));
+                        if (nl.dlo)
+                            nl.dlo[row + rank] = make_float4((float)(dx - (double)fx), (float)(dy - (double)fy),
+                                                             (float)(dz - (double)fz), 0.f);
                     }
                 }
             }
@@ -463,6 +468,8 @@ void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, 
                      const int *need, double rc2, const double *xw, const int *cell, const int *cell_start,
                      const int *sidx, const double *sxw, NeighborList nl, int *overflow, int wg_per_cu) {
     if (n <= 0) return;
+    // the per-atom build keeps FP32 displacements only (its callers build key-only candidate lists)
+    JNN_CHECK(!nl.dlo, ERR_ARG, "the per-atom neighbour build does not write FP64 displacements");
     int grid = n < 65536 * 4 ? n : 65536 * 4;
     if (wg_per_cu > 0) grid = std::min(grid, wg_per_cu * device_cu_count());
     hipLaunchKernelGGL(k_nl_build, dim3(grid), dim3(kNlBlock), 0, s, n, atoms, rep_of, reps, need, rc2, xw, cell,

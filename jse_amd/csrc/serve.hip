@@ -107,6 +107,7 @@ __global__ __launch_bounds__(256) void k_srv_refresh(int n, const int *__restric
         bool hit = false;
         int key = 0;
         float fdx = 0.f, fdy = 0.f, fdz = 0.f;
+        float4 lo = make_float4(0.f, 0.f, 0.f, 0.f);
         if (k < cv) {
             key = vrow[k];
             const int j = key >> 5, code = key & 31;
@@ -120,6 +121,7 @@ __global__ __launch_bounds__(256) void k_srv_refresh(int n, const int *__restric
             fdx = (float)dx;
             fdy = (float)dy;
             fdz = (float)dz;
+            lo = make_float4((float)(dx - (double)fdx), (float)(dy - (double)fdy), (float)(dz - (double)fdz), 0.f);
         }
         const unsigned long long mask = __ballot(hit);
         const unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
@@ -127,6 +129,7 @@ __global__ __launch_bounds__(256) void k_srv_refresh(int n, const int *__restric
         if (hit && pos < nl.cap) {
             nl.key[row + pos] = key;
             nl.d[row + pos] = make_float4(fdx, fdy, fdz, pair_tag(key >> 5, nl.types[key >> 5]));  // row j: atom j
+            if (nl.dlo) nl.dlo[row + pos] = lo;
         }
         hits += __popcll(mask);
     }

@@ -167,12 +167,12 @@ struct jse_nnap_server_s {
     // ------------------------------------------------------------------ memory
     size_t slot_bytes() const {
         return sizeof(double) * (15 + 13) + sizeof(int) * 5 + sizeof(int) * (1 + cap_v) +
-               (sizeof(int) * 2 + sizeof(float4) * 2) * cap_w + sizeof(int);
+               (sizeof(int) * 2 + sizeof(float4) * (precision_acc(precision) ? 3 : 2)) * cap_w + sizeof(int);
     }
     void free_arena() {
         dfree(x); dfree(xb); dfree(xwb); dfree(xw); dfree(sxw); dfree(E); dfree(F); dfree(V);
         dfree(types); dfree(rep_of); dfree(cell); dfree(rank); dfree(sidx); dfree(cell_cnt); dfree(cell_start);
-        dfree(vl.cnt); dfree(vl.key); dfree(nl.cnt); dfree(nl.key); dfree(nl.d); dfree(nl.rev); dfree(nl.g);
+        dfree(vl.cnt); dfree(vl.key); dfree(nl.cnt); dfree(nl.key); dfree(nl.d); dfree(nl.rev); dfree(nl.g); dfree(nl.dlo);
     }
     // Sizes the arena from what the budget leaves after lanes and tables.
     void alloc_arena() {
@@ -199,6 +199,7 @@ struct jse_nnap_server_s {
         nl.key = dalloc<int>((size_t)n * cap_w);
         nl.d = dalloc<float4>((size_t)n * cap_w);
         nl.g = dalloc<float4>((size_t)n * cap_w);
+        if (precision_acc(precision)) nl.dlo = dalloc<float4>((size_t)n * cap_w);
         arena_bytes = (size_t)((double)n * slot_bytes() + (double)cells * 2 * sizeof(int));
         atom_alloc.reset(n);
         cell_alloc.reset(cells);
@@ -226,7 +227,7 @@ struct jse_nnap_server_s {
         L.d_tot = dalloc<double>((size_t)max_batch_reps * 10);
         L.d_arrive = dalloc<int>((size_t)max_batch_reps);
         JNN_HIP(hipMemset(L.d_arrive, 0, sizeof(int) * (size_t)max_batch_reps));
-        L.d_x = dalloc<float>(nb * model->smax);
+        L.d_x = dalloc<float>(nb * model->smax * 2);  // room for FP64 rows (the plain mode)
         L.d_G = dalloc<float>(nb * model->smax);
         L.d_A = dalloc<float>(nb * model->pmmax);
         L.d_flags = dalloc<int>(2 + (size_t)max_batch_reps);
@@ -250,7 +251,8 @@ struct jse_nnap_server_s {
     void init(jse_nnap_model m, int prec, double skin_, double budget_, int max_batch_, int max_reps,
               double max_density) {
         JNN_CHECK(m, ERR_ARG, "null model");
-        JNN_CHECK(prec == JSE_NNAP_FP32 || prec == JSE_NNAP_BF16_MLP, ERR_ARG, "unknown precision");
+        check_precision(prec);
+        check_model_shapes(m, prec);
         JNN_CHECK(std::isfinite(skin_) && skin_ >= 0, ERR_ARG, "skin must be finite and >= 0");
         JNN_CHECK(max_batch_ >= 1 && max_batch_ < (1 << 26), ERR_ARG, "max_batch_atoms out of range");
         JNN_CHECK(max_reps >= 1 && max_reps <= (1 << 24), ERR_ARG, "max_replicas out of range");
@@ -273,7 +275,7 @@ struct jse_nnap_server_s {
         plan_cap = plan_bytes(max_batch, max_batch_reps, max_chunks, max_batch_reps, max_batch);
         lane_bytes = 2 * (plan_cap + sizeof(double) * 6 * (size_t)max_batch + sizeof(double) * 10 * max_chunks +
                           sizeof(double) * 10 * max_batch_reps +
-                          sizeof(float) * (size_t)max_batch * (2 * model->smax + model->pmmax));
+                          sizeof(float) * (size_t)max_batch * (3 * model->smax + model->pmmax));  // x (FP64 room), G, A
         if (budget_ <= 0) {
             size_t fr = 0, tot = 0;
             JNN_HIP(hipMemGetInfo(&fr, &tot));
@@ -571,7 +573,7 @@ struct jse_nnap_server_s {
             for (int t = 0; t < nt; ++t) {
                 segs.push_back(seg_args(L, t, seg, d_rows));
                 const SegmentArgs &a = segs.back();
-                ma.push_back(mlp_args(model, t, a.nrows, a.rows_atom, a.x, a.G, E, true));
+                ma.push_back(mlp_args(model, t, a, E, true));
             }
             enqueue_desc(s, model, segs.data(), false, nl);
             enqueue_mlp_all(s, model, precision, ma.data(), nt);
@@ -588,25 +590,12 @@ struct jse_nnap_server_s {
     }
 
     SegmentArgs seg_args(const Lane &L, int t, const std::vector<int> &seg, const int *d_rows) const {
-        const SpeciesHost &sp = model->host.species[t];
-        const SpeciesDev &dv = model->dev[t];
-        SegmentArgs a{};
-        a.self_t = t + 1;
-        a.mirror = model->host.species[t].mirror;
+        SegmentArgs a = species_segment(model, t, precision, types);
         a.nrows = seg[t + 1] - seg[t];
         a.rows_atom = d_rows + seg[t];
-        a.types = types;
-        a.rc = (float)sp.rcut;
-        a.ntypes = model->host.ntypes();
-        a.W = dv.W;
-        a.mu = dv.mu;
-        a.sig = dv.sig;
-        a.mu64 = dv.mu64;
-        a.sig64 = dv.sig64;
-        a.x = L.d_x + (size_t)seg[t] * model->smax;
+        set_segment_x(a, L.d_x, (size_t)seg[t], model->smax);
         a.A = L.d_A + (size_t)seg[t] * model->pmmax;
         a.G = L.d_G + (size_t)seg[t] * model->smax;
-        a.fp32_acc = precision == JSE_NNAP_BF16_MLP;
         return a;
     }
 

@@ -65,7 +65,7 @@ def test_batch_matches_oracle(O):
         ref = orc.system_efv(r)
         sl = slice(off, off + r.natoms)
         assert abs(e_rep[k] - ref["energy"]) / r.natoms <= 1e-6
-        assert np.sqrt(((full["forces"][sl] - ref["forces"]) ** 2).mean()) <= 1e-5
+        assert np.abs(full["forces"][sl] - ref["forces"]).max() <= 1e-5
         assert np.abs(v_rep[k] - ref["virial"]).max() / r.natoms <= 1e-5
         off += r.natoms
     pot.close()
@@ -100,21 +100,18 @@ def O(oracle_mod):
     return oracle_mod
 
 
-def test_autotuned_launch_shapes_keep_results():
+@pytest.mark.parametrize("precision", [nnap.BF16_MLP, nnap.FP32])
+def test_autotuned_launch_shapes_keep_results(precision):
     """Launch shapes and the backward's schedule change which workgroup takes
-    a row and how instructions are ordered, never a row's arithmetic: in the
-    BF16-MLP mode (whose matrix-core forward ignores the forward-tile knob)
-    the tuned knobs reproduce the default shapes' results bit for bit.  In
-    FP32 mode the forward variants sum the same FP32 products in different
-    orders (16-neighbour FP32 partial sums added into FP64 on the matrix
-    cores; FP64 register tiles on the vector ALU): results agree to within
-    the FP32 bar."""
+    a row and how instructions are ordered, never a row's arithmetic: the
+    tuned knobs reproduce the default shapes' results bit for bit, in the
+    BF16-MLP mode and in the plain mode."""
     from jse_amd import autotune
     md = W.ternary_model()
     reps = W.replicas(24, 64, 1500, 29)
     atoms = [nnap.AtomData(r.box, r.xyz, r.types) for r in reps]
     what = nnap.ENERGY | nnap.FORCE | nnap.VIRIAL
-    with nnap.NNAP(md, precision=nnap.BF16_MLP) as pot:
+    with nnap.NNAP(md, precision=precision) as pot:
         b = pot.batch(atoms)
         b.compute(what)
         r0 = b.results()
@@ -122,7 +119,7 @@ def test_autotuned_launch_shapes_keep_results():
         print("tuned", knobs, ms)
         # 0 = the kernel's built-in default shape kept
         assert knobs["mlp_wg_per_cu"] in (0, 1, 2, 4, 8) and knobs["gather_waves"] in (0, 1, 2, 4, 8)
-        assert knobs["desc_fwd_variant"] in (0, 1, 2) and knobs["desc_bwd_variant"] in (0, 1, 2)
+        assert knobs["desc_fwd_variant"] == 0 and knobs["desc_bwd_variant"] in (0, 1, 2)
         b.set_tuning(knobs)
         b.compute(what)
         r1 = b.results()
@@ -131,22 +128,6 @@ def test_autotuned_launch_shapes_keep_results():
         # a cached tuning is applied without re-timing
         knobs2, ms2, hit2 = autotune.tune(b, what)
         assert hit2 and knobs2 == knobs
-        b.close()
-    # the forward tiles in FP32
-    with nnap.NNAP(md, precision=nnap.FP32) as pot:
-        b = pot.batch(atoms)
-        b.compute(what)
-        r0 = b.results()
-        n = len(r0["energies"])
-        for fv in (1, 2):
-            b.set_tuning({"desc_fwd_variant": fv})
-            b.compute(what)
-            r2 = b.results()
-            de = np.abs(r2["energies"] - r0["energies"])
-            df = np.abs(r2["forces"] - r0["forces"])
-            print("forward variant", fv, "max |dE_i|", de.max(), "max |dF|", df.max())
-            assert abs(r2["energy"] - r0["energy"]) / n <= 1e-6
-            assert de.max() <= 2e-5 and df.max() <= 2e-4
         b.close()
 
 
@@ -185,6 +166,6 @@ def test_untiled_shape_in_captured_graphs(O):
         ref = orc.system_efv(r)
         sl = slice(off, off + r.natoms)
         assert abs(e_rep[k] - ref["energy"]) / r.natoms <= 1e-6
-        assert np.sqrt(((full["forces"][sl] - ref["forces"]) ** 2).mean()) <= 1e-5
+        assert np.abs(full["forces"][sl] - ref["forces"]).max() <= 1e-5
         assert np.abs(v_rep[k] - ref["virial"]).max() / r.natoms <= 1e-5
         off += r.natoms

@@ -69,7 +69,7 @@ def test_served_fp32_matches_oracle(O):
                 e, f_rms, f_max, v = _bar(res, ref, r.natoms)
                 print(f"step {s} replica {k} n={r.natoms}: e {e:.2e} f_rms {f_rms:.2e} f_max {f_max:.2e} v {v:.2e}",
                       flush=True)
-                assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-4 and v <= 1e-5
+                assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-5 and v <= 1e-5
         st = srv.stats()
         assert st["completed"] == 20
         # every replica built on admission and again after the jump
@@ -161,7 +161,7 @@ def test_mix_with_evictions_within_bar(O):
         ref = orc.system_efv(W.Structure(r.box, trajs[k][s], r.types))
         e, f_rms, f_max, v = _bar(res, ref, r.natoms)
         print("replica", k, "step", s, "e %.2e f_rms %.2e f_max %.2e v %.2e" % (e, f_rms, f_max, v))
-        assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-4 and v <= 1e-5, (k, s)
+        assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-5 and v <= 1e-5, (k, s)
 
 
 def test_rebuild_rule_counted():
@@ -252,4 +252,4 @@ def test_served_untiled_shape_matches_oracle(O):
                 ref = orc.system_efv(W.Structure(r.box, trajs[k][s], r.types))
                 e, f_rms, f_max, v = _bar(res, ref, r.natoms)
                 print(f"step {s} replica {k} n={r.natoms}: e {e:.2e} f_rms {f_rms:.2e} v {v:.2e}", flush=True)
-                assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-4 and v <= 1e-5
+                assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-5 and v <= 1e-5
