@@ -31,22 +31,27 @@ from jse_amd import model as M  # noqa: E402
 from jse_amd import nnap  # noqa: E402
 from jse_amd import workloads as W  # noqa: E402
 
-FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 peak (vector and matrix)
-FP64_PEAK_TFLOPS = 78.6    # MI355X published FP64 vector peak
+FP32_PEAK_TFLOPS = 157.3   # MI355X published FP32 peak (matrix cores; vector with packed-FP32 issue)
+FP64_PEAK_TFLOPS = 78.6    # MI355X published FP64 peak (matrix cores and vector)
 BF16_PEAK_TFLOPS = 2500.0  # MI355X published dense BF16 matrix peak
 
 
 def roofline_pipe(kernel: str, precision: str):
-    """(pipe, published peak in TFLOP/s) of the pipe a kernel's arithmetic runs on."""
+    """(pipe, published peak in TFLOP/s, bound) of the pipe a kernel's counted
+    arithmetic runs on in a precision mode ("fp32" = the plain mode, "fast",
+    "bf16")."""
     if kernel == "desc_fwd":
         if precision == "fp32":
-            return "fp64 vector FMA (FP32 products accumulated in FP64)", FP64_PEAK_TFLOPS
-        return "fp32 matrix cores (v_mfma_f32_16x16x4_f32)", FP32_PEAK_TFLOPS
+            return "fp64 matrix cores (v_mfma_f64_16x16x4_f64)", FP64_PEAK_TFLOPS, "mfma"
+        return "fp32 matrix cores (v_mfma_f32_16x16x4_f32)", FP32_PEAK_TFLOPS, "mfma"
     if kernel == "mlp":
         if precision == "fp32":
-            return "fp32 matrix cores (v_mfma_f32_16x16x4_f32)", FP32_PEAK_TFLOPS
-        return "bf16 matrix cores (v_mfma_f32_16x16x32_bf16)", BF16_PEAK_TFLOPS
-    return "fp32 vector ALU (FMA without packed-FP32 issue)", FP32_PEAK_TFLOPS
+            return "fp64 matrix cores (v_mfma_f64_16x16x4_f64)", FP64_PEAK_TFLOPS, "mfma"
+        if precision == "fast":
+            return "fp32 matrix cores (v_mfma_f32_16x16x4_f32)", FP32_PEAK_TFLOPS, "mfma"
+        return "bf16 matrix cores (v_mfma_f32_16x16x32_bf16)", BF16_PEAK_TFLOPS, "mfma"
+    # the backward's counted work is FP32 FMA on the vector ALU (its FP64 geometry is not counted)
+    return "fp32 vector ALU (FMA; the published FP32 peak assumes packed issue)", FP32_PEAK_TFLOPS, "valu"
 HBM_PEAK_GBS = 8000.0
 
 
@@ -79,7 +84,7 @@ def cpu_baseline(md, st, gpu_res, budget_s=12.0):
     import oracle as O
     mdl = M.load_model(md)
     orc = O.Oracle(mdl, fast=True)
-    cores = min(16, os.cpu_count() or 1)
+    cores = host_cores()
     t0 = time.perf_counter()
     reps = 0
     ref = None
@@ -97,8 +102,55 @@ def cpu_baseline(md, st, gpu_res, budget_s=12.0):
         "virial_per_atom_eV": float(np.abs(gpu_res["virial"] - ref["virial"]).max()) / n,
     }
     base = {"value": n * reps / dt, "unit": "atom-evals/s", "cores": cores, "kind": "port",
-            "sample": f"{reps} full E+F+virial evaluations of the {n}-atom workload ({dt:.1f} s)"}
+            "host_cpus_visible": os.cpu_count(),
+            "sample": f"{reps} full E+F+virial evaluations of the {n}-atom workload ({dt:.1f} s), "
+                      f"one OpenMP thread per core of this process's CPU share"}
     return base, err
+
+
+def host_cores():
+    """Threads for the CPU baseline: the cores this process may use (its CPU
+    affinity), at most the 16-core share of one GPU on a shared host."""
+    try:
+        n = len(os.sched_getaffinity(0))
+    except AttributeError:
+        n = os.cpu_count() or 1
+    return max(1, min(16, n))
+
+
+def cu_c1(steps=50, budget_s=10.0):
+    """The Cu FCC 256-atom crystal (a = 3.61 A, Gaussian displacements 0.1 A,
+    seed 1) with the one-species nmax 5 / lmax 6 / 32-32 model: the plain
+    path's time per E+F+virial evaluation on the GPU and the FP64 CPU
+    restatement's on the host cores, with the GPU result's error."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    md = W.cu_model()
+    st = W.cu_fcc(seed=1)
+    what = nnap.ENERGY | nnap.FORCE | nnap.VIRIAL
+    with nnap.NNAP(md) as pot:
+        s_ = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+        s_.compute(what)
+        r = s_.results()
+        ms, _ = s_.benchmark(what, warmup=5, iters=steps)
+    orc = O.Oracle(M.load_model(md), fast=True)
+    cores = host_cores()
+    t0 = time.perf_counter()
+    reps = 0
+    while True:
+        ref = O.system_efv_par(orc, st, cores)
+        reps += 1
+        if time.perf_counter() - t0 > budget_s or reps >= 20000:
+            break
+    dt = time.perf_counter() - t0
+    n = st.natoms
+    return {"natoms": n, "gpu_ms_per_step": ms, "gpu_atom_evals_per_s": n / (ms * 1e-3),
+            "cpu_baseline": {"value": n * reps / dt, "unit": "atom-evals/s", "cores": cores, "kind": "port",
+                             "sample": f"{reps} evaluations ({dt:.1f} s)"},
+            "error_vs_fp64": {"energy_per_atom_eV": abs(r["energy"] - ref["energy"]) / n,
+                              "force_max_eV_per_A": float(np.abs(r["forces"] - ref["forces"]).max()),
+                              "virial_per_atom_eV": float(np.abs(r["virial"] - ref["virial"]).max()) / n},
+            "note": "a 256-atom system fills a small fraction of the GPU; the time is launch-bound"}
 
 
 def batched_c4(nrep=512, seed=17, steps=10):
@@ -121,6 +173,13 @@ def batched_c4(nrep=512, seed=17, steps=10):
         out["batched_bf16_graph"] = {"ms_per_step": ms, "atom_evals_per_s": natoms / (ms * 1e-3),
                                      "default_launch_ms_per_step": ms0, "tuned_launch_shapes": knobs}
         b.close()
+    for name, prec in (("batched_plain_graph", nnap.FP32), ("batched_fp32_fast_graph", nnap.FP32_FAST)):
+        with nnap.NNAP(md, precision=prec) as pot:
+            b = pot.batch(atoms)
+            autotune.tune(b, what, use_cache=False)
+            ms, _ = b.benchmark(what, warmup=3, iters=steps)
+            out[name] = {"ms_per_step": ms, "atom_evals_per_s": natoms / (ms * 1e-3)}
+            b.close()
     with nnap.NNAP(md, precision=nnap.FP32) as pot:
         systems = []
         for a in atoms:
@@ -141,6 +200,8 @@ def batched_c4(nrep=512, seed=17, steps=10):
             s_.close()
     out["speedup"] = out["batched_bf16_graph"]["atom_evals_per_s"] / out["plain_fp32_one_replica_per_call"][
         "atom_evals_per_s"]
+    out["bf16_vs_plain_batched"] = out["batched_bf16_graph"]["atom_evals_per_s"] / out["batched_plain_graph"][
+        "atom_evals_per_s"]
     return out
 
 
@@ -153,22 +214,36 @@ def precision_c3(n=256000, seed=7, steps=10):
     import oracle as O
     md = W.ternary_model()
     st = W.ternary_liquid(n=n, seed=seed)
-    ref = O.system_efv_par(O.Oracle(M.load_model(md), fast=True), st, min(16, os.cpu_count() or 1))
+    ref = O.system_efv_par(O.Oracle(M.load_model(md), fast=True), st, host_cores())
     out = {"natoms": n, "box_A": float(st.box[0])}
-    for name, prec in (("bf16_mlp", nnap.BF16_MLP), ("fp32", nnap.FP32)):
+    names = ["neighbor_build", "reverse_index", "desc_fwd", "mlp", "desc_bwd", "force_gather"]
+    mdl = M.load_model(md)
+    for name, prec in (("bf16_mlp", nnap.BF16_MLP), ("fp32", nnap.FP32), ("fp32_fast", nnap.FP32_FAST)):
         with nnap.NNAP(md, precision=prec) as pot:
             s_ = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
             s_.compute(nnap.ENERGY | nnap.FORCE | nnap.VIRIAL)
             r = s_.results()
-            ms, _ = s_.benchmark(nnap.ENERGY | nnap.FORCE | nnap.VIRIAL, warmup=2, iters=steps)
+            ms, ph = s_.benchmark(nnap.ENERGY | nnap.FORCE | nnap.VIRIAL, warmup=2, iters=steps)
+            stats = s_.nl_stats()
+            cnt, _, _ = s_.neighbors(cap=stats["max_count"])
+            pairs = [int(cnt[st.types == t + 1].sum()) for t in range(pot.ntypes())]
         df = r["forces"] - ref["forces"]
-        out[name] = {"ms_per_step": ms, "atom_evals_per_s": n / (ms * 1e-3),
+        phase_ms = {k: float(v) for k, v in zip(names, ph)}
+        mlp_flops = algorithmic_flops(mdl, st.types, pairs)["mlp"]
+        pipe, peak, _ = roofline_pipe("mlp", {"bf16_mlp": "bf16", "fp32": "fp32", "fp32_fast": "fast"}[name])
+        out[name] = {"ms_per_step": ms, "atom_evals_per_s": n / (ms * 1e-3), "phase_ms": phase_ms,
+                     "mlp_matrix_core_utilisation": {
+                         "achieved_TFLOPs": mlp_flops / (phase_ms["mlp"] * 1e-3) / 1e12, "peak_TFLOPs": peak,
+                         "frac": mlp_flops / (phase_ms["mlp"] * 1e-3) / 1e12 / peak, "pipe": pipe},
                      "energy_per_atom_eV": abs(r["energy"] - ref["energy"]) / n,
                      "energy_rmse_per_atom_eV": float(np.sqrt(((r["energies"] - ref["energies"]) ** 2).mean())),
                      "force_rms_eV_per_A": float(np.sqrt((df ** 2).mean())),
                      "force_max_eV_per_A": float(np.abs(df).max()),
                      "virial_per_atom_eV": float(np.abs(r["virial"] - ref["virial"]).max()) / n}
     out["bf16_speedup_vs_fp32"] = out["bf16_mlp"]["atom_evals_per_s"] / out["fp32"]["atom_evals_per_s"]
+    out["note"] = ("fp32 = the plain path (FP64 geometry, descriptor and MLP; meets the reference bar); "
+                   "fp32_fast = FP32 geometry and descriptor with an FP32 MLP; bf16_mlp = FP32 geometry and "
+                   "descriptor with a BF16 MLP; the BF16 MLP's forward products carry hi + lo BF16 pairs")
     return out
 
 
@@ -300,6 +375,67 @@ def committed_traffic(kernel):
         return None
 
 
+def nl_bytes(natoms, npairs, fp64_rows):
+    """Algorithmic HBM bytes of the neighbour build: per atom the position read
+    and the wrapped position written (48), the cell id and rank (8), the
+    cell-sorted copy read and written (56), the count (4); per pair the key
+    and the FP32 displacement with its tag (20), plus the FP64 low parts (16)
+    in the plain mode."""
+    return natoms * 116 + npairs * (36 if fp64_rows else 20)
+
+
+def gather_bytes(natoms, npairs, fp64_rows):
+    """Algorithmic HBM bytes of the force/virial gather: per pair the own g,
+    key and displacement (36, plus 16 for the FP64 low parts in the plain
+    mode) and the reverse pair's g (16); per atom F and V written (96)."""
+    return npairs * (52 + (16 if fp64_rows else 0)) + natoms * 96
+
+
+def kernel_rooflines(flops, phase_ms, pmode, natoms, npairs, fp64_rows):
+    """Achieved work rate of every phase against the peak of the pipe that
+    bounds it: GB/s against HBM for the neighbour build and the gather,
+    TFLOP/s against the matrix-core or vector peak for the others."""
+    out = {}
+    for k, b in (("neighbor_build", nl_bytes(natoms, npairs, fp64_rows)),
+                 ("force_gather", gather_bytes(natoms, npairs, fp64_rows))):
+        gbs = b / (phase_ms[k] * 1e-3) / 1e9
+        out[k] = {"bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s", "frac": gbs / HBM_PEAK_GBS,
+                  "algorithmic_bytes": b}
+    for k in ("desc_fwd", "mlp", "desc_bwd"):
+        pipe, peak, bound = roofline_pipe(k, pmode)
+        tf = flops[k] / (phase_ms[k] * 1e-3) / 1e12
+        out[k] = {"bound": bound, "pipe": pipe, "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak}
+    return out
+
+
+def other_modes_c2(md, st):
+    """The FP32_FAST and BF16-MLP modes on the same 32,000-atom system
+    (autotuned launch shapes), with their errors against the FP64 CPU
+    restatement."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle as O
+    from jse_amd import autotune
+    ref = O.system_efv_par(O.Oracle(M.load_model(md), fast=True), st, host_cores())
+    what = nnap.ENERGY | nnap.FORCE | nnap.VIRIAL
+    names = ["neighbor_build", "reverse_index", "desc_fwd", "mlp", "desc_bwd", "force_gather"]
+    out = {}
+    for name, prec in (("fp32_fast", nnap.FP32_FAST), ("bf16_mlp", nnap.BF16_MLP)):
+        with nnap.NNAP(md, precision=prec) as pot:
+            s_ = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+            s_.compute(what)
+            r = s_.results()
+            knobs, _, _ = autotune.tune(s_, what, use_cache=False)
+            ms, ph = s_.benchmark(what, warmup=3, iters=20)
+        n = st.natoms
+        out[name] = {"ms_per_step": ms, "atom_evals_per_s": n / (ms * 1e-3),
+                     "phase_ms": {k: float(v) for k, v in zip(names, ph)},
+                     "energy_per_atom_eV": abs(r["energy"] - ref["energy"]) / n,
+                     "force_rms_eV_per_A": float(np.sqrt(((r["forces"] - ref["forces"]) ** 2).mean())),
+                     "force_max_eV_per_A": float(np.abs(r["forces"] - ref["forces"]).max()),
+                     "virial_per_atom_eV": float(np.abs(r["virial"] - ref["virial"]).max()) / n}
+    return out
+
+
 def _pct(a, q):
     return float(np.percentile(np.asarray(a), q)) if len(a) else None
 
@@ -418,12 +554,14 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--precision", choices=["fp32", "bf16"], default="fp32")
+    ap.add_argument("--precision", choices=["fp32", "fast", "bf16"], default="fp32")
     ap.add_argument("--natoms", type=int, default=32000)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--no-served", action="store_true")
     ap.add_argument("--no-256k", action="store_true")
+    ap.add_argument("--no-modes", action="store_true", help="skip the FP32_FAST / BF16 lines at 32,000 atoms")
+    ap.add_argument("--no-c1", action="store_true", help="skip the Cu-256 line")
     ap.add_argument("--served-replicas", type=int, default=2000)
     ap.add_argument("--knobs", type=str, default="",
                     help="JSON launch shapes for the 32,000-atom system instead of autotuning it "
@@ -448,7 +586,7 @@ def main():
 
     md = W.ternary_model()
     st = W.ternary_liquid(n=args.natoms, seed=7)
-    prec = nnap.FP32 if args.precision == "fp32" else nnap.BF16_MLP
+    prec = {"fp32": nnap.FP32, "fast": nnap.FP32_FAST, "bf16": nnap.BF16_MLP}[args.precision]
     pot = nnap.NNAP(md, precision=prec)
     sysm = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
     what = nnap.ENERGY | nnap.FORCE | nnap.VIRIAL
@@ -470,16 +608,17 @@ def main():
     # pairs per species of the centre atom (the kernels' work units)
     cnt, _, _ = sysm.neighbors(cap=stats["max_count"])
     pairs = [int(cnt[st.types == t + 1].sum()) for t in range(pot.ntypes())]
+    npairs = int(sum(pairs))
     flops = algorithmic_flops(pot.model, st.types, pairs)
     names = ["neighbor_build", "reverse_index", "desc_fwd", "mlp", "desc_bwd", "force_gather"]
     phase_ms = {k: float(v) for k, v in zip(names, phase)}
+    pmode = {nnap.FP32: "fp32", nnap.BF16_MLP: "bf16", nnap.FP32_FAST: "fast"}[prec]
+    kernels = kernel_rooflines(flops, phase_ms, pmode, st.natoms, npairs, prec == nnap.FP32)
     dom = max(("desc_fwd", "mlp", "desc_bwd"), key=lambda k: phase_ms[k])
-    achieved = flops[dom] / (phase_ms[dom] * 1e-3) / 1e12
-    pipe, peak = roofline_pipe(dom, args.precision)
-    roof = {"bound": "mfma", "pipe": pipe,
-            "kernel": dom, "achieved": achieved, "peak": peak, "unit": "TFLOP/s",
-            "frac": achieved / peak, "traffic": committed_traffic(dom),
-            "flop_per_step": {k: float(v) for k, v in flops.items()}}
+    roof = dict(kernels[dom])
+    roof["kernel"] = dom
+    roof["traffic"] = committed_traffic(dom)
+    roof["flop_per_step"] = {k: float(v) for k, v in flops.items()}
     value = st.natoms / (ms_step * 1e-3)
     line = {
         "metric": "NNAP atom-evals/s (E+F+virial)",
@@ -492,23 +631,29 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32" if prec == nnap.FP32 else "fp32+bf16-mlp",
+        "dtype": {"fp32": "fp32 storage; fp64 geometry, descriptor, MLP and sums",
+                  "fast": "fp32", "bf16": "fp32 + bf16 MLP"}[pmode],
         "data": "synthetic (seeded RSA liquid, seeded random-init model)",
         "config": {"workload": f"ternary Cu-Zr-Al liquid, {st.natoms} atoms, rho 0.058, rcut 6, nmax 8, lmax 6, "
-                               "MLP 64-64 SiLU, plain FP32 path" if prec == nnap.FP32 else
-                               f"ternary Cu-Zr-Al liquid, {st.natoms} atoms, BF16 MLP",
+                               "MLP 64-64 SiLU, " + {"fp32": "plain path (JSE_NNAP_FP32)",
+                                                     "fast": "FP32_FAST mode", "bf16": "BF16-MLP mode"}[pmode],
                    "natoms": st.natoms, "pairs": int(stats["pairs"]), "parallelism": "single GPU",
                    "launch_shapes": knobs, "default_shapes_ms_per_step": ms_default},
         "phase_ms": phase_ms,
         "wall_s_timed": wall,
         "roofline": roof,
+        "kernel_rooflines": kernels,
     }
+    if prec == nnap.FP32 and not args.no_modes:
+        line["other_modes_c2"] = other_modes_c2(md, st)
     line["pcie_inclusive"] = pcie_inclusive(pot, st)
     line["gpumd_hook"] = gpumd_hook(pot, st)
     if not args.no_cpu_baseline:
         base, err = cpu_baseline(md, st, res)
         line["cpu_baseline"] = base
         line["error_vs_fp64"] = err
+    if not args.no_c1:
+        line["cu256_c1"] = cu_c1()
     if not args.no_256k:
         line["precision_256k"] = precision_c3()
     if not args.no_batched:

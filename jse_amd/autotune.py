@@ -19,7 +19,7 @@ from . import nnap
 
 CACHE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "_tuned.json")
 # bumped whenever a knob's meaning or range changes, so older entries are not applied
-SCHEMA = 3
+SCHEMA = 4
 
 
 def _p2(v: float) -> int:

@@ -42,6 +42,7 @@ enum Status : int {
 enum MixKind : int { MIX_NONE = 0, MIX_DEFAULT = 1, MIX_DENSE = 2 };
 
 constexpr int kMaxLayers = 8;
+constexpr int kMaxSpecies = 15;  // a neighbour's species travels in 4 bits of its pair tag
 
 // One species of a model, FP64 blocks in the reference's packing plus derived shapes.
 struct SpeciesHost {

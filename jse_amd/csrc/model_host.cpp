@@ -575,6 +575,9 @@ ModelHost parse_model_json(const std::string &text) {
 
 void finalize_species(ModelHost &m) {
     const int nt = m.ntypes();
+    // a neighbour's species travels in 4 bits of its pair tag
+    JNN_CHECK(nt >= 1 && nt <= kMaxSpecies, ERR_UNSUPPORTED,
+              "at most " + std::to_string(kMaxSpecies) + " species are supported, the model has " + std::to_string(nt));
     for (auto &sp : m.species) {
         const int K = sp.K(), P = sp.P;
         JNN_CHECK((int)sp.W.size() == nt * P * K, ERR_ARG, "fp_param size mismatch");
