@@ -274,6 +274,18 @@ int jse_nnap_cuda2lammps(void *args);
 int jse_nnap_gpumd_bind(void *args);
 int jse_nnap_computeGPUMD(void *args);
 
+/* No reference counterpart: jse_nnap_computeGPUMD for callers that cannot
+ * allocate (tight MD loops, harnesses that forbid hipMalloc).  Same packed
+ * arguments and results; every buffer beyond GPUMD's inputs and outputs is
+ * carved from a static 64 MiB scratch array per GPU (held until the process
+ * ends), and the call makes no hipMalloc, hipFree or host allocation.  A call
+ * whose buffers exceed the array returns status 6 naming the limit (about
+ * 8.7 KB per centre for the benchmark model with rows of 64 neighbours, i.e.
+ * some 7,000 particles).  Calls on one GPU share its array and are serialised
+ * inside one process; calls from several processes on one GPU each have
+ * their own.  Runs on the null stream. */
+int jse_nnap_computeGPUMD_noalloc(void *args);
+
 #ifdef __cplusplus
 }
 #endif

@@ -4,6 +4,7 @@
 // FP32 displacements r_j - r_i and 0-based types.
 #include <hip/hip_runtime.h>
 
+#include "common.h"
 #include "kernels.h"
 
 namespace jnn {
@@ -187,6 +188,51 @@ __global__ __launch_bounds__(256) void k_gpumd_scatter(int N, int N1, int nrow, 
 
 }  // namespace
 
+// Species-grouped row order on the device, stable within a species: rows of
+// centres i = N1 + r grouped by model type t = g_type[i] + 1 in [1, nt];
+// seg[t] = first row of species t (seg[nt] = nrow).  One wavefront walks the
+// centres 64 at a time, ranking each lane among the lanes of its species with
+// a ballot.  A type outside [1, nt] sets *bad and the rows are not used.
+__global__ __launch_bounds__(64) void k_gpumd_group(int nrow, const int *__restrict__ g_type, int nt,
+                                                    int *__restrict__ rows, int *__restrict__ seg,
+                                                    int *__restrict__ bad) {
+    const int lane = threadIdx.x;
+    __shared__ int cnt[kMaxSpecies + 1];
+    if (lane <= kMaxSpecies) cnt[lane] = 0;
+    __syncthreads();
+    for (int r0 = 0; r0 < nrow; r0 += 64) {
+        const int r = r0 + lane;
+        const int t = r < nrow ? g_type[r] + 1 : 0;
+        if (r < nrow && (t < 1 || t > nt)) atomicOr(bad, 1);
+        if (t >= 1 && t <= nt) atomicAdd(&cnt[t], 1);
+    }
+    __syncthreads();
+    if (lane == 0) {
+        int run = 0;
+        for (int t = 1; t <= nt; ++t) {
+            const int c = cnt[t];
+            cnt[t] = run;
+            seg[t - 1] = run;
+            run += c;
+        }
+        seg[nt] = run;
+    }
+    __syncthreads();
+    for (int r0 = 0; r0 < nrow; r0 += 64) {
+        const int r = r0 + lane;
+        const int t = r < nrow ? g_type[r] + 1 : 0;
+        for (int u = 1; u <= nt; ++u) {
+            const unsigned long long m = __ballot(t == u);
+            if (t == u) rows[cnt[u] + __popcll(m & ((1ull << lane) - 1ull))] = r;
+            if (lane == 0) cnt[u] += __popcll(m);
+            __syncthreads();
+        }
+    }
+}
+
+void launch_gpumd_group(hipStream_t s, int nrow, const int *g_type, int nt, int *rows, int *seg, int *bad) {
+    hipLaunchKernelGGL(k_gpumd_group, dim3(1), dim3(64), 0, s, nrow, g_type, nt, rows, seg, bad);
+}
 void launch_gpumd_types(hipStream_t s, int n, const int *g_type, int *out) {
     if (n <= 0) return;
     hipLaunchKernelGGL(k_gpumd_types, dim3((n + 255) / 256), dim3(256), 0, s, n, g_type, out);

@@ -109,8 +109,9 @@ void enqueue_hook_rows(hipStream_t s, jse_nnap_model model, int precision, const
                        const int *types, float *xs, float *A, float *G, double *E, const NeighborList &nl) {
     const ModelHost &mh = model->host;
     const int nt = mh.ntypes();
-    std::vector<SegmentArgs> segs(nt);
-    std::vector<MlpArgs> mlps;
+    // fixed arrays (no host allocation: the no-alloc GPUMD entry runs through here too)
+    SegmentArgs segs[kMaxSpecies];
+    MlpArgs mlps[kMaxSpecies];
     for (int t = 0; t < nt; ++t) {
         SegmentArgs &sa = segs[t];
         sa = species_segment(model, t, precision, types);
@@ -119,16 +120,16 @@ void enqueue_hook_rows(hipStream_t s, jse_nnap_model model, int precision, const
         set_segment_x(sa, xs, (size_t)seg[t], model->smax);
         sa.A = A + (size_t)seg[t] * model->pmmax;
         sa.G = G + (size_t)seg[t] * model->smax;
-        mlps.push_back(mlp_args(model, t, sa, E, true));
+        mlps[t] = mlp_args(model, t, sa, E, true);
     }
     // one launch per kernel for all species of one descriptor shape, persistent
     // workgroups as the autotuner picks them for a 32,000-atom liquid (the hooks
     // are not tuned per call)
-    for (auto &sa : segs) sa.wg_per_cu = 32;
-    enqueue_desc(s, model, segs.data(), false, nl);
-    enqueue_mlp_all(s, model, precision, mlps.data(), nt);
-    for (auto &sa : segs) sa.wg_per_cu = 16;
-    enqueue_desc(s, model, segs.data(), true, nl);
+    for (int t = 0; t < nt; ++t) segs[t].wg_per_cu = 32;
+    enqueue_desc(s, model, segs, false, nl);
+    enqueue_mlp_all(s, model, precision, mlps, nt);
+    for (int t = 0; t < nt; ++t) segs[t].wg_per_cu = 16;
+    enqueue_desc(s, model, segs, true, nl);
 }
 }  // namespace jnn
 

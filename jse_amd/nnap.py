@@ -60,7 +60,7 @@ EXPORTS = [
     "jse_nnap_system_set_graphs", "jse_nnap_system_autotune", "jse_nnap_system_set_tuning", "jse_nnap_server_create", "jse_nnap_server_add_replica", "jse_nnap_server_submit",
     "jse_nnap_server_wait", "jse_nnap_server_collect", "jse_nnap_server_drain", "jse_nnap_server_stats",
     "jse_nnap_server_destroy", "jse_nnap_lammps_bind", "jse_nnap_statNeiNumLammps", "jse_nnap_lammps2cuda",
-    "jse_nnap_computeLammpsCuda", "jse_nnap_cuda2lammps",
+    "jse_nnap_computeLammpsCuda", "jse_nnap_cuda2lammps", "jse_nnap_computeGPUMD_noalloc",
 ]
 
 SERVER_STATS = ["completed", "atom_evals", "batches", "mean_batch_atoms", "rebuilds", "admissions", "evictions",
