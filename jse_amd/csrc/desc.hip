@@ -420,7 +420,7 @@ __host__ __device__ constexpr int bwd_row(int m) { return (m + 3) & ~3; }
 // rounded to FP32 for the dE/dA contraction, which stays FP32; the radial and
 // theta derivatives and the final chain rule are FP64.
 template <int LMAX, int K, int P, int MIX, int BAR, bool ACC>
-__global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList nl) {  // two waves per SIMD
+__global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList nl) {
     using GT = typename std::conditional<ACC, double, float>::type;  // geometry precision
     using SH = DescShape<LMAX, K, P, MIX>;
     constexpr int BSTEP = BAR > 0 ? BAR : 1;
@@ -583,31 +583,33 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
             if (on) {
                 const int tj = role_species(tag_species(dd.w), self_t, mirror);
                 // pair geometry in GT: r, 1/r, cos/sin theta, cos/sin phi, 1/dxy, d
-                GT gr_, gir, gct, gst, gcp, gsp, gidxy, gdx, gdy, gdz;
+                GT gr_, gir, gct, gst, gcp, gsp, gidxy;
                 bool gpole;
                 if constexpr (ACC) {
                     const PairGeomD g = pair_geom64(dd, pair_lo(nl, base + j));
                     gr_ = g.r; gir = g.ir; gct = g.ct; gst = g.st; gcp = g.cp; gsp = g.sp; gidxy = g.idxy;
-                    gdx = g.dx; gdy = g.dy; gdz = g.dz; gpole = g.pole;
+                    gpole = g.pole;
                 } else {
                     const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
                     gr_ = g.r; gir = g.ir; gct = g.ct; gst = g.st; gcp = g.cp; gsp = g.sp; gidxy = g.idxy;
-                    gdx = dd.x; gdy = dd.y; gdz = dd.z; gpole = g.pole;
+                    gpole = g.pole;
                 }
                 float ox = 0.f, oy = 0.f, oz = 0.f;
                 if (gr_ < rc) {
                     float y[M];
                     if constexpr (ACC) real_sph_rows<LMAX, double, float>(gct, gst, gcp, gsp, y);
                     else real_sph<LMAX>(gct, gst, gcp, gsp, y);
-                    const GT xr = (GT)1 - (GT)2 * gr_ * inv_rc;
-                    GT rnt[K];
-                    cheb_rn<K>(xr, rnt);
-                    float rn[K];
+                    // Rn and fc rounded to FP32 for the contraction; x, t and the
+                    // derivatives are formed again after it (fewer registers held across it)
+                    float rn[K], fc;
+                    {
+                        GT rnt[K];
+                        cheb_rn<K>((GT)1 - (GT)2 * gr_ * inv_rc, rnt);
     #pragma unroll
-                    for (int q = 0; q < K; ++q) rn[q] = (float)rnt[q];
-                    const GT t = (GT)1 - (gr_ * inv_rc) * (gr_ * inv_rc);
-                    const GT t2 = t * t;
-                    const float fc = (float)(t2 * t2);
+                        for (int q = 0; q < K; ++q) rn[q] = (float)rnt[q];
+                        const GT t = (GT)1 - (gr_ * inv_rc) * (gr_ * inv_rc);
+                        fc = (float)((t * t) * (t * t));
+                    }
                     int blk = slot * slot_f4 + (MIX == MIX_NONE ? 0 : (tj - 1) * K * MB);
                     // opaque to the compiler, so every read below is this base plus an immediate
                     // offset (folding the -1 into the reads would make the offsets negative)
@@ -643,12 +645,13 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
                         grn[q] = s * fc;
                         if (BAR > 0 && q % BSTEP == BSTEP - 1) __builtin_amdgcn_sched_barrier(0);
                     }
+                    const GT u = gr_ * inv_rc, t = (GT)1 - u * u;
                     GT drn[K];
-                    cheb_drn<K>(xr, inv_rc, drn);
+                    cheb_drn<K>((GT)1 - (GT)2 * u, inv_rc, drn);
                     GT dEdr = (GT)0;
     #pragma unroll
                     for (int q = 0; q < K; ++q) dEdr += (GT)grn[q] * drn[q];
-                    const GT dfc = (GT)(-8) * gr_ * inv_rc * inv_rc * t2 * t;
+                    const GT dfc = (GT)(-8) * u * inv_rc * (t * t) * t;
                     dEdr += (GT)gfc * dfc;
                     float dEdph = 0.f;
     #pragma unroll
@@ -662,12 +665,14 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
                         }
                         dEdph += pl;
                     }
-                    const GT dEdth = dtheta_contract<LMAX>(gct, gst, gcp, gsp, gy);
+                    const GT dEdth = dtheta_ladder<LMAX>(gcp, gsp, y, gy);
                     const GT ir = gir;
-                    // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi))
-                    GT tx = -(dEdr * gdx * ir + dEdth * gct * gcp * ir);
-                    GT ty = -(dEdr * gdy * ir + dEdth * gct * gsp * ir);
-                    const GT tz = -(dEdr * gdz * ir - dEdth * gst * ir);
+                    // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi)), with
+                    // d/r = (st cp, st sp, ct) and grad(theta) = (ct cp, ct sp, -st) / r
+                    const GT hr = dEdr * gst + dEdth * gct * ir;
+                    GT tx = -(hr * gcp);
+                    GT ty = -(hr * gsp);
+                    const GT tz = -(dEdr * gct - dEdth * gst * ir);
                     if (!gpole) {
                         tx -= (GT)dEdph * (-gsp * gidxy);
                         ty -= (GT)dEdph * (gcp * gidxy);

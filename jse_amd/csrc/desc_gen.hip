@@ -234,16 +234,16 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
         for (int j = lane; j < n; j += 64) {
             const float4 dd = nl.d[base + j];
             const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
-            GT gr_, gir, gct, gst, gcp, gsp, gidxy, gdx, gdy, gdz;
+            GT gr_, gir, gct, gst, gcp, gsp, gidxy;
             bool gpole;
             if constexpr (ACC) {
                 const PairGeomD g = pair_geom64(dd, pair_lo(nl, base + j));
                 gr_ = g.r; gir = g.ir; gct = g.ct; gst = g.st; gcp = g.cp; gsp = g.sp; gidxy = g.idxy;
-                gdx = g.dx; gdy = g.dy; gdz = g.dz; gpole = g.pole;
+                gpole = g.pole;
             } else {
                 const PairGeomR g = pair_geom_fast(dd.x, dd.y, dd.z);
                 gr_ = g.r; gir = g.ir; gct = g.ct; gst = g.st; gcp = g.cp; gsp = g.sp; gidxy = g.idxy;
-                gdx = dd.x; gdy = dd.y; gdz = dd.z; gpole = g.pole;
+                gpole = g.pole;
             }
             float ox = 0.f, oy = 0.f, oz = 0.f;
             if (gr_ < rc) {
@@ -311,12 +311,14 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
                     }
                     dEdph += pl;
                 }
-                const GT dEdth = dtheta_contract<LMAX>(gct, gst, gcp, gsp, gy);
+                const GT dEdth = dtheta_ladder<LMAX>(gcp, gsp, y, gy);
                 const GT ir = gir;
-                // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi))
-                GT tx = -(dEdr * gdx * ir + dEdth * gct * gcp * ir);
-                GT ty = -(dEdr * gdy * ir + dEdth * gct * gsp * ir);
-                const GT tz = -(dEdr * gdz * ir - dEdth * gst * ir);
+                // g = -(dE/dr d/r + dE/dtheta grad(theta) + dE/dphi grad(phi)), with
+                // d/r = (st cp, st sp, ct) and grad(theta) = (ct cp, ct sp, -st) / r
+                const GT hr = dEdr * gst + dEdth * gct * ir;
+                GT tx = -(hr * gcp);
+                GT ty = -(hr * gsp);
+                const GT tz = -(dEdr * gct - dEdth * gst * ir);
                 if (!gpole) {
                     tx -= (GT)dEdph * (-gsp * gidxy);
                     ty -= (GT)dEdph * (gcp * gidxy);

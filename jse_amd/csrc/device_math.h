@@ -2,9 +2,11 @@
 // and FP64 where a template parameter says so):
 // radial Chebyshev basis, smooth cutoff, real spherical harmonics and their
 // theta-derivatives.  The functions define the same quantities as the
-// reference's nnap_util.hpp:226-271 and basis_SphericalUtil.hpp:12-182; the
-// theta-derivative here is obtained by differentiating the Legendre recurrence
-// (forward-mode), not from the reference's closed form.
+// reference's nnap_util.hpp:226-271 and basis_SphericalUtil.hpp:12-182.  The
+// theta-derivative rests on the same ladder identity as the reference's
+// calYPtheta_ (basis_SphericalUtil.hpp:112-147) but is only ever needed
+// contracted with dE/dY, so it is evaluated in the phi-free frame, one row of
+// l at a time, and the dY/dtheta array is never formed.
 #pragma once
 #include <hip/hip_runtime.h>
 
@@ -27,11 +29,19 @@ struct ShTables {
     T B[kMaxL + 1][kMaxL + 1];
     T D1[kMaxL + 1];  // P_{l,l-1} = D1[l] cos t P_{l-1,l-1},  D1 = sqrt(2l+1)
     T D2[kMaxL + 1];  // P_{l,l}   = D2[l] sin t P_{l-1,l-1},  D2 = -sqrt((2l+1)/(2l))  (Condon-Shortley)
-    constexpr ShTables() : A(), B(), D1(), D2() {
+    // ladder identity dP_lm/dt = LB[l][m] P_{l,m+1} - LA[l][m] P_{l,m-1}:
+    // LA = sqrt((l+m)(l-m+1))/2, LB = sqrt((l-m)(l+m+1))/2; L0 = sqrt(l(l+1)/2)
+    T LA[kMaxL + 1][kMaxL + 1];
+    T LB[kMaxL + 1][kMaxL + 1];
+    T L0[kMaxL + 1];
+    constexpr ShTables() : A(), B(), D1(), D2(), LA(), LB(), L0() {
         for (int l = 0; l <= kMaxL; ++l) {
             D1[l] = (T)csqrt(2.0 * l + 1.0);
             D2[l] = l > 0 ? (T)(-csqrt((2.0 * l + 1.0) / (2.0 * l))) : (T)0;
+            L0[l] = (T)csqrt(0.5 * l * (l + 1.0));
             for (int m = 0; m <= kMaxL; ++m) {
+                LA[l][m] = m <= l ? (T)(0.5 * csqrt((double)(l + m) * (l - m + 1))) : (T)0;
+                LB[l][m] = m <= l ? (T)(0.5 * csqrt((double)(l - m) * (l + m + 1))) : (T)0;
                 if (l >= 2 && m <= l - 2) {
                     A[l][m] = (T)csqrt((4.0 * l * l - 1.0) / ((double)l * l - (double)m * m));
                     B[l][m] = (T)(-csqrt((((double)(l - 1) * (l - 1)) - (double)m * m) /
@@ -140,13 +150,16 @@ __device__ __forceinline__ void real_sph_rows(T ct, T st, T cp, T sp, O *y) {
     }
 }
 
-// sum_{lm} g[lm] * dY_lm/dtheta, with dP/dtheta from the differentiated
-// recurrence, evaluated in T (g may be stored in another type).
-template <int LMAX, typename T, typename G>
-__device__ __forceinline__ T dtheta_contract(T ct, T st, T cp, T sp, const G *g) {
+// sum_{lm} g[lm] dY_lm/dtheta from the harmonics themselves, in T.  Both y
+// and g are turned to the phi-free frame row by row: p_m = Y_{l,m} cos(m phi)
+// + Y_{l,-m} sin(m phi) = sqrt2 P_lm (m > 0), p_0 = sqrt2 Y_l0, and likewise
+// gm_m = g_{l,m} cos(m phi) + g_{l,-m} sin(m phi); then the ladder identity
+// dP_lm/dtheta = LB P_{l,m+1} - LA P_{l,m-1} (P_{l,-1} = -P_{l,1}, P_{l,l+1} = 0)
+// gives the row's term g_l0 L0 p_1 + sum_{m>0} gm_m (LB p_{m+1} - LA p_{m-1}).
+// No Legendre recurrence is rerun and dY/dtheta is never formed.
+template <int LMAX, typename T, typename Y, typename G>
+__device__ __forceinline__ T dtheta_ladder(T cp, T sp, const Y *y, const G *g) {
     const ShTables<T> &sh = sh_tables<T>();
-    // rows of P and dP for l-1 and l-2 (m = 0..l)
-    T p1[LMAX + 1] = {}, d1[LMAX + 1] = {}, p2[LMAX + 1] = {}, d2[LMAX + 1] = {};
     T cmv[LMAX + 1], smv[LMAX + 1];
     cmv[0] = (T)1;
     smv[0] = (T)0;
@@ -156,36 +169,22 @@ __device__ __forceinline__ T dtheta_contract(T ct, T st, T cp, T sp, const G *g)
         smv[m] = smv[m - 1] * cp + cmv[m - 1] * sp;
     }
     T acc = (T)0;
-    T pll = (T)kInvSqrt4PiD, dpll = (T)0;
-    p1[0] = pll;
-    d1[0] = (T)0;
 #pragma unroll
     for (int l = 1; l <= LMAX; ++l) {
-        T p[LMAX + 1] = {}, d[LMAX + 1] = {};
-#pragma unroll
-        for (int m = 0; m <= l - 2; ++m) {
-            p[m] = sh.A[l][m] * (ct * p1[m] + sh.B[l][m] * p2[m]);
-            d[m] = sh.A[l][m] * (ct * d1[m] - st * p1[m] + sh.B[l][m] * d2[m]);
-        }
-        p[l - 1] = sh.D1[l] * ct * pll;
-        d[l - 1] = sh.D1[l] * (ct * dpll - st * pll);
-        const T npll = sh.D2[l] * st * pll;
-        dpll = sh.D2[l] * (st * dpll + ct * pll);
-        pll = npll;
-        p[l] = pll;
-        d[l] = dpll;
         const int idx = l * l + l;
-        T al = (T)g[idx] * d[0];  // per-l partial sum: shorter accumulation chains
+        T p[LMAX + 1];
+        p[0] = (T)kSqrt2D * (T)y[idx];
 #pragma unroll
-        for (int m = 1; m <= l; ++m) al += (T)kSqrt2D * d[m] * ((T)g[idx + m] * cmv[m] + (T)g[idx - m] * smv[m]);
-        acc += al;
+        for (int m = 1; m <= l; ++m) p[m] = (T)y[idx + m] * cmv[m] + (T)y[idx - m] * smv[m];
+        T al = (T)g[idx] * sh.L0[l] * p[1];  // per-l partial sum: shorter accumulation chains
 #pragma unroll
-        for (int m = 0; m <= l; ++m) {
-            p2[m] = p1[m];
-            d2[m] = d1[m];
-            p1[m] = p[m];
-            d1[m] = d[m];
+        for (int m = 1; m <= l; ++m) {
+            const T gm = (T)g[idx + m] * cmv[m] + (T)g[idx - m] * smv[m];
+            T dp = -sh.LA[l][m] * p[m - 1];
+            if (m < l) dp += sh.LB[l][m] * p[m + 1];
+            al += gm * dp;
         }
+        acc += al;
     }
     return acc;
 }
