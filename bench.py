@@ -53,6 +53,22 @@ def roofline_pipe(kernel: str, precision: str):
     # the backward's counted work is FP32 FMA on the vector ALU (its FP64 geometry is not counted)
     return "fp32 vector ALU (FMA; the published FP32 peak assumes packed issue)", FP32_PEAK_TFLOPS, "valu"
 HBM_PEAK_GBS = 8000.0
+# Launch shapes of the 32,000-atom workload per precision mode, as the
+# autotuner (jse_amd/autotune.py) chose them on an MI355X; the line and the
+# committed profiles use the same shapes (--autotune tunes afresh instead).
+TUNED_C2 = {
+    "fp32": {"desc_fwd_wg_per_cu": 32, "desc_bwd_wg_per_cu": 16, "mlp_wg_per_cu": 0, "nl_wg_per_cu": 0,
+             "gather_waves": 8, "desc_fwd_variant": 0, "desc_bwd_variant": 0},
+    "fast": {"desc_fwd_wg_per_cu": 32, "desc_bwd_wg_per_cu": 16, "mlp_wg_per_cu": 8, "nl_wg_per_cu": 0,
+             "gather_waves": 8, "desc_fwd_variant": 0, "desc_bwd_variant": 2},
+    "bf16": {"desc_fwd_wg_per_cu": 32, "desc_bwd_wg_per_cu": 16, "mlp_wg_per_cu": 4, "nl_wg_per_cu": 0,
+             "gather_waves": 0, "desc_fwd_variant": 0, "desc_bwd_variant": 0},
+}
+# the batched 512-replica step's shapes (BF16 mode), for profiling that leg
+TUNED_C4 = {
+    "bf16": {"desc_fwd_wg_per_cu": 32, "desc_bwd_wg_per_cu": 32, "mlp_wg_per_cu": 8, "nl_wg_per_cu": 16,
+             "gather_waves": 8, "desc_fwd_variant": 0, "desc_bwd_variant": 0},
+}
 
 
 def algorithmic_flops(mdl, types, pairs_per_species):
@@ -203,6 +219,25 @@ def batched_c4(nrep=512, seed=17, steps=10):
     out["bf16_vs_plain_batched"] = out["batched_bf16_graph"]["atom_evals_per_s"] / out["batched_plain_graph"][
         "atom_evals_per_s"]
     return out
+
+
+def batched_leg(prec_name, knobs, steps):
+    """Only the batched graph step of the 512-replica workload in one precision
+    mode at the given launch shapes (for profiling runs)."""
+    md = W.ternary_model()
+    reps = W.replicas(512, 64, 4096, 17)
+    natoms = sum(r.natoms for r in reps)
+    atoms = [nnap.AtomData(r.box, r.xyz, r.types) for r in reps]
+    prec = {"fp32": nnap.FP32, "fast": nnap.FP32_FAST, "bf16": nnap.BF16_MLP}[prec_name]
+    what = nnap.ENERGY | nnap.FORCE | nnap.VIRIAL
+    with nnap.NNAP(md, precision=prec) as pot:
+        b = pot.batch(atoms)
+        if knobs:
+            b.set_tuning(knobs)
+        ms, _ = b.benchmark(what, warmup=3, iters=steps)
+        b.close()
+    print(json.dumps({"leg": "batched_512_replicas", "precision": prec_name, "atoms": natoms, "ms_per_step": ms,
+                      "atom_evals_per_s": natoms / (ms * 1e-3), "launch_shapes": knobs}), flush=True)
 
 
 def precision_c3(n=256000, seed=7, steps=10):
@@ -361,18 +396,28 @@ def untiled_shapes(natoms, steps, warmup):
                               "phase_ms": {k: float(v) for k, v in zip(names, ph)}}), flush=True)
 
 
-def committed_traffic(kernel):
-    """HBM bytes that `kernel` moves per evaluation (its species launches
-    together), from the committed counter summary
-    profiles/<kernel>_c2_traffic.md (line 'traffic_bytes_per_step: N')."""
+# kernel of each timed phase, by name prefix (the plain mode's; the traffic
+# summary covers that mode)
+PHASE_KERNEL = {"desc_fwd": "jnn::k_desc_fwd_acc", "mlp": "jnn::k_mlp2_mfma_multi", "desc_bwd": "jnn::k_desc_bwd",
+                "neighbor_build": "jnn::k_nl_build_cells", "force_gather": "jnn::k_gather"}
+
+
+def committed_traffic(phase):
+    """HBM bytes per launch of the phase's kernel in the plain mode, from the
+    committed counter summary profiles/ternary32k_plain_traffic.md (lines
+    'traffic_bytes_per_launch[<kernel>]: N', FETCH_SIZE and WRITE_SIZE scaled
+    by a device copy of known size)."""
     import re
-    path = os.path.join(ROOT, "profiles", f"{kernel}_c2_traffic.md")
+    path = os.path.join(ROOT, "profiles", "ternary32k_plain_traffic.md")
     try:
         with open(path) as f:
-            m = re.search(r"traffic_bytes_per_step:\s*([0-9.eE+]+)", f.read())
-        return float(m.group(1)) if m else None
+            text = f.read()
     except OSError:
         return None
+    for m in re.finditer(r"traffic_bytes_per_launch\[([^\]]+)\]:\s*([0-9.eE+]+)", text):
+        if m.group(1).startswith(PHASE_KERNEL.get(phase, "?")):
+            return float(m.group(2))
+    return None
 
 
 def nl_bytes(natoms, npairs, fp64_rows):
@@ -405,6 +450,9 @@ def kernel_rooflines(flops, phase_ms, pmode, natoms, npairs, fp64_rows):
         pipe, peak, bound = roofline_pipe(k, pmode)
         tf = flops[k] / (phase_ms[k] * 1e-3) / 1e12
         out[k] = {"bound": bound, "pipe": pipe, "achieved": tf, "peak": peak, "unit": "TFLOP/s", "frac": tf / peak}
+    if pmode == "fp32":  # measured HBM bytes per launch (committed counter summary of this mode)
+        for k in out:
+            out[k]["traffic"] = committed_traffic(k)
     return out
 
 
@@ -427,7 +475,7 @@ def other_modes_c2(md, st):
             knobs, _, _ = autotune.tune(s_, what, use_cache=False)
             ms, ph = s_.benchmark(what, warmup=3, iters=20)
         n = st.natoms
-        out[name] = {"ms_per_step": ms, "atom_evals_per_s": n / (ms * 1e-3),
+        out[name] = {"ms_per_step": ms, "atom_evals_per_s": n / (ms * 1e-3), "launch_shapes": knobs,
                      "phase_ms": {k: float(v) for k, v in zip(names, ph)},
                      "energy_per_atom_eV": abs(r["energy"] - ref["energy"]) / n,
                      "force_rms_eV_per_A": float(np.sqrt(((r["forces"] - ref["forces"]) ** 2).mean())),
@@ -562,13 +610,18 @@ def main():
     ap.add_argument("--no-256k", action="store_true")
     ap.add_argument("--no-modes", action="store_true", help="skip the FP32_FAST / BF16 lines at 32,000 atoms")
     ap.add_argument("--no-c1", action="store_true", help="skip the Cu-256 line")
+    ap.add_argument("--no-hooks", action="store_true", help="skip the PCIe-inclusive and GPUMD-hook lines")
     ap.add_argument("--served-replicas", type=int, default=2000)
     ap.add_argument("--knobs", type=str, default="",
-                    help="JSON launch shapes for the 32,000-atom system instead of autotuning it "
-                         "(profiling runs reuse the shapes a full run chose)")
+                    help="JSON launch shapes for the 32,000-atom system (default: the committed tuned shapes "
+                         "of the precision mode)")
+    ap.add_argument("--autotune", action="store_true", help="autotune the 32,000-atom system's launch shapes")
     ap.add_argument("--shapes", action="store_true",
                     help="only time model shapes of the untiled descriptor kernels on the 32,000-atom liquid "
                          "(one JSON line per shape, the tiled default shape first)")
+    ap.add_argument("--leg", choices=["batched"], default="",
+                    help="only run the batched graph step of the 512-replica workload (with --precision and "
+                         "--knobs; for profiling)")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -583,6 +636,10 @@ def main():
     if args.shapes:
         untiled_shapes(args.natoms, args.steps, args.warmup)
         return
+    if args.leg == "batched":
+        knobs = json.loads(args.knobs) if args.knobs else TUNED_C4.get(args.precision)
+        batched_leg(args.precision, knobs, args.steps)
+        return
 
     md = W.ternary_model()
     st = W.ternary_liquid(n=args.natoms, seed=7)
@@ -594,13 +651,14 @@ def main():
     res = sysm.results()
     stats = sysm.nl_stats()
 
-    ms_default, _ = sysm.benchmark(what, warmup=2, iters=5)
-    if args.knobs:
-        knobs = json.loads(args.knobs)
-        sysm.set_tuning(knobs)
-    else:
+    ms_default = None
+    if args.autotune:
+        ms_default, _ = sysm.benchmark(what, warmup=2, iters=5)
         from jse_amd import autotune
         knobs, _, _ = autotune.tune(sysm, what, use_cache=False)
+    else:
+        knobs = json.loads(args.knobs) if args.knobs else TUNED_C2[args.precision]
+        sysm.set_tuning(knobs)
     t0 = time.perf_counter()
     ms_step, phase = sysm.benchmark(what, warmup=args.warmup, iters=args.steps)
     wall = time.perf_counter() - t0
@@ -617,7 +675,7 @@ def main():
     dom = max(("desc_fwd", "mlp", "desc_bwd"), key=lambda k: phase_ms[k])
     roof = dict(kernels[dom])
     roof["kernel"] = dom
-    roof["traffic"] = committed_traffic(dom)
+    roof["traffic"] = committed_traffic(dom) if prec == nnap.FP32 else None
     roof["flop_per_step"] = {k: float(v) for k, v in flops.items()}
     value = st.natoms / (ms_step * 1e-3)
     line = {
@@ -638,7 +696,9 @@ def main():
                                "MLP 64-64 SiLU, " + {"fp32": "plain path (JSE_NNAP_FP32)",
                                                      "fast": "FP32_FAST mode", "bf16": "BF16-MLP mode"}[pmode],
                    "natoms": st.natoms, "pairs": int(stats["pairs"]), "parallelism": "single GPU",
-                   "launch_shapes": knobs, "default_shapes_ms_per_step": ms_default},
+                   "launch_shapes": knobs, "launch_shapes_source": "autotuned" if args.autotune else
+                   ("--knobs" if args.knobs else "committed tuned shapes"),
+                   "default_shapes_ms_per_step": ms_default},
         "phase_ms": phase_ms,
         "wall_s_timed": wall,
         "roofline": roof,
@@ -646,8 +706,9 @@ def main():
     }
     if prec == nnap.FP32 and not args.no_modes:
         line["other_modes_c2"] = other_modes_c2(md, st)
-    line["pcie_inclusive"] = pcie_inclusive(pot, st)
-    line["gpumd_hook"] = gpumd_hook(pot, st)
+    if not args.no_hooks:
+        line["pcie_inclusive"] = pcie_inclusive(pot, st)
+        line["gpumd_hook"] = gpumd_hook(pot, st)
     if not args.no_cpu_baseline:
         base, err = cpu_baseline(md, st, res)
         line["cpu_baseline"] = base
