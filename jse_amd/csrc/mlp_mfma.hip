@@ -172,7 +172,13 @@ __device__ __forceinline__ void stage(int tid, Ld ld, St st) {
 
 // S <= KP (KP a multiple of 16 and of the instruction k), H1 = 16*C1, H2 = 16*C2.
 // Workgroup lb of lg takes row tiles lb, lb + lg, ...
-template <typename T, int C1, int C2, int KP>
+// W1G (FP64 inputs wider than 64): W1 is not staged (beyond 128 inputs it would
+// not fit in LDS beside W2 and H; at 128 the staged form holds one workgroup
+// per CU and measured 9 % slower) and X is not held in registers: both
+// operands of the layer-1 product and W1 in the gX product are read from
+// memory (L2) per k-step, eight k-steps of loads issued ahead of their
+// products.
+template <typename T, int C1, int C2, int KP, bool W1G = false>
 __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     using MM = Mma<T>;
     using CT = typename MM::CT;    // epilogue arithmetic (and the weights' source type)
@@ -189,8 +195,9 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     constexpr bool TC = MM::kTransposedCopies;
     constexpr bool SP = MM::kSplit;
     static_assert(KP % MM::KSTEP == 0 && KP % 16 == 0, "padded input dim");
-    constexpr int NX = KP / MM::KSTEP * MM::KPL;                          // X values per lane (layer-1 A operand)
-    __shared__ __align__(16) T W1s[H1 * XS];                 // W1[j][k]
+    static_assert(!W1G || (std::is_same<T, double>::value && KP % 32 == 0), "W1G: FP64, KP % 32 == 0");
+    constexpr int NX = W1G ? 1 : KP / MM::KSTEP * MM::KPL;               // X values per lane (layer-1 A operand)
+    __shared__ __align__(16) T W1s[W1G ? 1 : H1 * XS];       // W1[j][k]
     __shared__ __align__(16) T W2s[H2 * HS];                 // W2[j2][j1]
     __shared__ __align__(16) T W1t[TC ? KP * HS : 1];        // W1[j][k] stored [k][j]
     __shared__ __align__(16) T W2t[TC ? H1 * HS : 1];        // W2[j2][j1] stored [j1][j2]
@@ -229,6 +236,7 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     if constexpr (F64) xsrc = a.x64;
     else xsrc = a.x;
     auto load_x = [&](int t) {
+        if constexpr (W1G) return;
         const int i = t * 64 + 16 * wave + lr;
         const bool ok = i < a.nrows;
         const CT *__restrict__ xp = xsrc + (size_t)(ok ? i : 0) * S;
@@ -239,12 +247,13 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         }
     };
     if (lb * 64 < a.nrows) load_x(lb);  // in flight while the weights are staged
-    stage<H1 * XS, CT>(
-        tid, [&](int e) { const int j = e / XS, k = e % XS; return k < S ? W1g[(size_t)j * S + k] : (CT)0; },
-        [&](int e, CT w) {
-            W1s[e] = MM::cvt(w);
-            if constexpr (SP) W1l[e] = MM::lo(w);
-        });
+    if constexpr (!W1G)
+        stage<H1 * XS, CT>(
+            tid, [&](int e) { const int j = e / XS, k = e % XS; return k < S ? W1g[(size_t)j * S + k] : (CT)0; },
+            [&](int e, CT w) {
+                W1s[e] = MM::cvt(w);
+                if constexpr (SP) W1l[e] = MM::lo(w);
+            });
     stage<H2 * HS, CT>(
         tid, [&](int e) { const int j = e / HS, k = e % HS; return k < H1 ? W2g[(size_t)j * H1 + k] : (CT)0; },
         [&](int e, CT w) {
@@ -272,7 +281,7 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     T *hrow_w = Hs + r0 * HS;                         // this wave's 16 rows of H
     const T *hrow = Hs + (r0 + lr) * HS + lk;         // A-operand pointer into H
     // per-lane constants of the output columns this lane holds
-    CT b1r[C1], b2r[C2], wor[C2], gscale[NS];
+    CT b1r[C1], b2r[C2], wor[C2], gscale[W1G ? 1 : NS];
 #pragma unroll
     for (int c = 0; c < C1; ++c) b1r[c] = b1[16 * c + lr];
 #pragma unroll
@@ -281,7 +290,7 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         wor[c] = wo[16 * c + lr];
     }
 #pragma unroll
-    for (int cs = 0; cs < NS; ++cs) {
+    for (int cs = 0; cs < (W1G ? 0 : NS); ++cs) {
         const int n = 16 * cs + lr;
         // G = dE/dx / sigma, times 2 4pi/(2l+1) for the L2 power spectrum's backward
         if constexpr (F64) {
@@ -306,8 +315,31 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         ACC acc1[C1];
 #pragma unroll
         for (int c = 0; c < C1; ++c) acc1[c] = ACC{};
+        if constexpr (W1G) {
+            // A[i][k] = X[row r0 + i][k] from lane i + 16 k', B[k][j] = W1[j][k]; padding
+            // columns (k >= S) read column S - 1 against a zero X
+            const int i = row0 + r0 + lr;
+            const CT *__restrict__ xp = a.x64 + (size_t)(i < a.nrows ? i : 0) * S;
+            const bool xok = i < a.nrows;
+#pragma unroll 1
+            for (int kb = 0; kb < KP; kb += 8 * MM::KSTEP) {
+                CT xv[8], wv[8][C1];
 #pragma unroll
-        for (int ks = 0; ks < KP / MM::KSTEP; ++ks) {
+                for (int u = 0; u < 8; ++u) {
+                    const int k = kb + u * MM::KSTEP + lk;
+                    const int kc = k < S ? k : S - 1;
+                    xv[u] = (xok && k < S) ? xp[kc] : (CT)0;
+#pragma unroll
+                    for (int c = 0; c < C1; ++c) wv[u][c] = W1g[(size_t)(16 * c + lr) * S + kc];
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u)
+#pragma unroll
+                    for (int c = 0; c < C1; ++c) acc1[c] = MM::mma_r(xv[u], &wv[u][c], acc1[c]);
+            }
+        }
+#pragma unroll
+        for (int ks = 0; ks < (W1G ? 0 : KP / MM::KSTEP); ++ks) {
             const int k0 = ks * MM::KSTEP;
             const typename MM::AV xh = MM::a_hi(xr + ks * MM::KPL);
             typename MM::AV xl{};
@@ -426,13 +458,42 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         }
         wave_lds_sync();
         // ---- gX = gZ1 W1 (k = j over H1) ----
+        if constexpr (W1G) {  // B[k][j] = W1[k][16 cs + j] from memory (columns past S: column S - 1)
+            static_assert(H1P % (8 * MM::KSTEP) == 0 || !W1G, "W1G: H1 a multiple of 32");
+#pragma unroll 1
+            for (int cs = 0; cs < NS; ++cs) {
+                ACC acc = ACC{};
+                const int n = 16 * cs + lr;
+                const int nc = n < S ? n : S - 1;
+#pragma unroll 1
+                for (int kb = 0; kb < H1P; kb += 8 * MM::KSTEP) {
+                    CT wv[8];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) wv[u] = W1g[(size_t)(kb + u * MM::KSTEP + lk) * S + nc];
+#pragma unroll
+                    for (int u = 0; u < 8; ++u) acc = MM::mma(hrow + kb + u * MM::KSTEP, &wv[u], acc);
+                }
+                if (n < S) {
+                    const CT scale = (a.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * (n % a.L1) + 1)) /
+                                     a.sig64[n];
+#pragma unroll
+                    for (int r = 0; r < 4; ++r) {
+                        const int i = r0 + MM::row(lane, r);
+                        if (i < nt) a.G[(size_t)(row0 + i) * S + n] = (float)((CT)acc[r] * scale);
+                    }
+                }
+            }
+            continue;
+        }
 #pragma unroll
         for (int cs = 0; cs < NS; ++cs) {
             ACC acc = ACC{};
             const int n = 16 * cs + lr;
-            for (int k0 = 0; k0 < H1P; k0 += MM::KSTEP) {
-                if constexpr (TC) acc = MM::mma(hrow + k0, W1t + n * HS + lk + k0, acc);
-                else acc = MM::mma(hrow + k0, W1s + (k0 + lk) * XS + n, acc);
+            {
+                for (int k0 = 0; k0 < H1P; k0 += MM::KSTEP) {
+                    if constexpr (TC) acc = MM::mma(hrow + k0, W1t + n * HS + lk + k0, acc);
+                    else acc = MM::mma(hrow + k0, W1s + (k0 + lk) * XS + n, acc);
+                }
             }
             if (n < S) {
                 const CT scale = gscale[cs];
@@ -446,17 +507,17 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
     }
 }
 
-template <typename T, int C1, int C2, int KP>
+template <typename T, int C1, int C2, int KP, bool W1G>
 __global__ __launch_bounds__(256) void k_mlp2_mfma(MlpArgs a) {
-    mlp2_tiles<T, C1, C2, KP>(a, blockIdx.x, gridDim.x);
+    mlp2_tiles<T, C1, C2, KP, W1G>(a, blockIdx.x, gridDim.x);
 }
 // one launch for all species: workgroups wg0[s] .. wg0[s+1]-1 serve species s
-template <typename T, int C1, int C2, int KP>
+template <typename T, int C1, int C2, int KP, bool W1G>
 __global__ __launch_bounds__(256) void k_mlp2_mfma_multi(MlpMulti m) {
     const int b = blockIdx.x;
     int sp = 0;
     while (sp + 1 < m.nsp && b >= m.wg0[sp + 1]) ++sp;
-    mlp2_tiles<T, C1, C2, KP>(m.a[sp], b - m.wg0[sp], m.wg0[sp + 1] - m.wg0[sp]);
+    mlp2_tiles<T, C1, C2, KP, W1G>(m.a[sp], b - m.wg0[sp], m.wg0[sp + 1] - m.wg0[sp]);
 }
 
 // ---- dispatch ----
@@ -466,7 +527,8 @@ static void run_mlp2(hipStream_t s, const MlpArgs &a) {
     const int per = a.wg_per_cu > 0 ? a.wg_per_cu : 2;
     const int tiles = (a.nrows + 63) / 64;
     const int grid = tiles < ncu * per ? tiles : ncu * per;
-    hipLaunchKernelGGL((k_mlp2_mfma<T, C1, C2, KP>), dim3(grid), dim3(256), 0, s, a);
+    constexpr bool W1G = std::is_same<T, double>::value && KP > 64;
+    hipLaunchKernelGGL((k_mlp2_mfma<T, C1, C2, KP, W1G>), dim3(grid), dim3(256), 0, s, a);
 }
 
 template <typename T>
@@ -483,9 +545,9 @@ static bool dispatch_mlp2(hipStream_t s, const MlpArgs &a) {
     JNN_MLP_SHAPE(32, 32, 128)
     JNN_MLP_SHAPE(64, 64, 64)
     JNN_MLP_SHAPE(64, 64, 128)
-    if constexpr (std::is_same<T, float>::value) {  // wider inputs, FP32 only (the BF16 transposed copies
-        JNN_MLP_SHAPE(32, 32, 256)                   // and the FP64 weights would not fit in LDS, and the
-        JNN_MLP_SHAPE(32, 32, 384)                   // FP64 X operands not in registers)
+    if constexpr (!std::is_same<T, __bf16>::value) {  // wider inputs: FP32, and FP64 with W1 and X read
+        JNN_MLP_SHAPE(32, 32, 256)                     // from memory (the BF16 transposed copies would not
+        JNN_MLP_SHAPE(32, 32, 384)                     // fit in LDS)
         JNN_MLP_SHAPE(64, 64, 256)
         JNN_MLP_SHAPE(64, 64, 384)
     }
@@ -495,7 +557,8 @@ static bool dispatch_mlp2(hipStream_t s, const MlpArgs &a) {
 
 template <typename T, int C1, int C2, int KP>
 static void run_mlp2_multi(hipStream_t s, const MlpMulti &m) {
-    hipLaunchKernelGGL((k_mlp2_mfma_multi<T, C1, C2, KP>), dim3(m.wg0[m.nsp]), dim3(256), 0, s, m);
+    constexpr bool W1G = std::is_same<T, double>::value && KP > 64;
+    hipLaunchKernelGGL((k_mlp2_mfma_multi<T, C1, C2, KP, W1G>), dim3(m.wg0[m.nsp]), dim3(256), 0, s, m);
 }
 template <typename T>
 static bool dispatch_mlp2_multi(hipStream_t s, const MlpMulti &m) {
@@ -511,7 +574,7 @@ static bool dispatch_mlp2_multi(hipStream_t s, const MlpMulti &m) {
     JNN_MLP_SHAPE(32, 32, 128)
     JNN_MLP_SHAPE(64, 64, 64)
     JNN_MLP_SHAPE(64, 64, 128)
-    if constexpr (std::is_same<T, float>::value) {
+    if constexpr (!std::is_same<T, __bf16>::value) {
         JNN_MLP_SHAPE(32, 32, 256)
         JNN_MLP_SHAPE(32, 32, 384)
         JNN_MLP_SHAPE(64, 64, 256)

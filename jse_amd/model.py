@@ -41,6 +41,32 @@ WS_EPS = 1e-5  # weight standardisation epsilon of the rfuse weight (WTypeBasis.
 
 
 @dataclass
+class Basis:
+    """One spherical-Chebyshev or radial-only Chebyshev basis (a part of a
+    merge basis)."""
+    nmax: int
+    lmax: int
+    rcut: float
+    wtype: int
+    typed_wtype: int
+    P: int
+    W: np.ndarray       # [ntypes][P][K]
+    radial: bool = False
+
+    @property
+    def K(self) -> int:
+        return self.nmax + 1
+
+    @property
+    def M(self) -> int:
+        return (self.lmax + 1) ** 2
+
+    @property
+    def S(self) -> int:
+        return self.P * (self.lmax + 1)
+
+
+@dataclass
 class Species:
     symbol: str
     nmax: int
@@ -55,6 +81,8 @@ class Species:
     norm_param: np.ndarray
     mirror: int = 0     # mirror basis: neighbour species `mirror` and this one swap roles (0: none)
     radial: bool = False  # radial-only chebyshev basis: fp[p] = sum_j fc Rnp[p] (sizes as lmax = 0)
+    parts: Optional[List[Basis]] = None  # merge basis (MergedBasis.java): fp = the parts' fp concatenated;
+                                         # the single-basis fields above then hold part 0, rcut the largest
 
     @property
     def K(self) -> int:
@@ -66,7 +94,15 @@ class Species:
 
     @property
     def S(self) -> int:
+        if self.parts:
+            return sum(b.S for b in self.parts)
         return self.P * (self.lmax + 1)
+
+    def bases(self) -> List[Basis]:
+        """The species' bases in fp order (one unless it is a merge basis)."""
+        if self.parts:
+            return list(self.parts)
+        return [Basis(self.nmax, self.lmax, self.rcut, self.wtype, self.typed_wtype, self.P, self.W, self.radial)]
 
 
 @dataclass
@@ -299,8 +335,7 @@ def _shared_feed_forward(nn: dict, S: int, base_of, i: int, bases):
     in_dim = nn.get("input_dim")
     if in_dim is not None and int(in_dim) != S:
         raise ValueError(f"Input dimension mismatch: {int(in_dim)} vs {S}")
-    nb, lb = bases[b - 1][5], bases[b - 1][1]
-    if nb * (lb + 1) != S:
+    if sum(bp.S for bp in bases[b - 1]) != S:
         raise ValueError("Input dimensions mismatch for shared nn")
     shd = _get(nn, "shared_hidden_dims", "shared_nnarch")
     if shd is None:
@@ -359,6 +394,43 @@ def _shared_feed_forward(nn: dict, S: int, base_of, i: int, bases):
     return list(hidden), out
 
 
+def _basis(b: dict, btype: str, ntypes: int) -> Basis:
+    """SphericalChebyshev.load / Chebyshev.load (SphericalChebyshev.java:60-100,
+    Chebyshev.java) with the WTypeBasis mixing (WTypeBasis.java:202-304)."""
+    radial = btype in ("chebyshev", "cheby")  # Chebyshev.java: no angular part
+    if not radial and btype not in ("spherical_chebyshev", "sph_cheby"):
+        raise ValueError(f"Unsupported basis type: {btype}")
+    nmax = int(_get(b, "nmax", default=DEFAULT_NMAX))
+    lmax = 0 if radial else int(_get(b, "lmax", default=DEFAULT_LMAX))
+    l3 = 0 if radial else int(_get(b, "l3max", default=0))
+    l4 = 0 if radial else int(_get(b, "l4max", default=0))
+    if lmax < 0 or lmax > 12:
+        raise ValueError(f"Input lmax MUST be in [0, 12], input: {lmax}")
+    if nmax < 0 or nmax > 20:   # WTypeBasis.java:56
+        raise ValueError(f"Input nmax MUST be in [0, 20], input: {nmax}")
+    if l3 != 0 or l4 != 0:
+        raise ValueError("l3max/l4max > 0 are not supported by this build")
+    rcut = float(_get(b, "rcut", default=DEFAULT_RCUT))
+    wt = _get(b, "wtype", default="default")
+    wtype = int(wt) if isinstance(wt, (int, float)) else WTYPES.get(str(wt))
+    if wtype is None:
+        raise ValueError(f"Input wtype MUST be in {{-1, 0, 2, 3, 4, 5, 6}}, input: {wt}")
+    if wtype not in WTYPES.values():
+        raise ValueError(f"Input wtype MUST be in {{-1, 0, 2, 3, 4, 5, 6}}, input: {wtype}")
+    fuse_w = _fuse_weight(b, wtype, ntypes)
+    rfuse_w = _rfuse_weight(b, wtype, _size_n(wtype, ntypes, nmax, 0))
+    rfuse_scale = float(b.get("rfuse_scale", 1.0) or 1.0)
+    ws = bool(_get(b, "weight_standardization", "ws", default=False))
+    post = _post_fuse(b, wtype, ntypes, nmax, fuse_w)
+    if post is not None:
+        # an older file's post-fuse weight becomes the rfuse weight it equals
+        # (SphericalChebyshev.java:81-94, WTypeBasis.postFuse2RFuse_)
+        rfuse_w, wtype, fuse_w = post, WTYPE_RFUSE, None
+        rfuse_scale = float(b.get("post_fuse_scale", 1.0) or 1.0)
+    typed, P, W = _mixing(wtype, ntypes, nmax, fuse_w, rfuse_w, rfuse_scale, ws)
+    return Basis(nmax, lmax, rcut, wtype, typed, P, W, radial)
+
+
 def load_model(src) -> Model:
     """Parse a model (path to .json/.yaml/.yml, JSON text, or a dict) as
     ``new NNAP(path)`` does (NNAP.java:129-307, 402-414)."""
@@ -392,7 +464,8 @@ def load_model(src) -> Model:
     mu_e = 0.0 if norm_mu_eng is None else float(norm_mu_eng)
     # pass 1 (Basis.java:22-50): the species' own bases; mirror / share entries
     # only record the species whose basis they take
-    bases = [None] * ntypes
+    bases = [None] * ntypes     # per species: its list of Basis (one unless merge)
+    btypes = [""] * ntypes
     ref = [0] * ntypes
     is_mirror = [False] * ntypes
     for i, m in enumerate(models):
@@ -402,6 +475,7 @@ def load_model(src) -> Model:
             raise ValueError("torch model is invalid now.")
         b = m.get("basis") or {"type": "spherical_chebyshev"}
         btype = str(b.get("type", "spherical_chebyshev"))
+        btypes[i] = btype
         if btype in ("mirror", "mirror_basis", "share", "shared_basis"):
             mir = btype.startswith("mirror")
             key = "mirror" if mir else "share"
@@ -412,38 +486,21 @@ def load_model(src) -> Model:
                 raise ValueError("mirror/share type out of range")
             is_mirror[i] = mir
             continue
-        radial = btype in ("chebyshev", "cheby")  # Chebyshev.java: no angular part
-        if not radial and btype not in ("spherical_chebyshev", "sph_cheby"):
-            raise ValueError(f"Unsupported basis type: {btype}")
-        nmax = int(_get(b, "nmax", default=DEFAULT_NMAX))
-        lmax = 0 if radial else int(_get(b, "lmax", default=DEFAULT_LMAX))
-        l3 = 0 if radial else int(_get(b, "l3max", default=0))
-        l4 = 0 if radial else int(_get(b, "l4max", default=0))
-        if lmax < 0 or lmax > 12:
-            raise ValueError(f"Input lmax MUST be in [0, 12], input: {lmax}")
-        if nmax < 0 or nmax > 20:   # WTypeBasis.java:56
-            raise ValueError(f"Input nmax MUST be in [0, 20], input: {nmax}")
-        if l3 != 0 or l4 != 0:
-            raise ValueError("l3max/l4max > 0 are not supported by this build")
-        rcut = float(_get(b, "rcut", default=DEFAULT_RCUT))
-        wt = _get(b, "wtype", default="default")
-        wtype = int(wt) if isinstance(wt, (int, float)) else WTYPES.get(str(wt))
-        if wtype is None:
-            raise ValueError(f"Input wtype MUST be in {{-1, 0, 2, 3, 4, 5, 6}}, input: {wt}")
-        if wtype not in WTYPES.values():
-            raise ValueError(f"Input wtype MUST be in {{-1, 0, 2, 3, 4, 5, 6}}, input: {wtype}")
-        fuse_w = _fuse_weight(b, wtype, ntypes)
-        rfuse_w = _rfuse_weight(b, wtype, _size_n(wtype, ntypes, nmax, 0))
-        rfuse_scale = float(b.get("rfuse_scale", 1.0) or 1.0)
-        ws = bool(_get(b, "weight_standardization", "ws", default=False))
-        post = _post_fuse(b, wtype, ntypes, nmax, fuse_w)
-        if post is not None:
-            # an older file's post-fuse weight becomes the rfuse weight it equals
-            # (SphericalChebyshev.java:81-94, WTypeBasis.postFuse2RFuse_)
-            rfuse_w, wtype, fuse_w = post, WTYPE_RFUSE, None
-            rfuse_scale = float(b.get("post_fuse_scale", 1.0) or 1.0)
-        typed, P, W = _mixing(wtype, ntypes, nmax, fuse_w, rfuse_w, rfuse_scale, ws)
-        bases[i] = (nmax, lmax, rcut, wtype, typed, P, W, radial)
+        if btype == "merge":  # MergedBasis.load (MergedBasis.java:204-229)
+            sub = b.get("basis")
+            if sub is None:
+                raise ValueError("Key `basis` required for merge load")
+            parts = []
+            for bm in sub:
+                st = str(bm.get("type", "spherical_chebyshev"))
+                if st not in ("spherical_chebyshev", "sph_cheby", "chebyshev", "cheby"):
+                    raise ValueError(f"Unsupported basis type: {st}")
+                parts.append(_basis(bm, st, ntypes))
+            if not parts:
+                raise ValueError("Merge basis can not be null or empty")
+            bases[i] = parts
+            continue
+        bases[i] = [_basis(b, btype, ntypes)]
     # pass 2 (Basis.java:51-66): mirror / share take the named species' basis
     for i in range(ntypes):
         if ref[i]:
@@ -458,8 +515,7 @@ def load_model(src) -> Model:
     nets = [None] * ntypes      # (hidden, nn_param)
     shared_nn = [False] * ntypes
     for i, m in enumerate(models):
-        nmax, lmax, rcut, wtype, typed, P, W, radial = bases[i]
-        S = P * (lmax + 1)
+        S = sum(bp.S for bp in bases[i])
         if is_mirror[i]:
             if m.get("nn") is not None:
                 raise ValueError("nn data in mirror ModelInfo MUST be empty")
@@ -483,15 +539,14 @@ def load_model(src) -> Model:
         if is_mirror[i]:
             nets[i] = base_of(i, ref[i])
         elif shared_nn[i]:
-            nmax, lmax, rcut, wtype, typed, P, W, radial = bases[i]
-            nets[i] = _shared_feed_forward(m["nn"], P * (lmax + 1), base_of, i, bases)
+            nets[i] = _shared_feed_forward(m["nn"], sum(bp.S for bp in bases[i]), base_of, i, bases)
     # normalisation (NNAP.java:272-306): a mirror species must not carry its
     # own and uses that of the species it mirrors
     out: List[Species] = []
     for i, m in enumerate(models):
         sym = m.get("symbol")
-        nmax, lmax, rcut, wtype, typed, P, W, radial = bases[i]
-        S = P * (lmax + 1)
+        b0 = bases[i][0]
+        S = sum(bp.S for bp in bases[i])
         src = m
         if is_mirror[i]:
             if m.get("ref_eng") is not None:
@@ -508,9 +563,57 @@ def load_model(src) -> Model:
             raise ValueError("norm_mu/norm_sigma size mismatch")
         norm = np.concatenate([[mu_e + ref_eng, sig_e], mu, sg])
         hidden, nn_param = nets[i]
-        out.append(Species(str(sym), nmax, lmax, rcut, wtype, typed, P, W, list(hidden), nn_param.copy(), norm,
-                           ref[i] if is_mirror[i] else 0, radial))
+        rcut = max(bp.rcut for bp in bases[i])
+        out.append(Species(str(sym), b0.nmax, b0.lmax, rcut, b0.wtype, b0.typed_wtype, b0.P, b0.W, list(hidden),
+                           nn_param.copy(), norm, ref[i] if is_mirror[i] else 0, b0.radial,
+                           list(bases[i]) if len(bases[i]) > 1 or btypes[i] == "merge" else None))
     return Model(out, info.get("units"), info)
+
+
+def _random_network(rng, S: int, hidden: Sequence[int]) -> dict:
+    """A feed_forward entry drawn as FeedForward.initParameters draws it
+    (FeedForward.java:105-128)."""
+    cols = S
+    hws, hbs = [], []
+    for h in hidden:
+        bw = math.sqrt(6.0 / cols)
+        hws.append(rng.uniform(-bw, bw, size=(h, cols)).tolist())
+        cols = h
+    cols = S
+    for h in hidden:
+        bb = math.sqrt(1.0 / cols)
+        hbs.append(rng.uniform(-bb, bb, size=h).tolist())
+        cols = h
+    bo = math.sqrt(3.0 / cols)
+    ow = rng.uniform(-bo, bo, size=cols).tolist()
+    ob = float(rng.uniform(-math.sqrt(1.0 / cols), math.sqrt(1.0 / cols)))
+    return {"type": "feed_forward", "input_dim": S, "hidden_dims": list(hidden), "hidden_weights": hws,
+            "hidden_biases": hbs, "output_weight": ow, "output_bias": ob}
+
+
+def random_merge_model(symbols: Sequence[str], parts: Sequence[dict], hidden: Sequence[int], seed: int) -> dict:
+    """Seeded random-init model whose species all carry a merge basis of the
+    given part entries (each a spherical_chebyshev or chebyshev basis entry
+    with its nmax, lmax, rcut, wtype; MergedBasis.java)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ntypes = len(symbols)
+    S = 0
+    for b in parts:
+        radial = str(b.get("type", "spherical_chebyshev")) in ("chebyshev", "cheby")
+        wt = WTYPES[str(b.get("wtype", "default"))]
+        nmax = int(b.get("nmax", DEFAULT_NMAX))
+        P = _size_n(wt, ntypes, nmax, 0)
+        if ntypes == 1 and wt in (WTYPE_DEFAULT, WTYPE_FULL, WTYPE_EXFULL):
+            P = _size_n(WTYPE_NONE, 1, nmax, 0)
+        S += P * (1 if radial else int(b.get("lmax", DEFAULT_LMAX)) + 1)
+    models = []
+    for sym in symbols:
+        models.append({"symbol": sym, "basis": {"type": "merge", "basis": [dict(b) for b in parts]},
+                       "nn": _random_network(rng, S, hidden), "ref_eng": 0.0,
+                       "norm_mu": [0.0] * S, "norm_sigma": [1.0] * S})
+    models[0]["norm_mu_eng"] = 0.0
+    models[0]["norm_sigma_eng"] = 1.0
+    return {"version": VERSION, "units": "metal", "models": models}
 
 
 def random_model(symbols: Sequence[str], nmax: int, lmax: int, rcut: float, hidden: Sequence[int],
@@ -522,33 +625,17 @@ def random_model(symbols: Sequence[str], nmax: int, lmax: int, rcut: float, hidd
     ``set_normalization`` is called."""
     rng = np.random.Generator(np.random.PCG64(seed))
     ntypes = len(symbols)
-    K = nmax + 1
     P = _size_n(WTYPES[wtype], ntypes, nmax, 0)
     if ntypes == 1 and WTYPES[wtype] in (WTYPE_DEFAULT, WTYPE_FULL, WTYPE_EXFULL):
         P = _size_n(WTYPE_NONE, 1, nmax, 0)
     S = P * (lmax + 1)
     models = []
     for sym in symbols:
-        cols = S
-        hws, hbs = [], []
-        for h in hidden:
-            bw = math.sqrt(6.0 / cols)
-            hws.append(rng.uniform(-bw, bw, size=(h, cols)).tolist())
-            cols = h
-        cols = S
-        for h in hidden:
-            bb = math.sqrt(1.0 / cols)
-            hbs.append(rng.uniform(-bb, bb, size=h).tolist())
-            cols = h
-        bo = math.sqrt(3.0 / cols)
-        ow = rng.uniform(-bo, bo, size=cols).tolist()
-        ob = float(rng.uniform(-math.sqrt(1.0 / cols), math.sqrt(1.0 / cols)))
         models.append({
             "symbol": sym,
             "basis": {"type": "spherical_chebyshev", "nmax": nmax, "lmax": lmax, "l3max": 0, "l4max": 0,
                       "rcut": rcut, "wtype": wtype},
-            "nn": {"type": "feed_forward", "input_dim": S, "hidden_dims": list(hidden),
-                   "hidden_weights": hws, "hidden_biases": hbs, "output_weight": ow, "output_bias": ob},
+            "nn": _random_network(rng, S, hidden),
             "ref_eng": 0.0,
             "norm_mu": [0.0] * S,
             "norm_sigma": [1.0] * S,

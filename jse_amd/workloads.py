@@ -161,29 +161,34 @@ def descriptors_np(mdl: _model.Model, st: Structure, atoms: Optional[np.ndarray]
     shifts = np.array([[sx, sy, sz] for sz in (-1, 0, 1) for sy in (-1, 0, 1) for sx in (-1, 0, 1)], float)
     for i in atoms:
         sp = mdl.species[st.types[i] - 1]
-        d = (x[None, :, :] + shifts[:, None, :] * box) - x[i]
-        d = d.reshape(-1, 3)
+        d0 = ((x[None, :, :] + shifts[:, None, :] * box) - x[i]).reshape(-1, 3)
         jidx = np.tile(np.arange(st.natoms), 27)
-        r2 = (d * d).sum(1)
-        keep = (r2 < sp.rcut ** 2) & (r2 > 0)
-        d = d[keep]
-        tj = st.types[jidx[keep]]
-        r = np.sqrt((d * d).sum(1))
-        Y = _real_sph_np(sp.lmax, d[:, 0], d[:, 1], d[:, 2])
-        xx = 1.0 - 2.0 * r / sp.rcut
-        Rn = np.zeros((len(r), sp.K))
-        Rn[:, 0] = 1.0
-        if sp.K > 1:
-            Rn[:, 1] = xx
-        for q in range(2, sp.K):
-            Rn[:, q] = 2 * xx * Rn[:, q - 1] - Rn[:, q - 2]
-        fc = (1.0 - (r / sp.rcut) ** 2) ** 4
-        c = np.einsum("jpk,jk->jp", sp.W[tj - 1], Rn) * fc[:, None]
-        A = c.T @ Y
-        fp = np.zeros((sp.P, sp.lmax + 1))
-        for l in range(sp.lmax + 1):
-            fp[:, l] = 4 * math.pi / (2 * l + 1) * (A[:, l * l:(l + 1) * (l + 1)] ** 2).sum(1)
-        out[st.types[i] - 1].append(fp.reshape(-1))
+        r20 = (d0 * d0).sum(1)
+        parts = []
+        for b in sp.bases():  # a merge basis: the parts' fp concatenated
+            keep = (r20 < b.rcut ** 2) & (r20 > 0)
+            d = d0[keep]
+            tj = st.types[jidx[keep]]
+            r = np.sqrt((d * d).sum(1))
+            xx = 1.0 - 2.0 * r / b.rcut
+            Rn = np.zeros((len(r), b.K))
+            Rn[:, 0] = 1.0
+            if b.K > 1:
+                Rn[:, 1] = xx
+            for q in range(2, b.K):
+                Rn[:, q] = 2 * xx * Rn[:, q - 1] - Rn[:, q - 2]
+            fc = (1.0 - (r / b.rcut) ** 2) ** 4
+            c = np.einsum("jpk,jk->jp", b.W[tj - 1], Rn) * fc[:, None]
+            if b.radial:
+                parts.append(c.sum(0))
+                continue
+            Y = _real_sph_np(b.lmax, d[:, 0], d[:, 1], d[:, 2])
+            A = c.T @ Y
+            fp = np.zeros((b.P, b.lmax + 1))
+            for l in range(b.lmax + 1):
+                fp[:, l] = 4 * math.pi / (2 * l + 1) * (A[:, l * l:(l + 1) * (l + 1)] ** 2).sum(1)
+            parts.append(fp.reshape(-1))
+        out[st.types[i] - 1].append(np.concatenate(parts))
     return [np.array(v) if v else None for v in out]
 
 

@@ -43,20 +43,37 @@
 #define ORC_MAX_LAYERS 8
 #define ORC_MAX_H 1024
 
+/* Bases are indexed by basis number b: species t owns bases part0[t] ..
+ * part0[t] + nparts[t] - 1 (a merge basis, MergedBasis.java: its fp is the
+ * parts' fp concatenated); with part0 / nparts NULL species t owns basis t. */
 typedef struct {
     int ntypes;
-    const int *nmax;          /* [ntypes] */
-    const int *lmax;          /* [ntypes] */
-    const int *size_np;       /* [ntypes]  P */
-    const double *rcut;       /* [ntypes] */
+    const int *nmax;          /* [bases] */
+    const int *lmax;          /* [bases] */
+    const int *size_np;       /* [bases]  P */
+    const double *rcut;       /* [bases] */
     const int *nlayers;       /* [ntypes] */
     const int *hidden;        /* [ntypes][ORC_MAX_LAYERS] */
-    const double *const *fp_param;
-    const double *const *nn_param;
-    const double *const *norm_param;
+    const double *const *fp_param;   /* [bases] W[ntypes][P][K] */
+    const double *const *nn_param;   /* [ntypes] */
+    const double *const *norm_param; /* [ntypes] */
     const int *mirror;        /* [ntypes] mirror basis: the mirrored species, 0 none (NULL: none at all) */
-    const int *radial;        /* [ntypes] 1: radial-only chebyshev basis (lmax 0 sizes), NULL: none */
+    const int *radial;        /* [bases] 1: radial-only chebyshev basis (lmax 0 sizes), NULL: none */
+    const int *part0;         /* [ntypes] first basis of each species (NULL: basis t) */
+    const int *nparts;        /* [ntypes] number of bases of each species (NULL: one) */
 } orc_model;
+
+static int nparts_of(const orc_model *m, int t) { return m->nparts ? m->nparts[t] : 1; }
+static int basis_index(const orc_model *m, int t, int k) { return (m->part0 ? m->part0[t] : t) + k; }
+/* a species' cutoff: the largest of its bases' (MergedBasis.rcutMax) */
+static double species_rcut(const orc_model *m, int t) {
+    double r = 0.0;
+    for (int k = 0; k < nparts_of(m, t); ++k) {
+        const double rk = m->rcut[basis_index(m, t, k)];
+        if (rk > r) r = rk;
+    }
+    return r;
+}
 
 /* ------------------------------------------------------------------ */
 /* wrap: IBox.wrapPBC for an orthogonal box (IBox.java:145-151).       */
@@ -381,21 +398,28 @@ typedef struct {
     const double *nn, *norm;
 } sp_view;
 
-static sp_view species(const orc_model *m, int ctype) {
+/* view of basis k of species ctype (S: that basis' size) */
+static sp_view basis_view(const orc_model *m, int ctype, int k) {
     sp_view v;
-    int t = ctype - 1;
-    v.K = m->nmax[t] + 1;
-    v.P = m->size_np[t];
-    v.L1 = m->lmax[t] + 1;
+    int t = ctype - 1, b = basis_index(m, t, k);
+    v.K = m->nmax[b] + 1;
+    v.P = m->size_np[b];
+    v.L1 = m->lmax[b] + 1;
     v.M = v.L1 * v.L1;
     v.S = v.P * v.L1;
-    v.rc = m->rcut[t];
-    v.W = m->fp_param[t];
+    v.rc = m->rcut[b];
+    v.W = m->fp_param[b];
     v.nl = m->nlayers[t];
     v.h = m->hidden + t * ORC_MAX_LAYERS;
     v.nn = m->nn_param[t];
     v.norm = m->norm_param[t];
-    v.radial = m->radial ? m->radial[t] : 0;
+    v.radial = m->radial ? m->radial[b] : 0;
+    return v;
+}
+/* the species' view for its network: S = the size of all its bases */
+static sp_view species(const orc_model *m, int ctype) {
+    sp_view v = basis_view(m, ctype, 0);
+    for (int k = 1; k < nparts_of(m, ctype - 1); ++k) v.S += basis_view(m, ctype, k).S;
     return v;
 }
 
@@ -613,16 +637,51 @@ static const int *role_types(const orc_model *m, int ctype, const int *ty, int n
     return buf;
 }
 
+/* fp of every basis of species ctype, concatenated in order; A of basis k
+ * (spherical) at A + k * ORC_MAX_P * ORC_MAX_M */
+static void fp_all(const orc_model *m, int ctype, const double *dx, const double *dy, const double *dz,
+                   const int *ty, int nn, double *A, double *fp) {
+    int off = 0;
+    for (int k = 0; k < nparts_of(m, ctype - 1); ++k) {
+        sp_view v = basis_view(m, ctype, k);
+        if (v.radial) descriptor_radial(&v, dx, dy, dz, ty, nn, fp + off);
+        else descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A + (size_t)k * ORC_MAX_P * ORC_MAX_M, fp + off);
+        off += v.S;
+    }
+}
+/* g_j summed over the bases (each basis' backward with its slice of gfp) */
+static void backward_all(const orc_model *m, int ctype, const double *dx, const double *dy, const double *dz,
+                         const int *ty, int nn, const double *A, const double *gfp, double *gx, double *gy,
+                         double *gz) {
+    const int np = nparts_of(m, ctype - 1);
+    double *t = np > 1 ? (double *)malloc(sizeof(double) * 3 * (size_t)(nn > 0 ? nn : 1)) : NULL;
+    int off = 0;
+    for (int k = 0; k < np; ++k) {
+        sp_view v = basis_view(m, ctype, k);
+        double *ox = k == 0 ? gx : t, *oy = k == 0 ? gy : t + nn, *oz = k == 0 ? gz : t + 2 * nn;
+        if (v.radial) descriptor_radial_backward(&v, dx, dy, dz, ty, nn, gfp + off, ox, oy, oz);
+        else descriptor_backward(&v, v.L1 - 1, dx, dy, dz, ty, nn, A + (size_t)k * ORC_MAX_P * ORC_MAX_M,
+                                 gfp + off, ox, oy, oz);
+        if (k > 0)
+            for (int j = 0; j < nn; ++j) {
+                gx[j] += ox[j];
+                gy[j] += oy[j];
+                gz[j] += oz[j];
+            }
+        off += v.S;
+    }
+    free(t);
+}
+
 /* jse_nnap_calFp restated (nnap_interface.cpp:11-24) */
 int orc_cal_fp(const orc_model *m, int ctype, const double *dx, const double *dy, const double *dz,
                const int *ty_in, int nn, double *fp) {
     if (ctype < 1 || ctype > m->ntypes) return 1;
     int *buf = (int *)malloc(sizeof(int) * (size_t)(nn > 0 ? nn : 1));
     const int *ty = role_types(m, ctype, ty_in, nn, buf);
-    sp_view v = species(m, ctype);
-    double A[ORC_MAX_P * ORC_MAX_M];
-    if (v.radial) descriptor_radial(&v, dx, dy, dz, ty, nn, fp);
-    else descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
+    double *A = (double *)malloc(sizeof(double) * (size_t)nparts_of(m, ctype - 1) * ORC_MAX_P * ORC_MAX_M);
+    fp_all(m, ctype, dx, dy, dz, ty, nn, A, fp);
+    free(A);
     free(buf);
     return 0;
 }
@@ -633,10 +692,12 @@ int orc_cal_energy(const orc_model *m, int ctype, const double *dx, const double
     int *buf = (int *)malloc(sizeof(int) * (size_t)(nn > 0 ? nn : 1));
     const int *ty = role_types(m, ctype, ty_in, nn, buf);
     sp_view v = species(m, ctype);
-    double A[ORC_MAX_P * ORC_MAX_M], fp[ORC_MAX_P * (ORC_MAX_LMAX + 1)];
-    if (v.radial) descriptor_radial(&v, dx, dy, dz, ty, nn, fp);
-    else descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
+    double *A = (double *)malloc(sizeof(double) * (size_t)nparts_of(m, ctype - 1) * ORC_MAX_P * ORC_MAX_M);
+    double *fp = (double *)malloc(sizeof(double) * (size_t)v.S);
+    fp_all(m, ctype, dx, dy, dz, ty, nn, A, fp);
     *e = mlp(&v, fp, NULL);
+    free(fp);
+    free(A);
     free(buf);
     return 0;
 }
@@ -647,16 +708,14 @@ int orc_cal_energy_force(const orc_model *m, int ctype, const double *dx, const 
     int *buf = (int *)malloc(sizeof(int) * (size_t)(nn > 0 ? nn : 1));
     const int *ty = role_types(m, ctype, ty_in, nn, buf);
     sp_view v = species(m, ctype);
-    double A[ORC_MAX_P * ORC_MAX_M], fp[ORC_MAX_P * (ORC_MAX_LMAX + 1)], gfp[ORC_MAX_P * (ORC_MAX_LMAX + 1)];
-    if (v.radial) {
-        descriptor_radial(&v, dx, dy, dz, ty, nn, fp);
-        *e = mlp(&v, fp, gfp);
-        descriptor_radial_backward(&v, dx, dy, dz, ty, nn, gfp, gx, gy, gz);
-    } else {
-        descriptor(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, fp);
-        *e = mlp(&v, fp, gfp);
-        descriptor_backward(&v, v.L1 - 1, dx, dy, dz, ty, nn, A, gfp, gx, gy, gz);
-    }
+    double *A = (double *)malloc(sizeof(double) * (size_t)nparts_of(m, ctype - 1) * ORC_MAX_P * ORC_MAX_M);
+    double *fp = (double *)malloc(sizeof(double) * 2 * (size_t)v.S);
+    double *gfp = fp + v.S;
+    fp_all(m, ctype, dx, dy, dz, ty, nn, A, fp);
+    *e = mlp(&v, fp, gfp);
+    backward_all(m, ctype, dx, dy, dz, ty, nn, A, gfp, gx, gy, gz);
+    free(fp);
+    free(A);
     free(buf);
     return 0;
 }
@@ -679,7 +738,7 @@ int orc_mlp(const orc_model *m, int ctype, const double *fp, double *e, double *
 int orc_system_efv(const orc_model *m, int n, const double *xyz_in, const int *type, const double *box,
                    int cap, double *E, double *F, double *V, int *nei_count) {
     double rmax = 0.0;
-    for (int t = 0; t < m->ntypes; ++t) if (m->rcut[t] > rmax) rmax = m->rcut[t];
+    for (int t = 0; t < m->ntypes; ++t) if (species_rcut(m, t) > rmax) rmax = species_rcut(m, t);
     double *x = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
     orc_wrap(n, xyz_in, box, x);
     int *count = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
@@ -698,7 +757,7 @@ int orc_system_efv(const orc_model *m, int n, const double *xyz_in, const int *t
     for (int i = 0; i < n; ++i) {
         int ct = type[i];
         if (ct < 1 || ct > m->ntypes) { rc = 3; break; }
-        double rci = m->rcut[ct - 1];
+        double rci = species_rcut(m, ct - 1);
         double rc2 = rci * rci;
         int nn = 0;
         for (int k = 0; k < count[i]; ++k) {
@@ -733,7 +792,7 @@ int orc_system_efv(const orc_model *m, int n, const double *xyz_in, const int *t
 int orc_system_fp(const orc_model *m, int n, const double *xyz_in, const int *type, const double *box, int cap,
                   int smax, double *fp_out) {
     double rmax = 0.0;
-    for (int t = 0; t < m->ntypes; ++t) if (m->rcut[t] > rmax) rmax = m->rcut[t];
+    for (int t = 0; t < m->ntypes; ++t) if (species_rcut(m, t) > rmax) rmax = species_rcut(m, t);
     double *x = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
     orc_wrap(n, xyz_in, box, x);
     int *count = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
@@ -770,7 +829,7 @@ int orc_system_fp(const orc_model *m, int n, const double *xyz_in, const int *ty
 int orc_system_efv_par(const orc_model *m, int n, const double *xyz_in, const int *type, const double *box,
                        int cap, int nthreads, double *E, double *F, double *V) {
     double rmax = 0.0;
-    for (int t = 0; t < m->ntypes; ++t) if (m->rcut[t] > rmax) rmax = m->rcut[t];
+    for (int t = 0; t < m->ntypes; ++t) if (species_rcut(m, t) > rmax) rmax = species_rcut(m, t);
     double *x = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
     orc_wrap(n, xyz_in, box, x);
     int *count = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));

@@ -32,6 +32,8 @@ class OrcModel(ctypes.Structure):
         ("norm_param", ctypes.POINTER(_c_dbl_p)),
         ("mirror", _c_int_p),
         ("radial", _c_int_p),
+        ("part0", _c_int_p),
+        ("nparts", _c_int_p),
     ]
 
 
@@ -72,25 +74,30 @@ class Oracle:
         self.L = lib(fast)
         nt = mdl.ntypes
         sp = mdl.species
-        self._nmax = np.array([s.nmax for s in sp], dtype=np.int32)
-        self._lmax = np.array([s.lmax for s in sp], dtype=np.int32)
-        self._P = np.array([s.P for s in sp], dtype=np.int32)
-        self._rc = np.array([s.rcut for s in sp], dtype=np.float64)
+        # bases in species order (a merge species owns several, in fp order)
+        bl = [b for s in sp for b in s.bases()]
+        self._part0 = np.cumsum([0] + [len(s.bases()) for s in sp])[:-1].astype(np.int32)
+        self._nparts = np.array([len(s.bases()) for s in sp], dtype=np.int32)
+        self._nmax = np.array([b.nmax for b in bl], dtype=np.int32)
+        self._lmax = np.array([b.lmax for b in bl], dtype=np.int32)
+        self._P = np.array([b.P for b in bl], dtype=np.int32)
+        self._rc = np.array([b.rcut for b in bl], dtype=np.float64)
         self._nl = np.array([len(s.hidden) for s in sp], dtype=np.int32)
         hid = np.zeros((nt, MAX_LAYERS), dtype=np.int32)
         for t, s in enumerate(sp):
             hid[t, :len(s.hidden)] = s.hidden
         self._hid = hid
-        self._W = [np.ascontiguousarray(s.W.reshape(-1), dtype=np.float64) for s in sp]
+        self._W = [np.ascontiguousarray(b.W.reshape(-1), dtype=np.float64) for b in bl]
         self._nn = [np.ascontiguousarray(s.nn_param, dtype=np.float64) for s in sp]
         self._norm = [np.ascontiguousarray(s.norm_param, dtype=np.float64) for s in sp]
-        self._Wp = (_c_dbl_p * nt)(*[_dp(a) for a in self._W])
+        self._Wp = (_c_dbl_p * len(bl))(*[_dp(a) for a in self._W])
         self._nnp = (_c_dbl_p * nt)(*[_dp(a) for a in self._nn])
         self._normp = (_c_dbl_p * nt)(*[_dp(a) for a in self._norm])
         self._mirror = np.array([getattr(s, "mirror", 0) for s in sp], dtype=np.int32)
-        self._radial = np.array([1 if getattr(s, "radial", False) else 0 for s in sp], dtype=np.int32)
+        self._radial = np.array([1 if b.radial else 0 for b in bl], dtype=np.int32)
         self.c = OrcModel(nt, _ip(self._nmax), _ip(self._lmax), _ip(self._P), _dp(self._rc), _ip(self._nl),
-                          _ip(self._hid), self._Wp, self._nnp, self._normp, _ip(self._mirror), _ip(self._radial))
+                          _ip(self._hid), self._Wp, self._nnp, self._normp, _ip(self._mirror), _ip(self._radial),
+                          _ip(self._part0), _ip(self._nparts))
 
     # --- per-atom entry points (jse_nnap_calFp / calEnergy / calEnergyForce) ---
     def cal_fp(self, ctype, d, tj):

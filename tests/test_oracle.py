@@ -539,3 +539,87 @@ def test_radial_basis_forces_are_minus_energy_gradient(O):
             ep = orc.system_efv(W.Structure(st.box, xp, st.types))["energy"]
             em = orc.system_efv(W.Structure(st.box, xm, st.types))["energy"]
             assert abs(-(ep - em) / (2 * h) - res["forces"][i, c]) < 2e-7 * max(1, abs(res["forces"][i, c]))
+
+
+# ---------------------------------------------------------------- merge basis
+_MERGE_PARTS = [
+    {"type": "spherical_chebyshev", "nmax": 4, "lmax": 4, "rcut": 5.5, "wtype": "default"},
+    {"type": "chebyshev", "nmax": 6, "rcut": 4.0, "wtype": "full"},
+    {"type": "spherical_chebyshev", "nmax": 3, "lmax": 2, "rcut": 3.5, "wtype": "none"},
+]
+
+
+def _merge_model(parts, seed, hidden=(12, 12)):
+    st = W.ternary_liquid(n=200, seed=13)
+    md = M.random_merge_model(["Cu", "Zr", "Al"], parts, hidden, seed)
+    return st, W._normalized(md, st, 100, seed)
+
+
+def test_merge_basis_fp_is_the_parts_concatenated(O):
+    """MergedBasis (MergedBasis.java:22-56, 204-229): the fp of a merge basis is
+    its parts' fp in order, each part evaluated with its own cutoff.  Checked
+    against models that carry each part alone, on a neighbour set that reaches
+    past the smaller cutoffs."""
+    _, md = _merge_model(_MERGE_PARTS, 71)
+    mdl = M.load_model(md)
+    sp = mdl.species[0]
+    assert sp.parts is not None and [b.S for b in sp.bases()] == [50, 21, 12] and sp.S == 83
+    assert sp.rcut == 5.5 and mdl.rcut_max == 5.5
+    orc = O.Oracle(mdl)
+    rng = np.random.default_rng(5)
+    d = rng.normal(size=(40, 3)) * 2.2
+    tj = rng.integers(1, 4, size=40)
+    fp = orc.cal_fp(2, d, tj)
+    off = 0
+    for b in _MERGE_PARTS:
+        one = M.random_merge_model(["Cu", "Zr", "Al"], [b], (12, 12), 71)
+        mo = M.load_model(one)
+        fpk = O.Oracle(mo).cal_fp(2, d, tj)
+        assert np.array_equal(fp[off:off + len(fpk)], fpk)
+        off += len(fpk)
+    assert off == sp.S
+
+
+def test_merge_of_one_basis_equals_that_basis(O):
+    """A merge basis of a single part is that basis: same fp, energies, forces
+    and virials as the plain entry (same network weights)."""
+    st = W.ternary_liquid(n=200, seed=13)
+    part = {"type": "spherical_chebyshev", "nmax": 4, "lmax": 4, "rcut": 5.5, "wtype": "default"}
+    md = M.random_merge_model(["Cu", "Zr", "Al"], [part], (12, 12), 72)
+    md = W._normalized(md, st, 100, 72)
+    plain = json.loads(json.dumps(md))
+    for m in plain["models"]:
+        m["basis"] = dict(part)
+    a = O.Oracle(M.load_model(md)).system_efv(st)
+    b = O.Oracle(M.load_model(plain)).system_efv(st)
+    for k in ("energies", "forces", "virials"):
+        assert np.array_equal(a[k], b[k])
+
+
+def test_merge_basis_forces_are_minus_energy_gradient(O):
+    """The merge basis' backward (each part's g summed) against central finite
+    differences of the total energy; pairs between the parts' cutoffs enter
+    only the parts that reach them."""
+    st, md = _merge_model(_MERGE_PARTS, 73)
+    orc = O.Oracle(M.load_model(md))
+    res = orc.system_efv(st)
+    h = 1e-5
+    for i in (0, 17, 61):
+        for c in range(3):
+            xp = st.xyz.copy(); xp[i, c] += h
+            xm = st.xyz.copy(); xm[i, c] -= h
+            ep = orc.system_efv(W.Structure(st.box, xp, st.types))["energy"]
+            em = orc.system_efv(W.Structure(st.box, xm, st.types))["energy"]
+            assert abs(-(ep - em) / (2 * h) - res["forces"][i, c]) < 2e-7 * max(1, abs(res["forces"][i, c]))
+
+
+def test_merge_basis_loader_errors():
+    """MergedBasis.load's refusals: no `basis` key, an empty list, a part of
+    an unsupported type (MergedBasis.java:23-30, 204-229)."""
+    base = M.random_merge_model(["Cu"], [_MERGE_PARTS[0]], (4, 4), 1)
+    for bad, msg in (({"type": "merge"}, "Key `basis`"), ({"type": "merge", "basis": []}, "empty"),
+                     ({"type": "merge", "basis": [{"type": "merge", "basis": []}]}, "Unsupported basis type")):
+        md = json.loads(json.dumps(base))
+        md["models"][0]["basis"] = bad
+        with pytest.raises(ValueError, match=msg):
+            M.load_model(md)
