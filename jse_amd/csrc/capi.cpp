@@ -90,8 +90,18 @@ void jse_nnap_model_s::upload() {
         for (size_t q = 0; q < wb.size(); ++q) bb[q] = to_bf16(wb[q]);
         d.wb_fwd = dupload(bf);
         d.wb_bwd = dupload(bb);
+        if (!sp.parts.empty()) {  // a merge basis: the parts' mixings and G's per-column factor
+            std::vector<double> l2;
+            for (const SpeciesHost &q : sp.parts) {
+                d.pW.push_back(dupload(std::vector<float>(q.W.begin(), q.W.end())));
+                d.pW64.push_back(dupload(q.W));
+                for (int e = 0; e < q.P * (q.lmax + 1); ++e)
+                    l2.push_back(q.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * (e % (q.lmax + 1)) + 1));
+            }
+            d.l2f = dupload(l2);
+        }
         smax = std::max(smax, S);
-        pmmax = std::max(pmmax, sp.P * sp.M());
+        pmmax = std::max(pmmax, sp.pm());
     }
     std::vector<float> z((size_t)smax, 0.f), o((size_t)smax, 1.f);
     zeros = dupload(z);
@@ -104,6 +114,9 @@ jse_nnap_model_s::~jse_nnap_model_s() {
     for (auto &d : dev) {
         dfree(d.W); dfree(d.W64); dfree(d.mu); dfree(d.sig); dfree(d.mu64); dfree(d.sig64); dfree(d.w_fwd); dfree(d.w_bwd); dfree(d.bias); dfree(d.w_fwd64); dfree(d.w_bwd64); dfree(d.bias64);
         dfree(d.wb_fwd); dfree(d.wb_bwd);
+        for (float *p : d.pW) dfree(p);
+        for (double *p : d.pW64) dfree(p);
+        dfree(d.l2f);
     }
     dfree(zeros);
     dfree(ones);
@@ -115,14 +128,16 @@ namespace jnn {
 void check_model_shapes(const jse_nnap_model_s *m, int precision) {
     const int nt = m->host.ntypes();
     for (int t = 0; t < nt; ++t) {
-        const SpeciesHost &sp = m->host.species[t];
-        const int lm = sp.radial ? kRadialLmax : sp.lmax;
-        if (desc_tiled(lm, sp.K(), sp.P, sp.mix)) continue;
-        const std::string why = desc_gen_limit(lm, sp.K(), sp.P, nt, precision_acc(precision));
-        JNN_CHECK(why.empty(), ERR_UNSUPPORTED,
-                  "species " + std::to_string(t + 1) + " (lmax=" + std::to_string(sp.lmax) + " nmax=" +
-                      std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P) + ", " + std::to_string(nt) +
-                      " species): " + why);
+        for (int k = 0; k < m->host.species[t].nbases(); ++k) {
+            const SpeciesHost &sp = m->host.species[t].basis(k);
+            const int lm = sp.radial ? kRadialLmax : sp.lmax;
+            if (desc_tiled(lm, sp.K(), sp.P, sp.mix)) continue;
+            const std::string why = desc_gen_limit(lm, sp.K(), sp.P, nt, precision_acc(precision));
+            JNN_CHECK(why.empty(), ERR_UNSUPPORTED,
+                      "species " + std::to_string(t + 1) + " (lmax=" + std::to_string(sp.lmax) + " nmax=" +
+                          std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P) + ", " + std::to_string(nt) +
+                          " species): " + why);
+        }
     }
 }
 SegmentArgs species_segment(const jse_nnap_model_s *m, int t, int precision, const int *types) {
@@ -143,6 +158,8 @@ SegmentArgs species_segment(const jse_nnap_model_s *m, int t, int precision, con
     a.sig64 = dv.sig64;
     a.acc = precision_acc(precision) ? 1 : 0;
     a.fp32_acc = a.acc ? 0 : 1;
+    a.xs = sp.S();
+    a.accum_g = 0;
     return a;
 }
 MlpArgs mlp_args(const jse_nnap_model_s *m, int t, const SegmentArgs &sa, double *E, bool grad, int wg_per_cu) {
@@ -172,6 +189,7 @@ MlpArgs mlp_args(const jse_nnap_model_s *m, int t, const SegmentArgs &sa, double
     a.G = sa.G;
     a.want_grad = grad;
     a.wg_per_cu = wg_per_cu;
+    a.l2f = dv.l2f;
     return a;
 }
 void set_segment_x(SegmentArgs &a, float *xbuf, size_t row0, int smax) {
@@ -191,9 +209,10 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
         return a.radial == b.radial && a.lmax == b.lmax && a.K() == b.K() && a.P == b.P && a.mix == b.mix;
     };
     int first = -1;
-    bool one = nt <= kSegMax;
+    bool one = nt <= kSegMax, merged = false;
     for (int t = 0; t < nt; ++t) {
         if (segs[t].nrows <= 0) continue;
+        if (!m->host.species[t].parts.empty()) merged = true;
         if (first < 0) first = t;
         else if (!same_shape(t, first)) one = false;
     }
@@ -204,6 +223,44 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
                                          "kernel for lmax=" + std::to_string(sp.lmax) + " nmax=" +
                                          std::to_string(sp.nmax) + " size_np=" + std::to_string(sp.P));
     };
+    if (merged) {
+        // a merge basis (MergedBasis.java): one launch per species and part; part k
+        // writes its slice of the species' x row (x, mu, sigma and G advanced by the
+        // sizes of parts 0..k-1), keeps its own A, and its backward adds into g
+        for (int t = 0; t < nt; ++t) {
+            if (segs[t].nrows <= 0) continue;
+            const SpeciesHost &sp = m->host.species[t];
+            const SpeciesDev &dv = m->dev[t];
+            int xoff = 0;
+            size_t aoff = 0;
+            for (int k = 0; k < sp.nbases(); ++k) {
+                const SpeciesHost &b = sp.basis(k);
+                SegmentArgs a = segs[t];
+                if (!sp.parts.empty()) {
+                    a.W = dv.pW[k];
+                    a.W64 = dv.pW64[k];
+                    a.rc = (float)b.rcut;
+                    a.rc64 = b.rcut;
+                    if (a.x) a.x += xoff;
+                    if (a.x64) a.x64 += xoff;
+                    a.mu += xoff;
+                    a.sig += xoff;
+                    a.mu64 += xoff;
+                    a.sig64 += xoff;
+                    a.G += xoff;
+                    a.A += aoff;
+                    a.accum_g = k > 0 ? 1 : 0;
+                }
+                const int lm = b.radial ? kRadialLmax : b.lmax;
+                const bool ok = backward ? launch_desc_bwd(s, lm, b.K(), b.P, b.mix, a, nl)
+                                         : launch_desc_fwd(s, lm, b.K(), b.P, b.mix, a, nl);
+                if (!ok) fail(t);
+                xoff += b.P * (b.lmax + 1);
+                aoff += (size_t)segs[t].nrows * b.P * b.M();
+            }
+        }
+        return;
+    }
     if (one) {
         SegmentSet set{};
         int rows = 0;

@@ -104,6 +104,10 @@ struct SpeciesDev {
     float *w_fwd = nullptr, *w_bwd = nullptr, *bias = nullptr;
     double *w_fwd64 = nullptr, *w_bwd64 = nullptr, *bias64 = nullptr;
     uint16_t *wb_fwd = nullptr, *wb_bwd = nullptr;
+    // a merge basis: each part's mixing (FP32, FP64) and the MLP's per-column G factor
+    std::vector<float *> pW;
+    std::vector<double *> pW64;
+    double *l2f = nullptr;
 };
 
 }  // namespace jnn

@@ -55,11 +55,30 @@ struct SpeciesHost {
     std::vector<double> norm;         // [muE+refE, sigmaE, mu[S], sigma[S]]
     int K() const { return nmax + 1; }
     int M() const { return (lmax + 1) * (lmax + 1); }
-    int S() const { return P * (lmax + 1); }
     int mix = MIX_DENSE;              // kernel specialisation for W
     int mirror = 0;                   // mirror basis: for this species' centres, neighbour species
                                       // `mirror` and this species swap roles (0: none)
     bool radial = false;              // radial-only chebyshev basis: fp[p] = sum_j fc Rnp[p] (lmax 0 sizes)
+    // merge basis (MergedBasis.java): the parts in fp order, each with the basis
+    // fields above; the species' own basis fields then hold part 0 and rcut the
+    // largest part cutoff.  Empty for a single basis.
+    std::vector<SpeciesHost> parts;
+    // descriptor size (all parts of a merge basis)
+    int S() const {
+        if (parts.empty()) return P * (lmax + 1);
+        int s = 0;
+        for (const SpeciesHost &q : parts) s += q.P * (q.lmax + 1);
+        return s;
+    }
+    // the bases evaluated for this species, in fp order
+    int nbases() const { return parts.empty() ? 1 : (int)parts.size(); }
+    const SpeciesHost &basis(int k) const { return parts.empty() ? *this : parts[k]; }
+    // A floats per row, all bases
+    int pm() const {
+        int s = 0;
+        for (int k = 0; k < nbases(); ++k) s += basis(k).P * basis(k).M();
+        return s;
+    }
 };
 
 struct ModelHost {

@@ -226,7 +226,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd_acc(SegmentSet set, NeighborLis
                     const double v = sA[p * M + m];
                     s += v * v;
                 }
-                store_x(a, (size_t)row * S + e, ((ew[q] * s - emu[q]) * eis[q]));
+                store_x(a, (size_t)row * a.xs + e, ((ew[q] * s - emu[q]) * eis[q]));
             }
         }
     }
@@ -397,7 +397,7 @@ __global__ __launch_bounds__(64) void k_desc_fwd_mf(SegmentSet set, NeighborList
                     const double v = sA[p * M + m];
                     s += v * v;
                 }
-                store_x(a, (size_t)row * S + e, ((ew[q] * s - emu[q]) * eis[q]));
+                store_x(a, (size_t)row * a.xs + e, ((ew[q] * s - emu[q]) * eis[q]));
             }
         }
     }
@@ -434,6 +434,7 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
     const int lane = threadIdx.x;
     const int nblk = bwd_blocks<MIX>(set.a[0].ntypes);
     const int total = set.row0[set.nseg];
+    const int accum = set.a[0].accum_g;  // add into g (a merge basis' later parts; one segment then)
     // The next row's Gc and A inputs of the dE/dA fold (mixing without weights or the
     // default mixing) are loaded while this row's pairs are worked on: pg[it][h], pa[it][h]
     // for slot e = lane + 64 it and half h (p = q or K + q).
@@ -448,7 +449,7 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
         const SegmentArgs &b = set.a[s2];
         const int row2 = gr2 - set.row0[s2];
         const float *__restrict__ A2 = b.A + (size_t)row2 * (P * M);
-        const float *__restrict__ G2 = b.G + (size_t)row2 * S;
+        const float *__restrict__ G2 = b.G + (size_t)row2 * b.xs;
 #pragma unroll
         for (int it = 0; it < (PF ? NIT : 1); ++it) {
             const int e = lane + 64 * it;
@@ -526,7 +527,7 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
             const SegmentArgs &a = set.a[sg];
             const int row = r.gr - set.row0[sg];
             const float *__restrict__ Arow = a.A + (size_t)row * (P * M);
-            const float *__restrict__ Grow = a.G + (size_t)row * S;
+            const float *__restrict__ Grow = a.G + (size_t)row * a.xs;
             for (int e = lane; e < K * M; e += 64) {
                 const int q = e / M, m = e % M;
                 const int l = (int)__builtin_amdgcn_sqrtf((float)m + 0.5f);  // l(m) = floor(sqrt(m))
@@ -681,7 +682,7 @@ __global__ __launch_bounds__(64, 2) void k_desc_bwd(SegmentSet set, NeighborList
                     oy = (float)ty;
                     oz = (float)tz;
                 }
-                nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
+                put_g(nl, base + j, ox, oy, oz, accum);
             }
         }
         kA += cntA;
@@ -787,7 +788,7 @@ __global__ __launch_bounds__(64) void k_rad_fwd(SegmentSet set, NeighborList nl)
 #pragma unroll
         for (int p = 0; p < P; ++p) {
             const double fp = wave_sum_f64(acc[p]);
-            if (lane == p) store_x(a, (size_t)row * P + p, ((fp - a.mu64[p]) * (1.0 / a.sig64[p])));
+            if (lane == p) store_x(a, (size_t)row * a.xs + p, ((fp - a.mu64[p]) * (1.0 / a.sig64[p])));
         }
     }
 }
@@ -813,7 +814,7 @@ __global__ __launch_bounds__(64) void k_rad_bwd(SegmentSet set, NeighborList nl)
         else W = a.W;
         GT gp[P];
 #pragma unroll
-        for (int p = 0; p < P; ++p) gp[p] = (GT)a.G[(size_t)row * P + p];
+        for (int p = 0; p < P; ++p) gp[p] = (GT)a.G[(size_t)row * a.xs + p];
         for (int j = lane; j < n; j += 64) {
             const float4 dd = nl.d[base + j];
             const int tj = role_species(tag_species(dd.w), a.self_t, a.mirror);
@@ -867,7 +868,7 @@ __global__ __launch_bounds__(64) void k_rad_bwd(SegmentSet set, NeighborList nl)
                 oy = (float)(-dEdr * dy * ir);
                 oz = (float)(-dEdr * dz * ir);
             }
-            nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
+            put_g(nl, base + j, ox, oy, oz, a.accum_g);
         }
     }
 }

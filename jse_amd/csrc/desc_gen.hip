@@ -178,7 +178,7 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
                 s += v * v;
             }
             const double w4 = 4.0 * 3.14159265358979323846 / (double)(2 * l + 1);
-            store_x(a, (size_t)row * S + e, ((w4 * s - a.mu64[e]) * (1.0 / a.sig64[e])));
+            store_x(a, (size_t)row * a.xs + e, ((w4 * s - a.mu64[e]) * (1.0 / a.sig64[e])));
         }
     }
 }
@@ -218,7 +218,7 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
             const int p = e / MP, m = e - p * MP;
             int l = 0;
             while ((l + 1) * (l + 1) <= m) ++l;
-            sGA[e] = m < M ? a.G[(size_t)row * S + p * L1 + l] * a.A[(size_t)row * PM + p * M + m] : 0.f;
+            sGA[e] = m < M ? a.G[(size_t)row * a.xs + p * L1 + l] * a.A[(size_t)row * PM + p * M + m] : 0.f;
         }
         __syncthreads();
         for (int e = lane; e < nt * K * MP; e += 64) {
@@ -327,7 +327,7 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
                 oy = (float)ty;
                 oz = (float)tz;
             }
-            nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
+            put_g(nl, base + j, ox, oy, oz, a.accum_g);
         }
     }
 }
@@ -380,9 +380,9 @@ __global__ __launch_bounds__(64) void k_gen_rad_fwd(SegmentSet set, NeighborList
                 if (lane + 64 < P) acc1 += (double)sC[j * P + lane + 64];
             }
         }
-        if (lane < P) store_x(a, (size_t)row * P + lane, ((acc0 - a.mu64[lane]) * (1.0 / a.sig64[lane])));
+        if (lane < P) store_x(a, (size_t)row * a.xs + lane, ((acc0 - a.mu64[lane]) * (1.0 / a.sig64[lane])));
         if (lane + 64 < P)
-            store_x(a, (size_t)row * P + lane + 64, ((acc1 - a.mu64[lane + 64]) * (1.0 / a.sig64[lane + 64])));
+            store_x(a, (size_t)row * a.xs + lane + 64, ((acc1 - a.mu64[lane + 64]) * (1.0 / a.sig64[lane + 64])));
     }
 }
 
@@ -411,7 +411,7 @@ __global__ __launch_bounds__(64) void k_gen_rad_bwd(SegmentSet set, NeighborList
         const int n = min(nl.cnt[i], nl.cap);
         const size_t base = (size_t)i * nl.cap;
         __syncthreads();
-        for (int p = lane; p < P; p += 64) sG[p] = a.G[(size_t)row * P + p];
+        for (int p = lane; p < P; p += 64) sG[p] = a.G[(size_t)row * a.xs + p];
         __syncthreads();
         for (int j = lane; j < n; j += 64) {
             const float4 dd = nl.d[base + j];
@@ -456,7 +456,7 @@ __global__ __launch_bounds__(64) void k_gen_rad_bwd(SegmentSet set, NeighborList
                 oy = (float)(-dEdr * dy * ir);
                 oz = (float)(-dEdr * dz * ir);
             }
-            nl.g[base + j] = make_float4(ox, oy, oz, 0.f);
+            put_g(nl, base + j, ox, oy, oz, a.accum_g);
         }
     }
 }

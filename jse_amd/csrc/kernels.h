@@ -161,7 +161,20 @@ struct SegmentArgs {
     const double *W64;      // [ntypes][P][K] FP64 mixing (the plain mode's forward)
     int self_t;             // this segment's species (1-based)
     int mirror;             // mirror basis: neighbours of species `mirror` and of self_t swap roles (0: none)
+    int xs;                 // row stride of x and G: the species' descriptor size (a merge basis' part
+                            // writes its slice of the species' row; x, G, mu and sigma then point at it)
+    int accum_g;            // backward: add into the pairs' g (the parts of a merge basis after the first)
 };
+// g of a pair: stored, or added to what an earlier part of a merge basis stored
+__device__ __forceinline__ void put_g(const NeighborList &nl, size_t k, float gx, float gy, float gz, int accum) {
+    if (accum) {
+        const float4 o = nl.g[k];
+        gx += o.x;
+        gy += o.y;
+        gz += o.z;
+    }
+    nl.g[k] = make_float4(gx, gy, gz, 0.f);
+}
 // Stores a normalised descriptor value: FP64 rows in the plain mode, else FP32.
 __device__ __forceinline__ void store_x(const SegmentArgs &a, size_t idx, double v) {
     if (a.x64) a.x64[idx] = v;
@@ -220,6 +233,8 @@ struct MlpArgs {
     int want_grad;
     int wg_per_cu;           // persistent grid size (0: 2 per CU)
     int radial;              // radial-only basis: G = dE/dfp / sigma (no L2 power-spectrum factor)
+    const double *l2f;       // [S] per-column factor of G (a merge basis: 2 4pi/(2l+1) per spherical
+                             // part's column, 1 per radial part's); nullptr: from L1 and radial
 };
 // Any layer count and widths; false when a 32-row tile of the input and the
 // hidden activations (and their SiLU derivatives) exceeds 160 KB of LDS:

@@ -163,10 +163,10 @@ __global__ __launch_bounds__(256) void k_mlp_vec(MlpArgs a) {
                     const int l = e % a.L1;
                     // times 2 4pi/(2l+1) for the L2 power spectrum's backward (not for a radial basis)
                     if constexpr (sizeof(T) == 8) {
-                        const double l2 = a.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * l + 1);
+                        const double l2 = a.l2f ? a.l2f[e] : (a.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * l + 1));
                         a.G[(size_t)(row0 + r) * S + e] = (float)(smem[e * TR + r] / a.sig64[e] * l2);
                     } else {
-                        const float l2 = a.radial ? 1.f : 2.f * kPi4f / (float)(2 * l + 1);
+                        const float l2 = a.l2f ? (float)a.l2f[e] : (a.radial ? 1.f : 2.f * kPi4f / (float)(2 * l + 1));
                         a.G[(size_t)(row0 + r) * S + e] = smem[e * TR + r] / a.sig[e] * l2;
                     }
                 }

@@ -294,10 +294,12 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
         const int n = 16 * cs + lr;
         // G = dE/dx / sigma, times 2 4pi/(2l+1) for the L2 power spectrum's backward
         if constexpr (F64) {
-            const double l2 = a.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * (n % a.L1) + 1);
+            const double l2 = a.l2f ? (n < S ? a.l2f[n] : 0.0)
+                                    : (a.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * (n % a.L1) + 1));
             gscale[cs] = n < S ? l2 / a.sig64[n] : 0.0;
         } else {
-            const float l2 = a.radial ? 1.f : 2.f * kPi4m / (float)(2 * (n % a.L1) + 1);
+            const float l2 = a.l2f ? (n < S ? (float)a.l2f[n] : 0.f)
+                                   : (a.radial ? 1.f : 2.f * kPi4m / (float)(2 * (n % a.L1) + 1));
             gscale[cs] = n < S ? l2 / a.sig[n] : 0.f;
         }
     }
@@ -474,7 +476,8 @@ __device__ __forceinline__ void mlp2_tiles(const MlpArgs &a, int lb, int lg) {
                     for (int u = 0; u < 8; ++u) acc = MM::mma(hrow + kb + u * MM::KSTEP, &wv[u], acc);
                 }
                 if (n < S) {
-                    const CT scale = (a.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * (n % a.L1) + 1)) /
+                    const CT scale = (a.l2f ? a.l2f[n]
+                                            : (a.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * (n % a.L1) + 1))) /
                                      a.sig64[n];
 #pragma unroll
                     for (int r = 0; r < 4; ++r) {

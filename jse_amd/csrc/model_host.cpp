@@ -157,6 +157,190 @@ int size_n(int wtype, int ntypes, int nmax, int fuse) {  // WTypeBasis.java:331-
 
 }  // namespace
 
+// the single-basis fields of `dst` from `src` (a mirror / share species, or a
+// merge species' part 0)
+static void set_basis_fields(SpeciesHost &dst, const SpeciesHost &src) {
+    dst.nmax = src.nmax;
+    dst.lmax = src.lmax;
+    dst.P = src.P;
+    dst.wtype = src.wtype;
+    dst.typed_wtype = src.typed_wtype;
+    dst.rcut = src.rcut;
+    dst.W = src.W;
+    dst.radial = src.radial;
+}
+
+// SphericalChebyshev.load / Chebyshev.load (SphericalChebyshev.java:60-103,
+// Chebyshev.java) with the WTypeBasis mixing (WTypeBasis.java:202-354): one
+// basis entry `b` of type `type` into the basis fields of `sp`.
+static void parse_basis(const JVal *b, const std::string &type, int ntypes, SpeciesHost &sp) {
+    // Chebyshev.java: the radial-only basis (no angular part)
+    sp.radial = type == "chebyshev" || type == "cheby";
+    JNN_CHECK(sp.radial || type == "spherical_chebyshev" || type == "sph_cheby", ERR_UNSUPPORTED,
+              "Unsupported basis type: " + type);
+    sp.nmax = (int)num(b->get("nmax"), 5);
+    sp.lmax = sp.radial ? 0 : (int)num(b->get("lmax"), 6);
+    JNN_CHECK(sp.lmax >= 0 && sp.lmax <= 12, ERR_ARG, "Input lmax MUST be in [0, 12], input: " + std::to_string(sp.lmax));
+    JNN_CHECK(sp.nmax >= 0 && sp.nmax <= 20, ERR_ARG,  // WTypeBasis.java:56
+              "Input nmax MUST be in [0, 20], input: " + std::to_string(sp.nmax));
+    JNN_CHECK(sp.radial || ((int)num(b->get("l3max"), 0) == 0 && (int)num(b->get("l4max"), 0) == 0), ERR_UNSUPPORTED,
+              "l3max/l4max > 0 are not supported by this build");
+    sp.rcut = num(b->get("rcut"), 6.0);
+    JNN_CHECK(sp.rcut > 0, ERR_ARG, "rcut must be positive");
+    int wtype = WT_DEFAULT;
+    if (auto w = b->get("wtype")) {
+        if (w->kind == JVal::NUM) wtype = (int)w->num;
+        else {
+            static const std::map<std::string, int> names = {{"default", WT_DEFAULT}, {"none", WT_NONE},
+                                                             {"full", WT_FULL},       {"exfull", WT_EXFULL},
+                                                             {"fuse", WT_FUSE},       {"rfuse", WT_RFUSE},
+                                                             {"exfuse", WT_EXFUSE}};
+            auto it = names.find(w->str);
+            JNN_CHECK(it != names.end(), ERR_ARG, "Input wtype MUST be in {-1, 0, 2, 3, 4, 5, 6}, input: " + w->str);
+            wtype = it->second;
+        }
+    }
+    sp.wtype = wtype;
+    JNN_CHECK(wtype == WT_DEFAULT || wtype == WT_NONE || (wtype >= WT_FULL && wtype <= WT_EXFUSE), ERR_ARG,
+              "Input wtype MUST be in {-1, 0, 2, 3, 4, 5, 6}, input: " + std::to_string(wtype));
+    // the fuse weight is read for the fuse wtypes only and the rfuse weight for
+    // rfuse only; other wtypes ignore those keys (WTypeBasis.java:365-410)
+    std::vector<double> fuse_w, rfuse_w;
+    if (wtype == WT_FUSE || wtype == WT_EXFUSE) {
+        const JVal *fw = b->get("fuse_weight");
+        const JVal *fs = b->get("fuse_size");
+        if (fw) {
+            JNN_CHECK(fw->kind == JVal::ARR && !fw->arr.empty(), ERR_ARG, "model JSON: fuse_weight must be a non-empty array");
+            const bool nested = fw->arr[0].kind == JVal::ARR;
+            if (!fs && nested) {
+                // an older file: the inner lists are the columns of the [ntypes][fuse] matrix
+                const size_t ncol = fw->arr.size(), nrow = fw->arr[0].arr.size();
+                fuse_w.assign(nrow * ncol, 0.0);
+                for (size_t c = 0; c < ncol; ++c) {
+                    std::vector<double> col = vec(&fw->arr[c]);
+                    JNN_CHECK(col.size() == nrow, ERR_ARG, "Size of fuse weight mismatch");
+                    for (size_t r = 0; r < nrow; ++r) fuse_w[r * ncol + c] = col[r];
+                }
+            } else if (nested) {
+                for (auto &row : fw->arr) {
+                    std::vector<double> rr = vec(&row);
+                    JNN_CHECK((int)rr.size() == (int)num(fs, 0), ERR_ARG, "Column number of fuse weight mismatch");
+                    fuse_w.insert(fuse_w.end(), rr.begin(), rr.end());
+                }
+            } else {
+                fuse_w = vec(fw);
+                if (fs) JNN_CHECK((long)fuse_w.size() == (long)num(fs, 0) * ntypes, ERR_ARG, "Size of fuse weight mismatch");
+            }
+        } else {
+            JNN_CHECK(fs, ERR_ARG, "Key `fuse_weight` or `fuse_size` required for fuse wtype");
+            fuse_w.assign((size_t)num(fs, 0) * ntypes, 0.0);
+        }
+        JNN_CHECK(!fuse_w.empty() && fuse_w.size() % ntypes == 0, ERR_ARG, "Size of fuse weight mismatch");
+    }
+    const int fuse0 = (wtype == WT_FUSE || wtype == WT_EXFUSE) ? (int)fuse_w.size() / ntypes : 0;
+    if (wtype == WT_RFUSE) {
+        JNN_CHECK(!b->get("post_fuse_size") && !b->get("post_fuse_weight"), ERR_ARG,
+                  "Key `post_fuse_size` or `post_fuse_weight` is invalid for wtype=='rfuse'");
+        const int sn0 = size_n(wtype, ntypes, sp.nmax, 0);
+        const JVal *rs = b->get("rfuse_size");
+        if (const JVal *rw = b->get("rfuse_weight")) {
+            rfuse_w = vec(rw);
+            if (rs) JNN_CHECK((long)rfuse_w.size() == (long)num(rs, 0) * sn0, ERR_ARG, "Size of rfuse weight mismatch");
+        } else {
+            JNN_CHECK(rs, ERR_ARG, "Key `rfuse_weight` or `rfuse_size` required for wtype=='rfuse'");
+            rfuse_w.assign((size_t)num(rs, 0) * sn0, 0.0);
+        }
+        JNN_CHECK(!rfuse_w.empty() && rfuse_w.size() % sn0 == 0, ERR_ARG, "Size of rfuse weight mismatch");
+    }
+    double rfuse_scale = num(b->get("rfuse_scale"), 1.0);
+    bool ws = num(first(*b, {"weight_standardization", "ws"}), 0.0) != 0.0;
+    const int K = sp.K();
+    // base mixing W[t][q][n] of a non-rfuse wtype (sizeN rows), WTypeBasis.java:202-304
+    auto base_mixing = [&](int typed_, int sn_, int fuse_) {
+        std::vector<double> Wb((size_t)ntypes * sn_ * K, 0.0);
+        auto at = [&](int t, int q, int n) -> double & { return Wb[((size_t)(t - 1) * sn_ + q) * K + n]; };
+        for (int t = 1; t <= ntypes; ++t)
+            for (int n = 0; n < K; ++n) {
+                switch (typed_) {
+                    case WT_NONE: at(t, n, n) = 1.0; break;
+                    case WT_DEFAULT: at(t, n, n) = 1.0; at(t, K + n, n) = (t & 1) ? (double)t : -(double)t; break;
+                    case WT_FULL: at(t, (t - 1) * K + n, n) = 1.0; break;
+                    case WT_EXFULL: at(t, n, n) = 1.0; at(t, t * K + n, n) = 1.0; break;
+                    case WT_FUSE:
+                        for (int k = 0; k < fuse_; ++k) at(t, k * K + n, n) = fuse_w[(size_t)(t - 1) * fuse_ + k];
+                        break;
+                    case WT_EXFUSE:
+                        at(t, n, n) = 1.0;
+                        for (int k = 0; k < fuse_; ++k) at(t, (k + 1) * K + n, n) = fuse_w[(size_t)(t - 1) * fuse_ + k];
+                        break;
+                    default: throw Error(ERR_ARG, "unsupported wtype");
+                }
+            }
+        return Wb;
+    };
+    auto typed_of = [&](int wt_) {
+        return (ntypes == 1 && (wt_ == WT_EXFULL || wt_ == WT_FULL || wt_ == WT_NONE || wt_ == WT_DEFAULT)) ? WT_NONE
+                                                                                                        : wt_;
+    };
+    // an older file's post-fuse weight becomes the rfuse weight it equals, rfuse[t] = post . W_base[t]
+    // (SphericalChebyshev.java:81-94; WTypeBasis.getPostFuseWeight_ / postFuse2RFuse_, :391-540)
+    const bool post_keys = b->get("post_fuse_size") || b->get("post_fuse_weight");
+    if (wtype == WT_RFUSE) {
+        JNN_CHECK(!post_keys, ERR_ARG, "Key `post_fuse_size` or `post_fuse_weight` is invalid for wtype=='rfuse'");
+    } else {
+        const JVal *pf = b->get("post_fuse");
+        const bool off = pf && pf->kind == JVal::BOOL && !pf->b;
+        if (!off && (pf || post_keys)) {
+            const std::vector<double> post = vec(b->get("post_fuse_weight"));
+            JNN_CHECK(!post.empty(), ERR_ARG,
+                      "Key `post_fuse_weight` always required for post_fuse, training with post_fuse is invalid now");
+            const int fz = fuse0;
+            const int sn0 = size_n(wtype, ntypes, sp.nmax, fz);
+            if (auto ps = b->get("post_fuse_size"))
+                JNN_CHECK(post.size() == (size_t)num(ps, 0) * sn0, ERR_ARG, "Size of post fuse weight mismatch");
+            const int pp = (int)post.size() / sn0;
+            const std::vector<double> Wb = base_mixing(typed_of(wtype), sn0, fz);
+            rfuse_w.assign((size_t)ntypes * pp * K, 0.0);
+            for (int t = 0; t < ntypes; ++t)
+                for (int p = 0; p < pp; ++p)
+                    for (int q = 0; q < sn0; ++q) {
+                        const double w = post[(size_t)p * sn0 + q];
+                        if (w == 0.0) continue;
+                        for (int n = 0; n < K; ++n)
+                            rfuse_w[((size_t)t * pp + p) * K + n] += w * Wb[((size_t)t * sn0 + q) * K + n];
+                    }
+            rfuse_scale = num(b->get("post_fuse_scale"), 1.0);
+            wtype = WT_RFUSE;
+            sp.wtype = wtype;
+            fuse_w.clear();
+        }
+    }
+    int typed = typed_of(wtype);
+    sp.typed_wtype = typed;
+    const int fuse = (wtype == WT_FUSE || wtype == WT_EXFUSE) ? (int)fuse_w.size() / ntypes : 0;
+    const int sn = size_n(wtype, ntypes, sp.nmax, fuse);
+    sp.P = rfuse_w.empty() ? sn : (int)rfuse_w.size() / sn;
+    JNN_CHECK(sp.P >= 1 && sp.P <= 128, ERR_ARG, "basis size_np out of range");
+    sp.W.assign((size_t)ntypes * sp.P * K, 0.0);
+    if (!rfuse_w.empty()) {
+        JNN_CHECK(rfuse_w.size() == (size_t)sp.P * sn, ERR_ARG, "Size of rfuse weight mismatch");
+        std::vector<double> rw = rfuse_w;
+        if (ws) {
+            for (int p = 0; p < sp.P; ++p) {
+                double norm = kWsEps;
+                for (int t = 1; t <= ntypes; ++t)
+                    for (int n = 0; n < K; ++n) norm += std::fabs(rfuse_w[((size_t)(t - 1) * sp.P + p) * K + n]);
+                norm /= sn;
+                for (int t = 1; t <= ntypes; ++t)
+                    for (int n = 0; n < K; ++n) rw[((size_t)(t - 1) * sp.P + p) * K + n] /= norm;
+            }
+        }
+        for (size_t q = 0; q < rw.size(); ++q) sp.W[q] = rfuse_scale * rw[q];
+    } else {
+        sp.W = base_mixing(typed, sn, fuse);
+    }
+}
+
 ModelHost parse_model_json(const std::string &text) {
     JParser jp(text);
     JVal root = jp.value();
@@ -190,6 +374,7 @@ ModelHost parse_model_json(const std::string &text) {
         dflt_basis.kind = JVal::OBJ;
         const JVal *b = m.get("basis");
         if (!b) b = &dflt_basis;
+        std::string btype = "spherical_chebyshev";
         if (auto t = b->get("type")) {
             JNN_CHECK(t->kind == JVal::STR, ERR_UNSUPPORTED, "Unsupported basis type: ?");
             const bool mir = t->str == "mirror" || t->str == "mirror_basis";
@@ -203,172 +388,29 @@ ModelHost parse_model_json(const std::string &text) {
                 is_mirror[i] = mir;
                 continue;
             }
-            // Chebyshev.java: the radial-only basis (no angular part)
-            sp.radial = t->str == "chebyshev" || t->str == "cheby";
-            JNN_CHECK(sp.radial || t->str == "spherical_chebyshev" || t->str == "sph_cheby", ERR_UNSUPPORTED,
-                      "Unsupported basis type: " + t->str);
+            btype = t->str;
         }
-        sp.nmax = (int)num(b->get("nmax"), 5);
-        sp.lmax = sp.radial ? 0 : (int)num(b->get("lmax"), 6);
-        JNN_CHECK(sp.lmax >= 0 && sp.lmax <= 12, ERR_ARG, "Input lmax MUST be in [0, 12], input: " + std::to_string(sp.lmax));
-        JNN_CHECK(sp.nmax >= 0 && sp.nmax <= 20, ERR_ARG,  // WTypeBasis.java:56
-                  "Input nmax MUST be in [0, 20], input: " + std::to_string(sp.nmax));
-        JNN_CHECK(sp.radial || ((int)num(b->get("l3max"), 0) == 0 && (int)num(b->get("l4max"), 0) == 0), ERR_UNSUPPORTED,
-                  "l3max/l4max > 0 are not supported by this build");
-        sp.rcut = num(b->get("rcut"), 6.0);
-        JNN_CHECK(sp.rcut > 0, ERR_ARG, "rcut must be positive");
-        int wtype = WT_DEFAULT;
-        if (auto w = b->get("wtype")) {
-            if (w->kind == JVal::NUM) wtype = (int)w->num;
-            else {
-                static const std::map<std::string, int> names = {{"default", WT_DEFAULT}, {"none", WT_NONE},
-                                                                 {"full", WT_FULL},       {"exfull", WT_EXFULL},
-                                                                 {"fuse", WT_FUSE},       {"rfuse", WT_RFUSE},
-                                                                 {"exfuse", WT_EXFUSE}};
-                auto it = names.find(w->str);
-                JNN_CHECK(it != names.end(), ERR_ARG, "Input wtype MUST be in {-1, 0, 2, 3, 4, 5, 6}, input: " + w->str);
-                wtype = it->second;
-            }
-        }
-        sp.wtype = wtype;
-        JNN_CHECK(wtype == WT_DEFAULT || wtype == WT_NONE || (wtype >= WT_FULL && wtype <= WT_EXFUSE), ERR_ARG,
-                  "Input wtype MUST be in {-1, 0, 2, 3, 4, 5, 6}, input: " + std::to_string(wtype));
-        // the fuse weight is read for the fuse wtypes only and the rfuse weight for
-        // rfuse only; other wtypes ignore those keys (WTypeBasis.java:365-410)
-        std::vector<double> fuse_w, rfuse_w;
-        if (wtype == WT_FUSE || wtype == WT_EXFUSE) {
-            const JVal *fw = b->get("fuse_weight");
-            const JVal *fs = b->get("fuse_size");
-            if (fw) {
-                JNN_CHECK(fw->kind == JVal::ARR && !fw->arr.empty(), ERR_ARG, "model JSON: fuse_weight must be a non-empty array");
-                const bool nested = fw->arr[0].kind == JVal::ARR;
-                if (!fs && nested) {
-                    // an older file: the inner lists are the columns of the [ntypes][fuse] matrix
-                    const size_t ncol = fw->arr.size(), nrow = fw->arr[0].arr.size();
-                    fuse_w.assign(nrow * ncol, 0.0);
-                    for (size_t c = 0; c < ncol; ++c) {
-                        std::vector<double> col = vec(&fw->arr[c]);
-                        JNN_CHECK(col.size() == nrow, ERR_ARG, "Size of fuse weight mismatch");
-                        for (size_t r = 0; r < nrow; ++r) fuse_w[r * ncol + c] = col[r];
-                    }
-                } else if (nested) {
-                    for (auto &row : fw->arr) {
-                        std::vector<double> rr = vec(&row);
-                        JNN_CHECK((int)rr.size() == (int)num(fs, 0), ERR_ARG, "Column number of fuse weight mismatch");
-                        fuse_w.insert(fuse_w.end(), rr.begin(), rr.end());
-                    }
-                } else {
-                    fuse_w = vec(fw);
-                    if (fs) JNN_CHECK((long)fuse_w.size() == (long)num(fs, 0) * ntypes, ERR_ARG, "Size of fuse weight mismatch");
+        if (btype == "merge") {  // MergedBasis.load (MergedBasis.java:204-229)
+            const JVal *lst = b->get("basis");
+            JNN_CHECK(lst, ERR_ARG, "Key `basis` required for merge load");
+            JNN_CHECK(lst->kind == JVal::ARR && !lst->arr.empty(), ERR_ARG, "Merge basis can not be null or empty");
+            for (const JVal &pb : lst->arr) {
+                JNN_CHECK(pb.kind == JVal::OBJ, ERR_ARG, "model JSON: a merge part must be an object");
+                std::string pt = "spherical_chebyshev";
+                if (auto t2 = pb.get("type")) {
+                    JNN_CHECK(t2->kind == JVal::STR, ERR_UNSUPPORTED, "Unsupported basis type: ?");
+                    pt = t2->str;
                 }
-            } else {
-                JNN_CHECK(fs, ERR_ARG, "Key `fuse_weight` or `fuse_size` required for fuse wtype");
-                fuse_w.assign((size_t)num(fs, 0) * ntypes, 0.0);
+                SpeciesHost part;
+                parse_basis(&pb, pt, ntypes, part);
+                sp.parts.push_back(std::move(part));
             }
-            JNN_CHECK(!fuse_w.empty() && fuse_w.size() % ntypes == 0, ERR_ARG, "Size of fuse weight mismatch");
+            // the species' single-basis fields hold part 0, rcut the largest (MergedBasis.rcutMax)
+            set_basis_fields(sp, sp.parts[0]);
+            for (const SpeciesHost &q : sp.parts) sp.rcut = std::max(sp.rcut, q.rcut);
+            continue;
         }
-        const int fuse0 = (wtype == WT_FUSE || wtype == WT_EXFUSE) ? (int)fuse_w.size() / ntypes : 0;
-        if (wtype == WT_RFUSE) {
-            JNN_CHECK(!b->get("post_fuse_size") && !b->get("post_fuse_weight"), ERR_ARG,
-                      "Key `post_fuse_size` or `post_fuse_weight` is invalid for wtype=='rfuse'");
-            const int sn0 = size_n(wtype, ntypes, sp.nmax, 0);
-            const JVal *rs = b->get("rfuse_size");
-            if (const JVal *rw = b->get("rfuse_weight")) {
-                rfuse_w = vec(rw);
-                if (rs) JNN_CHECK((long)rfuse_w.size() == (long)num(rs, 0) * sn0, ERR_ARG, "Size of rfuse weight mismatch");
-            } else {
-                JNN_CHECK(rs, ERR_ARG, "Key `rfuse_weight` or `rfuse_size` required for wtype=='rfuse'");
-                rfuse_w.assign((size_t)num(rs, 0) * sn0, 0.0);
-            }
-            JNN_CHECK(!rfuse_w.empty() && rfuse_w.size() % sn0 == 0, ERR_ARG, "Size of rfuse weight mismatch");
-        }
-        double rfuse_scale = num(b->get("rfuse_scale"), 1.0);
-        bool ws = num(first(*b, {"weight_standardization", "ws"}), 0.0) != 0.0;
-        const int K = sp.K();
-        // base mixing W[t][q][n] of a non-rfuse wtype (sizeN rows), WTypeBasis.java:202-304
-        auto base_mixing = [&](int typed_, int sn_, int fuse_) {
-            std::vector<double> Wb((size_t)ntypes * sn_ * K, 0.0);
-            auto at = [&](int t, int q, int n) -> double & { return Wb[((size_t)(t - 1) * sn_ + q) * K + n]; };
-            for (int t = 1; t <= ntypes; ++t)
-                for (int n = 0; n < K; ++n) {
-                    switch (typed_) {
-                        case WT_NONE: at(t, n, n) = 1.0; break;
-                        case WT_DEFAULT: at(t, n, n) = 1.0; at(t, K + n, n) = (t & 1) ? (double)t : -(double)t; break;
-                        case WT_FULL: at(t, (t - 1) * K + n, n) = 1.0; break;
-                        case WT_EXFULL: at(t, n, n) = 1.0; at(t, t * K + n, n) = 1.0; break;
-                        case WT_FUSE:
-                            for (int k = 0; k < fuse_; ++k) at(t, k * K + n, n) = fuse_w[(size_t)(t - 1) * fuse_ + k];
-                            break;
-                        case WT_EXFUSE:
-                            at(t, n, n) = 1.0;
-                            for (int k = 0; k < fuse_; ++k) at(t, (k + 1) * K + n, n) = fuse_w[(size_t)(t - 1) * fuse_ + k];
-                            break;
-                        default: throw Error(ERR_ARG, "unsupported wtype");
-                    }
-                }
-            return Wb;
-        };
-        auto typed_of = [&](int wt_) {
-            return (ntypes == 1 && (wt_ == WT_EXFULL || wt_ == WT_FULL || wt_ == WT_NONE || wt_ == WT_DEFAULT)) ? WT_NONE
-                                                                                                            : wt_;
-        };
-        // an older file's post-fuse weight becomes the rfuse weight it equals, rfuse[t] = post . W_base[t]
-        // (SphericalChebyshev.java:81-94; WTypeBasis.getPostFuseWeight_ / postFuse2RFuse_, :391-540)
-        const bool post_keys = b->get("post_fuse_size") || b->get("post_fuse_weight");
-        if (wtype == WT_RFUSE) {
-            JNN_CHECK(!post_keys, ERR_ARG, "Key `post_fuse_size` or `post_fuse_weight` is invalid for wtype=='rfuse'");
-        } else {
-            const JVal *pf = b->get("post_fuse");
-            const bool off = pf && pf->kind == JVal::BOOL && !pf->b;
-            if (!off && (pf || post_keys)) {
-                const std::vector<double> post = vec(b->get("post_fuse_weight"));
-                JNN_CHECK(!post.empty(), ERR_ARG,
-                          "Key `post_fuse_weight` always required for post_fuse, training with post_fuse is invalid now");
-                const int fz = fuse0;
-                const int sn0 = size_n(wtype, ntypes, sp.nmax, fz);
-                if (auto ps = b->get("post_fuse_size"))
-                    JNN_CHECK(post.size() == (size_t)num(ps, 0) * sn0, ERR_ARG, "Size of post fuse weight mismatch");
-                const int pp = (int)post.size() / sn0;
-                const std::vector<double> Wb = base_mixing(typed_of(wtype), sn0, fz);
-                rfuse_w.assign((size_t)ntypes * pp * K, 0.0);
-                for (int t = 0; t < ntypes; ++t)
-                    for (int p = 0; p < pp; ++p)
-                        for (int q = 0; q < sn0; ++q) {
-                            const double w = post[(size_t)p * sn0 + q];
-                            if (w == 0.0) continue;
-                            for (int n = 0; n < K; ++n)
-                                rfuse_w[((size_t)t * pp + p) * K + n] += w * Wb[((size_t)t * sn0 + q) * K + n];
-                        }
-                rfuse_scale = num(b->get("post_fuse_scale"), 1.0);
-                wtype = WT_RFUSE;
-                sp.wtype = wtype;
-                fuse_w.clear();
-            }
-        }
-        int typed = typed_of(wtype);
-        sp.typed_wtype = typed;
-        const int fuse = (wtype == WT_FUSE || wtype == WT_EXFUSE) ? (int)fuse_w.size() / ntypes : 0;
-        const int sn = size_n(wtype, ntypes, sp.nmax, fuse);
-        sp.P = rfuse_w.empty() ? sn : (int)rfuse_w.size() / sn;
-        JNN_CHECK(sp.P >= 1 && sp.P <= 128, ERR_ARG, "basis size_np out of range");
-        sp.W.assign((size_t)ntypes * sp.P * K, 0.0);
-        if (!rfuse_w.empty()) {
-            JNN_CHECK(rfuse_w.size() == (size_t)sp.P * sn, ERR_ARG, "Size of rfuse weight mismatch");
-            std::vector<double> rw = rfuse_w;
-            if (ws) {
-                for (int p = 0; p < sp.P; ++p) {
-                    double norm = kWsEps;
-                    for (int t = 1; t <= ntypes; ++t)
-                        for (int n = 0; n < K; ++n) norm += std::fabs(rfuse_w[((size_t)(t - 1) * sp.P + p) * K + n]);
-                    norm /= sn;
-                    for (int t = 1; t <= ntypes; ++t)
-                        for (int n = 0; n < K; ++n) rw[((size_t)(t - 1) * sp.P + p) * K + n] /= norm;
-                }
-            }
-            for (size_t q = 0; q < rw.size(); ++q) sp.W[q] = rfuse_scale * rw[q];
-        } else {
-            sp.W = base_mixing(typed, sn, fuse);
-        }
+        parse_basis(b, btype, ntypes, sp);
     }
     // pass 2 (Basis.java:51-66): mirror / share take the named species' basis
     for (int i = 0; i < ntypes; ++i) {
@@ -376,14 +418,8 @@ ModelHost parse_model_json(const std::string &text) {
         JNN_CHECK(!ref[ref[i] - 1], ERR_UNSUPPORTED, "a mirror/share basis of a mirror/share basis is not supported");
         SpeciesHost &sp = out.species[i];
         const SpeciesHost &src = out.species[ref[i] - 1];
-        sp.nmax = src.nmax;
-        sp.lmax = src.lmax;
-        sp.P = src.P;
-        sp.wtype = src.wtype;
-        sp.typed_wtype = src.typed_wtype;
-        sp.rcut = src.rcut;
-        sp.W = src.W;
-        sp.radial = src.radial;
+        set_basis_fields(sp, src);
+        sp.parts = src.parts;
         sp.mirror = is_mirror[i] ? ref[i] : 0;
     }
     // pass 3 (NeuralNetwork.java:24-65): the networks.  feed_forward entries
@@ -578,27 +614,31 @@ void finalize_species(ModelHost &m) {
     // a neighbour's species travels in 4 bits of its pair tag
     JNN_CHECK(nt >= 1 && nt <= kMaxSpecies, ERR_UNSUPPORTED,
               "at most " + std::to_string(kMaxSpecies) + " species are supported, the model has " + std::to_string(nt));
+    // recognise the structured mixings so the kernels can skip the dense product
+    auto mix_of = [&](SpeciesHost &b) {
+        const int K = b.K(), P = b.P;
+        JNN_CHECK((int)b.W.size() == nt * P * K, ERR_ARG, "fp_param size mismatch");
+        bool none = (P == K), def = (P == 2 * K);
+        for (int t = 1; t <= nt; ++t)
+            for (int p = 0; p < P; ++p)
+                for (int n = 0; n < K; ++n) {
+                    double w = b.W[((size_t)(t - 1) * P + p) * K + n];
+                    if (none && w != (p == n ? 1.0 : 0.0)) none = false;
+                    double wt = (t & 1) ? (double)t : -(double)t;
+                    double want = (p == n) ? 1.0 : (p == K + n ? wt : 0.0);
+                    if (def && w != want) def = false;
+                }
+        b.mix = none ? MIX_NONE : (def ? MIX_DEFAULT : MIX_DENSE);
+    };
     for (auto &sp : m.species) {
-        const int K = sp.K(), P = sp.P;
-        JNN_CHECK((int)sp.W.size() == nt * P * K, ERR_ARG, "fp_param size mismatch");
         size_t need = 0;
         int cols = sp.S();
         for (int h : sp.hidden) { need += (size_t)h * cols + h; cols = h; }
         need += (size_t)cols + 1;
         JNN_CHECK(sp.nn.size() == need, ERR_ARG, "nn_param size mismatch");
         JNN_CHECK((int)sp.norm.size() == 2 + 2 * sp.S(), ERR_ARG, "norm_param size mismatch");
-        // recognise the structured mixings so the kernels can skip the dense product
-        bool none = (P == K), def = (P == 2 * K);
-        for (int t = 1; t <= nt; ++t)
-            for (int p = 0; p < P; ++p)
-                for (int n = 0; n < K; ++n) {
-                    double w = sp.W[((size_t)(t - 1) * P + p) * K + n];
-                    if (none && w != (p == n ? 1.0 : 0.0)) none = false;
-                    double wt = (t & 1) ? (double)t : -(double)t;
-                    double want = (p == n) ? 1.0 : (p == K + n ? wt : 0.0);
-                    if (def && w != want) def = false;
-                }
-        sp.mix = none ? MIX_NONE : (def ? MIX_DEFAULT : MIX_DENSE);
+        mix_of(sp);
+        for (auto &q : sp.parts) mix_of(q);
     }
 }
 

@@ -363,3 +363,26 @@ def test_fuse_weight_formats(form):
     for t in range(3):
         for k in range(2):
             assert W[t, k * 4 + 1, 1] == want[t, k]
+
+
+def test_cpp_parser_merge_basis():
+    """merge (MergedBasis.java:204-229): both loaders give each species the
+    parts' total descriptor size, the network over that size and part 0's
+    mixing in the packed blocks; both refuse a merge without `basis`, with an
+    empty list, or with a part of another type."""
+    parts = [{"type": "spherical_chebyshev", "nmax": 4, "lmax": 4, "rcut": 5.5, "wtype": "default"},
+             {"type": "chebyshev", "nmax": 6, "rcut": 4.0, "wtype": "full"},
+             {"type": "spherical_chebyshev", "nmax": 3, "lmax": 2, "rcut": 3.5, "wtype": "none"}]
+    md = M.random_merge_model(["Cu", "Zr", "Al"], parts, (8, 8), 3)
+    rc, nt, P, S = _parse(json.dumps(md))
+    assert rc == 0, nnap.lib().jse_nnap_last_error()
+    mdl = _both_equal(md)
+    assert S == [s.S for s in mdl.species] == [83, 83, 83] and P == [10, 10, 10]
+    assert mdl.rcut_max == 5.5
+    for bad, msg in (({"type": "merge"}, "Key `basis`"), ({"type": "merge", "basis": []}, "empty"),
+                     ({"type": "merge", "basis": [{"type": "mirror", "mirror": 1}]}, "Unsupported basis type")):
+        m2 = json.loads(json.dumps(md))
+        m2["models"][1]["basis"] = bad
+        assert _parse(json.dumps(m2))[0] != 0 and msg in nnap.lib().jse_nnap_last_error().decode()
+        with pytest.raises(ValueError, match=msg):
+            M.load_model(m2)

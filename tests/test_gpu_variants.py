@@ -4,7 +4,8 @@
 nmax = 5 (one species: no mixing; three species: the default mixing),
 mirror / share bases, a post-fuse weight, the radial-only chebyshev basis,
 shapes without a tiled kernel (the untiled kernels: lmax 0-12, nmax up to
-20, any wtype), and a network without matrix-core tiles (the vector MLP).
+20, any wtype), merge bases, and a network without matrix-core tiles (the
+vector MLP).
 Plain path bar: 1e-6 eV/atom, every force component within 1e-5 eV/A,
 1e-5 eV/atom virial."""
 import numpy as np
@@ -183,3 +184,36 @@ def test_bf16_mode_network_without_bf16_tiles(O):
     f_rms = float(np.sqrt(((res["forces"] - ref["forces"]) ** 2).mean()))
     print(f"e {e:.2e} f_rms {f_rms:.2e}")
     assert e <= 1e-3 and f_rms <= 2e-2
+
+
+_MERGE_PARTS = [
+    {"type": "spherical_chebyshev", "nmax": 8, "lmax": 6, "rcut": 6.0, "wtype": "default"},   # tiled kernels
+    {"type": "chebyshev", "nmax": 5, "rcut": 4.5, "wtype": "default"},                        # radial kernels
+    {"type": "spherical_chebyshev", "nmax": 3, "lmax": 2, "rcut": 3.8, "wtype": "none"},       # untiled kernels
+]
+
+
+def test_merge_basis(O):
+    """merge (MergedBasis.java): a tiled spherical part, a radial part and an
+    untiled spherical part with three cutoffs; each part writes its slice of
+    the descriptor row and the later parts' backwards add into the pairs' g.
+    Plain path at the full bar."""
+    st = _liquid(["Cu", "Zr", "Al"], 1500, 27)
+    md = M.random_merge_model(["Cu", "Zr", "Al"], _MERGE_PARTS, (64, 64), 37)
+    assert M.load_model(md).species[0].S == 126 + 12 + 12
+    md = W._normalized(md, st, 300, 37)
+    _check(O, md, st)
+
+
+def test_merge_basis_fp32_fast(O):
+    """The same merge model in FP32_FAST mode, within that mode's bar
+    (RMS 2e-5 eV/A, max 2e-4 eV/A)."""
+    st = _liquid(["Cu", "Zr", "Al"], 1500, 27)
+    md = W._normalized(M.random_merge_model(["Cu", "Zr", "Al"], _MERGE_PARTS, (64, 64), 37), st, 300, 37)
+    with nnap.NNAP(md, precision=nnap.FP32_FAST) as pot:
+        res = pot.calEnergyForceVirials(nnap.AtomData(st.box, st.xyz, st.types))
+    ref = O.Oracle(M.load_model(md)).system_efv(st)
+    df = res["forces"] - ref["forces"]
+    print(f"f_rms {np.sqrt((df ** 2).mean()):.2e} f_max {np.abs(df).max():.2e}")
+    assert abs(res["energy"] - ref["energies"].sum()) / st.natoms <= 1e-6
+    assert np.sqrt((df ** 2).mean()) <= 2e-5 and np.abs(df).max() <= 2e-4
