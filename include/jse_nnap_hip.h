@@ -128,6 +128,19 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
  * (nnap_interface.cpp:11-24).  Runs the neighbour build itself. */
 int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax);
 
+/* Training derivative of the networks: grad_nn[t] (one block per species, in
+ * the packed nn_param layout [hidden W per layer [out][in] | output W | hidden
+ * b | output b], FP64) = sum over the atoms i of species t of
+ * w[i] dE_i/dtheta at the current positions.  This is the network part of
+ * jse_nnap_backwardEnergy, nnap_interface.cpp:111-134, summed over a system:
+ * the reference's trainer calls that per atom with aGradEng = w[i] and adds
+ * rGradNnParam; here one call covers every atom (the rGradFpParam part, the
+ * trainable fuse / rfuse mixing weights, is not provided).  A mirror or
+ * shared-network species reports the gradient of its own expanded block.
+ * Runs the neighbour build and the descriptors itself; status 2 when a
+ * network's 16-row tile of layers exceeds 160 KB of LDS. */
+int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *const *grad_nn);
+
 /* The neighbour list of the last compute: count[natoms] and, for rows of
  * `cap` slots, key = j*32 + shift_code (shift_code = (sx+1)+3(sy+1)+9(sz+1))
  * sorted ascending and the FP32 displacement d[natoms][cap][3].  Returns

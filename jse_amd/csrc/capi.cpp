@@ -972,6 +972,49 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
     });
 }
 
+int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *const *grad_nn) {
+    return guard([&] {
+        JNN_CHECK(s && grad_nn && (w || s->n == 0), ERR_ARG, "null argument");
+        const int nt = s->model->host.ntypes();
+        for (int t = 0; t < nt; ++t) JNN_CHECK(grad_nn[t], ERR_ARG, "null gradient block");
+        // the normalised descriptors of the current positions, as the networks read them
+        s->last_what = 0;
+        s->pending = true;
+        s->build_nl(false);
+        s->finish();
+        s->have_nl = true;
+        s->desc_fwd(false);
+        double *d_w = dalloc<double>((size_t)std::max(s->n, 1));
+        std::vector<double *> bufs;
+        try {
+            if (s->n) JNN_HIP(hipMemcpyAsync(d_w, w, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+            for (int t = 0; t < nt; ++t) {
+                const MlpArgs a = mlp_args(s->model, t, s->seg_args(t, false), s->d_E, false, 0);
+                const int np = mlp_param_count(a);
+                if (a.nrows <= 0) {
+                    std::fill(grad_nn[t], grad_nn[t] + np, 0.0);
+                    continue;
+                }
+                double *partial = dalloc<double>((size_t)mlp_pgrad_groups(a) * np);
+                bufs.push_back(partial);
+                double *out = dalloc<double>((size_t)np);
+                bufs.push_back(out);
+                JNN_CHECK(launch_mlp_param_grad(s->stream, a, d_w, partial, out), ERR_UNSUPPORTED,
+                          "network too wide for the parameter-gradient kernel (16 rows of its layers in 160 KB)");
+                JNN_HIP(hipMemcpyAsync(grad_nn[t], out, sizeof(double) * np, hipMemcpyDeviceToHost, s->stream));
+            }
+            JNN_HIP(hipStreamSynchronize(s->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(s->stream);
+            for (double *p : bufs) dfree(p);
+            dfree(d_w);
+            throw;
+        }
+        for (double *p : bufs) dfree(p);
+        dfree(d_w);
+    });
+}
+
 int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
     return guard([&] {
         JNN_CHECK(s && (fp || s->n == 0), ERR_ARG, "null argument");

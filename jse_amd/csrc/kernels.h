@@ -249,6 +249,13 @@ bool launch_mlp_mfma_fp64(hipStream_t s, const MlpArgs &a);
 bool launch_mlp_fp64(hipStream_t s, const MlpArgs &a);
 constexpr const char *kMlpFp64Limit = "the FP64 MLP needs input_dim + 2 * sum(hidden_dims) <= 640";
 enum { kMlpFp32 = 0, kMlpBf16 = 1, kMlpFp64 = 2 };  // element types of the MLP kernels
+// Training derivative: out[p] = sum_rows w[atom] dE/dtheta_p over the packed
+// network parameters (FP64 weights; x64 or x as input); partial holds
+// mlp_pgrad_groups(a) rows of mlp_param_count(a) doubles.  False when a
+// 16-row tile does not fit in LDS.
+int mlp_param_count(const MlpArgs &a);
+int mlp_pgrad_groups(const MlpArgs &a);
+bool launch_mlp_param_grad(hipStream_t s, const MlpArgs &a, const double *w, double *partial, double *out);
 // All species in one launch: the persistent workgroups are split among the
 // species in proportion to their row tiles (each stages its species' weights
 // once).  False when the species' MLP shapes differ, there are more than

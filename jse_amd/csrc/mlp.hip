@@ -207,3 +207,171 @@ bool launch_mlp_fp32(hipStream_t s, const MlpArgs &a) { return launch_mlp_vec<fl
 bool launch_mlp_fp64(hipStream_t s, const MlpArgs &a) { return launch_mlp_vec<double>(s, a); }
 
 }  // namespace jnn
+
+namespace jnn {
+
+// ---- training derivative: the network parameters' gradient of sum_i w_i E_i ----
+// (reference: jse_nnap_backwardEnergy, nnap_interface.cpp:111-134, through
+// normedNnBackward with GRAD_PARAM, nnap_main.hpp:297-323, and the layer
+// backward of nn_FeedForward.hpp:131-261).  One workgroup per tile of TR atoms
+// at a time, FP64 throughout: the forward is formed again from x (act[l][TR]
+// in LDS, with the SiLU derivatives), dL/dz of every layer is taken back from
+// dL/dy = w_i sigma_E, and the tile's
+//   dW_l[o][i] = sum_r dz_l[o][r] act_{l-1}[i][r],  db_l[o] = sum_r dz_l[o][r],
+//   dw_o[h] = sum_r dy_r act_L[h][r],  db_o = sum_r dy_r
+// are added into this workgroup's row of `partial` (its first tile stores), in
+// the packed layout [hidden W per layer [out][in] | output W | hidden b | output b].
+// Tiles of a workgroup come in a fixed order, so the sums are deterministic.
+template <int TR>
+__global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__restrict__ w, int nparam,
+                                                   double *__restrict__ partial) {
+    extern __shared__ __align__(16) double pg_smem[];
+    const int S = a.S, L = a.nlayers;
+    const double *__restrict__ wb = a.w_out_in64;  // per layer [out][in]
+    const double *__restrict__ bs = a.bias64;
+    // LDS: act_0 = x [S][TR], then per hidden layer l: act [h][TR], der [h][TR], dz [h][TR]; dy [TR]
+    int aoff[kMaxLayers + 1], doff[kMaxLayers + 1], goff[kMaxLayers + 1];
+    aoff[0] = 0;
+    int off = S * TR;
+    for (int l = 1; l <= L; ++l) {
+        const int h = a.hidden[l - 1];
+        aoff[l] = off;
+        doff[l] = off + h * TR;
+        goff[l] = off + 2 * h * TR;
+        off += 3 * h * TR;
+    }
+    double *dy = pg_smem + off;
+    // packed offsets: hidden W of layer l, output W, hidden b of layer l, output b
+    int woff[kMaxLayers + 1], boff[kMaxLayers + 1];
+    int pw = 0;
+    for (int l = 1; l <= L; ++l) {
+        woff[l] = pw;
+        pw += a.hidden[l - 1] * (l == 1 ? S : a.hidden[l - 2]);
+    }
+    const int wo_off = pw;
+    const int hL = a.hidden[L - 1];
+    int pb = wo_off + hL;
+    for (int l = 1; l <= L; ++l) {
+        boff[l] = pb;
+        pb += a.hidden[l - 1];
+    }
+    const int bo_off = pb;
+    // the output weight follows the hidden weights in the forward layout
+    const double *__restrict__ wout = a.w_in_out64 + wo_off;
+    const int tid = threadIdx.x;
+    double *__restrict__ mine = partial + (size_t)blockIdx.x * nparam;
+    bool first = true;
+    for (int tile = blockIdx.x; tile * TR < a.nrows; tile += gridDim.x) {
+        const int row0 = tile * TR;
+        const int nt = min(TR, a.nrows - row0);
+        for (int idx = tid; idx < TR * S; idx += blockDim.x) {
+            const int r = idx / S, e = idx % S;
+            double v = 0.0;
+            if (r < nt) v = a.x64 ? a.x64[(size_t)(row0 + r) * S + e] : (double)a.x[(size_t)(row0 + r) * S + e];
+            pg_smem[e * TR + r] = v;
+        }
+        if (tid < TR) dy[tid] = tid < nt ? w[a.rows_atom[row0 + tid]] * a.sig_e : 0.0;
+        __syncthreads();
+        // forward
+        for (int l = 1; l <= L; ++l) {
+            const int h = a.hidden[l - 1], in = l == 1 ? S : a.hidden[l - 2];
+            const double *__restrict__ W = wb + woff[l];
+            for (int idx = tid; idx < h * TR; idx += blockDim.x) {
+                const int o = idx / TR, r = idx % TR;
+                double z = 0.0;
+                for (int i = 0; i < in; ++i) z += W[(size_t)o * in + i] * pg_smem[aoff[l - 1] + i * TR + r];
+                z += bs[boff[l] - wo_off - hL + o];  // hidden biases come first in the bias block
+                double g;
+                pg_smem[aoff[l] + idx] = silu_and_grad(z, &g);
+                pg_smem[doff[l] + idx] = g;
+            }
+            __syncthreads();
+        }
+        // dz of the last hidden layer, then back through the layers
+        for (int idx = tid; idx < hL * TR; idx += blockDim.x) {
+            const int o = idx / TR, r = idx % TR;
+            pg_smem[goff[L] + idx] = dy[r] * wout[o] * pg_smem[doff[L] + idx];
+        }
+        __syncthreads();
+        for (int l = L; l >= 2; --l) {
+            const int h = a.hidden[l - 1], in = a.hidden[l - 2];
+            const double *__restrict__ W = wb + woff[l];
+            for (int idx = tid; idx < in * TR; idx += blockDim.x) {
+                const int i = idx / TR, r = idx % TR;
+                double s = 0.0;
+                for (int o = 0; o < h; ++o) s += W[(size_t)o * in + i] * pg_smem[goff[l] + o * TR + r];
+                pg_smem[goff[l - 1] + idx] = s * pg_smem[doff[l - 1] + idx];
+            }
+            __syncthreads();
+        }
+        // this tile's parameter gradients into the workgroup's partial
+        auto put = [&](int p, double v) { mine[p] = first ? v : mine[p] + v; };
+        for (int l = 1; l <= L; ++l) {
+            const int h = a.hidden[l - 1], in = l == 1 ? S : a.hidden[l - 2];
+            for (int idx = tid; idx < h * in; idx += blockDim.x) {
+                const int o = idx / in, i = idx % in;
+                double s = 0.0;
+                for (int r = 0; r < TR; ++r) s += pg_smem[goff[l] + o * TR + r] * pg_smem[aoff[l - 1] + i * TR + r];
+                put(woff[l] + idx, s);
+            }
+            for (int o = tid; o < h; o += blockDim.x) {
+                double s = 0.0;
+                for (int r = 0; r < TR; ++r) s += pg_smem[goff[l] + o * TR + r];
+                put(boff[l] + o, s);
+            }
+        }
+        for (int o = tid; o < hL; o += blockDim.x) {
+            double s = 0.0;
+            for (int r = 0; r < TR; ++r) s += dy[r] * pg_smem[aoff[L] + o * TR + r];
+            put(wo_off + o, s);
+        }
+        if (tid == 0) {
+            double s = 0.0;
+            for (int r = 0; r < TR; ++r) s += dy[r];
+            put(bo_off, s);
+        }
+        first = false;
+        __syncthreads();
+    }
+}
+// out[p] = sum over the workgroups' partials, in workgroup order
+__global__ void k_pgrad_reduce(int nwg, int nparam, const double *__restrict__ partial, double *__restrict__ out) {
+    const int p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= nparam) return;
+    double s = 0.0;
+    for (int g = 0; g < nwg; ++g) s += partial[(size_t)g * nparam + p];
+    out[p] = s;
+}
+
+int mlp_param_count(const MlpArgs &a) {
+    int n = 0, in = a.S;
+    for (int l = 0; l < a.nlayers; ++l) {
+        n += a.hidden[l] * in + a.hidden[l];
+        in = a.hidden[l];
+    }
+    return n + in + 1;
+}
+int mlp_pgrad_groups(const MlpArgs &a) {
+    const int tiles = (a.nrows + 15) / 16;
+    return std::max(1, std::min(tiles, device_cu_count()));
+}
+bool launch_mlp_param_grad(hipStream_t s, const MlpArgs &a, const double *w, double *partial, double *out) {
+    if (a.nrows <= 0) return true;
+    constexpr int TR = 16;
+    size_t f = (size_t)a.S * TR + TR;
+    for (int l = 0; l < a.nlayers; ++l) f += 3 * (size_t)a.hidden[l] * TR;
+    const size_t lds = f * sizeof(double);
+    if (lds > 160 * 1024) return false;
+    static const bool attr_set = [] {
+        JNN_HIP(hipFuncSetAttribute((const void *)k_mlp_pgrad<16>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024));
+        return true;
+    }();
+    (void)attr_set;
+    const int np = mlp_param_count(a), nwg = mlp_pgrad_groups(a);
+    hipLaunchKernelGGL((k_mlp_pgrad<TR>), dim3(nwg), dim3(256), lds, s, a, w, np, partial);
+    hipLaunchKernelGGL(k_pgrad_reduce, dim3((np + 255) / 256), dim3(256), 0, s, nwg, np, partial, out);
+    return true;
+}
+
+}  // namespace jnn

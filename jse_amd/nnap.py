@@ -55,6 +55,7 @@ EXPORTS = [
     "jse_nnap_model_fp_size", "jse_nnap_model_destroy", "jse_nnap_system_create", "jse_nnap_system_destroy",
     "jse_nnap_system_set_box", "jse_nnap_system_set_positions", "jse_nnap_system_set_positions_device",
     "jse_nnap_system_compute", "jse_nnap_system_get_results", "jse_nnap_system_compute_fp",
+    "jse_nnap_system_backward_energy",
     "jse_nnap_system_get_neighbors", "jse_nnap_system_benchmark", "jse_nnap_system_nl_stats",
     "jse_nnap_system_get_pair_gradients", "jse_nnap_batch_create", "jse_nnap_system_get_replica_results",
     "jse_nnap_system_set_graphs", "jse_nnap_system_autotune", "jse_nnap_system_set_tuning", "jse_nnap_server_create", "jse_nnap_server_add_replica", "jse_nnap_server_submit",
@@ -107,6 +108,7 @@ def lib():
     L.jse_nnap_system_compute.argtypes = [vp, ctypes.c_int]
     L.jse_nnap_system_get_results.argtypes = [vp, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     L.jse_nnap_system_compute_fp.argtypes = [vp, _c_dbl_p, ctypes.c_int]
+    L.jse_nnap_system_backward_energy.argtypes = [vp, _c_dbl_p, ctypes.POINTER(_c_dbl_p)]
     L.jse_nnap_system_get_neighbors.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_flt_p]
     L.jse_nnap_system_benchmark.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                             ctypes.c_int]
@@ -251,6 +253,19 @@ class System:
         fp = np.zeros((self.natoms, smax))
         _check(lib().jse_nnap_system_compute_fp(self._h, _dp(fp), smax))
         return fp
+
+    def backward_energy(self, w) -> List[np.ndarray]:
+        """Network parameter gradients of sum_i w[i] E_i (jse_nnap_backwardEnergy's
+        rGradNnParam summed over the atoms, aGradEng = w[i]): one array per
+        species in the packed nn_param layout."""
+        w = np.ascontiguousarray(w, dtype=np.float64)
+        if w.shape != (self.natoms,):
+            raise ValueError("w must have one entry per atom")
+        sp = self.nnap.model.species
+        outs = [np.zeros(s.nn_param.size) for s in sp]
+        ptrs = (_c_dbl_p * len(outs))(*[_dp(o) for o in outs])
+        _check(lib().jse_nnap_system_backward_energy(self._h, _dp(w), ptrs))
+        return outs
 
     def neighbors(self, cap: Optional[int] = None):
         n = self.natoms

@@ -156,6 +156,69 @@ class Oracle:
                     "nei_count": cnt}
 
 
+def system_fp(orc: "Oracle", st, cap: int = 256):
+    """Descriptors of every atom (orc_system_fp: jse_nnap_calFp per atom over the
+    cell-list neighbours), rows [natoms][Smax]."""
+    n = st.natoms
+    smax = max(s.S for s in orc.mdl.species)
+    xyz = np.ascontiguousarray(st.xyz, dtype=np.float64)
+    ty = np.ascontiguousarray(st.types, dtype=np.int32)
+    box = np.ascontiguousarray(st.box, dtype=np.float64)
+    out = np.zeros((n, smax))
+    while True:
+        rc = orc.L.orc_system_fp(ctypes.byref(orc.c), n, _dp(xyz), _ip(ty), _dp(box), cap, smax, _dp(out))
+        if rc == 2:
+            cap *= 2
+            continue
+        if rc:
+            raise ValueError(f"oracle system_fp code {rc}")
+        return out
+
+
+def nn_param_grad(sp, fp, w):
+    """The network part of jse_nnap_backwardEnergy summed over rows
+    (nnap_interface.cpp:111-134; normedNnBackward with GRAD_PARAM,
+    nnap_main.hpp:297-323; the layer backward of nn_FeedForward.hpp:131-261):
+    d(sum_i w_i E_i)/d nn_param for one species, fp rows [n][S], in the packed
+    layout [hidden W per layer [out][in] | output W | hidden b | output b].
+    Plain numpy FP64."""
+    S = sp.S
+    fp = np.asarray(fp, dtype=np.float64)[:, :S]
+    w = np.asarray(w, dtype=np.float64)
+    nn = sp.nn_param
+    norm = sp.norm_param
+    mu, sg = norm[2:2 + S], norm[2 + S:2 + 2 * S]
+    hidden = list(sp.hidden)
+    Ws, off, cin = [], 0, S
+    for h in hidden:
+        Ws.append(nn[off:off + h * cin].reshape(h, cin))
+        off += h * cin
+        cin = h
+    wo = nn[off:off + cin]
+    off += cin
+    bs = []
+    for h in hidden:
+        bs.append(nn[off:off + h])
+        off += h
+    acts, ders = [(fp - mu) / sg], []
+    for W, b in zip(Ws, bs):
+        z = acts[-1] @ W.T + b
+        s = 1.0 / (1.0 + np.exp(-z))
+        acts.append(z * s)
+        ders.append(s * (1.0 + z * (1.0 - s)))
+    dy = w * norm[1]                      # dL/dy = w_i sigma_E
+    g_wo = acts[-1].T @ dy
+    g_bo = dy.sum()
+    dz = (dy[:, None] * wo[None, :]) * ders[-1]
+    gW, gb = [None] * len(hidden), [None] * len(hidden)
+    for l in range(len(hidden) - 1, -1, -1):
+        gW[l] = dz.T @ acts[l]
+        gb[l] = dz.sum(0)
+        if l > 0:
+            dz = (dz @ Ws[l]) * ders[l - 1]
+    return np.concatenate([g.ravel() for g in gW] + [g_wo] + gb + [[g_bo]])
+
+
 def wrap(xyz, box):
     xyz = np.ascontiguousarray(xyz, dtype=np.float64)
     out = np.zeros_like(xyz)

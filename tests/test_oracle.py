@@ -623,3 +623,75 @@ def test_merge_basis_loader_errors():
         md["models"][0]["basis"] = bad
         with pytest.raises(ValueError, match=msg):
             M.load_model(md)
+
+
+# ------------------------------------------------ training derivative (networks)
+def _torch_param_grad(sp, fp, w):
+    import torch
+    S = sp.S
+    nn = torch.tensor(sp.nn_param, dtype=torch.float64, requires_grad=True)
+    norm = torch.tensor(sp.norm_param, dtype=torch.float64)
+    x = (torch.tensor(fp[:, :S]) - norm[2:2 + S]) / norm[2 + S:2 + 2 * S]
+    off, cin, Ws = 0, S, []
+    for h in sp.hidden:
+        Ws.append(nn[off:off + h * cin].reshape(h, cin))
+        off += h * cin
+        cin = h
+    wo = nn[off:off + cin]
+    off += cin
+    bs = []
+    for h in sp.hidden:
+        bs.append(nn[off:off + h])
+        off += h
+    bo = nn[off]
+    a = x
+    for W, b in zip(Ws, bs):
+        z = a @ W.T + b
+        a = z * torch.sigmoid(z)
+    E = (a @ wo + bo) * norm[1] + norm[0]
+    (torch.tensor(w) * E).sum().backward()
+    return nn.grad.numpy()
+
+
+def test_nn_param_grad_against_torch_autograd(O):
+    """The oracle's network parameter gradient (jse_nnap_backwardEnergy's
+    rGradNnParam summed over atoms) equals torch autograd of the same
+    normalised network, for three hidden layers of different widths."""
+    md = M.random_model(["Cu", "Zr"], 3, 3, 5.0, (7, 5, 6), seed=81)
+    mdl = M.load_model(md)
+    rng = np.random.default_rng(81)
+    for sp in mdl.species:
+        sp.norm_param[2:] = np.concatenate([rng.normal(size=sp.S), rng.uniform(0.5, 2.0, size=sp.S)])
+        sp.norm_param[:2] = [0.3, 1.7]
+        fp = rng.normal(size=(40, sp.S)) * 2.0
+        w = rng.normal(size=40)
+        g = O.nn_param_grad(sp, fp, w)
+        assert g.shape == sp.nn_param.shape
+        assert np.allclose(g, _torch_param_grad(sp, fp, w), rtol=1e-12, atol=1e-13)
+
+
+def test_nn_param_grad_finite_differences(O):
+    """The same gradient against central differences of sum_i w_i E_i, with
+    E_i from the C restatement's network (orc_mlp), for entries of every
+    block (hidden W, output W, hidden b, output b)."""
+    md = M.random_model(["Cu"], 3, 2, 5.0, (6, 4), seed=82)
+    mdl = M.load_model(md)
+    sp = mdl.species[0]
+    rng = np.random.default_rng(82)
+    fp = rng.normal(size=(12, sp.S))
+    w = rng.normal(size=12)
+    g = O.nn_param_grad(sp, fp, w)
+
+    def loss():
+        orc = O.Oracle(mdl)
+        return sum(wi * orc.mlp(1, f)[0] for wi, f in zip(w, fp))
+    nW1 = 6 * sp.S
+    for p in (0, nW1 - 1, nW1 + 5, nW1 + 24 + 1, sp.nn_param.size - 7, sp.nn_param.size - 1):
+        h = 1e-6
+        v = sp.nn_param[p]
+        sp.nn_param[p] = v + h
+        lp = loss()
+        sp.nn_param[p] = v - h
+        lm = loss()
+        sp.nn_param[p] = v
+        assert abs((lp - lm) / (2 * h) - g[p]) < 1e-7 * max(1.0, abs(g[p]))
