@@ -24,13 +24,14 @@ typedef struct jse_nnap_system_s *jse_nnap_system;
 
 /* precision of a system (the reference's CUDA path is FP32 only, NNAP.java:145-147).
  * JSE_NNAP_FP32, the plain path, meets 1e-6 eV/atom, 1e-5 eV/A and 1e-5 eV
- * against the reference's FP64 CPU path: storage, MLP and the backward's
- * contractions are FP32, while the displacement, the pair geometry, the
- * harmonics and radial basis (forward and backward), the contraction A and the
- * power spectrum are FP64, as are the sums.  The other two modes evaluate the
- * geometry and the descriptor in FP32 (errors near 3e-5 eV/A on well
- * conditioned models, more where sigma << mu in the normalisation). */
-#define JSE_NNAP_FP32 0       /* plain: FP32 storage and MLP, FP64 geometry/descriptor/sums */
+ * against the reference's FP64 CPU path: the displacement, the pair geometry,
+ * the harmonics and radial basis, the contraction A (FP64 matrix cores), the
+ * power spectrum, the MLP and the chain rule are FP64, as are the sums; only
+ * the backward's dE/dA contraction rounds its inputs to FP32.  The other two
+ * modes evaluate the geometry and the descriptor in FP32 (errors near 3e-5
+ * eV/A on well conditioned models, more where sigma << mu in the
+ * normalisation). */
+#define JSE_NNAP_FP32 0       /* plain: FP64 geometry/descriptor/MLP/sums, FP32 dE/dA contraction */
 #define JSE_NNAP_BF16_MLP 1   /* FP32 geometry and descriptor, MLP on BF16 MFMA with FP32 accumulation */
 #define JSE_NNAP_FP32_FAST 2  /* FP32 geometry and descriptor, FP32 MLP */
 

@@ -36,9 +36,10 @@ _HERE = os.path.dirname(os.path.abspath(__file__))
 LIB_PATH = os.path.join(_HERE, "libjsenn_hip.so")
 
 # precision modes (include/jse_nnap_hip.h): FP32 is the plain path that meets the
-# reference's FP64 results to 1e-6 eV/atom, 1e-5 eV/A (FP64 geometry and
-# descriptor, FP32 MLP); BF16_MLP and FP32_FAST evaluate the geometry and the
-# descriptor in FP32, with a BF16 or FP32 MLP
+# reference's FP64 results to 1e-6 eV/atom, 1e-5 eV/A (FP64 geometry, descriptor
+# and MLP; only the backward's dE/dA contraction rounds to FP32); BF16_MLP and
+# FP32_FAST evaluate the geometry and the descriptor in FP32, with a BF16 or
+# FP32 MLP
 FP32 = 0
 BF16_MLP = 1
 FP32_FAST = 2
