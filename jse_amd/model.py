@@ -40,6 +40,12 @@ DEFAULT_HIDDEN = (32, 32)  # FeedForward.java:28
 WS_EPS = 1e-5  # weight standardisation epsilon of the rfuse weight (WTypeBasis.java:20, used at :216)
 
 
+# columns the L3 / L4 invariants add per radial channel, by l3max / l4max
+# (SphericalChebyshev.java:15-16)
+L3_COLS = (0, 0, 2, 4, 9, 14, 23)
+L4_COLS = (0, 1, 3, 9)
+
+
 @dataclass
 class Basis:
     """One spherical-Chebyshev or radial-only Chebyshev basis (a part of a
@@ -52,18 +58,30 @@ class Basis:
     P: int
     W: np.ndarray       # [ntypes][P][K]
     radial: bool = False
+    l3max: int = 0      # L3 / L4 invariants (SphericalChebyshev.java:33-42)
+    l4max: int = 0
 
     @property
     def K(self) -> int:
         return self.nmax + 1
 
     @property
+    def lmm(self) -> int:
+        """The harmonics' lmax: max(lmax, l3max, l4max)."""
+        return max(self.lmax, self.l3max, self.l4max)
+
+    @property
     def M(self) -> int:
-        return (self.lmax + 1) ** 2
+        return (self.lmm + 1) ** 2
+
+    @property
+    def SL(self) -> int:
+        """Descriptor columns per radial channel: L2, then L3, then L4."""
+        return self.lmax + 1 + L3_COLS[self.l3max] + L4_COLS[self.l4max]
 
     @property
     def S(self) -> int:
-        return self.P * (self.lmax + 1)
+        return self.P * self.SL
 
 
 @dataclass
@@ -83,6 +101,8 @@ class Species:
     radial: bool = False  # radial-only chebyshev basis: fp[p] = sum_j fc Rnp[p] (sizes as lmax = 0)
     parts: Optional[List[Basis]] = None  # merge basis (MergedBasis.java): fp = the parts' fp concatenated;
                                          # the single-basis fields above then hold part 0, rcut the largest
+    l3max: int = 0
+    l4max: int = 0
 
     @property
     def K(self) -> int:
@@ -90,19 +110,18 @@ class Species:
 
     @property
     def M(self) -> int:
-        return (self.lmax + 1) ** 2
+        return (max(self.lmax, self.l3max, self.l4max) + 1) ** 2
 
     @property
     def S(self) -> int:
-        if self.parts:
-            return sum(b.S for b in self.parts)
-        return self.P * (self.lmax + 1)
+        return sum(b.S for b in self.bases())
 
     def bases(self) -> List[Basis]:
         """The species' bases in fp order (one unless it is a merge basis)."""
         if self.parts:
             return list(self.parts)
-        return [Basis(self.nmax, self.lmax, self.rcut, self.wtype, self.typed_wtype, self.P, self.W, self.radial)]
+        return [Basis(self.nmax, self.lmax, self.rcut, self.wtype, self.typed_wtype, self.P, self.W, self.radial,
+                      self.l3max, self.l4max)]
 
 
 @dataclass
@@ -404,12 +423,14 @@ def _basis(b: dict, btype: str, ntypes: int) -> Basis:
     lmax = 0 if radial else int(_get(b, "lmax", default=DEFAULT_LMAX))
     l3 = 0 if radial else int(_get(b, "l3max", default=0))
     l4 = 0 if radial else int(_get(b, "l4max", default=0))
-    if lmax < 0 or lmax > 12:
+    if lmax < 0 or lmax > 12:   # SphericalChebyshev.java:32-34 (lmax checked as max(lmax, l3max, l4max))
         raise ValueError(f"Input lmax MUST be in [0, 12], input: {lmax}")
+    if l3 < 0 or l3 > 6:
+        raise ValueError(f"Input l3max MUST be in [0, 6], input: {l3}")
+    if l4 < 0 or l4 > 3:
+        raise ValueError(f"Input l4max MUST be in [0, 3], input: {l4}")
     if nmax < 0 or nmax > 20:   # WTypeBasis.java:56
         raise ValueError(f"Input nmax MUST be in [0, 20], input: {nmax}")
-    if l3 != 0 or l4 != 0:
-        raise ValueError("l3max/l4max > 0 are not supported by this build")
     rcut = float(_get(b, "rcut", default=DEFAULT_RCUT))
     wt = _get(b, "wtype", default="default")
     wtype = int(wt) if isinstance(wt, (int, float)) else WTYPES.get(str(wt))
@@ -428,7 +449,7 @@ def _basis(b: dict, btype: str, ntypes: int) -> Basis:
         rfuse_w, wtype, fuse_w = post, WTYPE_RFUSE, None
         rfuse_scale = float(b.get("post_fuse_scale", 1.0) or 1.0)
     typed, P, W = _mixing(wtype, ntypes, nmax, fuse_w, rfuse_w, rfuse_scale, ws)
-    return Basis(nmax, lmax, rcut, wtype, typed, P, W, radial)
+    return Basis(nmax, lmax, rcut, wtype, typed, P, W, radial, l3, l4)
 
 
 def load_model(src) -> Model:
@@ -566,7 +587,8 @@ def load_model(src) -> Model:
         rcut = max(bp.rcut for bp in bases[i])
         out.append(Species(str(sym), b0.nmax, b0.lmax, rcut, b0.wtype, b0.typed_wtype, b0.P, b0.W, list(hidden),
                            nn_param.copy(), norm, ref[i] if is_mirror[i] else 0, b0.radial,
-                           list(bases[i]) if len(bases[i]) > 1 or btypes[i] == "merge" else None))
+                           list(bases[i]) if len(bases[i]) > 1 or btypes[i] == "merge" else None,
+                           b0.l3max, b0.l4max))
     return Model(out, info.get("units"), info)
 
 
@@ -605,7 +627,8 @@ def random_merge_model(symbols: Sequence[str], parts: Sequence[dict], hidden: Se
         P = _size_n(wt, ntypes, nmax, 0)
         if ntypes == 1 and wt in (WTYPE_DEFAULT, WTYPE_FULL, WTYPE_EXFULL):
             P = _size_n(WTYPE_NONE, 1, nmax, 0)
-        S += P * (1 if radial else int(b.get("lmax", DEFAULT_LMAX)) + 1)
+        S += P * (1 if radial else int(b.get("lmax", DEFAULT_LMAX)) + 1 + L3_COLS[int(b.get("l3max", 0))]
+                  + L4_COLS[int(b.get("l4max", 0))])
     models = []
     for sym in symbols:
         models.append({"symbol": sym, "basis": {"type": "merge", "basis": [dict(b) for b in parts]},
@@ -617,7 +640,7 @@ def random_merge_model(symbols: Sequence[str], parts: Sequence[dict], hidden: Se
 
 
 def random_model(symbols: Sequence[str], nmax: int, lmax: int, rcut: float, hidden: Sequence[int],
-                 seed: int, wtype: str = "default") -> dict:
+                 seed: int, wtype: str = "default", l3max: int = 0, l4max: int = 0) -> dict:
     """Seeded random-init model in the reference file format, drawn from the
     distributions of FeedForward.initParameters (FeedForward.java:105-128):
     hidden W ~ U(+-sqrt(6/fan_in)), biases ~ U(+-sqrt(1/fan_in)),
@@ -628,12 +651,12 @@ def random_model(symbols: Sequence[str], nmax: int, lmax: int, rcut: float, hidd
     P = _size_n(WTYPES[wtype], ntypes, nmax, 0)
     if ntypes == 1 and WTYPES[wtype] in (WTYPE_DEFAULT, WTYPE_FULL, WTYPE_EXFULL):
         P = _size_n(WTYPE_NONE, 1, nmax, 0)
-    S = P * (lmax + 1)
+    S = P * (lmax + 1 + L3_COLS[l3max] + L4_COLS[l4max])
     models = []
     for sym in symbols:
         models.append({
             "symbol": sym,
-            "basis": {"type": "spherical_chebyshev", "nmax": nmax, "lmax": lmax, "l3max": 0, "l4max": 0,
+            "basis": {"type": "spherical_chebyshev", "nmax": nmax, "lmax": lmax, "l3max": l3max, "l4max": l4max,
                       "rcut": rcut, "wtype": wtype},
             "nn": _random_network(rng, S, hidden),
             "ref_eng": 0.0,

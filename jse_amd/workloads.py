@@ -182,6 +182,8 @@ def descriptors_np(mdl: _model.Model, st: Structure, atoms: Optional[np.ndarray]
             if b.radial:
                 parts.append(c.sum(0))
                 continue
+            if b.l3max or b.l4max:
+                raise ValueError("descriptors_np evaluates the L2 power spectrum only (l3max = l4max = 0)")
             Y = _real_sph_np(b.lmax, d[:, 0], d[:, 1], d[:, 2])
             A = c.T @ Y
             fp = np.zeros((b.P, b.lmax + 1))

@@ -20,6 +20,10 @@
  *                   their closed forms)
  *   dY/dtheta,dphi  basis_SphericalUtil.hpp:103-182
  *   A, fp (L2)      basis_SphericalChebyshev.hpp:264-351, basis_SphericalUtil.hpp:378-394
+ *   fp (L3, L4)     basis_SphericalUtil.hpp:395-467 (coefficient tables in
+ *                   basis_SphericalUtil0.hpp:10-500, recomputed here from the
+ *                   Wigner 3j couplings they hold; column counts
+ *                   SphericalChebyshev.java:15-16,39)
  *   wtype mixing W  src/nnap/java/jsex/nnap/basis/WTypeBasis.java:202-304
  *   MLP fwd/bwd     nn_FeedForward.hpp:8-21,131-261; nnap_main.hpp:267-323
  *   fp backward     basis_SphericalChebyshev.hpp:353-525; basis_SphericalUtil.hpp:469-484
@@ -35,6 +39,8 @@
 #include <stdlib.h>
 #include <string.h>
 #include <stdint.h>
+#include <complex.h>
+#include <pthread.h>
 
 #define ORC_MAX_LMAX 12
 #define ORC_MAX_M ((ORC_MAX_LMAX + 1) * (ORC_MAX_LMAX + 1))
@@ -61,6 +67,8 @@ typedef struct {
     const int *radial;        /* [bases] 1: radial-only chebyshev basis (lmax 0 sizes), NULL: none */
     const int *part0;         /* [ntypes] first basis of each species (NULL: basis t) */
     const int *nparts;        /* [ntypes] number of bases of each species (NULL: one) */
+    const int *l3max;         /* [bases] L3 invariants' lmax (0: none; NULL: none at all) */
+    const int *l4max;         /* [bases] L4 invariants' lmax (0: none; NULL: none at all) */
 } orc_model;
 
 static int nparts_of(const orc_model *m, int t) { return m->nparts ? m->nparts[t] : 1; }
@@ -388,10 +396,206 @@ int orc_real_sph(int lmax, double dx, double dy, double dz, double *y, double *y
 }
 
 /* ------------------------------------------------------------------ */
+/* L3 / L4 invariants of one radial channel's A row (index l^2+l+m),   */
+/* basis_SphericalUtil.hpp:395-467.  The reference holds them as sparse */
+/* coefficient tables; they are the rotation invariants                 */
+/*   L3(l1 l2 l3)    = sum_{m1 m2 m3} R(l1 l2 l3)_{m1 m2 m3} A_{l1 m1} A_{l2 m2} A_{l3 m3}  */
+/*   L4(l1 l2 l3 l4) = sum_L sum_M u_{LM} v_{LM},                         */
+/*     u_{LM} = sum_{m1 m2} R(l1 l2 L)_{m1 m2 M} A_{l1 m1} A_{l2 m2},      */
+/*     v_{LM} = sum_{m3 m4} R(l3 l4 L)_{m3 m4 M} A_{l3 m3} A_{l4 m4},      */
+/* L over every value both couplings allow, each with weight one, where */
+/* R is the Wigner 3j symbol carried to the real harmonics of real_sph: */
+/* with Condon-Shortley complex Y^c, Y_{l,m} = sum_k U_{mk} Y^c_{l,k},    */
+/*   U_{00} = 1;  m > 0: U_{m,m} = 1/sqrt2, U_{m,-m} = (-1)^m/sqrt2;      */
+/*   m < 0 (a = -m): U_{m,-a} = i(-1)^a/sqrt2, U_{m,a} = -i/sqrt2;        */
+/*   R(l1 l2 l3)_{m1 m2 m3} = sum_k (l1 l2 l3; k1 k2 k3) conj(U_{m1 k1} U_{m2 k2} U_{m3 k3}). */
+/* R is real for l1 + l2 + l3 even and imaginary for odd (then u v is   */
+/* real).  The invariants' order within a radial channel's columns,      */
+/* after its lmax + 1 L2 values, is the reference's:                     */
+/* ------------------------------------------------------------------ */
+static const int L3_COLS[7] = {0, 0, 2, 4, 9, 14, 23};
+static const int L4_COLS[4] = {0, 1, 3, 9};
+static const int L3_LIST[23][3] = {
+    {2, 2, 2}, {1, 1, 2},                                                   /* l3max 2 */
+    {2, 3, 3}, {1, 2, 3},                                                   /* 3 */
+    {4, 4, 4}, {2, 2, 4}, {3, 3, 4}, {2, 4, 4}, {1, 3, 4},                  /* 4 */
+    {2, 5, 5}, {4, 5, 5}, {2, 3, 5}, {1, 4, 5}, {3, 4, 5},                  /* 5 */
+    {6, 6, 6}, {3, 3, 6}, {4, 4, 6}, {5, 5, 6}, {2, 6, 6}, {4, 6, 6}, {2, 4, 6}, {1, 5, 6}, {3, 5, 6}}; /* 6 */
+static const int L4_LIST[9][4] = {
+    {1, 1, 1, 1},                                                           /* l4max 1 */
+    {2, 2, 2, 2}, {1, 1, 2, 2},                                             /* 2 */
+    {3, 3, 3, 3}, {1, 1, 3, 3}, {2, 2, 3, 3}, {1, 1, 1, 3}, {1, 3, 3, 3}, {1, 2, 2, 3}}; /* 3 */
+
+static double FACT[48];
+static double w3j(int j1, int j2, int j3, int m1, int m2, int m3) {
+    if (m1 + m2 + m3 != 0 || j3 < abs(j1 - j2) || j3 > j1 + j2 || abs(m1) > j1 || abs(m2) > j2 || abs(m3) > j3)
+        return 0.0;
+    /* Racah's formula */
+    double pre = sqrt(FACT[j1 + j2 - j3] * FACT[j1 - j2 + j3] * FACT[-j1 + j2 + j3] / FACT[j1 + j2 + j3 + 1] *
+                      FACT[j1 + m1] * FACT[j1 - m1] * FACT[j2 + m2] * FACT[j2 - m2] * FACT[j3 + m3] * FACT[j3 - m3]);
+    int kmin = 0, kmax = j1 + j2 - j3;
+    if (j2 - j3 - m1 > kmin) kmin = j2 - j3 - m1;
+    if (j1 - j3 + m2 > kmin) kmin = j1 - j3 + m2;
+    if (j1 - m1 < kmax) kmax = j1 - m1;
+    if (j2 + m2 < kmax) kmax = j2 + m2;
+    double s = 0.0;
+    for (int k = kmin; k <= kmax; ++k)
+        s += ((k & 1) ? -1.0 : 1.0) / (FACT[k] * FACT[j1 + j2 - j3 - k] * FACT[j1 - m1 - k] * FACT[j2 + m2 - k] *
+                                        FACT[j3 - j2 + m1 + k] * FACT[j3 - j1 - m2 + k]);
+    return (((j1 - j2 - m3) % 2) ? -1.0 : 1.0) * pre * s;
+}
+/* the (up to two) complex components k of real harmonic m */
+static int u_row(int m, int *k, double complex *u) {
+    if (m == 0) { k[0] = 0; u[0] = 1.0; return 1; }
+    const int a = abs(m);
+    const double h = M_SQRT1_2, sg = (a & 1) ? -1.0 : 1.0;
+    if (m > 0) { k[0] = a; u[0] = h; k[1] = -a; u[1] = sg * h; }
+    else { k[0] = -a; u[0] = I * sg * h; k[1] = a; u[1] = -I * h; }
+    return 2;
+}
+/* R(l1 l2 l3) as [(2l1+1)][(2l2+1)][(2l3+1)], m = index - l */
+static void real_coupling(int l1, int l2, int l3, double complex *R) {
+    const int n1 = 2 * l1 + 1, n2 = 2 * l2 + 1, n3 = 2 * l3 + 1;
+    for (int a = 0; a < n1; ++a)
+        for (int b = 0; b < n2; ++b)
+            for (int c = 0; c < n3; ++c) {
+                int k1[2], k2[2], k3[2];
+                double complex u1[2], u2[2], u3[2], s = 0.0;
+                const int r1 = u_row(a - l1, k1, u1), r2 = u_row(b - l2, k2, u2), r3 = u_row(c - l3, k3, u3);
+                for (int x = 0; x < r1; ++x)
+                    for (int y = 0; y < r2; ++y)
+                        for (int z = 0; z < r3; ++z)
+                            if (k1[x] + k2[y] + k3[z] == 0)
+                                s += w3j(l1, l2, l3, k1[x], k2[y], k3[z]) * conj(u1[x]) * conj(u2[y]) * conj(u3[z]);
+                R[(a * n2 + b) * n3 + c] = s;
+            }
+}
+/* couplings computed once per process: R3[k] for the L3 list, R4[k][L]
+ * (the pair (l1 l2 L) then (l3 l4 L)) for the L4 list */
+#define ORC_MAX_L34 6
+static double complex *R3[23];
+static double complex *R4[9][2 * ORC_MAX_L34 + 1][2];
+static pthread_once_t coupling_once = PTHREAD_ONCE_INIT;
+static void coupling_init(void) {
+    FACT[0] = 1.0;
+    for (int k = 1; k < 48; ++k) FACT[k] = FACT[k - 1] * k;
+    for (int k = 0; k < 23; ++k) {
+        const int *l = L3_LIST[k];
+        R3[k] = (double complex *)malloc(sizeof(double complex) * (2 * l[0] + 1) * (2 * l[1] + 1) * (2 * l[2] + 1));
+        real_coupling(l[0], l[1], l[2], R3[k]);
+    }
+    for (int k = 0; k < 9; ++k) {
+        const int *l = L4_LIST[k];
+        for (int L = 0; L <= 2 * ORC_MAX_L34; ++L) {
+            R4[k][L][0] = R4[k][L][1] = NULL;
+            if (L < abs(l[0] - l[1]) || L > l[0] + l[1] || L < abs(l[2] - l[3]) || L > l[2] + l[3]) continue;
+            for (int h = 0; h < 2; ++h) {
+                const int a = l[2 * h], b = l[2 * h + 1];
+                R4[k][L][h] = (double complex *)malloc(sizeof(double complex) * (2 * a + 1) * (2 * b + 1) * (2 * L + 1));
+                real_coupling(a, b, L, R4[k][L][h]);
+            }
+        }
+    }
+}
+/* u_{LM} of one pair (a, b) of blocks */
+static void pair_coupled(int a, int b, int L, const double complex *R, const double *A, double complex *u) {
+    const int na = 2 * a + 1, nb = 2 * b + 1, nL = 2 * L + 1;
+    for (int M = 0; M < nL; ++M) u[M] = 0.0;
+    for (int x = 0; x < na; ++x)
+        for (int y = 0; y < nb; ++y) {
+            const double w = A[a * a + x] * A[b * b + y];
+            for (int M = 0; M < nL; ++M) u[M] += R[(x * nb + y) * nL + M] * w;
+        }
+}
+/* out[0 .. L3_COLS[l3max] + L4_COLS[l4max] - 1] of one A row */
+static void sh_invariants(int l3max, int l4max, const double *A, double *out) {
+    pthread_once(&coupling_once, coupling_init);
+    int o = 0;
+    for (int k = 0; k < L3_COLS[l3max]; ++k) {
+        const int *l = L3_LIST[k];
+        const int n2 = 2 * l[1] + 1, n3 = 2 * l[2] + 1;
+        double s = 0.0;
+        for (int a = 0; a < 2 * l[0] + 1; ++a)
+            for (int b = 0; b < n2; ++b)
+                for (int c = 0; c < n3; ++c)
+                    s += creal(R3[k][(a * n2 + b) * n3 + c]) * A[l[0] * l[0] + a] * A[l[1] * l[1] + b] *
+                         A[l[2] * l[2] + c];
+        out[o++] = s;
+    }
+    for (int k = 0; k < L4_COLS[l4max]; ++k) {
+        const int *l = L4_LIST[k];
+        double s = 0.0;
+        for (int L = 0; L <= 2 * ORC_MAX_L34; ++L) {
+            if (!R4[k][L][0]) continue;
+            double complex u[2 * 2 * ORC_MAX_L34 + 1], v[2 * 2 * ORC_MAX_L34 + 1], t = 0.0;
+            pair_coupled(l[0], l[1], L, R4[k][L][0], A, u);
+            pair_coupled(l[2], l[3], L, R4[k][L][1], A, v);
+            for (int M = 0; M < 2 * L + 1; ++M) t += u[M] * v[M];
+            s += creal(t);
+        }
+        out[o++] = s;
+    }
+}
+/* gA += sum_k g[k] d(invariant k)/dA for one A row */
+static void sh_invariants_grad(int l3max, int l4max, const double *A, const double *g, double *gA) {
+    pthread_once(&coupling_once, coupling_init);
+    int o = 0;
+    for (int k = 0; k < L3_COLS[l3max]; ++k, ++o) {
+        const int *l = L3_LIST[k];
+        const int i1 = l[0] * l[0], i2 = l[1] * l[1], i3 = l[2] * l[2];
+        const int n2 = 2 * l[1] + 1, n3 = 2 * l[2] + 1;
+        for (int a = 0; a < 2 * l[0] + 1; ++a)
+            for (int b = 0; b < n2; ++b)
+                for (int c = 0; c < n3; ++c) {
+                    const double r = g[o] * creal(R3[k][(a * n2 + b) * n3 + c]);
+                    gA[i1 + a] += r * A[i2 + b] * A[i3 + c];
+                    gA[i2 + b] += r * A[i1 + a] * A[i3 + c];
+                    gA[i3 + c] += r * A[i1 + a] * A[i2 + b];
+                }
+    }
+    for (int k = 0; k < L4_COLS[l4max]; ++k, ++o) {
+        const int *l = L4_LIST[k];
+        for (int L = 0; L <= 2 * ORC_MAX_L34; ++L) {
+            if (!R4[k][L][0]) continue;
+            double complex uv[2][2 * 2 * ORC_MAX_L34 + 1];
+            pair_coupled(l[0], l[1], L, R4[k][L][0], A, uv[0]);
+            pair_coupled(l[2], l[3], L, R4[k][L][1], A, uv[1]);
+            const int nL = 2 * L + 1;
+            /* d(sum_M u_M v_M) = sum_M (du_M v_M + u_M dv_M) */
+            for (int h = 0; h < 2; ++h) {
+                const int a = l[2 * h], b = l[2 * h + 1], nb = 2 * b + 1;
+                const double complex *R = R4[k][L][h], *other = uv[1 - h];
+                for (int x = 0; x < 2 * a + 1; ++x)
+                    for (int y = 0; y < nb; ++y) {
+                        double complex t = 0.0;
+                        for (int M = 0; M < nL; ++M) t += R[(x * nb + y) * nL + M] * other[M];
+                        const double r = g[o] * creal(t);
+                        gA[a * a + x] += r * A[b * b + y];
+                        gA[b * b + y] += r * A[a * a + x];
+                    }
+            }
+        }
+    }
+}
+/* exported for the invariant pins: out has L3_COLS[l3max] + L4_COLS[l4max]
+ * values; gA (of the A row's size) receives sum_k g[k] d out[k] / dA */
+int orc_sh_invariants(int l3max, int l4max, const double *A, double *out) {
+    if (l3max < 0 || l3max > 6 || l4max < 0 || l4max > 3) return 1;
+    sh_invariants(l3max, l4max, A, out);
+    return 0;
+}
+int orc_sh_invariants_grad(int l3max, int l4max, const double *A, const double *g, double *gA) {
+    if (l3max < 0 || l3max > 6 || l4max < 0 || l4max > 3) return 1;
+    sh_invariants_grad(l3max, l4max, A, g, gA);
+    return 0;
+}
+
+/* ------------------------------------------------------------------ */
 /* Per-atom model pieces                                               */
 /* ------------------------------------------------------------------ */
 typedef struct {
     int K, P, M, L1, S, nl, radial;
+    int l3, l4, lmm, SL;   /* L3 / L4 lmax, the harmonics' lmax, columns per radial channel */
     double rc;
     const double *W;
     const int *h;
@@ -405,8 +609,15 @@ static sp_view basis_view(const orc_model *m, int ctype, int k) {
     v.K = m->nmax[b] + 1;
     v.P = m->size_np[b];
     v.L1 = m->lmax[b] + 1;
-    v.M = v.L1 * v.L1;
-    v.S = v.P * v.L1;
+    v.l3 = m->l3max ? m->l3max[b] : 0;
+    v.l4 = m->l4max ? m->l4max[b] : 0;
+    /* A and Y run to max(lmax, l3max, l4max) (SphericalChebyshev.java:36-42) */
+    v.lmm = v.L1 - 1;
+    if (v.l3 > v.lmm) v.lmm = v.l3;
+    if (v.l4 > v.lmm) v.lmm = v.l4;
+    v.M = (v.lmm + 1) * (v.lmm + 1);
+    v.SL = v.L1 + L3_COLS[v.l3] + L4_COLS[v.l4];
+    v.S = v.P * v.SL;
     v.rc = m->rcut[b];
     v.W = m->fp_param[b];
     v.nl = m->nlayers[t];
@@ -471,7 +682,8 @@ static void descriptor_radial_backward(const sp_view *v, const double *dx, const
     }
 }
 
-/* A[p][lm] = sum_j fc*Rnp[p]*Y[lm], Rnp = W[type_j] Rn;  fp[p*(lmax+1)+l] = 4pi/(2l+1) sum_m A^2 */
+/* A[p][lm] = sum_j fc*Rnp[p]*Y[lm], Rnp = W[type_j] Rn;  fp[p*SL+l] = 4pi/(2l+1) sum_m A^2 (l <= lmax),
+ * then the L3 and L4 invariants of A[p] (basis_SphericalChebyshev.hpp:344-350) */
 static void descriptor(const sp_view *v, int lmax, const double *dx, const double *dy, const double *dz,
                        const int *ty, int nn, double *A, double *fp) {
     memset(A, 0, sizeof(double) * (size_t)v->P * v->M);
@@ -480,7 +692,7 @@ static void descriptor(const sp_view *v, int lmax, const double *dx, const doubl
         double r = sqrt(dx[j] * dx[j] + dy[j] * dy[j] + dz[j] * dz[j]);
         if (r >= v->rc) continue;
         angles a = get_angles(dx[j], dy[j], dz[j], r);
-        real_sph(lmax, a.ct, a.st, a.cp, a.sp, y);
+        real_sph(v->lmm, a.ct, a.st, a.cp, a.sp, y);
         cal_rn(v->K - 1, r, v->rc, rn);
         double fc = cal_fc(r, v->rc);
         const double *W = v->W + (size_t)(ty[j] - 1) * v->P * v->K;
@@ -498,8 +710,10 @@ static void descriptor(const sp_view *v, int lmax, const double *dx, const doubl
                 double q = A[p * v->M + l * l + l + m];
                 s += q * q;
             }
-            fp[p * (lmax + 1) + l] = (4.0 * M_PI / (2.0 * l + 1.0)) * s;
+            fp[p * v->SL + l] = (4.0 * M_PI / (2.0 * l + 1.0)) * s;
         }
+    if (v->SL > lmax + 1)
+        for (int p = 0; p < v->P; ++p) sh_invariants(v->l3, v->l4, A + p * v->M, fp + p * v->SL + lmax + 1);
 }
 
 static double silu(double x) { return x / (1.0 + exp(-x)); }
@@ -569,11 +783,15 @@ static void descriptor_backward(const sp_view *v, int lmax, const double *dx, co
                                 const double *dz, const int *ty, int nn, const double *A, const double *gfp,
                                 double *gx, double *gy, double *gz) {
     double gA[ORC_MAX_P * ORC_MAX_M];
-    for (int p = 0; p < v->P; ++p)
+    for (int k = 0; k < v->P * v->M; ++k) gA[k] = 0.0;
+    for (int p = 0; p < v->P; ++p) {
         for (int l = 0; l <= lmax; ++l) {
-            double mul = (2.0 * 4.0 * M_PI / (2.0 * l + 1.0)) * gfp[p * (lmax + 1) + l];
+            double mul = (2.0 * 4.0 * M_PI / (2.0 * l + 1.0)) * gfp[p * v->SL + l];
             for (int m = -l; m <= l; ++m) gA[p * v->M + l * l + l + m] = mul * A[p * v->M + l * l + l + m];
         }
+        if (v->SL > lmax + 1)
+            sh_invariants_grad(v->l3, v->l4, A + p * v->M, gfp + p * v->SL + lmax + 1, gA + p * v->M);
+    }
     double rn[ORC_MAX_K], rng[ORC_MAX_K], y[ORC_MAX_M], yt[ORC_MAX_M];
     double rnp[ORC_MAX_P], grn[ORC_MAX_K], gy_[ORC_MAX_M];
     for (int j = 0; j < nn; ++j) {
@@ -581,7 +799,7 @@ static void descriptor_backward(const sp_view *v, int lmax, const double *dx, co
         double r = sqrt(dx[j] * dx[j] + dy[j] * dy[j] + dz[j] * dz[j]);
         if (r >= v->rc) continue;
         angles a = get_angles(dx[j], dy[j], dz[j], r);
-        real_sph(lmax, a.ct, a.st, a.cp, a.sp, y);
+        real_sph(v->lmm, a.ct, a.st, a.cp, a.sp, y);
         cal_rn(v->K - 1, r, v->rc, rn);
         double fc = cal_fc(r, v->rc);
         const double *W = v->W + (size_t)(ty[j] - 1) * v->P * v->K;
@@ -606,12 +824,12 @@ static void descriptor_backward(const sp_view *v, int lmax, const double *dx, co
         }
         cal_rn_grad(v->K - 1, r, v->rc, rng);
         double fcg = cal_fc_grad(r, v->rc);
-        y_dtheta(lmax, y, a.cp, a.sp, yt);
+        y_dtheta(v->lmm, y, a.cp, a.sp, yt);
         double gr = 0.0;
         for (int q = 0; q < v->K; ++q) gr += grn[q] * rng[q];
         gr += gfc * fcg;
         double gth = 0.0, gph = 0.0;
-        for (int l = 0; l <= lmax; ++l)
+        for (int l = 0; l <= v->lmm; ++l)
             for (int m = -l; m <= l; ++m) {
                 int k = l * l + l + m;
                 gth += gy_[k] * yt[k];

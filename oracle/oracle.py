@@ -34,6 +34,8 @@ class OrcModel(ctypes.Structure):
         ("radial", _c_int_p),
         ("part0", _c_int_p),
         ("nparts", _c_int_p),
+        ("l3max", _c_int_p),
+        ("l4max", _c_int_p),
     ]
 
 
@@ -95,9 +97,11 @@ class Oracle:
         self._normp = (_c_dbl_p * nt)(*[_dp(a) for a in self._norm])
         self._mirror = np.array([getattr(s, "mirror", 0) for s in sp], dtype=np.int32)
         self._radial = np.array([1 if b.radial else 0 for b in bl], dtype=np.int32)
+        self._l3 = np.array([getattr(b, "l3max", 0) for b in bl], dtype=np.int32)
+        self._l4 = np.array([getattr(b, "l4max", 0) for b in bl], dtype=np.int32)
         self.c = OrcModel(nt, _ip(self._nmax), _ip(self._lmax), _ip(self._P), _dp(self._rc), _ip(self._nl),
                           _ip(self._hid), self._Wp, self._nnp, self._normp, _ip(self._mirror), _ip(self._radial),
-                          _ip(self._part0), _ip(self._nparts))
+                          _ip(self._part0), _ip(self._nparts), _ip(self._l3), _ip(self._l4))
 
     # --- per-atom entry points (jse_nnap_calFp / calEnergy / calEnergyForce) ---
     def cal_fp(self, ctype, d, tj):
@@ -255,6 +259,29 @@ def real_sph(lmax, d):
     L.orc_real_sph(lmax, ctypes.c_double(d[0]), ctypes.c_double(d[1]), ctypes.c_double(d[2]), _dp(y), _dp(yt),
                    _dp(yp))
     return y, yt, yp
+
+
+L3_COLS = (0, 0, 2, 4, 9, 14, 23)
+L4_COLS = (0, 1, 3, 9)
+
+
+def sh_invariants(l3max: int, l4max: int, A):
+    """The L3 then L4 invariants of one radial channel's A row (index l^2+l+m)."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    out = np.zeros(L3_COLS[l3max] + L4_COLS[l4max])
+    if lib().orc_sh_invariants(l3max, l4max, _dp(A), _dp(out)):
+        raise ValueError("l3max / l4max out of range")
+    return out
+
+
+def sh_invariants_grad(l3max: int, l4max: int, A, g):
+    """sum_k g[k] d invariant_k / dA for one A row."""
+    A = np.ascontiguousarray(A, dtype=np.float64)
+    g = np.ascontiguousarray(g, dtype=np.float64)
+    gA = np.zeros_like(A)
+    if lib().orc_sh_invariants_grad(l3max, l4max, _dp(A), _dp(g), _dp(gA)):
+        raise ValueError("l3max / l4max out of range")
+    return gA
 
 
 def system_efv_par(orc: "Oracle", st, nthreads: int, cap: int = 256):

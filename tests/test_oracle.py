@@ -695,3 +695,150 @@ def test_nn_param_grad_finite_differences(O):
         lm = loss()
         sp.nn_param[p] = v
         assert abs((lp - lm) / (2 * h) - g[p]) < 1e-7 * max(1.0, abs(g[p]))
+
+
+# ------------------------------------------------------------ L3 / L4 invariants
+def _w3j000(a, b, c):
+    """(a b c; 0 0 0) in closed form (zero for odd a + b + c)."""
+    J = a + b + c
+    if J % 2 or c < abs(a - b) or c > a + b:
+        return 0.0
+    g = J // 2
+    f = math.factorial
+    return (-1) ** g * math.sqrt(f(J - 2 * a) * f(J - 2 * b) * f(J - 2 * c) / f(J + 1)) * f(g) / (
+        f(g - a) * f(g - b) * f(g - c))
+
+
+_L3_LIST = [(2, 2, 2), (1, 1, 2), (2, 3, 3), (1, 2, 3), (4, 4, 4), (2, 2, 4), (3, 3, 4), (2, 4, 4), (1, 3, 4),
+            (2, 5, 5), (4, 5, 5), (2, 3, 5), (1, 4, 5), (3, 4, 5), (6, 6, 6), (3, 3, 6), (4, 4, 6), (5, 5, 6),
+            (2, 6, 6), (4, 6, 6), (2, 4, 6), (1, 5, 6), (3, 5, 6)]
+_L4_LIST = [(1, 1, 1, 1), (2, 2, 2, 2), (1, 1, 2, 2), (3, 3, 3, 3), (1, 1, 3, 3), (2, 2, 3, 3), (1, 1, 1, 3),
+            (1, 3, 3, 3), (1, 2, 2, 3)]
+
+
+def test_l34_invariants_match_the_reference_tables(O):
+    """Every L3 / L4 invariant against values of the reference's own sparse
+    coefficient tables (basis_SphericalUtil0.hpp, evaluated as calL3Sub_ /
+    calL4Sub_ do) on seeded random A rows, written by
+    tests/golden/make_l34_vectors.py."""
+    g = np.load(os.path.join(GOLDEN, "l34_invariants.npz"))
+    for A, l3, l4 in zip(g["A"], g["l3"], g["l4"]):
+        got = O.sh_invariants(6, 3, A)
+        want = np.concatenate([l3, l4])
+        assert np.allclose(got, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+        # smaller l3max / l4max give the leading columns of each group
+        assert np.array_equal(O.sh_invariants(4, 2, A), np.concatenate([got[:9], got[23:26]]))
+
+
+def test_l34_single_neighbour_closed_form(O):
+    """One neighbour: A_lm = c Y_lm(u) for any direction u, so by rotation
+    invariance (u on the pole, Y_lm = delta_m0 sqrt((2l+1)/4pi))
+    L3 = c^3 (l1 l2 l3; 0 0 0) prod sqrt((2l+1)/4pi) and
+    L4 = c^4 sum_L (l1 l2 L; 0 0 0)(l3 l4 L; 0 0 0) prod sqrt((2l+1)/4pi)."""
+    rng = np.random.default_rng(8)
+    for _ in range(3):
+        u = rng.normal(size=3)
+        c = rng.uniform(0.5, 2.0)
+        A = c * O.real_sph(6, u)[0]
+        got = O.sh_invariants(6, 3, A)
+        for k, ls in enumerate(_L3_LIST):
+            nrm = np.prod([math.sqrt((2 * l + 1) / (4 * math.pi)) for l in ls])
+            assert got[k] == pytest.approx(c ** 3 * _w3j000(*ls) * nrm, rel=1e-12, abs=1e-14)
+        for k, (a, b, cc, d) in enumerate(_L4_LIST):
+            nrm = np.prod([math.sqrt((2 * l + 1) / (4 * math.pi)) for l in (a, b, cc, d)])
+            want = sum(_w3j000(a, b, L) * _w3j000(cc, d, L) for L in range(0, 7)) * nrm * c ** 4
+            assert got[23 + k] == pytest.approx(want, rel=1e-12, abs=1e-14)
+
+
+def test_l34_invariants_rotation_invariant_and_gradient(O):
+    """A summed over several neighbours: the invariants do not change when the
+    neighbours rotate (a wrong sign or index in the real-basis coupling breaks
+    this), and their gradient is that of central finite differences."""
+    from scipy.spatial.transform import Rotation
+    rng = np.random.default_rng(9)
+    d = rng.normal(size=(7, 3))
+    c = rng.uniform(-1, 1, size=7)
+    A = sum(cj * O.real_sph(6, dj)[0] for cj, dj in zip(c, d))
+    R = Rotation.random(random_state=5).as_matrix()
+    Ar = sum(cj * O.real_sph(6, R @ dj)[0] for cj, dj in zip(c, d))
+    v0 = O.sh_invariants(6, 3, A)
+    assert np.allclose(O.sh_invariants(6, 3, Ar), v0, rtol=1e-11, atol=1e-12 * np.abs(v0).max())
+    w = rng.normal(size=32)
+    gA = O.sh_invariants_grad(6, 3, A, w)
+    h = 1e-6
+    for m in range(49):
+        ap, am = A.copy(), A.copy()
+        ap[m] += h
+        am[m] -= h
+        fd = (w @ O.sh_invariants(6, 3, ap) - w @ O.sh_invariants(6, 3, am)) / (2 * h)
+        assert abs(fd - gA[m]) < 1e-7 * max(1.0, abs(gA[m]))
+
+
+def _l34_system(O, l3max, l4max, lmax, seed, wtype="default", symbols=("Cu", "Zr", "Al"), nmax=3):
+    """A 96-atom random packing and a random L3 / L4 model normalised on it."""
+    st = W.Structure(np.array([12.0, 12.5, 13.0]), W.rsa(96, [12.0, 12.5, 13.0], 2.0, seed),
+                     (np.arange(96) % len(symbols) + 1).astype(np.int32))
+    md = M.random_model(list(symbols), nmax, lmax, 5.0, (10, 10), seed=seed, wtype=wtype, l3max=l3max,
+                        l4max=l4max)
+    mdl = M.load_model(md)
+    fp = O.system_fp(O.Oracle(mdl), st)
+    fps = [fp[st.types == t + 1, :mdl.species[t].S] for t in range(mdl.ntypes)]
+    return st, M.load_model(M.set_normalization(md, fps))
+
+
+def test_l34_descriptor_layout(O):
+    """fp of a basis with l3max / l4max: per radial channel the lmax + 1 L2
+    values, then the L3 and the L4 invariants of that channel's A, whose
+    harmonics run to max(lmax, l3max, l4max) (basis_SphericalChebyshev.hpp:
+    321-351, SphericalChebyshev.java:36-42)."""
+    mdl = M.load_model(M.random_model(["Cu", "Zr"], 3, 2, 5.0, (6,), seed=2, l3max=4, l4max=3))
+    sp = mdl.species[0]
+    assert sp.bases()[0].SL == 3 + 9 + 9 and sp.S == sp.P * 21 and sp.M == 25
+    orc = O.Oracle(mdl)
+    rng = np.random.default_rng(4)
+    d = rng.uniform(-3, 3, size=(12, 3))
+    tj = rng.integers(1, 3, size=12)
+    fp = orc.cal_fp(1, d, tj).reshape(sp.P, 21)
+    r = np.linalg.norm(d, axis=1)
+    keep = r < sp.rcut
+    d, tj, r = d[keep], tj[keep], r[keep]
+    x = 1 - 2 * r / sp.rcut
+    T = np.array([np.cos(n * np.arccos(x)) for n in range(sp.K)]).T
+    fc = (1 - (r / sp.rcut) ** 2) ** 4
+    c = np.einsum("jpk,jk->jp", sp.W[tj - 1], T) * fc[:, None]
+    Y = np.array([O.real_sph(4, v)[0] for v in d])
+    A = c.T @ Y
+    for p in range(sp.P):
+        for l in range(3):
+            assert fp[p, l] == pytest.approx(4 * math.pi / (2 * l + 1) * (A[p, l * l:(l + 1) ** 2] ** 2).sum(),
+                                             rel=1e-12, abs=1e-14)
+        Ap = np.zeros(49)
+        Ap[:25] = A[p]
+        assert np.allclose(fp[p, 3:], O.sh_invariants(4, 3, Ap), rtol=1e-11, atol=1e-14)
+
+
+@pytest.mark.parametrize("l3max,l4max,lmax", [(4, 3, 3), (6, 0, 2), (0, 2, 4)])
+def test_l34_forces_are_minus_energy_gradient(O, l3max, l4max, lmax):
+    """The descriptor backward through the L3 / L4 invariants against central
+    finite differences of the total energy."""
+    st, mdl = _l34_system(O, l3max, l4max, lmax, 21 + l3max)
+    orc = O.Oracle(mdl)
+    res = orc.system_efv(st)
+    h = 1e-5
+    for i in (0, 31):
+        for c in range(3):
+            xp = st.xyz.copy(); xp[i, c] += h
+            xm = st.xyz.copy(); xm[i, c] -= h
+            ep = orc.system_efv(W.Structure(st.box, xp, st.types))["energy"]
+            em = orc.system_efv(W.Structure(st.box, xm, st.types))["energy"]
+            assert abs(-(ep - em) / (2 * h) - res["forces"][i, c]) < 2e-7 * max(1, abs(res["forces"][i, c]))
+
+
+def test_l34_loader_limits():
+    """SphericalChebyshev.java:32-34: l3max in [0, 6], l4max in [0, 3]."""
+    base = M.random_model(["Cu"], 2, 2, 5.0, (4,), seed=1)
+    for key, val, msg in (("l3max", 7, "l3max"), ("l4max", 4, "l4max"), ("l3max", -1, "l3max")):
+        md = json.loads(json.dumps(base))
+        md["models"][0]["basis"][key] = val
+        with pytest.raises(ValueError, match=msg):
+            M.load_model(md)
