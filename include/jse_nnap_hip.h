@@ -63,6 +63,13 @@ int jse_nnap_model_parse(const char *json_or_path, int *ntypes, int *size_np, in
 int jse_nnap_model_parse_blocks(const char *json_or_path, int ctype, double *fp_param, double *nn_param,
                                 double *norm_param, int *nlayers, int *hidden, int *mirror);
 
+/* Host-only: the L3 then L4 invariants of one radial channel's A row, as
+ * calSphL3 / calSphL4 form them (basis_SphericalUtil.hpp:395-467), from the
+ * coupling terms the descriptor kernels use.  A holds (L+1)^2 values,
+ * L = max(l3max, l4max), at index l^2+l+m; out receives 0, 0, 2, 4, 9, 14, 23
+ * values for l3max 0..6 followed by 0, 1, 3, 9 for l4max 0..3.  No device use. */
+int jse_nnap_sh_invariants(int l3max, int l4max, const double *A, double *out);
+
 /* Replaces the packed parameter blocks the reference hands to its native
  * code (NNAP.java:226-307): per species t (0-based here):
  *   fp_param[t]   = W[ntypes][size_np[t]][nmax[t]+1]      (WTypeBasis.java:202-304)

@@ -16,6 +16,7 @@
 #include "capi_internal.h"
 #include "common.h"
 #include "kernels.h"
+#include "sh_coupling.h"
 
 using namespace jnn;
 
@@ -90,13 +91,36 @@ void jse_nnap_model_s::upload() {
         for (size_t q = 0; q < wb.size(); ++q) bb[q] = to_bf16(wb[q]);
         d.wb_fwd = dupload(bf);
         d.wb_bwd = dupload(bb);
-        if (!sp.parts.empty()) {  // a merge basis: the parts' mixings and G's per-column factor
+        for (const SpeciesHost &q : sp.parts) {  // a merge basis: the parts' mixings
+            d.pW.push_back(dupload(std::vector<float>(q.W.begin(), q.W.end())));
+            d.pW64.push_back(dupload(q.W));
+        }
+        bool any34 = false;
+        d.cpl.resize(sp.nbases());
+        for (int k = 0; k < sp.nbases(); ++k) {  // L3 / L4 coupling terms per basis
+            const SpeciesHost &q = sp.basis(k);
+            if (!q.l34()) continue;
+            any34 = true;
+            const CouplingTerms ct = coupling_terms(q.l3max, q.l4max, q.M());
+            SpeciesDev::Coupling &c = d.cpl[k];
+            c.n34 = ct.ninv;
+            c.idx = reinterpret_cast<int4 *>(dupload(ct.idx));
+            c.gidx = reinterpret_cast<int4 *>(dupload(ct.gidx));
+            c.coef = dupload(ct.coef);
+            c.gcoef = dupload(ct.gcoef);
+            c.off = dupload(ct.off);
+            c.goff = dupload(ct.goff);
+        }
+        if (!sp.parts.empty() || any34) {
+            // G's per-column factor: 2 4pi/(2l+1) on a spherical basis' L2 columns, 1 on
+            // its L3 / L4 columns and on a radial basis'
             std::vector<double> l2;
-            for (const SpeciesHost &q : sp.parts) {
-                d.pW.push_back(dupload(std::vector<float>(q.W.begin(), q.W.end())));
-                d.pW64.push_back(dupload(q.W));
-                for (int e = 0; e < q.P * (q.lmax + 1); ++e)
-                    l2.push_back(q.radial ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * (e % (q.lmax + 1)) + 1));
+            for (int k = 0; k < sp.nbases(); ++k) {
+                const SpeciesHost &q = sp.basis(k);
+                for (int e = 0; e < q.P * q.SL(); ++e) {
+                    const int c = e % q.SL();
+                    l2.push_back(q.radial || c > q.lmax ? 1.0 : 2.0 * 12.566370614359172954 / (double)(2 * c + 1));
+                }
             }
             d.l2f = dupload(l2);
         }
@@ -117,6 +141,9 @@ jse_nnap_model_s::~jse_nnap_model_s() {
         for (float *p : d.pW) dfree(p);
         for (double *p : d.pW64) dfree(p);
         dfree(d.l2f);
+        for (SpeciesDev::Coupling &c : d.cpl) {
+            dfree(c.idx); dfree(c.gidx); dfree(c.coef); dfree(c.gcoef); dfree(c.off); dfree(c.goff);
+        }
     }
     dfree(zeros);
     dfree(ones);
@@ -130,8 +157,8 @@ void check_model_shapes(const jse_nnap_model_s *m, int precision) {
     for (int t = 0; t < nt; ++t) {
         for (int k = 0; k < m->host.species[t].nbases(); ++k) {
             const SpeciesHost &sp = m->host.species[t].basis(k);
-            const int lm = sp.radial ? kRadialLmax : sp.lmax;
-            if (desc_tiled(lm, sp.K(), sp.P, sp.mix)) continue;
+            const int lm = desc_lmax(sp);
+            if (!sp.l34() && desc_tiled(lm, sp.K(), sp.P, sp.mix)) continue;
             const std::string why = desc_gen_limit(lm, sp.K(), sp.P, nt, precision_acc(precision));
             JNN_CHECK(why.empty(), ERR_UNSUPPORTED,
                       "species " + std::to_string(t + 1) + " (lmax=" + std::to_string(sp.lmax) + " nmax=" +
@@ -160,7 +187,21 @@ SegmentArgs species_segment(const jse_nnap_model_s *m, int t, int precision, con
     a.fp32_acc = a.acc ? 0 : 1;
     a.xs = sp.S();
     a.accum_g = 0;
+    set_segment_basis(a, m, t, 0);
     return a;
+}
+void set_segment_basis(SegmentArgs &a, const jse_nnap_model_s *m, int t, int k) {
+    const SpeciesHost &b = m->host.species[t].basis(k);
+    const SpeciesDev::Coupling &c = m->dev[t].cpl[k];
+    a.l2n = b.lmax + 1;
+    a.sl = b.SL();
+    a.n34 = c.n34;
+    a.cpl_idx = c.idx;
+    a.cplg_idx = c.gidx;
+    a.cpl_coef = c.coef;
+    a.cplg_coef = c.gcoef;
+    a.cpl_off = c.off;
+    a.cplg_off = c.goff;
 }
 MlpArgs mlp_args(const jse_nnap_model_s *m, int t, const SegmentArgs &sa, double *E, bool grad, int wg_per_cu) {
     const SpeciesHost &sp = m->host.species[t];
@@ -206,7 +247,8 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
     const int nt = m->host.ntypes();
     auto same_shape = [&](int t, int u) {
         const SpeciesHost &a = m->host.species[t], &b = m->host.species[u];
-        return a.radial == b.radial && a.lmax == b.lmax && a.K() == b.K() && a.P == b.P && a.mix == b.mix;
+        return a.radial == b.radial && a.lmax == b.lmax && a.l3max == b.l3max && a.l4max == b.l4max &&
+               a.K() == b.K() && a.P == b.P && a.mix == b.mix;
     };
     int first = -1;
     bool one = nt <= kSegMax, merged = false;
@@ -250,12 +292,13 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
                     a.G += xoff;
                     a.A += aoff;
                     a.accum_g = k > 0 ? 1 : 0;
+                    set_segment_basis(a, m, t, k);
                 }
-                const int lm = b.radial ? kRadialLmax : b.lmax;
+                const int lm = desc_lmax(b);
                 const bool ok = backward ? launch_desc_bwd(s, lm, b.K(), b.P, b.mix, a, nl)
                                          : launch_desc_fwd(s, lm, b.K(), b.P, b.mix, a, nl);
                 if (!ok) fail(t);
-                xoff += b.P * (b.lmax + 1);
+                xoff += b.P * b.SL();
                 aoff += (size_t)segs[t].nrows * b.P * b.M();
             }
         }
@@ -273,7 +316,7 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
         }
         set.row0[set.nseg] = rows;
         const SpeciesHost &sp = m->host.species[first];
-        const int lm = sp.radial ? kRadialLmax : sp.lmax;
+        const int lm = desc_lmax(sp);
         const bool ok = backward ? launch_desc_bwd_set(s, lm, sp.K(), sp.P, sp.mix, set, nl)
                                  : launch_desc_fwd_set(s, lm, sp.K(), sp.P, sp.mix, set, nl);
         if (!ok) fail(first);
@@ -282,7 +325,7 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
     for (int t = 0; t < nt; ++t) {
         if (segs[t].nrows <= 0) continue;
         const SpeciesHost &sp = m->host.species[t];
-        const int lm = sp.radial ? kRadialLmax : sp.lmax;
+        const int lm = desc_lmax(sp);
         const bool ok = backward ? launch_desc_bwd(s, lm, sp.K(), sp.P, sp.mix, segs[t], nl)
                                  : launch_desc_fwd(s, lm, sp.K(), sp.P, sp.mix, segs[t], nl);
         if (!ok) fail(t);
@@ -833,6 +876,24 @@ int jse_nnap_model_fp_size(jse_nnap_model m, int ctype, int *size) {
     });
 }
 void jse_nnap_model_destroy(jse_nnap_model m) { delete m; }
+
+int jse_nnap_sh_invariants(int l3max, int l4max, const double *A, double *out) {
+    return guard([&] {
+        JNN_CHECK(A && out, ERR_ARG, "null argument");
+        const int L = std::max(l3max, l4max);
+        const CouplingTerms t = coupling_terms(l3max, l4max, (L + 1) * (L + 1));
+        for (int k = 0; k < t.ninv; ++k) {
+            double f = 0.0;
+            for (int q = t.off[k]; q < t.off[k + 1]; ++q) {
+                const int *ix = &t.idx[4 * (size_t)q];
+                double v = t.coef[q] * A[ix[0]] * A[ix[1]] * A[ix[2]];
+                if (ix[3] >= 0) v *= A[ix[3]];
+                f += v;
+            }
+            out[k] = f;
+        }
+    });
+}
 
 int jse_nnap_system_create(jse_nnap_model m, int natoms, const int *types, const double *box, int precision,
                            jse_nnap_system *out) {

@@ -104,10 +104,19 @@ struct SpeciesDev {
     float *w_fwd = nullptr, *w_bwd = nullptr, *bias = nullptr;
     double *w_fwd64 = nullptr, *w_bwd64 = nullptr, *bias64 = nullptr;
     uint16_t *wb_fwd = nullptr, *wb_bwd = nullptr;
-    // a merge basis: each part's mixing (FP32, FP64) and the MLP's per-column G factor
+    // a merge basis: each part's mixing (FP32, FP64); the MLP's per-column G factor
+    // (a merge basis, or a basis with L3 / L4 invariants)
     std::vector<float *> pW;
     std::vector<double *> pW64;
     double *l2f = nullptr;
+    // per basis (basis(k)): the L3 / L4 coupling terms (n34 = 0: none)
+    struct Coupling {
+        int n34 = 0;
+        int4 *idx = nullptr, *gidx = nullptr;
+        double *coef = nullptr, *gcoef = nullptr;
+        int *off = nullptr, *goff = nullptr;
+    };
+    std::vector<Coupling> cpl;
 };
 
 }  // namespace jnn
@@ -136,6 +145,11 @@ void check_model_shapes(const jse_nnap_model_s *m, int precision);
 // The descriptor fields of species t's segment shared by every caller (rows,
 // outputs and launch shape are the caller's).
 SegmentArgs species_segment(const jse_nnap_model_s *m, int t, int precision, const int *types);
+// The column layout and L3 / L4 coupling terms of basis k of species t.
+void set_segment_basis(SegmentArgs &a, const jse_nnap_model_s *m, int t, int k);
+// The lmax the descriptor kernels are chosen by: max(lmax, l3max, l4max), or
+// kRadialLmax for the radial basis.
+inline int desc_lmax(const SpeciesHost &b) { return b.radial ? kRadialLmax : b.lmm(); }
 // The MlpArgs of species t's segment (its rows, descriptors and gradients).
 MlpArgs mlp_args(const jse_nnap_model_s *m, int t, const SegmentArgs &sa, double *E, bool grad, int wg_per_cu = 0);
 // Points the segment's descriptor rows at row0 of a buffer of smax values per

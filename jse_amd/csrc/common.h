@@ -42,6 +42,10 @@ enum Status : int {
 enum MixKind : int { MIX_NONE = 0, MIX_DEFAULT = 1, MIX_DENSE = 2 };
 
 constexpr int kMaxLayers = 8;
+// columns the L3 / L4 invariants add per radial channel, by l3max (0..6) and
+// l4max (0..3) (SphericalChebyshev.java:15-16)
+constexpr int kL3Cols[7] = {0, 0, 2, 4, 9, 14, 23};
+constexpr int kL4Cols[4] = {0, 1, 3, 9};
 constexpr int kMaxSpecies = 15;  // a neighbour's species travels in 4 bits of its pair tag
 
 // One species of a model, FP64 blocks in the reference's packing plus derived shapes.
@@ -53,8 +57,14 @@ struct SpeciesHost {
     std::vector<double> W;            // [ntypes][P][K]
     std::vector<double> nn;           // [hidden W | out W | hidden b | out b]
     std::vector<double> norm;         // [muE+refE, sigmaE, mu[S], sigma[S]]
+    int l3max = 0, l4max = 0;         // L3 / L4 invariants (SphericalChebyshev.java:33-42)
     int K() const { return nmax + 1; }
-    int M() const { return (lmax + 1) * (lmax + 1); }
+    // the harmonics' and A's lmax: max(lmax, l3max, l4max)
+    int lmm() const { return lmax > l3max ? (lmax > l4max ? lmax : l4max) : (l3max > l4max ? l3max : l4max); }
+    int M() const { return (lmm() + 1) * (lmm() + 1); }
+    bool l34() const { return l3max > 0 || l4max > 0; }
+    // descriptor columns per radial channel: lmax + 1 L2 values, then the L3 and L4 invariants
+    int SL() const { return lmax + 1 + kL3Cols[l3max] + kL4Cols[l4max]; }
     int mix = MIX_DENSE;              // kernel specialisation for W
     int mirror = 0;                   // mirror basis: for this species' centres, neighbour species
                                       // `mirror` and this species swap roles (0: none)
@@ -65,9 +75,9 @@ struct SpeciesHost {
     std::vector<SpeciesHost> parts;
     // descriptor size (all parts of a merge basis)
     int S() const {
-        if (parts.empty()) return P * (lmax + 1);
+        if (parts.empty()) return P * SL();
         int s = 0;
-        for (const SpeciesHost &q : parts) s += q.P * (q.lmax + 1);
+        for (const SpeciesHost &q : parts) s += q.P * q.SL();
         return s;
     }
     // the bases evaluated for this species, in fp order

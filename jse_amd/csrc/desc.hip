@@ -959,6 +959,7 @@ static void run_bwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl
 bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
                          const NeighborList &nl) {
     if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
+    if (set.a[0].n34 > 0) return launch_desc_gen_set(s, lmax, K, P, set, nl, false);  // L3 / L4 invariants
     if (lmax == kRadialLmax)
         return launch_rad_set(s, K, P, mix, set, nl, false) || launch_desc_gen_set(s, lmax, K, P, set, nl, false);
 #define X(L_, K_, P_, M_)                                              \
@@ -973,6 +974,7 @@ bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const S
 bool launch_desc_bwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
                          const NeighborList &nl) {
     if (set.nseg <= 0 || set.row0[set.nseg] <= 0) return true;
+    if (set.a[0].n34 > 0) return launch_desc_gen_set(s, lmax, K, P, set, nl, true);
     if (lmax == kRadialLmax)
         return launch_rad_set(s, K, P, mix, set, nl, true) || launch_desc_gen_set(s, lmax, K, P, set, nl, true);
 #define X(L_, K_, P_, M_)                                              \

@@ -13,10 +13,14 @@
 // round-robin: per neighbour a tile costs two 16-byte LDS reads and 16 FP64
 // multiply-adds (every c * Y product of two floats is exact in FP64), summed
 // over the chunk in list order in registers and then added into the FP64
-// copy of A in LDS.  fp and x are formed in FP64.
+// copy of A in LDS.  fp and x are formed in FP64: per radial channel the L2
+// power spectrum, then (l3max / l4max > 0) the L3 and L4 invariants of the
+// channel's A row from their coupling terms (basis_SphericalUtil.hpp:395-467;
+// A and Y then run to max(lmax, l3max, l4max), the kernels' LMAX).
 //
-// Backward, one lane per neighbour: dE/dA[p][m] = Gc[p][l(m)] A[p][m] goes to
-// LDS once per atom and is folded through every species' mixing, Gt[t][n][m]
+// Backward, one lane per neighbour: dE/dA[p][m] = Gc[p][l(m)] A[p][m] (plus
+// the L3 / L4 invariants' terms) goes to LDS once per atom and is folded
+// through every species' mixing, Gt[t][n][m]
 // = sum_p W[t][p][n] dE/dA[p][m] (c_j is linear in Rn), as in the tiled
 // backward; a lane then forms gY[m] = fc sum_n Rn[n] Gt[n][m] and gct[n] =
 // sum_m Gt[n][m] Y[m] (2KM multiply-adds per pair for any P, rows read 16
@@ -84,7 +88,7 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
     ST *sC = sY + ch * MS;
     ST *sW = sC + ch * PS;
     const int lane = threadIdx.x;
-    const int PM = P * M, S = P * L1;
+    const int PM = P * M;
     const int ntile = (PS / kTile) * NMB;
     const int wsz = set.a[0].ntypes * P * K;
     int cur = -1;
@@ -170,15 +174,26 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
         }
         __syncthreads();
         for (int e = lane; e < PM; e += kFwdThreads) a.A[(size_t)row * PM + e] = (float)sA[e];
-        for (int e = lane; e < S; e += kFwdThreads) {
-            const int p = e / L1, l = e - p * L1;
-            double s = 0.0;
-            for (int m = l * l; m < (l + 1) * (l + 1); ++m) {
-                const double v = sA[p * M + m];
-                s += v * v;
+        const int sl = a.sl, l2n = a.l2n;
+        for (int e = lane; e < P * sl; e += kFwdThreads) {
+            const int p = e / sl, l = e - p * sl;
+            const double *Ap = sA + p * M;
+            double f;
+            if (l < l2n) {
+                double s = 0.0;
+                for (int m = l * l; m < (l + 1) * (l + 1); ++m) s += Ap[m] * Ap[m];
+                f = 4.0 * 3.14159265358979323846 / (double)(2 * l + 1) * s;
+            } else {  // an L3 / L4 invariant
+                const int k = l - l2n;
+                f = 0.0;
+                for (int q = a.cpl_off[k]; q < a.cpl_off[k + 1]; ++q) {
+                    const int4 ix = a.cpl_idx[q];
+                    double v = a.cpl_coef[q] * Ap[ix.x] * Ap[ix.y] * Ap[ix.z];
+                    if (ix.w >= 0) v *= Ap[ix.w];
+                    f += v;
+                }
             }
-            const double w4 = 4.0 * 3.14159265358979323846 / (double)(2 * l + 1);
-            store_x(a, (size_t)row * a.xs + e, ((w4 * s - a.mu64[e]) * (1.0 / a.sig64[e])));
+            store_x(a, (size_t)row * a.xs + e, ((f - a.mu64[e]) * (1.0 / a.sig64[e])));
         }
     }
 }
@@ -198,7 +213,7 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
     float *sGt = sGA + P * MP;
     float *sW = sGt + nt * K * MP;
     const int lane = threadIdx.x;
-    const int PM = P * M, S = P * L1;
+    const int PM = P * M;
     const int wsz = nt * P * K;
     int cur = -1;
     for (int gr = blockIdx.x; gr < set.row0[set.nseg]; gr += gridDim.x) {
@@ -218,7 +233,23 @@ __global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl,
             const int p = e / MP, m = e - p * MP;
             int l = 0;
             while ((l + 1) * (l + 1) <= m) ++l;
-            sGA[e] = m < M ? a.G[(size_t)row * a.xs + p * L1 + l] * a.A[(size_t)row * PM + p * M + m] : 0.f;
+            float v = 0.f;
+            if (m < M) {
+                const float *Gp = a.G + (size_t)row * a.xs + p * a.sl;
+                const float *Ap = a.A + (size_t)row * PM + p * M;
+                if (l < a.l2n) v = Gp[l] * Ap[m];
+                if (a.n34) {  // the L3 / L4 invariants' dE/dA (G = dE/dfp on their columns)
+                    double t = 0.0;
+                    for (int q = a.cplg_off[m]; q < a.cplg_off[m + 1]; ++q) {
+                        const int4 ix = a.cplg_idx[q];
+                        double u = a.cplg_coef[q] * (double)Gp[a.l2n + ix.w] * (double)Ap[ix.x] * (double)Ap[ix.y];
+                        if (ix.z >= 0) u *= (double)Ap[ix.z];
+                        t += u;
+                    }
+                    v += (float)t;
+                }
+            }
+            sGA[e] = v;
         }
         __syncthreads();
         for (int e = lane; e < nt * K * MP; e += 64) {

@@ -164,6 +164,17 @@ struct SegmentArgs {
     int xs;                 // row stride of x and G: the species' descriptor size (a merge basis' part
                             // writes its slice of the species' row; x, G, mu and sigma then point at it)
     int accum_g;            // backward: add into the pairs' g (the parts of a merge basis after the first)
+    // descriptor columns: per radial channel l2n L2 values (lmax + 1), then n34 L3 / L4
+    // invariants, sl = l2n + n34 columns in all; A and the harmonics run to
+    // max(lmax, l3max, l4max).  Invariant k of a channel's A row is the sum over
+    // terms off[k] .. off[k+1]-1 of coef A[i.x] A[i.y] A[i.z] (times A[i.w] when
+    // i.w >= 0); dE/dA[m] gains, over terms goff[m] .. goff[m+1]-1, gcoef G[k]
+    // A[i.x] A[i.y] (times A[i.z] when i.z >= 0) with k = i.w.  Only the untiled
+    // kernels evaluate n34 > 0.
+    int l2n, sl, n34;
+    const int4 *cpl_idx, *cplg_idx;
+    const double *cpl_coef, *cplg_coef;
+    const int *cpl_off, *cplg_off;
 };
 // g of a pair: stored, or added to what an earlier part of a merge basis stored
 __device__ __forceinline__ void put_g(const NeighborList &nl, size_t k, float gx, float gy, float gz, int accum) {

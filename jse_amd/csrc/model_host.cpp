@@ -168,6 +168,8 @@ static void set_basis_fields(SpeciesHost &dst, const SpeciesHost &src) {
     dst.rcut = src.rcut;
     dst.W = src.W;
     dst.radial = src.radial;
+    dst.l3max = src.l3max;
+    dst.l4max = src.l4max;
 }
 
 // SphericalChebyshev.load / Chebyshev.load (SphericalChebyshev.java:60-103,
@@ -181,10 +183,13 @@ static void parse_basis(const JVal *b, const std::string &type, int ntypes, Spec
     sp.nmax = (int)num(b->get("nmax"), 5);
     sp.lmax = sp.radial ? 0 : (int)num(b->get("lmax"), 6);
     JNN_CHECK(sp.lmax >= 0 && sp.lmax <= 12, ERR_ARG, "Input lmax MUST be in [0, 12], input: " + std::to_string(sp.lmax));
+    // SphericalChebyshev.java:33-34 (the radial basis has no angular part)
+    sp.l3max = sp.radial ? 0 : (int)num(b->get("l3max"), 0);
+    sp.l4max = sp.radial ? 0 : (int)num(b->get("l4max"), 0);
+    JNN_CHECK(sp.l3max >= 0 && sp.l3max <= 6, ERR_ARG, "Input l3max MUST be in [0, 6], input: " + std::to_string(sp.l3max));
+    JNN_CHECK(sp.l4max >= 0 && sp.l4max <= 3, ERR_ARG, "Input l4max MUST be in [0, 3], input: " + std::to_string(sp.l4max));
     JNN_CHECK(sp.nmax >= 0 && sp.nmax <= 20, ERR_ARG,  // WTypeBasis.java:56
               "Input nmax MUST be in [0, 20], input: " + std::to_string(sp.nmax));
-    JNN_CHECK(sp.radial || ((int)num(b->get("l3max"), 0) == 0 && (int)num(b->get("l4max"), 0) == 0), ERR_UNSUPPORTED,
-              "l3max/l4max > 0 are not supported by this build");
     sp.rcut = num(b->get("rcut"), 6.0);
     JNN_CHECK(sp.rcut > 0, ERR_ARG, "rcut must be positive");
     int wtype = WT_DEFAULT;

@@ -52,6 +52,7 @@ _c_ll_p = ctypes.POINTER(ctypes.c_longlong)
 
 EXPORTS = [
     "jse_nnap_last_error", "jse_nnap_model_load", "jse_nnap_model_parse", "jse_nnap_model_parse_blocks",
+    "jse_nnap_sh_invariants",
     "jse_nnap_model_create", "jse_nnap_model_info",
     "jse_nnap_model_fp_size", "jse_nnap_model_destroy", "jse_nnap_system_create", "jse_nnap_system_destroy",
     "jse_nnap_system_set_box", "jse_nnap_system_set_positions", "jse_nnap_system_set_positions_device",
@@ -93,6 +94,7 @@ def lib():
     L.jse_nnap_model_parse.argtypes = [ctypes.c_char_p, _c_int_p, _c_int_p, _c_int_p]
     L.jse_nnap_model_parse_blocks.argtypes = [ctypes.c_char_p, ctypes.c_int, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_int_p,
                                               _c_int_p, _c_int_p]
+    L.jse_nnap_sh_invariants.argtypes = [ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_dbl_p]
     L.jse_nnap_model_create.argtypes = [ctypes.c_int, _c_int_p, _c_int_p, _c_int_p, _c_dbl_p, _c_int_p, _c_int_p,
                                         ctypes.POINTER(_c_dbl_p), ctypes.POINTER(_c_dbl_p), ctypes.POINTER(_c_dbl_p),
                                         ctypes.POINTER(vp)]

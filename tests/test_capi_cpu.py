@@ -386,3 +386,43 @@ def test_cpp_parser_merge_basis():
         assert _parse(json.dumps(m2))[0] != 0 and msg in nnap.lib().jse_nnap_last_error().decode()
         with pytest.raises(ValueError, match=msg):
             M.load_model(m2)
+
+
+def test_coupling_terms_match_the_reference_tables():
+    """The L3 / L4 coupling terms the descriptor kernels read, evaluated on the
+    host, against values of the reference's own coefficient tables
+    (basis_SphericalUtil0.hpp, tests/golden/make_l34_vectors.py)."""
+    from conftest import GOLDEN
+    g = np.load(os.path.join(GOLDEN, "l34_invariants.npz"))
+    L = nnap.lib()
+    for A, l3, l4 in zip(g["A"], g["l3"], g["l4"]):
+        A = np.ascontiguousarray(A)
+        out = np.zeros(32)
+        dp = ctypes.POINTER(ctypes.c_double)
+        assert L.jse_nnap_sh_invariants(6, 3, A.ctypes.data_as(dp), out.ctypes.data_as(dp)) == 0
+        want = np.concatenate([l3, l4])
+        assert np.allclose(out, want, rtol=1e-13, atol=1e-13 * np.abs(want).max())
+        small = np.zeros(12)
+        A5 = np.ascontiguousarray(A[:25])
+        assert L.jse_nnap_sh_invariants(4, 2, A5.ctypes.data_as(dp), small.ctypes.data_as(dp)) == 0
+        assert np.allclose(small, np.concatenate([out[:9], out[23:26]]), rtol=1e-13, atol=1e-12)
+    out = np.zeros(32)
+    dp = ctypes.POINTER(ctypes.c_double)
+    assert L.jse_nnap_sh_invariants(7, 0, out.ctypes.data_as(dp), out.ctypes.data_as(dp)) != 0
+
+
+def test_cpp_parser_l34():
+    """l3max / l4max (SphericalChebyshev.java:32-42): both loaders size the
+    descriptor as P (lmax + 1 + L3 columns + L4 columns) and refuse values out
+    of range with the reference's messages."""
+    md = M.random_model(["Cu", "Zr"], 3, 2, 5.0, (8,), seed=2, l3max=5, l4max=2)
+    rc, nt, P, S = _parse(json.dumps(md))
+    assert rc == 0, nnap.lib().jse_nnap_last_error()
+    mdl = _both_equal(md)
+    assert S == [s.S for s in mdl.species] == [P[0] * (3 + 14 + 3)] * 2
+    for key, val, msg in (("l3max", 7, "l3max MUST be in [0, 6]"), ("l4max", 4, "l4max MUST be in [0, 3]")):
+        m2 = json.loads(json.dumps(md))
+        m2["models"][0]["basis"][key] = val
+        assert _parse(json.dumps(m2))[0] != 0 and msg in nnap.lib().jse_nnap_last_error().decode()
+        with pytest.raises(ValueError, match=msg.split(" MUST")[0]):
+            M.load_model(m2)

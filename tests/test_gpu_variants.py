@@ -4,8 +4,8 @@
 nmax = 5 (one species: no mixing; three species: the default mixing),
 mirror / share bases, a post-fuse weight, the radial-only chebyshev basis,
 shapes without a tiled kernel (the untiled kernels: lmax 0-12, nmax up to
-20, any wtype), merge bases, and a network without matrix-core tiles (the
-vector MLP).
+20, any wtype), merge bases, L3 / L4 invariants, and a network without
+matrix-core tiles (the vector MLP).
 Plain path bar: 1e-6 eV/atom, every force component within 1e-5 eV/A,
 1e-5 eV/atom virial."""
 import numpy as np
@@ -215,5 +215,48 @@ def test_merge_basis_fp32_fast(O):
     ref = O.Oracle(M.load_model(md)).system_efv(st)
     df = res["forces"] - ref["forces"]
     print(f"f_rms {np.sqrt((df ** 2).mean()):.2e} f_max {np.abs(df).max():.2e}")
+    assert abs(res["energy"] - ref["energies"].sum()) / st.natoms <= 1e-6
+    assert np.sqrt((df ** 2).mean()) <= 2e-5 and np.abs(df).max() <= 2e-4
+
+
+def _normalized_by_oracle(O, md, st):
+    """Normalisation from the oracle's descriptors of every atom (the L3 / L4
+    columns have no numpy restatement)."""
+    mdl = M.load_model(md)
+    fp = O.system_fp(O.Oracle(mdl), st)
+    return M.set_normalization(md, [fp[st.types == t + 1, :mdl.species[t].S] for t in range(mdl.ntypes)])
+
+
+@pytest.mark.parametrize("symbols,nmax,lmax,l3max,l4max,wtype,hidden", [
+    (["Cu", "Zr", "Al"], 5, 4, 4, 2, "default", (64, 64)),   # P = 12, S = 12 (5 + 9 + 3) = 204
+    (["Cu"], 4, 2, 6, 3, "none", (32, 32)),                  # harmonics to l = 6 for L3, S = 5 (3 + 23 + 9)
+    (["Cu", "Zr"], 3, 6, 2, 1, "full", (32, 32)),            # P = 8, S = 8 (7 + 2 + 1)
+])
+def test_l34_invariants(O, symbols, nmax, lmax, l3max, l4max, wtype, hidden):
+    """l3max / l4max > 0 (basis_SphericalUtil.hpp:395-467): the untiled kernels
+    form each channel's L3 / L4 invariants from their coupling terms in the
+    forward and add their dE/dA in the backward.  Plain path at the full bar."""
+    st = _liquid(symbols, 800, 41)
+    md = M.random_model(symbols, nmax=nmax, lmax=lmax, rcut=6.0, hidden=hidden, seed=43, wtype=wtype,
+                        l3max=l3max, l4max=l4max)
+    _check(O, _normalized_by_oracle(O, md, st), st)
+
+
+def test_l34_invariants_fp32_fast_and_merge(O):
+    """An L3 / L4 basis as a part of a merge basis beside a radial part, in the
+    plain mode (full bar) and in FP32_FAST (RMS 2e-5 eV/A, max 2e-4 eV/A)."""
+    st = _liquid(["Cu", "Zr", "Al"], 800, 42)
+    parts = [{"type": "spherical_chebyshev", "nmax": 4, "lmax": 3, "l3max": 4, "l4max": 3, "rcut": 6.0,
+              "wtype": "default"},
+             {"type": "chebyshev", "nmax": 6, "rcut": 4.5, "wtype": "full"}]
+    md = M.random_merge_model(["Cu", "Zr", "Al"], parts, (64, 64), 44)
+    assert M.load_model(md).species[0].S == 10 * (4 + 9 + 9) + 21
+    md = _normalized_by_oracle(O, md, st)
+    _check(O, md, st)
+    with nnap.NNAP(md, precision=nnap.FP32_FAST) as pot:
+        res = pot.calEnergyForceVirials(nnap.AtomData(st.box, st.xyz, st.types))
+    ref = O.Oracle(M.load_model(md)).system_efv(st)
+    df = res["forces"] - ref["forces"]
+    print(f"fast f_rms {np.sqrt((df ** 2).mean()):.2e} f_max {np.abs(df).max():.2e}")
     assert abs(res["energy"] - ref["energies"].sum()) / st.natoms <= 1e-6
     assert np.sqrt((df ** 2).mean()) <= 2e-5 and np.abs(df).max() <= 2e-4
