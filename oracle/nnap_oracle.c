@@ -69,6 +69,7 @@ typedef struct {
     const int *nparts;        /* [ntypes] number of bases of each species (NULL: one) */
     const int *l3max;         /* [bases] L3 invariants' lmax (0: none; NULL: none at all) */
     const int *l4max;         /* [bases] L4 invariants' lmax (0: none; NULL: none at all) */
+    const int *train_w;       /* [bases] 1: W is a trainable parameter block (fuse, rfuse, exfuse) */
 } orc_model;
 
 static int nparts_of(const orc_model *m, int t) { return m->nparts ? m->nparts[t] : 1; }
@@ -778,11 +779,9 @@ static double mlp(const sp_view *v, const double *fp, double *gfp) {
     return E;
 }
 
-/* backward of descriptor: g_j = -dE_i/d(dx_j)  (basis_SphericalChebyshev.hpp:353-465) */
-static void descriptor_backward(const sp_view *v, int lmax, const double *dx, const double *dy,
-                                const double *dz, const int *ty, int nn, const double *A, const double *gfp,
-                                double *gx, double *gy, double *gz) {
-    double gA[ORC_MAX_P * ORC_MAX_M];
+/* dE/dA of one basis from dE/dfp: the L2 part 2 4pi/(2l+1) gfp A (calGradSphL2,
+ * basis_SphericalUtil.hpp:469-484) and the L3 / L4 invariants' derivatives */
+static void basis_grad_A(const sp_view *v, int lmax, const double *A, const double *gfp, double *gA) {
     for (int k = 0; k < v->P * v->M; ++k) gA[k] = 0.0;
     for (int p = 0; p < v->P; ++p) {
         for (int l = 0; l <= lmax; ++l) {
@@ -792,6 +791,14 @@ static void descriptor_backward(const sp_view *v, int lmax, const double *dx, co
         if (v->SL > lmax + 1)
             sh_invariants_grad(v->l3, v->l4, A + p * v->M, gfp + p * v->SL + lmax + 1, gA + p * v->M);
     }
+}
+
+/* backward of descriptor: g_j = -dE_i/d(dx_j)  (basis_SphericalChebyshev.hpp:353-465) */
+static void descriptor_backward(const sp_view *v, int lmax, const double *dx, const double *dy,
+                                const double *dz, const int *ty, int nn, const double *A, const double *gfp,
+                                double *gx, double *gy, double *gz) {
+    double gA[ORC_MAX_P * ORC_MAX_M];
+    basis_grad_A(v, lmax, A, gfp, gA);
     double rn[ORC_MAX_K], rng[ORC_MAX_K], y[ORC_MAX_M], yt[ORC_MAX_M];
     double rnp[ORC_MAX_P], grn[ORC_MAX_K], gy_[ORC_MAX_M];
     for (int j = 0; j < nn; ++j) {
@@ -1032,6 +1039,98 @@ int orc_system_fp(const orc_model *m, int n, const double *xyz_in, const int *ty
         rc = orc_cal_fp(m, type[i], dx, dy, dz, ty, nn, fp_out + (size_t)i * smax);
     }
     free(x); free(count); free(key); free(d); free(dx); free(ty);
+    return rc;
+}
+
+/* ------------------------------------------------------------------ */
+/* Training derivative of the trainable mixing weights: the rGradFpParam */
+/* part of jse_nnap_backwardEnergy (nnap_interface.cpp:111-134) summed    */
+/* over a system with per-atom weights w[i] (the trainer's aGradEng).     */
+/* For a basis whose W is a parameter block (fuse, rfuse, exfuse; the    */
+/* others return at once, basis_SphericalChebyshev.hpp:361-364):          */
+/*   dE/dW[t][p][n] += w_i sum_{j of species t} Rn_j[n] dE_i/dRnp_j[p],    */
+/*   dE_i/dRnp_j[p] = fc_j sum_m dE_i/dA[p][m] Y_j[m]                      */
+/* (backwardMplusAnlm then backwardRnp, nnap_util.hpp:282-296); for the  */
+/* radial basis dE_i/dRnp_j[p] = fc_j dE_i/dfp[p] (basis_Chebyshev.hpp:    */
+/* 135-220).  grad[t]                                                     */
+/* holds, per basis of species t in order, ntypes x P x K values (zeros   */
+/* for a basis without parameters), with neighbour species as the basis  */
+/* sees them (a mirror basis' swapped roles).                             */
+/* ------------------------------------------------------------------ */
+int orc_system_fp_param_grad(const orc_model *m, int n, const double *xyz_in, const int *type, const double *box,
+                             int cap, const double *w, double *const *grad) {
+    double rmax = 0.0;
+    for (int t = 0; t < m->ntypes; ++t) if (species_rcut(m, t) > rmax) rmax = species_rcut(m, t);
+    for (int t = 0; t < m->ntypes; ++t) {
+        size_t sz = 0;
+        for (int k = 0; k < nparts_of(m, t); ++k) {
+            sp_view v = basis_view(m, t + 1, k);
+            sz += (size_t)m->ntypes * v.P * v.K;
+        }
+        memset(grad[t], 0, sizeof(double) * sz);
+    }
+    double *x = (double *)malloc(sizeof(double) * 3 * (size_t)(n > 0 ? n : 1));
+    orc_wrap(n, xyz_in, box, x);
+    int *count = (int *)malloc(sizeof(int) * (size_t)(n > 0 ? n : 1));
+    int *key = (int *)malloc(sizeof(int) * (size_t)n * cap + 1);
+    double *d = (double *)malloc(sizeof(double) * 3 * (size_t)n * cap + 1);
+    int rc = orc_neighbors_celllist(n, x, box, rmax, cap, count, key, d);
+    double *dx = (double *)malloc(sizeof(double) * (size_t)cap * 3 + 8);
+    double *dy = dx + cap, *dz = dy + cap;
+    int *ty = (int *)malloc(sizeof(int) * (size_t)cap + 4), *buf = (int *)malloc(sizeof(int) * (size_t)cap + 4);
+    double *A = (double *)malloc(sizeof(double) * ORC_MAX_P * ORC_MAX_M * 8);
+    double *gA = (double *)malloc(sizeof(double) * ORC_MAX_P * ORC_MAX_M);
+    for (int i = 0; rc == 0 && i < n; ++i) {
+        if (count[i] > cap) { rc = 2; break; }
+        const int ct = type[i];
+        if (ct < 1 || ct > m->ntypes || nparts_of(m, ct - 1) > 8) { rc = 3; break; }
+        const double rci = species_rcut(m, ct - 1);
+        int nn = 0;
+        for (int k = 0; k < count[i]; ++k) {
+            const double *dd = d + ((size_t)i * cap + k) * 3;
+            if (dd[0] * dd[0] + dd[1] * dd[1] + dd[2] * dd[2] >= rci * rci && rci < rmax) continue;
+            dx[nn] = dd[0]; dy[nn] = dd[1]; dz[nn] = dd[2];
+            ty[nn] = type[key[(size_t)i * cap + k] >> 5];
+            ++nn;
+        }
+        const int *rt = role_types(m, ct, ty, nn, buf);
+        sp_view sv = species(m, ct);
+        double *fp = (double *)malloc(sizeof(double) * 2 * (size_t)sv.S);
+        double *gfp = fp + sv.S;
+        fp_all(m, ct, dx, dy, dz, rt, nn, A, fp);
+        mlp(&sv, fp, gfp);
+        size_t goff = 0;
+        int off = 0;
+        for (int k = 0; k < nparts_of(m, ct - 1); ++k) {
+            sp_view v = basis_view(m, ct, k);
+            const int b = basis_index(m, ct - 1, k);
+            if (m->train_w && m->train_w[b]) {
+                if (!v.radial) basis_grad_A(&v, v.L1 - 1, A + (size_t)k * ORC_MAX_P * ORC_MAX_M, gfp + off, gA);
+                double rn[ORC_MAX_K], y[ORC_MAX_M];
+                for (int j = 0; j < nn; ++j) {
+                    const double r = sqrt(dx[j] * dx[j] + dy[j] * dy[j] + dz[j] * dz[j]);
+                    if (r >= v.rc) continue;
+                    angles a = get_angles(dx[j], dy[j], dz[j], r);
+                    if (!v.radial) real_sph(v.lmm, a.ct, a.st, a.cp, a.sp, y);
+                    cal_rn(v.K - 1, r, v.rc, rn);
+                    const double fc = cal_fc(r, v.rc);
+                    double *g = grad[ct - 1] + goff + (size_t)(rt[j] - 1) * v.P * v.K;
+                    for (int p = 0; p < v.P; ++p) {
+                        double s = 0.0;
+                        if (v.radial) s = gfp[off + p];
+                        else
+                            for (int q = 0; q < v.M; ++q) s += gA[p * v.M + q] * y[q];
+                        const double grnp = fc * s;
+                        for (int q = 0; q < v.K; ++q) g[p * v.K + q] += w[i] * grnp * rn[q];
+                    }
+                }
+            }
+            goff += (size_t)m->ntypes * v.P * v.K;
+            off += v.S;
+        }
+        free(fp);
+    }
+    free(x); free(count); free(key); free(d); free(dx); free(ty); free(buf); free(A); free(gA);
     return rc;
 }
 

@@ -36,6 +36,7 @@ class OrcModel(ctypes.Structure):
         ("nparts", _c_int_p),
         ("l3max", _c_int_p),
         ("l4max", _c_int_p),
+        ("train_w", _c_int_p),
     ]
 
 
@@ -99,9 +100,12 @@ class Oracle:
         self._radial = np.array([1 if b.radial else 0 for b in bl], dtype=np.int32)
         self._l3 = np.array([getattr(b, "l3max", 0) for b in bl], dtype=np.int32)
         self._l4 = np.array([getattr(b, "l4max", 0) for b in bl], dtype=np.int32)
+        # W is a parameter block for the fuse, rfuse and exfuse wtypes (WTypeBasis.parameterSize)
+        self._train = np.array([1 if b.typed_wtype in (4, 5, 6) else 0 for b in bl], dtype=np.int32)
         self.c = OrcModel(nt, _ip(self._nmax), _ip(self._lmax), _ip(self._P), _dp(self._rc), _ip(self._nl),
                           _ip(self._hid), self._Wp, self._nnp, self._normp, _ip(self._mirror), _ip(self._radial),
-                          _ip(self._part0), _ip(self._nparts), _ip(self._l3), _ip(self._l4))
+                          _ip(self._part0), _ip(self._nparts), _ip(self._l3), _ip(self._l4),
+                          _ip(self._train))
 
     # --- per-atom entry points (jse_nnap_calFp / calEnergy / calEnergyForce) ---
     def cal_fp(self, ctype, d, tj):
@@ -176,6 +180,29 @@ def system_fp(orc: "Oracle", st, cap: int = 256):
             continue
         if rc:
             raise ValueError(f"oracle system_fp code {rc}")
+        return out
+
+
+def fp_param_grad(orc: "Oracle", st, w, cap: int = 256):
+    """Sum over atoms of w[i] dE_i/dW for every basis whose mixing W is a
+    parameter block (orc_system_fp_param_grad); one array per species, its
+    bases' [ntypes][P][K] blocks concatenated (zeros for a basis without
+    parameters)."""
+    mdl = orc.mdl
+    n = st.natoms
+    xyz = np.ascontiguousarray(st.xyz, dtype=np.float64)
+    ty = np.ascontiguousarray(st.types, dtype=np.int32)
+    box = np.ascontiguousarray(st.box, dtype=np.float64)
+    w = np.ascontiguousarray(w, dtype=np.float64)
+    out = [np.zeros(sum(mdl.ntypes * b.P * b.K for b in sp.bases())) for sp in mdl.species]
+    ptrs = (_c_dbl_p * len(out))(*[_dp(a) for a in out])
+    while True:
+        rc = orc.L.orc_system_fp_param_grad(ctypes.byref(orc.c), n, _dp(xyz), _ip(ty), _dp(box), cap, _dp(w), ptrs)
+        if rc == 2:
+            cap *= 2
+            continue
+        if rc:
+            raise ValueError(f"oracle fp_param_grad code {rc}")
         return out
 
 

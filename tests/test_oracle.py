@@ -842,3 +842,100 @@ def test_l34_loader_limits():
         md["models"][0]["basis"][key] = val
         with pytest.raises(ValueError, match=msg):
             M.load_model(md)
+
+
+# ---------------------------------------- training derivative (mixing weights)
+def _trainable_mixing_model(O, wtype, seed, symbols=("Cu", "Zr"), nmax=3, lmax=3, n=64, l3max=0):
+    """A random model whose basis mixing is a parameter block (rfuse: P = ntypes K
+    channels of random weights; fuse: fuse_size ntypes), normalised on a random
+    packing by the oracle's descriptors."""
+    nt = len(symbols)
+    K = nmax + 1
+    md = M.random_model(list(symbols), nmax, lmax, 5.0, (8, 8), seed=seed, wtype="full", l3max=l3max)
+    rng = np.random.default_rng(seed)
+    for m in md["models"]:
+        m["basis"]["wtype"] = wtype
+        if wtype == "rfuse":
+            m["basis"]["rfuse_weight"] = rng.normal(size=nt * K * nt * K).tolist()
+        else:
+            m["basis"]["fuse_weight"] = rng.normal(size=nt * nt).tolist()
+    box = np.array([11.0, 11.5, 12.0])
+    st = W.Structure(box, W.rsa(n, box, 2.0, seed), (np.arange(n) % nt + 1).astype(np.int32))
+    mdl = M.load_model(md)
+    fp = O.system_fp(O.Oracle(mdl), st)
+    md = M.set_normalization(md, [fp[st.types == t + 1, :mdl.species[t].S] for t in range(nt)])
+    return st, M.load_model(md)
+
+
+@pytest.mark.parametrize("wtype,l3max", [("rfuse", 0), ("fuse", 0), ("rfuse", 3)])
+def test_fp_param_grad_finite_differences(O, wtype, l3max):
+    """sum_i w_i dE_i/dW (the rGradFpParam part of jse_nnap_backwardEnergy,
+    nnap_interface.cpp:111-134, for a basis whose mixing W is a parameter
+    block) against central finite differences of sum_i w_i E_i with one W
+    entry moved at a time."""
+    st, mdl = _trainable_mixing_model(O, wtype, 61 + l3max, l3max=l3max)
+    orc = O.Oracle(mdl)
+    rng = np.random.default_rng(3)
+    w = rng.uniform(0.5, 1.5, size=st.natoms)
+    grads = O.fp_param_grad(orc, st, w)
+    h = 1e-6
+    for t, sp in enumerate(mdl.species):
+        Wb = orc._W[t]
+        assert grads[t].shape == Wb.shape
+        for idx in rng.choice(Wb.size, size=6, replace=False):
+            old = Wb[idx]
+            Wb[idx] = old + h
+            ep = w @ orc.system_efv(st)["energies"]
+            Wb[idx] = old - h
+            em = w @ orc.system_efv(st)["energies"]
+            Wb[idx] = old
+            fd = (ep - em) / (2 * h)
+            assert abs(fd - grads[t][idx]) < 2e-6 * max(1.0, abs(fd)), (t, idx, fd, grads[t][idx])
+
+
+def test_fp_param_grad_is_zero_without_parameters(O):
+    """The full, default and none mixings carry no parameters: their block is
+    zero (basis_SphericalChebyshev.hpp:361-364)."""
+    st, mdl = _small_system(seed=7)
+    g = O.fp_param_grad(O.Oracle(mdl), st, np.ones(st.natoms))
+    assert all(np.array_equal(b, np.zeros_like(b)) for b in g)
+
+
+def test_fp_param_grad_merge_with_radial_rfuse(O):
+    """A merge basis of an rfuse spherical part and an rfuse radial part (the
+    radial basis' parameter gradient, basis_Chebyshev.hpp:135-220): the
+    species' block is the two parts' blocks in order, each against finite
+    differences."""
+    K1, K2 = 3, 5
+    rng = np.random.default_rng(12)
+    parts = [{"type": "spherical_chebyshev", "nmax": K1 - 1, "lmax": 3, "rcut": 5.0, "wtype": "rfuse",
+              "rfuse_weight": rng.normal(size=6 * 2 * K1).tolist()},
+             {"type": "chebyshev", "nmax": K2 - 1, "rcut": 4.0, "wtype": "rfuse",
+              "rfuse_weight": rng.normal(size=10 * 2 * K2).tolist()}]
+    md = M.random_merge_model(["Cu", "Zr"], [dict(p, wtype="full") for p in parts], (8,), 5)
+    for m in md["models"]:
+        m["basis"]["basis"] = parts
+    mdl = M.load_model(md)
+    assert [b.P for b in mdl.species[0].bases()] == [6, 10]    # the networks of the full mixing fit
+    box = np.array([11.0, 11.5, 12.0])
+    st = W.Structure(box, W.rsa(60, box, 2.0, 4), (np.arange(60) % 2 + 1).astype(np.int32))
+    fp = O.system_fp(O.Oracle(mdl), st)
+    md = M.set_normalization(md, [fp[st.types == t + 1, :mdl.species[t].S] for t in range(2)])
+    mdl = M.load_model(md)
+    orc = O.Oracle(mdl)
+    w = rng.uniform(0.5, 1.5, size=st.natoms)
+    g = O.fp_param_grad(orc, st, w)
+    sizes = [2 * 6 * K1, 2 * 10 * K2]
+    assert g[1].shape == (sum(sizes),)
+    h = 1e-6
+    for k, sz in enumerate(sizes):
+        Wb = orc._W[2 + k]     # species 2's bases
+        for idx in rng.choice(sz, size=5, replace=False):
+            old = Wb[idx]
+            Wb[idx] = old + h
+            ep = w @ orc.system_efv(st)["energies"]
+            Wb[idx] = old - h
+            em = w @ orc.system_efv(st)["energies"]
+            Wb[idx] = old
+            fd = (ep - em) / (2 * h)
+            assert abs(fd - g[1][sum(sizes[:k]) + idx]) < 2e-6 * max(1.0, abs(fd))
