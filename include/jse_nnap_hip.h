@@ -136,18 +136,24 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
  * (nnap_interface.cpp:11-24).  Runs the neighbour build itself. */
 int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax);
 
-/* Training derivative of the networks: grad_nn[t] (one block per species, in
- * the packed nn_param layout [hidden W per layer [out][in] | output W | hidden
- * b | output b], FP64) = sum over the atoms i of species t of
- * w[i] dE_i/dtheta at the current positions.  This is the network part of
- * jse_nnap_backwardEnergy, nnap_interface.cpp:111-134, summed over a system:
- * the reference's trainer calls that per atom with aGradEng = w[i] and adds
- * rGradNnParam; here one call covers every atom (the rGradFpParam part, the
- * trainable fuse / rfuse mixing weights, is not provided).  A mirror or
- * shared-network species reports the gradient of its own expanded block.
- * Runs the neighbour build and the descriptors itself; status 2 when a
- * network's 16-row tile of layers exceeds 160 KB of LDS. */
-int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *const *grad_nn);
+/* Training derivatives: the reference's jse_nnap_backwardEnergy,
+ * nnap_interface.cpp:111-134, summed over a system.  The reference's trainer calls it per atom with
+ * aGradEng = w[i] and adds rGradNnParam / rGradFpParam; one call here covers
+ * every atom at the current positions (FP64 sums in a fixed order):
+ *   grad_nn[t] = sum over the atoms i of species t of w[i] dE_i/dtheta, in the
+ *     packed nn_param layout [hidden W per layer [out][in] | output W | hidden b
+ *     | output b];
+ *   grad_fp[t] (may be NULL: not computed) = the same for the mixing weights
+ *     W[ntypes][P][K] of each of the species' bases in order (the merge parts'
+ *     blocks one after another): sum_i w[i] dE_i/dW for a fuse / rfuse / exfuse
+ *     mixing, zeros for the other wtypes, which have no parameters
+ *     (basis_SphericalChebyshev.hpp:353-440, basis_Chebyshev.hpp:135-220).
+ * A mirror or shared species reports the gradient of its own blocks (a mirror's
+ * mixing with its neighbour species swapped, as it evaluates them).  Runs the
+ * neighbour build, descriptors and MLP itself; status 2 when a network's
+ * 16-row tile of layers exceeds 160 KB of LDS. */
+int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *const *grad_nn,
+                                    double *const *grad_fp);
 
 /* The neighbour list of the last compute: count[natoms] and, for rows of
  * `cap` slots, key = j*32 + shift_code (shift_code = (sx+1)+3(sy+1)+9(sz+1))

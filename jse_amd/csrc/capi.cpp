@@ -190,6 +190,34 @@ SegmentArgs species_segment(const jse_nnap_model_s *m, int t, int precision, con
     set_segment_basis(a, m, t, 0);
     return a;
 }
+SegmentArgs part_segment(const SegmentArgs &seg, const jse_nnap_model_s *m, int t, int k) {
+    const SpeciesHost &sp = m->host.species[t];
+    SegmentArgs a = seg;
+    if (sp.parts.empty()) return a;
+    const SpeciesDev &dv = m->dev[t];
+    int xoff = 0;
+    size_t aoff = 0;
+    for (int q = 0; q < k; ++q) {
+        xoff += sp.basis(q).P * sp.basis(q).SL();
+        aoff += (size_t)seg.nrows * sp.basis(q).P * sp.basis(q).M();
+    }
+    const SpeciesHost &b = sp.basis(k);
+    a.W = dv.pW[k];
+    a.W64 = dv.pW64[k];
+    a.rc = (float)b.rcut;
+    a.rc64 = b.rcut;
+    if (a.x) a.x += xoff;
+    if (a.x64) a.x64 += xoff;
+    a.mu += xoff;
+    a.sig += xoff;
+    a.mu64 += xoff;
+    a.sig64 += xoff;
+    a.G += xoff;
+    a.A += aoff;
+    a.accum_g = k > 0 ? 1 : 0;
+    set_segment_basis(a, m, t, k);
+    return a;
+}
 void set_segment_basis(SegmentArgs &a, const jse_nnap_model_s *m, int t, int k) {
     const SpeciesHost &b = m->host.species[t].basis(k);
     const SpeciesDev::Coupling &c = m->dev[t].cpl[k];
@@ -272,34 +300,13 @@ void enqueue_desc(hipStream_t s, const jse_nnap_model_s *m, const SegmentArgs *s
         for (int t = 0; t < nt; ++t) {
             if (segs[t].nrows <= 0) continue;
             const SpeciesHost &sp = m->host.species[t];
-            const SpeciesDev &dv = m->dev[t];
-            int xoff = 0;
-            size_t aoff = 0;
             for (int k = 0; k < sp.nbases(); ++k) {
                 const SpeciesHost &b = sp.basis(k);
-                SegmentArgs a = segs[t];
-                if (!sp.parts.empty()) {
-                    a.W = dv.pW[k];
-                    a.W64 = dv.pW64[k];
-                    a.rc = (float)b.rcut;
-                    a.rc64 = b.rcut;
-                    if (a.x) a.x += xoff;
-                    if (a.x64) a.x64 += xoff;
-                    a.mu += xoff;
-                    a.sig += xoff;
-                    a.mu64 += xoff;
-                    a.sig64 += xoff;
-                    a.G += xoff;
-                    a.A += aoff;
-                    a.accum_g = k > 0 ? 1 : 0;
-                    set_segment_basis(a, m, t, k);
-                }
+                const SegmentArgs a = part_segment(segs[t], m, t, k);
                 const int lm = desc_lmax(b);
                 const bool ok = backward ? launch_desc_bwd(s, lm, b.K(), b.P, b.mix, a, nl)
                                          : launch_desc_fwd(s, lm, b.K(), b.P, b.mix, a, nl);
                 if (!ok) fail(t);
-                xoff += b.P * b.SL();
-                aoff += (size_t)segs[t].nrows * b.P * b.M();
             }
         }
         return;
@@ -1033,11 +1040,12 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
     });
 }
 
-int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *const *grad_nn) {
+int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *const *grad_nn,
+                                    double *const *grad_fp) {
     return guard([&] {
         JNN_CHECK(s && grad_nn && (w || s->n == 0), ERR_ARG, "null argument");
         const int nt = s->model->host.ntypes();
-        for (int t = 0; t < nt; ++t) JNN_CHECK(grad_nn[t], ERR_ARG, "null gradient block");
+        for (int t = 0; t < nt; ++t) JNN_CHECK(grad_nn[t] && (!grad_fp || grad_fp[t]), ERR_ARG, "null gradient block");
         // the normalised descriptors of the current positions, as the networks read them
         s->last_what = 0;
         s->pending = true;
@@ -1063,6 +1071,37 @@ int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *
                 JNN_CHECK(launch_mlp_param_grad(s->stream, a, d_w, partial, out), ERR_UNSUPPORTED,
                           "network too wide for the parameter-gradient kernel (16 rows of its layers in 160 KB)");
                 JNN_HIP(hipMemcpyAsync(grad_nn[t], out, sizeof(double) * np, hipMemcpyDeviceToHost, s->stream));
+            }
+            if (grad_fp) {
+                // dE/dfp of every atom (the MLP with gradients), then per basis with a
+                // trainable mixing sum_i w_i dE_i/dW; zeros for the other bases
+                std::vector<MlpArgs> ma;
+                for (int t = 0; t < nt; ++t) ma.push_back(mlp_args(s->model, t, s->seg_args(t, false), s->d_E, true, 0));
+                enqueue_mlp_all(s->stream, s->model, s->precision, ma.data(), nt);
+                for (int t = 0; t < nt; ++t) {
+                    const SpeciesHost &sp = s->model->host.species[t];
+                    const SegmentArgs seg = s->seg_args(t, false);
+                    size_t goff = 0;
+                    for (int k = 0; k < sp.nbases(); ++k) {
+                        const SpeciesHost &b = sp.basis(k);
+                        const int np = nt * b.P * b.K();
+                        if (!b.mix_trainable() || seg.nrows <= 0) {
+                            std::fill(grad_fp[t] + goff, grad_fp[t] + goff + np, 0.0);
+                        } else {
+                            double *partial = dalloc<double>((size_t)mix_pgrad_groups(seg.nrows) * np);
+                            bufs.push_back(partial);
+                            double *out = dalloc<double>((size_t)np);
+                            bufs.push_back(out);
+                            JNN_CHECK(launch_mix_param_grad(s->stream, desc_lmax(b), b.K(), b.P,
+                                                            part_segment(seg, s->model, t, k), s->nl, d_w, partial,
+                                                            out),
+                                      ERR_UNSUPPORTED, "mixing too wide for the parameter-gradient kernel");
+                            JNN_HIP(hipMemcpyAsync(grad_fp[t] + goff, out, sizeof(double) * np, hipMemcpyDeviceToHost,
+                                                   s->stream));
+                        }
+                        goff += (size_t)np;
+                    }
+                }
             }
             JNN_HIP(hipStreamSynchronize(s->stream));
         } catch (...) {

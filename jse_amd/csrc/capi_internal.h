@@ -147,6 +147,11 @@ void check_model_shapes(const jse_nnap_model_s *m, int precision);
 SegmentArgs species_segment(const jse_nnap_model_s *m, int t, int precision, const int *types);
 // The column layout and L3 / L4 coupling terms of basis k of species t.
 void set_segment_basis(SegmentArgs &a, const jse_nnap_model_s *m, int t, int k);
+// Species t's segment narrowed to basis k (a merge basis, MergedBasis.java: the
+// part's mixing and cutoff, its slice of the x / mu / sigma / G rows (advanced
+// by the sizes of parts 0..k-1), its own A block, g added to by parts after
+// the first); the segment itself for a single basis.
+SegmentArgs part_segment(const SegmentArgs &seg, const jse_nnap_model_s *m, int t, int k);
 // The lmax the descriptor kernels are chosen by: max(lmax, l3max, l4max), or
 // kRadialLmax for the radial basis.
 inline int desc_lmax(const SpeciesHost &b) { return b.radial ? kRadialLmax : b.lmm(); }

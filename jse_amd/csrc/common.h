@@ -63,6 +63,8 @@ struct SpeciesHost {
     int lmm() const { return lmax > l3max ? (lmax > l4max ? lmax : l4max) : (l3max > l4max ? l3max : l4max); }
     int M() const { return (lmm() + 1) * (lmm() + 1); }
     bool l34() const { return l3max > 0 || l4max > 0; }
+    // W is a parameter block (fuse, rfuse, exfuse; WTypeBasis.parameterSize)
+    bool mix_trainable() const { return typed_wtype == 4 || typed_wtype == 5 || typed_wtype == 6; }
     // descriptor columns per radial channel: lmax + 1 L2 values, then the L3 and L4 invariants
     int SL() const { return lmax + 1 + kL3Cols[l3max] + kL4Cols[l4max]; }
     int mix = MIX_DENSE;              // kernel specialisation for W

@@ -267,6 +267,15 @@ enum { kMlpFp32 = 0, kMlpBf16 = 1, kMlpFp64 = 2 };  // element types of the MLP 
 int mlp_param_count(const MlpArgs &a);
 int mlp_pgrad_groups(const MlpArgs &a);
 bool launch_mlp_param_grad(hipStream_t s, const MlpArgs &a, const double *w, double *partial, double *out);
+// out[e] = sum over the nwg rows of partial[g][e], in row order (np entries)
+void launch_pgrad_reduce(hipStream_t s, int nwg, int np, const double *partial, double *out);
+// Training derivative of a segment's trainable mixing: out[ntypes][P][K] = sum
+// over its atoms of w[atom] dE/dW (a's G from an MLP pass with gradients; lmax
+// = kRadialLmax: the radial basis).  partial holds mix_pgrad_groups(nrows)
+// rows of ntypes P K doubles.  False outside the untiled kernels' limits.
+int mix_pgrad_groups(int nrows);
+bool launch_mix_param_grad(hipStream_t s, int lmax, int K, int P, const SegmentArgs &a, const NeighborList &nl,
+                           const double *w, double *partial, double *out);
 // All species in one launch: the persistent workgroups are split among the
 // species in proportion to their row tiles (each stages its species' weights
 // once).  False when the species' MLP shapes differ, there are more than

@@ -351,6 +351,9 @@ int mlp_param_count(const MlpArgs &a) {
     }
     return n + in + 1;
 }
+void launch_pgrad_reduce(hipStream_t s, int nwg, int np, const double *partial, double *out) {
+    hipLaunchKernelGGL(k_pgrad_reduce, dim3((np + 255) / 256), dim3(256), 0, s, nwg, np, partial, out);
+}
 int mlp_pgrad_groups(const MlpArgs &a) {
     const int tiles = (a.nrows + 15) / 16;
     return std::max(1, std::min(tiles, device_cu_count()));

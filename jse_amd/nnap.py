@@ -111,7 +111,7 @@ def lib():
     L.jse_nnap_system_compute.argtypes = [vp, ctypes.c_int]
     L.jse_nnap_system_get_results.argtypes = [vp, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     L.jse_nnap_system_compute_fp.argtypes = [vp, _c_dbl_p, ctypes.c_int]
-    L.jse_nnap_system_backward_energy.argtypes = [vp, _c_dbl_p, ctypes.POINTER(_c_dbl_p)]
+    L.jse_nnap_system_backward_energy.argtypes = [vp, _c_dbl_p, ctypes.POINTER(_c_dbl_p), ctypes.POINTER(_c_dbl_p)]
     L.jse_nnap_system_get_neighbors.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_flt_p]
     L.jse_nnap_system_benchmark.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                             ctypes.c_int]
@@ -257,18 +257,24 @@ class System:
         _check(lib().jse_nnap_system_compute_fp(self._h, _dp(fp), smax))
         return fp
 
-    def backward_energy(self, w) -> List[np.ndarray]:
-        """Network parameter gradients of sum_i w[i] E_i (jse_nnap_backwardEnergy's
-        rGradNnParam summed over the atoms, aGradEng = w[i]): one array per
-        species in the packed nn_param layout."""
+    def backward_energy(self, w, mixing: bool = False):
+        """Parameter gradients of sum_i w[i] E_i (jse_nnap_backwardEnergy summed over
+        the atoms, aGradEng = w[i]): one array per species in the packed nn_param
+        layout (rGradNnParam); with ``mixing`` also, per species, the
+        concatenated [ntypes][P][K] blocks of its bases' mixing weights
+        (rGradFpParam: non-zero for fuse / rfuse / exfuse), returned as a second
+        list."""
         w = np.ascontiguousarray(w, dtype=np.float64)
         if w.shape != (self.natoms,):
             raise ValueError("w must have one entry per atom")
         sp = self.nnap.model.species
+        nt = len(sp)
         outs = [np.zeros(s.nn_param.size) for s in sp]
-        ptrs = (_c_dbl_p * len(outs))(*[_dp(o) for o in outs])
-        _check(lib().jse_nnap_system_backward_energy(self._h, _dp(w), ptrs))
-        return outs
+        ptrs = (_c_dbl_p * nt)(*[_dp(o) for o in outs])
+        fps = [np.zeros(sum(nt * b.P * b.K for b in s.bases())) for s in sp] if mixing else None
+        fptrs = (_c_dbl_p * nt)(*[_dp(o) for o in fps]) if mixing else None
+        _check(lib().jse_nnap_system_backward_energy(self._h, _dp(w), ptrs, fptrs))
+        return (outs, fps) if mixing else outs
 
     def neighbors(self, cap: Optional[int] = None):
         n = self.natoms

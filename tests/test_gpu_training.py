@@ -1,7 +1,7 @@
-"""GPU parity of the training derivative of the networks (the rGradNnParam
-part of jse_nnap_backwardEnergy, nnap_interface.cpp:111-134, summed over a
-system with per-atom weights) against the oracle's FP64 restatement on the
-oracle's own descriptors.  Plain mode: the GPU descriptors are FP64, so the
+"""GPU parity of the training derivatives (the rGradNnParam and rGradFpParam
+parts of jse_nnap_backwardEnergy, nnap_interface.cpp:111-134, summed over a
+system with per-atom weights) against the oracle's FP64 restatement (the
+network part on the oracle's own descriptors).  Plain mode: the GPU descriptors are FP64, so the
 gradients agree to ~1e-9 of their largest entry; FP32_FAST (FP32 descriptors
 into an FP64 gradient): 1e-5."""
 import numpy as np
@@ -54,3 +54,59 @@ def test_param_grad_three_layers_merge_basis(O):
              {"type": "chebyshev", "nmax": 5, "rcut": 4.5, "wtype": "default"}]
     md = W._normalized(M.random_merge_model(["Cu", "Zr", "Al"], parts, (24, 16, 8), 91), st, 300, 91)
     _check(O, md, st, nnap.FP32, 1e-9)
+
+
+def _trainable_mixing_model(symbols, parts, hidden, seed, st, O):
+    """A merge (or single) basis whose parts use the rfuse mixing with random
+    weights; the networks are drawn for the full mixing of the same channel
+    counts.  Normalised by the oracle's descriptors."""
+    nt = len(symbols)
+    rng = np.random.default_rng(seed)
+    full = [dict(p, wtype="full") if p.get("wtype") == "rfuse" else dict(p) for p in parts]
+    md = M.random_merge_model(symbols, full, hidden, seed)
+    real = []
+    for p in parts:
+        q = dict(p)
+        if q.get("wtype") == "rfuse":
+            K = q["nmax"] + 1
+            q["rfuse_weight"] = rng.normal(size=nt * K * nt * K).tolist()   # P = ntypes K, as the full mixing
+        real.append(q)
+    for m in md["models"]:
+        m["basis"]["basis"] = real
+    mdl = M.load_model(md)
+    fp = O.system_fp(O.Oracle(mdl), st)
+    return M.set_normalization(md, [fp[st.types == t + 1, :mdl.species[t].S] for t in range(nt)])
+
+
+@pytest.mark.parametrize("precision,tol", [(nnap.FP32, 1e-7), (nnap.FP32_FAST, 2e-6)])
+def test_mixing_param_grad(O, precision, tol):
+    """The rGradFpParam part of jse_nnap_backwardEnergy (rfuse mixing weights,
+    basis_SphericalChebyshev.hpp:353-440, basis_Chebyshev.hpp:135-220) summed
+    over a system: a merge of an rfuse spherical part with L3 invariants, a
+    default-mixing part (no parameters: zeros) and an rfuse radial part,
+    against the oracle: 5e-9 of the largest entry measured in the plain mode
+    (dE/dA formed from the FP32 copies of G and A), 2e-7 in FP32_FAST."""
+    st = W.ternary_liquid(n=1200, seed=17)
+    parts = [{"type": "spherical_chebyshev", "nmax": 3, "lmax": 4, "l3max": 3, "rcut": 6.0, "wtype": "rfuse"},
+             {"type": "spherical_chebyshev", "nmax": 2, "lmax": 2, "rcut": 4.5, "wtype": "default"},
+             {"type": "chebyshev", "nmax": 5, "rcut": 5.0, "wtype": "rfuse"}]
+    md = _trainable_mixing_model(["Cu", "Zr", "Al"], parts, (32, 32), 23, st, O)
+    mdl = M.load_model(md)
+    w = np.random.default_rng(6).uniform(0.5, 1.5, size=st.natoms)
+    with nnap.NNAP(md, precision=precision) as pot:
+        sysm = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+        gnn, gfp = sysm.backward_energy(w, mixing=True)
+    want = O.fp_param_grad(O.Oracle(mdl), st, w)
+    sizes = [3 * b.P * b.K for b in mdl.species[0].bases()]
+    for t in range(3):
+        assert gfp[t].shape == want[t].shape == (sum(sizes),)
+        off = 0
+        for k, sz in enumerate(sizes):
+            g, r = gfp[t][off:off + sz], want[t][off:off + sz]
+            if k == 1:
+                assert not g.any() and not r.any()
+            else:
+                err = np.abs(g - r).max() / np.abs(r).max()
+                print(f"species {t + 1} part {k}: max |dg| / max |g| = {err:.2e}")
+                assert err <= tol
+            off += sz
