@@ -155,6 +155,26 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax);
 int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *const *grad_nn,
                                     double *const *grad_fp);
 
+/* Training derivatives of the force / stress loss: the reference's
+ * jse_nnap_forwardEnergyForce then jse_nnap_backwardEnergyForce,
+ * nnap_interface.cpp:141-232, for every atom of a system, as TrainerNNAP drives
+ * them (TrainerNNAP.java:1102-1176).  The loss is given by its gradients:
+ * w[natoms] by the per-atom energies (aGradEng; NULL: zeros), bgrad_forces
+ * [natoms][3] by the forces (NULL: zeros) and bgrad_virial[9] by the total
+ * virial, order xx yy zz xy xz yz yx zx zy (NULL: zeros); the trainer's
+ * per-pair upstream is then bgrad_forces[j] - bgrad_forces[i] + sum_a
+ * bgrad_virial[ab] d_a for pair (i, j) with displacement d.  grad_nn and
+ * grad_fp receive, in the layouts of jse_nnap_system_backward_energy, the
+ * gradient of sum_i w_i E_i + sum_k bgrad_forces[k] . F_k + sum bgrad_virial .
+ * virial by the network and mixing parameters (second derivatives: the
+ * descriptor's directional derivative along the pairs' upstream, the networks
+ * carried with that tangent).  FP64 sums in a fixed order.  Status 2 for a
+ * network whose 8-row tile of layers exceeds 160 KB of LDS, or a trainable
+ * mixing with 16 P (lmax+1)^2 bytes over it. */
+int jse_nnap_system_backward_energy_force(jse_nnap_system s, const double *w, const double *bgrad_forces,
+                                          const double *bgrad_virial, double *const *grad_nn,
+                                          double *const *grad_fp);
+
 /* The neighbour list of the last compute: count[natoms] and, for rows of
  * `cap` slots, key = j*32 + shift_code (shift_code = (sx+1)+3(sy+1)+9(sz+1))
  * sorted ascending and the FP32 displacement d[natoms][cap][3].  Returns

@@ -18,7 +18,7 @@ OBJ = os.path.join(HERE, "_build")
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("JSE_AMD_ARCH", "gfx950")
 
-HIP_SOURCES = ["nl.hip", "desc.hip", "desc_gen.hip", "mlp.hip", "mlp_mfma.hip", "force.hip", "serve.hip",
+HIP_SOURCES = ["nl.hip", "desc.hip", "desc_gen.hip", "desc_train.hip", "mlp.hip", "mlp_mfma.hip", "force.hip", "serve.hip",
                "lammps.hip", "gpumd.hip", "capi.cpp", "server.cpp", "lammps.cpp", "gpumd.cpp", "model_host.cpp", "sh_coupling.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",

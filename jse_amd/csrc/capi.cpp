@@ -1115,6 +1115,128 @@ int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *
     });
 }
 
+int jse_nnap_system_backward_energy_force(jse_nnap_system s, const double *w, const double *bgrad_forces,
+                                          const double *bgrad_virial, double *const *grad_nn,
+                                          double *const *grad_fp) {
+    return guard([&] {
+        JNN_CHECK(s && grad_nn, ERR_ARG, "null argument");
+        const int nt = s->model->host.ntypes();
+        for (int t = 0; t < nt; ++t) JNN_CHECK(grad_nn[t] && (!grad_fp || grad_fp[t]), ERR_ARG, "null gradient block");
+        jse_nnap_model_s *m = s->model;
+        s->last_what = 0;
+        s->pending = true;
+        s->build_nl(false);
+        s->finish();
+        s->have_nl = true;
+        s->desc_fwd(false);
+        const size_t n = (size_t)std::max(s->n, 1);
+        std::vector<void *> bufs;
+        auto alloc = [&](size_t cnt) {
+            double *p = dalloc<double>(cnt);
+            bufs.push_back(p);
+            return p;
+        };
+        try {
+            double *d_w = alloc(n);
+            if (s->n) {
+                if (w) JNN_HIP(hipMemcpyAsync(d_w, w, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+                else JNN_HIP(hipMemsetAsync(d_w, 0, sizeof(double) * s->n, s->stream));
+            }
+            TangentArgs base{};
+            if (bgrad_forces) {
+                double *d_bf = alloc(3 * n);
+                if (s->n)
+                    JNN_HIP(hipMemcpyAsync(d_bf, bgrad_forces, sizeof(double) * 3 * s->n, hipMemcpyHostToDevice,
+                                           s->stream));
+                base.bf = d_bf;
+            }
+            if (bgrad_virial)
+                for (int q = 0; q < 9; ++q) base.bv[q] = bgrad_virial[q];
+            double *d_xdot = alloc(n * m->smax), *d_adot = alloc(n * std::max(m->pmmax, 1));
+            double *d_xbar = grad_fp ? alloc(n * m->smax) : nullptr, *d_xdbar = grad_fp ? alloc(n * m->smax) : nullptr;
+            // the tangent args of basis k of species t (x slices and A blocks as part_segment narrows them)
+            auto tangent = [&](int t, int k, int nrows) {
+                const SpeciesHost &sp = m->host.species[t];
+                size_t xoff = (size_t)s->seg[t] * m->smax, aoff = (size_t)s->seg[t] * m->pmmax;
+                for (int q = 0; q < k; ++q) {
+                    xoff += (size_t)sp.basis(q).P * sp.basis(q).SL();
+                    aoff += (size_t)nrows * sp.basis(q).P * sp.basis(q).M();
+                }
+                TangentArgs ta = base;
+                ta.xdot = d_xdot + xoff;
+                ta.adot = d_adot + aoff;
+                ta.xbar = d_xbar ? d_xbar + xoff : nullptr;
+                ta.xdbar = d_xdbar ? d_xdbar + xoff : nullptr;
+                return ta;
+            };
+            // D_u x of every atom: each basis' descriptor tangent
+            for (int t = 0; t < nt; ++t) {
+                const SegmentArgs seg = s->seg_args(t, false);
+                if (seg.nrows <= 0) continue;
+                const SpeciesHost &sp = m->host.species[t];
+                for (int k = 0; k < sp.nbases(); ++k) {
+                    const SpeciesHost &b = sp.basis(k);
+                    JNN_CHECK(launch_desc_jvp(s->stream, desc_lmax(b), b.K(), b.P, part_segment(seg, m, t, k), s->nl,
+                                              tangent(t, k, seg.nrows)),
+                              ERR_UNSUPPORTED, "basis outside the descriptor tangent kernel's limits");
+                }
+            }
+            // the networks: sum_i (w_i E_i - D_u E_i) through the dual network
+            for (int t = 0; t < nt; ++t) {
+                const SegmentArgs seg = s->seg_args(t, false);
+                const MlpArgs a = mlp_args(m, t, seg, s->d_E, false, 0);
+                const int np = mlp_param_count(a);
+                if (a.nrows <= 0) {
+                    std::fill(grad_nn[t], grad_nn[t] + np, 0.0);
+                    continue;
+                }
+                const TangentArgs ta = tangent(t, 0, seg.nrows);
+                double *partial = alloc((size_t)mlp_dual_groups(a) * np), *out = alloc((size_t)np);
+                JNN_CHECK(launch_mlp_dual_grad(s->stream, a, d_w, ta.xdot, const_cast<double *>(ta.xbar),
+                                               const_cast<double *>(ta.xdbar), partial, out),
+                          ERR_UNSUPPORTED, "network too wide for the force-loss gradient kernel (8 rows of its layers in 160 KB)");
+                JNN_HIP(hipMemcpyAsync(grad_nn[t], out, sizeof(double) * np, hipMemcpyDeviceToHost, s->stream));
+            }
+            // the mixing weights
+            if (grad_fp) {
+                for (int t = 0; t < nt; ++t) {
+                    const SpeciesHost &sp = m->host.species[t];
+                    const SegmentArgs seg = s->seg_args(t, false);
+                    size_t goff = 0;
+                    for (int k = 0; k < sp.nbases(); ++k) {
+                        const SpeciesHost &b = sp.basis(k);
+                        const int np = nt * b.P * b.K();
+                        if (!b.mix_trainable() || seg.nrows <= 0) {
+                            std::fill(grad_fp[t] + goff, grad_fp[t] + goff + np, 0.0);
+                        } else {
+                            double *partial = alloc((size_t)mix_pgrad_groups(seg.nrows) * np), *out = alloc((size_t)np);
+                            JNN_CHECK(launch_mix_param_grad2(s->stream, desc_lmax(b), b.K(), b.P,
+                                                             part_segment(seg, m, t, k), s->nl,
+                                                             tangent(t, k, seg.nrows), partial, out),
+                                      ERR_UNSUPPORTED, "basis outside the force-loss mixing-gradient kernel's limits");
+                            JNN_HIP(hipMemcpyAsync(grad_fp[t] + goff, out, sizeof(double) * np, hipMemcpyDeviceToHost,
+                                                   s->stream));
+                        }
+                        goff += (size_t)np;
+                    }
+                }
+            }
+            JNN_HIP(hipStreamSynchronize(s->stream));
+        } catch (...) {
+            (void)hipStreamSynchronize(s->stream);
+            for (void *p : bufs) {
+                double *q = (double *)p;
+                dfree(q);
+            }
+            throw;
+        }
+        for (void *p : bufs) {
+            double *q = (double *)p;
+            dfree(q);
+        }
+    });
+}
+
 int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
     return guard([&] {
         JNN_CHECK(s && (fp || s->n == 0), ERR_ARG, "null argument");

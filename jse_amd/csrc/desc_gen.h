@@ -1,0 +1,44 @@
+// Shared pieces of the untiled descriptor kernels (desc_gen.hip) and their
+// training derivatives (desc_train.hip).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+
+namespace jnn {
+
+constexpr int kGenKMax = 21;        // nmax <= 20
+constexpr int kGenPMax = 128;       // mixing rows handled here
+constexpr size_t kGenLds = 160 * 1024;
+__host__ __device__ constexpr int pad4(int v) { return (v + 3) & ~3; }
+
+// dE/dA of one atom into LDS (P rows of MP floats, zero past M): Gc[p][l]
+// A[p][m] on the L2 columns (G carries 2 4pi/(2l+1) dE/dfp there) plus the L3 /
+// L4 invariants' terms (G = dE/dfp on their columns); a 64-lane workgroup.
+__device__ inline void fill_dEdA(const SegmentArgs &a, int row, int P, int M, int MP, float *sGA) {
+    const int PM = P * M;
+    for (int e = threadIdx.x; e < P * MP; e += 64) {
+        const int p = e / MP, m = e - p * MP;
+        int l = 0;
+        while ((l + 1) * (l + 1) <= m) ++l;
+        float v = 0.f;
+        if (m < M) {
+            const float *Gp = a.G + (size_t)row * a.xs + p * a.sl;
+            const float *Ap = a.A + (size_t)row * PM + p * M;
+            if (l < a.l2n) v = Gp[l] * Ap[m];
+            if (a.n34) {
+                double t = 0.0;
+                for (int q = a.cplg_off[m]; q < a.cplg_off[m + 1]; ++q) {
+                    const int4 ix = a.cplg_idx[q];
+                    double u = a.cplg_coef[q] * (double)Gp[a.l2n + ix.w] * (double)Ap[ix.x] * (double)Ap[ix.y];
+                    if (ix.z >= 0) u *= (double)Ap[ix.z];
+                    t += u;
+                }
+                v += (float)t;
+            }
+        }
+        sGA[e] = v;
+    }
+}
+
+}  // namespace jnn

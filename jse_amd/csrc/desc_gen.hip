@@ -30,15 +30,13 @@
 #include <hip/hip_runtime.h>
 
 #include "common.h"
+#include "desc_gen.h"
 #include "device_math.h"
 #include "kernels.h"
 
 namespace jnn {
 namespace {
 
-constexpr int kGenKMax = 21;        // nmax <= 20
-constexpr int kGenPMax = 128;       // mixing rows handled here
-constexpr size_t kGenLds = 160 * 1024;
 
 __device__ __forceinline__ int seg_of(const SegmentSet &set, int gr) {
     int sg = 0;
@@ -57,7 +55,6 @@ __device__ __forceinline__ T mix_row(const T *__restrict__ w, int K, const T *rn
 }
 
 constexpr int kTile = 4;  // A tile edge of the forward
-__host__ __device__ constexpr int pad4(int v) { return (v + 3) & ~3; }
 
 // The centre species' mixing W[ntypes][P][K] is copied into LDS whenever the
 // workgroup's rows move to another segment.
@@ -195,35 +192,6 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
             }
             store_x(a, (size_t)row * a.xs + e, ((f - a.mu64[e]) * (1.0 / a.sig64[e])));
         }
-    }
-}
-
-// dE/dA of one atom into LDS (P rows of MP floats, zero past M): Gc[p][l]
-// A[p][m] on the L2 columns (G carries 2 4pi/(2l+1) dE/dfp there) plus the L3 /
-// L4 invariants' terms (G = dE/dfp on their columns); a 64-lane workgroup.
-__device__ void fill_dEdA(const SegmentArgs &a, int row, int P, int M, int MP, float *sGA) {
-    const int PM = P * M;
-    for (int e = threadIdx.x; e < P * MP; e += 64) {
-        const int p = e / MP, m = e - p * MP;
-        int l = 0;
-        while ((l + 1) * (l + 1) <= m) ++l;
-        float v = 0.f;
-        if (m < M) {
-            const float *Gp = a.G + (size_t)row * a.xs + p * a.sl;
-            const float *Ap = a.A + (size_t)row * PM + p * M;
-            if (l < a.l2n) v = Gp[l] * Ap[m];
-            if (a.n34) {
-                double t = 0.0;
-                for (int q = a.cplg_off[m]; q < a.cplg_off[m + 1]; ++q) {
-                    const int4 ix = a.cplg_idx[q];
-                    double u = a.cplg_coef[q] * (double)Gp[a.l2n + ix.w] * (double)Ap[ix.x] * (double)Ap[ix.y];
-                    if (ix.z >= 0) u *= (double)Ap[ix.z];
-                    t += u;
-                }
-                v += (float)t;
-            }
-        }
-        sGA[e] = v;
     }
 }
 
@@ -499,94 +467,6 @@ __global__ __launch_bounds__(64) void k_gen_rad_bwd(SegmentSet set, NeighborList
     }
 }
 
-// Training derivative of a trainable mixing W[ntypes][P][K] (fuse, rfuse,
-// exfuse: the reference's rGradFpParam, basis_SphericalChebyshev.hpp:353-440
-// and basis_Chebyshev.hpp:135-220 with GRAD_PARAM; backwardRnp,
-// nnap_util.hpp:282-296), summed over the segment's atoms with weights w:
-//   dE/dW[t][p][n] += w_i sum_{j of species t} fc_j Rn_j[n] q_j[p],
-//   q_j[p] = sum_m dE_i/dA[p][m] Y_j[m]   (RADIAL: q_j[p] = dE_i/dfp[p]).
-// One 64-lane workgroup per atom at a time (persistent).  Per chunk of ch
-// neighbours a lane per neighbour puts Y_j (FP64, from the FP64 displacement),
-// fc_j Rn_j and the species into LDS; then the lanes form q for every
-// (neighbour, p), and lane o of the P K outputs adds, species by species,
-// w_i sum_j q_j[p] fc_j Rn_j[n] over the chunk's neighbours in list order to
-// its entries of partial[blockIdx.x][ntypes][P][K] (the workgroup's own row,
-// zeroed by it first).  Deterministic; reduced by k_pgrad_reduce.
-// LDS: Y [ch][MP] and q [ch][P] and fc Rn [ch][K] (FP64), dE/dA [P][MP]
-// (FP32), species [ch].
-template <int LMAX, bool RADIAL>
-__global__ __launch_bounds__(64) void k_gen_wgrad(SegmentArgs a, NeighborList nl, int K, int P, int ch,
-                                                  const double *__restrict__ w, double *__restrict__ partial) {
-    constexpr int L1 = LMAX + 1, M = L1 * L1, MP = pad4(M);
-    extern __shared__ __align__(16) double gen_smem[];
-    double *sY = gen_smem;
-    double *sQ = sY + (size_t)ch * MP;
-    double *sR = sQ + (size_t)ch * P;
-    float *sGA = reinterpret_cast<float *>(sR + (size_t)ch * K);
-    int *sT = reinterpret_cast<int *>(sGA + (size_t)P * MP);
-    const int lane = threadIdx.x;
-    const int nt = a.ntypes, PK = P * K;
-    double *part = partial + (size_t)blockIdx.x * nt * PK;
-    for (int e = lane; e < nt * PK; e += 64) part[e] = 0.0;
-    const double rc = a.rc64, inv_rc = 1.0 / rc;
-    for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
-        const int atom = a.rows_atom[row];
-        const int i = a.nl_of ? a.nl_of[atom] : atom;
-        const int n = min(nl.cnt[i], nl.cap);
-        const size_t base = (size_t)i * nl.cap;
-        const double wi = w[atom];
-        __syncthreads();  // the previous atom's reads of sGA are done
-        if constexpr (RADIAL) {
-            for (int p = lane; p < P; p += 64) sGA[p] = a.G[(size_t)row * a.xs + p];
-        } else {
-            fill_dEdA(a, row, P, M, MP, sGA);
-        }
-        for (int c0 = 0; c0 < n; c0 += ch) {
-            const int nc = min(ch, n - c0);
-            __syncthreads();  // dE/dA is written; the previous chunk's reads are done
-            if (lane < nc) {
-                const float4 dd = nl.d[base + c0 + lane];
-                sT[lane] = role_species(tag_species(dd.w), a.self_t, a.mirror);
-                const PairGeomD g = pair_geom64(dd, pair_lo(nl, base + c0 + lane));
-                if constexpr (!RADIAL) real_sph_rows<LMAX, double, double>(g.ct, g.st, g.cp, g.sp, sY + lane * MP);
-                double rn[kGenKMax];
-                cheb_rn<kGenKMax>(1.0 - 2.0 * g.r * inv_rc, rn);
-                const double t = 1.0 - (g.r * inv_rc) * (g.r * inv_rc);
-                const double fc = g.r < rc ? t * t * t * t : 0.0;  // pairs beyond this basis' cutoff add nothing
-                for (int q = 0; q < K; ++q) sR[lane * K + q] = fc * rn[q];
-            }
-            __syncthreads();
-            for (int e = lane; e < nc * P; e += 64) {
-                const int j = e / P, p = e - j * P;
-                double s;
-                if constexpr (RADIAL) {
-                    s = (double)sGA[p];
-                } else {
-                    const float *gp = sGA + p * MP;
-                    const double *y = sY + j * MP;
-                    s = 0.0;
-                    for (int m = 0; m < M; ++m) s += (double)gp[m] * y[m];
-                }
-                sQ[e] = s;
-            }
-            __syncthreads();
-            for (int o = lane; o < PK; o += 64) {
-                const int p = o / K, q = o - p * K;
-                for (int t = 1; t <= nt; ++t) {
-                    double s = 0.0;
-                    bool any = false;
-                    for (int j = 0; j < nc; ++j)
-                        if (sT[j] == t) {
-                            s += sQ[j * P + p] * sR[j * K + q];
-                            any = true;
-                        }
-                    if (any) part[(size_t)(t - 1) * PK + o] += wi * s;
-                }
-            }
-        }
-    }
-}
-
 int gen_grid(const SegmentSet &set) {
     const int rows = set.row0[set.nseg];
     return rows < 262144 ? rows : 262144;
@@ -683,58 +563,6 @@ bool launch_desc_gen_set(hipStream_t s, int lmax, int K, int P, const SegmentSet
     if (K < 1 || K > kGenKMax || P < 1 || P > kGenPMax) return false;
     return set.a[0].acc ? launch_gen<true>(s, lmax, K, P, set, nl, backward)
                         : launch_gen<false>(s, lmax, K, P, set, nl, backward);
-}
-
-static size_t wgrad_lds(int M, int P, int K, int ch) {
-    return sizeof(double) * (size_t)ch * (pad4(M) + P + K) + sizeof(float) * (size_t)P * pad4(M) +
-           sizeof(int) * (size_t)ch;
-}
-template <int LMAX, bool RADIAL>
-static bool run_wgrad(hipStream_t s, const SegmentArgs &a, const NeighborList &nl, int K, int P, const double *w,
-               double *partial, int nwg) {
-    constexpr int M = (LMAX + 1) * (LMAX + 1);
-    static const bool attr_set = [] {
-        JNN_HIP(hipFuncSetAttribute((const void *)k_gen_wgrad<LMAX, RADIAL>,
-                                    hipFuncAttributeMaxDynamicSharedMemorySize, (int)kGenLds));
-        return true;
-    }();
-    (void)attr_set;
-    int ch = 64;
-    while (ch > 8 && wgrad_lds(M, P, K, ch) > kGenLds) ch /= 2;
-    if (wgrad_lds(M, P, K, ch) > kGenLds) return false;
-    hipLaunchKernelGGL((k_gen_wgrad<LMAX, RADIAL>), dim3(nwg), dim3(64), wgrad_lds(M, P, K, ch), s, a, nl, K, P, ch,
-                       w, partial);
-    return true;
-}
-
-int mix_pgrad_groups(int nrows) { return std::max(1, std::min(nrows, 4 * device_cu_count())); }
-
-bool launch_mix_param_grad(hipStream_t s, int lmax, int K, int P, const SegmentArgs &a, const NeighborList &nl,
-                           const double *w, double *partial, double *out) {
-    if (K < 1 || K > kGenKMax || P < 1 || P > kGenPMax) return false;
-    const int nwg = mix_pgrad_groups(a.nrows);
-    bool ok;
-    switch (lmax) {
-        case kRadialLmax: ok = run_wgrad<0, true>(s, a, nl, K, P, w, partial, nwg); break;
-        case 0: ok = run_wgrad<0, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 1: ok = run_wgrad<1, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 2: ok = run_wgrad<2, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 3: ok = run_wgrad<3, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 4: ok = run_wgrad<4, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 5: ok = run_wgrad<5, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 6: ok = run_wgrad<6, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 7: ok = run_wgrad<7, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 8: ok = run_wgrad<8, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 9: ok = run_wgrad<9, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 10: ok = run_wgrad<10, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 11: ok = run_wgrad<11, false>(s, a, nl, K, P, w, partial, nwg); break;
-        case 12: ok = run_wgrad<12, false>(s, a, nl, K, P, w, partial, nwg); break;
-        default: return false;
-    }
-    if (!ok) return false;
-    const int np = a.ntypes * P * K;
-    launch_pgrad_reduce(s, nwg, np, partial, out);
-    return true;
 }
 
 std::string desc_gen_limit(int lmax, int K, int P, int ntypes, bool acc) {

@@ -195,6 +195,33 @@ __device__ __forceinline__ T dtheta_ladder(T cp, T sp, const Y *y, const G *g) {
 // dxy below 8 FLT_MIN is on the pole (cos phi = 1, sin phi = 0), the
 // reference's dxy == 0 convention (basis_SphericalUtil.hpp:165-181) with a
 // guard against denormal dxy.
+// dY_lm/dtheta of every real harmonic from the harmonics themselves (index
+// l^2+l+m), by the ladder identity of the associated Legendre functions
+// (dP_lm/dt = LB P_{l,m+1} - LA P_{l,m-1}) written back in the real basis.
+template <int LMAX, typename T>
+__device__ __forceinline__ void sph_dtheta(const T *y, T cp, T sp, T *yt) {
+    const ShTables<T> &sh = sh_tables<T>();
+    yt[0] = (T)0;
+#pragma unroll
+    for (int l = 1; l <= LMAX; ++l) {
+        const int idx = l * l + l;
+        yt[idx] = sh.L0[l] * (cp * y[idx + 1] + sp * y[idx - 1]);
+        yt[idx + 1] = -sh.L0[l] * cp * y[idx];
+        yt[idx - 1] = -sh.L0[l] * sp * y[idx];
+#pragma unroll
+        for (int m = 1; m <= l; ++m) {
+            if (m > 1) {
+                yt[idx + m] = -sh.LA[l][m] * (cp * y[idx + m - 1] - sp * y[idx - m + 1]);
+                yt[idx - m] = -sh.LA[l][m] * (cp * y[idx - m + 1] + sp * y[idx + m - 1]);
+            }
+            if (m < l) {
+                yt[idx + m] += sh.LB[l][m] * (cp * y[idx + m + 1] + sp * y[idx - m - 1]);
+                yt[idx - m] += sh.LB[l][m] * (cp * y[idx - m - 1] - sp * y[idx + m + 1]);
+            }
+        }
+    }
+}
+
 struct PairGeomR {
     float r, ir, ct, st, cp, sp, dxy, idxy;
     bool pole;

@@ -276,6 +276,40 @@ void launch_pgrad_reduce(hipStream_t s, int nwg, int np, const double *partial, 
 int mix_pgrad_groups(int nrows);
 bool launch_mix_param_grad(hipStream_t s, int lmax, int K, int P, const SegmentArgs &a, const NeighborList &nl,
                            const double *w, double *partial, double *out);
+
+// ---- training derivative of the force / virial loss ----
+// The loss is taken linear in the outputs: L = sum_i w_i E_i + sum_k bf_k . F_k
+// + sum_ab bv_ab V_ab (w, bf, bv: its gradients by the energies, forces and
+// total virial).  With F_k = sum over pairs (i,j) of (delta_kj - delta_ki) g_ij
+// and V_ab = sum over pairs of d_a g_b, the pairs' g = -dE_i/dd_ij enter as
+// sum_ij u_ij . g_ij = -D_u E_i, u_ij = bf_j - bf_i + sum_a bv_ab d_a: a
+// directional derivative of E_i with every displacement moved along u_ij.
+// The tangents and adjoints below are per segment row (a part's slices, as
+// part_segment narrows x and A).
+struct TangentArgs {
+    const double *bf;       // [N][3] dL/dF by atom (nullptr: none)
+    double bv[9];           // dL/dV, components xx yy zz xy xz yz yx zx zy
+    double *xdot;           // [nrows][xs] D_u x (the x slice of this part)
+    double *adot;           // [nrows][P M] D_u A of this part
+    const double *xbar;     // [nrows][xs] dL/dx
+    const double *xdbar;    // [nrows][xs] dL/d(D_u x)
+};
+// D_u A and D_u x of a segment (lmax = kRadialLmax: the radial basis, no A).
+// False outside the untiled kernels' limits.
+bool launch_desc_jvp(hipStream_t s, int lmax, int K, int P, const SegmentArgs &a, const NeighborList &nl,
+                     const TangentArgs &ta);
+// out[ntypes][P][K] = dL/dW of a trainable mixing from dL/dx and dL/d(D_u x)
+// (tangent args), summed over the segment's atoms; partial as
+// launch_mix_param_grad.
+bool launch_mix_param_grad2(hipStream_t s, int lmax, int K, int P, const SegmentArgs &a, const NeighborList &nl,
+                            const TangentArgs &ta, double *partial, double *out);
+// The networks' parameter gradient of sum_i (w_i E_i - D_u E_i) from x and
+// xdot = D_u x, with dL/dx and dL/dxdot of every row into xbar / xdbar
+// (nullptr: not written); partial holds mlp_dual_groups rows.  False when an
+// 8-row tile of the layers does not fit in LDS.
+int mlp_dual_groups(const MlpArgs &a);
+bool launch_mlp_dual_grad(hipStream_t s, const MlpArgs &a, const double *w, const double *xdot, double *xbar,
+                          double *xdbar, double *partial, double *out);
 // All species in one launch: the persistent workgroups are split among the
 // species in proportion to their row tiles (each stages its species' weights
 // once).  False when the species' MLP shapes differ, there are more than

@@ -334,6 +334,165 @@ __global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__re
         __syncthreads();
     }
 }
+// ---- training derivative of the force / virial loss: the networks' part ----
+// (reference: jse_nnap_backwardEnergyForce, nnap_interface.cpp:189-232, with
+// normedNnBackwardBackward and normedNnBackward, nnap_main.hpp; the layer
+// grad-grad of nn_FeedForward.hpp:264-320).  Per atom the network runs with a
+// tangent: xd = D_u x (the descriptor's directional derivative along the
+// pairs' upstream directions), z_l = W_l a_{l-1} + b_l, a_l = s(z_l),
+// zd_l = W_l ad_{l-1}, ad_l = s'(z_l) zd_l, E = sigma_E (w_o a_L + b_o) + mu_E,
+// D_u E = sigma_E w_o ad_L, and the loss of the atom is w_i E - D_u E (the
+// pairs' g enter the loss as -D_u E).  Reverse, with abar_L = w_i sigma_E w_o,
+// adbar_L = -sigma_E w_o:
+//   zdbar_l = adbar_l s'(z_l),  zbar_l = abar_l s'(z_l) + adbar_l zd_l s''(z_l),
+//   dW_l += zbar_l a_{l-1}^T + zdbar_l ad_{l-1}^T,  db_l += zbar_l,
+//   abar_{l-1} = W_l^T zbar_l,  adbar_{l-1} = W_l^T zdbar_l,
+//   dw_o += w_i sigma_E a_L - sigma_E ad_L,  db_o += w_i sigma_E,
+// and abar_0 / adbar_0 = dL/dx / dL/dxd go to xbar / xdbar for the mixing
+// weights' derivative.  Tiles of TR atoms, FP64, partial rows as k_mlp_pgrad.
+template <int TR>
+__global__ __launch_bounds__(256) void k_mlp_dual_pgrad(MlpArgs a, const double *__restrict__ w,
+                                                        const double *__restrict__ xdot, double *__restrict__ xbar,
+                                                        double *__restrict__ xdbar, int nparam,
+                                                        double *__restrict__ partial) {
+    extern __shared__ __align__(16) double pg_smem[];
+    const int S = a.S, L = a.nlayers;
+    const double *__restrict__ wb = a.w_out_in64;
+    const double *__restrict__ bs = a.bias64;
+    // LDS: x, xd [S][TR]; per hidden layer: a, s', s'', zd, ad, zbar, zdbar [h][TR]; ybar, ydbar [TR]
+    constexpr int NA = 7;
+    int aoff[kMaxLayers + 1];
+    aoff[0] = 0;
+    int off = 2 * S * TR;
+    for (int l = 1; l <= L; ++l) {
+        aoff[l] = off;
+        off += NA * a.hidden[l - 1] * TR;
+    }
+    double *yb = pg_smem + off, *ydb = yb + TR;
+    auto act = [&](int l) { return l == 0 ? pg_smem : pg_smem + aoff[l]; };
+    auto actd = [&](int l) { return l == 0 ? pg_smem + S * TR : pg_smem + aoff[l] + 4 * a.hidden[l - 1] * TR; };
+    auto der = [&](int l) { return pg_smem + aoff[l] + a.hidden[l - 1] * TR; };
+    auto der2 = [&](int l) { return pg_smem + aoff[l] + 2 * a.hidden[l - 1] * TR; };
+    auto zd = [&](int l) { return pg_smem + aoff[l] + 3 * a.hidden[l - 1] * TR; };
+    auto zb = [&](int l) { return pg_smem + aoff[l] + 5 * a.hidden[l - 1] * TR; };
+    auto zdb = [&](int l) { return pg_smem + aoff[l] + 6 * a.hidden[l - 1] * TR; };
+    int woff[kMaxLayers + 1], boff[kMaxLayers + 1];
+    int pw = 0;
+    for (int l = 1; l <= L; ++l) {
+        woff[l] = pw;
+        pw += a.hidden[l - 1] * (l == 1 ? S : a.hidden[l - 2]);
+    }
+    const int wo_off = pw;
+    const int hL = a.hidden[L - 1];
+    int pb = wo_off + hL;
+    for (int l = 1; l <= L; ++l) {
+        boff[l] = pb;
+        pb += a.hidden[l - 1];
+    }
+    const int bo_off = pb;
+    const double *__restrict__ wout = a.w_in_out64 + wo_off;
+    const int tid = threadIdx.x;
+    double *__restrict__ mine = partial + (size_t)blockIdx.x * nparam;
+    bool first = true;
+    for (int tile = blockIdx.x; tile * TR < a.nrows; tile += gridDim.x) {
+        const int row0 = tile * TR;
+        const int nt = min(TR, a.nrows - row0);
+        for (int idx = tid; idx < TR * S; idx += blockDim.x) {
+            const int r = idx / S, e = idx % S;
+            double v = 0.0, vd = 0.0;
+            if (r < nt) {
+                v = a.x64 ? a.x64[(size_t)(row0 + r) * S + e] : (double)a.x[(size_t)(row0 + r) * S + e];
+                vd = xdot[(size_t)(row0 + r) * S + e];
+            }
+            act(0)[e * TR + r] = v;
+            actd(0)[e * TR + r] = vd;
+        }
+        if (tid < TR) {
+            yb[tid] = tid < nt ? w[a.rows_atom[row0 + tid]] * a.sig_e : 0.0;
+            ydb[tid] = tid < nt ? -a.sig_e : 0.0;
+        }
+        __syncthreads();
+        for (int l = 1; l <= L; ++l) {  // forward with the tangent
+            const int h = a.hidden[l - 1], in = l == 1 ? S : a.hidden[l - 2];
+            const double *__restrict__ W = wb + woff[l];
+            const double *ap = act(l - 1), *adp = actd(l - 1);
+            for (int idx = tid; idx < h * TR; idx += blockDim.x) {
+                const int o = idx / TR, r = idx % TR;
+                double z = 0.0, t = 0.0;
+                for (int i = 0; i < in; ++i) {
+                    z += W[(size_t)o * in + i] * ap[i * TR + r];
+                    t += W[(size_t)o * in + i] * adp[i * TR + r];
+                }
+                z += bs[boff[l] - wo_off - hL + o];
+                const double sg = 1.0 / (1.0 + exp(-z));
+                const double d1 = sg * (1.0 + z * (1.0 - sg));
+                act(l)[idx] = z * sg;
+                der(l)[idx] = d1;
+                der2(l)[idx] = sg * (1.0 - sg) * (2.0 + z * (1.0 - 2.0 * sg));
+                zd(l)[idx] = t;
+                actd(l)[idx] = d1 * t;
+            }
+            __syncthreads();
+        }
+        for (int idx = tid; idx < hL * TR; idx += blockDim.x) {  // the last layer's adjoints
+            const int o = idx / TR, r = idx % TR;
+            const double ab = yb[r] * wout[o], adb = ydb[r] * wout[o];
+            zdb(L)[idx] = adb * der(L)[idx];
+            zb(L)[idx] = ab * der(L)[idx] + adb * zd(L)[idx] * der2(L)[idx];
+        }
+        __syncthreads();
+        for (int l = L; l >= 1; --l) {
+            const int h = a.hidden[l - 1], in = l == 1 ? S : a.hidden[l - 2];
+            const double *__restrict__ W = wb + woff[l];
+            if (l == 1 && !xbar) break;
+            for (int idx = tid; idx < in * TR; idx += blockDim.x) {
+                const int i = idx / TR, r = idx % TR;
+                double ab = 0.0, adb = 0.0;
+                for (int o = 0; o < h; ++o) {
+                    ab += W[(size_t)o * in + i] * zb(l)[o * TR + r];
+                    adb += W[(size_t)o * in + i] * zdb(l)[o * TR + r];
+                }
+                if (l > 1) {
+                    zdb(l - 1)[idx] = adb * der(l - 1)[idx];
+                    zb(l - 1)[idx] = ab * der(l - 1)[idx] + adb * zd(l - 1)[idx] * der2(l - 1)[idx];
+                } else if (r < nt) {
+                    xbar[(size_t)(row0 + r) * S + i] = ab;
+                    xdbar[(size_t)(row0 + r) * S + i] = adb;
+                }
+            }
+            __syncthreads();
+        }
+        auto put = [&](int p, double v) { mine[p] = first ? v : mine[p] + v; };
+        for (int l = 1; l <= L; ++l) {
+            const int h = a.hidden[l - 1], in = l == 1 ? S : a.hidden[l - 2];
+            const double *ap = act(l - 1), *adp = actd(l - 1);
+            for (int idx = tid; idx < h * in; idx += blockDim.x) {
+                const int o = idx / in, i = idx % in;
+                double s = 0.0;
+                for (int r = 0; r < TR; ++r)
+                    s += zb(l)[o * TR + r] * ap[i * TR + r] + zdb(l)[o * TR + r] * adp[i * TR + r];
+                put(woff[l] + idx, s);
+            }
+            for (int o = tid; o < h; o += blockDim.x) {
+                double s = 0.0;
+                for (int r = 0; r < TR; ++r) s += zb(l)[o * TR + r];
+                put(boff[l] + o, s);
+            }
+        }
+        for (int o = tid; o < hL; o += blockDim.x) {
+            double s = 0.0;
+            for (int r = 0; r < TR; ++r) s += yb[r] * act(L)[o * TR + r] + ydb[r] * actd(L)[o * TR + r];
+            put(wo_off + o, s);
+        }
+        if (tid == 0) {
+            double s = 0.0;
+            for (int r = 0; r < TR; ++r) s += yb[r];
+            put(bo_off, s);
+        }
+        first = false;
+        __syncthreads();
+    }
+}
 // out[p] = sum over the workgroups' partials, in workgroup order
 __global__ void k_pgrad_reduce(int nwg, int nparam, const double *__restrict__ partial, double *__restrict__ out) {
     const int p = blockIdx.x * blockDim.x + threadIdx.x;
@@ -354,6 +513,27 @@ int mlp_param_count(const MlpArgs &a) {
 void launch_pgrad_reduce(hipStream_t s, int nwg, int np, const double *partial, double *out) {
     hipLaunchKernelGGL(k_pgrad_reduce, dim3((np + 255) / 256), dim3(256), 0, s, nwg, np, partial, out);
 }
+bool launch_mlp_dual_grad(hipStream_t s, const MlpArgs &a, const double *w, const double *xdot, double *xbar,
+                          double *xdbar, double *partial, double *out) {
+    if (a.nrows <= 0) return true;
+    constexpr int TR = 8;
+    size_t f = (size_t)2 * a.S * TR + 2 * TR;
+    for (int l = 0; l < a.nlayers; ++l) f += 7 * (size_t)a.hidden[l] * TR;
+    const size_t lds = f * sizeof(double);
+    if (lds > 160 * 1024) return false;
+    static const bool attr_set = [] {
+        JNN_HIP(hipFuncSetAttribute((const void *)k_mlp_dual_pgrad<TR>, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                    160 * 1024));
+        return true;
+    }();
+    (void)attr_set;
+    const int np = mlp_param_count(a);
+    const int nwg = std::max(1, std::min((a.nrows + TR - 1) / TR, device_cu_count()));
+    hipLaunchKernelGGL((k_mlp_dual_pgrad<TR>), dim3(nwg), dim3(256), lds, s, a, w, xdot, xbar, xdbar, np, partial);
+    launch_pgrad_reduce(s, nwg, np, partial, out);
+    return true;
+}
+int mlp_dual_groups(const MlpArgs &a) { return std::max(1, std::min((a.nrows + 7) / 8, device_cu_count())); }
 int mlp_pgrad_groups(const MlpArgs &a) {
     const int tiles = (a.nrows + 15) / 16;
     return std::max(1, std::min(tiles, device_cu_count()));

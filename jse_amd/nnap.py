@@ -57,7 +57,7 @@ EXPORTS = [
     "jse_nnap_model_fp_size", "jse_nnap_model_destroy", "jse_nnap_system_create", "jse_nnap_system_destroy",
     "jse_nnap_system_set_box", "jse_nnap_system_set_positions", "jse_nnap_system_set_positions_device",
     "jse_nnap_system_compute", "jse_nnap_system_get_results", "jse_nnap_system_compute_fp",
-    "jse_nnap_system_backward_energy",
+    "jse_nnap_system_backward_energy", "jse_nnap_system_backward_energy_force",
     "jse_nnap_system_get_neighbors", "jse_nnap_system_benchmark", "jse_nnap_system_nl_stats",
     "jse_nnap_system_get_pair_gradients", "jse_nnap_batch_create", "jse_nnap_system_get_replica_results",
     "jse_nnap_system_set_graphs", "jse_nnap_system_autotune", "jse_nnap_system_set_tuning", "jse_nnap_server_create", "jse_nnap_server_add_replica", "jse_nnap_server_submit",
@@ -112,6 +112,8 @@ def lib():
     L.jse_nnap_system_get_results.argtypes = [vp, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p, _c_dbl_p]
     L.jse_nnap_system_compute_fp.argtypes = [vp, _c_dbl_p, ctypes.c_int]
     L.jse_nnap_system_backward_energy.argtypes = [vp, _c_dbl_p, ctypes.POINTER(_c_dbl_p), ctypes.POINTER(_c_dbl_p)]
+    L.jse_nnap_system_backward_energy_force.argtypes = [vp, _c_dbl_p, _c_dbl_p, _c_dbl_p, ctypes.POINTER(_c_dbl_p),
+                                                        ctypes.POINTER(_c_dbl_p)]
     L.jse_nnap_system_get_neighbors.argtypes = [vp, ctypes.c_int, _c_int_p, _c_int_p, _c_flt_p]
     L.jse_nnap_system_benchmark.argtypes = [vp, ctypes.c_int, ctypes.c_int, ctypes.c_int, _c_dbl_p, _c_dbl_p,
                                             ctypes.c_int]
@@ -274,6 +276,34 @@ class System:
         fps = [np.zeros(sum(nt * b.P * b.K for b in s.bases())) for s in sp] if mixing else None
         fptrs = (_c_dbl_p * nt)(*[_dp(o) for o in fps]) if mixing else None
         _check(lib().jse_nnap_system_backward_energy(self._h, _dp(w), ptrs, fptrs))
+        return (outs, fps) if mixing else outs
+
+    def backward_energy_force(self, w=None, bgrad_forces=None, bgrad_virial=None, mixing: bool = False):
+        """Parameter gradients of sum_i w[i] E_i + sum_k bgrad_forces[k] . F_k +
+        sum bgrad_virial . virial (jse_nnap_backwardEnergyForce over every atom, as
+        TrainerNNAP drives it): the network blocks, and with ``mixing`` also the
+        mixing-weight blocks, as backward_energy returns them."""
+        n = self.natoms
+        def arr(v, shape):
+            if v is None:
+                return None
+            v = np.ascontiguousarray(v, dtype=np.float64)
+            if v.shape != shape:
+                raise ValueError(f"expected shape {shape}")
+            return v
+        w = arr(w, (n,))
+        bf = arr(bgrad_forces, (n, 3))
+        bv = arr(bgrad_virial, (9,))
+        sp = self.nnap.model.species
+        nt = len(sp)
+        outs = [np.zeros(s.nn_param.size) for s in sp]
+        ptrs = (_c_dbl_p * nt)(*[_dp(o) for o in outs])
+        fps = [np.zeros(sum(nt * b.P * b.K for b in s.bases())) for s in sp] if mixing else None
+        fptrs = (_c_dbl_p * nt)(*[_dp(o) for o in fps]) if mixing else None
+        nul = ctypes.cast(None, _c_dbl_p)
+        _check(lib().jse_nnap_system_backward_energy_force(
+            self._h, _dp(w) if w is not None else nul, _dp(bf) if bf is not None else nul,
+            _dp(bv) if bv is not None else nul, ptrs, fptrs))
         return (outs, fps) if mixing else outs
 
     def neighbors(self, cap: Optional[int] = None):

@@ -110,3 +110,104 @@ def test_mixing_param_grad(O, precision, tol):
                 print(f"species {t + 1} part {k}: max |dg| / max |g| = {err:.2e}")
                 assert err <= tol
             off += sz
+
+
+# --------------------------------------------------------- force / stress loss
+def _lin_loss(orc, st, w, bf, bv):
+    """sum_i w_i E_i + sum_k bf_k . F_k + bv . virial on the oracle."""
+    r = orc.system_efv(st)
+    return w @ r["energies"] + float((bf * r["forces"]).sum()) + bv @ r["virial"]
+
+
+def _fd_check(O, mdl, st, got_nn, got_fp, w, bf, bv, tol, n_nn=8, n_w=6, seed=0):
+    """Central differences (h = 1e-5) of the oracle's linear loss by random
+    network and mixing entries against the gradient blocks."""
+    orc = O.Oracle(mdl)
+    rng = np.random.default_rng(seed)
+    h = 1e-5
+    def fd(arr, idx):
+        old = arr[idx]
+        arr[idx] = old + h
+        lp = _lin_loss(orc, st, w, bf, bv)
+        arr[idx] = old - h
+        lm = _lin_loss(orc, st, w, bf, bv)
+        arr[idx] = old
+        return (lp - lm) / (2 * h)
+    worst = 0.0
+    b0 = 0
+    for t, sp in enumerate(mdl.species):
+        scale = max(1.0, np.abs(got_nn[t]).max())
+        for idx in rng.choice(got_nn[t].size, size=n_nn, replace=False):
+            e = abs(fd(orc._nn[t], idx) - got_nn[t][idx]) / scale
+            worst = max(worst, e)
+        if got_fp is not None:
+            off = 0
+            for k, b in enumerate(sp.bases()):
+                sz = mdl.ntypes * b.P * b.K
+                blk = got_fp[t][off:off + sz]
+                if b.typed_wtype in (4, 5, 6):
+                    scale = max(1.0, np.abs(blk).max())
+                    for idx in rng.choice(sz, size=n_w, replace=False):
+                        e = abs(fd(orc._W[b0 + k], idx) - blk[idx]) / scale
+                        worst = max(worst, e)
+                else:
+                    assert not blk.any()
+                off += sz
+        b0 += len(sp.bases())
+    print(f"max |fd - g| / max |g| = {worst:.2e}")
+    assert worst <= tol
+
+
+def _upstream(st, seed):
+    rng = np.random.default_rng(seed)
+    return (rng.uniform(0.5, 1.5, size=st.natoms), rng.normal(size=(st.natoms, 3)),
+            np.array([0.3, -0.2, 0.1, 0.05, -0.07, 0.02, 0.0, 0.0, 0.0]))
+
+
+@pytest.mark.parametrize("precision,tol", [(nnap.FP32, 1e-7), (nnap.FP32_FAST, 2e-6)])
+def test_force_loss_grad_networks(O, precision, tol):
+    """The force / stress loss' derivative by the networks (backwardEnergyForce's
+    rBGradNnParam) on the ternary shapes, against central differences of the
+    oracle's linear loss sum w E + bf . F + bv . virial (measured 4.5e-9 of the
+    largest entry in the plain mode, 2.4e-7 in FP32_FAST)."""
+    st = W.ternary_liquid(n=400, seed=19)
+    md = M.random_model(["Cu", "Zr", "Al"], nmax=4, lmax=4, rcut=5.5, hidden=(16, 16), seed=29)
+    md = W._normalized(md, st, 200, 29)
+    mdl = M.load_model(md)
+    w, bf, bv = _upstream(st, 1)
+    with nnap.NNAP(md, precision=precision) as pot:
+        sysm = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+        gnn = sysm.backward_energy_force(w, bf, bv)
+        # without force / virial terms it is the energy loss' derivative
+        ge = sysm.backward_energy(w)
+        ge2 = sysm.backward_energy_force(w)
+    for a, b in zip(ge, ge2):
+        assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(a).max())
+    _fd_check(O, mdl, st, gnn, None, w, bf, bv, tol)
+
+
+def test_force_loss_grad_mixing_merge_l34(O):
+    """The same for every parameter block of a merge basis: an rfuse spherical
+    part with L3 / L4 invariants, a default part (no parameters) and an rfuse
+    radial part (backwardEnergyForce's rBGradFpParam: the descriptor's
+    backward-backward, basis_SphericalChebyshev.hpp:527-656)."""
+    st = W.ternary_liquid(n=400, seed=21)
+    parts = [{"type": "spherical_chebyshev", "nmax": 3, "lmax": 3, "l3max": 3, "l4max": 2, "rcut": 5.5,
+              "wtype": "rfuse"},
+             {"type": "spherical_chebyshev", "nmax": 2, "lmax": 2, "rcut": 4.5, "wtype": "default"},
+             {"type": "chebyshev", "nmax": 4, "rcut": 5.0, "wtype": "rfuse"}]
+    md = _trainable_mixing_model(["Cu", "Zr", "Al"], parts, (16, 16), 31, st, O)
+    mdl = M.load_model(md)
+    w, bf, bv = _upstream(st, 2)
+    with nnap.NNAP(md, precision=nnap.FP32) as pot:
+        sysm = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+        gnn, gfp = sysm.backward_energy_force(w, bf, bv, mixing=True)
+        ge = sysm.backward_energy(w, mixing=True)
+        ge2 = sysm.backward_energy_force(w, mixing=True)
+    # the energy loss' mixing gradient forms dE/dA from the FP32 copies of G and
+    # A, this one from FP64 adjoints: they agree to FP32 rounding
+    for a, b in zip(ge[0], ge2[0]):
+        assert np.abs(a - b).max() <= 1e-12 * max(1.0, np.abs(a).max())
+    for a, b in zip(ge[1], ge2[1]):
+        assert np.abs(a - b).max() <= 1e-6 * max(1.0, np.abs(a).max())
+    _fd_check(O, mdl, st, gnn, gfp, w, bf, bv, 2e-7)   # measured 2.5e-8
