@@ -200,8 +200,10 @@ __global__ __launch_bounds__(kFwdThreads) void k_gen_fwd(SegmentSet set, Neighbo
 // dE/dA[p][m] (ntypes x K rows of MP floats), then W.
 // ACC (the plain mode): the pair geometry, Y, Rn, fc and the derivatives in
 // FP64 from d + dlo; Y, Rn and fc rounded to FP32 for the contraction.
+// Two waves per SIMD are asked for up to lmax 6, where they fit in the
+// register file; the higher lmax need more registers than two waves hold.
 template <int LMAX, bool ACC>
-__global__ __launch_bounds__(64) void k_gen_bwd(SegmentSet set, NeighborList nl, int K, int P) {
+__global__ __launch_bounds__(64, (LMAX <= 6 ? 2 : 1)) void k_gen_bwd(SegmentSet set, NeighborList nl, int K, int P) {
     using GT = typename std::conditional<ACC, double, float>::type;  // geometry precision
     constexpr int L1 = LMAX + 1, M = L1 * L1, MP = pad4(M);
     extern __shared__ __align__(16) double gen_smem[];
