@@ -131,13 +131,23 @@ def test_autotuned_launch_shapes_keep_results(precision):
         b.close()
 
 
-def test_untiled_shape_in_captured_graphs(O):
-    """A model without tiled kernels (nmax 8, lmax 7, exfull mixing) in the
-    batched path: graph replays equal kernel-by-kernel evaluation, and the
-    replicas meet the FP32 bar against the oracle."""
+@pytest.mark.parametrize("kind", ["exfull_lmax7", "l3max4_l4max3"])
+def test_untiled_shape_in_captured_graphs(O, kind):
+    """A model without tiled kernels (nmax 8, lmax 7, exfull mixing; or
+    nmax 4, lmax 4 with L3 / L4 invariants) in the batched path: graph replays
+    equal kernel-by-kernel evaluation, and the replicas meet the FP32 bar
+    against the oracle."""
     reps = _reps(6, 64, 500, seed=14)
-    md = M.random_model(["Cu", "Zr", "Al"], nmax=8, lmax=7, rcut=6.0, hidden=(32, 32), seed=44, wtype="exfull")
-    md = W._normalized(md, reps[-1], 200, 44)
+    if kind == "exfull_lmax7":
+        md = M.random_model(["Cu", "Zr", "Al"], nmax=8, lmax=7, rcut=6.0, hidden=(32, 32), seed=44, wtype="exfull")
+        md = W._normalized(md, reps[-1], 200, 44)
+    else:
+        md = M.random_model(["Cu", "Zr", "Al"], nmax=4, lmax=4, rcut=6.0, hidden=(32, 32), seed=45, l3max=4,
+                            l4max=3)
+        mdl = M.load_model(md)
+        st = reps[-1]
+        fp = O.system_fp(O.Oracle(mdl), st)
+        md = M.set_normalization(md, [fp[st.types == t + 1, :mdl.species[t].S] for t in range(3)])
     pot = nnap.NNAP(md)
     b = pot.batch([_atoms(r) for r in reps])
     eager = nnap.System(pot, b.natoms, b.types, b.box, nnap.FP32, natoms_per_replica=b.rep_natoms)

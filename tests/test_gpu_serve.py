@@ -233,13 +233,22 @@ def test_server_errors():
         assert np.isfinite(res["energy"])
 
 
-def test_served_untiled_shape_matches_oracle(O):
+@pytest.mark.parametrize("kind", ["full_lmax5", "l3max3_l4max2"])
+def test_served_untiled_shape_matches_oracle(O, kind):
     """A model without tiled descriptor kernels (nmax 6, lmax 5, full mixing
-    of three species) served by continuous batching: every result meets the
-    FP32 bar of the oracle."""
+    of three species; or nmax 4, lmax 3 with L3 / L4 invariants) served by
+    continuous batching: every result meets the FP32 bar of the oracle."""
     reps = _replica_set(3, 150, 1200, 81)
-    md = M.random_model(["Cu", "Zr", "Al"], nmax=6, lmax=5, rcut=6.0, hidden=(32, 32), seed=45, wtype="full")
-    md = W._normalized(md, max(reps, key=lambda r: r.natoms), 200, 45)
+    big = max(reps, key=lambda r: r.natoms)
+    if kind == "full_lmax5":
+        md = M.random_model(["Cu", "Zr", "Al"], nmax=6, lmax=5, rcut=6.0, hidden=(32, 32), seed=45, wtype="full")
+        md = W._normalized(md, big, 200, 45)
+    else:
+        md = M.random_model(["Cu", "Zr", "Al"], nmax=4, lmax=3, rcut=6.0, hidden=(32, 32), seed=46, l3max=3,
+                            l4max=2)
+        mdl = M.load_model(md)
+        fp = O.system_fp(O.Oracle(mdl), big)
+        md = M.set_normalization(md, [fp[big.types == t + 1, :mdl.species[t].S] for t in range(3)])
     orc = O.Oracle(M.load_model(md))
     with nnap.NNAP(md) as pot, pot.server(precision=nnap.FP32, max_batch_atoms=8192) as srv:
         rids = [srv.add_replica(nnap.AtomData(r.box, r.xyz, r.types)) for r in reps]
