@@ -211,3 +211,40 @@ def test_force_loss_grad_mixing_merge_l34(O):
     for a, b in zip(ge[1], ge2[1]):
         assert np.abs(a - b).max() <= 1e-6 * max(1.0, np.abs(a).max())
     _fd_check(O, mdl, st, gnn, gfp, w, bf, bv, 2e-7)   # measured 2.5e-8
+
+
+def test_force_loss_grad_mirror_and_absent_species(O):
+    """A mirror species (its neighbours' roles swapped, evaluated with its own
+    copies of the mirrored species' network and rfuse mixing) and a species
+    with no atoms in the structure (zero blocks): every block against central
+    differences of the oracle's linear loss."""
+    st0 = W.ternary_liquid(n=400, seed=23)
+    st = W.Structure(st0.box, st0.xyz, np.where(st0.types == 3, 1, st0.types).astype(np.int32))  # no Al atoms
+    parts = [{"type": "spherical_chebyshev", "nmax": 3, "lmax": 3, "rcut": 5.5, "wtype": "rfuse"}]
+    md = _trainable_mixing_model(["Cu", "Zr", "Al"], parts, (12, 12), 37, st, O)
+    md["models"][1] = {"symbol": "Zr", "basis": {"type": "mirror", "mirror": 1}}
+    mdl = M.load_model(md)
+    assert mdl.species[1].mirror == 1
+    w, bf, bv = _upstream(st, 3)
+    with nnap.NNAP(md, precision=nnap.FP32) as pot:
+        sysm = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+        gnn, gfp = sysm.backward_energy_force(w, bf, bv, mixing=True)
+    assert not gnn[2].any() and not gfp[2].any()
+    _fd_check(O, mdl, st, gnn, None, w, bf, bv, 2e-7, seed=4)
+    # the mirror species evaluates with the mirrored species' mixing (the
+    # reference adds its mixing gradient into that species' block,
+    # nnap_main.hpp:345-346): one perturbed W moves both, so the difference
+    # quotient is the sum of the two blocks
+    orc = O.Oracle(mdl)
+    assert np.shares_memory(orc._W[0], orc._W[1])
+    tot = gfp[0] + gfp[1]
+    scale = np.abs(tot).max()
+    h = 1e-5
+    for idx in np.random.default_rng(5).choice(tot.size, size=8, replace=False):
+        old = orc._W[0][idx]
+        orc._W[0][idx] = old + h
+        lp = _lin_loss(orc, st, w, bf, bv)
+        orc._W[0][idx] = old - h
+        lm = _lin_loss(orc, st, w, bf, bv)
+        orc._W[0][idx] = old
+        assert abs((lp - lm) / (2 * h) - tot[idx]) <= 2e-7 * scale
