@@ -597,6 +597,32 @@ def served_c5(nrep=2000, nmin=100, nmax=20000, seed=23, sat_s=8.0, rate_s=6.0, f
     return out
 
 
+def training_derivatives(pot, st, iters=5):
+    """The training entry points on the same configuration (plain mode), per
+    call from host arrays to host gradient blocks: the energy loss'
+    derivative (jse_nnap_system_backward_energy) and the force / stress
+    loss' (jse_nnap_system_backward_energy_force), each running its own
+    neighbour build, descriptors and network passes.  Atom-evals/s counts one
+    atom per atom and call."""
+    a = nnap.AtomData(st.box, st.xyz, st.types)
+    s_ = pot.system(a)
+    rng = np.random.default_rng(0)
+    w = rng.uniform(0.5, 1.5, size=st.natoms)
+    bf = rng.normal(size=(st.natoms, 3))
+    bv = np.array([0.3, -0.2, 0.1, 0.05, -0.07, 0.02, 0.0, 0.0, 0.0])
+    out = {}
+    for name, call in (("energy_loss", lambda: s_.backward_energy(w)),
+                       ("force_loss", lambda: s_.backward_energy_force(w, bf, bv))):
+        call()
+        t0 = time.perf_counter()
+        for _ in range(iters):
+            call()
+        dt = (time.perf_counter() - t0) / iters
+        out[name] = {"ms_per_call": dt * 1e3, "atom_evals_per_s": st.natoms / dt}
+    out["note"] = "parameter gradients summed over the 32,000 atoms; host upstream in, host gradient blocks out"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -611,6 +637,7 @@ def main():
     ap.add_argument("--no-modes", action="store_true", help="skip the FP32_FAST / BF16 lines at 32,000 atoms")
     ap.add_argument("--no-c1", action="store_true", help="skip the Cu-256 line")
     ap.add_argument("--no-hooks", action="store_true", help="skip the PCIe-inclusive and GPUMD-hook lines")
+    ap.add_argument("--no-training", action="store_true", help="skip the training-derivative line")
     ap.add_argument("--served-replicas", type=int, default=2000)
     ap.add_argument("--knobs", type=str, default="",
                     help="JSON launch shapes for the 32,000-atom system (default: the committed tuned shapes "
@@ -709,6 +736,8 @@ def main():
     if not args.no_hooks:
         line["pcie_inclusive"] = pcie_inclusive(pot, st)
         line["gpumd_hook"] = gpumd_hook(pot, st)
+    if prec == nnap.FP32 and not args.no_training:
+        line["training_derivatives"] = training_derivatives(pot, st)
     if not args.no_cpu_baseline:
         base, err = cpu_baseline(md, st, res)
         line["cpu_baseline"] = base

@@ -66,7 +66,9 @@ __global__ __launch_bounds__(64) void k_gen_wgrad(SegmentArgs a, NeighborList nl
                 cheb_rn<kGenKMax>(1.0 - 2.0 * g.r * inv_rc, rn);
                 const double t = 1.0 - (g.r * inv_rc) * (g.r * inv_rc);
                 const double fc = g.r < rc ? t * t * t * t : 0.0;  // pairs beyond this basis' cutoff add nothing
-                for (int q = 0; q < K; ++q) sR[lane * K + q] = fc * rn[q];
+#pragma unroll
+                for (int q = 0; q < kGenKMax; ++q)  // unrolled: rn stays in registers
+                    if (q < K) sR[lane * K + q] = fc * rn[q];
             }
             __syncthreads();
             for (int e = lane; e < nc * P; e += 64) {
@@ -118,7 +120,9 @@ __device__ void pair_tangent(const SegmentArgs &a, const NeighborList &nl, const
     if (!(g.r < rc)) {
         if constexpr (!RADIAL)
             for (int m = 0; m < M; ++m) y[m] = yd[m] = 0.0;
-        for (int q = 0; q < K; ++q) R[q] = Rd[q] = 0.0;
+#pragma unroll
+        for (int q = 0; q < kGenKMax; ++q)
+            if (q < K) R[q] = Rd[q] = 0.0;
         return;
     }
     double u[3] = {0.0, 0.0, 0.0};
@@ -137,10 +141,12 @@ __device__ void pair_tangent(const SegmentArgs &a, const NeighborList &nl, const
     const double t = 1.0 - (g.r * inv_rc) * (g.r * inv_rc);
     const double t2 = t * t;
     const double fc = t2 * t2, dfc = -8.0 * g.r * inv_rc * inv_rc * t2 * t;
-    for (int q = 0; q < K; ++q) {
-        R[q] = fc * rn[q];
-        Rd[q] = ud * (dfc * rn[q] + fc * drn[q]);
-    }
+#pragma unroll
+    for (int q = 0; q < kGenKMax; ++q)  // unrolled: rn and drn stay in registers
+        if (q < K) {
+            R[q] = fc * rn[q];
+            Rd[q] = ud * (dfc * rn[q] + fc * drn[q]);
+        }
     if constexpr (!RADIAL) {
         real_sph_rows<LMAX, double, double>(g.ct, g.st, g.cp, g.sp, y);
         double yt[M];
@@ -160,17 +166,22 @@ __device__ void pair_tangent(const SegmentArgs &a, const NeighborList &nl, const
 // D_u A[p][m] = sum_j (D_u c_j[p] Y_j[m] + c_j[p] D_u Y_j[m]) (c = W R, D_u c = W D_u R)
 // and D_u x = D_u fp / sigma: 2 4pi/(2l+1) sum_m A D_u A on the L2 columns, the
 // invariants' directional derivatives on the L3 / L4 columns, sum_j D_u c_j
-// for the radial basis.  One 64-lane workgroup per atom (persistent), FP64.
-// LDS: Y, D_u Y [ch][MP], c, D_u c [ch][P].
+// for the radial basis.  One 64-lane workgroup per atom (persistent), FP64;
+// D_u A in 4 x 4 register tiles (eight 16-byte LDS reads and 32 multiply-adds
+// per neighbour and tile), summed over the neighbours in list order.
+// LDS: Y, D_u Y [ch][MP], c, D_u c [ch][PS] (rows padded with zeros), R, D_u R [ch][K].
 template <int LMAX, bool RADIAL>
 __global__ __launch_bounds__(64) void k_gen_jvp(SegmentArgs a, NeighborList nl, int K, int P, int ch,
                                                 TangentArgs ta) {
     constexpr int L1 = LMAX + 1, M = L1 * L1, MP = pad4(M);
     extern __shared__ __align__(16) double gen_smem[];
+    const int PS = pad4(P);
     double *sY = gen_smem, *sYd = sY + (size_t)ch * MP;
-    double *sC = sYd + (size_t)ch * MP, *sCd = sC + (size_t)ch * P;
+    double *sC = sYd + (size_t)ch * MP, *sCd = sC + (size_t)ch * PS;
+    double *sR = sCd + (size_t)ch * PS, *sRd = sR + (size_t)ch * K;
     const int lane = threadIdx.x;
     const int PM = P * M;
+    const int ntile = (PS / 4) * (MP / 4);  // 4 x 4 tiles of D_u A, taken by the lanes round-robin
     for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
         const int atom = a.rows_atom[row];
         const int i = a.nl_of ? a.nl_of[atom] : atom;
@@ -184,33 +195,55 @@ __global__ __launch_bounds__(64) void k_gen_jvp(SegmentArgs a, NeighborList nl, 
             const int nc = min(ch, n - c0);
             __syncthreads();  // the previous chunk's reads are done
             if (lane < nc) {
-                double R[kGenKMax], Rd[kGenKMax];
+                double *R = sR + lane * K, *Rd = sRd + lane * K;
                 int tj;
                 pair_tangent<LMAX, RADIAL>(a, nl, ta, base + c0 + lane, atom, K, sY + lane * MP, sYd + lane * MP,
                                            R, Rd, tj);
+                if constexpr (!RADIAL)
+                    for (int m = M; m < MP; ++m) sY[lane * MP + m] = sYd[lane * MP + m] = 0.0;
                 const double *w = a.W64 + (size_t)(tj - 1) * P * K;
-                for (int p = 0; p < P; ++p) {
+                for (int p = 0; p < PS; ++p) {
                     double c = 0.0, cd = 0.0;
-                    for (int q = 0; q < K; ++q) {
-                        c += w[p * K + q] * R[q];
-                        cd += w[p * K + q] * Rd[q];
-                    }
-                    sC[lane * P + p] = c;
-                    sCd[lane * P + p] = cd;
+                    if (p < P)
+                        for (int q = 0; q < K; ++q) {
+                            c += w[p * K + q] * R[q];
+                            cd += w[p * K + q] * Rd[q];
+                        }
+                    sC[lane * PS + p] = c;
+                    sCd[lane * PS + p] = cd;
                 }
             }
             __syncthreads();
             if constexpr (RADIAL) {
                 for (int j = 0; j < nc; ++j) {
-                    if (lane < P) racc[0] += sCd[j * P + lane];
-                    if (lane + 64 < P) racc[1] += sCd[j * P + lane + 64];
+                    if (lane < P) racc[0] += sCd[j * PS + lane];
+                    if (lane + 64 < P) racc[1] += sCd[j * PS + lane + 64];
                 }
             } else {
-                for (int e = lane; e < PM; e += 64) {
-                    const int p = e / M, m = e - p * M;
-                    double s = 0.0;
-                    for (int j = 0; j < nc; ++j) s += sCd[j * P + p] * sY[j * MP + m] + sC[j * P + p] * sYd[j * MP + m];
-                    adot[e] += s;
+                for (int tl = lane; tl < ntile; tl += 64) {
+                    const int p0 = (tl / (MP / 4)) * 4, m0 = (tl % (MP / 4)) * 4;
+                    double acc[4][4] = {};
+                    for (int j = 0; j < nc; ++j) {
+                        const double2 c01 = *reinterpret_cast<const double2 *>(sC + j * PS + p0);
+                        const double2 c23 = *reinterpret_cast<const double2 *>(sC + j * PS + p0 + 2);
+                        const double2 d01 = *reinterpret_cast<const double2 *>(sCd + j * PS + p0);
+                        const double2 d23 = *reinterpret_cast<const double2 *>(sCd + j * PS + p0 + 2);
+                        const double2 y01 = *reinterpret_cast<const double2 *>(sY + j * MP + m0);
+                        const double2 y23 = *reinterpret_cast<const double2 *>(sY + j * MP + m0 + 2);
+                        const double2 e01 = *reinterpret_cast<const double2 *>(sYd + j * MP + m0);
+                        const double2 e23 = *reinterpret_cast<const double2 *>(sYd + j * MP + m0 + 2);
+                        const double c[4] = {c01.x, c01.y, c23.x, c23.y}, cd[4] = {d01.x, d01.y, d23.x, d23.y};
+                        const double y[4] = {y01.x, y01.y, y23.x, y23.y}, yd[4] = {e01.x, e01.y, e23.x, e23.y};
+#pragma unroll
+                        for (int u = 0; u < 4; ++u)
+#pragma unroll
+                            for (int v = 0; v < 4; ++v) acc[u][v] += cd[u] * y[v] + c[u] * yd[v];
+                    }
+#pragma unroll
+                    for (int u = 0; u < 4; ++u)
+#pragma unroll
+                        for (int v = 0; v < 4; ++v)
+                            if (p0 + u < P && m0 + v < M) adot[(p0 + u) * M + m0 + v] += acc[u][v];
                 }
             }
         }
@@ -366,6 +399,9 @@ __global__ __launch_bounds__(64) void k_gen_wgrad2(SegmentArgs a, NeighborList n
 
 }  // namespace
 
+// Neighbour chunks are cut so that a workgroup's LDS stays near 24 KB (six
+// one-wave workgroups per CU) where the shape allows; up to 160 KB otherwise.
+constexpr size_t kTrainLds = 24 * 1024;
 static size_t wgrad_lds(int M, int P, int K, int ch) {
     return sizeof(double) * (size_t)ch * (pad4(M) + P + K) + sizeof(float) * (size_t)P * pad4(M) +
            sizeof(int) * (size_t)ch;
@@ -381,14 +417,14 @@ static bool run_wgrad(hipStream_t s, const SegmentArgs &a, const NeighborList &n
     }();
     (void)attr_set;
     int ch = 64;
-    while (ch > 8 && wgrad_lds(M, P, K, ch) > kGenLds) ch /= 2;
+    while (ch > 8 && wgrad_lds(M, P, K, ch) > kTrainLds) ch /= 2;
     if (wgrad_lds(M, P, K, ch) > kGenLds) return false;
     hipLaunchKernelGGL((k_gen_wgrad<LMAX, RADIAL>), dim3(nwg), dim3(64), wgrad_lds(M, P, K, ch), s, a, nl, K, P, ch,
                        w, partial);
     return true;
 }
 
-int mix_pgrad_groups(int nrows) { return std::max(1, std::min(nrows, 4 * device_cu_count())); }
+int mix_pgrad_groups(int nrows) { return std::max(1, std::min(nrows, 8 * device_cu_count())); }
 
 bool launch_mix_param_grad(hipStream_t s, int lmax, int K, int P, const SegmentArgs &a, const NeighborList &nl,
                            const double *w, double *partial, double *out) {
@@ -419,7 +455,9 @@ bool launch_mix_param_grad(hipStream_t s, int lmax, int K, int P, const SegmentA
 }
 
 
-static size_t jvp_lds(int M, int P, int ch) { return sizeof(double) * (size_t)ch * 2 * (pad4(M) + P); }
+static size_t jvp_lds(int M, int P, int K, int ch) {
+    return sizeof(double) * (size_t)ch * 2 * (pad4(M) + pad4(P) + K);
+}
 static size_t wgrad2_lds(int M, int P, int K, int ch) {
     return sizeof(double) * ((size_t)2 * P * pad4(M) + (size_t)ch * (2 * pad4(M) + 2 * K + 2 * P)) +
            sizeof(int) * (size_t)ch;
@@ -435,10 +473,10 @@ static bool run_jvp(hipStream_t s, const SegmentArgs &a, const NeighborList &nl,
     }();
     (void)attr_set;
     int ch = 64;
-    while (ch > 8 && jvp_lds(M, P, ch) > kGenLds) ch /= 2;
-    if (jvp_lds(M, P, ch) > kGenLds) return false;
-    const int grid = std::max(1, std::min(a.nrows, 4 * device_cu_count()));
-    hipLaunchKernelGGL((k_gen_jvp<LMAX, RADIAL>), dim3(grid), dim3(64), jvp_lds(M, P, ch), s, a, nl, K, P, ch, ta);
+    while (ch > 8 && jvp_lds(M, P, K, ch) > kTrainLds) ch /= 2;
+    if (jvp_lds(M, P, K, ch) > kGenLds) return false;
+    const int grid = std::max(1, std::min(a.nrows, 8 * device_cu_count()));
+    hipLaunchKernelGGL((k_gen_jvp<LMAX, RADIAL>), dim3(grid), dim3(64), jvp_lds(M, P, K, ch), s, a, nl, K, P, ch, ta);
     return true;
 }
 template <int LMAX, bool RADIAL>
@@ -452,7 +490,7 @@ static bool run_wgrad2(hipStream_t s, const SegmentArgs &a, const NeighborList &
     }();
     (void)attr_set;
     int ch = 64;
-    while (ch > 8 && wgrad2_lds(M, P, K, ch) > kGenLds) ch /= 2;
+    while (ch > 8 && wgrad2_lds(M, P, K, ch) > kTrainLds) ch /= 2;
     if (wgrad2_lds(M, P, K, ch) > kGenLds) return false;
     hipLaunchKernelGGL((k_gen_wgrad2<LMAX, RADIAL>), dim3(nwg), dim3(64), wgrad2_lds(M, P, K, ch), s, a, nl, K, P,
                        ch, ta, partial);

@@ -528,15 +528,16 @@ bool launch_mlp_dual_grad(hipStream_t s, const MlpArgs &a, const double *w, cons
     }();
     (void)attr_set;
     const int np = mlp_param_count(a);
-    const int nwg = std::max(1, std::min((a.nrows + TR - 1) / TR, device_cu_count()));
+    const int nwg = mlp_dual_groups(a);
     hipLaunchKernelGGL((k_mlp_dual_pgrad<TR>), dim3(nwg), dim3(256), lds, s, a, w, xdot, xbar, xdbar, np, partial);
     launch_pgrad_reduce(s, nwg, np, partial, out);
     return true;
 }
-int mlp_dual_groups(const MlpArgs &a) { return std::max(1, std::min((a.nrows + 7) / 8, device_cu_count())); }
-int mlp_pgrad_groups(const MlpArgs &a) {
+// two workgroups per CU (their LDS tiles fit twice); one partial row each
+int mlp_dual_groups(const MlpArgs &a) { return std::max(1, std::min((a.nrows + 7) / 8, 2 * device_cu_count())); }
+int mlp_pgrad_groups(const MlpArgs &a) {  // two workgroups per CU, one partial row each
     const int tiles = (a.nrows + 15) / 16;
-    return std::max(1, std::min(tiles, device_cu_count()));
+    return std::max(1, std::min(tiles, 2 * device_cu_count()));
 }
 bool launch_mlp_param_grad(hipStream_t s, const MlpArgs &a, const double *w, double *partial, double *out) {
     if (a.nrows <= 0) return true;
