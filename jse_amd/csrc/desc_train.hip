@@ -169,7 +169,8 @@ __device__ void pair_tangent(const SegmentArgs &a, const NeighborList &nl, const
 // for the radial basis.  One 64-lane workgroup per atom (persistent), FP64;
 // D_u A in 4 x 4 register tiles (eight 16-byte LDS reads and 32 multiply-adds
 // per neighbour and tile), summed over the neighbours in list order.
-// LDS: Y, D_u Y [ch][MP], c, D_u c [ch][PS] (rows padded with zeros), R, D_u R [ch][K].
+// LDS: Y, D_u Y [ch][MP], c, D_u c [ch][PS] (rows padded with zeros), R, D_u R [ch][K],
+// species [ch].
 template <int LMAX, bool RADIAL>
 __global__ __launch_bounds__(64) void k_gen_jvp(SegmentArgs a, NeighborList nl, int K, int P, int ch,
                                                 TangentArgs ta) {
@@ -179,6 +180,7 @@ __global__ __launch_bounds__(64) void k_gen_jvp(SegmentArgs a, NeighborList nl, 
     double *sY = gen_smem, *sYd = sY + (size_t)ch * MP;
     double *sC = sYd + (size_t)ch * MP, *sCd = sC + (size_t)ch * PS;
     double *sR = sCd + (size_t)ch * PS, *sRd = sR + (size_t)ch * K;
+    int *sT = reinterpret_cast<int *>(sRd + (size_t)ch * K);
     const int lane = threadIdx.x;
     const int PM = P * M;
     const int ntile = (PS / 4) * (MP / 4);  // 4 x 4 tiles of D_u A, taken by the lanes round-robin
@@ -195,23 +197,27 @@ __global__ __launch_bounds__(64) void k_gen_jvp(SegmentArgs a, NeighborList nl, 
             const int nc = min(ch, n - c0);
             __syncthreads();  // the previous chunk's reads are done
             if (lane < nc) {
-                double *R = sR + lane * K, *Rd = sRd + lane * K;
                 int tj;
                 pair_tangent<LMAX, RADIAL>(a, nl, ta, base + c0 + lane, atom, K, sY + lane * MP, sYd + lane * MP,
-                                           R, Rd, tj);
+                                           sR + lane * K, sRd + lane * K, tj);
+                sT[lane] = tj;
                 if constexpr (!RADIAL)
                     for (int m = M; m < MP; ++m) sY[lane * MP + m] = sYd[lane * MP + m] = 0.0;
-                const double *w = a.W64 + (size_t)(tj - 1) * P * K;
-                for (int p = 0; p < PS; ++p) {
-                    double c = 0.0, cd = 0.0;
-                    if (p < P)
-                        for (int q = 0; q < K; ++q) {
-                            c += w[p * K + q] * R[q];
-                            cd += w[p * K + q] * Rd[q];
-                        }
-                    sC[lane * PS + p] = c;
-                    sCd[lane * PS + p] = cd;
+            }
+            __syncthreads();
+            for (int e = lane; e < nc * PS; e += 64) {  // c, D_u c: every lane, the mixing read from L1 / L2
+                const int j = e / PS, p = e - j * PS;
+                double c = 0.0, cd = 0.0;
+                if (p < P) {
+                    const double *w = a.W64 + ((size_t)(sT[j] - 1) * P + p) * K;
+                    const double *R = sR + j * K, *Rd = sRd + j * K;
+                    for (int q = 0; q < K; ++q) {
+                        c += w[q] * R[q];
+                        cd += w[q] * Rd[q];
+                    }
                 }
+                sC[e] = c;
+                sCd[e] = cd;
             }
             __syncthreads();
             if constexpr (RADIAL) {
@@ -456,7 +462,7 @@ bool launch_mix_param_grad(hipStream_t s, int lmax, int K, int P, const SegmentA
 
 
 static size_t jvp_lds(int M, int P, int K, int ch) {
-    return sizeof(double) * (size_t)ch * 2 * (pad4(M) + pad4(P) + K);
+    return sizeof(double) * (size_t)ch * 2 * (pad4(M) + pad4(P) + K) + sizeof(int) * (size_t)ch;
 }
 static size_t wgrad2_lds(int M, int P, int K, int ch) {
     return sizeof(double) * ((size_t)2 * P * pad4(M) + (size_t)ch * (2 * pad4(M) + 2 * K + 2 * P)) +

@@ -210,6 +210,46 @@ bool launch_mlp_fp64(hipStream_t s, const MlpArgs &a) { return launch_mlp_vec<do
 
 namespace jnn {
 
+// sum_i W[i ws] v[i TR] over i < n in order, one accumulator (the plain
+// loop's sum, bit for bit), with the weights loaded from global memory eight
+// at a time so that eight loads are in flight; dot_gw2 forms two such sums
+// over the same weights.
+template <int TR>
+__device__ __forceinline__ double dot_gw(const double *__restrict__ W, int ws, const double *v, int n) {
+    double z = 0.0;
+    int i = 0;
+    for (; i + 8 <= n; i += 8) {
+        double wv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) wv[u] = W[(size_t)(i + u) * ws];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) z += wv[u] * v[(i + u) * TR];
+    }
+    for (; i < n; ++i) z += W[(size_t)i * ws] * v[i * TR];
+    return z;
+}
+template <int TR>
+__device__ __forceinline__ void dot_gw2(const double *__restrict__ W, int ws, const double *v1, const double *v2,
+                                        int n, double &z1, double &z2) {
+    z1 = 0.0;
+    z2 = 0.0;
+    int i = 0;
+    for (; i + 8 <= n; i += 8) {
+        double wv[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) wv[u] = W[(size_t)(i + u) * ws];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) {
+            z1 += wv[u] * v1[(i + u) * TR];
+            z2 += wv[u] * v2[(i + u) * TR];
+        }
+    }
+    for (; i < n; ++i) {
+        z1 += W[(size_t)i * ws] * v1[i * TR];
+        z2 += W[(size_t)i * ws] * v2[i * TR];
+    }
+}
+
 // ---- training derivative: the network parameters' gradient of sum_i w_i E_i ----
 // (reference: jse_nnap_backwardEnergy, nnap_interface.cpp:111-134, through
 // normedNnBackward with GRAD_PARAM, nnap_main.hpp:297-323, and the layer
@@ -225,6 +265,7 @@ namespace jnn {
 template <int TR>
 __global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__restrict__ w, int nparam,
                                                    double *__restrict__ partial) {
+    static_assert(TR <= 32 && (TR & (TR - 1)) == 0, "the tile's rows are a power of two up to 32");
     extern __shared__ __align__(16) double pg_smem[];
     const int S = a.S, L = a.nlayers;
     const double *__restrict__ wb = a.w_out_in64;  // per layer [out][in]
@@ -278,8 +319,7 @@ __global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__re
             const double *__restrict__ W = wb + woff[l];
             for (int idx = tid; idx < h * TR; idx += blockDim.x) {
                 const int o = idx / TR, r = idx % TR;
-                double z = 0.0;
-                for (int i = 0; i < in; ++i) z += W[(size_t)o * in + i] * pg_smem[aoff[l - 1] + i * TR + r];
+                double z = dot_gw<TR>(W + (size_t)o * in, 1, pg_smem + aoff[l - 1] + r, in);
                 z += bs[boff[l] - wo_off - hL + o];  // hidden biases come first in the bias block
                 double g;
                 pg_smem[aoff[l] + idx] = silu_and_grad(z, &g);
@@ -298,8 +338,7 @@ __global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__re
             const double *__restrict__ W = wb + woff[l];
             for (int idx = tid; idx < in * TR; idx += blockDim.x) {
                 const int i = idx / TR, r = idx % TR;
-                double s = 0.0;
-                for (int o = 0; o < h; ++o) s += W[(size_t)o * in + i] * pg_smem[goff[l] + o * TR + r];
+                const double s = dot_gw<TR>(W + i, in, pg_smem + goff[l] + r, h);
                 pg_smem[goff[l - 1] + idx] = s * pg_smem[doff[l - 1] + idx];
             }
             __syncthreads();
@@ -311,7 +350,10 @@ __global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__re
             for (int idx = tid; idx < h * in; idx += blockDim.x) {
                 const int o = idx / in, i = idx % in;
                 double s = 0.0;
-                for (int r = 0; r < TR; ++r) s += pg_smem[goff[l] + o * TR + r] * pg_smem[aoff[l - 1] + i * TR + r];
+                for (int u = 0; u < TR; ++u) {  // rows rotated per i: no two lanes of a half-wave on one bank
+                    const int r = (u + i / (32 / TR)) & (TR - 1);
+                    s += pg_smem[goff[l] + o * TR + r] * pg_smem[aoff[l - 1] + i * TR + r];
+                }
                 put(woff[l] + idx, s);
             }
             for (int o = tid; o < h; o += blockDim.x) {
@@ -355,6 +397,7 @@ __global__ __launch_bounds__(256) void k_mlp_dual_pgrad(MlpArgs a, const double 
                                                         const double *__restrict__ xdot, double *__restrict__ xbar,
                                                         double *__restrict__ xdbar, int nparam,
                                                         double *__restrict__ partial) {
+    static_assert(TR <= 32 && (TR & (TR - 1)) == 0, "the tile's rows are a power of two up to 32");
     extern __shared__ __align__(16) double pg_smem[];
     const int S = a.S, L = a.nlayers;
     const double *__restrict__ wb = a.w_out_in64;
@@ -418,11 +461,8 @@ __global__ __launch_bounds__(256) void k_mlp_dual_pgrad(MlpArgs a, const double 
             const double *ap = act(l - 1), *adp = actd(l - 1);
             for (int idx = tid; idx < h * TR; idx += blockDim.x) {
                 const int o = idx / TR, r = idx % TR;
-                double z = 0.0, t = 0.0;
-                for (int i = 0; i < in; ++i) {
-                    z += W[(size_t)o * in + i] * ap[i * TR + r];
-                    t += W[(size_t)o * in + i] * adp[i * TR + r];
-                }
+                double z, t;
+                dot_gw2<TR>(W + (size_t)o * in, 1, ap + r, adp + r, in, z, t);
                 z += bs[boff[l] - wo_off - hL + o];
                 const double sg = 1.0 / (1.0 + exp(-z));
                 const double d1 = sg * (1.0 + z * (1.0 - sg));
@@ -447,11 +487,8 @@ __global__ __launch_bounds__(256) void k_mlp_dual_pgrad(MlpArgs a, const double 
             if (l == 1 && !xbar) break;
             for (int idx = tid; idx < in * TR; idx += blockDim.x) {
                 const int i = idx / TR, r = idx % TR;
-                double ab = 0.0, adb = 0.0;
-                for (int o = 0; o < h; ++o) {
-                    ab += W[(size_t)o * in + i] * zb(l)[o * TR + r];
-                    adb += W[(size_t)o * in + i] * zdb(l)[o * TR + r];
-                }
+                double ab, adb;
+                dot_gw2<TR>(W + i, in, zb(l) + r, zdb(l) + r, h, ab, adb);
                 if (l > 1) {
                     zdb(l - 1)[idx] = adb * der(l - 1)[idx];
                     zb(l - 1)[idx] = ab * der(l - 1)[idx] + adb * zd(l - 1)[idx] * der2(l - 1)[idx];
@@ -469,8 +506,10 @@ __global__ __launch_bounds__(256) void k_mlp_dual_pgrad(MlpArgs a, const double 
             for (int idx = tid; idx < h * in; idx += blockDim.x) {
                 const int o = idx / in, i = idx % in;
                 double s = 0.0;
-                for (int r = 0; r < TR; ++r)
+                for (int u = 0; u < TR; ++u) {  // rows rotated per i: no two lanes of a half-wave on one bank
+                    const int r = (u + i / (32 / TR)) & (TR - 1);
                     s += zb(l)[o * TR + r] * ap[i * TR + r] + zdb(l)[o * TR + r] * adp[i * TR + r];
+                }
                 put(woff[l] + idx, s);
             }
             for (int o = tid; o < h; o += blockDim.x) {
@@ -493,13 +532,28 @@ __global__ __launch_bounds__(256) void k_mlp_dual_pgrad(MlpArgs a, const double 
         __syncthreads();
     }
 }
-// out[p] = sum over the workgroups' partials, in workgroup order
-__global__ void k_pgrad_reduce(int nwg, int nparam, const double *__restrict__ partial, double *__restrict__ out) {
-    const int p = blockIdx.x * blockDim.x + threadIdx.x;
-    if (p >= nparam) return;
+// out[p] = sum over the workgroups' partial rows: 64 parameters per
+// workgroup; wave v sums rows v, v + 8, v + 16, ... in order, then the eight
+// sums are added in wave order (a fixed order, whatever the timing).
+constexpr int kReduceWaves = 8;
+__global__ __launch_bounds__(64 * kReduceWaves) void k_pgrad_reduce(int nwg, int nparam,
+                                                                    const double *__restrict__ partial,
+                                                                    double *__restrict__ out) {
+    __shared__ double s_sum[kReduceWaves][64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int p = blockIdx.x * 64 + lane;
     double s = 0.0;
-    for (int g = 0; g < nwg; ++g) s += partial[(size_t)g * nparam + p];
-    out[p] = s;
+    if (p < nparam) {
+#pragma unroll 8
+        for (int g = wv; g < nwg; g += kReduceWaves) s += partial[(size_t)g * nparam + p];
+    }
+    s_sum[wv][lane] = s;
+    __syncthreads();
+    if (wv == 0 && p < nparam) {
+        double t = 0.0;
+        for (int v = 0; v < kReduceWaves; ++v) t += s_sum[v][lane];
+        out[p] = t;
+    }
 }
 
 int mlp_param_count(const MlpArgs &a) {
@@ -511,7 +565,7 @@ int mlp_param_count(const MlpArgs &a) {
     return n + in + 1;
 }
 void launch_pgrad_reduce(hipStream_t s, int nwg, int np, const double *partial, double *out) {
-    hipLaunchKernelGGL(k_pgrad_reduce, dim3((np + 255) / 256), dim3(256), 0, s, nwg, np, partial, out);
+    hipLaunchKernelGGL(k_pgrad_reduce, dim3((np + 63) / 64), dim3(64 * kReduceWaves), 0, s, nwg, np, partial, out);
 }
 bool launch_mlp_dual_grad(hipStream_t s, const MlpArgs &a, const double *w, const double *xdot, double *xbar,
                           double *xdbar, double *partial, double *out) {
@@ -554,7 +608,7 @@ bool launch_mlp_param_grad(hipStream_t s, const MlpArgs &a, const double *w, dou
     (void)attr_set;
     const int np = mlp_param_count(a), nwg = mlp_pgrad_groups(a);
     hipLaunchKernelGGL((k_mlp_pgrad<TR>), dim3(nwg), dim3(256), lds, s, a, w, np, partial);
-    hipLaunchKernelGGL(k_pgrad_reduce, dim3((np + 255) / 256), dim3(256), 0, s, nwg, np, partial, out);
+    launch_pgrad_reduce(s, nwg, np, partial, out);
     return true;
 }
 
