@@ -361,6 +361,17 @@ struct jse_nnap_system_s {
     jse_nnap_model model = nullptr;
     int n = 0;       // atoms over all replicas
     int nrep = 0;    // replicas (independent periodic boxes)
+    // A box with an edge shorter than the cutoff (the reference's
+    // ExpandLinkedCell, NeighborListGetter.java:704-776, 985-995; single
+    // systems): the system holds the supercell of cp[0] x cp[1] x cp[2] copies of
+    // the box, every edge of which reaches the cutoff, so that the cell list
+    // finds each pair within the cutoff once per copy.  Copy c = cx + cp[0]
+    // (cy + cp[1] cz) of the caller's atom i is atom c * nuser + i at
+    // x_i + (cx, cy, cz) * box; copy 0 is the caller's atom and carries its
+    // results, and totals are the supercell's divided by ncopy.  nuser == n
+    // without copies.
+    int nuser = 0, ncopy = 1, cp[3] = {1, 1, 1};
+    double ubox[3] = {0.0, 0.0, 0.0};
     int precision = JSE_NNAP_FP32;
     std::vector<int> types;
     std::vector<int> rep_natoms;
@@ -482,6 +493,7 @@ struct jse_nnap_system_s {
         }
         JNN_CHECK(tot < (1 << 26), ERR_ARG, "natoms out of range [0, 2^26)");
         n = (int)tot;
+        nuser = n;
         rep_natoms.assign(natoms_per, natoms_per + nr);
         const int nt = m->host.ntypes();
         types.assign(ty, ty + n);
@@ -590,6 +602,16 @@ struct jse_nnap_system_s {
             else build_nl(false);
             JNN_HIP(hipStreamSynchronize(stream));
         }
+    }
+
+    // The caller's per-atom rows (width values each; nullptr: zeros) into a
+    // device array of n rows: the caller's atoms fill the first nuser rows and
+    // a supercell's copies get zeros (their terms are those of the caller's atoms).
+    void upload_atom_rows(double *dst, const double *src, int width) {
+        const size_t nu = (size_t)nuser * width, all = (size_t)n * width;
+        if (src && nu) JNN_HIP(hipMemcpyAsync(dst, src, sizeof(double) * nu, hipMemcpyHostToDevice, stream));
+        const size_t z0 = src ? nu : 0;
+        if (all > z0) JNN_HIP(hipMemsetAsync(dst + z0, 0, sizeof(double) * (all - z0), stream));
     }
 
     SegmentArgs seg_args(int t, bool raw) const {
@@ -902,12 +924,46 @@ int jse_nnap_sh_invariants(int l3max, int l4max, const double *A, double *out) {
     });
 }
 
+// Copies per axis of the supercell of a box: 1 where the edge reaches rc,
+// else the fewest whose total length does.
+static void box_copies(const double *box, double rc, int cp[3]) {
+    for (int d = 0; d < 3; ++d) {
+        JNN_CHECK(std::isfinite(box[d]) && box[d] > 0, ERR_ARG, "box lengths must be positive and finite");
+        int c = 1;
+        if (box[d] < rc) {
+            c = (int)std::ceil(rc / box[d]);
+            while (c * box[d] < rc) ++c;
+        }
+        cp[d] = c;
+    }
+}
+
 int jse_nnap_system_create(jse_nnap_model m, int natoms, const int *types, const double *box, int precision,
                            jse_nnap_system *out) {
     return guard([&] {
         JNN_CHECK(m && out && box && (types || natoms == 0), ERR_ARG, "null argument");
+        JNN_CHECK(natoms >= 0, ERR_ARG, "negative atom count");
+        int cp[3];
+        box_copies(box, m->host.rcut_max(), cp);
+        const long long ncopy = (long long)cp[0] * cp[1] * cp[2];
+        JNN_CHECK((long long)natoms * ncopy < (1 << 26), ERR_ARG,
+                  "natoms times the box copies of a box shorter than the cutoff out of range [0, 2^26)");
         auto s = std::make_unique<jse_nnap_system_s>();
-        s->init(m, 1, &natoms, types, box, precision);
+        if (ncopy == 1) {
+            s->init(m, 1, &natoms, types, box, precision);
+        } else {
+            const int nin = (int)(natoms * ncopy);
+            std::vector<int> ty((size_t)nin);
+            for (long long c = 0; c < ncopy; ++c) std::copy(types, types + natoms, ty.begin() + c * natoms);
+            const double sbox[3] = {box[0] * cp[0], box[1] * cp[1], box[2] * cp[2]};
+            s->init(m, 1, &nin, ty.data(), sbox, precision);
+        }
+        s->nuser = natoms;
+        s->ncopy = (int)ncopy;
+        for (int d = 0; d < 3; ++d) {
+            s->cp[d] = cp[d];
+            s->ubox[d] = box[d];
+        }
         *out = s.release();
     });
 }
@@ -932,10 +988,11 @@ int jse_nnap_system_get_replica_results(jse_nnap_system s, double *energy, doubl
         std::vector<double> tot((size_t)10 * s->nrep);
         JNN_HIP(hipMemcpyAsync(tot.data(), s->d_tot, sizeof(double) * tot.size(), hipMemcpyDeviceToHost, s->stream));
         JNN_HIP(hipStreamSynchronize(s->stream));
+        const double inv = 1.0 / s->ncopy;  // a supercell's totals per box
         for (int q = 0; q < s->nrep; ++q) {
-            if (energy) energy[q] = tot[(size_t)10 * q];
+            if (energy) energy[q] = tot[(size_t)10 * q] * inv;
             if (virial)
-                for (int c = 0; c < 9; ++c) virial[(size_t)9 * q + c] = tot[(size_t)10 * q + 1 + c];
+                for (int c = 0; c < 9; ++c) virial[(size_t)9 * q + c] = tot[(size_t)10 * q + 1 + c] * inv;
         }
     });
 }
@@ -973,25 +1030,51 @@ void jse_nnap_system_destroy(jse_nnap_system s) { delete s; }
 int jse_nnap_system_set_box(jse_nnap_system s, const double *box) {
     return guard([&] {
         JNN_CHECK(s && box, ERR_ARG, "null argument");
-        s->set_boxes(box);
+        if (s->nrep == 1) {
+            int cp[3];
+            box_copies(box, s->model->host.rcut_max(), cp);
+            JNN_CHECK(cp[0] == s->cp[0] && cp[1] == s->cp[1] && cp[2] == s->cp[2], ERR_UNSUPPORTED,
+                      "the new box needs another number of copies of a box shorter than the cutoff "
+                      "(a box edge crossed the cutoff): create a new system for it");
+            const double sbox[3] = {box[0] * cp[0], box[1] * cp[1], box[2] * cp[2]};
+            s->set_boxes(sbox);
+            for (int d = 0; d < 3; ++d) s->ubox[d] = box[d];
+        } else {
+            s->set_boxes(box);
+        }
     });
 }
 
 int jse_nnap_system_set_positions(jse_nnap_system s, const double *xyz) {
     return guard([&] {
-        JNN_CHECK(s && (xyz || s->n == 0), ERR_ARG, "null argument");
-        for (size_t q = 0; q < (size_t)3 * s->n; ++q)
+        JNN_CHECK(s && (xyz || s->nuser == 0), ERR_ARG, "null argument");
+        for (size_t q = 0; q < (size_t)3 * s->nuser; ++q)
             JNN_CHECK(std::isfinite(xyz[q]), ERR_ARG, "non-finite position at atom " + std::to_string(q / 3));
-        if (s->n) JNN_HIP(hipMemcpyAsync(s->d_xyz, xyz, sizeof(double) * 3 * s->n, hipMemcpyHostToDevice, s->stream));
-        JNN_HIP(hipStreamSynchronize(s->stream));
+        if (s->ncopy > 1) {
+            std::vector<double> x((size_t)3 * s->n);
+            for (int c = 0; c < s->ncopy; ++c) {
+                const int cx = c % s->cp[0], cy = (c / s->cp[0]) % s->cp[1], cz = c / (s->cp[0] * s->cp[1]);
+                const double sh[3] = {s->ubox[0] * cx, s->ubox[1] * cy, s->ubox[2] * cz};
+                for (size_t i = 0; i < (size_t)s->nuser; ++i)
+                    for (int d = 0; d < 3; ++d) x[3 * ((size_t)c * s->nuser + i) + d] = xyz[3 * i + d] + sh[d];
+            }
+            JNN_HIP(hipMemcpyAsync(s->d_xyz, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice, s->stream));
+            JNN_HIP(hipStreamSynchronize(s->stream));
+        } else {
+            if (s->n)
+                JNN_HIP(hipMemcpyAsync(s->d_xyz, xyz, sizeof(double) * 3 * s->n, hipMemcpyHostToDevice, s->stream));
+            JNN_HIP(hipStreamSynchronize(s->stream));
+        }
         s->have_pos = true;
         s->have_nl = false;
     });
 }
 int jse_nnap_system_set_positions_device(jse_nnap_system s, const double *d_xyz) {
     return guard([&] {
-        JNN_CHECK(s && (d_xyz || s->n == 0), ERR_ARG, "null argument");
-        if (s->n)
+        JNN_CHECK(s && (d_xyz || s->nuser == 0), ERR_ARG, "null argument");
+        if (s->ncopy > 1)
+            launch_expand_copies(s->stream, s->nuser, s->cp, s->ubox, d_xyz, s->d_xyz);
+        else if (s->n)
             JNN_HIP(hipMemcpyAsync(s->d_xyz, d_xyz, sizeof(double) * 3 * s->n, hipMemcpyDeviceToDevice, s->stream));
         s->have_pos = true;
         s->have_nl = false;
@@ -1018,24 +1101,27 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
         JNN_CHECK(!((virial || virials) && !vir), ERR_ARG, "virials were not computed");
         std::vector<double> tot((size_t)10 * s->nrep, 0.0);
         JNN_HIP(hipMemcpyAsync(tot.data(), s->d_tot, sizeof(double) * tot.size(), hipMemcpyDeviceToHost, s->stream));
-        if (energies && s->n)
-            JNN_HIP(hipMemcpyAsync(energies, s->d_E, sizeof(double) * s->n, hipMemcpyDeviceToHost, s->stream));
-        if (forces && s->n)
-            JNN_HIP(hipMemcpyAsync(forces, s->d_F, sizeof(double) * 3 * s->n, hipMemcpyDeviceToHost, s->stream));
-        if (virials && s->n)
-            JNN_HIP(hipMemcpyAsync(virials, s->d_V, sizeof(double) * 9 * s->n, hipMemcpyDeviceToHost, s->stream));
+        // per-atom rows of the caller's atoms (copy 0 of a supercell: its first nuser rows)
+        const size_t nu = (size_t)s->nuser;
+        if (energies && nu)
+            JNN_HIP(hipMemcpyAsync(energies, s->d_E, sizeof(double) * nu, hipMemcpyDeviceToHost, s->stream));
+        if (forces && nu)
+            JNN_HIP(hipMemcpyAsync(forces, s->d_F, sizeof(double) * 3 * nu, hipMemcpyDeviceToHost, s->stream));
+        if (virials && nu)
+            JNN_HIP(hipMemcpyAsync(virials, s->d_V, sizeof(double) * 9 * nu, hipMemcpyDeviceToHost, s->stream));
         JNN_HIP(hipStreamSynchronize(s->stream));
-        // whole-system totals: replica totals added in replica order
+        // whole-system totals: replica totals added in replica order (a supercell's per box)
+        const double inv = 1.0 / s->ncopy;
         if (energy) {
             double e = 0.0;
             for (int q = 0; q < s->nrep; ++q) e += tot[(size_t)10 * q];
-            *energy = e;
+            *energy = e * inv;
         }
         if (virial)
             for (int c = 0; c < 9; ++c) {
                 double v = 0.0;
                 for (int q = 0; q < s->nrep; ++q) v += tot[(size_t)10 * q + 1 + c];
-                virial[c] = v;
+                virial[c] = v * inv;
             }
     });
 }
@@ -1043,7 +1129,7 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
 int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *const *grad_nn,
                                     double *const *grad_fp) {
     return guard([&] {
-        JNN_CHECK(s && grad_nn && (w || s->n == 0), ERR_ARG, "null argument");
+        JNN_CHECK(s && grad_nn && (w || s->nuser == 0), ERR_ARG, "null argument");
         const int nt = s->model->host.ntypes();
         for (int t = 0; t < nt; ++t) JNN_CHECK(grad_nn[t] && (!grad_fp || grad_fp[t]), ERR_ARG, "null gradient block");
         // the normalised descriptors of the current positions, as the networks read them
@@ -1056,7 +1142,7 @@ int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *
         double *d_w = dalloc<double>((size_t)std::max(s->n, 1));
         std::vector<double *> bufs;
         try {
-            if (s->n) JNN_HIP(hipMemcpyAsync(d_w, w, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
+            s->upload_atom_rows(d_w, w, 1);
             for (int t = 0; t < nt; ++t) {
                 const MlpArgs a = mlp_args(s->model, t, s->seg_args(t, false), s->d_E, false, 0);
                 const int np = mlp_param_count(a);
@@ -1138,20 +1224,15 @@ int jse_nnap_system_backward_energy_force(jse_nnap_system s, const double *w, co
         };
         try {
             double *d_w = alloc(n);
-            if (s->n) {
-                if (w) JNN_HIP(hipMemcpyAsync(d_w, w, sizeof(double) * s->n, hipMemcpyHostToDevice, s->stream));
-                else JNN_HIP(hipMemsetAsync(d_w, 0, sizeof(double) * s->n, s->stream));
-            }
+            s->upload_atom_rows(d_w, w, 1);
             TangentArgs base{};
             if (bgrad_forces) {
                 double *d_bf = alloc(3 * n);
-                if (s->n)
-                    JNN_HIP(hipMemcpyAsync(d_bf, bgrad_forces, sizeof(double) * 3 * s->n, hipMemcpyHostToDevice,
-                                           s->stream));
+                s->upload_atom_rows(d_bf, bgrad_forces, 3);
                 base.bf = d_bf;
             }
-            if (bgrad_virial)
-                for (int q = 0; q < 9; ++q) base.bv[q] = bgrad_virial[q];
+            if (bgrad_virial)  // a supercell's virial is ncopy times the box's
+                for (int q = 0; q < 9; ++q) base.bv[q] = bgrad_virial[q] / s->ncopy;
             double *d_xdot = alloc(n * m->smax), *d_adot = alloc(n * std::max(m->pmmax, 1));
             double *d_xbar = grad_fp ? alloc(n * m->smax) : nullptr, *d_xdbar = grad_fp ? alloc(n * m->smax) : nullptr;
             // the tangent args of basis k of species t (x slices and A blocks as part_segment narrows them)
@@ -1239,7 +1320,7 @@ int jse_nnap_system_backward_energy_force(jse_nnap_system s, const double *w, co
 
 int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
     return guard([&] {
-        JNN_CHECK(s && (fp || s->n == 0), ERR_ARG, "null argument");
+        JNN_CHECK(s && (fp || s->nuser == 0), ERR_ARG, "null argument");
         JNN_CHECK(smax >= s->model->smax, ERR_ARG, "smax smaller than the largest descriptor");
         s->last_what = 0;
         s->pending = true;
@@ -1260,6 +1341,7 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
         for (int t = 0; t < s->model->host.ntypes(); ++t) {
             const int S = s->model->host.species[t].S();
             for (int r = s->seg[t]; r < s->seg[t + 1]; ++r) {
+                if (rows[r] >= s->nuser) continue;  // a copy of a supercell
                 double *o = fp + (size_t)rows[r] * smax;
                 const size_t b = (size_t)r * s->model->smax;
                 for (int e = 0; e < smax; ++e) o[e] = e < S ? (f64 ? x64[b + e] : (double)x[b + e]) : 0.0;
@@ -1271,6 +1353,8 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
 int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *key, float *d) {
     int rc = guard([&] {
         JNN_CHECK(s && count, ERR_ARG, "null argument");
+        JNN_CHECK(s->ncopy == 1, ERR_UNSUPPORTED,
+                  "neighbour rows of a box shorter than the cutoff name supercell copies: not exported");
         if (!s->have_nl) {
             s->last_what = 0;
             s->pending = true;
@@ -1309,6 +1393,8 @@ int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *k
 int jse_nnap_system_get_pair_gradients(jse_nnap_system s, int cap, float *g) {
     return guard([&] {
         JNN_CHECK(s && g, ERR_ARG, "null argument");
+        JNN_CHECK(s->ncopy == 1, ERR_UNSUPPORTED,
+                  "pair rows of a box shorter than the cutoff name supercell copies: not exported");
         JNN_CHECK(s->last_what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL), ERR_ARG, "forces were not computed");
         s->finish();
         const int n = s->n, c = s->nl.cap;

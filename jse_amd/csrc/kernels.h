@@ -109,6 +109,10 @@ __device__ __forceinline__ int find_reverse(const NeighborList &nl, int i, int k
 
 // ---- neighbour build ----
 void launch_fill_int(hipStream_t s, int n, int *p, int v);  // kernel fill (graph-capture safe)
+// Copies of a box's atoms for its supercell of cp[0] x cp[1] x cp[2] boxes:
+// out row c * nuser + i = xyz[i] + (cx, cy, cz) * box, c = cx + cp[0] (cy + cp[1] cz).
+void launch_expand_copies(hipStream_t s, int nuser, const int *cp, const double *box, const double *xyz,
+                          double *out);
 // `atoms` lists the atoms to process (nullptr = atoms 0..n-1); `rep_list`
 // the replicas whose cells are scanned (nullptr = replicas 0..nrep-1);
 // `need` (nullptr = all) skips the atoms and cells of replicas whose flag is 0.

@@ -432,6 +432,26 @@ void launch_fill_int(hipStream_t s, int n, int *p, int v) {
     hipLaunchKernelGGL(k_fill_int, dim3((n + 255) / 256), dim3(256), 0, s, n, p, v);
 }
 
+// Copy c of atom i at xyz[i] + (cx L_x, cy L_y, cz L_z), c = cx + cp0 (cy + cp1 cz),
+// written to row c * nuser + i of out.
+__global__ void k_expand_copies(int nuser, int cp0, int cp1, int ncopy, double lx, double ly, double lz,
+                                const double *__restrict__ xyz, double *__restrict__ out) {
+    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long long)nuser * ncopy) return;
+    const int c = (int)(t / nuser), i = (int)(t % nuser);
+    const int cx = c % cp0, cy = (c / cp0) % cp1, cz = c / (cp0 * cp1);
+    out[3 * t] = xyz[3 * i] + lx * cx;
+    out[3 * t + 1] = xyz[3 * i + 1] + ly * cy;
+    out[3 * t + 2] = xyz[3 * i + 2] + lz * cz;
+}
+void launch_expand_copies(hipStream_t s, int nuser, const int *cp, const double *box, const double *xyz,
+                          double *out) {
+    const long long tot = (long long)nuser * cp[0] * cp[1] * cp[2];
+    if (tot <= 0) return;
+    hipLaunchKernelGGL(k_expand_copies, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, s, nuser, cp[0], cp[1],
+                       cp[0] * cp[1] * cp[2], box[0], box[1], box[2], xyz, out);
+}
+
 void launch_wrap_bin(hipStream_t s, int n, const int *atoms, const double *xyz_in, const int *rep_of,
                      const ReplicaGrid *reps, const int *need, double *xw, int *cell, int *cell_cnt, int *rank) {
     if (n <= 0) return;

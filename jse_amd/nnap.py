@@ -525,10 +525,17 @@ class NNAP:
         t = self._types_of(atoms)
         key = (atoms.natoms, t.tobytes())
         s = self._systems.get(key)
+        if s is not None:
+            try:
+                s.set_box(atoms.box)
+            except NNAPError as e:
+                if e.code != 2:  # 2: a box edge crossed the cutoff (another supercell): a new system
+                    raise
+                s.close()
+                s = None
         if s is None:
             s = System(self, atoms.natoms, t, atoms.box, self.precision)
             self._systems = {key: s}   # keep the last configuration resident
-        s.set_box(atoms.box)
         s.set_positions(atoms.xyz)
         return s
 

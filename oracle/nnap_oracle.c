@@ -14,6 +14,7 @@
  *   cell binning    NeighborListGetter.java:191-217
  *   27-cell walk    NeighborListGetter.java:265-286 (cell), 330-372, 402-~480
  *   pair filter     NeighborListGetter.java:1018-1054 (dx=(xj+shift)-xi, d2<rc2)
+ *   expanded walk   NeighborListGetter.java:704-776, 985-995 (rmax > min box)
  *   Rn, fc          src/nnap/resources/assets/nnap/src/nnap_util.hpp:226-271
  *   Y_lm            basis_SphericalUtil.hpp:12-101 (+ coefficient tables in
  *                   basis_SphericalUtil0.hpp:501-603, recomputed here from
@@ -107,7 +108,7 @@ int orc_cell_grid(const double *box, double rmax, int *size) {
     double minbox = box[0];
     if (box[1] < minbox) minbox = box[1];
     if (box[2] < minbox) minbox = box[2];
-    if (rmax > minbox) return -1; /* ExpandLinkedCell: not restated */
+    if (rmax > minbox) return -1; /* no grid: the expanded walk (neighbors_expanded) */
     int div = (int)floor(minbox / rmax);
     double cell_len = minbox / (double)div;
     for (int d = 0; d < 3; ++d) {
@@ -148,16 +149,66 @@ static inline int try_pair(const double *x, const double *box, int i, int j, int
     return 0;
 }
 
+/* Image shift past one box length: the key's shift code field holds this
+ * marker (the 27 codes of shift_code cover |s| <= 1 only). */
+#define EXPAND_CODE 31
+
+/*
+ * ExpandLinkedCell (NeighborListGetter.java:704-776; multipliers :985-995),
+ * taken when rmax exceeds the smallest box length: multi = ceil(rmax/minbox),
+ * mul_d = ceil(multi*minbox / L_d); for atom i, every atom j in index order and
+ * every image (a, b, c) in [-mul_x, mul_x] x [-mul_y, mul_y] x [-mul_z, mul_z]
+ * (x outermost), the image position x_j + (L_x a, L_y b, L_z c), self skipped
+ * only at zero shift, kept when (dx*dx + dy*dy) + dz*dz < rmax^2.
+ * key = j*32 + EXPAND_CODE.
+ */
+static int neighbors_expanded(int n, const double *x, const double *box, double rmax, int cap, int *count,
+                              int *key, double *d) {
+    double minbox = box[0];
+    if (box[1] < minbox) minbox = box[1];
+    if (box[2] < minbox) minbox = box[2];
+    const int multi = (int)ceil(rmax / minbox);
+    const double span = minbox * multi;
+    int mul[3];
+    for (int q = 0; q < 3; ++q) mul[q] = (int)ceil(span / box[q]);
+    const double rc2 = rmax * rmax;
+    for (int i = 0; i < n; ++i) {
+        int k = 0;
+        for (int j = 0; j < n; ++j)
+            for (int a = -mul[0]; a <= mul[0]; ++a)
+                for (int b = -mul[1]; b <= mul[1]; ++b)
+                    for (int c = -mul[2]; c <= mul[2]; ++c) {
+                        if (j == i && a == 0 && b == 0 && c == 0) continue;
+                        const double px = x[3 * j] + box[0] * a;
+                        const double py = x[3 * j + 1] + box[1] * b;
+                        const double pz = x[3 * j + 2] + box[2] * c;
+                        const double dx = px - x[3 * i], dy = py - x[3 * i + 1], dz = pz - x[3 * i + 2];
+                        const double xx = dx * dx, yy = dy * dy, zz = dz * dz;
+                        if ((xx + yy) + zz < rc2) {
+                            if (k < cap) {
+                                key[(size_t)i * cap + k] = j * 32 + EXPAND_CODE;
+                                double *o = d + ((size_t)i * cap + k) * 3;
+                                o[0] = dx; o[1] = dy; o[2] = dz;
+                            }
+                            ++k;
+                        }
+                    }
+        count[i] = k;
+    }
+    return 0;
+}
+
 /*
  * Reference-order neighbour walk (cell list).  x must already be wrapped.
  * Outputs per atom i, in the reference's visiting order, up to cap entries:
  *   key[i*cap+k] = j*32 + shift_code, d[(i*cap+k)*3 + c] = dx,dy,dz.
  * count[i] is the true number of neighbours (may exceed cap: caller resizes).
+ * rmax above the smallest box length takes the expanded walk above.
  */
 int orc_neighbors_celllist(int n, const double *x, const double *box, double rmax, int cap, int *count,
                            int *key, double *d) {
     int size[3];
-    if (orc_cell_grid(box, rmax, size) != 0) return -1;
+    if (orc_cell_grid(box, rmax, size) != 0) return neighbors_expanded(n, x, box, rmax, cap, count, key, d);
     const double rc2 = rmax * rmax;
     const int ncell = size[0] * size[1] * size[2];
     int single = (size[0] == 1 && size[1] == 1 && size[2] == 1);

@@ -232,6 +232,64 @@ def test_small_box_multiple_images(O):
     pot.close()
 
 
+# ------------------------------------------------ boxes shorter than the cutoff
+def _short_box(case):
+    """Boxes with an edge below the 6 A cutoff (the reference's ExpandLinkedCell
+    case): every edge short, a slab, and 2 atoms in a 2.9 A cube (three copies
+    per axis) with positions several boxes outside [0, L)."""
+    box, n = {"all_short": ((4.6, 3.9, 5.2), 5), "slab": ((13.0, 12.2, 3.4), 30),
+              "tiny_outside": ((2.9, 2.9, 2.9), 2)}[case]
+    box = np.array(box)
+    types = np.random.default_rng(43).integers(1, 4, size=n).astype(np.int32)
+    xyz = W.rsa(n, box, 2.0, 43)
+    if case == "tiny_outside":
+        xyz = xyz + np.array([[-2, 1, 3], [4, -1, 0]]) * box
+    return W.Structure(box, xyz, types)
+
+
+@pytest.mark.parametrize("case", ["all_short", "slab", "tiny_outside"])
+def test_box_shorter_than_cutoff(O, case):
+    """The system evaluates such a box as its supercell (every edge >= rc); the
+    caller's atoms get the oracle's energies, forces and virials (its expanded
+    walk, NeighborListGetter.java:704-776) to the FP32 bar, totals per box.
+    Descriptors and device positions take the same mapping; neighbour rows
+    (which name supercell copies) are refused with status 2."""
+    import torch
+    st = _short_box(case)
+    md = W.ternary_model()
+    mdl = M.load_model(md)
+    orc = O.Oracle(mdl)
+    with nnap.NNAP(md) as pot:
+        res = pot.calEnergyForceVirials(_atoms(st))
+        fp = pot.calFp(_atoms(st))
+        s = pot.system(_atoms(st))
+        with pytest.raises(nnap.NNAPError) as e:
+            s.neighbors()
+        assert e.value.code == 2
+        x = torch.tensor(st.xyz, dtype=torch.float64, device="cuda")
+        s.set_positions_device(x.data_ptr())
+        s.compute()
+        dev = s.results()
+    ref = orc.system_efv(st)
+    _assert_fp32_bar(_errors(res, ref, st.natoms))
+    for k in ("energies", "forces", "virials"):
+        assert np.array_equal(dev[k], res[k])
+    want = O.system_fp(orc, st)
+    assert np.abs(fp - want).max() <= 1e-9 * max(1.0, np.abs(want).max())
+
+
+def test_box_crossing_the_cutoff_gets_a_new_system(O):
+    """The same atoms in a box above the cutoff, below it, and above again: the
+    cached system is replaced when the box needs another supercell."""
+    st = _short_box("all_short")
+    big = W.Structure(st.box * 2.0, st.xyz * 2.0, st.types)
+    md = W.ternary_model()
+    orc = O.Oracle(M.load_model(md))
+    with nnap.NNAP(md) as pot:
+        for s_ in (big, st, big):
+            _assert_fp32_bar(_errors(pot.calEnergyForceVirials(_atoms(s_)), orc.system_efv(s_), s_.natoms))
+
+
 def test_api_variants_agree():
     md, st, g = _golden("golden_cu256")
     pot = nnap.NNAP(md)

@@ -248,3 +248,23 @@ def test_force_loss_grad_mirror_and_absent_species(O):
         lm = _lin_loss(orc, st, w, bf, bv)
         orc._W[0][idx] = old
         assert abs((lp - lm) / (2 * h) - tot[idx]) <= 2e-7 * scale
+
+
+def test_force_loss_grad_box_shorter_than_cutoff(O):
+    """Training on a structure whose box is shorter than the cutoff (a DFT-sized
+    cell: 6 atoms in 4.6 x 3.9 x 5.2 A, cutoff 5.5 A, evaluated as its
+    supercell): the energy and force / stress loss' gradients against central
+    differences of the oracle's linear loss (its expanded walk)."""
+    box = np.array([4.6, 3.9, 5.2])
+    st = W.Structure(box, W.rsa(6, box, 2.0, 47), np.array([1, 2, 3, 1, 2, 1], dtype=np.int32))
+    parts = [{"type": "spherical_chebyshev", "nmax": 3, "lmax": 3, "rcut": 5.5, "wtype": "rfuse"}]
+    big = W.ternary_liquid(n=400, seed=23)
+    md = _trainable_mixing_model(["Cu", "Zr", "Al"], parts, (12, 12), 41, big, O)
+    mdl = M.load_model(md)
+    w, bf, bv = _upstream(st, 5)
+    with nnap.NNAP(md, precision=nnap.FP32) as pot:
+        sysm = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+        gnn, gfp = sysm.backward_energy_force(w, bf, bv, mixing=True)
+        ge = sysm.backward_energy(w, mixing=True)
+    _fd_check(O, mdl, st, gnn, gfp, w, bf, bv, 2e-7, seed=6)
+    _fd_check(O, mdl, st, ge[0], ge[1], w, np.zeros_like(bf), np.zeros(9), 2e-7, seed=7)

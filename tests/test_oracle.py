@@ -83,6 +83,68 @@ def test_reference_fixture_fcc108(O):
     assert np.all(cnt == 12)  # nearest shell at 2.553 A of a = 3.61
 
 
+# ----------------------------------------- boxes shorter than rmax (expanded walk)
+def _supercell(st, rep):
+    """The explicit rep[0] x rep[1] x rep[2] supercell: copy c = cx + rep0 (cy +
+    rep1 cz) of atom i is row c n + i, at x_i + (cx, cy, cz) * box."""
+    cop = [st.xyz + np.array([cx, cy, cz]) * st.box
+           for cz in range(rep[2]) for cy in range(rep[1]) for cx in range(rep[0])]
+    return W.Structure(st.box * np.array(rep), np.concatenate(cop), np.tile(st.types, len(cop)))
+
+
+def _short_box(box, n, seed):
+    box = np.array(box, dtype=float)
+    types = np.random.default_rng(seed).integers(1, 4, size=n).astype(np.int32)
+    return W.Structure(box, W.rsa(n, box, 2.0, seed), types)
+
+
+def test_expanded_walk_fcc_unit_cell_shells(O):
+    """rmax above every box length (ExpandLinkedCell, NeighborListGetter.java:
+    704-776): the 4-atom fcc cell has the crystal's shells, 12 + 6 + 24 + 12 + 24
+    neighbours within 6 A at a/sqrt2, a, a sqrt(3/2), a sqrt2, a sqrt(5/2)."""
+    st = W.cu_fcc(ncell=1, sigma=0.0)
+    x = O.wrap(st.xyz, st.box)
+    cnt, key, d = O.neighbors(x, st.box, 6.0, "celllist")
+    assert np.all(cnt == 78)
+    a = 3.61
+    for i in range(4):
+        r = np.sort(np.sqrt((d[i, :78] ** 2).sum(1)))
+        want = np.repeat(a * np.sqrt([0.5, 1.0, 1.5, 2.0, 2.5]), [12, 6, 24, 12, 24])
+        assert np.allclose(r, want, rtol=0, atol=1e-12)
+        assert np.all((key[i, :78] & 31) == 31)  # shifts past one box carry no image code
+        assert set(key[i, :78] >> 5) == {0, 1, 2, 3}  # its own images among them
+
+
+@pytest.mark.parametrize("box,n,rep", [((4.6, 3.9, 5.2), 5, (2, 2, 2)),     # every edge short
+                                       ((13.0, 12.2, 3.4), 30, (1, 1, 2)),  # a slab
+                                       ((2.9, 2.9, 2.9), 2, (3, 3, 3))])    # three copies per axis
+def test_expanded_walk_equals_supercell(O, box, n, rep):
+    """A box shorter than rmax (the expanded walk) against its explicit
+    supercell whose edges reach rmax (the 27-cell walk, pinned against brute
+    force above): the same neighbour distances per atom, and the same per-atom
+    energies, forces and per-atom virials on copy 0, the supercell's total
+    virial being the box's times the number of copies."""
+    st = _short_box(box, n, 41)
+    sup = _supercell(st, rep)
+    nc = int(np.prod(rep))
+    x1 = O.wrap(st.xyz, st.box)
+    x2 = O.wrap(sup.xyz, sup.box)
+    c1, k1, d1 = O.neighbors(x1, st.box, 6.0, "celllist")
+    c2, k2, d2 = O.neighbors(x2, sup.box, 6.0, "celllist")
+    assert np.array_equal(c1, c2[:n])
+    for i in range(n):
+        r1 = np.sort((d1[i, :c1[i]] ** 2).sum(1))
+        r2 = np.sort((d2[i, :c2[i]] ** 2).sum(1))
+        assert np.allclose(r1, r2, rtol=1e-13, atol=1e-12)
+        assert np.array_equal(np.sort(k1[i, :c1[i]] >> 5), np.sort((k2[i, :c2[i]] >> 5) % n))
+    orc = O.Oracle(M.load_model(W.ternary_model()))
+    a, b = orc.system_efv(st), orc.system_efv(sup)
+    assert np.abs(a["energies"] - b["energies"][:n]).max() <= 1e-12
+    assert np.abs(a["forces"] - b["forces"][:n]).max() <= 1e-10
+    assert np.abs(a["virials"] - b["virials"][:n]).max() <= 1e-10
+    assert np.abs(a["virial"] - b["virial"] / nc).max() <= 1e-10
+
+
 # ------------------------------------------------------------- harmonics / radial
 def _real_sph_scipy(lmax, v):
     return W._real_sph_np(lmax, np.array([v[0]]), np.array([v[1]]), np.array([v[2]]))[0]
