@@ -44,9 +44,12 @@ const char *jse_nnap_last_error(void);
 
 /* ---- model ---------------------------------------------------------------
  * Replaces `new NNAP(String path)` / `new NNAP(Map)` (NNAP.java:402-414,
- * model parsing :129-307).  `json_or_path` is a path to a .json model file or
- * the JSON text itself (a text starting with '{').  YAML is converted to JSON
- * by the caller. */
+ * model parsing :129-307).  `json_or_path` is the JSON text itself (a text
+ * starting with '{') or a path to a model file: YAML when the name ends in
+ * .yaml / .yml (as NNAP(String) chooses, NNAP.java:407-409; block and flow
+ * collections, plain and quoted scalars, comments; anchors, tags, block
+ * scalars and multi-line scalars are refused with status 1 naming the line),
+ * JSON otherwise. */
 int jse_nnap_model_load(const char *json_or_path, jse_nnap_model *out);
 
 /* Host-only check of a model file: parses it as jse_nnap_model_load does and

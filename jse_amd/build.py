@@ -19,7 +19,8 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = os.environ.get("JSE_AMD_ARCH", "gfx950")
 
 HIP_SOURCES = ["nl.hip", "desc.hip", "desc_gen.hip", "desc_train.hip", "mlp.hip", "mlp_mfma.hip", "force.hip", "serve.hip",
-               "lammps.hip", "gpumd.hip", "capi.cpp", "server.cpp", "lammps.cpp", "gpumd.cpp", "model_host.cpp", "sh_coupling.cpp"]
+               "lammps.hip", "gpumd.hip", "capi.cpp", "server.cpp", "lammps.cpp", "gpumd.cpp", "model_host.cpp", "sh_coupling.cpp",
+               "yaml_json.cpp"]
 HIP_FLAGS = [
     f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-Wall", "-Wno-unused-function",
     # packed FP32 VALU ops are kept out of every kernel

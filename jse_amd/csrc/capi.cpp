@@ -796,16 +796,24 @@ struct jse_nnap_system_s {
 };
 
 namespace {
-// The model text: the argument itself when it starts with '{', else the file it names.
+// The model's JSON text: the argument itself when it starts with '{', else the
+// file it names, read as YAML when its name ends in .yaml / .yml (as NNAP(String)
+// chooses, NNAP.java:407-409) and as JSON otherwise.
 std::string model_text(const char *json_or_path) {
     std::string text(json_or_path);
     size_t k = text.find_first_not_of(" \t\r\n");
     if (k == std::string::npos || text[k] != '{') {
-        std::ifstream f(text);
-        JNN_CHECK(f.good(), ERR_ARG, "cannot open model file: " + text);
+        const std::string path = text;
+        std::ifstream f(path);
+        JNN_CHECK(f.good(), ERR_ARG, "cannot open model file: " + path);
         std::stringstream ss;
         ss << f.rdbuf();
         text = ss.str();
+        auto ends = [&](const char *suf) {
+            const size_t n = std::strlen(suf);
+            return path.size() >= n && path.compare(path.size() - n, n, suf) == 0;
+        };
+        if (ends(".yaml") || ends(".yml")) text = yaml_to_json(text);
     }
     return text;
 }

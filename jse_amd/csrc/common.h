@@ -105,6 +105,9 @@ struct ModelHost {
 
 // model_host.cpp
 ModelHost parse_model_json(const std::string &text);
+// The JSON text of a YAML model file's data (the YAML subset a model file
+// needs; status 1 naming the line for anything outside it).
+std::string yaml_to_json(const std::string &yaml);
 void finalize_species(ModelHost &m);   // derive mix kind, validate sizes
 
 }  // namespace jnn

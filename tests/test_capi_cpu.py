@@ -426,3 +426,96 @@ def test_cpp_parser_l34():
         assert _parse(json.dumps(m2))[0] != 0 and msg in nnap.lib().jse_nnap_last_error().decode()
         with pytest.raises(ValueError, match=msg.split(" MUST")[0]):
             M.load_model(m2)
+
+
+# ------------------------------------------------------------- YAML model files
+def _jackson_style(obj, ind=0):
+    """A YAML text in the layout of a Jackson / Groovy YamlBuilder writer: a
+    '---' line, double-quoted strings, sequences at their key's column and
+    nested sequences opened on the item's line ('- - - 0.1')."""
+    pad = " " * ind
+    if isinstance(obj, dict):
+        out = []
+        for k, v in obj.items():
+            if isinstance(v, (dict, list)) and v:
+                out.append(f"{pad}{k}:")
+                out.append(_jackson_style(v, ind if isinstance(v, list) else ind + 2))
+            else:
+                out.append(f"{pad}{k}: {_jackson_scalar(v)}")
+        return "\n".join(out)
+    out = []
+    for v in obj:
+        if isinstance(v, (dict, list)) and v:
+            inner = _jackson_style(v, ind + 2)
+            out.append(pad + "- " + inner[ind + 2:])
+        else:
+            out.append(f"{pad}- {_jackson_scalar(v)}")
+    return "\n".join(out)
+
+
+def _jackson_scalar(v):
+    if isinstance(v, str):
+        return json.dumps(v)
+    if isinstance(v, (list, dict)):
+        return "[]" if isinstance(v, list) else "{}"
+    if isinstance(v, bool):
+        return "true" if v else "false"
+    return repr(v)
+
+
+def _yaml_texts(md):
+    import yaml
+    return {
+        "block": yaml.safe_dump(md, default_flow_style=False, sort_keys=False),
+        "flow": yaml.safe_dump(md, default_flow_style=True, width=60),   # flow collections over many lines
+        "json": json.dumps(md, indent=2),                                 # JSON is YAML
+        "jackson": "%YAML 1.2\n---\n# written in the Jackson layout\n" + _jackson_style(md) + "\n...\n",
+    }
+
+
+@pytest.mark.parametrize("style", ["block", "flow", "json", "jackson"])
+def test_yaml_model_file_gives_the_json_blocks(tmp_path, style):
+    """A model file ending in .yaml / .yml is read as YAML (NNAP(String),
+    NNAP.java:407-409): the C++ loader builds bit-identical parameter blocks
+    from four YAML layouts of the same model (merge bases with L3 invariants,
+    a mirror species, FP64 weights written with 17 digits) as from its JSON."""
+    parts = [{"type": "spherical_chebyshev", "nmax": 3, "lmax": 3, "l3max": 2, "rcut": 5.0, "wtype": "full"},
+             {"type": "chebyshev", "nmax": 4, "rcut": 4.5}]
+    md = M.random_merge_model(["Cu", "Zr", "Al"], parts, (8, 8), 9)
+    rng = np.random.default_rng(4)
+    for m in md["models"]:
+        S = m["nn"]["input_dim"]
+        m["norm_mu"] = rng.normal(size=S).tolist()
+        m["norm_sigma"] = (1.0 + rng.random(S)).tolist()
+        m["ref_eng"] = float(rng.normal())
+    md["models"][1] = {"symbol": "Zr", "basis": {"type": "mirror", "mirror": 1}}
+    mdl = _both_equal(md)
+    text = _yaml_texts(md)[style]
+    for ext in (".yaml", ".yml"):
+        path = tmp_path / ("model" + ext)
+        path.write_text(text)
+        for t, sp in enumerate(mdl.species):
+            fp, nn, norm, hid, mir = _blocks(str(path), t + 1, sp)
+            assert np.array_equal(fp, sp.W.ravel()) and np.array_equal(nn, sp.nn_param)
+            assert np.array_equal(norm, sp.norm_param) and hid == list(sp.hidden) and mir == sp.mirror
+        assert _parse(str(path)) == _parse(json.dumps(md))
+
+
+@pytest.mark.parametrize("text,what", [
+    ("models: &a\n- symbol: Cu\n", "anchors"),
+    ("models:\n\t- symbol: Cu\n", "tabs"),
+    ("units: metal\nunits: real\n", "duplicate key"),
+    ("units: |\n  metal\n", "block scalars"),
+    ("units: met\n  al\n", "multi-line plain"),
+    ("models:\n  - a: 1\n - b: 2\n", "indentation"),
+    ("models: [1, 2\n", "unterminated flow"),
+    ("w: .inf\n", "non-finite"),
+    ("a: 1\n---\nb: 2\n", "more than one document"),
+])
+def test_yaml_outside_the_subset_names_the_line(tmp_path, text, what):
+    path = tmp_path / "bad.yaml"
+    path.write_text(text)
+    rc, *_ = _parse(str(path))
+    assert rc == 1
+    msg = nnap.lib().jse_nnap_last_error().decode()
+    assert "YAML model line" in msg and what in msg, msg
