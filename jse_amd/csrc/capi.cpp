@@ -405,6 +405,23 @@ struct jse_nnap_system_s {
     hipGraphExec_t graph_exec[8] = {};
     bool use_graph = true;
     LaunchTune tune;                   // launch shapes (autotuner)
+    // Device buffers of the training entry points, kept between calls: the k-th
+    // request of a call gets the k-th buffer, reallocated only when it must grow
+    // (each call ends with a stream synchronisation, so reuse is ordered).
+    std::vector<std::pair<double *, size_t>> tws;
+    size_t tws_next = 0;
+    double *tws_get(size_t cnt) {
+        cnt = std::max<size_t>(cnt, 1);
+        if (tws_next == tws.size()) tws.push_back({nullptr, 0});
+        auto &b = tws[tws_next++];
+        if (b.second < cnt) {
+            dfree(b.first);
+            b.second = 0;
+            b.first = dalloc<double>(cnt);
+            b.second = cnt;
+        }
+        return b.first;
+    }
 
     ~jse_nnap_system_s() {
         if (stream) (void)hipStreamSynchronize(stream);
@@ -414,6 +431,7 @@ struct jse_nnap_system_s {
         dfree(d_rows); dfree(d_rep_of); dfree(d_slot_of); dfree(d_reps); dfree(d_cell_cnt); dfree(d_cell_start); dfree(d_cell_rep); dfree(d_overflow);
         dfree(d_x); dfree(d_A); dfree(d_G); dfree(d_E); dfree(d_F); dfree(d_V); dfree(d_partial); dfree(d_tot); dfree(d_arrive);
         dfree(d_chunks); dfree(d_rep_chunk);
+        for (auto &b : tws) dfree(b.first);
         for (auto &e : ev)
             if (e) (void)hipEventDestroy(e);
         if (h_overflow) (void)hipHostFree(h_overflow);
@@ -1147,8 +1165,8 @@ int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *
         s->finish();
         s->have_nl = true;
         s->desc_fwd(false);
-        double *d_w = dalloc<double>((size_t)std::max(s->n, 1));
-        std::vector<double *> bufs;
+        s->tws_next = 0;
+        double *d_w = s->tws_get((size_t)s->n);
         try {
             s->upload_atom_rows(d_w, w, 1);
             for (int t = 0; t < nt; ++t) {
@@ -1158,10 +1176,8 @@ int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *
                     std::fill(grad_nn[t], grad_nn[t] + np, 0.0);
                     continue;
                 }
-                double *partial = dalloc<double>((size_t)mlp_pgrad_groups(a) * np);
-                bufs.push_back(partial);
-                double *out = dalloc<double>((size_t)np);
-                bufs.push_back(out);
+                double *partial = s->tws_get((size_t)mlp_pgrad_groups(a) * np);
+                double *out = s->tws_get((size_t)np);
                 JNN_CHECK(launch_mlp_param_grad(s->stream, a, d_w, partial, out), ERR_UNSUPPORTED,
                           "network too wide for the parameter-gradient kernel (16 rows of its layers in 160 KB)");
                 JNN_HIP(hipMemcpyAsync(grad_nn[t], out, sizeof(double) * np, hipMemcpyDeviceToHost, s->stream));
@@ -1182,10 +1198,8 @@ int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *
                         if (!b.mix_trainable() || seg.nrows <= 0) {
                             std::fill(grad_fp[t] + goff, grad_fp[t] + goff + np, 0.0);
                         } else {
-                            double *partial = dalloc<double>((size_t)mix_pgrad_groups(seg.nrows) * np);
-                            bufs.push_back(partial);
-                            double *out = dalloc<double>((size_t)np);
-                            bufs.push_back(out);
+                            double *partial = s->tws_get((size_t)mix_pgrad_groups(seg.nrows) * np);
+                            double *out = s->tws_get((size_t)np);
                             JNN_CHECK(launch_mix_param_grad(s->stream, desc_lmax(b), b.K(), b.P,
                                                             part_segment(seg, s->model, t, k), s->nl, d_w, partial,
                                                             out),
@@ -1200,12 +1214,8 @@ int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *
             JNN_HIP(hipStreamSynchronize(s->stream));
         } catch (...) {
             (void)hipStreamSynchronize(s->stream);
-            for (double *p : bufs) dfree(p);
-            dfree(d_w);
             throw;
         }
-        for (double *p : bufs) dfree(p);
-        dfree(d_w);
     });
 }
 
@@ -1224,12 +1234,8 @@ int jse_nnap_system_backward_energy_force(jse_nnap_system s, const double *w, co
         s->have_nl = true;
         s->desc_fwd(false);
         const size_t n = (size_t)std::max(s->n, 1);
-        std::vector<void *> bufs;
-        auto alloc = [&](size_t cnt) {
-            double *p = dalloc<double>(cnt);
-            bufs.push_back(p);
-            return p;
-        };
+        s->tws_next = 0;
+        auto alloc = [&](size_t cnt) { return s->tws_get(cnt); };
         try {
             double *d_w = alloc(n);
             s->upload_atom_rows(d_w, w, 1);
@@ -1265,7 +1271,7 @@ int jse_nnap_system_backward_energy_force(jse_nnap_system s, const double *w, co
                 const SpeciesHost &sp = m->host.species[t];
                 for (int k = 0; k < sp.nbases(); ++k) {
                     const SpeciesHost &b = sp.basis(k);
-                    JNN_CHECK(launch_desc_jvp(s->stream, desc_lmax(b), b.K(), b.P, part_segment(seg, m, t, k), s->nl,
+                    JNN_CHECK(launch_desc_jvp(s->stream, desc_lmax(b), b.K(), b.P, b.mix, part_segment(seg, m, t, k), s->nl,
                                               tangent(t, k, seg.nrows)),
                               ERR_UNSUPPORTED, "basis outside the descriptor tangent kernel's limits");
                 }
@@ -1313,15 +1319,7 @@ int jse_nnap_system_backward_energy_force(jse_nnap_system s, const double *w, co
             JNN_HIP(hipStreamSynchronize(s->stream));
         } catch (...) {
             (void)hipStreamSynchronize(s->stream);
-            for (void *p : bufs) {
-                double *q = (double *)p;
-                dfree(q);
-            }
             throw;
-        }
-        for (void *p : bufs) {
-            double *q = (double *)p;
-            dfree(q);
         }
     });
 }

@@ -26,6 +26,7 @@
 // convention of the reference (dxy == 0 -> cos phi = 1, sin phi = 0, the phi
 // term dropped; basis_SphericalUtil.hpp:165-181) is kept.
 #include "common.h"
+#include "desc_gen.h"
 #include "device_math.h"
 #include "kernels.h"
 
@@ -47,30 +48,6 @@ struct DescShape {
     static constexpr int YBUF = (64 * M > RED) ? 64 * M : RED;
 };
 
-// c[p] = fc * sum_n W[t][p][n] Rn[n] for the three mixing specialisations
-template <int K, int P, int MIX, typename T>
-__device__ __forceinline__ void mix_rnp(const T *rn, int tj, const T *__restrict__ W, T *rnp) {
-    if (MIX == MIX_NONE) {
-#pragma unroll
-        for (int p = 0; p < P; ++p) rnp[p] = rn[p];
-    } else if (MIX == MIX_DEFAULT) {
-        const T wt = (tj & 1) ? (T)tj : -(T)tj;
-#pragma unroll
-        for (int q = 0; q < K; ++q) {
-            rnp[q] = rn[q];
-            rnp[K + q] = wt * rn[q];
-        }
-    } else {
-        const T *w = W + (size_t)(tj - 1) * P * K;
-#pragma unroll
-        for (int p = 0; p < P; ++p) {
-            T s = (T)0;
-#pragma unroll
-            for (int q = 0; q < K; ++q) s += w[p * K + q] * rn[q];
-            rnp[p] = s;
-        }
-    }
-}
 // The plain mode's forward on the FP64 matrix cores (v_mfma_f64_16x16x4_f64).
 // Each lane evaluates one neighbour from its FP64 displacement (d + dlo): the
 // pair geometry, Y, Rn, fc and c in FP64.  c goes to LDS as C^T[p][j]; Y stays
@@ -945,16 +922,6 @@ static void run_bwd(hipStream_t s, const SegmentSet &set, const NeighborList &nl
             hipLaunchKernelGGL((k_desc_bwd<LMAX, K, P, MIX, 1, false>), dim3(grid), dim3(64), lds, s, set, nl);
     }
 }
-
-#define JNN_SHAPES(X)              \
-    X(6, 6, 6, MIX_NONE)           \
-    X(6, 6, 12, MIX_DEFAULT)       \
-    X(6, 6, 12, MIX_DENSE)         \
-    X(6, 9, 9, MIX_NONE)           \
-    X(6, 9, 18, MIX_DEFAULT)       \
-    X(6, 9, 18, MIX_DENSE)         \
-    X(4, 6, 6, MIX_NONE)           \
-    X(4, 6, 12, MIX_DEFAULT)
 
 bool launch_desc_fwd_set(hipStream_t s, int lmax, int K, int P, int mix, const SegmentSet &set,
                          const NeighborList &nl) {

@@ -110,7 +110,7 @@ __global__ __launch_bounds__(64) void k_gen_wgrad(SegmentArgs a, NeighborList nl
 // D_u R = (u.d/r) dR/dr, and the neighbour's species; beyond the cutoff every
 // value is zero.  FP64 from the FP64 displacement.
 template <int LMAX, bool RADIAL>
-__device__ void pair_tangent(const SegmentArgs &a, const NeighborList &nl, const TangentArgs &ta, size_t k,
+__device__ __forceinline__ void pair_tangent(const SegmentArgs &a, const NeighborList &nl, const TangentArgs &ta, size_t k,
                              int iatom, int K, double *y, double *yd, double *R, double *Rd, int &tj) {
     constexpr int L1 = LMAX + 1, M = L1 * L1;
     const float4 dd = nl.d[k];
@@ -281,6 +281,125 @@ __global__ __launch_bounds__(64) void k_gen_jvp(SegmentArgs a, NeighborList nl, 
                 ta.xdot[(size_t)row * a.xs + e] = f / a.sig64[e];
             }
         }
+    }
+}
+
+// The tangent on the FP64 matrix cores for the register-tiled shapes (no L3 /
+// L4 columns): one wavefront per atom, a lane per neighbour of each chunk of
+// 64 forms the pair's Y, D_u Y (registers), R, D_u R and then c = W R, D_u c =
+// W D_u R (LDS as C^T[p][j] rows, a zero row P); each 16-row m-tile of Y^T and
+// D_u Y^T is staged in LDS, and its k-steps add
+//   D_u A^T += Y^T(16 m x 4 neighbours) D_u C(4 x 16 p) + D_u Y^T C
+// with v_mfma_f64_16x16x4_f64 (lane maps as in the plain forward: A[i][k] from
+// lane i + 16 k, B[k][j] from lane j + 16 k, D[i][j] in lane j + 16 (i % 4),
+// element i / 4).  Lanes past the last neighbour take c = D_u c = 0.  The
+// epilogue puts D_u A (FP64) in LDS, stores it, and forms D_u x = 2 4pi/(2l+1)
+// sum_m A D_u A / sigma.  Rows of 65 doubles: distinct bank pairs per
+// quarter-wave.
+template <int LMAX, int K, int P, int MIX>
+__global__ __launch_bounds__(64) void k_desc_jvp_acc(SegmentArgs a, NeighborList nl, TangentArgs ta) {
+    constexpr int L1 = LMAX + 1, M = L1 * L1, S = P * L1;
+    constexpr int NMT = (M + 15) / 16, NPT = (P + 15) / 16;
+    constexpr int NS = 65;
+    constexpr int FC = (P + 1) * NS, FY = 16 * NS;
+    constexpr int F = 2 * FC + 2 * FY;
+    static_assert(P * M <= F, "D_u A must fit the operand buffer");
+    __shared__ double sbuf[F];
+    double *sC = sbuf, *sCd = sbuf + FC, *sY = sbuf + 2 * FC, *sYd = sY + FY, *sA = sbuf;
+    using d4 = __attribute__((__vector_size__(4 * sizeof(double)))) double;
+    constexpr d4 kZero = {0.0, 0.0, 0.0, 0.0};
+    const int lane = threadIdx.x;
+    const int li = lane & 15, lk = lane >> 4;
+    int boff[NPT];
+#pragma unroll
+    for (int pt = 0; pt < NPT; ++pt) boff[pt] = (pt * 16 + li < P ? pt * 16 + li : P) * NS;
+    for (int row = blockIdx.x; row < a.nrows; row += gridDim.x) {
+        const int atom = a.rows_atom[row];
+        const int i = a.nl_of ? a.nl_of[atom] : atom;
+        const int n = min(nl.cnt[i], nl.cap);
+        const size_t base = (size_t)i * nl.cap;
+        d4 acc[NMT][NPT];
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
+            for (int pt = 0; pt < NPT; ++pt) acc[mt][pt] = kZero;
+        for (int c0 = 0; c0 < n; c0 += 64) {
+            const int j = c0 + lane;
+            const int jr = j < n ? j : n - 1;
+            const bool live = j < n;
+            const int nks = (min(64, n - c0) + 3) >> 2;
+            __syncthreads();  // the previous chunk's (or atom's) LDS reads are done
+            double y[M], yd[M], R[K], Rd[K];
+            int tj;
+            pair_tangent<LMAX, false>(a, nl, ta, base + jr, atom, K, y, yd, R, Rd, tj);
+            if constexpr (MIX == MIX_DENSE) {  // c, D_u c row by row into LDS (few registers live)
+                const double *w = a.W64 + (size_t)(tj - 1) * P * K;
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    double cp = 0.0, cdp = 0.0;
+#pragma unroll
+                    for (int q = 0; q < K; ++q) {
+                        cp += w[p * K + q] * R[q];
+                        cdp += w[p * K + q] * Rd[q];
+                    }
+                    sC[p * NS + lane] = live ? cp : 0.0;
+                    sCd[p * NS + lane] = live ? cdp : 0.0;
+                }
+            } else {
+                double c[P], cd[P];
+                mix_rnp<K, P, MIX>(R, tj, a.W64, c);
+                mix_rnp<K, P, MIX>(Rd, tj, a.W64, cd);
+#pragma unroll
+                for (int p = 0; p < P; ++p) {
+                    sC[p * NS + lane] = live ? c[p] : 0.0;
+                    sCd[p * NS + lane] = live ? cd[p] : 0.0;
+                }
+            }
+            sC[P * NS + lane] = 0.0;  // the zero row (padding p of the last p-tile)
+            sCd[P * NS + lane] = 0.0;
+#pragma unroll
+            for (int mt = 0; mt < NMT; ++mt) {
+                if (mt > 0) __syncthreads();  // the previous m-tile's operand reads are done
+#pragma unroll
+                for (int r = 0; r < 16; ++r) {
+                    const int m = mt * 16 + r;
+                    sY[r * NS + lane] = m < M ? y[m] : 0.0;
+                    sYd[r * NS + lane] = m < M ? yd[m] : 0.0;
+                }
+                __syncthreads();
+                for (int ks = 0; ks < nks; ++ks) {
+                    const int col = 4 * ks + lk;
+                    const double av = sY[li * NS + col], avd = sYd[li * NS + col];
+#pragma unroll
+                    for (int pt = 0; pt < NPT; ++pt) {
+                        acc[mt][pt] = __builtin_amdgcn_mfma_f64_16x16x4f64(av, sCd[boff[pt] + col], acc[mt][pt], 0, 0, 0);
+                        acc[mt][pt] = __builtin_amdgcn_mfma_f64_16x16x4f64(avd, sC[boff[pt] + col], acc[mt][pt], 0, 0, 0);
+                    }
+                }
+            }
+        }
+        __syncthreads();  // operand reads are done: the buffer takes D_u A
+#pragma unroll
+        for (int mt = 0; mt < NMT; ++mt)
+#pragma unroll
+            for (int pt = 0; pt < NPT; ++pt)
+#pragma unroll
+                for (int r = 0; r < 4; ++r) {
+                    const int m = mt * 16 + 4 * r + lk, p = pt * 16 + li;
+                    if (m < M && p < P) sA[p * M + m] = acc[mt][pt][r];
+                }
+        __syncthreads();
+        double *adot = ta.adot + (size_t)row * (P * M);
+        for (int e = lane; e < P * M; e += 64) adot[e] = sA[e];
+        const float *A = a.A + (size_t)row * (P * M);
+        for (int e = lane; e < S; e += 64) {
+            const int p = e / L1, l = e - p * L1;
+            double f = 0.0;
+            for (int m = l * l; m < (l + 1) * (l + 1); ++m) f += (double)A[p * M + m] * sA[p * M + m];
+            f *= 2.0 * 4.0 * 3.14159265358979323846 / (double)(2 * l + 1);
+            ta.xdot[(size_t)row * a.xs + e] = f / a.sig64[e];
+        }
+        __syncthreads();  // the epilogue's reads of sA are done before the next atom's operands
     }
 }
 
@@ -520,9 +639,19 @@ static bool run_wgrad2(hipStream_t s, const SegmentArgs &a, const NeighborList &
         case 12: ok = CALL_SPH(12); break;                             \
         default: return false;                                         \
     }
-bool launch_desc_jvp(hipStream_t s, int lmax, int K, int P, const SegmentArgs &a, const NeighborList &nl,
+bool launch_desc_jvp(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl,
                      const TangentArgs &ta) {
     if (a.nrows <= 0) return true;
+    if (a.n34 == 0 && lmax != kRadialLmax) {  // a register-tiled shape: the matrix-core kernel
+        const int grid = std::max(1, std::min(a.nrows, 4 * device_cu_count()));
+#define X(L_, K_, P_, M_)                                                                                   \
+        if (lmax == L_ && K == K_ && P == P_ && mix == M_) {                                                 \
+            hipLaunchKernelGGL((k_desc_jvp_acc<L_, K_, P_, M_>), dim3(grid), dim3(64), 0, s, a, nl, ta);      \
+            return true;                                                                                     \
+        }
+        JNN_SHAPES(X)
+#undef X
+    }
     if (K < 1 || K > kGenKMax || P < 1 || P > kGenPMax) return false;
     bool ok;
 #define SPH(L) run_jvp<L, false>(s, a, nl, K, P, ta)

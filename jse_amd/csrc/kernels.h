@@ -300,7 +300,7 @@ struct TangentArgs {
 };
 // D_u A and D_u x of a segment (lmax = kRadialLmax: the radial basis, no A).
 // False outside the untiled kernels' limits.
-bool launch_desc_jvp(hipStream_t s, int lmax, int K, int P, const SegmentArgs &a, const NeighborList &nl,
+bool launch_desc_jvp(hipStream_t s, int lmax, int K, int P, int mix, const SegmentArgs &a, const NeighborList &nl,
                      const TangentArgs &ta);
 // out[ntypes][P][K] = dL/dW of a trainable mixing from dL/dx and dL/d(D_u x)
 // (tangent args), summed over the segment's atoms; partial as

@@ -268,3 +268,30 @@ def test_force_loss_grad_box_shorter_than_cutoff(O):
         ge = sysm.backward_energy(w, mixing=True)
     _fd_check(O, mdl, st, gnn, gfp, w, bf, bv, 2e-7, seed=6)
     _fd_check(O, mdl, st, ge[0], ge[1], w, np.zeros_like(bf), np.zeros(9), 2e-7, seed=7)
+
+
+@pytest.mark.parametrize("case", ["default_lmax6", "rfuse_lmax6"])
+def test_force_loss_grad_tiled_shapes(O, case):
+    """The register-tiled shapes take the matrix-core tangent kernel: the
+    ternary default mixing (lmax 6, nmax 8, P 18) for the networks, and a
+    two-species rfuse mixing of the same shape (P 18, trainable) for every
+    block, against central differences of the oracle's linear loss."""
+    if case == "default_lmax6":
+        st = W.ternary_liquid(n=300, seed=29)
+        md = M.random_model(["Cu", "Zr", "Al"], nmax=8, lmax=6, rcut=5.5, hidden=(16, 16), seed=31)
+        md = W._normalized(md, st, 150, 31)
+        mixing = False
+    else:
+        st0 = W.ternary_liquid(n=300, seed=31)
+        st = W.Structure(st0.box, st0.xyz, np.where(st0.types == 3, 1, st0.types).astype(np.int32))
+        parts = [{"type": "spherical_chebyshev", "nmax": 8, "lmax": 6, "rcut": 5.5, "wtype": "rfuse"}]
+        md = _trainable_mixing_model(["Cu", "Zr"], parts, (12, 12), 33, st, O)
+        mixing = True
+    mdl = M.load_model(md)
+    assert all(b.lmax == 6 and b.K == 9 and b.P == 18 for s in mdl.species for b in s.bases())
+    w, bf, bv = _upstream(st, 8)
+    with nnap.NNAP(md, precision=nnap.FP32) as pot:
+        sysm = pot.system(nnap.AtomData(st.box, st.xyz, st.types))
+        got = sysm.backward_energy_force(w, bf, bv, mixing=mixing)
+    gnn, gfp = got if mixing else (got, None)
+    _fd_check(O, mdl, st, gnn, gfp, w, bf, bv, 2e-7, seed=9)
