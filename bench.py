@@ -646,9 +646,10 @@ def main():
     ap.add_argument("--shapes", action="store_true",
                     help="only time model shapes of the untiled descriptor kernels on the 32,000-atom liquid "
                          "(one JSON line per shape, the tiled default shape first)")
-    ap.add_argument("--leg", choices=["batched"], default="",
-                    help="only run the batched graph step of the 512-replica workload (with --precision and "
-                         "--knobs; for profiling)")
+    ap.add_argument("--leg", choices=["batched", "training"], default="",
+                    help="batched: only the batched graph step of the 512-replica workload (with --precision and "
+                         "--knobs); training: only the training entry points on the 32,000-atom liquid "
+                         "(plain mode, --steps calls each); for profiling")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -666,6 +667,12 @@ def main():
     if args.leg == "batched":
         knobs = json.loads(args.knobs) if args.knobs else TUNED_C4.get(args.precision)
         batched_leg(args.precision, knobs, args.steps)
+        return
+    if args.leg == "training":
+        st = W.ternary_liquid(n=args.natoms, seed=7)
+        with nnap.NNAP(W.ternary_model(), precision=nnap.FP32) as pot:
+            out = training_derivatives(pot, st, iters=args.steps)
+        print(json.dumps({"leg": "training_derivatives", "atoms": st.natoms, **out}), flush=True)
         return
 
     md = W.ternary_model()
