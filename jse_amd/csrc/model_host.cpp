@@ -32,6 +32,7 @@ struct JVal {
 struct JParser {
     const std::string &s;
     size_t p = 0;
+    int depth = 0;  // open values: a model file nests a few levels; past 64 the text is refused
     explicit JParser(const std::string &t) : s(t) {}
     [[noreturn]] void fail(const char *what) {
         throw Error(ERR_ARG, std::string("model JSON: ") + what + " at offset " + std::to_string(p));
@@ -39,7 +40,16 @@ struct JParser {
     void ws() {
         while (p < s.size() && (s[p] == ' ' || s[p] == '\n' || s[p] == '\r' || s[p] == '\t')) ++p;
     }
+    struct Nest {
+        int &d;
+        explicit Nest(JParser &j) : d(j.depth) {
+            if (d >= 64) j.fail("nesting deeper than 64 levels");
+            ++d;
+        }
+        ~Nest() { --d; }
+    };
     JVal value() {
+        const Nest nest(*this);
         ws();
         if (p >= s.size()) fail("unexpected end");
         char c = s[p];

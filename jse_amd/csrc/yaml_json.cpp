@@ -257,6 +257,15 @@ class Converter {
     std::vector<Line> L_;
     size_t p_ = 0;
     std::string out_;
+    int depth_ = 0;  // open collections: past 64 the text is refused
+    struct Nest {
+        int &d;
+        Nest(int &depth, int no) : d(depth) {
+            if (d >= 64) fail(no, "nesting deeper than 64 levels");
+            ++d;
+        }
+        ~Nest() { --d; }
+    };
 
     static bool seq_item(const std::string &t) { return t == "-" || (t.size() > 1 && t[0] == '-' && t[1] == ' '); }
     // The colon of a block mapping entry "key: value" / "key:", npos if none.
@@ -276,6 +285,7 @@ class Converter {
     }
 
     void node(int parent) {
+        const Nest nest(depth_, p_ < L_.size() ? L_[p_].no : 0);
         if (p_ >= L_.size() || L_[p_].indent <= parent) {
             out_ += "null";
             return;
@@ -418,6 +428,7 @@ class Converter {
         return key ? json_string(t) : plain_json(t, no);
     }
     void flow(const std::string &s, size_t &i, int no) {
+        const Nest nest(depth_, no);
         skip_ws(s, i);
         if (i >= s.size()) fail(no, "missing value in a flow collection");
         if (s[i] == '[') {

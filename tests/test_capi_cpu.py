@@ -519,3 +519,16 @@ def test_yaml_outside_the_subset_names_the_line(tmp_path, text, what):
     assert rc == 1
     msg = nnap.lib().jse_nnap_last_error().decode()
     assert "YAML model line" in msg and what in msg, msg
+
+
+@pytest.mark.parametrize("ext,text", [(".yaml", "a: " + "[" * 5000 + "]" * 5000 + "\n"),
+                                      (".yaml", "".join(" " * k + "- \n" for k in range(200))),
+                                      (".json", "[" * 100000 + "]" * 100000)])
+def test_deep_nesting_is_refused(tmp_path, ext, text):
+    """A file nested past 64 levels is refused with status 1 (no recursion
+    without bound in either parser)."""
+    path = tmp_path / ("deep" + ext)
+    path.write_text(text)
+    rc, *_ = _parse(str(path))
+    assert rc == 1
+    assert "nesting deeper than 64" in nnap.lib().jse_nnap_last_error().decode()
