@@ -169,7 +169,8 @@ int jse_nnap_system_backward_energy(jse_nnap_system s, const double *w, double *
  * them (TrainerNNAP.java:1102-1176).  The loss is given by its gradients:
  * w[natoms] by the per-atom energies (aGradEng; NULL: zeros), bgrad_forces
  * [natoms][3] by the forces (NULL: zeros) and bgrad_virial[9] by the total
- * virial, order xx yy zz xy xz yz yx zx zy (NULL: zeros); the trainer's
+ * virial, order xx yy zz xy xz yz yx zx zy (NULL: zeros; for a batch of
+ * jse_nnap_batch_create, [nrep][9], replica r's by its own virial); the trainer's
  * per-pair upstream is then bgrad_forces[j] - bgrad_forces[i] + sum_a
  * bgrad_virial[ab] d_a for pair (i, j) with displacement d.  grad_nn and
  * grad_fp receive, in the layouts of jse_nnap_system_backward_energy, the

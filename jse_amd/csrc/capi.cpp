@@ -1245,8 +1245,15 @@ int jse_nnap_system_backward_energy_force(jse_nnap_system s, const double *w, co
                 s->upload_atom_rows(d_bf, bgrad_forces, 3);
                 base.bf = d_bf;
             }
-            if (bgrad_virial)  // a supercell's virial is ncopy times the box's
+            if (bgrad_virial && s->nrep > 1) {  // a batch: one dL/dvirial per replica
+                double *d_bv = alloc((size_t)9 * s->nrep);
+                JNN_HIP(hipMemcpyAsync(d_bv, bgrad_virial, sizeof(double) * 9 * s->nrep, hipMemcpyHostToDevice,
+                                       s->stream));
+                base.bv_rep = d_bv;
+                base.rep_of = s->d_rep_of;
+            } else if (bgrad_virial) {  // a supercell's virial is ncopy times the box's
                 for (int q = 0; q < 9; ++q) base.bv[q] = bgrad_virial[q] / s->ncopy;
+            }
             double *d_xdot = alloc(n * m->smax), *d_adot = alloc(n * std::max(m->pmmax, 1));
             double *d_xbar = grad_fp ? alloc(n * m->smax) : nullptr, *d_xdbar = grad_fp ? alloc(n * m->smax) : nullptr;
             // the tangent args of basis k of species t (x slices and A blocks as part_segment narrows them)

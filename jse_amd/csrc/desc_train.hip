@@ -104,7 +104,7 @@ __global__ __launch_bounds__(64) void k_gen_wgrad(SegmentArgs a, NeighborList nl
 
 // ---- the force / virial loss ----
 // One pair's values for the tangent and its adjoint: the pair's upstream
-// direction u = bf_j - bf_i + sum_a bv_ab d_a, Y and D_u Y (the angular
+// direction u = bf_j - bf_i + sum_a bv_ab d_a (bv of atom i's replica in a batch), Y and D_u Y (the angular
 // gradient along u: (u.theta_hat)/r dY/dtheta + (u.phi_hat)/(r sin theta)
 // dY/dphi, the phi term dropped at the pole as in the backward), R = fc Rn and
 // D_u R = (u.d/r) dR/dr, and the neighbour's species; beyond the cutoff every
@@ -132,7 +132,8 @@ __device__ __forceinline__ void pair_tangent(const SegmentArgs &a, const Neighbo
     }
     const double d[3] = {g.dx, g.dy, g.dz};
     const int ca[9] = {0, 1, 2, 0, 0, 1, 1, 2, 2}, cb[9] = {0, 1, 2, 1, 2, 2, 0, 0, 1};
-    for (int q = 0; q < 9; ++q) u[cb[q]] += ta.bv[q] * d[ca[q]];
+    const double *bv = ta.bv_rep ? ta.bv_rep + 9 * (size_t)ta.rep_of[iatom] : ta.bv;
+    for (int q = 0; q < 9; ++q) u[cb[q]] += bv[q] * d[ca[q]];
     const double ud = (u[0] * g.dx + u[1] * g.dy + u[2] * g.dz) * g.ir;
     const double xr = 1.0 - 2.0 * g.r * inv_rc;
     double rn[kGenKMax], drn[kGenKMax];

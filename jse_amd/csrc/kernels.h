@@ -293,6 +293,8 @@ bool launch_mix_param_grad(hipStream_t s, int lmax, int K, int P, const SegmentA
 struct TangentArgs {
     const double *bf;       // [N][3] dL/dF by atom (nullptr: none)
     double bv[9];           // dL/dV, components xx yy zz xy xz yz yx zx zy
+    const double *bv_rep;   // [nrep][9] dL/dV of each replica of a batch (nullptr: bv for every pair)
+    const int *rep_of;      // [N] replica of each atom (with bv_rep)
     double *xdot;           // [nrows][xs] D_u x (the x slice of this part)
     double *adot;           // [nrows][P M] D_u A of this part
     const double *xbar;     // [nrows][xs] dL/dx

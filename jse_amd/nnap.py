@@ -282,7 +282,9 @@ class System:
         """Parameter gradients of sum_i w[i] E_i + sum_k bgrad_forces[k] . F_k +
         sum bgrad_virial . virial (jse_nnap_backwardEnergyForce over every atom, as
         TrainerNNAP drives it): the network blocks, and with ``mixing`` also the
-        mixing-weight blocks, as backward_energy returns them."""
+        mixing-weight blocks, as backward_energy returns them.  For a batch,
+        bgrad_virial is [nrep][9] (each replica's by its own virial), so one
+        call gives the summed gradient of many training structures."""
         n = self.natoms
         def arr(v, shape):
             if v is None:
@@ -293,7 +295,7 @@ class System:
             return v
         w = arr(w, (n,))
         bf = arr(bgrad_forces, (n, 3))
-        bv = arr(bgrad_virial, (9,))
+        bv = arr(bgrad_virial, (9,) if self.nrep == 1 else (self.nrep, 9))  # a batch: one per replica
         sp = self.nnap.model.species
         nt = len(sp)
         outs = [np.zeros(s.nn_param.size) for s in sp]

@@ -295,3 +295,36 @@ def test_force_loss_grad_tiled_shapes(O, case):
         got = sysm.backward_energy_force(w, bf, bv, mixing=mixing)
     gnn, gfp = got if mixing else (got, None)
     _fd_check(O, mdl, st, gnn, gfp, w, bf, bv, 2e-7, seed=9)
+
+
+def test_batch_of_structures_sums_the_per_structure_gradients(O):
+    """Many training structures in one call: a batch (jse_nnap_batch_create) of
+    three liquids with per-atom w, dL/dF and one dL/dvirial per replica gives
+    the sum of the three single-system gradients, for the energy and the force
+    / stress loss, networks and rfuse mixing (to summation order)."""
+    reps = [W.ternary_liquid(n=n, seed=s) for n, s in ((300, 41), (420, 42), (360, 43))]
+    parts = [{"type": "spherical_chebyshev", "nmax": 3, "lmax": 4, "rcut": 5.5, "wtype": "rfuse"}]
+    md = _trainable_mixing_model(["Cu", "Zr", "Al"], parts, (12, 12), 45, reps[0], O)
+    rng = np.random.default_rng(11)
+    ups = [(rng.uniform(0.5, 1.5, r.natoms), rng.normal(size=(r.natoms, 3)), rng.normal(size=9)) for r in reps]
+    with nnap.NNAP(md, precision=nnap.FP32) as pot:
+        singles_f, singles_e = [], []
+        for r, (w, bf, bv) in zip(reps, ups):
+            s1 = nnap.System(pot, r.natoms, r.types, r.box, nnap.FP32)
+            s1.set_positions(r.xyz)
+            singles_f.append(s1.backward_energy_force(w, bf, bv, mixing=True))
+            singles_e.append(s1.backward_energy(w, mixing=True))
+            s1.close()
+        b = pot.batch([nnap.AtomData(r.box, r.xyz, r.types) for r in reps])
+        W_ = np.concatenate([u[0] for u in ups])
+        BF = np.concatenate([u[1] for u in ups])
+        BV = np.stack([u[2] for u in ups])
+        bat_f = b.backward_energy_force(W_, BF, BV, mixing=True)
+        bat_e = b.backward_energy(W_, mixing=True)
+        b.close()
+    for bat, singles in ((bat_f, singles_f), (bat_e, singles_e)):
+        for blk in (0, 1):
+            for t in range(3):
+                want = sum(s[blk][t] for s in singles)
+                err = np.abs(bat[blk][t] - want).max() / max(1.0, np.abs(want).max())
+                assert err <= 1e-11, (blk, t, err)
