@@ -96,9 +96,10 @@ void jse_nnap_model_destroy(jse_nnap_model m);
  * [0,Lx) x [0,Ly) x [0,Lz).  Types are fixed for the life of the system.
  * A box edge shorter than the largest cutoff (the reference's ExpandLinkedCell,
  * NeighborListGetter.java:704-776) is evaluated as the box's supercell whose
- * edges reach the cutoff; per-atom outputs are the caller's atoms', totals
- * per box.  Such a system refuses neighbour and pair-gradient exports (status
- * 2), and set_box returns status 2 when the new box needs another supercell. */
+ * edges reach the cutoff (so in a batch, replica by replica); per-atom outputs
+ * are the caller's atoms', totals per box.  Such a system refuses neighbour and
+ * pair-gradient exports (status 2), and set_box returns status 2 when a new box
+ * needs another supercell. */
 int jse_nnap_system_create(jse_nnap_model m, int natoms, const int *types, const double *box, int precision,
                            jse_nnap_system *out);
 void jse_nnap_system_destroy(jse_nnap_system s);

@@ -361,17 +361,25 @@ struct jse_nnap_system_s {
     jse_nnap_model model = nullptr;
     int n = 0;       // atoms over all replicas
     int nrep = 0;    // replicas (independent periodic boxes)
-    // A box with an edge shorter than the cutoff (the reference's
-    // ExpandLinkedCell, NeighborListGetter.java:704-776, 985-995; single
-    // systems): the system holds the supercell of cp[0] x cp[1] x cp[2] copies of
-    // the box, every edge of which reaches the cutoff, so that the cell list
-    // finds each pair within the cutoff once per copy.  Copy c = cx + cp[0]
-    // (cy + cp[1] cz) of the caller's atom i is atom c * nuser + i at
-    // x_i + (cx, cy, cz) * box; copy 0 is the caller's atom and carries its
-    // results, and totals are the supercell's divided by ncopy.  nuser == n
-    // without copies.
-    int nuser = 0, ncopy = 1, cp[3] = {1, 1, 1};
-    double ubox[3] = {0.0, 0.0, 0.0};
+    // Boxes with an edge shorter than the cutoff (the reference's
+    // ExpandLinkedCell, NeighborListGetter.java:704-776, 985-995): such a
+    // replica is held as its supercell of cp[0] x cp[1] x cp[2] copies of the
+    // box, every edge of which reaches the cutoff, so that the cell list finds
+    // each pair within the cutoff once per copy.  Copy c = cx + cp[0] (cy +
+    // cp[1] cz) of the caller's atom i of a replica is held atom ioff + c * unat
+    // + i, at x_i + (cx, cy, cz) * box; copy 0 is the caller's atom and carries
+    // its results, and the replica's totals are its supercell's divided by
+    // ncopy.  Without copies every map is the identity.
+    struct RepCopies {
+        int cp[3] = {1, 1, 1};
+        int ncopy = 1;
+        double box[3] = {0.0, 0.0, 0.0};  // the caller's box
+        int unat = 0;                     // the caller's atoms
+        int uoff = 0, ioff = 0;           // first caller row, first held row
+    };
+    std::vector<RepCopies> rcopy;  // per replica
+    bool copies = false;           // some replica is held as a supercell
+    int nuser = 0;                 // the caller's atoms (n without copies)
     int precision = JSE_NNAP_FP32;
     std::vector<int> types;
     std::vector<int> rep_natoms;
@@ -511,7 +519,6 @@ struct jse_nnap_system_s {
         }
         JNN_CHECK(tot < (1 << 26), ERR_ARG, "natoms out of range [0, 2^26)");
         n = (int)tot;
-        nuser = n;
         rep_natoms.assign(natoms_per, natoms_per + nr);
         const int nt = m->host.ntypes();
         types.assign(ty, ty + n);
@@ -623,13 +630,44 @@ struct jse_nnap_system_s {
     }
 
     // The caller's per-atom rows (width values each; nullptr: zeros) into a
-    // device array of n rows: the caller's atoms fill the first nuser rows and
-    // a supercell's copies get zeros (their terms are those of the caller's atoms).
+    // device array of n rows: each caller atom on its copy 0, zeros on the
+    // other copies of a supercell (their terms are those of the caller's atoms).
     void upload_atom_rows(double *dst, const double *src, int width) {
-        const size_t nu = (size_t)nuser * width, all = (size_t)n * width;
-        if (src && nu) JNN_HIP(hipMemcpyAsync(dst, src, sizeof(double) * nu, hipMemcpyHostToDevice, stream));
-        const size_t z0 = src ? nu : 0;
-        if (all > z0) JNN_HIP(hipMemsetAsync(dst + z0, 0, sizeof(double) * (all - z0), stream));
+        const size_t all = (size_t)n * width;
+        if (!copies) {
+            if (src && all) JNN_HIP(hipMemcpyAsync(dst, src, sizeof(double) * all, hipMemcpyHostToDevice, stream));
+            else if (all) JNN_HIP(hipMemsetAsync(dst, 0, sizeof(double) * all, stream));
+            return;
+        }
+        if (all) JNN_HIP(hipMemsetAsync(dst, 0, sizeof(double) * all, stream));
+        if (src)
+            for (const RepCopies &r : rcopy)
+                if (r.unat)
+                    JNN_HIP(hipMemcpyAsync(dst + (size_t)r.ioff * width, src + (size_t)r.uoff * width,
+                                           sizeof(double) * r.unat * width, hipMemcpyHostToDevice, stream));
+    }
+    // Per-atom rows of copy 0 of every replica (the caller's atoms) to a host array.
+    void download_atom_rows(double *dst, const double *src, int width) {
+        if (!copies) {
+            if (n) JNN_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n * width, hipMemcpyDeviceToHost, stream));
+            return;
+        }
+        for (const RepCopies &r : rcopy)
+            if (r.unat)
+                JNN_HIP(hipMemcpyAsync(dst + (size_t)r.uoff * width, src + (size_t)r.ioff * width,
+                                       sizeof(double) * r.unat * width, hipMemcpyDeviceToHost, stream));
+    }
+    // The caller's row of held atom a, -1 for the other copies of a supercell.
+    int user_row(int a) const {
+        if (!copies) return a;
+        size_t lo = 0, hi = rcopy.size();
+        while (hi - lo > 1) {  // the last replica whose first held row is <= a
+            const size_t mid = (lo + hi) / 2;
+            if (rcopy[mid].ioff <= a) lo = mid;
+            else hi = mid;
+        }
+        const RepCopies &r = rcopy[lo];
+        return a - r.ioff < r.unat ? r.uoff + (a - r.ioff) : -1;
     }
 
     SegmentArgs seg_args(int t, bool raw) const {
@@ -955,6 +993,7 @@ int jse_nnap_sh_invariants(int l3max, int l4max, const double *A, double *out) {
 static void box_copies(const double *box, double rc, int cp[3]) {
     for (int d = 0; d < 3; ++d) {
         JNN_CHECK(std::isfinite(box[d]) && box[d] > 0, ERR_ARG, "box lengths must be positive and finite");
+        JNN_CHECK(rc / box[d] <= 1024.0, ERR_ARG, "a box edge below 1/1024 of the cutoff");
         int c = 1;
         if (box[d] < rc) {
             c = (int)std::ceil(rc / box[d]);
@@ -964,33 +1003,57 @@ static void box_copies(const double *box, double rc, int cp[3]) {
     }
 }
 
+// A system of nrep replicas (natoms[q] atoms of replica q after those of the
+// replicas before it, box boxes[3q..3q+2]); a replica with a box edge below
+// the cutoff is held as its supercell.
+static jse_nnap_system_s *make_system(jse_nnap_model m, int nrep, const int *natoms, const int *types,
+                                      const double *boxes, int precision) {
+    const double rcut = m->host.rcut_max();
+    std::vector<jse_nnap_system_s::RepCopies> rc((size_t)nrep);
+    long long nu = 0, ni = 0;
+    bool any = false;
+    for (int q = 0; q < nrep; ++q) {
+        JNN_CHECK(natoms[q] >= 0, ERR_ARG, "negative atom count");
+        jse_nnap_system_s::RepCopies &r = rc[q];
+        box_copies(boxes + 3 * q, rcut, r.cp);
+        const long long nc = (long long)r.cp[0] * r.cp[1] * r.cp[2];
+        JNN_CHECK(ni + (long long)natoms[q] * nc < (1 << 26), ERR_ARG,
+                  "natoms (times the copies of boxes shorter than the cutoff) out of range [0, 2^26)");
+        r.ncopy = (int)nc;
+        for (int d = 0; d < 3; ++d) r.box[d] = boxes[3 * q + d];
+        r.unat = natoms[q];
+        r.uoff = (int)nu;
+        r.ioff = (int)ni;
+        nu += natoms[q];
+        ni += (long long)natoms[q] * nc;
+        any = any || nc > 1;
+    }
+    auto s = std::make_unique<jse_nnap_system_s>();
+    if (!any) {
+        s->init(m, nrep, natoms, types, boxes, precision);
+    } else {
+        std::vector<int> inat((size_t)nrep), ity((size_t)ni);
+        std::vector<double> ibox((size_t)3 * nrep);
+        for (int q = 0; q < nrep; ++q) {
+            const jse_nnap_system_s::RepCopies &r = rc[q];
+            inat[q] = r.unat * r.ncopy;
+            for (int c = 0; c < r.ncopy; ++c)
+                std::copy(types + r.uoff, types + r.uoff + r.unat, ity.begin() + r.ioff + (size_t)c * r.unat);
+            for (int d = 0; d < 3; ++d) ibox[3 * q + d] = r.box[d] * r.cp[d];
+        }
+        s->init(m, nrep, inat.data(), ity.data(), ibox.data(), precision);
+    }
+    s->rcopy = rc;
+    s->copies = any;
+    s->nuser = (int)nu;
+    return s.release();
+}
+
 int jse_nnap_system_create(jse_nnap_model m, int natoms, const int *types, const double *box, int precision,
                            jse_nnap_system *out) {
     return guard([&] {
         JNN_CHECK(m && out && box && (types || natoms == 0), ERR_ARG, "null argument");
-        JNN_CHECK(natoms >= 0, ERR_ARG, "negative atom count");
-        int cp[3];
-        box_copies(box, m->host.rcut_max(), cp);
-        const long long ncopy = (long long)cp[0] * cp[1] * cp[2];
-        JNN_CHECK((long long)natoms * ncopy < (1 << 26), ERR_ARG,
-                  "natoms times the box copies of a box shorter than the cutoff out of range [0, 2^26)");
-        auto s = std::make_unique<jse_nnap_system_s>();
-        if (ncopy == 1) {
-            s->init(m, 1, &natoms, types, box, precision);
-        } else {
-            const int nin = (int)(natoms * ncopy);
-            std::vector<int> ty((size_t)nin);
-            for (long long c = 0; c < ncopy; ++c) std::copy(types, types + natoms, ty.begin() + c * natoms);
-            const double sbox[3] = {box[0] * cp[0], box[1] * cp[1], box[2] * cp[2]};
-            s->init(m, 1, &nin, ty.data(), sbox, precision);
-        }
-        s->nuser = natoms;
-        s->ncopy = (int)ncopy;
-        for (int d = 0; d < 3; ++d) {
-            s->cp[d] = cp[d];
-            s->ubox[d] = box[d];
-        }
-        *out = s.release();
+        *out = make_system(m, 1, &natoms, types, box, precision);
     });
 }
 
@@ -998,9 +1061,7 @@ int jse_nnap_batch_create(jse_nnap_model m, int nrep, const int *natoms, const i
                           int precision, jse_nnap_system *out) {
     return guard([&] {
         JNN_CHECK(m && out && natoms && boxes && nrep >= 1, ERR_ARG, "null argument");
-        auto s = std::make_unique<jse_nnap_system_s>();
-        s->init(m, nrep, natoms, types, boxes, precision);
-        *out = s.release();
+        *out = make_system(m, nrep, natoms, types, boxes, precision);
     });
 }
 
@@ -1014,8 +1075,8 @@ int jse_nnap_system_get_replica_results(jse_nnap_system s, double *energy, doubl
         std::vector<double> tot((size_t)10 * s->nrep);
         JNN_HIP(hipMemcpyAsync(tot.data(), s->d_tot, sizeof(double) * tot.size(), hipMemcpyDeviceToHost, s->stream));
         JNN_HIP(hipStreamSynchronize(s->stream));
-        const double inv = 1.0 / s->ncopy;  // a supercell's totals per box
         for (int q = 0; q < s->nrep; ++q) {
+            const double inv = 1.0 / s->rcopy[q].ncopy;  // a supercell's totals per box
             if (energy) energy[q] = tot[(size_t)10 * q] * inv;
             if (virial)
                 for (int c = 0; c < 9; ++c) virial[(size_t)9 * q + c] = tot[(size_t)10 * q + 1 + c] * inv;
@@ -1056,18 +1117,19 @@ void jse_nnap_system_destroy(jse_nnap_system s) { delete s; }
 int jse_nnap_system_set_box(jse_nnap_system s, const double *box) {
     return guard([&] {
         JNN_CHECK(s && box, ERR_ARG, "null argument");
-        if (s->nrep == 1) {
+        std::vector<double> ibox((size_t)3 * s->nrep);
+        for (int q = 0; q < s->nrep; ++q) {
+            const auto &r = s->rcopy[q];
             int cp[3];
-            box_copies(box, s->model->host.rcut_max(), cp);
-            JNN_CHECK(cp[0] == s->cp[0] && cp[1] == s->cp[1] && cp[2] == s->cp[2], ERR_UNSUPPORTED,
-                      "the new box needs another number of copies of a box shorter than the cutoff "
-                      "(a box edge crossed the cutoff): create a new system for it");
-            const double sbox[3] = {box[0] * cp[0], box[1] * cp[1], box[2] * cp[2]};
-            s->set_boxes(sbox);
-            for (int d = 0; d < 3; ++d) s->ubox[d] = box[d];
-        } else {
-            s->set_boxes(box);
+            box_copies(box + 3 * q, s->model->host.rcut_max(), cp);
+            JNN_CHECK(cp[0] == r.cp[0] && cp[1] == r.cp[1] && cp[2] == r.cp[2], ERR_UNSUPPORTED,
+                      "replica " + std::to_string(q) + ": the new box needs another number of copies of a box "
+                      "shorter than the cutoff (a box edge crossed the cutoff): create a new system for it");
+            for (int d = 0; d < 3; ++d) ibox[3 * q + d] = box[3 * q + d] * cp[d];
         }
+        s->set_boxes(ibox.data());
+        for (int q = 0; q < s->nrep; ++q)
+            for (int d = 0; d < 3; ++d) s->rcopy[q].box[d] = box[3 * q + d];
     });
 }
 
@@ -1076,14 +1138,16 @@ int jse_nnap_system_set_positions(jse_nnap_system s, const double *xyz) {
         JNN_CHECK(s && (xyz || s->nuser == 0), ERR_ARG, "null argument");
         for (size_t q = 0; q < (size_t)3 * s->nuser; ++q)
             JNN_CHECK(std::isfinite(xyz[q]), ERR_ARG, "non-finite position at atom " + std::to_string(q / 3));
-        if (s->ncopy > 1) {
+        if (s->copies) {
             std::vector<double> x((size_t)3 * s->n);
-            for (int c = 0; c < s->ncopy; ++c) {
-                const int cx = c % s->cp[0], cy = (c / s->cp[0]) % s->cp[1], cz = c / (s->cp[0] * s->cp[1]);
-                const double sh[3] = {s->ubox[0] * cx, s->ubox[1] * cy, s->ubox[2] * cz};
-                for (size_t i = 0; i < (size_t)s->nuser; ++i)
-                    for (int d = 0; d < 3; ++d) x[3 * ((size_t)c * s->nuser + i) + d] = xyz[3 * i + d] + sh[d];
-            }
+            for (const auto &r : s->rcopy)
+                for (int c = 0; c < r.ncopy; ++c) {
+                    const int cx = c % r.cp[0], cy = (c / r.cp[0]) % r.cp[1], cz = c / (r.cp[0] * r.cp[1]);
+                    const double sh[3] = {r.box[0] * cx, r.box[1] * cy, r.box[2] * cz};
+                    for (size_t i = 0; i < (size_t)r.unat; ++i)
+                        for (int d = 0; d < 3; ++d)
+                            x[3 * ((size_t)r.ioff + (size_t)c * r.unat + i) + d] = xyz[3 * ((size_t)r.uoff + i) + d] + sh[d];
+                }
             JNN_HIP(hipMemcpyAsync(s->d_xyz, x.data(), sizeof(double) * x.size(), hipMemcpyHostToDevice, s->stream));
             JNN_HIP(hipStreamSynchronize(s->stream));
         } else {
@@ -1098,9 +1162,11 @@ int jse_nnap_system_set_positions(jse_nnap_system s, const double *xyz) {
 int jse_nnap_system_set_positions_device(jse_nnap_system s, const double *d_xyz) {
     return guard([&] {
         JNN_CHECK(s && (d_xyz || s->nuser == 0), ERR_ARG, "null argument");
-        if (s->ncopy > 1)
-            launch_expand_copies(s->stream, s->nuser, s->cp, s->ubox, d_xyz, s->d_xyz);
-        else if (s->n)
+        if (s->copies) {
+            for (const auto &r : s->rcopy)
+                launch_expand_copies(s->stream, r.unat, r.cp, r.box, d_xyz + 3 * (size_t)r.uoff,
+                                     s->d_xyz + 3 * (size_t)r.ioff);
+        } else if (s->n)
             JNN_HIP(hipMemcpyAsync(s->d_xyz, d_xyz, sizeof(double) * 3 * s->n, hipMemcpyDeviceToDevice, s->stream));
         s->have_pos = true;
         s->have_nl = false;
@@ -1127,27 +1193,22 @@ int jse_nnap_system_get_results(jse_nnap_system s, double *energy, double *energ
         JNN_CHECK(!((virial || virials) && !vir), ERR_ARG, "virials were not computed");
         std::vector<double> tot((size_t)10 * s->nrep, 0.0);
         JNN_HIP(hipMemcpyAsync(tot.data(), s->d_tot, sizeof(double) * tot.size(), hipMemcpyDeviceToHost, s->stream));
-        // per-atom rows of the caller's atoms (copy 0 of a supercell: its first nuser rows)
-        const size_t nu = (size_t)s->nuser;
-        if (energies && nu)
-            JNN_HIP(hipMemcpyAsync(energies, s->d_E, sizeof(double) * nu, hipMemcpyDeviceToHost, s->stream));
-        if (forces && nu)
-            JNN_HIP(hipMemcpyAsync(forces, s->d_F, sizeof(double) * 3 * nu, hipMemcpyDeviceToHost, s->stream));
-        if (virials && nu)
-            JNN_HIP(hipMemcpyAsync(virials, s->d_V, sizeof(double) * 9 * nu, hipMemcpyDeviceToHost, s->stream));
+        // per-atom rows of the caller's atoms (copy 0 of each supercell)
+        if (energies) s->download_atom_rows(energies, s->d_E, 1);
+        if (forces) s->download_atom_rows(forces, s->d_F, 3);
+        if (virials) s->download_atom_rows(virials, s->d_V, 9);
         JNN_HIP(hipStreamSynchronize(s->stream));
         // whole-system totals: replica totals added in replica order (a supercell's per box)
-        const double inv = 1.0 / s->ncopy;
         if (energy) {
             double e = 0.0;
-            for (int q = 0; q < s->nrep; ++q) e += tot[(size_t)10 * q];
-            *energy = e * inv;
+            for (int q = 0; q < s->nrep; ++q) e += tot[(size_t)10 * q] / s->rcopy[q].ncopy;
+            *energy = e;
         }
         if (virial)
             for (int c = 0; c < 9; ++c) {
                 double v = 0.0;
-                for (int q = 0; q < s->nrep; ++q) v += tot[(size_t)10 * q + 1 + c];
-                virial[c] = v * inv;
+                for (int q = 0; q < s->nrep; ++q) v += tot[(size_t)10 * q + 1 + c] / s->rcopy[q].ncopy;
+                virial[c] = v;
             }
     });
 }
@@ -1245,14 +1306,18 @@ int jse_nnap_system_backward_energy_force(jse_nnap_system s, const double *w, co
                 s->upload_atom_rows(d_bf, bgrad_forces, 3);
                 base.bf = d_bf;
             }
+            // a supercell's virial is ncopy times its box's
             if (bgrad_virial && s->nrep > 1) {  // a batch: one dL/dvirial per replica
-                double *d_bv = alloc((size_t)9 * s->nrep);
-                JNN_HIP(hipMemcpyAsync(d_bv, bgrad_virial, sizeof(double) * 9 * s->nrep, hipMemcpyHostToDevice,
+                std::vector<double> bv((size_t)9 * s->nrep);
+                for (int q = 0; q < s->nrep; ++q)
+                    for (int c = 0; c < 9; ++c) bv[9 * q + c] = bgrad_virial[9 * q + c] / s->rcopy[q].ncopy;
+                double *d_bv = alloc(bv.size());
+                JNN_HIP(hipMemcpyAsync(d_bv, bv.data(), sizeof(double) * bv.size(), hipMemcpyHostToDevice,
                                        s->stream));
                 base.bv_rep = d_bv;
                 base.rep_of = s->d_rep_of;
-            } else if (bgrad_virial) {  // a supercell's virial is ncopy times the box's
-                for (int q = 0; q < 9; ++q) base.bv[q] = bgrad_virial[q] / s->ncopy;
+            } else if (bgrad_virial) {
+                for (int q = 0; q < 9; ++q) base.bv[q] = bgrad_virial[q] / s->rcopy[0].ncopy;
             }
             double *d_xdot = alloc(n * m->smax), *d_adot = alloc(n * std::max(m->pmmax, 1));
             double *d_xbar = grad_fp ? alloc(n * m->smax) : nullptr, *d_xdbar = grad_fp ? alloc(n * m->smax) : nullptr;
@@ -1354,8 +1419,9 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
         for (int t = 0; t < s->model->host.ntypes(); ++t) {
             const int S = s->model->host.species[t].S();
             for (int r = s->seg[t]; r < s->seg[t + 1]; ++r) {
-                if (rows[r] >= s->nuser) continue;  // a copy of a supercell
-                double *o = fp + (size_t)rows[r] * smax;
+                const int u = s->user_row(rows[r]);
+                if (u < 0) continue;  // another copy of a supercell
+                double *o = fp + (size_t)u * smax;
                 const size_t b = (size_t)r * s->model->smax;
                 for (int e = 0; e < smax; ++e) o[e] = e < S ? (f64 ? x64[b + e] : (double)x[b + e]) : 0.0;
             }
@@ -1366,7 +1432,7 @@ int jse_nnap_system_compute_fp(jse_nnap_system s, double *fp, int smax) {
 int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *key, float *d) {
     int rc = guard([&] {
         JNN_CHECK(s && count, ERR_ARG, "null argument");
-        JNN_CHECK(s->ncopy == 1, ERR_UNSUPPORTED,
+        JNN_CHECK(!s->copies, ERR_UNSUPPORTED,
                   "neighbour rows of a box shorter than the cutoff name supercell copies: not exported");
         if (!s->have_nl) {
             s->last_what = 0;
@@ -1406,7 +1472,7 @@ int jse_nnap_system_get_neighbors(jse_nnap_system s, int cap, int *count, int *k
 int jse_nnap_system_get_pair_gradients(jse_nnap_system s, int cap, float *g) {
     return guard([&] {
         JNN_CHECK(s && g, ERR_ARG, "null argument");
-        JNN_CHECK(s->ncopy == 1, ERR_UNSUPPORTED,
+        JNN_CHECK(!s->copies, ERR_UNSUPPORTED,
                   "pair rows of a box shorter than the cutoff name supercell copies: not exported");
         JNN_CHECK(s->last_what & (JSE_NNAP_FORCE | JSE_NNAP_VIRIAL), ERR_ARG, "forces were not computed");
         s->finish();

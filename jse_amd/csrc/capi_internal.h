@@ -86,8 +86,8 @@ inline void make_grid(const double *b, double rcut, ReplicaGrid &g) {
         JNN_CHECK(std::isfinite(b[d]) && b[d] > 0, ERR_ARG, "box lengths must be positive and finite");
     const double minbox = std::min(b[0], std::min(b[1], b[2]));
     JNN_CHECK(rcut <= minbox, ERR_UNSUPPORTED,
-              "rcut larger than the smallest box length: a single system (jse_nnap_system_create) evaluates such a "
-              "box as its supercell; batches and served replicas need every box edge >= rcut");
+              "rcut larger than the smallest box length: systems and batches evaluate such a box as its supercell; "
+              "served replicas need every box edge >= rcut");
     const int div = (int)std::floor(minbox / rcut);
     const double cell_len = minbox / (double)div;
     for (int d = 0; d < 3; ++d) {

@@ -290,6 +290,37 @@ def test_box_crossing_the_cutoff_gets_a_new_system(O):
             _assert_fp32_bar(_errors(pot.calEnergyForceVirials(_atoms(s_)), orc.system_efv(s_), s_.natoms))
 
 
+def test_batch_with_boxes_shorter_than_cutoff(O):
+    """A batch whose replicas include boxes below the cutoff (each held as its
+    own supercell) next to an ordinary liquid: every replica's atoms and totals
+    against the oracle, and device positions equal to host positions."""
+    import torch
+    reps = [_short_box("all_short"), W.ternary_liquid(n=400, seed=9), _short_box("slab"),
+            _short_box("tiny_outside")]
+    md = W.ternary_model()
+    orc = O.Oracle(M.load_model(md))
+    with nnap.NNAP(md) as pot:
+        b = pot.batch([_atoms(r) for r in reps])
+        b.compute()
+        res = b.results()
+        e_rep, v_rep = b.replica_results()
+        x = torch.tensor(np.concatenate([r.xyz for r in reps]), dtype=torch.float64, device="cuda")
+        b.set_positions_device(x.data_ptr())
+        b.compute()
+        dev = b.results()
+        b.close()
+    off = 0
+    for q, r in enumerate(reps):
+        sl = slice(off, off + r.natoms)
+        off += r.natoms
+        one = {"energy": e_rep[q], "energies": res["energies"][sl], "forces": res["forces"][sl],
+               "virial": v_rep[q], "virials": res["virials"][sl]}
+        _assert_fp32_bar(_errors(one, orc.system_efv(r), r.natoms))
+    for k in ("energies", "forces", "virials"):
+        assert np.array_equal(dev[k], res[k])
+    assert abs(res["energy"] - e_rep.sum()) <= 1e-9 * max(1.0, abs(res["energy"]))
+
+
 def test_api_variants_agree():
     md, st, g = _golden("golden_cu256")
     pot = nnap.NNAP(md)

@@ -299,10 +299,13 @@ def test_force_loss_grad_tiled_shapes(O, case):
 
 def test_batch_of_structures_sums_the_per_structure_gradients(O):
     """Many training structures in one call: a batch (jse_nnap_batch_create) of
-    three liquids with per-atom w, dL/dF and one dL/dvirial per replica gives
-    the sum of the three single-system gradients, for the energy and the force
-    / stress loss, networks and rfuse mixing (to summation order)."""
+    three liquids and a 6-atom cell below the cutoff, with per-atom w, dL/dF
+    and one dL/dvirial per replica, gives the sum of the single-system
+    gradients, for the energy and the force / stress loss, networks and rfuse
+    mixing (to summation order)."""
     reps = [W.ternary_liquid(n=n, seed=s) for n, s in ((300, 41), (420, 42), (360, 43))]
+    small = np.array([4.6, 3.9, 5.2])   # a box below the cutoff: held as its supercell in the batch too
+    reps.append(W.Structure(small, W.rsa(6, small, 2.0, 44), np.array([1, 2, 3, 1, 2, 3], dtype=np.int32)))
     parts = [{"type": "spherical_chebyshev", "nmax": 3, "lmax": 4, "rcut": 5.5, "wtype": "rfuse"}]
     md = _trainable_mixing_model(["Cu", "Zr", "Al"], parts, (12, 12), 45, reps[0], O)
     rng = np.random.default_rng(11)
@@ -325,6 +328,6 @@ def test_batch_of_structures_sums_the_per_structure_gradients(O):
     for bat, singles in ((bat_f, singles_f), (bat_e, singles_e)):
         for blk in (0, 1):
             for t in range(3):
-                want = sum(s[blk][t] for s in singles)
+                want = sum(s_[blk][t] for s_ in singles)
                 err = np.abs(bat[blk][t] - want).max() / max(1.0, np.abs(want).max())
                 assert err <= 1e-11, (blk, t, err)
