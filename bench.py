@@ -620,6 +620,24 @@ def training_derivatives(pot, st, iters=5):
         dt = (time.perf_counter() - t0) / iters
         out[name] = {"ms_per_call": dt * 1e3, "atom_evals_per_s": st.natoms / dt}
     out["note"] = "parameter gradients summed over the 32,000 atoms; host upstream in, host gradient blocks out"
+    # a training mini-batch: 256 small cells (8-64 atoms, boxes from 4.9 A up, the
+    # short ones held as supercells) in one batch, one force / stress loss call
+    reps = W.replicas(256, 8, 64, 23, rho_range=(0.055, 0.065))
+    nat = sum(r.natoms for r in reps)
+    b = pot.batch([nnap.AtomData(r.box, r.xyz, r.types) for r in reps])
+    wb = rng.uniform(0.5, 1.5, size=nat)
+    bfb = rng.normal(size=(nat, 3))
+    bvb = rng.normal(size=(len(reps), 9))
+    b.backward_energy_force(wb, bfb, bvb)
+    t0 = time.perf_counter()
+    for _ in range(iters):
+        b.backward_energy_force(wb, bfb, bvb)
+    dt = (time.perf_counter() - t0) / iters
+    b.close()
+    short = sum(1 for r in reps if min(r.box) < pot.rcutMax())
+    out["force_loss_minibatch_256_cells"] = {
+        "ms_per_call": dt * 1e3, "structures_per_s": len(reps) / dt, "atom_evals_per_s": nat / dt,
+        "atoms": nat, "cells_below_cutoff": short}
     return out
 
 
