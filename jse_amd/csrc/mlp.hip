@@ -254,10 +254,11 @@ __device__ __forceinline__ void dot_gw2(const double *__restrict__ W, int ws, co
 // (reference: jse_nnap_backwardEnergy, nnap_interface.cpp:111-134, through
 // normedNnBackward with GRAD_PARAM, nnap_main.hpp:297-323, and the layer
 // backward of nn_FeedForward.hpp:131-261).  One workgroup per tile of TR atoms
-// at a time, FP64 throughout: the forward is formed again from x (act[l][TR]
+// at a time, FP64 throughout (16-row tiles: the layer products on the FP64
+// matrix cores): the forward is formed again from x (act[l][TR]
 // in LDS, with the SiLU derivatives), dL/dz of every layer is taken back from
 // dL/dy = w_i sigma_E, and the tile's
-//   dW_l[o][i] = sum_r dz_l[o][r] act_{l-1}[i][r],  db_l[o] = sum_r dz_l[o][r],
+//   dW_l[o][i] = sum_r dz_l[o][r] act_{l-1}[i][r] (FP64 matrix cores),  db_l[o] = sum_r dz_l[o][r],
 //   dw_o[h] = sum_r dy_r act_L[h][r],  db_o = sum_r dy_r
 // are added into this workgroup's row of `partial` (its first tile stores), in
 // the packed layout [hidden W per layer [out][in] | output W | hidden b | output b].
@@ -300,6 +301,9 @@ __global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__re
     // the output weight follows the hidden weights in the forward layout
     const double *__restrict__ wout = a.w_in_out64 + wo_off;
     const int tid = threadIdx.x;
+    const int wave = tid >> 6, nwave = blockDim.x >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    using d4 = __attribute__((__vector_size__(4 * sizeof(double)))) double;
+    static_assert(TR % 4 == 0, "the tile's rows are whole k-steps of the matrix instruction");
     double *__restrict__ mine = partial + (size_t)blockIdx.x * nparam;
     bool first = true;
     for (int tile = blockIdx.x; tile * TR < a.nrows; tile += gridDim.x) {
@@ -317,13 +321,40 @@ __global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__re
         for (int l = 1; l <= L; ++l) {
             const int h = a.hidden[l - 1], in = l == 1 ? S : a.hidden[l - 2];
             const double *__restrict__ W = wb + woff[l];
-            for (int idx = tid; idx < h * TR; idx += blockDim.x) {
-                const int o = idx / TR, r = idx % TR;
-                double z = dot_gw<TR>(W + (size_t)o * in, 1, pg_smem + aoff[l - 1] + r, in);
-                z += bs[boff[l] - wo_off - hL + o];  // hidden biases come first in the bias block
-                double g;
-                pg_smem[aoff[l] + idx] = silu_and_grad(z, &g);
-                pg_smem[doff[l] + idx] = g;
+            const double *__restrict__ bl = bs + (boff[l] - wo_off - hL);  // hidden biases come first
+            if constexpr (TR == 16) {
+                // z = W act on the FP64 matrix cores: a 16-row block of outputs per
+                // wave, the inputs as k-steps (A[o][k] = W from lane o + 16 k, B[k][r]
+                // = act from lane r + 16 k, D[o][r] in lane r + 16 (o % 4), element o / 4)
+                const double *ac = pg_smem + aoff[l - 1];
+                for (int o0 = 16 * wave; o0 < h; o0 += 16 * nwave) {
+                    const bool ov = o0 + li < h;
+                    const double *wr = W + (size_t)(ov ? o0 + li : 0) * in;
+                    d4 acc = {0.0, 0.0, 0.0, 0.0};
+                    for (int k0 = 0; k0 < in; k0 += 4) {
+                        const int k = k0 + lk;
+                        const double av = ov && k < in ? wr[k] : 0.0;
+                        const double bv = k < in ? ac[k * TR + li] : 0.0;
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int o = o0 + 4 * e + lk;
+                        if (o < h) {
+                            double g;
+                            pg_smem[aoff[l] + o * TR + li] = silu_and_grad(acc[e] + bl[o], &g);
+                            pg_smem[doff[l] + o * TR + li] = g;
+                        }
+                    }
+                }
+            } else {
+                for (int idx = tid; idx < h * TR; idx += blockDim.x) {
+                    const int o = idx / TR, r = idx % TR;
+                    const double z = dot_gw<TR>(W + (size_t)o * in, 1, pg_smem + aoff[l - 1] + r, in) + bl[o];
+                    double g;
+                    pg_smem[aoff[l] + idx] = silu_and_grad(z, &g);
+                    pg_smem[doff[l] + idx] = g;
+                }
             }
             __syncthreads();
         }
@@ -336,10 +367,31 @@ __global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__re
         for (int l = L; l >= 2; --l) {
             const int h = a.hidden[l - 1], in = a.hidden[l - 2];
             const double *__restrict__ W = wb + woff[l];
-            for (int idx = tid; idx < in * TR; idx += blockDim.x) {
-                const int i = idx / TR, r = idx % TR;
-                const double s = dot_gw<TR>(W + i, in, pg_smem + goff[l] + r, h);
-                pg_smem[goff[l - 1] + idx] = s * pg_smem[doff[l - 1] + idx];
+            if constexpr (TR == 16) {
+                // dz_{l-1} = (W^T dz_l) s': a 16-row block of inputs per wave, the
+                // outputs as k-steps (A[i][k] = W[k][i], B[k][r] = dz_l)
+                const double *dz = pg_smem + goff[l];
+                for (int i0 = 16 * wave; i0 < in; i0 += 16 * nwave) {
+                    const bool iv = i0 + li < in;
+                    d4 acc = {0.0, 0.0, 0.0, 0.0};
+                    for (int k0 = 0; k0 < h; k0 += 4) {
+                        const int k = k0 + lk;
+                        const double av = iv && k < h ? W[(size_t)k * in + i0 + li] : 0.0;
+                        const double bv = k < h ? dz[k * TR + li] : 0.0;
+                        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+                    }
+#pragma unroll
+                    for (int e = 0; e < 4; ++e) {
+                        const int i = i0 + 4 * e + lk;
+                        if (i < in) pg_smem[goff[l - 1] + i * TR + li] = acc[e] * pg_smem[doff[l - 1] + i * TR + li];
+                    }
+                }
+            } else {
+                for (int idx = tid; idx < in * TR; idx += blockDim.x) {
+                    const int i = idx / TR, r = idx % TR;
+                    const double s = dot_gw<TR>(W + i, in, pg_smem + goff[l] + r, h);
+                    pg_smem[goff[l - 1] + idx] = s * pg_smem[doff[l - 1] + idx];
+                }
             }
             __syncthreads();
         }
@@ -347,14 +399,27 @@ __global__ __launch_bounds__(256) void k_mlp_pgrad(MlpArgs a, const double *__re
         auto put = [&](int p, double v) { mine[p] = first ? v : mine[p] + v; };
         for (int l = 1; l <= L; ++l) {
             const int h = a.hidden[l - 1], in = l == 1 ? S : a.hidden[l - 2];
-            for (int idx = tid; idx < h * in; idx += blockDim.x) {
-                const int o = idx / in, i = idx % in;
-                double s = 0.0;
-                for (int u = 0; u < TR; ++u) {  // rows rotated per i: no two lanes of a half-wave on one bank
-                    const int r = (u + i / (32 / TR)) & (TR - 1);
-                    s += pg_smem[goff[l] + o * TR + r] * pg_smem[aoff[l - 1] + i * TR + r];
+            // dW_l = dz_l act_{l-1}^T on the FP64 matrix cores: 16 x 16 blocks of
+            // (o, i) taken by the waves round-robin, the tile's TR rows as the
+            // k-steps (A[o][r] = dz from lane o + 16 r, B[r][i] = act from lane
+            // i + 16 r, D[o][i] in lane i + 16 (o % 4), element o / 4)
+            const double *dz = pg_smem + goff[l], *ac = pg_smem + aoff[l - 1];
+            const int nto = (h + 15) / 16, nti = (in + 15) / 16;
+            for (int tt = wave; tt < nto * nti; tt += nwave) {
+                const int o0 = (tt / nti) * 16, i0 = (tt % nti) * 16;
+                d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < TR; k0 += 4) {
+                    const int r = k0 + lk;
+                    const double av = o0 + li < h ? dz[(o0 + li) * TR + r] : 0.0;
+                    const double bv = i0 + li < in ? ac[(i0 + li) * TR + r] : 0.0;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
                 }
-                put(woff[l] + idx, s);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int o = o0 + 4 * e + lk, i = i0 + li;
+                    if (o < h && i < in) put(woff[l] + o * in + i, acc[e]);
+                }
             }
             for (int o = tid; o < h; o += blockDim.x) {
                 double s = 0.0;
@@ -435,6 +500,9 @@ __global__ __launch_bounds__(256) void k_mlp_dual_pgrad(MlpArgs a, const double 
     const int bo_off = pb;
     const double *__restrict__ wout = a.w_in_out64 + wo_off;
     const int tid = threadIdx.x;
+    const int wave = tid >> 6, nwave = blockDim.x >> 6, lane = tid & 63, li = lane & 15, lk = lane >> 4;
+    using d4 = __attribute__((__vector_size__(4 * sizeof(double)))) double;
+    static_assert(TR % 4 == 0, "the tile's rows are whole k-steps of the matrix instruction");
     double *__restrict__ mine = partial + (size_t)blockIdx.x * nparam;
     bool first = true;
     for (int tile = blockIdx.x; tile * TR < a.nrows; tile += gridDim.x) {
@@ -502,15 +570,27 @@ __global__ __launch_bounds__(256) void k_mlp_dual_pgrad(MlpArgs a, const double 
         auto put = [&](int p, double v) { mine[p] = first ? v : mine[p] + v; };
         for (int l = 1; l <= L; ++l) {
             const int h = a.hidden[l - 1], in = l == 1 ? S : a.hidden[l - 2];
-            const double *ap = act(l - 1), *adp = actd(l - 1);
-            for (int idx = tid; idx < h * in; idx += blockDim.x) {
-                const int o = idx / in, i = idx % in;
-                double s = 0.0;
-                for (int u = 0; u < TR; ++u) {  // rows rotated per i: no two lanes of a half-wave on one bank
-                    const int r = (u + i / (32 / TR)) & (TR - 1);
-                    s += zb(l)[o * TR + r] * ap[i * TR + r] + zdb(l)[o * TR + r] * adp[i * TR + r];
+            const double *ap = act(l - 1), *adp = actd(l - 1), *zbl = zb(l), *zdbl = zdb(l);
+            // dW_l = zbar act^T + zdbar actd^T on the FP64 matrix cores, in 16 x 16
+            // blocks taken by the waves round-robin (lane maps as in k_mlp_pgrad)
+            const int nto = (h + 15) / 16, nti = (in + 15) / 16;
+            for (int tt = wave; tt < nto * nti; tt += nwave) {
+                const int o0 = (tt / nti) * 16, i0 = (tt % nti) * 16;
+                const bool ov = o0 + li < h, iv = i0 + li < in;
+                d4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+                for (int k0 = 0; k0 < TR; k0 += 4) {
+                    const int r = k0 + lk;
+                    const double a1 = ov ? zbl[(o0 + li) * TR + r] : 0.0, b1 = iv ? ap[(i0 + li) * TR + r] : 0.0;
+                    const double a2 = ov ? zdbl[(o0 + li) * TR + r] : 0.0, b2 = iv ? adp[(i0 + li) * TR + r] : 0.0;
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a1, b1, acc, 0, 0, 0);
+                    acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a2, b2, acc, 0, 0, 0);
                 }
-                put(woff[l] + idx, s);
+#pragma unroll
+                for (int e = 0; e < 4; ++e) {
+                    const int o = o0 + 4 * e + lk, i = i0 + li;
+                    if (o < h && i < in) put(woff[l] + o * in + i, acc[e]);
+                }
             }
             for (int o = tid; o < h; o += blockDim.x) {
                 double s = 0.0;
