@@ -597,7 +597,7 @@ def served_c5(nrep=2000, nmin=100, nmax=20000, seed=23, sat_s=8.0, rate_s=6.0, f
     return out
 
 
-def training_derivatives(pot, st, iters=5):
+def training_derivatives(pot, st, iters=5, minibatch=True):
     """The training entry points on the same configuration (plain mode), per
     call from host arrays to host gradient blocks: the energy loss'
     derivative (jse_nnap_system_backward_energy) and the force / stress
@@ -620,6 +620,8 @@ def training_derivatives(pot, st, iters=5):
         dt = (time.perf_counter() - t0) / iters
         out[name] = {"ms_per_call": dt * 1e3, "atom_evals_per_s": st.natoms / dt}
     out["note"] = "parameter gradients summed over the 32,000 atoms; host upstream in, host gradient blocks out"
+    if not minibatch:
+        return out
     # a training mini-batch: 256 small cells (8-64 atoms, boxes from 4.9 A up, the
     # short ones held as supercells) in one batch, one force / stress loss call
     reps = W.replicas(256, 8, 64, 23, rho_range=(0.055, 0.065))
@@ -667,7 +669,7 @@ def main():
     ap.add_argument("--leg", choices=["batched", "training"], default="",
                     help="batched: only the batched graph step of the 512-replica workload (with --precision and "
                          "--knobs); training: only the training entry points on the 32,000-atom liquid "
-                         "(plain mode, --steps calls each); for profiling")
+                         "(plain mode, --steps calls each, no mini-batch); for profiling")
     ap.add_argument("--copy-bytes", type=int, default=0,
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
@@ -689,7 +691,7 @@ def main():
     if args.leg == "training":
         st = W.ternary_liquid(n=args.natoms, seed=7)
         with nnap.NNAP(W.ternary_model(), precision=nnap.FP32) as pot:
-            out = training_derivatives(pot, st, iters=args.steps)
+            out = training_derivatives(pot, st, iters=args.steps, minibatch=False)
         print(json.dumps({"leg": "training_derivatives", "atoms": st.natoms, **out}), flush=True)
         return
 

@@ -639,12 +639,16 @@ struct jse_nnap_system_s {
             else if (all) JNN_HIP(hipMemsetAsync(dst, 0, sizeof(double) * all, stream));
             return;
         }
-        if (all) JNN_HIP(hipMemsetAsync(dst, 0, sizeof(double) * all, stream));
-        if (src)
-            for (const RepCopies &r : rcopy)
-                if (r.unat)
-                    JNN_HIP(hipMemcpyAsync(dst + (size_t)r.ioff * width, src + (size_t)r.uoff * width,
-                                           sizeof(double) * r.unat * width, hipMemcpyHostToDevice, stream));
+        if (!src) {
+            if (all) JNN_HIP(hipMemsetAsync(dst, 0, sizeof(double) * all, stream));
+            return;
+        }
+        std::vector<double> h(all, 0.0);  // laid out on the host: one copy for every replica
+        for (const RepCopies &r : rcopy)
+            std::copy(src + (size_t)r.uoff * width, src + ((size_t)r.uoff + r.unat) * width,
+                      h.begin() + (size_t)r.ioff * width);
+        if (all) JNN_HIP(hipMemcpyAsync(dst, h.data(), sizeof(double) * all, hipMemcpyHostToDevice, stream));
+        JNN_HIP(hipStreamSynchronize(stream));  // h is released on return
     }
     // Per-atom rows of copy 0 of every replica (the caller's atoms) to a host array.
     void download_atom_rows(double *dst, const double *src, int width) {
@@ -652,10 +656,12 @@ struct jse_nnap_system_s {
             if (n) JNN_HIP(hipMemcpyAsync(dst, src, sizeof(double) * n * width, hipMemcpyDeviceToHost, stream));
             return;
         }
+        std::vector<double> h((size_t)n * width);  // one copy, then copy 0 of each replica picked on the host
+        if (n) JNN_HIP(hipMemcpyAsync(h.data(), src, sizeof(double) * h.size(), hipMemcpyDeviceToHost, stream));
+        JNN_HIP(hipStreamSynchronize(stream));
         for (const RepCopies &r : rcopy)
-            if (r.unat)
-                JNN_HIP(hipMemcpyAsync(dst + (size_t)r.uoff * width, src + (size_t)r.ioff * width,
-                                       sizeof(double) * r.unat * width, hipMemcpyDeviceToHost, stream));
+            std::copy(h.begin() + (size_t)r.ioff * width, h.begin() + ((size_t)r.ioff + r.unat) * width,
+                      dst + (size_t)r.uoff * width);
     }
     // The caller's row of held atom a, -1 for the other copies of a supercell.
     int user_row(int a) const {
