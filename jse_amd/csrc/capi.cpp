@@ -1034,6 +1034,7 @@ static jse_nnap_system_s *make_system(jse_nnap_model m, int nrep, const int *nat
         ni += (long long)natoms[q] * nc;
         any = any || nc > 1;
     }
+    JNN_CHECK(types || nu == 0, ERR_ARG, "null types");
     auto s = std::make_unique<jse_nnap_system_s>();
     if (!any) {
         s->init(m, nrep, natoms, types, boxes, precision);
