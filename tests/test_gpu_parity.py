@@ -17,7 +17,7 @@ import os
 import numpy as np
 import pytest
 
-from conftest import GOLDEN
+from conftest import DevBuf, GOLDEN
 from jse_amd import model as M
 from jse_amd import nnap
 from jse_amd import workloads as W
@@ -254,7 +254,6 @@ def test_box_shorter_than_cutoff(O, case):
     walk, NeighborListGetter.java:704-776) to the FP32 bar, totals per box.
     Descriptors and device positions take the same mapping; neighbour rows
     (which name supercell copies) are refused with status 2."""
-    import torch
     st = _short_box(case)
     md = W.ternary_model()
     mdl = M.load_model(md)
@@ -266,8 +265,8 @@ def test_box_shorter_than_cutoff(O, case):
         with pytest.raises(nnap.NNAPError) as e:
             s.neighbors()
         assert e.value.code == 2
-        x = torch.tensor(st.xyz, dtype=torch.float64, device="cuda")
-        s.set_positions_device(x.data_ptr())
+        x = DevBuf(np.ascontiguousarray(st.xyz, dtype=np.float64))
+        s.set_positions_device(x.ptr)
         s.compute()
         dev = s.results()
     ref = orc.system_efv(st)
@@ -294,7 +293,6 @@ def test_batch_with_boxes_shorter_than_cutoff(O):
     """A batch whose replicas include boxes below the cutoff (each held as its
     own supercell) next to an ordinary liquid: every replica's atoms and totals
     against the oracle, and device positions equal to host positions."""
-    import torch
     reps = [_short_box("all_short"), W.ternary_liquid(n=400, seed=9), _short_box("slab"),
             _short_box("tiny_outside")]
     md = W.ternary_model()
@@ -304,8 +302,8 @@ def test_batch_with_boxes_shorter_than_cutoff(O):
         b.compute()
         res = b.results()
         e_rep, v_rep = b.replica_results()
-        x = torch.tensor(np.concatenate([r.xyz for r in reps]), dtype=torch.float64, device="cuda")
-        b.set_positions_device(x.data_ptr())
+        x = DevBuf(np.ascontiguousarray(np.concatenate([r.xyz for r in reps]), dtype=np.float64))
+        b.set_positions_device(x.ptr)
         b.compute()
         dev = b.results()
         b.close()
