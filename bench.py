@@ -10,9 +10,9 @@ energy + force + virial evaluation of the configuration: cell-list neighbour
 build, descriptor forward, MLP, backward, FP64 force/virial accumulation.
 Positions are resident in HBM before the timed region starts.
 
-Prints one JSON line.  ``--gpus N`` with N > 1 exits: this build runs on one
-GPU (replicas and spatial domains would shard across GPUs with no change to
-the per-GPU kernels).
+Prints one JSON line.  ``--gpus N`` with N > 1 exits: the benchmark runs on
+one GPU (replicas and spatial domains would shard across GPUs with no change
+to the per-GPU kernels).
 """
 from __future__ import annotations
 
@@ -674,7 +674,7 @@ def main():
                     help="only run a plain device copy of this many bytes (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on exactly one GPU; --gpus {args.gpus} is not supported "
+        print(f"bench.py: the benchmark runs on exactly one GPU; --gpus {args.gpus} is not supported "
               "(replicas would shard across GPUs by HBM budget, one large system by spatial bricks)",
               file=sys.stderr)
         sys.exit(2)
