@@ -247,7 +247,9 @@ typedef struct jse_nnap_server_s *jse_nnap_server;
  * if exceeded). */
 int jse_nnap_server_create(jse_nnap_model model, int precision, double skin, double hbm_budget_bytes,
                            int max_batch_atoms, int max_replicas, double max_density, jse_nnap_server *out);
-/* Registers a replica (host-side only; fixed types and orthogonal box). */
+/* Registers a replica (host-side only; fixed types and orthogonal box).  A
+ * box edge below rcut + skin is held as the box's supercell: results are the
+ * caller's atoms' forces and per-box totals. */
 int jse_nnap_server_add_replica(jse_nnap_server s, int natoms, const int *types, const double *box,
                                 int *replica_id);
 /* Queues an evaluation of replica `replica_id` at positions xyz[natoms][3]

@@ -994,21 +994,6 @@ int jse_nnap_sh_invariants(int l3max, int l4max, const double *A, double *out) {
     });
 }
 
-// Copies per axis of the supercell of a box: 1 where the edge reaches rc,
-// else the fewest whose total length does.
-static void box_copies(const double *box, double rc, int cp[3]) {
-    for (int d = 0; d < 3; ++d) {
-        JNN_CHECK(std::isfinite(box[d]) && box[d] > 0, ERR_ARG, "box lengths must be positive and finite");
-        JNN_CHECK(rc / box[d] <= 1024.0, ERR_ARG, "a box edge below 1/1024 of the cutoff");
-        int c = 1;
-        if (box[d] < rc) {
-            c = (int)std::ceil(rc / box[d]);
-            while (c * box[d] < rc) ++c;
-        }
-        cp[d] = c;
-    }
-}
-
 // A system of nrep replicas (natoms[q] atoms of replica q after those of the
 // replicas before it, box boxes[3q..3q+2]); a replica with a box edge below
 // the cutoff is held as its supercell.

@@ -86,8 +86,7 @@ inline void make_grid(const double *b, double rcut, ReplicaGrid &g) {
         JNN_CHECK(std::isfinite(b[d]) && b[d] > 0, ERR_ARG, "box lengths must be positive and finite");
     const double minbox = std::min(b[0], std::min(b[1], b[2]));
     JNN_CHECK(rcut <= minbox, ERR_UNSUPPORTED,
-              "rcut larger than the smallest box length: systems and batches evaluate such a box as its supercell; "
-              "served replicas need every box edge >= rcut");
+              "rcut larger than the smallest box length (such a box is evaluated as its supercell)");
     const int div = (int)std::floor(minbox / rcut);
     const double cell_len = minbox / (double)div;
     for (int d = 0; d < 3; ++d) {
@@ -97,6 +96,21 @@ inline void make_grid(const double *b, double rcut, ReplicaGrid &g) {
         g.box[d] = b[d];
     }
     g.ncell = g.size[0] * g.size[1] * g.size[2];
+}
+
+// Copies per axis of the supercell of a box: 1 where the edge reaches rc,
+// else the fewest whose total length does.
+inline void box_copies(const double *box, double rc, int cp[3]) {
+    for (int d = 0; d < 3; ++d) {
+        JNN_CHECK(std::isfinite(box[d]) && box[d] > 0, ERR_ARG, "box lengths must be positive and finite");
+        JNN_CHECK(rc / box[d] <= 1024.0, ERR_ARG, "a box edge below 1/1024 of the cutoff");
+        int c = 1;
+        if (box[d] < rc) {
+            c = (int)std::ceil(rc / box[d]);
+            while (c * box[d] < rc) ++c;
+        }
+        cp[d] = c;
+    }
 }
 
 struct SpeciesDev {

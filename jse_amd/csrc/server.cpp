@@ -90,7 +90,13 @@ struct Request {
 };
 
 struct Replica {
-    int natoms = 0;
+    int natoms = 0;                  // atoms held (the caller's times ncopy)
+    // A box edge below rc + skin: the replica is held as its supercell of
+    // cp[0] x cp[1] x cp[2] copies of the box (copy c = cx + cp[0] (cy + cp[1] cz)
+    // of the caller's atom i is atom c * unat + i at x_i + (cx, cy, cz) * box);
+    // copy 0 carries the caller's forces, totals are divided by ncopy.
+    int unat = 0, ncopy = 1, cp[3] = {1, 1, 1};
+    double ubox[3] = {0.0, 0.0, 0.0};
     std::vector<int> types;
     ReplicaGrid grid{};              // size / cb / box / ncell at rc + skin
     std::vector<int> by_species;     // local atom indices grouped by species (stable)
@@ -308,16 +314,27 @@ struct jse_nnap_server_s {
     int add_replica(int natoms, const int *ty, const double *box) {
         const int nt = model->host.ntypes();
         JNN_CHECK(natoms >= 1 && ty && box, ERR_ARG, "a replica needs at least one atom, its types and its box");
-        JNN_CHECK(natoms <= max_batch, ERR_CAPACITY,
-                  "replica larger than max_batch_atoms (" + std::to_string(max_batch) + ")");
         Replica r;
+        box_copies(box, rv, r.cp);
+        const long long nc = (long long)r.cp[0] * r.cp[1] * r.cp[2];
+        JNN_CHECK((long long)natoms * nc <= max_batch, ERR_CAPACITY,
+                  "replica (times the copies of a box below rcut + skin) larger than max_batch_atoms (" +
+                      std::to_string(max_batch) + ")");
+        r.ncopy = (int)nc;
+        r.unat = natoms;
+        natoms *= r.ncopy;
         r.natoms = natoms;
-        r.types.assign(ty, ty + natoms);
-        for (int t : r.types)
-            JNN_CHECK(t >= 1 && t <= nt, ERR_TYPE,
-                      "Exist type (" + std::to_string(t) + ") greater than the input ntypes (" + std::to_string(nt) +
-                          ")");
-        make_grid(box, rv, r.grid);
+        for (int k = 0; k < r.unat; ++k)
+            JNN_CHECK(ty[k] >= 1 && ty[k] <= nt, ERR_TYPE,
+                      "Exist type (" + std::to_string(ty[k]) + ") greater than the input ntypes (" +
+                          std::to_string(nt) + ")");
+        for (int c = 0; c < r.ncopy; ++c) r.types.insert(r.types.end(), ty, ty + r.unat);
+        double sbox[3];
+        for (int d = 0; d < 3; ++d) {
+            r.ubox[d] = box[d];
+            sbox[d] = box[d] * r.cp[d];
+        }
+        make_grid(sbox, rv, r.grid);
         r.grid.natoms = natoms;
         r.species_off.assign(nt + 1, 0);
         for (int t : r.types) r.species_off[t]++;
@@ -346,10 +363,15 @@ struct jse_nnap_server_s {
         JNN_CHECK(fatal.empty(), ERR_HIP, "server stopped after a device error: " + fatal);
         Replica &R = reps[rid];
         lk.unlock();
-        const size_t n3 = (size_t)3 * R.natoms;
+        const size_t n3 = (size_t)3 * R.unat;
         for (size_t k = 0; k < n3; ++k)
             JNN_CHECK(std::isfinite(xyz[k]), ERR_ARG, "non-finite position in the request");
         q->xyz.assign(xyz, xyz + n3);
+        for (int c = 1; c < R.ncopy; ++c) {  // the supercell's other copies
+            const int cx = c % R.cp[0], cy = (c / R.cp[0]) % R.cp[1], cz = c / (R.cp[0] * R.cp[1]);
+            const double sh[3] = {R.ubox[0] * cx, R.ubox[1] * cy, R.ubox[2] * cz};
+            for (size_t k = 0; k < n3; ++k) q->xyz.push_back(xyz[k] + sh[k % 3]);
+        }
         q->rid = rid;
         lk.lock();
         q->ticket = next_ticket++;
@@ -632,12 +654,15 @@ struct jse_nnap_server_s {
         for (int b = 0; b < L.nrep; ++b) {
             Request *q = L.reqs[b];
             const double *tot = L.h_tot + 10 * (size_t)b;
-            if (q->E) *q->E = tot[0];
-            if (q->V && (q->what & JSE_NNAP_VIRIAL)) std::memcpy(q->V, tot + 1, sizeof(double) * 9);
-            if (q->F && (q->what & JSE_NNAP_FORCE))
-                std::memcpy(q->F, L.h_Fout + 3 * (size_t)L.t_off[b], sizeof(double) * 3 * reps[q->rid].natoms);
+            const Replica &R = reps[q->rid];
+            const double inv = 1.0 / R.ncopy;  // a supercell's totals per box
+            if (q->E) *q->E = tot[0] * inv;
+            if (q->V && (q->what & JSE_NNAP_VIRIAL))
+                for (int c = 0; c < 9; ++c) q->V[c] = tot[1 + c] * inv;
+            if (q->F && (q->what & JSE_NNAP_FORCE))  // copy 0: the replica's first unat rows
+                std::memcpy(q->F, L.h_Fout + 3 * (size_t)L.t_off[b], sizeof(double) * 3 * R.unat);
             n_rebuilds += L.h_flags[2 + b];
-            atoms_done += reps[q->rid].natoms;
+            atoms_done += R.unat;
             finish_request(q, OK, "");
         }
         L.reqs.clear();

@@ -223,9 +223,9 @@ def test_server_errors():
         with pytest.raises(nnap.NNAPError) as ei:
             srv.add_replica(nnap.AtomData(big.box, big.xyz, big.types))
         assert ei.value.code == 4
-        with pytest.raises(nnap.NNAPError) as ei:   # box below rcut + skin
-            srv.add_replica(nnap.AtomData(np.array([6.5, 20.0, 20.0]), st.xyz[:10], st.types[:10]))
-        assert ei.value.code == 2
+        with pytest.raises(nnap.NNAPError) as ei:   # a box below rcut + skin whose supercell exceeds max_batch
+            srv.add_replica(nnap.AtomData(np.array([2.5, 30.0, 30.0]), st.xyz[:400], st.types[:400]))
+        assert ei.value.code == 4 and "copies" in str(ei.value)
         with pytest.raises(ValueError):
             srv.add_replica(nnap.AtomData(st.box, st.xyz[:5], np.array([1, 2, 4, 1, 1], np.int32)))
         # the server still serves after the rejected calls
@@ -261,4 +261,32 @@ def test_served_untiled_shape_matches_oracle(O, kind):
                 ref = orc.system_efv(W.Structure(r.box, trajs[k][s], r.types))
                 e, f_rms, f_max, v = _bar(res, ref, r.natoms)
                 print(f"step {s} replica {k} n={r.natoms}: e {e:.2e} f_rms {f_rms:.2e} v {v:.2e}", flush=True)
+                assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-5 and v <= 1e-5
+
+
+def test_served_boxes_below_the_cutoff(O):
+    """Replicas whose box edges are below rc + skin (every edge short, a slab)
+    are held as their supercells; along a short trajectory each served result
+    (energy and virial per box, forces of the caller's atoms) meets the FP32
+    bar against the oracle, next to an ordinary replica."""
+    md = W.ternary_model()
+    orc = O.Oracle(M.load_model(md))
+    rng = np.random.default_rng(5)
+    reps = []
+    for box, n in (((4.6, 3.9, 5.2), 5), ((13.0, 12.2, 3.4), 30), ((6.5, 6.8, 7.2), 16)):
+        box = np.array(box)
+        reps.append(W.Structure(box, W.rsa(n, box, 2.0, 7 + n), rng.integers(1, 4, size=n).astype(np.int32)))
+    reps.append(_replica_set(1, 300, 300, 33)[0])
+    with nnap.NNAP(md) as pot, pot.server(precision=nnap.FP32, max_batch_atoms=8192) as srv:
+        rids = [srv.add_replica(nnap.AtomData(r.box, r.xyz, r.types)) for r in reps]
+        trajs = [_trajectory(r, 3, 0.01, 200 + k) for k, r in enumerate(reps)]
+        for s in range(3):
+            tks = [srv.submit(rid, trajs[k][s]) for k, rid in enumerate(rids)]
+            for k, tk in enumerate(tks):
+                res = srv.wait(tk)
+                r = reps[k]
+                ref = orc.system_efv(W.Structure(r.box, trajs[k][s], r.types))
+                e, f_rms, f_max, v = _bar(res, ref, r.natoms)
+                print(f"step {s} replica {k} n={r.natoms}: e {e:.2e} f_max {f_max:.2e} v {v:.2e}", flush=True)
+                assert res["forces"].shape == (r.natoms, 3)
                 assert e <= 1e-6 and f_rms <= 1e-5 and f_max <= 1e-5 and v <= 1e-5
