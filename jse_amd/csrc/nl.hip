@@ -246,6 +246,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
     const int *__restrict__ cstart, const int *__restrict__ sidx, const double *__restrict__ sxw, NeighborList nl,
     int *__restrict__ overflow) {
     __shared__ int s_start[27], s_cnt[27], s_code[27], s_pref[28];
+    __shared__ double s_lo[27][3], s_hi[27][3];  // each neighbour cell's extent, image shift applied
     __shared__ double s_cx[kStage], s_cy[kStage], s_cz[kStage];
     __shared__ int s_ckey[kStage], s_cslot[kStage], s_ctype[kStage];
     __shared__ int s_hkey[kCellWaves][kNlLds], s_hq[kCellWaves][kNlLds];  // hits: key, candidate index
@@ -274,6 +275,14 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                 s_start[w] = st;
                 s_cnt[w] = cnt;
                 s_code[w] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
+                // widened by 1e-6 of a cell edge, so that a binned atom never lies outside
+                const int ic[3] = {ix, iy, iz}, sc[3] = {sx, sy, sz};
+#pragma unroll
+                for (int d = 0; d < 3; ++d) {
+                    const double m = 1e-6 * g.cb[d];
+                    s_lo[w][d] = ic[d] * g.cb[d] + sc[d] * g.box[d] - m;
+                    s_hi[w][d] = (ic[d] + 1) * g.cb[d] + sc[d] * g.box[d] + m;
+                }
             }
             int incl = cnt;
 #pragma unroll
@@ -349,24 +358,46 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                 dz = zj - zi;
             };
             int hits = 0;
-            for (int q0 = 0; q0 < total; q0 += 64) {
-                const int q = q0 + lane;
-                bool hit = false;
-                int key = 0;
-                if (q < total) {
-                    int slot;
-                    double dx, dy, dz;
-                    cand(q, key, slot, dx, dy, dz);
-                    hit = (dist2_no_fma(dx, dy, dz) < rc2) && !((key >> 5) == i && (key & 31) == 13);
+            auto test_range = [&](int qa, int qb) {  // candidates qa .. qb-1, 64 at a time
+                for (int q0 = qa; q0 < qb; q0 += 64) {
+                    const int q = q0 + lane;
+                    bool hit = false;
+                    int key = 0;
+                    if (q < qb) {
+                        int slot;
+                        double dx, dy, dz;
+                        cand(q, key, slot, dx, dy, dz);
+                        hit = (dist2_no_fma(dx, dy, dz) < rc2) && !((key >> 5) == i && (key & 31) == 13);
+                    }
+                    const unsigned long long mask = __ballot(hit);
+                    const unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
+                    const int pos = hits + __popcll(below);
+                    if (hit && pos < kNlLds) {
+                        hkey[pos] = key;
+                        hq[pos] = q;
+                    }
+                    hits += __popcll(mask);
                 }
-                const unsigned long long mask = __ballot(hit);
-                const unsigned long long below = lane ? (mask & ((1ull << lane) - 1ull)) : 0ull;
-                const int pos = hits + __popcll(below);
-                if (hit && pos < kNlLds) {
-                    hkey[pos] = key;
-                    hq[pos] = q;
+            };
+            if (total >= 27 * 32) {
+                // large cells (a replica of one or two cells per axis): a neighbour cell
+                // whose shifted extent lies rc or farther from atom i along the three axes
+                // together holds no neighbour, and its block is skipped (rows are ranked
+                // by key below, so the order the blocks are visited in does not matter)
+                const double xa[3] = {xi, yi, zi};
+                for (int w = 0; w < 27; ++w) {
+                    const int qa = s_pref[w], qb = s_pref[w + 1];
+                    if (qa == qb) continue;
+                    double d2 = 0.0;
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const double t = s_lo[w][d] > xa[d] ? s_lo[w][d] - xa[d] : (xa[d] > s_hi[w][d] ? xa[d] - s_hi[w][d] : 0.0);
+                        d2 += t * t;
+                    }
+                    if (d2 < rc2) test_range(qa, qb);
                 }
-                hits += __popcll(mask);
+            } else {
+                test_range(0, total);
             }
             wave_lds_sync();
             // keys are unique: a hit's slot in the sorted row is the number of smaller
