@@ -413,6 +413,7 @@ struct jse_nnap_system_s {
     hipGraphExec_t graph_exec[8] = {};
     bool use_graph = true;
     LaunchTune tune;                   // launch shapes (autotuner)
+    bool large_cells = false;          // cell build variant (set with the boxes)
     // Device buffers of the training entry points, kept between calls: the k-th
     // request of a call gets the k-th buffer, reallocated only when it must grow
     // (each call ends with a stream synchronisation, so reuse is ordered).
@@ -484,6 +485,11 @@ struct jse_nnap_system_s {
         JNN_HIP(hipStreamSynchronize(stream));
         reps = r;
         ncell_total = coff;
+        // the cell build's large-cell variant when some replica's 27 neighbour
+        // blocks average 32 atoms or more (a replica of one or two cells per axis)
+        large_cells = false;
+        for (int q = 0; q < nrep; ++q)
+            if (r[q].natoms >= 32 * std::max(1, r[q].ncell)) large_cells = true;
         if (ncell_total + 1 > ncell_alloc) {
             dfree(d_cell_cnt);
             dfree(d_cell_start);
@@ -600,7 +606,7 @@ struct jse_nnap_system_s {
                             d_slot_of);
         nl.types = d_types;
         launch_nl_build_cells(stream, ncell_total, d_cell_rep, d_reps, rc2, d_cell_start, d_sidx, d_sxw, nl, d_overflow,
-                              tune.nl_wg_per_cu);
+                              tune.nl_wg_per_cu, large_cells);
         if (timed) JNN_HIP(hipEventRecord(ev[1], stream));
         // (the force gather finds reverse pairs itself; phase 1 stays empty)
         if (timed) JNN_HIP(hipEventRecord(ev[2], stream));

@@ -138,7 +138,7 @@ void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, 
 // cell c, -1 for a replica's end entry.
 void launch_nl_build_cells(hipStream_t s, int ncell_total, const int *cell_rep, const ReplicaGrid *reps, double rc2,
                            const int *cell_start, const int *sidx, const double *sxw, NeighborList nl, int *overflow,
-                           int wg_per_cu = 0);
+                           int wg_per_cu = 0, bool large_cells = false);
 void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl, const int *out_of = nullptr,
                        const int *row_of = nullptr);
 

@@ -241,6 +241,10 @@ constexpr int kCellWaves = 4;
 constexpr int kStage = 512;
 
 
+// BLOCKS: the variant for replicas of large cells (a neighbour block is skipped
+// whole when its cell lies beyond the cutoff from the atom); without it the
+// code is the plain walk.
+template <bool BLOCKS>
 __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
     int ncell_total, const int *__restrict__ cell_rep, const ReplicaGrid *__restrict__ reps, double rc2,
     const int *__restrict__ cstart, const int *__restrict__ sidx, const double *__restrict__ sxw, NeighborList nl,
@@ -275,13 +279,14 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                 s_start[w] = st;
                 s_cnt[w] = cnt;
                 s_code[w] = (sx + 1) + 3 * (sy + 1) + 9 * (sz + 1);
-                // widened by 1e-6 of a cell edge, so that a binned atom never lies outside
-                const int ic[3] = {ix, iy, iz}, sc[3] = {sx, sy, sz};
+                if constexpr (BLOCKS) {  // widened by 1e-6 of a cell edge: a binned atom never lies outside
+                    const int ic[3] = {ix, iy, iz}, sc[3] = {sx, sy, sz};
 #pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const double m = 1e-6 * g.cb[d];
-                    s_lo[w][d] = ic[d] * g.cb[d] + sc[d] * g.box[d] - m;
-                    s_hi[w][d] = (ic[d] + 1) * g.cb[d] + sc[d] * g.box[d] + m;
+                    for (int d = 0; d < 3; ++d) {
+                        const double m = 1e-6 * g.cb[d];
+                        s_lo[w][d] = ic[d] * g.cb[d] + sc[d] * g.box[d] - m;
+                        s_hi[w][d] = (ic[d] + 1) * g.cb[d] + sc[d] * g.box[d] + m;
+                    }
                 }
             }
             int incl = cnt;
@@ -379,7 +384,7 @@ __global__ __launch_bounds__(64 * kCellWaves) void k_nl_build_cells(
                     hits += __popcll(mask);
                 }
             };
-            if (total >= 27 * 32) {
+            if (BLOCKS && total >= 27 * 32) {
                 // large cells (a replica of one or two cells per axis): a neighbour cell
                 // whose shifted extent lies rc or farther from atom i along the three axes
                 // together holds no neighbour, and its block is skipped (rows are ranked
@@ -528,13 +533,17 @@ void launch_nl_build(hipStream_t s, int n, const int *atoms, const int *rep_of, 
 }
 void launch_nl_build_cells(hipStream_t s, int ncell_total, const int *cell_rep, const ReplicaGrid *reps, double rc2,
                            const int *cell_start, const int *sidx, const double *sxw, NeighborList nl, int *overflow,
-                           int wg_per_cu) {
+                           int wg_per_cu, bool large_cells) {
     if (ncell_total <= 0) return;
     int grid = ncell_total;
     if (wg_per_cu > 0) grid = std::min(grid, wg_per_cu * device_cu_count());
     grid = xcd_grid(grid);
-    hipLaunchKernelGGL(k_nl_build_cells, dim3(grid), dim3(64 * kCellWaves), 0, s, ncell_total, cell_rep, reps, rc2,
-                       cell_start, sidx, sxw, nl, overflow);
+    if (large_cells)
+        hipLaunchKernelGGL(k_nl_build_cells<true>, dim3(grid), dim3(64 * kCellWaves), 0, s, ncell_total, cell_rep,
+                           reps, rc2, cell_start, sidx, sxw, nl, overflow);
+    else
+        hipLaunchKernelGGL(k_nl_build_cells<false>, dim3(grid), dim3(64 * kCellWaves), 0, s, ncell_total, cell_rep,
+                           reps, rc2, cell_start, sidx, sxw, nl, overflow);
 }
 void launch_nl_reverse(hipStream_t s, int n, const int *atoms, NeighborList nl, const int *out_of, const int *row_of) {
     if (n <= 0) return;
